@@ -1,0 +1,187 @@
+"""Benchmark of the GRMHD step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 128] [--no-cpu]
+
+Workload: 3-D linear MHD fast eigenmode on the periodic unit cube, 128^3 zones
+in one meshblock (4 ghost zones), gamma = 4/3, WENO5 + LLF, flux-CT, RK2,
+onedw inverter, drift-frame floors -- the precision-form configuration.  Both
+forms run back to back on the same input:
+  * plain FP64,
+  * split precision (FP32 reconstruction / fluxes / EMFs over FP64 U, inverter, dt).
+A "step" is one full RK2 step; a zone-cycle is one interior zone advanced by
+one step.  `value` is the split form's zone-cycles/s with the state resident in
+HBM before the timed region starts.
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "zone-cycles/s on 1 MI355X: FP32-split vs FP64 plain; adjoint/forward time ratio"
+HBM_PEAK_GBS = 8000.0
+
+
+def _bench_form(cfg, P0, steps, warmup):
+    import torch
+    import kharma_amd as kh
+    drv = kh.Driver(cfg)
+    drv.init_prims(torch.from_numpy(P0).cuda())
+    drv.step(warmup)
+    torch.cuda.synchronize()
+    drv.set_kernel_timing(True)
+    drv.kernel_times()  # reset accumulators
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    drv.step(steps)
+    stop.record()
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(stop)
+    ktimes = drv.kernel_times()
+    drv.set_kernel_timing(False)
+    s = drv.scalars()
+    P, _ = drv.state()
+    torch.cuda.synchronize()
+    return dict(ms=ms, kernels=ktimes, scalars=s, P=P)
+
+
+def _algorithmic_bytes(kind, cfg, flux_bytes):
+    """Bytes one launch of `kind` must move at minimum (each array element once)."""
+    nx1, nx2, nx3 = cfg["nx"]
+    ng = cfg.get("ng", 4)
+    nz = nx1 * nx2 * nx3
+    if kind.startswith("flux_x"):
+        d = int(kind[-1])
+        faces = [nx1 + 2, nx2 + 2, nx3 + 2]
+        faces[d - 1] -= 1
+        nfaces = faces[0] * faces[1] * faces[2]
+        cells = list(faces)
+        cells[d - 1] += 5                       # 6-point stencil along d
+        ncells = cells[0] * cells[1] * cells[2]
+        return ncells * 8 * 8 + nfaces * (8 + 2) * flux_bytes
+    if kind.startswith("update"):
+        last = kind.endswith("s2")
+        per_zone = (3 * 8 + 3) * flux_bytes + (8 + 8 + 5) * 8 + (8 + 8) * 8 + 2 * 4
+        if last:
+            per_zone += 6 * flux_bytes
+        return nz * per_zone
+    if kind == "emf":
+        return (nx1 + 1) * (nx2 + 1) * (nx3 + 1) * (6 + 3) * flux_bytes
+    return None
+
+
+def _cpu_baseline(cfg, P0, budget_s=12.0, max_steps=8):
+    from oracle import oracle
+    oracle.build()
+    ob = oracle.OracleBlock(cfg)
+    ob.init_state(P0)
+    ob.step(1)  # warm-up (first touch, dt ramp start)
+    nz = cfg["nx"][0] * cfg["nx"][1] * cfg["nx"][2]
+    t0 = time.perf_counter()
+    n = 0
+    while n < max_steps and (time.perf_counter() - t0) < budget_s:
+        ob.step(1)
+        n += 1
+    dt = time.perf_counter() - t0
+    return dict(value=nz * n / dt, unit="zone-cycles/s", cores=oracle.num_threads(), kind="port",
+                sample=f"{n} RK2 steps of the same 3-D fast mode {cfg['nx'][0]}^3 block after 1 warm-up step, "
+                       f"CPU restatement of KHARMA's algorithm (C++ FP64, OpenMP over k,j)")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this benchmark runs on exactly one GPU; multi-GPU runs are not part of this build",
+                          "n_gpus": a.gpus}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from kharma_amd import problems as pr
+
+    cfg = pr.mhdmode_config(n=a.n, ndim=3, mode="fast", one_period=False)
+    P0 = pr.mhdmode_prims(cfg)
+    nz = a.n ** 3
+
+    r64 = _bench_form(dict(cfg, precision="fp64"), P0, a.steps, a.warmup)
+    r32 = _bench_form(dict(cfg, precision="split"), P0, a.steps, a.warmup)
+    zc64 = nz * a.steps / (r64["ms"] * 1e-3)
+    zc32 = nz * a.steps / (r32["ms"] * 1e-3)
+
+    # deviation of the split form from the FP64 form after the same step count
+    sl = (slice(None),) + pr.interior(cfg)
+    P64 = r64["P"].double().cpu().numpy()[sl]
+    P32 = r32["P"].double().cpu().numpy()[sl]
+    rel = (np.abs(P32 - P64).sum(axis=(1, 2, 3)) / np.abs(P64).sum(axis=(1, 2, 3)).clip(1e-300))
+
+    # dominant kernel of the reported (split) form
+    kt = {k: v for k, v in r32["kernels"].items() if v[1] > 0}
+    dom = max(kt, key=lambda k: kt[k][0])
+    tot_ms, nl = kt[dom]
+    avg_ms = tot_ms / nl
+    algo = _algorithmic_bytes(dom, cfg, 4)
+    achieved = (algo / (avg_ms * 1e-3) / 1e9) if algo else None
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{dom}_split_{a.n}.json")
+    if os.path.exists(tfile):
+        traffic = json.load(open(tfile)).get("hbm_bytes_per_launch")
+    roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic,
+            "kernel": dom, "avg_launch_ms": avg_ms, "algorithmic_bytes_per_launch": algo}
+    step_bytes = 576 * nz  # FP64 state contract per zone-cycle
+    kernel_ms = {k: {"total_ms": v[0], "launches": v[1], "avg_ms": (v[0] / v[1]) if v[1] else None}
+                 for k, v in r32["kernels"].items()}
+    kernel_ms64 = {k: {"total_ms": v[0], "launches": v[1], "avg_ms": (v[0] / v[1]) if v[1] else None}
+                   for k, v in r64["kernels"].items()}
+
+    out = {
+        "metric": METRIC,
+        "value": zc32,
+        "unit": "zone-cycles/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": r32["ms"] / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32/f64",
+        "data": "synthetic",
+        "config": {"workload": f"3-D linear MHD fast mode, periodic unit cube {a.n}^3 in 1 meshblock, gamma 4/3, "
+                               "WENO5+LLF, flux-CT, RK2, onedw FP64 inverter, drift floors",
+                   "zones": nz, "ghosts": cfg["ng"], "forms": ["fp64", "split"]},
+        "fp64": {"value": zc64, "ms_per_step": r64["ms"] / a.steps, "dtype": "f64"},
+        "split_over_fp64": zc32 / zc64,
+        "split_rel_l1_vs_fp64": {"max": float(rel.max()), "per_var": [float(x) for x in rel]},
+        "adjoint_over_forward": None,
+        "step_state_roofline": {"bytes_per_zone_cycle": 576, "achieved_GBs_fp64": step_bytes * a.steps / (r64["ms"] * 1e-3) / 1e9,
+                                "achieved_GBs_split": step_bytes * a.steps / (r32["ms"] * 1e-3) / 1e9,
+                                "peak": HBM_PEAK_GBS},
+        "roofline": roof,
+        "kernels_split": kernel_ms,
+        "kernels_fp64": kernel_ms64,
+        "sim_time": {"fp64": r64["scalars"]["time"], "split": r32["scalars"]["time"],
+                     "divb_fp64": r64["scalars"]["divb"], "divb_split": r32["scalars"]["divb"]},
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = _cpu_baseline(cfg, P0)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,170 @@
+/*
+ * kharma_amd.h -- C ABI of the MI355X (gfx950) GRMHD step.
+ *
+ * The entry points replace, one for one, the task functions KHARMA's driver
+ * schedules for one explicit GRMHD stage (driver/kharma_step.cpp:77-319).  Each
+ * declaration cites the reference interface it stands in for.  All data
+ * pointers are NON-OWNING device pointers laid out like a Parthenon
+ * VariablePack of one meshblock:
+ *
+ *   cell field A[var][k][j][i] over the entire block (interior + ng ghosts on each
+ *   side of every active dimension), i fastest; var stride = n1*n2*n3 ("nc").
+ *   Primitive / conserved order: rho, u, u1, u2, u3, B1, B2, B3.
+ *   Fluxes F[dir-1][var][k][j][i]; face index i is the LEFT face of cell i.
+ *   Signal speeds cmax/cmin[dir-1][k][j][i] (cmin stored positive).
+ *   Flags pflag/fflag: int32 [k][j][i] (the reference stores them as Real fields).
+ *
+ * With precision KH_PREC_SPLIT the flux-like arrays (F, cmax, cmin, emf) hold
+ * float instead of double; U and P are always double.
+ *
+ * Every function returns 0 (KH_OK) or a nonzero status; no C++ exception
+ * crosses this boundary.  kh_last_error() describes the last failure of the
+ * calling thread.  Work is enqueued on the block's stream and is asynchronous
+ * unless a function says it synchronises.
+ */
+#ifndef KHARMA_AMD_H
+#define KHARMA_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KH_OK 0
+#define KH_ERR_INVALID 1
+#define KH_ERR_HIP 2
+#define KH_ERR_UNSUPPORTED 3
+
+#define KH_PREC_FP64 0
+#define KH_PREC_SPLIT 1
+
+#define KH_RECON_DONOR_CELL_C 0
+#define KH_RECON_LINEAR_MC 1
+#define KH_RECON_WENO5 2
+#define KH_RECON_PPM 3
+
+/* Run parameters of one meshblock: the .par options this path reads
+ * (GRMHD/*, flux/*, floors/*, inverter/*, parthenon/time/*, parthenon/mesh/*). */
+typedef struct kh_params {
+    int nx1, nx2, nx3, ng;          /* interior zones (nx3 == 1 for 2D, nx2 == nx3 == 1 for 1D); ghost depth */
+    double x1min, x1max, x2min, x2max, x3min, x3max;
+    int periodic1, periodic2, periodic3;
+    double gam;                     /* GRMHD/gamma */
+    double cfl, dt_min, dt_start, max_dt_increase, tlim;
+    int recon;                      /* KH_RECON_* (flux/reconstruction) */
+    int use_hlle;                   /* flux/type == hlle */
+    int floors_on;                  /* Floors package loaded */
+    int recon_floors;               /* flux/reconstruction_floors */
+    int floor_frame;                /* 0 fluid, 1 normal, 2 drift (floors/frame) */
+    double rho_min_geom, u_min_geom, rho_min_const, u_min_const, r_char;
+    double bsq_over_rho_max, bsq_over_u_max, u_over_rho_max, ktot_max, gamma_max;
+    int use_r_char, temp_adjust_u;
+    int iter_max;                   /* inverter/iter_max (onedw) */
+    double err_tol;                 /* inverter/err_tol */
+    int fix_average, fix_atmo;      /* inverter/fix_average_neighbors, fix_atmosphere */
+    int integrator;                 /* 0 rk2, 1 vl2 (parthenon/time/integrator) */
+    int coords;                     /* 0 cartesian_minkowski */
+    int precision;                  /* KH_PREC_FP64 or KH_PREC_SPLIT */
+} kh_params;
+
+typedef struct kh_block kh_block;
+typedef struct kh_driver kh_driver;
+
+const char* kh_last_error(void);
+int kh_version(void);
+
+/* Geometry and index ranges of one meshblock (replaces GRCoordinates'
+ * construction, coordinates/gr_coordinates.cpp:64-77, for the block).
+ * Allocates only a small reduction scratch area. */
+int kh_block_create(const kh_params* p, kh_block** out);
+void kh_block_destroy(kh_block* b);
+/* hipStream_t, or NULL for the default stream (one stream per call sequence) */
+int kh_block_set_stream(kh_block* b, void* hip_stream);
+/* entire sizes and the variable stride nc */
+int kh_block_shape(const kh_block* b, int* n1, int* n2, int* n3, long* nc);
+
+/* Flux::GetFlux<Recon,dir> (flux/get_flux.hpp:60-373): reconstruct P to the faces
+ * normal to dir (1..3), apply reconstruction floors, compute the face fluxes
+ * (LLF or HLLE) and the signal speeds.  F_dir / cmax_dir / cmin_dir point at this
+ * direction's [var][nc] and [nc] arrays.  Faces written: those the step consumes,
+ * i.e. X1: i in [is, ie+1], j in [js-1, je+1], k in [ks-1, ke+1] (cyclic for X2/X3). */
+int kh_get_flux(kh_block* b, int dir, const double* P, void* F_dir, void* cmax_dir, void* cmin_dir);
+
+/* B_FluxCT::FluxCT (b_flux_ct/b_flux_ct.cpp:307-370): corner EMFs into emf[3][nc]
+ * and the rewritten B fluxes in F[3][8][nc]. */
+int kh_flux_ct(kh_block* b, void* F, void* emf);
+
+/* KHARMADriver::FluxDivergence (driver/kharma_driver.hpp:208-231): dUdt[8][nc] on the interior. */
+int kh_flux_divergence(kh_block* b, const void* F, double* dUdt);
+
+/* Update::WeightedSumData pair of KHARMADriver::AddStateUpdate
+ * (driver/kharma_driver.cpp:339-387), entire block:
+ * Uout = gam0*Usub + gam1*Ufull; Uout = Uout + beta_dt*dUdt. */
+int kh_weighted_sum(kh_block* b, const double* Usub, const double* Ufull, const double* dUdt,
+                    double gam0, double gam1, double beta_dt, double* Uout);
+
+/* Inverter::BlockUtoP with the onedw solver (inverter/inverter.cpp:131-200,
+ * inverter/onedw.hpp:89-233) then B_FluxCT::BlockUtoP (b_flux_ct.cpp:195-217),
+ * entire block.  P holds the guess on entry. */
+int kh_utop(kh_block* b, const double* U, double* P, int* pflag, int* fflag);
+
+/* Floors::ApplyGRMHDFloors (floors/floors.cpp:233-250, floors_impl.hpp:45-128), entire block. */
+int kh_apply_floors(kh_block* b, double* U, double* P, int* pflag, int* fflag);
+
+/* Inverter::FixUtoP (inverter/fixup.cpp:47-156), entire block. */
+int kh_fix_utop(kh_block* b, double* U, double* P, const int* pflag);
+
+/* Flux::MeshPtoU (flux/flux.cpp:276-322), entire block. */
+int kh_ptou(kh_block* b, const double* P, double* U);
+
+/* Boundary exchange of nvar cell variables of a single periodic block with itself
+ * (the parthenon exchange behind KHARMADriver::AddBoundarySync, kharma_driver.cpp:138-183). */
+int kh_sync_bounds(kh_block* b, double* A, int nvar);
+
+/* GRMHD::EstimateTimestep in-loop branch (grmhd/grmhd.cpp:310-339): CFL minimum over
+ * the interior from cmax/cmin[3][nc], clipped to [dt_min, max_dt_increase*dt_last].
+ * Synchronises the stream. */
+int kh_estimate_timestep(kh_block* b, const void* cmax, const void* cmin, double dt_last, double* ndt);
+
+/* B_FluxCT::MaxDivB (b_flux_ct.cpp:566-607).  Synchronises the stream. */
+int kh_max_divb(kh_block* b, const double* U, double* out);
+
+/* ---- Stage driver --------------------------------------------------------
+ * KHARMADriver::MakeDefaultTaskCollection (driver/kharma_step.cpp:77-319) for a
+ * single periodic meshblock, with Parthenon's RK2/VL2 stage containers and
+ * KHARMA's dt control (kharma_driver.cpp:439-460).  The driver owns its
+ * containers (U, P for the base and stage-1 states, fluxes, flags).  dt is
+ * computed and consumed on the device; kh_driver_step does not synchronise. */
+int kh_driver_create(const kh_params* p, kh_driver** out);
+void kh_driver_destroy(kh_driver* d);
+int kh_driver_set_stream(kh_driver* d, void* hip_stream);
+/* Initial primitives P[8][nc] (device pointer): sync, P->U over the entire
+ * block, dt = dt_start (KHARMA::PostInitialize, prob/post_initialize.cpp:55-193). */
+int kh_driver_init_prims(kh_driver* d, const double* P);
+int kh_driver_step(kh_driver* d, int nsteps);
+/* Copy the base-container state out (device pointers; either may be NULL). */
+int kh_driver_get_state(kh_driver* d, double* P, double* U);
+/* Synchronises; any output may be NULL. */
+int kh_driver_get_scalars(kh_driver* d, double* time, double* dt, double* last_divb, int* ncycle, int* total_fails);
+/* dt taken by each of the first n steps since init (host array). Synchronises. */
+int kh_driver_get_dt_history(kh_driver* d, double* out, int n);
+int kh_driver_block(kh_driver* d, kh_block** out);
+/* Device pointers to the driver's base-container arrays (U, P: [8][nc] doubles). */
+int kh_driver_arrays(kh_driver* d, double** U, double** P);
+/* Per-kernel timing with HIP events recorded on the driver's stream around every
+ * launch of kh_driver_step while enabled (no counterpart in the reference; it
+ * plays the role of the Kokkos-tools kernel timer hooked in run.sh:11-91).
+ * Kernel kinds: 0..2 face fluxes X1..X3, 3 corner EMFs, 4 stage-1 update,
+ * 5 stage-2 update, 6 inversion fix-up, 7 divB, 8 timestep finalize. */
+#define KH_NKERNEL_KINDS 9
+int kh_driver_set_kernel_timing(kh_driver* d, int enable);
+/* Sum of launch durations (ms) and launch counts per kind since the last call;
+ * synchronises and resets the accumulators. */
+int kh_driver_kernel_times(kh_driver* d, double* total_ms, int* launches, int nkinds);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KHARMA_AMD_H */

@@ -1,0 +1,332 @@
+"""kharma_amd -- MI355X-native GRMHD step (KHARMA's explicit step on gfx950).
+
+Python host binding over the C ABI in include/kharma_amd.h.  Device memory and
+streams come from PyTorch; every computation runs in the HIP kernels of
+libkharma_amd.so.  There is no CPU fallback: importing the runtime raises if
+the library was not built, and calls raise if the GPU is unavailable.
+
+Layout of every cell array: float64 [8, n3, n2, n1] (entire block, ghosts included),
+variables rho, u, u1, u2, u3, B1, B2, B3 -- the layout of KHARMA's VariablePack.
+"""
+import ctypes
+import os
+import re
+
+from . import problems  # noqa: F401  (host-side initial conditions)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libkharma_amd.so")
+HEADER = os.path.abspath(os.path.join(HERE, "..", "include", "kharma_amd.h"))
+
+NV = 8
+PREC_FP64 = 0
+PREC_SPLIT = 1
+RECON = {"donor_cell_c": 0, "linear_mc": 1, "weno5": 2, "ppm": 3}
+FRAME = {"fluid": 0, "normal": 1, "drift": 2}
+
+
+class KhParams(ctypes.Structure):
+    _fields_ = [
+        ("nx1", ctypes.c_int), ("nx2", ctypes.c_int), ("nx3", ctypes.c_int), ("ng", ctypes.c_int),
+        ("x1min", ctypes.c_double), ("x1max", ctypes.c_double),
+        ("x2min", ctypes.c_double), ("x2max", ctypes.c_double),
+        ("x3min", ctypes.c_double), ("x3max", ctypes.c_double),
+        ("periodic1", ctypes.c_int), ("periodic2", ctypes.c_int), ("periodic3", ctypes.c_int),
+        ("gam", ctypes.c_double),
+        ("cfl", ctypes.c_double), ("dt_min", ctypes.c_double), ("dt_start", ctypes.c_double),
+        ("max_dt_increase", ctypes.c_double), ("tlim", ctypes.c_double),
+        ("recon", ctypes.c_int), ("use_hlle", ctypes.c_int),
+        ("floors_on", ctypes.c_int), ("recon_floors", ctypes.c_int), ("floor_frame", ctypes.c_int),
+        ("rho_min_geom", ctypes.c_double), ("u_min_geom", ctypes.c_double),
+        ("rho_min_const", ctypes.c_double), ("u_min_const", ctypes.c_double), ("r_char", ctypes.c_double),
+        ("bsq_over_rho_max", ctypes.c_double), ("bsq_over_u_max", ctypes.c_double),
+        ("u_over_rho_max", ctypes.c_double), ("ktot_max", ctypes.c_double), ("gamma_max", ctypes.c_double),
+        ("use_r_char", ctypes.c_int), ("temp_adjust_u", ctypes.c_int),
+        ("iter_max", ctypes.c_int), ("err_tol", ctypes.c_double),
+        ("fix_average", ctypes.c_int), ("fix_atmo", ctypes.c_int),
+        ("integrator", ctypes.c_int), ("coords", ctypes.c_int), ("precision", ctypes.c_int),
+    ]
+
+
+class KharmaError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def header_functions():
+    """Names of the functions declared in include/kharma_amd.h."""
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\*?\s+\*?(kh_\w+)\s*\(", txt, flags=re.M)))
+
+
+def lib():
+    """Load libkharma_amd.so (build it with kharma_amd.build or __graft_entry__.build())."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise KharmaError(f"{LIB_PATH} is missing: build it with `python -m kharma_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i, d, l = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_long
+    pvp = ctypes.POINTER(ctypes.c_void_p)
+    sig = {
+        "kh_last_error": (ctypes.c_char_p, []),
+        "kh_version": (i, []),
+        "kh_block_create": (i, [ctypes.POINTER(KhParams), pvp]),
+        "kh_block_destroy": (None, [vp]),
+        "kh_block_set_stream": (i, [vp, vp]),
+        "kh_block_shape": (i, [vp, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(l)]),
+        "kh_get_flux": (i, [vp, i, vp, vp, vp, vp]),
+        "kh_flux_ct": (i, [vp, vp, vp]),
+        "kh_flux_divergence": (i, [vp, vp, vp]),
+        "kh_weighted_sum": (i, [vp, vp, vp, vp, d, d, d, vp]),
+        "kh_utop": (i, [vp, vp, vp, vp, vp]),
+        "kh_apply_floors": (i, [vp, vp, vp, vp, vp]),
+        "kh_fix_utop": (i, [vp, vp, vp, vp]),
+        "kh_ptou": (i, [vp, vp, vp]),
+        "kh_sync_bounds": (i, [vp, vp, i]),
+        "kh_estimate_timestep": (i, [vp, vp, vp, d, ctypes.POINTER(d)]),
+        "kh_max_divb": (i, [vp, vp, ctypes.POINTER(d)]),
+        "kh_driver_create": (i, [ctypes.POINTER(KhParams), pvp]),
+        "kh_driver_destroy": (None, [vp]),
+        "kh_driver_set_stream": (i, [vp, vp]),
+        "kh_driver_init_prims": (i, [vp, vp]),
+        "kh_driver_step": (i, [vp, i]),
+        "kh_driver_get_state": (i, [vp, vp, vp]),
+        "kh_driver_get_scalars": (i, [vp, ctypes.POINTER(d), ctypes.POINTER(d), ctypes.POINTER(d),
+                                      ctypes.POINTER(i), ctypes.POINTER(i)]),
+        "kh_driver_get_dt_history": (i, [vp, ctypes.POINTER(d), i]),
+        "kh_driver_block": (i, [vp, pvp]),
+        "kh_driver_arrays": (i, [vp, pvp, pvp]),
+        "kh_driver_set_kernel_timing": (i, [vp, i]),
+        "kh_driver_kernel_times": (i, [vp, ctypes.POINTER(d), ctypes.POINTER(i), i]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(status):
+    if status != 0:
+        raise KharmaError(f"kharma_amd error {status}: {lib().kh_last_error().decode()}")
+
+
+def make_params(cfg):
+    """kh_params from a run configuration dict (see kharma_amd.problems)."""
+    p = KhParams()
+    p.nx1, p.nx2, p.nx3 = cfg["nx"]
+    p.ng = cfg.get("ng", 4)
+    (p.x1min, p.x1max), (p.x2min, p.x2max), (p.x3min, p.x3max) = cfg["bounds"]
+    p.periodic1 = p.periodic2 = p.periodic3 = 1
+    p.gam = cfg["gamma"]
+    p.cfl = cfg.get("cfl", 0.9)
+    p.dt_min = cfg.get("dt_min", 1e-5)
+    p.dt_start = cfg.get("dt_start", p.dt_min)
+    p.max_dt_increase = cfg.get("max_dt_increase", 2.0)
+    p.tlim = cfg.get("tlim", 1e30)
+    p.recon = RECON[cfg.get("recon", "weno5")]
+    p.use_hlle = int(cfg.get("riemann", "llf") == "hlle")
+    p.floors_on = int(cfg.get("floors", True))
+    p.recon_floors = int(cfg.get("recon_floors", p.floors_on and cfg.get("recon", "weno5") == "weno5"))
+    p.floor_frame = FRAME[cfg.get("floor_frame", "drift")]
+    fl = cfg.get("floor_values", {})
+    p.rho_min_geom = fl.get("rho_min_geom", 1e-6)
+    p.u_min_geom = fl.get("u_min_geom", 1e-8)
+    p.rho_min_const = fl.get("rho_min_const", 1e-8)
+    p.u_min_const = fl.get("u_min_const", 1e-10)
+    p.r_char = fl.get("r_char", 10.0)
+    p.bsq_over_rho_max = fl.get("bsq_over_rho_max", 1e20)
+    p.bsq_over_u_max = fl.get("bsq_over_u_max", 1e20)
+    p.u_over_rho_max = fl.get("u_over_rho_max", 1e20)
+    p.ktot_max = fl.get("ktot_max", 1e20)
+    p.gamma_max = fl.get("gamma_max", 50.0)
+    p.use_r_char = int(fl.get("use_r_char", False))
+    p.temp_adjust_u = int(fl.get("temp_adjust_u", False))
+    p.iter_max = cfg.get("iter_max", 8)
+    p.err_tol = cfg.get("err_tol", 1e-8)
+    p.fix_average = int(cfg.get("fix_average", True))
+    p.fix_atmo = int(cfg.get("fix_atmo", True))
+    p.integrator = {"rk2": 0, "vl2": 1}[cfg.get("integrator", "rk2")]
+    p.coords = {"cartesian_minkowski": 0}[cfg.get("coords", "cartesian_minkowski")]
+    p.precision = {"fp64": PREC_FP64, "split": PREC_SPLIT}[cfg.get("precision", "fp64")]
+    return p
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise KharmaError("kharma_amd needs a HIP device (torch.cuda.is_available() is False)")
+    return torch
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _stream():
+    torch = _torch()
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class Block:
+    """One meshblock's geometry + index ranges; task-level entry points."""
+
+    def __init__(self, cfg):
+        self.cfg = cfg
+        self._p = make_params(cfg)
+        h = ctypes.c_void_p()
+        _check(lib().kh_block_create(ctypes.byref(self._p), ctypes.byref(h)))
+        self.h = h
+        n1, n2, n3, nc = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_long()
+        _check(lib().kh_block_shape(self.h, ctypes.byref(n1), ctypes.byref(n2), ctypes.byref(n3), ctypes.byref(nc)))
+        self.shape = (n3.value, n2.value, n1.value)
+        self.nc = nc.value
+        self.split = self._p.precision == PREC_SPLIT
+
+    def __del__(self):
+        try:
+            lib().kh_block_destroy(self.h)
+        except Exception:
+            pass
+
+    @property
+    def flux_dtype(self):
+        torch = _torch()
+        return torch.float32 if self.split else torch.float64
+
+    def _bind(self):
+        _check(lib().kh_block_set_stream(self.h, _stream()))
+
+    def new_fluxes(self):
+        torch = _torch()
+        F = torch.zeros((3, NV) + self.shape, dtype=self.flux_dtype, device="cuda")
+        cmax = torch.zeros((3,) + self.shape, dtype=self.flux_dtype, device="cuda")
+        cmin = torch.zeros((3,) + self.shape, dtype=self.flux_dtype, device="cuda")
+        return F, cmax, cmin
+
+    def get_flux(self, d, P, F, cmax, cmin):
+        self._bind()
+        _check(lib().kh_get_flux(self.h, d, _ptr(P), _ptr(F[d - 1]), _ptr(cmax[d - 1]), _ptr(cmin[d - 1])))
+
+    def flux_ct(self, F, emf):
+        self._bind()
+        _check(lib().kh_flux_ct(self.h, _ptr(F), _ptr(emf)))
+
+    def flux_divergence(self, F, dUdt):
+        self._bind()
+        _check(lib().kh_flux_divergence(self.h, _ptr(F), _ptr(dUdt)))
+
+    def weighted_sum(self, Usub, Ufull, dUdt, gam0, gam1, beta_dt, Uout):
+        self._bind()
+        _check(lib().kh_weighted_sum(self.h, _ptr(Usub), _ptr(Ufull), _ptr(dUdt), gam0, gam1, beta_dt, _ptr(Uout)))
+
+    def utop(self, U, P, pflag, fflag):
+        self._bind()
+        _check(lib().kh_utop(self.h, _ptr(U), _ptr(P), _ptr(pflag), _ptr(fflag)))
+
+    def apply_floors(self, U, P, pflag, fflag):
+        self._bind()
+        _check(lib().kh_apply_floors(self.h, _ptr(U), _ptr(P), _ptr(pflag), _ptr(fflag)))
+
+    def fix_utop(self, U, P, pflag):
+        self._bind()
+        _check(lib().kh_fix_utop(self.h, _ptr(U), _ptr(P), _ptr(pflag)))
+
+    def ptou(self, P, U):
+        self._bind()
+        _check(lib().kh_ptou(self.h, _ptr(P), _ptr(U)))
+
+    def sync_bounds(self, A, nvar):
+        self._bind()
+        _check(lib().kh_sync_bounds(self.h, _ptr(A), nvar))
+
+    def estimate_timestep(self, cmax, cmin, dt_last):
+        self._bind()
+        out = ctypes.c_double()
+        _check(lib().kh_estimate_timestep(self.h, _ptr(cmax), _ptr(cmin), dt_last, ctypes.byref(out)))
+        return out.value
+
+    def max_divb(self, U):
+        self._bind()
+        out = ctypes.c_double()
+        _check(lib().kh_max_divb(self.h, _ptr(U), ctypes.byref(out)))
+        return out.value
+
+
+class Driver:
+    """KHARMA's default stage driver for one periodic block, run on the GPU."""
+
+    def __init__(self, cfg):
+        self.cfg = cfg
+        self._p = make_params(cfg)
+        h = ctypes.c_void_p()
+        _check(lib().kh_driver_create(ctypes.byref(self._p), ctypes.byref(h)))
+        self.h = h
+        b = ctypes.c_void_p()
+        _check(lib().kh_driver_block(self.h, ctypes.byref(b)))
+        n1, n2, n3, nc = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_long()
+        _check(lib().kh_block_shape(b, ctypes.byref(n1), ctypes.byref(n2), ctypes.byref(n3), ctypes.byref(nc)))
+        self.shape = (n3.value, n2.value, n1.value)
+        self.nc = nc.value
+
+    def __del__(self):
+        try:
+            lib().kh_driver_destroy(self.h)
+        except Exception:
+            pass
+
+    def _bind(self):
+        _check(lib().kh_driver_set_stream(self.h, _stream()))
+
+    def init_prims(self, P):
+        torch = _torch()
+        P = torch.as_tensor(P, dtype=torch.float64).to("cuda").contiguous()
+        assert tuple(P.shape) == (NV,) + self.shape, (P.shape, self.shape)
+        self._bind()
+        _check(lib().kh_driver_init_prims(self.h, _ptr(P)))
+        self._P_init = P  # keep alive until the copy is done
+
+    def step(self, n=1):
+        self._bind()
+        _check(lib().kh_driver_step(self.h, n))
+
+    def state(self):
+        torch = _torch()
+        P = torch.empty((NV,) + self.shape, dtype=torch.float64, device="cuda")
+        U = torch.empty_like(P)
+        self._bind()
+        _check(lib().kh_driver_get_state(self.h, _ptr(P), _ptr(U)))
+        return P, U
+
+    def scalars(self):
+        t, dt, divb = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        nc, nf = ctypes.c_int(), ctypes.c_int()
+        self._bind()
+        _check(lib().kh_driver_get_scalars(self.h, ctypes.byref(t), ctypes.byref(dt), ctypes.byref(divb),
+                                           ctypes.byref(nc), ctypes.byref(nf)))
+        return dict(time=t.value, dt=dt.value, divb=divb.value, ncycle=nc.value, fails=nf.value)
+
+    KERNEL_KINDS = ("flux_x1", "flux_x2", "flux_x3", "emf", "update_s1", "update_s2", "fixup", "divb", "finalize")
+
+    def set_kernel_timing(self, enable):
+        _check(lib().kh_driver_set_kernel_timing(self.h, int(bool(enable))))
+
+    def kernel_times(self):
+        """{kind: (total_ms, launches)} since the last call (synchronises)."""
+        n = len(self.KERNEL_KINDS)
+        ms = (ctypes.c_double * n)()
+        cnt = (ctypes.c_int * n)()
+        _check(lib().kh_driver_kernel_times(self.h, ms, cnt, n))
+        return {k: (ms[q], cnt[q]) for q, k in enumerate(self.KERNEL_KINDS)}
+
+    def dt_history(self, n):
+        arr = (ctypes.c_double * n)()
+        self._bind()
+        _check(lib().kh_driver_get_dt_history(self.h, arr, n))
+        return list(arr)

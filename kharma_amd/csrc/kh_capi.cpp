@@ -1,0 +1,453 @@
+// kh_capi.cpp -- the C ABI (include/kharma_amd.h) and the C++ stage driver.
+//
+// Host code only: it validates arguments, fills the index/constant structs,
+// and enqueues the gfx950 kernels of kh_flux.hip / kh_step.hip on one stream.
+// No exception crosses the ABI; every HIP status is checked.
+#include "../../include/kharma_amd.h"
+
+#include <cfloat>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "kh_internal.hpp"
+
+using namespace kh;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define KH_HIP(call)                                                                         \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess) return fail(KH_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+}  // namespace
+
+struct kh_block {
+    kh_params p;
+    BlockDims dims;
+    StepConsts consts;
+    hipStream_t stream = nullptr;
+    DeviceState* scratch = nullptr;  // reductions of the task-level entry points
+};
+
+struct kh_driver {
+    kh_block* blk = nullptr;
+    double *U0 = nullptr, *P0 = nullptr, *U1 = nullptr, *P1 = nullptr;
+    void *F = nullptr, *cmax = nullptr, *cmin = nullptr, *emf = nullptr;
+    int *pflag = nullptr, *fflag = nullptr, *fail_list = nullptr;
+    DeviceState* ds = nullptr;
+    double* dt_hist = nullptr;
+    int hist_cap = 0;
+    // kernel timing (kh_driver_set_kernel_timing)
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    struct Rec { int kind; size_t a, b; };
+    std::vector<Rec> recs;
+
+    hipError_t mark(hipEvent_t* out) {
+        if (ev_used == ev_pool.size()) {
+            hipEvent_t e;
+            hipError_t st = hipEventCreate(&e);
+            if (st != hipSuccess) return st;
+            ev_pool.push_back(e);
+        }
+        *out = ev_pool[ev_used++];
+        return hipEventRecord(*out, blk->stream);
+    }
+};
+
+namespace {
+
+int validate(const kh_params* p) {
+    if (!p) return fail(KH_ERR_INVALID, "null params");
+    if (p->nx1 < 1 || p->nx2 < 1 || p->nx3 < 1) return fail(KH_ERR_INVALID, "zone counts must be >= 1");
+    if (p->ng < 3) return fail(KH_ERR_INVALID, "need at least 3 ghost zones for a 5-point reconstruction");
+    if (p->nx3 > 1 && p->nx2 == 1) return fail(KH_ERR_INVALID, "3D blocks need nx2 > 1");
+    if (p->coords != 0) return fail(KH_ERR_UNSUPPORTED, "only cartesian_minkowski coordinates are implemented");
+    if (!p->periodic1 || (p->nx2 > 1 && !p->periodic2) || (p->nx3 > 1 && !p->periodic3))
+        return fail(KH_ERR_UNSUPPORTED, "only periodic boundaries are implemented");
+    if (p->nx1 < p->ng || (p->nx2 > 1 && p->nx2 < p->ng) || (p->nx3 > 1 && p->nx3 < p->ng))
+        return fail(KH_ERR_INVALID, "periodic dimensions need at least ng interior zones");
+    if (p->recon < 0 || p->recon > 3) return fail(KH_ERR_INVALID, "unknown reconstruction");
+    if (p->precision != KH_PREC_FP64 && p->precision != KH_PREC_SPLIT) return fail(KH_ERR_INVALID, "unknown precision");
+    if (p->integrator != 0 && p->integrator != 1) return fail(KH_ERR_INVALID, "unknown integrator");
+    if (p->floor_frame < 0 || p->floor_frame > 2) return fail(KH_ERR_INVALID, "unknown floor frame");
+    if (p->iter_max < 0) return fail(KH_ERR_INVALID, "iter_max < 0");
+    const double n = (double)(p->nx1 + 2 * p->ng) * (p->nx2 + 2 * p->ng) * (p->nx3 + 2 * p->ng);
+    if (n * 8 > 2.0e9) return fail(KH_ERR_INVALID, "block too large for 32-bit zone indexing");
+    return KH_OK;
+}
+
+void fill_dims(const kh_params* p, BlockDims& b) {
+    b.ndim = (p->nx3 > 1) ? 3 : ((p->nx2 > 1) ? 2 : 1);
+    b.ng = p->ng;
+    b.nx1 = p->nx1; b.nx2 = p->nx2; b.nx3 = p->nx3;
+    b.n1 = p->nx1 + 2 * p->ng;
+    b.n2 = (b.ndim > 1) ? p->nx2 + 2 * p->ng : 1;
+    b.n3 = (b.ndim > 2) ? p->nx3 + 2 * p->ng : 1;
+    b.is = p->ng; b.ie = p->ng + p->nx1 - 1;
+    b.js = (b.ndim > 1) ? p->ng : 0; b.je = (b.ndim > 1) ? p->ng + p->nx2 - 1 : 0;
+    b.ks = (b.ndim > 2) ? p->ng : 0; b.ke = (b.ndim > 2) ? p->ng + p->nx3 - 1 : 0;
+    b.periodic[0] = p->periodic1; b.periodic[1] = p->periodic2; b.periodic[2] = p->periodic3;
+    b.s2 = b.n1;
+    b.s3 = (long)b.n1 * b.n2;
+    b.nc = (long)b.n1 * b.n2 * b.n3;
+    b.dx1 = (p->x1max - p->x1min) / p->nx1;
+    b.dx2 = (p->x2max - p->x2min) / p->nx2;
+    b.dx3 = (p->x3max - p->x3min) / p->nx3;
+}
+
+void fill_consts(const kh_params* p, StepConsts& c) {
+    c.gam = p->gam; c.cfl = p->cfl; c.dt_min = p->dt_min; c.max_dt_increase = p->max_dt_increase; c.tlim = p->tlim;
+    if (p->integrator == 1) {  // vl2
+        c.gam0[0] = 0.; c.gam1[0] = 1.; c.beta[0] = 0.5;
+        c.gam0[1] = 0.; c.gam1[1] = 1.; c.beta[1] = 1.0;
+    } else {                   // rk2
+        c.gam0[0] = 0.; c.gam1[0] = 1.; c.beta[0] = 1.0;
+        c.gam0[1] = 0.5; c.gam1[1] = 0.5; c.beta[1] = 0.5;
+    }
+    c.recon_floors = p->recon_floors;
+    c.iter_max = p->iter_max;
+    c.err_tol = p->err_tol;
+    c.fix_average = p->fix_average;
+    c.fix_atmo = p->fix_atmo;
+    c.use_hlle = p->use_hlle;
+    FloorParams& f = c.floors;
+    f.rho_min_geom = p->rho_min_geom; f.u_min_geom = p->u_min_geom;
+    f.rho_min_const = p->rho_min_const; f.u_min_const = p->u_min_const; f.r_char = p->r_char;
+    f.bsq_over_rho_max = p->bsq_over_rho_max; f.bsq_over_u_max = p->bsq_over_u_max;
+    f.u_over_rho_max = p->u_over_rho_max; f.ktot_max = p->ktot_max; f.gamma_max = p->gamma_max;
+    f.use_r_char = p->use_r_char; f.temp_adjust_u = p->temp_adjust_u; f.frame = p->floor_frame;
+    f.on = p->floors_on;
+}
+
+size_t flux_elem(const kh_block* b) { return b->p.precision == KH_PREC_SPLIT ? sizeof(float) : sizeof(double); }
+
+}  // namespace
+
+extern "C" {
+
+const char* kh_last_error(void) { return g_err.c_str(); }
+int kh_version(void) { return 1; }
+
+int kh_block_create(const kh_params* p, kh_block** out) {
+    if (!out) return fail(KH_ERR_INVALID, "null output pointer");
+    *out = nullptr;
+    int st = validate(p);
+    if (st != KH_OK) return st;
+    kh_block* b = new kh_block();
+    b->p = *p;
+    fill_dims(p, b->dims);
+    fill_consts(p, b->consts);
+    hipError_t e = hipMalloc(&b->scratch, sizeof(DeviceState));
+    if (e != hipSuccess) { delete b; return fail(KH_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e)); }
+    *out = b;
+    return KH_OK;
+}
+
+void kh_block_destroy(kh_block* b) {
+    if (!b) return;
+    if (b->scratch) (void)hipFree(b->scratch);
+    delete b;
+}
+
+int kh_block_set_stream(kh_block* b, void* s) {
+    if (!b) return fail(KH_ERR_INVALID, "null block");
+    b->stream = (hipStream_t)s;
+    return KH_OK;
+}
+
+int kh_block_shape(const kh_block* b, int* n1, int* n2, int* n3, long* nc) {
+    if (!b) return fail(KH_ERR_INVALID, "null block");
+    if (n1) *n1 = b->dims.n1;
+    if (n2) *n2 = b->dims.n2;
+    if (n3) *n3 = b->dims.n3;
+    if (nc) *nc = b->dims.nc;
+    return KH_OK;
+}
+
+int kh_get_flux(kh_block* b, int dir, const double* P, void* F_dir, void* cmax_dir, void* cmin_dir) {
+    if (!b || !P || !F_dir || !cmax_dir || !cmin_dir) return fail(KH_ERR_INVALID, "null argument");
+    if (dir < 1 || dir > 3) return fail(KH_ERR_INVALID, "dir must be 1, 2 or 3");
+    KH_HIP(launch_flux(b->dims, b->consts, dir, b->p.recon, b->p.precision, P, F_dir, cmax_dir, cmin_dir, b->stream));
+    return KH_OK;
+}
+
+int kh_flux_ct(kh_block* b, void* F, void* emf) {
+    if (!b || !F || !emf) return fail(KH_ERR_INVALID, "null argument");
+    KH_HIP(launch_emf(b->dims, b->p.precision, F, emf, b->stream));
+    KH_HIP(launch_flux_ct_rewrite(b->dims, b->p.precision, F, emf, b->stream));
+    return KH_OK;
+}
+
+int kh_flux_divergence(kh_block* b, const void* F, double* dUdt) {
+    if (!b || !F || !dUdt) return fail(KH_ERR_INVALID, "null argument");
+    KH_HIP(launch_flux_divergence(b->dims, b->p.precision, F, dUdt, b->stream));
+    return KH_OK;
+}
+
+int kh_weighted_sum(kh_block* b, const double* Usub, const double* Ufull, const double* dUdt,
+                    double gam0, double gam1, double beta_dt, double* Uout) {
+    if (!b || !Usub || !Ufull || !dUdt || !Uout) return fail(KH_ERR_INVALID, "null argument");
+    KH_HIP(launch_weighted_sum(b->dims, Usub, Ufull, dUdt, gam0, gam1, beta_dt, Uout, b->stream));
+    return KH_OK;
+}
+
+int kh_utop(kh_block* b, const double* U, double* P, int* pflag, int* fflag) {
+    if (!b || !U || !P || !pflag || !fflag) return fail(KH_ERR_INVALID, "null argument");
+    KH_HIP(launch_utop(b->dims, b->consts, U, P, pflag, fflag, b->stream));
+    return KH_OK;
+}
+
+int kh_apply_floors(kh_block* b, double* U, double* P, int* pflag, int* fflag) {
+    if (!b || !U || !P || !pflag || !fflag) return fail(KH_ERR_INVALID, "null argument");
+    KH_HIP(launch_floors(b->dims, b->consts, U, P, pflag, fflag, b->stream));
+    return KH_OK;
+}
+
+int kh_fix_utop(kh_block* b, double* U, double* P, const int* pflag) {
+    if (!b || !U || !P || !pflag) return fail(KH_ERR_INVALID, "null argument");
+    KH_HIP(launch_fixup_entire(b->dims, b->consts, U, P, pflag, b->stream));
+    return KH_OK;
+}
+
+int kh_ptou(kh_block* b, const double* P, double* U) {
+    if (!b || !P || !U) return fail(KH_ERR_INVALID, "null argument");
+    KH_HIP(launch_ptou(b->dims, b->consts, P, U, b->stream));
+    return KH_OK;
+}
+
+int kh_sync_bounds(kh_block* b, double* A, int nvar) {
+    if (!b || !A || nvar < 1) return fail(KH_ERR_INVALID, "bad argument");
+    KH_HIP(launch_sync(b->dims, A, nvar, b->stream));
+    return KH_OK;
+}
+
+int kh_estimate_timestep(kh_block* b, const void* cmax, const void* cmin, double dt_last, double* ndt) {
+    if (!b || !cmax || !cmin || !ndt) return fail(KH_ERR_INVALID, "null argument");
+    KH_HIP(launch_reset_state(b->scratch, 0., b->stream));
+    KH_HIP(launch_estimate_dt(b->dims, b->consts, b->p.precision, cmax, cmin, b->scratch, b->stream));
+    DeviceState h;
+    KH_HIP(hipMemcpyAsync(&h, b->scratch, sizeof(h), hipMemcpyDeviceToHost, b->stream));
+    KH_HIP(hipStreamSynchronize(b->stream));
+    double m;
+    std::memcpy(&m, &h.min_ndt_bits, sizeof(m));
+    const double hi = b->p.max_dt_increase * dt_last;
+    double v = m * b->p.cfl;
+    v = (v < b->p.dt_min) ? b->p.dt_min : v;   // min(max(lower, n), upper)
+    v = (hi < v) ? hi : v;
+    *ndt = v;
+    return KH_OK;
+}
+
+int kh_max_divb(kh_block* b, const double* U, double* out) {
+    if (!b || !U || !out) return fail(KH_ERR_INVALID, "null argument");
+    KH_HIP(launch_reset_state(b->scratch, 0., b->stream));
+    KH_HIP(launch_max_divb(b->dims, U, b->scratch, b->stream));
+    DeviceState h;
+    KH_HIP(hipMemcpyAsync(&h, b->scratch, sizeof(h), hipMemcpyDeviceToHost, b->stream));
+    KH_HIP(hipStreamSynchronize(b->stream));
+    std::memcpy(out, &h.divb_bits, sizeof(double));
+    return KH_OK;
+}
+
+// ------------------------------------------------------------------ driver
+int kh_driver_create(const kh_params* p, kh_driver** out) {
+    if (!out) return fail(KH_ERR_INVALID, "null output pointer");
+    *out = nullptr;
+    kh_block* blk = nullptr;
+    int st = kh_block_create(p, &blk);
+    if (st != KH_OK) return st;
+    kh_driver* d = new kh_driver();
+    d->blk = blk;
+    const long nc = blk->dims.nc;
+    const size_t st_bytes = sizeof(double) * NV * nc;
+    const size_t fe = flux_elem(blk);
+    d->hist_cap = 1 << 16;
+    auto cleanup = [&](hipError_t e, const char* what) {
+        kh_driver_destroy(d);
+        return fail(KH_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    };
+    hipError_t e;
+    if ((e = hipMalloc(&d->U0, st_bytes)) != hipSuccess) return cleanup(e, "hipMalloc U0");
+    if ((e = hipMalloc(&d->P0, st_bytes)) != hipSuccess) return cleanup(e, "hipMalloc P0");
+    if ((e = hipMalloc(&d->U1, st_bytes)) != hipSuccess) return cleanup(e, "hipMalloc U1");
+    if ((e = hipMalloc(&d->P1, st_bytes)) != hipSuccess) return cleanup(e, "hipMalloc P1");
+    if ((e = hipMalloc(&d->F, fe * 3 * NV * nc)) != hipSuccess) return cleanup(e, "hipMalloc F");
+    if ((e = hipMalloc(&d->cmax, fe * 3 * nc)) != hipSuccess) return cleanup(e, "hipMalloc cmax");
+    if ((e = hipMalloc(&d->cmin, fe * 3 * nc)) != hipSuccess) return cleanup(e, "hipMalloc cmin");
+    if ((e = hipMalloc(&d->emf, fe * 3 * nc)) != hipSuccess) return cleanup(e, "hipMalloc emf");
+    if ((e = hipMalloc(&d->pflag, sizeof(int) * nc)) != hipSuccess) return cleanup(e, "hipMalloc pflag");
+    if ((e = hipMalloc(&d->fflag, sizeof(int) * nc)) != hipSuccess) return cleanup(e, "hipMalloc fflag");
+    if ((e = hipMalloc(&d->fail_list, sizeof(int) * nc)) != hipSuccess) return cleanup(e, "hipMalloc fail_list");
+    if ((e = hipMalloc(&d->ds, sizeof(DeviceState))) != hipSuccess) return cleanup(e, "hipMalloc state");
+    if ((e = hipMalloc(&d->dt_hist, sizeof(double) * d->hist_cap)) != hipSuccess) return cleanup(e, "hipMalloc dt_hist");
+    // Zero everything once at creation (outside any captured work): the signal
+    // speeds of an inactive direction must read 0, as the reference's zero-initialised
+    // Flux.cmax/cmin do for X3 in 2D.
+    if ((e = hipMemset(d->U0, 0, st_bytes)) != hipSuccess) return cleanup(e, "hipMemset");
+    if ((e = hipMemset(d->P0, 0, st_bytes)) != hipSuccess) return cleanup(e, "hipMemset");
+    if ((e = hipMemset(d->U1, 0, st_bytes)) != hipSuccess) return cleanup(e, "hipMemset");
+    if ((e = hipMemset(d->P1, 0, st_bytes)) != hipSuccess) return cleanup(e, "hipMemset");
+    if ((e = hipMemset(d->F, 0, fe * 3 * NV * nc)) != hipSuccess) return cleanup(e, "hipMemset");
+    if ((e = hipMemset(d->cmax, 0, fe * 3 * nc)) != hipSuccess) return cleanup(e, "hipMemset");
+    if ((e = hipMemset(d->cmin, 0, fe * 3 * nc)) != hipSuccess) return cleanup(e, "hipMemset");
+    if ((e = hipMemset(d->emf, 0, fe * 3 * nc)) != hipSuccess) return cleanup(e, "hipMemset");
+    if ((e = hipMemset(d->pflag, 0, sizeof(int) * nc)) != hipSuccess) return cleanup(e, "hipMemset");
+    if ((e = hipMemset(d->fflag, 0, sizeof(int) * nc)) != hipSuccess) return cleanup(e, "hipMemset");
+    if ((e = hipMemset(d->dt_hist, 0, sizeof(double) * d->hist_cap)) != hipSuccess) return cleanup(e, "hipMemset");
+    if ((e = hipDeviceSynchronize()) != hipSuccess) return cleanup(e, "hipDeviceSynchronize");
+    *out = d;
+    return KH_OK;
+}
+
+void kh_driver_destroy(kh_driver* d) {
+    if (!d) return;
+    (void)hipDeviceSynchronize();
+    for (hipEvent_t e : d->ev_pool) (void)hipEventDestroy(e);
+    for (void* ptr : {(void*)d->U0, (void*)d->P0, (void*)d->U1, (void*)d->P1, d->F, d->cmax, d->cmin, d->emf,
+                      (void*)d->pflag, (void*)d->fflag, (void*)d->fail_list, (void*)d->ds, (void*)d->dt_hist})
+        if (ptr) (void)hipFree(ptr);
+    kh_block_destroy(d->blk);
+    delete d;
+}
+
+int kh_driver_set_stream(kh_driver* d, void* s) {
+    if (!d) return fail(KH_ERR_INVALID, "null driver");
+    return kh_block_set_stream(d->blk, s);
+}
+
+int kh_driver_block(kh_driver* d, kh_block** out) {
+    if (!d || !out) return fail(KH_ERR_INVALID, "null argument");
+    *out = d->blk;
+    return KH_OK;
+}
+
+int kh_driver_arrays(kh_driver* d, double** U, double** P) {
+    if (!d) return fail(KH_ERR_INVALID, "null driver");
+    if (U) *U = d->U0;
+    if (P) *P = d->P0;
+    return KH_OK;
+}
+
+int kh_driver_init_prims(kh_driver* d, const double* P) {
+    if (!d || !P) return fail(KH_ERR_INVALID, "null argument");
+    kh_block* b = d->blk;
+    const size_t st_bytes = sizeof(double) * NV * b->dims.nc;
+    KH_HIP(hipMemcpyAsync(d->P0, P, st_bytes, hipMemcpyDeviceToDevice, b->stream));
+    KH_HIP(launch_sync(b->dims, d->P0, NV, b->stream));
+    KH_HIP(launch_ptou(b->dims, b->consts, d->P0, d->U0, b->stream));
+    KH_HIP(launch_sync(b->dims, d->U0, NV, b->stream));
+    KH_HIP(hipMemcpyAsync(d->P1, d->P0, st_bytes, hipMemcpyDeviceToDevice, b->stream));
+    KH_HIP(hipMemcpyAsync(d->U1, d->U0, st_bytes, hipMemcpyDeviceToDevice, b->stream));
+    KH_HIP(launch_reset_state(d->ds, b->p.dt_start, b->stream));
+    return KH_OK;
+}
+
+// One full step: two stages of the default task collection, then the divB
+// diagnostic and the device-side timestep update.
+// Launch `expr` with optional start/stop events for kernel kind `kind`.
+#define KH_TIMED(kind, expr)                                                   \
+    do {                                                                        \
+        size_t ia_ = 0;                                                         \
+        hipEvent_t ev_;                                                         \
+        if (d->timing) { ia_ = d->ev_used; KH_HIP(d->mark(&ev_)); }            \
+        KH_HIP(expr);                                                           \
+        if (d->timing) { KH_HIP(d->mark(&ev_)); d->recs.push_back({(kind), ia_, d->ev_used - 1}); } \
+    } while (0)
+
+int kh_driver_step(kh_driver* d, int nsteps) {
+    if (!d) return fail(KH_ERR_INVALID, "null driver");
+    kh_block* b = d->blk;
+    const BlockDims& bd = b->dims;
+    const StepConsts& c = b->consts;
+    const int prec = b->p.precision;
+    const size_t fe = flux_elem(b);
+    hipStream_t s = b->stream;
+    char* F = (char*)d->F;
+    char* cmx = (char*)d->cmax;
+    char* cmn = (char*)d->cmin;
+    for (int n = 0; n < nsteps; ++n) {
+        for (int stage = 1; stage <= 2; ++stage) {
+            const double* Psub = (stage == 1) ? d->P0 : d->P1;
+            const double* Usub = (stage == 1) ? d->U0 : d->U1;
+            double* Pout = (stage == 1) ? d->P1 : d->P0;
+            double* Uout = (stage == 1) ? d->U1 : d->U0;
+            for (int dir = 1; dir <= bd.ndim; ++dir)
+                KH_TIMED(dir - 1, launch_flux(bd, c, dir, b->p.recon, prec, Psub, F + fe * (dir - 1) * NV * bd.nc,
+                                              cmx + fe * (dir - 1) * bd.nc, cmn + fe * (dir - 1) * bd.nc, s));
+            KH_TIMED(3, launch_emf(bd, prec, d->F, d->emf, s));
+            KH_TIMED(3 + stage, launch_update(bd, c, prec, stage, stage == 2, d->F, d->emf, d->cmax, d->cmin,
+                                              Usub, d->U0, Psub, Uout, Pout, d->pflag, d->fflag, d->fail_list, d->ds, s));
+            if (c.fix_average || c.fix_atmo)
+                KH_TIMED(6, launch_fixup(bd, c, stage, Uout, Pout, d->pflag, d->fail_list, d->ds, s));
+        }
+        KH_TIMED(7, launch_max_divb(bd, d->U0, d->ds, s));
+        KH_TIMED(8, launch_finalize(c, d->ds, d->dt_hist, d->hist_cap, s));
+    }
+    return KH_OK;
+}
+
+int kh_driver_get_state(kh_driver* d, double* P, double* U) {
+    if (!d) return fail(KH_ERR_INVALID, "null driver");
+    kh_block* b = d->blk;
+    const size_t st_bytes = sizeof(double) * NV * b->dims.nc;
+    if (P) KH_HIP(hipMemcpyAsync(P, d->P0, st_bytes, hipMemcpyDeviceToDevice, b->stream));
+    if (U) KH_HIP(hipMemcpyAsync(U, d->U0, st_bytes, hipMemcpyDeviceToDevice, b->stream));
+    return KH_OK;
+}
+
+int kh_driver_get_scalars(kh_driver* d, double* time, double* dt, double* last_divb, int* ncycle, int* total_fails) {
+    if (!d) return fail(KH_ERR_INVALID, "null driver");
+    DeviceState h;
+    KH_HIP(hipMemcpyAsync(&h, d->ds, sizeof(h), hipMemcpyDeviceToHost, d->blk->stream));
+    KH_HIP(hipStreamSynchronize(d->blk->stream));
+    if (time) *time = h.time;
+    if (dt) *dt = h.dt;
+    if (last_divb) *last_divb = h.last_divb;
+    if (ncycle) *ncycle = h.ncycle;
+    if (total_fails) *total_fails = h.total_fails;
+    return KH_OK;
+}
+
+int kh_driver_set_kernel_timing(kh_driver* d, int enable) {
+    if (!d) return fail(KH_ERR_INVALID, "null driver");
+    d->timing = enable != 0;
+    return KH_OK;
+}
+
+int kh_driver_kernel_times(kh_driver* d, double* total_ms, int* launches, int nkinds) {
+    if (!d || !total_ms || !launches || nkinds < KH_NKERNEL_KINDS) return fail(KH_ERR_INVALID, "bad argument");
+    KH_HIP(hipStreamSynchronize(d->blk->stream));
+    for (int k = 0; k < nkinds; ++k) { total_ms[k] = 0.; launches[k] = 0; }
+    for (const auto& r : d->recs) {
+        float ms = 0.f;
+        KH_HIP(hipEventElapsedTime(&ms, d->ev_pool[r.a], d->ev_pool[r.b]));
+        total_ms[r.kind] += ms;
+        launches[r.kind] += 1;
+    }
+    d->recs.clear();
+    d->ev_used = 0;
+    return KH_OK;
+}
+
+int kh_driver_get_dt_history(kh_driver* d, double* out, int n) {
+    if (!d || !out || n < 0) return fail(KH_ERR_INVALID, "bad argument");
+    if (n > d->hist_cap) return fail(KH_ERR_INVALID, "history request exceeds capacity");
+    KH_HIP(hipMemcpyAsync(out, d->dt_hist, sizeof(double) * n, hipMemcpyDeviceToHost, d->blk->stream));
+    KH_HIP(hipStreamSynchronize(d->blk->stream));
+    return KH_OK;
+}
+
+}  // extern "C"

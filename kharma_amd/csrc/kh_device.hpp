@@ -1,0 +1,498 @@
+// kh_device.hpp -- device-side GRMHD physics for the gfx950 kernels.
+//
+// Every function here is templated on the arithmetic type R (double for the
+// plain FP64 path, float for the split-precision flux path) and on a geometry
+// policy G that supplies the metric at one point.  MinkowskiGeom specialises
+// lower/raise/lorentz/alpha to the flat metric so that no 4x4 metric is read or
+// multiplied; its results are identical to contracting with diag(-1,1,1,1).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace kh {
+
+enum { RHO = 0, UU = 1, U1 = 2, U2 = 3, U3 = 4, B1 = 5, B2 = 6, B3 = 7, NV = 8 };
+
+// Floor flag bits (floors/floors.hpp:47-68) and inversion status codes
+// (inverter/invert_template.hpp:53): the flag arrays carry the same values.
+constexpr int F_GEOM_RHO = 32, F_GEOM_U = 64, F_B_RHO = 128, F_B_U = 256, F_TEMP = 512,
+              F_GAMMA = 1024, F_KTOT = 2048, F_GEOM_RHO_FLUX = 4096, F_GEOM_U_FLUX = 8192;
+enum { S_SUCCESS = 0, S_NEG_INPUT, S_MAX_ITER, S_BAD_UT, S_BAD_GAMMA, S_NEG_RHO, S_NEG_U, S_NEG_RHOU };
+
+template <typename R> struct FourVec { R ucon[4], ucov[4], bcon[4], bcov[4]; };
+
+template <typename R> __device__ __forceinline__ R dot4(const R a[4], const R b[4]) {
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3];
+}
+template <typename R> __device__ __forceinline__ R dmax(R a, R b) { return a > b ? a : (b != b ? b : a); }
+// std::max / std::min semantics (return the first argument unless the second compares greater/less)
+template <typename R> __device__ __forceinline__ R smax(R a, R b) { return (a < b) ? b : a; }
+template <typename R> __device__ __forceinline__ R smin(R a, R b) { return (b < a) ? b : a; }
+// kharma_utils.hpp:100-110: min(max(lower, n), upper)
+template <typename R> __device__ __forceinline__ R clip(R n, R lo, R hi) { return smin(smax(lo, n), hi); }
+
+// Flat space, Cartesian coordinates.
+template <typename R> struct MinkowskiGeom {
+    static constexpr bool flat = true;
+    __device__ __forceinline__ R gdet() const { return R(1); }
+    __device__ __forceinline__ void lower(const R v[4], R o[4]) const { o[0] = -v[0]; o[1] = v[1]; o[2] = v[2]; o[3] = v[3]; }
+    __device__ __forceinline__ void raise(const R v[4], R o[4]) const { o[0] = -v[0]; o[1] = v[1]; o[2] = v[2]; o[3] = v[3]; }
+    __device__ __forceinline__ R lorentz(R u1, R u2, R u3) const {
+        const R qsq = u1 * u1 + u2 * u2 + u3 * u3;
+        return sqrt(R(1) + qsq);
+    }
+};
+
+// ------------------------------------------------------------ reconstruction
+// WENO5 of Tchekhovskoy+07 / Shu 2011 as in flux/reconstruction.hpp:128-204.
+// The nonlinear weights (c_i / (EPS+beta_i)^2) / sum are evaluated in the
+// equivalent scaled form c_i r_i^2 / sum(c r^2), r_i = min_j(EPS+beta_j)/(EPS+beta_i),
+// which cannot under/overflow in FP32 (EPS^2 = 1e-52 underflows a float).
+template <typename R>
+__device__ __forceinline__ void weno5_beta(R x1, R x2, R x3, R x4, R x5, R t[3]) {
+    constexpr R EPS = R(1.e-26);
+    R c1 = x1 - R(2) * x2 + x3, c2 = x1 - R(4) * x2 + R(3) * x3;
+    t[0] = EPS + (R(13. / 12.) * c1 * c1 + R(0.25) * c2 * c2);
+    c1 = x2 - R(2) * x3 + x4; c2 = x4 - x2;
+    t[1] = EPS + (R(13. / 12.) * c1 * c1 + R(0.25) * c2 * c2);
+    c1 = x3 - R(2) * x4 + x5; c2 = x5 - R(4) * x4 + R(3) * x3;
+    t[2] = EPS + (R(13. / 12.) * c1 * c1 + R(0.25) * c2 * c2);
+}
+// value at the right face of the centre cell (x3)
+template <typename R>
+__device__ __forceinline__ R weno5_right(R x1, R x2, R x3, R x4, R x5) {
+    R t[3];
+    weno5_beta(x1, x2, x3, x4, x5, t);
+    const R tm = smin(smin(t[0], t[1]), t[2]);
+    const R r0 = tm / t[0], r1 = tm / t[1], r2 = tm / t[2];
+    const R w0 = R(1. / 16.) * r0 * r0, w1 = R(5. / 8.) * r1 * r1, w2 = R(5. / 16.) * r2 * r2;
+    const R iW = R(1) / (w0 + w1 + w2);
+    return ((R(3. / 8.) * x1 - R(5. / 4.) * x2 + R(15. / 8.) * x3) * (w0 * iW) +
+            (R(-1. / 8.) * x2 + R(3. / 4.) * x3 + R(3. / 8.) * x4) * (w1 * iW) +
+            (R(3. / 8.) * x3 + R(3. / 4.) * x4 - R(1. / 8.) * x5) * (w2 * iW));
+}
+// value at the left face of the centre cell (x3)
+template <typename R>
+__device__ __forceinline__ R weno5_left(R x1, R x2, R x3, R x4, R x5) {
+    R t[3];
+    weno5_beta(x1, x2, x3, x4, x5, t);
+    const R tm = smin(smin(t[0], t[1]), t[2]);
+    const R r0 = tm / t[0], r1 = tm / t[1], r2 = tm / t[2];
+    const R w0 = R(1. / 16.) * r2 * r2, w1 = R(5. / 8.) * r1 * r1, w2 = R(5. / 16.) * r0 * r0;
+    const R iW = R(1) / (w0 + w1 + w2);
+    return ((R(3. / 8.) * x5 - R(5. / 4.) * x4 + R(15. / 8.) * x3) * (w0 * iW) +
+            (R(-1. / 8.) * x4 + R(3. / 4.) * x3 + R(3. / 8.) * x2) * (w1 * iW) +
+            (R(3. / 8.) * x3 + R(3. / 4.) * x2 - R(1. / 8.) * x1) * (w2 * iW));
+}
+// MC-limited linear (reconstruction.hpp:52-56,107-123)
+template <typename R> __device__ __forceinline__ R mc_lim(R dm, R dp) {
+    const R r = (fabs(dp) > R(0) ? dm / dp : R(2));
+    return smax(R(0), smin(R(2), smin(R(2) * r, R(0.5) * (R(1) + r))));
+}
+// PPM with the CW84 limiter (reconstruction.hpp:351-382)
+template <typename R>
+__device__ __forceinline__ void ppm(R q_im2, R q_im1, R q_i, R q_ip1, R q_ip2, R& qlv, R& qrv) {
+    qlv = (R(7) * (q_i + q_im1) - (q_im2 + q_ip1)) / R(12);
+    qrv = (R(7) * (q_i + q_ip1) - (q_im1 + q_ip2)) / R(12);
+    qlv = smax(qlv, smin(q_i, q_im1));
+    qlv = smin(qlv, smax(q_i, q_im1));
+    qrv = smax(qrv, smin(q_i, q_ip1));
+    qrv = smin(qrv, smax(q_i, q_ip1));
+    const R qc = qrv - q_i, qd = qlv - q_i;
+    if ((qc * qd) >= R(0)) { qlv = q_i; qrv = q_i; }
+    else {
+        if (fabs(qc) >= R(2) * fabs(qd)) qrv = q_i - R(2) * qd;
+        if (fabs(qd) >= R(2) * fabs(qc)) qlv = q_i - R(2) * qc;
+    }
+}
+
+enum Recon { RECON_DONOR = 0, RECON_LINEAR_MC = 1, RECON_WENO5 = 2, RECON_PPM = 3 };
+
+// Face states from a 6-point stencil x[-3..2] around face "0" (left face of cell 0):
+// ql = right edge of cell -1, qr = left edge of cell 0.
+template <int RECON, typename R>
+__device__ __forceinline__ void face_states(const R x[6], R& ql, R& qr) {
+    if constexpr (RECON == RECON_WENO5) {
+        ql = weno5_right(x[0], x[1], x[2], x[3], x[4]);
+        qr = weno5_left(x[1], x[2], x[3], x[4], x[5]);
+    } else if constexpr (RECON == RECON_LINEAR_MC) {
+        ql = x[2] + R(0.5) * (mc_lim(x[2] - x[1], x[3] - x[2]) * (x[3] - x[2]));
+        qr = x[3] - R(0.5) * (mc_lim(x[3] - x[2], x[4] - x[3]) * (x[4] - x[3]));
+    } else if constexpr (RECON == RECON_PPM) {
+        R a, b;
+        ppm(x[0], x[1], x[2], x[3], x[4], a, ql);
+        ppm(x[1], x[2], x[3], x[4], x[5], qr, b);
+    } else {
+        ql = x[2];
+        qr = x[3];
+    }
+}
+
+// ----------------------------------------------------------------- GRMHD
+// GRMHD::calc_4vecs, grmhd/grmhd_functions.hpp:225-246
+template <typename R, class G>
+__device__ __forceinline__ void calc_4vecs(const G& g, const R P[NV], FourVec<R>& D) {
+    const R gamma = g.lorentz(P[U1], P[U2], P[U3]);
+    if constexpr (G::flat) {
+        D.ucon[0] = gamma;
+        D.ucon[1] = P[U1]; D.ucon[2] = P[U2]; D.ucon[3] = P[U3];
+    } else {
+        const R alpha = g.alpha();
+        D.ucon[0] = gamma / alpha;
+        for (int v = 0; v < 3; ++v) D.ucon[v + 1] = P[U1 + v] - gamma * alpha * g.gcon0(v + 1);
+    }
+    g.lower(D.ucon, D.ucov);
+    D.bcon[0] = P[B1] * D.ucov[1] + P[B2] * D.ucov[2] + P[B3] * D.ucov[3];
+    const R iu0 = R(1) / D.ucon[0];
+    for (int v = 0; v < 3; ++v) D.bcon[v + 1] = (P[B1 + v] + D.bcon[0] * D.ucon[v + 1]) * iu0;
+    g.lower(D.bcon, D.bcov);
+}
+
+// Flux::prim_to_flux for dir = 0 (conserved) and dir = d together, sharing the
+// tensor invariants.  flux_functions.hpp:116-178 + grmhd_functions.hpp:142-155.
+template <typename R, class G, int DIR>
+__device__ __forceinline__ void prim_to_u_and_flux(const G& g, const R P[NV], const FourVec<R>& D, R gam,
+                                                   R U[NV], R F[NV]) {
+    const R gdet = g.gdet();
+    const R bsq = dot4(D.bcon, D.bcov);
+    const R pgas = (gam - R(1)) * P[UU];
+    const R eta = pgas + P[RHO] + P[UU] + bsq;
+    const R ptot = pgas + R(0.5) * bsq;
+    // dir = 0
+    U[RHO] = P[RHO] * D.ucon[0] * gdet;
+    {
+        const R eu = eta * D.ucon[0];
+        U[UU] = (eu * D.ucov[0] + ptot - D.bcon[0] * D.bcov[0]) * gdet + U[RHO];
+        U[U1] = (eu * D.ucov[1] - D.bcon[0] * D.bcov[1]) * gdet;
+        U[U2] = (eu * D.ucov[2] - D.bcon[0] * D.bcov[2]) * gdet;
+        U[U3] = (eu * D.ucov[3] - D.bcon[0] * D.bcov[3]) * gdet;
+        U[B1] = P[B1] * gdet; U[B2] = P[B2] * gdet; U[B3] = P[B3] * gdet;
+    }
+    // dir = DIR
+    F[RHO] = P[RHO] * D.ucon[DIR] * gdet;
+    {
+        const R eu = eta * D.ucon[DIR];
+        const R bd = D.bcon[DIR];
+        F[UU] = (eu * D.ucov[0] - bd * D.bcov[0]) * gdet + F[RHO];
+        F[U1] = (eu * D.ucov[1] + (DIR == 1 ? ptot : R(0)) - bd * D.bcov[1]) * gdet;
+        F[U2] = (eu * D.ucov[2] + (DIR == 2 ? ptot : R(0)) - bd * D.bcov[2]) * gdet;
+        F[U3] = (eu * D.ucov[3] + (DIR == 3 ? ptot : R(0)) - bd * D.bcov[3]) * gdet;
+        const R ud = D.ucon[DIR];
+        F[B1] = (D.bcon[1] * ud - bd * D.ucon[1]) * gdet;
+        F[B2] = (D.bcon[2] * ud - bd * D.ucon[2]) * gdet;
+        F[B3] = (D.bcon[3] * ud - bd * D.ucon[3]) * gdet;
+    }
+}
+
+// Flux::vchar, flux_functions.hpp:368-439 (the two clip() calls there are no-ops,
+// so cs2 and cms2 are deliberately left unclamped here too).
+template <typename R, class G, int DIR>
+__device__ __forceinline__ void vchar(const G& g, const R P[NV], const FourVec<R>& D, R gam, R& cmax, R& cmin) {
+    const R ef = P[RHO] + gam * P[UU];
+    const R cs2 = gam * (gam - R(1)) * P[UU] / ef;
+    const R bsq = dot4(D.bcon, D.bcov);
+    const R va2 = bsq / (bsq + ef);
+    const R cms2 = cs2 + va2 - cs2 * va2;
+    R A, B, C;
+    if constexpr (G::flat) {
+        const R Au = D.ucon[DIR], Bu = D.ucon[0];
+        A = Bu * Bu - (R(-1) + Bu * Bu) * cms2;
+        B = R(2) * (Au * Bu - (Au * Bu) * cms2);
+        C = Au * Au - (R(1) + Au * Au) * cms2;
+    } else {
+        const R Asq = g.gcon(DIR, DIR), Bsq = g.gcon(0, 0), AB = g.gcon(0, DIR);
+        const R Au = D.ucon[DIR], Bu = D.ucon[0];
+        A = Bu * Bu - (Bsq + Bu * Bu) * cms2;
+        B = R(2) * (Au * Bu - (AB + Au * Bu) * cms2);
+        C = Au * Au - (Asq + Au * Au) * cms2;
+    }
+    const R discr = sqrt(smax(B * B - R(4) * A * C, R(0)));
+    const R i2A = R(1) / (R(2) * A);
+    const R vp = -(-B + discr) * i2A;
+    const R vm = -(-B - discr) * i2A;
+    cmax = smax(vp, vm);
+    cmin = smin(vp, vm);
+}
+
+// ----------------------------------------------------------- inversion (FP64)
+// Inverter::u_to_p<onedw>, inverter/onedw.hpp:49-233.  Always FP64.
+__device__ __forceinline__ double lorentz_calc_w(double Bsq, double D, double QdB, double Qtsq, double Wp) {
+    const double QdBsq = QdB * QdB;
+    const double W = Wp + D;
+    const double W2 = W * W;
+    const double WB = W + Bsq;
+    const double utsq = -((W + WB) * QdBsq + W2 * Qtsq) / (QdBsq * (W + WB) + W2 * (Qtsq - WB * WB));
+    if (utsq < -1.e-15 || utsq > 1.e7) return -1.;
+    return sqrt(1. + fabs(utsq));
+}
+__device__ __forceinline__ double err_eqn(double gam, double Bsq, double D, double Ep, double QdB, double Qtsq,
+                                          double Wp, int& eflag) {
+    const double W = Wp + D;
+    const double gamma = lorentz_calc_w(Bsq, D, QdB, Qtsq, Wp);
+    if (gamma < 1) eflag = S_BAD_UT;
+    const double w = W / (gamma * gamma);
+    const double rho = D / gamma;
+    const double p = (w - rho) * (gam - 1) / gam;
+    const double bw = Bsq + W;
+    return -Ep + Wp - p + 0.5 * Bsq + 0.5 * (Bsq * Qtsq - QdB * QdB) / (bw * bw);
+}
+// U: 8 conserved at the point; P: in = guess (rho,u,u1..3), out = recovered (only on success)
+template <class G>
+__device__ __forceinline__ int u_to_p_onedw(const G& g, const double U[NV], double P[NV], double gam,
+                                            int max_iterations, double tol) {
+    if (U[RHO] <= 0.) return S_NEG_INPUT;
+    double alpha, a_over_g;
+    if constexpr (G::flat) { alpha = 1.; a_over_g = 1.; }
+    else { alpha = g.alpha(); a_over_g = alpha / g.gdet(); }
+    const double D = U[RHO] * a_over_g;
+    double Bcon[4] = {0., U[B1] * a_over_g, U[B2] * a_over_g, U[B3] * a_over_g};
+    const double Qcov[4] = {(U[UU] - U[RHO]) * a_over_g, U[U1] * a_over_g, U[U2] * a_over_g, U[U3] * a_over_g};
+    const double ncov[4] = {-alpha, 0., 0., 0.};
+    double Bcov[4], Qcon[4], ncon[4];
+    g.lower(Bcon, Bcov);
+    g.raise(Qcov, Qcon);
+    if constexpr (G::flat) { ncon[0] = alpha; ncon[1] = 0.; ncon[2] = 0.; ncon[3] = 0.; }
+    else g.raise(ncov, ncon);
+    const double Bsq = dot4(Bcon, Bcov);
+    const double QdB = dot4(Bcon, Qcov);
+    const double Qdotn = dot4(Qcon, ncov);
+    double Qtcon[4];
+    for (int mu = 0; mu < 4; ++mu) Qtcon[mu] = Qcon[mu] + ncon[mu] * Qdotn;
+    const double Qtsq = dot4(Qcon, Qcov) + Qdotn * Qdotn;
+    const double Ep = -Qdotn - D;
+    int eflag = S_SUCCESS;
+    double Wp, err;
+    {
+        const double gamma = g.lorentz(P[U1], P[U2], P[U3]);
+        if (gamma < 1) return S_BAD_UT;
+        const double rho = P[RHO], u = P[UU];
+        Wp = (rho + u + (gam - 1) * u) * gamma * gamma - rho * gamma;
+        err = err_eqn(gam, Bsq, D, Ep, QdB, Qtsq, Wp, eflag);
+    }
+    double dW;
+    {
+        const double Wpm = (1. - 1.e-5) * Wp;
+        const double h = Wp - Wpm;
+        const double Wpp = Wp + h;
+        const double errm = err_eqn(gam, Bsq, D, Ep, QdB, Qtsq, Wpm, eflag);
+        const double errp = err_eqn(gam, Bsq, D, Ep, QdB, Qtsq, Wpp, eflag);
+        const double dedW = (errp - errm) / (Wpp - Wpm);
+        const double dedW2 = (errp - 2. * err + errm) / (h * h);
+        const double f = clip(0.5 * err * dedW2 / (dedW * dedW), -0.3, 0.3);
+        dW = clip(-err / dedW / (1. - f), -0.5 * Wp, 2.0 * Wp);
+    }
+    double Wp1 = Wp;
+    double err1 = err;
+    Wp += dW;
+    err = err_eqn(gam, Bsq, D, Ep, QdB, Qtsq, Wp, eflag);
+    int iter = 0;
+    for (iter = 0; iter < max_iterations; iter++) {
+        dW = clip((Wp1 - Wp) * err / (err - err1), -0.5 * Wp, 2.0 * Wp);
+        Wp1 = Wp;
+        err1 = err;
+        Wp += dW;
+        if (fabs(dW / Wp) < tol) break;
+        err = err_eqn(gam, Bsq, D, Ep, QdB, Qtsq, Wp, eflag);
+        if (fabs(err / Wp) < tol) break;
+    }
+    if (iter == max_iterations) return S_MAX_ITER;
+    const double gamma = lorentz_calc_w(Bsq, D, QdB, Qtsq, Wp);
+    if (gamma < 1) return S_BAD_UT;
+    const double rho = D / gamma;
+    const double W = Wp + D;
+    const double w = W / (gamma * gamma);
+    const double p = (w - rho) * (gam - 1) / gam;
+    const double u = w - (rho + p);
+    if (rho < 0 && u < 0) return S_NEG_RHOU;
+    else if (rho < 0) return S_NEG_RHO;
+    else if (u < 0) return S_NEG_U;
+    P[RHO] = rho;
+    P[UU] = u;
+    const double pre = (gamma / (W + Bsq));
+    P[U1] = pre * (Qtcon[1] + QdB * Bcon[1] / W);
+    P[U2] = pre * (Qtcon[2] + QdB * Bcon[2] / W);
+    P[U3] = pre * (Qtcon[3] + QdB * Bcon[3] / W);
+    return S_SUCCESS;
+}
+
+// Conserved variables of all 8 fields (Flux::p_to_u, flux_functions.hpp:343-352)
+template <class G>
+__device__ __forceinline__ void p_to_u(const G& g, const double P[NV], double gam, double U[NV]) {
+    FourVec<double> D;
+    calc_4vecs<double>(g, P, D);
+    const double gdet = g.gdet();
+    const double bsq = dot4(D.bcon, D.bcov);
+    const double pgas = (gam - 1.) * P[UU];
+    const double eta = pgas + P[RHO] + P[UU] + bsq;
+    const double ptot = pgas + 0.5 * bsq;
+    const double eu = eta * D.ucon[0];
+    U[RHO] = P[RHO] * D.ucon[0] * gdet;
+    U[UU] = (eu * D.ucov[0] + ptot - D.bcon[0] * D.bcov[0]) * gdet + U[RHO];
+    U[U1] = (eu * D.ucov[1] - D.bcon[0] * D.bcov[1]) * gdet;
+    U[U2] = (eu * D.ucov[2] - D.bcon[0] * D.bcov[2]) * gdet;
+    U[U3] = (eu * D.ucov[3] - D.bcon[0] * D.bcov[3]) * gdet;
+    U[B1] = P[B1] * gdet; U[B2] = P[B2] * gdet; U[B3] = P[B3] * gdet;
+}
+
+// ------------------------------------------------------------------- floors
+struct FloorParams {
+    double rho_min_geom, u_min_geom, rho_min_const, u_min_const, r_char;
+    double bsq_over_rho_max, bsq_over_u_max, u_over_rho_max, ktot_max, gamma_max;
+    int use_r_char, temp_adjust_u, frame, on;
+};
+
+// Floors::determine_floors, floors/floors_functions.hpp:84-160 (non-spherical geometric floors)
+template <class G>
+__device__ __forceinline__ int determine_floors(const G& g, const double P[NV], double gam, const FloorParams& f,
+                                                double& rhoflr_max, double& uflr_max) {
+    const double rhoflr_geom = f.rho_min_const;
+    const double uflr_geom = f.u_min_const;
+    FourVec<double> D;
+    calc_4vecs<double>(g, P, D);
+    const double bsq = dot4(D.bcon, D.bcov);
+    const double rhoflr_b = bsq / f.bsq_over_rho_max;
+    const double uflr_b = bsq / f.bsq_over_u_max;
+    uflr_max = smax(uflr_geom, uflr_b);
+    const double rho = P[RHO], u = P[UU];
+    int fflag = 0;
+    if (!f.temp_adjust_u) {
+        const double rhoflr_temp = smax(u, uflr_max) / f.u_over_rho_max;
+        fflag |= (rhoflr_temp > rho) * F_TEMP;
+        rhoflr_max = smax(smax(rhoflr_geom, rhoflr_b), rhoflr_temp);
+    } else {
+        rhoflr_max = smax(rhoflr_geom, rhoflr_b);
+    }
+    if (rhoflr_max > rho || uflr_max > u) {
+        fflag |= (rhoflr_geom > rho) * F_GEOM_RHO;
+        fflag |= (uflr_geom > u) * F_GEOM_U;
+        fflag |= (rhoflr_b > rho) * F_B_RHO;
+        fflag |= (uflr_b > u) * F_B_U;
+    }
+    if (g.lorentz(P[U1], P[U2], P[U3]) > f.gamma_max) fflag |= F_GAMMA;
+    if ((gam - 1.) * P[UU] / pow(P[RHO], gam) > f.ktot_max) fflag |= F_KTOT;
+    if (f.temp_adjust_u && (P[UU] / P[RHO] > f.u_over_rho_max)) fflag |= F_TEMP;
+    return fflag;
+}
+
+// Drift-frame floors, floors_functions.hpp:186-257
+template <class G>
+__device__ __forceinline__ void apply_floors_drift(const G& g, double P[NV], double gam, double rhoflr_max, double uflr_max) {
+    double beta[4] = {0., 0., 0., 0.};
+    if constexpr (!G::flat) {
+        const double lapse2 = 1. / (-g.gcon(0, 0));
+        beta[1] = lapse2 * g.gcon(0, 1);
+        beta[2] = lapse2 * g.gcon(0, 2);
+        beta[3] = lapse2 * g.gcon(0, 3);
+    }
+    const double rho = P[RHO], uu = P[UU];
+    const double pg = (gam - 1.) * uu;
+    const double w_old = smax(rho + uu + pg, 1e-20);
+    double Bcon[4] = {0., P[B1], P[B2], P[B3]};
+    double Bcov[4];
+    g.lower(Bcon, Bcov);
+    const double Bsq = smax(dot4(Bcon, Bcov), 1e-20);
+    const double B_mag = sqrt(Bsq);
+    FourVec<double> D;
+    calc_4vecs<double>(g, P, D);
+    double Qcov[4];
+    Qcov[0] = w_old * D.ucon[0] * D.ucov[0] + pg;
+    Qcov[1] = w_old * D.ucon[0] * D.ucov[1];
+    Qcov[2] = w_old * D.ucon[0] * D.ucov[2];
+    Qcov[3] = w_old * D.ucon[0] * D.ucov[3];
+    const double QdotB = dot4(Bcon, Qcov);
+    double vpar = QdotB / (B_mag * w_old * D.ucon[0] * D.ucon[0]);
+    double ucon_dr[4];
+    ucon_dr[0] = 1. / sqrt(1. / (D.ucon[0] * D.ucon[0]) + vpar * vpar);
+    for (int mu = 0; mu < 4; ++mu)
+        ucon_dr[mu] = D.ucon[mu] * (ucon_dr[0] / D.ucon[0]) - (vpar * Bcon[mu] * ucon_dr[0] / B_mag);
+    P[RHO] = smax(rho, rhoflr_max);
+    P[UU] = smax(uu, uflr_max);
+    const double pg_new = (gam - 1.) * P[UU];
+    const double w_new = P[RHO] + P[UU] + pg_new;
+    const double x = (2. * QdotB) / (B_mag * w_new * ucon_dr[0]);
+    vpar = x / (1 + sqrt(1 + x * x)) * (1. / ucon_dr[0]);
+    double ucon[4];
+    ucon[0] = 1. / sqrt(1 / (ucon_dr[0] * ucon_dr[0]) - vpar * vpar);
+    for (int mu = 0; mu < 4; ++mu) ucon[mu] = ucon_dr[mu] * (ucon[0] / ucon_dr[0]) + (vpar * Bcon[mu] * ucon[0] / B_mag);
+    P[U1] = ucon[1] + (beta[1] * ucon[0]);
+    P[U2] = ucon[2] + (beta[2] * ucon[0]);
+    P[U3] = ucon[3] + (beta[3] * ucon[0]);
+}
+
+// Normal-observer-frame floors, floors_functions.hpp:260-289 (U is modified in registers)
+template <class G>
+__device__ __forceinline__ int apply_floors_normal(const G& g, double P[NV], double U[NV], double gam,
+                                                   double rhoflr_max, double uflr_max) {
+    const double rho_add = smax(0., rhoflr_max - P[RHO]);
+    const double u_add = smax(0., uflr_max - P[UU]);
+    double Padd[NV] = {rho_add, u_add, 0., 0., 0., 0., 0., 0.};
+    double Uadd[NV];
+    p_to_u(g, Padd, gam, Uadd);
+    P[RHO] += rho_add;
+    P[UU] += u_add;
+    for (int v = 0; v < 5; ++v) U[v] += Uadd[v];
+    return u_to_p_onedw(g, U, P, gam, 8, 1e-8);
+}
+
+// Floors::apply_ceilings, floors_functions.hpp:50-82
+template <class G>
+__device__ __forceinline__ void apply_ceilings(const G& g, double P[NV], double gam, const FloorParams& f) {
+    const double gamma = g.lorentz(P[U1], P[U2], P[U3]);
+    const double ktot = (gam - 1.) * P[UU] / pow(P[RHO], gam);
+    if (gamma > f.gamma_max) {
+        const double fr = sqrt((f.gamma_max * f.gamma_max - 1.) / (gamma * gamma - 1.));
+        P[U1] *= fr; P[U2] *= fr; P[U3] *= fr;
+    }
+    if (ktot > f.ktot_max) P[UU] = f.ktot_max / ktot * P[UU];
+    if (f.temp_adjust_u && P[UU] / P[RHO] > f.u_over_rho_max) P[UU] = f.u_over_rho_max * P[RHO];
+}
+
+// One zone of Floors::ApplyFloorsInFrame (floors_impl.hpp:45-128): determine,
+// apply in the configured frame, ceilings.  Returns fflag; may overwrite pflag.
+template <class G>
+__device__ __forceinline__ int floors_zone(const G& g, double P[NV], double U[NV], double gam, const FloorParams& f,
+                                           int fflag_in, int& pflag) {
+    double rf, uf;
+    const int fflag = fflag_in | determine_floors(g, P, gam, f, rf, uf);
+    if (fflag) {
+        int pflag_l = 0;
+        if (f.frame == 0) {
+            P[RHO] += smax(0., rf - P[RHO]);
+            P[UU] += smax(0., uf - P[UU]);
+        } else if (f.frame == 1) {
+            pflag_l = apply_floors_normal(g, P, U, gam, rf, uf);
+        } else {
+            apply_floors_drift(g, P, gam, rf, uf);
+        }
+        if (pflag_l) pflag = pflag_l;
+        apply_ceilings(g, P, gam, f);
+    }
+    return fflag;
+}
+
+// Geometric floors on one state (Floors::apply_geo_floors, floors_functions.hpp:300-367)
+template <typename R>
+__device__ __forceinline__ void apply_geo_floors(R P[NV], R rho_min, R u_min) {
+    P[RHO] += smax(R(0), rho_min - P[RHO]);
+    P[UU] += smax(R(0), u_min - P[UU]);
+}
+
+// ------------------------------------------------------------ reductions
+// Wavefront reduction with cross-lane shuffles (wave64), then LDS across waves.
+__device__ __forceinline__ double wave_min(double v) {
+    for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+// Non-negative doubles order like their bit patterns: atomic min/max on u64.
+__device__ __forceinline__ void atomic_min_pos(unsigned long long* addr, double v) {
+    atomicMin(addr, (unsigned long long)__double_as_longlong(v));
+}
+__device__ __forceinline__ void atomic_max_pos(unsigned long long* addr, double v) {
+    atomicMax(addr, (unsigned long long)__double_as_longlong(v));
+}
+
+}  // namespace kh

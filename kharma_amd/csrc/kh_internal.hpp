@@ -1,0 +1,89 @@
+// kh_internal.hpp -- shared host/device declarations between the kernel
+// translation units and the C-ABI layer.  Not part of the public interface.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "kh_device.hpp"
+
+namespace kh {
+
+// Index space of one meshblock.  Fields are SoA [var][k][j][i] over the entire
+// block (interior + ng ghosts per side in each active dimension), i fastest:
+// the layout of KHARMA's VariablePack(v, k, j, i).
+struct BlockDims {
+    int n1, n2, n3;        // entire sizes
+    int nx1, nx2, nx3;     // interior sizes
+    int is, ie, js, je, ks, ke;
+    int ng, ndim;
+    int periodic[3];
+    long s2, s3;           // strides of j and k
+    long nc;               // stride between variables
+    double dx1, dx2, dx3;
+};
+
+// Scalars that change every step live on the device so a run of steps needs no
+// host round trip (dt is produced and consumed on the GPU).
+struct DeviceState {
+    double dt, dt_last, time;
+    unsigned long long min_ndt_bits;   // running min of the CFL estimate (non-negative double bits)
+    unsigned long long divb_bits;      // running max of |divB| (non-negative double bits)
+    double last_divb;
+    double last_ndt;
+    int fail_count[2];                 // failed inversions per stage of the current step
+    int ncycle;
+    int total_fails;
+};
+
+// Run-wide constants of the step.
+struct StepConsts {
+    double gam, cfl, dt_min, max_dt_increase, tlim;
+    double gam0[2], gam1[2], beta[2];   // integrator weights per stage
+    int recon_floors;
+    int iter_max;
+    double err_tol;
+    int fix_average, fix_atmo;
+    int use_hlle;
+    FloorParams floors;
+};
+
+enum Precision { PREC_FP64 = 0, PREC_SPLIT = 1 };
+
+// ---- launchers (kh_flux.hip) ----
+// Face fluxes through the faces normal to `dir` that the step consumes:
+// X1 faces i in [is, ie+1], j in [js-1, je+1], k in [ks-1, ke+1] (collapsed in
+// inactive dimensions), cyclically for X2/X3.  F/cmax/cmin point at this
+// direction's arrays ([8][nc] and [nc]); element type float when prec == PREC_SPLIT.
+hipError_t launch_flux(const BlockDims& b, const StepConsts& c, int dir, int recon, int prec,
+                       const double* P, void* F, void* cmax, void* cmin, hipStream_t s);
+
+// ---- launchers (kh_step.hip) ----
+hipError_t launch_emf(const BlockDims& b, int prec, const void* F, void* emf, hipStream_t s);
+hipError_t launch_flux_ct_rewrite(const BlockDims& b, int prec, void* F, const void* emf, hipStream_t s);
+hipError_t launch_flux_divergence(const BlockDims& b, int prec, const void* F, double* dUdt, hipStream_t s);
+hipError_t launch_update(const BlockDims& b, const StepConsts& c, int prec, int stage, bool do_dt,
+                         const void* F, const void* emf, const void* cmax, const void* cmin,
+                         const double* Usub, const double* Ufull, const double* Psub,
+                         double* Uout, double* Pout, int* pflag, int* fflag,
+                         int* fail_list, DeviceState* ds, hipStream_t s);
+hipError_t launch_fixup(const BlockDims& b, const StepConsts& c, int stage, double* U, double* P,
+                        const int* pflag, const int* fail_list, DeviceState* ds, hipStream_t s);
+hipError_t launch_finalize(const StepConsts& c, DeviceState* ds, double* dt_hist, int hist_cap, hipStream_t s);
+hipError_t launch_max_divb(const BlockDims& b, const double* U, DeviceState* ds, hipStream_t s);
+hipError_t launch_reset_state(DeviceState* ds, double dt, hipStream_t s);
+// Task-level kernels over the entire block (the reference's IndexDomain::entire).
+hipError_t launch_weighted_sum(const BlockDims& b, const double* Usub, const double* Ufull, const double* dUdt,
+                               double gam0, double gam1, double beta_dt, double* Uout, hipStream_t s);
+hipError_t launch_utop(const BlockDims& b, const StepConsts& c, const double* U, double* P, int* pflag, int* fflag,
+                       hipStream_t s);
+hipError_t launch_floors(const BlockDims& b, const StepConsts& c, double* U, double* P, int* pflag, int* fflag,
+                         hipStream_t s);
+hipError_t launch_fixup_entire(const BlockDims& b, const StepConsts& c, double* U, double* P, const int* pflag,
+                               hipStream_t s);
+hipError_t launch_ptou(const BlockDims& b, const StepConsts& c, const double* P, double* U, hipStream_t s);
+hipError_t launch_sync(const BlockDims& b, double* A, int nvar, hipStream_t s);
+hipError_t launch_sync_int(const BlockDims& b, int* A, hipStream_t s);
+hipError_t launch_estimate_dt(const BlockDims& b, const StepConsts& c, int prec, const void* cmax, const void* cmin,
+                              DeviceState* ds, hipStream_t s);
+
+}  // namespace kh

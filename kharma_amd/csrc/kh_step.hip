@@ -1,0 +1,614 @@
+// kh_step.hip -- flux-CT, the fused stage update, inversion fix-ups, reductions
+// and the task-level kernels that mirror the reference's individual tasks.
+//
+// The fused update kernel performs, for one interior zone and in registers:
+//   flux-CT B fluxes from the corner EMFs (B_FluxCT::FluxCT, b_flux_ct.cpp:347-369)
+//   + flux divergence (KHARMADriver::FluxDivergence, kharma_driver.hpp:208-231)
+//   + RK state update (KHARMADriver::AddStateUpdate, kharma_driver.cpp:339-387)
+//   + onedw U->P (Inverter::BlockUtoP, inverter.cpp:131-182) and B U->P (b_flux_ct.cpp:195-217)
+//   + floors (Floors::ApplyGRMHDFloors, floors_impl.hpp:45-128)
+//   + P->U (Flux::MeshPtoU, flux.cpp:276-322)
+//   + (last stage) the CFL estimate (GRMHD::EstimateTimestep, grmhd.cpp:313-328)
+// and writes U, P and the flags once, also into the periodic ghost images of
+// the zone (the boundary exchange), so no separate sync pass reads the state.
+#include "kh_internal.hpp"
+
+#include <cfloat>
+
+namespace kh {
+
+static constexpr int BS = 256;
+
+static inline dim3 grid_for(long n, int bs = BS) { return dim3((unsigned)((n + bs - 1) / bs)); }
+
+// ------------------------------------------------------------ ghost images
+// Periodic images of interior zone (k,j,i) inside the ghost region: offsets
+// per axis (0 = the zone itself).  Needs nx >= ng on periodic axes.
+struct ImageSet {
+    int ni, nj, nk;
+    int di[3], dj[3], dk[3];
+};
+__device__ __forceinline__ void image_set(const BlockDims& b, int k, int j, int i, ImageSet& s) {
+    s.ni = s.nj = s.nk = 1;
+    s.di[0] = s.dj[0] = s.dk[0] = 0;
+    if (b.periodic[0]) {
+        if (i - b.is < b.ng) s.di[s.ni++] = b.nx1;
+        if (b.ie - i < b.ng) s.di[s.ni++] = -b.nx1;
+    }
+    if (b.ndim > 1 && b.periodic[1]) {
+        if (j - b.js < b.ng) s.dj[s.nj++] = b.nx2;
+        if (b.je - j < b.ng) s.dj[s.nj++] = -b.nx2;
+    }
+    if (b.ndim > 2 && b.periodic[2]) {
+        if (k - b.ks < b.ng) s.dk[s.nk++] = b.nx3;
+        if (b.ke - k < b.ng) s.dk[s.nk++] = -b.nx3;
+    }
+}
+template <typename T>
+__device__ __forceinline__ void store_with_images(const BlockDims& b, const ImageSet& s, T* base, long c, T val) {
+    base[c] = val;
+    if (s.ni * s.nj * s.nk == 1) return;
+    for (int a = 0; a < s.nk; ++a)
+        for (int e = 0; e < s.nj; ++e)
+            for (int d = 0; d < s.ni; ++d) {
+                if (a == 0 && e == 0 && d == 0) continue;
+                base[c + (long)s.dk[a] * b.s3 + (long)s.dj[e] * b.s2 + s.di[d]] = val;
+            }
+}
+
+__device__ __forceinline__ void interior_index(const BlockDims& b, unsigned t, int& k, int& j, int& i) {
+    const unsigned nij = (unsigned)b.nx1 * (unsigned)(b.ndim > 1 ? b.nx2 : 1);
+    const unsigned kk = t / nij;
+    const unsigned rem = t - kk * nij;
+    const unsigned jj = rem / (unsigned)b.nx1;
+    i = b.is + (int)(rem - jj * (unsigned)b.nx1);
+    j = b.js + (int)jj;
+    k = b.ks + (int)kk;
+}
+__device__ __forceinline__ long ninterior(const BlockDims& b) {
+    return (long)b.nx1 * (b.ndim > 1 ? b.nx2 : 1) * (b.ndim > 2 ? b.nx3 : 1);
+}
+
+// ----------------------------------------------------------------- flux-CT
+// Corner EMFs, b_flux_ct.cpp:331-342: emf_d at corners i in [is, ie+1],
+// j in [js, je+1], k in [ks, ke(+1 in 3D)].  F = [3][8][nc], emf = [3][nc].
+template <typename R>
+__global__ void __launch_bounds__(BS) emf_kernel(BlockDims b, const R* __restrict__ F, R* __restrict__ emf) {
+    const int ni = b.nx1 + 1, nj = b.nx2 + 1, nk = (b.ndim > 2) ? b.nx3 + 1 : 1;
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (unsigned)(ni * nj * nk)) return;
+    const int kk = t / (ni * nj), rem = t - kk * (ni * nj), jj = rem / ni;
+    const int i = b.is + rem - jj * ni, j = b.js + jj, k = b.ks + kk;
+    const long c = (long)k * b.s3 + (long)j * b.s2 + i;
+    const R* F1 = F;
+    const R* F2 = F + 8 * b.nc;
+    const R* F3 = F + 16 * b.nc;
+    if (b.ndim > 2) {
+        emf[c] = R(0.25) * (F2[B3 * b.nc + c] + F2[B3 * b.nc + c - b.s3] - F3[B2 * b.nc + c] - F3[B2 * b.nc + c - b.s2]);
+        emf[b.nc + c] = R(0.25) * (F3[B1 * b.nc + c] + F3[B1 * b.nc + c - 1] - F1[B3 * b.nc + c] - F1[B3 * b.nc + c - b.s3]);
+    }
+    emf[2 * b.nc + c] = R(0.25) * (F1[B2 * b.nc + c] + F1[B2 * b.nc + c - b.s2] - F2[B1 * b.nc + c] - F2[B1 * b.nc + c - 1]);
+}
+
+// Rewrite the B fluxes from the EMFs (b_flux_ct.cpp:347-369), for the task-level API.
+template <typename R>
+__global__ void __launch_bounds__(BS) ct_rewrite_kernel(BlockDims b, R* __restrict__ F, const R* __restrict__ emf) {
+    const int ni = b.nx1 + 1, nj = b.nx2 + 1, nk = (b.ndim > 2) ? b.nx3 + 1 : 1;
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (unsigned)(ni * nj * nk)) return;
+    const int kk = t / (ni * nj), rem = t - kk * (ni * nj), jj = rem / ni;
+    const int i = b.is + rem - jj * ni, j = b.js + jj, k = b.ks + kk;
+    const long c = (long)k * b.s3 + (long)j * b.s2 + i;
+    R* F1 = F;
+    R* F2 = F + 8 * b.nc;
+    R* F3 = F + 16 * b.nc;
+    const R* e1 = emf;
+    const R* e2 = emf + b.nc;
+    const R* e3 = emf + 2 * b.nc;
+    const bool d3 = b.ndim > 2;
+    const bool kin = k <= b.ke, jin = j <= b.je, iin = i <= b.ie;
+    if (kin && jin) {  // flux_ct_1: i in [is, ie+1]
+        F1[B1 * b.nc + c] = R(0);
+        F1[B2 * b.nc + c] = R(0.5) * (e3[c] + e3[c + b.s2]);
+        if (d3) F1[B3 * b.nc + c] = R(-0.5) * (e2[c] + e2[c + b.s3]);
+    }
+    if (kin && iin) {  // flux_ct_2: j in [js, je+1]
+        F2[B1 * b.nc + c] = R(-0.5) * (e3[c] + e3[c + 1]);
+        F2[B2 * b.nc + c] = R(0);
+        if (d3) F2[B3 * b.nc + c] = R(0.5) * (e1[c] + e1[c + b.s3]);
+    }
+    if (d3 && jin && iin) {  // flux_ct_3: k in [ks, ke+1]
+        F3[B1 * b.nc + c] = R(0.5) * (e2[c] + e2[c + 1]);
+        F3[B2 * b.nc + c] = R(-0.5) * (e1[c] + e1[c + b.s2]);
+        F3[B3 * b.nc + c] = R(0);
+    }
+}
+
+// Flux divergence of all 8 variables of zone c (FluxDivHelper form:
+// -(sum_d A_d (F_d(hi) - F_d(lo))) / V), with the flux-CT B fluxes
+// evaluated from the EMFs when `ct` is set.
+template <typename R>
+__device__ __forceinline__ void zone_divergence(const BlockDims& b, const R* __restrict__ F, const R* __restrict__ emf,
+                                                long c, bool ct, double dudt[NV]) {
+    const double A1 = b.dx2 * b.dx3, A2 = b.dx1 * b.dx3, A3 = b.dx1 * b.dx2, V = b.dx1 * b.dx2 * b.dx3;
+    const R* F1 = F;
+    const R* F2 = F + 8 * b.nc;
+    const R* F3 = F + 16 * b.nc;
+    const long s2 = b.s2, s3 = b.s3;
+    const int nd = b.ndim;
+    double f1lo[NV], f1hi[NV], f2lo[NV], f2hi[NV], f3lo[NV], f3hi[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        f1lo[v] = (double)F1[v * b.nc + c];
+        f1hi[v] = (double)F1[v * b.nc + c + 1];
+        f2lo[v] = f2hi[v] = f3lo[v] = f3hi[v] = 0.;
+        if (nd >= 2 && !(ct && v >= B1)) { f2lo[v] = (double)F2[v * b.nc + c]; f2hi[v] = (double)F2[v * b.nc + c + s2]; }
+        if (nd == 3 && !(ct && v >= B1)) { f3lo[v] = (double)F3[v * b.nc + c]; f3hi[v] = (double)F3[v * b.nc + c + s3]; }
+    }
+    if (ct) {
+        const R* e1 = emf;
+        const R* e2 = emf + b.nc;
+        const R* e3 = emf + 2 * b.nc;
+        f1lo[B1] = 0.; f1hi[B1] = 0.;
+        f1lo[B2] = (double)(R(0.5) * (e3[c] + e3[c + s2]));
+        f1hi[B2] = (double)(R(0.5) * (e3[c + 1] + e3[c + 1 + s2]));
+        f2lo[B1] = (double)(R(-0.5) * (e3[c] + e3[c + 1]));
+        f2hi[B1] = (double)(R(-0.5) * (e3[c + s2] + e3[c + s2 + 1]));
+        f2lo[B2] = 0.; f2hi[B2] = 0.;
+        if (nd == 3) {
+            f1lo[B3] = (double)(R(-0.5) * (e2[c] + e2[c + s3]));
+            f1hi[B3] = (double)(R(-0.5) * (e2[c + 1] + e2[c + 1 + s3]));
+            f2lo[B3] = (double)(R(0.5) * (e1[c] + e1[c + s3]));
+            f2hi[B3] = (double)(R(0.5) * (e1[c + s2] + e1[c + s2 + s3]));
+            f3lo[B1] = (double)(R(0.5) * (e2[c] + e2[c + 1]));
+            f3hi[B1] = (double)(R(0.5) * (e2[c + s3] + e2[c + s3 + 1]));
+            f3lo[B2] = (double)(R(-0.5) * (e1[c] + e1[c + s2]));
+            f3hi[B2] = (double)(R(-0.5) * (e1[c + s3] + e1[c + s3 + s2]));
+            f3lo[B3] = 0.; f3hi[B3] = 0.;
+        } else {
+            // 2D: B3 keeps its Riemann fluxes
+            f2lo[B3] = (double)F2[B3 * b.nc + c];
+            f2hi[B3] = (double)F2[B3 * b.nc + c + s2];
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        double du = (A1 * f1hi[v] - A1 * f1lo[v]);
+        if (nd >= 2) du += (A2 * f2hi[v] - A2 * f2lo[v]);
+        if (nd == 3) du += (A3 * f3hi[v] - A3 * f3lo[v]);
+        dudt[v] = -du / V;
+    }
+}
+
+__device__ __forceinline__ double zone_ndt(const BlockDims& b, double c1, double n1, double c2, double n2, double c3, double n3) {
+    return 1.0 / (1 / (b.dx1 / smax(c1, n1)) + 1 / (b.dx2 / smax(c2, n2)) + 1 / (b.dx3 / smax(c3, n3)));
+}
+
+__device__ __forceinline__ void block_min_to(double v, unsigned long long* dst) {
+    __shared__ double red[BS / 64];
+    v = wave_min(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = red[0];
+        for (int q = 1; q < (int)(blockDim.x >> 6); ++q) m = fmin(m, red[q]);
+        if (m < DBL_MAX) atomic_min_pos(dst, m);
+    }
+}
+__device__ __forceinline__ void block_max_to(double v, unsigned long long* dst) {
+    __shared__ double red[BS / 64];
+    v = wave_max(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = red[0];
+        for (int q = 1; q < (int)(blockDim.x >> 6); ++q) m = fmax(m, red[q]);
+        atomic_max_pos(dst, m);
+    }
+}
+
+// ------------------------------------------------------- fused stage update
+template <typename R, bool DO_DT>
+__global__ void __launch_bounds__(BS)
+update_kernel(BlockDims b, StepConsts cst, int stage,
+              const R* __restrict__ F, const R* __restrict__ emf,
+              const R* __restrict__ cmax, const R* __restrict__ cmin,
+              const double* __restrict__ Usub, const double* Ufull, const double* __restrict__ Psub,
+              double* Uout, double* __restrict__ Pout, int* __restrict__ pflag, int* __restrict__ fflag,
+              int* __restrict__ fail_list, DeviceState* __restrict__ ds) {
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = t < (unsigned)ninterior(b);
+    double ndt = DBL_MAX;
+    if (active) {
+        int k, j, i;
+        interior_index(b, t, k, j, i);
+        const long c = (long)k * b.s3 + (long)j * b.s2 + i;
+        const double gam = cst.gam;
+        double dudt[NV];
+        zone_divergence<R>(b, F, emf, c, b.ndim > 1, dudt);
+        const double g0 = cst.gam0[stage - 1], g1 = cst.gam1[stage - 1];
+        const double beta_dt = cst.beta[stage - 1] * ds->dt;
+        double U[NV], P[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            double u = g0 * Usub[v * b.nc + c] + g1 * Ufull[v * b.nc + c];
+            U[v] = 1.0 * u + beta_dt * dudt[v];
+        }
+        // U->P guess: the stage's starting primitives (kharma_driver.cpp:379-384)
+#pragma unroll
+        for (int v = 0; v < 5; ++v) P[v] = Psub[v * b.nc + c];
+        MinkowskiGeom<double> g;
+        int pf = u_to_p_onedw(g, U, P, gam, cst.iter_max, cst.err_tol);
+        const int ff0 = (pf / F_GEOM_RHO) * F_GEOM_RHO;
+        pf = pf % F_GEOM_RHO;
+        P[B1] = U[B1] / g.gdet(); P[B2] = U[B2] / g.gdet(); P[B3] = U[B3] / g.gdet();
+        int ff = ff0;
+        if (cst.floors.on) ff = floors_zone(g, P, U, gam, cst.floors, ff0, pf);
+        p_to_u(g, P, gam, U);
+        ImageSet s;
+        image_set(b, k, j, i, s);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            store_with_images(b, s, Uout + v * b.nc, c, U[v]);
+            store_with_images(b, s, Pout + v * b.nc, c, P[v]);
+        }
+        store_with_images(b, s, pflag, c, pf);
+        fflag[c] = ff;
+        if (pf > 0) {
+            const int slot = atomicAdd(&ds->fail_count[stage - 1], 1);
+            fail_list[slot] = (int)c;
+        }
+        if (DO_DT) {
+            const double n = zone_ndt(b, (double)cmax[c], (double)cmin[c], (double)cmax[b.nc + c], (double)cmin[b.nc + c],
+                                      (double)cmax[2 * b.nc + c], (double)cmin[2 * b.nc + c]);
+            if (!(n != n)) ndt = n;
+        }
+    }
+    if (DO_DT) block_min_to(ndt, &ds->min_ndt_bits);
+}
+
+// Inverter::FixUtoP (inverter/fixup.cpp:47-156) on the zones listed by the
+// update kernel: distance-weighted average of non-failed neighbours, atmosphere
+// fallback, geometric floors, then the conserved state of the final P->U.
+__global__ void __launch_bounds__(BS)
+fixup_list_kernel(BlockDims b, StepConsts cst, int stage, double* __restrict__ U, double* __restrict__ P,
+                  const int* __restrict__ pflag, const int* __restrict__ fail_list, const DeviceState* __restrict__ ds) {
+    const int n = ds->fail_count[stage - 1];
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        const long c = fail_list[q];
+        const int k = (int)(c / b.s3), j = (int)((c - (long)k * b.s3) / b.s2), i = (int)(c - (long)k * b.s3 - (long)j * b.s2);
+        double Pz[NV];
+        for (int v = 0; v < NV; ++v) Pz[v] = P[v * b.nc + c];
+        double wsum = 0., sum[5] = {0., 0., 0., 0., 0.};
+        if (cst.fix_average) {
+            for (int dn = -1; dn <= 1; dn++) for (int dm = -1; dm <= 1; dm++) for (int dl = -1; dl <= 1; dl++) {
+                const int ii = i + dl, jj = j + dm, kk = k + dn;
+                if (ii < 0 || ii >= b.n1 || jj < 0 || jj >= b.n2 || kk < 0 || kk >= b.n3) continue;
+                const long cc = (long)kk * b.s3 + (long)jj * b.s2 + ii;
+                if (!(pflag[cc] > 0)) {
+                    const double w = 1. / (abs(dl) + abs(dm) + abs(dn) + 1);
+                    wsum += w;
+                    for (int p = 0; p < 5; ++p) sum[p] += w * P[p * b.nc + cc];
+                }
+            }
+        }
+        if (wsum < 1.e-10) { for (int p = 0; p < 5; ++p) Pz[p] = 0.; }
+        else { for (int p = 0; p < 5; ++p) Pz[p] = sum[p] / wsum; }
+        apply_geo_floors<double>(Pz, cst.floors.rho_min_const, cst.floors.u_min_const);
+        MinkowskiGeom<double> g;
+        double Uz[NV];
+        p_to_u(g, Pz, cst.gam, Uz);
+        ImageSet s;
+        image_set(b, k, j, i, s);
+        for (int v = 0; v < 5; ++v) store_with_images(b, s, P + v * b.nc, c, Pz[v]);
+        for (int v = 0; v < NV; ++v) store_with_images(b, s, U + v * b.nc, c, Uz[v]);
+    }
+}
+
+// End-of-step scalar work on the device: SetGlobalTimeStep (kharma_driver.cpp:439-460)
+// with GRMHD::EstimateTimestep's clip (grmhd.cpp:335-339), time advance, resets.
+__global__ void finalize_kernel(StepConsts cst, DeviceState* ds, double* dt_hist, int hist_cap) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const double min_ndt = __longlong_as_double((long long)ds->min_ndt_bits);
+    const double dt = ds->dt;
+    const double dt_max = cst.max_dt_increase * dt;
+    const double new_dt = clip(min_ndt * cst.cfl, cst.dt_min, dt_max);
+    if (dt_hist && ds->ncycle < hist_cap) dt_hist[ds->ncycle] = dt;
+    ds->time += dt;
+    double nd = dt;
+    if (nd < 0.1 * DBL_MAX) nd *= 2.0;
+    nd = smin(nd, new_dt);
+    if (ds->time < cst.tlim && (cst.tlim - ds->time) < nd) nd = cst.tlim - ds->time;
+    ds->dt_last = dt;
+    ds->dt = nd;
+    ds->last_ndt = min_ndt;
+    ds->ncycle += 1;
+    ds->total_fails += ds->fail_count[0] + ds->fail_count[1];
+    ds->last_divb = __longlong_as_double((long long)ds->divb_bits);
+    ds->divb_bits = 0ull;
+    ds->min_ndt_bits = (unsigned long long)__double_as_longlong(DBL_MAX);
+    ds->fail_count[0] = 0;
+    ds->fail_count[1] = 0;
+}
+
+__global__ void reset_state_kernel(DeviceState* ds, double dt) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    ds->dt = dt; ds->dt_last = dt; ds->time = 0.;
+    ds->min_ndt_bits = (unsigned long long)__double_as_longlong(DBL_MAX);
+    ds->divb_bits = 0ull;
+    ds->last_divb = 0.; ds->last_ndt = 0.;
+    ds->fail_count[0] = ds->fail_count[1] = 0;
+    ds->ncycle = 0; ds->total_fails = 0;
+}
+
+// B_FluxCT::MaxDivB with corner_div (b_flux_ct.cpp:566-607, b_flux_ct_functions.hpp:52-75)
+__global__ void __launch_bounds__(BS) max_divb_kernel(BlockDims b, const double* __restrict__ U, DeviceState* ds) {
+    const int ni = b.nx1 + 1, nj = b.nx2 + 1, nk = (b.ndim > 2) ? b.nx3 + 1 : 1;
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    double d = 0.;
+    if (t < (unsigned)(ni * nj * nk)) {
+        const int kk = t / (ni * nj), rem = t - kk * (ni * nj), jj = rem / ni;
+        const int i = b.is + rem - jj * ni, j = b.js + jj, k = b.ks + kk;
+        const long c = (long)k * b.s3 + (long)j * b.s2 + i;
+        const double* Bx = U + B1 * b.nc;
+        const double* By = U + B2 * b.nc;
+        const double* Bz = U + B3 * b.nc;
+        const long s2 = b.s2, s3 = b.s3;
+        const bool do3 = b.ndim > 2;
+        const double norm = do3 ? 0.25 : 0.5;
+        double term1 = Bx[c] - Bx[c - 1] + Bx[c - s2] - Bx[c - s2 - 1];
+        double term2 = By[c] - By[c - s2] + By[c - 1] - By[c - s2 - 1];
+        double term3 = 0.;
+        if (do3) {
+            term1 += Bx[c - s3] + Bx[c - s3 - s2] - Bx[c - s3 - 1] - Bx[c - s3 - s2 - 1];
+            term2 += By[c - s3] + By[c - s3 - 1] - By[c - s3 - s2] - By[c - s3 - s2 - 1];
+            term3 = Bz[c] + Bz[c - s2] + Bz[c - 1] + Bz[c - s2 - 1]
+                  - Bz[c - s3] - Bz[c - s3 - s2] - Bz[c - s3 - 1] - Bz[c - s3 - s2 - 1];
+        }
+        d = fabs(norm * term1 / b.dx1 + norm * term2 / b.dx2 + norm * term3 / b.dx3);
+    }
+    block_max_to(d, &ds->divb_bits);
+}
+
+// ----------------------------------------------------- task-level kernels
+template <typename R>
+__global__ void __launch_bounds__(BS) divergence_kernel(BlockDims b, const R* __restrict__ F, double* __restrict__ dUdt) {
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (unsigned)ninterior(b)) return;
+    int k, j, i;
+    interior_index(b, t, k, j, i);
+    const long c = (long)k * b.s3 + (long)j * b.s2 + i;
+    double d[NV];
+    zone_divergence<R>(b, F, nullptr, c, false, d);
+    for (int v = 0; v < NV; ++v) dUdt[v * b.nc + c] = d[v];
+}
+
+// parthenon::Update::WeightedSumData pair used by AddStateUpdate, entire block
+__global__ void __launch_bounds__(BS) weighted_sum_kernel(long n, const double* __restrict__ Usub, const double* __restrict__ Ufull,
+                                                          const double* __restrict__ dUdt, double g0, double g1, double bdt,
+                                                          double* __restrict__ Uout) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const double u = g0 * Usub[t] + g1 * Ufull[t];
+    Uout[t] = 1.0 * u + bdt * dUdt[t];
+}
+
+// Inverter::BlockUtoP + B_FluxCT::BlockUtoP over the entire block
+__global__ void __launch_bounds__(BS) utop_kernel(BlockDims b, StepConsts cst, const double* __restrict__ U, double* __restrict__ P,
+                                                  int* __restrict__ pflag, int* __restrict__ fflag) {
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= b.nc) return;
+    double Uz[NV], Pz[NV];
+    for (int v = 0; v < NV; ++v) { Uz[v] = U[v * b.nc + c]; Pz[v] = P[v * b.nc + c]; }
+    MinkowskiGeom<double> g;
+    const int r = u_to_p_onedw(g, Uz, Pz, cst.gam, cst.iter_max, cst.err_tol);
+    for (int v = 0; v < 5; ++v) P[v * b.nc + c] = Pz[v];
+    for (int v = 0; v < 3; ++v) P[(B1 + v) * b.nc + c] = U[(B1 + v) * b.nc + c] / g.gdet();
+    pflag[c] = r % F_GEOM_RHO;
+    fflag[c] = (r / F_GEOM_RHO) * F_GEOM_RHO;
+}
+
+// Floors::ApplyGRMHDFloors over the entire block: floors, ceilings and the
+// 5 hydrodynamic conserved variables of modified zones (p_to_u_mhd).
+__global__ void __launch_bounds__(BS) floors_kernel(BlockDims b, StepConsts cst, double* __restrict__ U, double* __restrict__ P,
+                                                    int* __restrict__ pflag, int* __restrict__ fflag) {
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= b.nc) return;
+    double Uz[NV], Pz[NV];
+    for (int v = 0; v < NV; ++v) { Uz[v] = U[v * b.nc + c]; Pz[v] = P[v * b.nc + c]; }
+    MinkowskiGeom<double> g;
+    int pf = pflag[c];
+    const int ff = floors_zone(g, Pz, Uz, cst.gam, cst.floors, fflag[c], pf);
+    fflag[c] = ff;
+    pflag[c] = pf;
+    if (ff) {
+        double Un[NV];
+        p_to_u(g, Pz, cst.gam, Un);
+        for (int v = 0; v < 5; ++v) { P[v * b.nc + c] = Pz[v]; U[v * b.nc + c] = Un[v]; }
+    }
+}
+
+// Inverter::FixUtoP over the entire block (the reference's physical range for
+// a periodic block); two passes so averages only read non-failed zones.
+__global__ void __launch_bounds__(BS) fixup_avg_kernel(BlockDims b, StepConsts cst, double* __restrict__ P,
+                                                       const int* __restrict__ pflag) {
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= b.nc || !(pflag[c] > 0)) return;
+    const int k = (int)(c / b.s3), j = (int)((c - (long)k * b.s3) / b.s2), i = (int)(c - (long)k * b.s3 - (long)j * b.s2);
+    double wsum = 0., sum[5] = {0., 0., 0., 0., 0.};
+    if (cst.fix_average) {
+        for (int dn = -1; dn <= 1; dn++) for (int dm = -1; dm <= 1; dm++) for (int dl = -1; dl <= 1; dl++) {
+            const int ii = i + dl, jj = j + dm, kk = k + dn;
+            if (ii < 0 || ii >= b.n1 || jj < 0 || jj >= b.n2 || kk < 0 || kk >= b.n3) continue;
+            const long cc = (long)kk * b.s3 + (long)jj * b.s2 + ii;
+            if (!(pflag[cc] > 0)) {
+                const double w = 1. / (abs(dl) + abs(dm) + abs(dn) + 1);
+                wsum += w;
+                for (int p = 0; p < 5; ++p) sum[p] += w * P[p * b.nc + cc];
+            }
+        }
+    }
+    for (int p = 0; p < 5; ++p) P[p * b.nc + c] = (wsum < 1.e-10) ? 0. : sum[p] / wsum;
+}
+__global__ void __launch_bounds__(BS) fixup_floor_kernel(BlockDims b, StepConsts cst, double* __restrict__ U, double* __restrict__ P,
+                                                         const int* __restrict__ pflag) {
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= b.nc || !(pflag[c] > 0)) return;
+    double Pz[NV], Uz[NV];
+    for (int v = 0; v < NV; ++v) Pz[v] = P[v * b.nc + c];
+    apply_geo_floors<double>(Pz, cst.floors.rho_min_const, cst.floors.u_min_const);
+    MinkowskiGeom<double> g;
+    p_to_u(g, Pz, cst.gam, Uz);
+    for (int v = 0; v < 5; ++v) { P[v * b.nc + c] = Pz[v]; U[v * b.nc + c] = Uz[v]; }
+}
+
+// Flux::MeshPtoU over the entire block
+__global__ void __launch_bounds__(BS) ptou_kernel(BlockDims b, StepConsts cst, const double* __restrict__ P, double* __restrict__ U) {
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= b.nc) return;
+    double Pz[NV], Uz[NV];
+    for (int v = 0; v < NV; ++v) Pz[v] = P[v * b.nc + c];
+    MinkowskiGeom<double> g;
+    p_to_u(g, Pz, cst.gam, Uz);
+    for (int v = 0; v < NV; ++v) U[v * b.nc + c] = Uz[v];
+}
+
+// Periodic ghost fill of nvar variables: every ghost zone copies its interior image.
+template <typename T>
+__global__ void __launch_bounds__(BS) sync_kernel(BlockDims b, T* __restrict__ A, int nvar) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= b.nc * nvar) return;
+    const int v = (int)(t / b.nc);
+    const long c = t - (long)v * b.nc;
+    const int k = (int)(c / b.s3), j = (int)((c - (long)k * b.s3) / b.s2), i = (int)(c - (long)k * b.s3 - (long)j * b.s2);
+    const bool in_i = i >= b.is && i <= b.ie, in_j = j >= b.js && j <= b.je, in_k = k >= b.ks && k <= b.ke;
+    if (in_i && in_j && in_k) return;
+    auto wrap = [](int x, int s, int n) { int r = (x - s) % n; if (r < 0) r += n; return s + r; };
+    const int si = (b.periodic[0] && !in_i) ? wrap(i, b.is, b.nx1) : i;
+    const int sj = (b.ndim > 1 && b.periodic[1] && !in_j) ? wrap(j, b.js, b.nx2) : j;
+    const int sk = (b.ndim > 2 && b.periodic[2] && !in_k) ? wrap(k, b.ks, b.nx3) : k;
+    const long src = (long)sk * b.s3 + (long)sj * b.s2 + si;
+    if (src != c) A[(long)v * b.nc + c] = A[(long)v * b.nc + src];
+}
+
+template <typename R>
+__global__ void __launch_bounds__(BS) estimate_dt_kernel(BlockDims b, const R* __restrict__ cmax, const R* __restrict__ cmin,
+                                                         DeviceState* ds) {
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    double ndt = DBL_MAX;
+    if (t < (unsigned)ninterior(b)) {
+        int k, j, i;
+        interior_index(b, t, k, j, i);
+        const long c = (long)k * b.s3 + (long)j * b.s2 + i;
+        const double n = zone_ndt(b, (double)cmax[c], (double)cmin[c], (double)cmax[b.nc + c], (double)cmin[b.nc + c],
+                                  (double)cmax[2 * b.nc + c], (double)cmin[2 * b.nc + c]);
+        if (!(n != n)) ndt = n;
+    }
+    block_min_to(ndt, &ds->min_ndt_bits);
+}
+
+// ---------------------------------------------------------------- launchers
+hipError_t launch_emf(const BlockDims& b, int prec, const void* F, void* emf, hipStream_t s) {
+    if (b.ndim < 2) return hipSuccess;
+    const long n = (long)(b.nx1 + 1) * (b.nx2 + 1) * ((b.ndim > 2) ? b.nx3 + 1 : 1);
+    if (prec == PREC_SPLIT) hipLaunchKernelGGL(emf_kernel<float>, grid_for(n), dim3(BS), 0, s, b, (const float*)F, (float*)emf);
+    else hipLaunchKernelGGL(emf_kernel<double>, grid_for(n), dim3(BS), 0, s, b, (const double*)F, (double*)emf);
+    return hipGetLastError();
+}
+hipError_t launch_flux_ct_rewrite(const BlockDims& b, int prec, void* F, const void* emf, hipStream_t s) {
+    if (b.ndim < 2) return hipSuccess;
+    const long n = (long)(b.nx1 + 1) * (b.nx2 + 1) * ((b.ndim > 2) ? b.nx3 + 1 : 1);
+    if (prec == PREC_SPLIT) hipLaunchKernelGGL(ct_rewrite_kernel<float>, grid_for(n), dim3(BS), 0, s, b, (float*)F, (const float*)emf);
+    else hipLaunchKernelGGL(ct_rewrite_kernel<double>, grid_for(n), dim3(BS), 0, s, b, (double*)F, (const double*)emf);
+    return hipGetLastError();
+}
+hipError_t launch_flux_divergence(const BlockDims& b, int prec, const void* F, double* dUdt, hipStream_t s) {
+    const long n = (long)b.nx1 * (b.ndim > 1 ? b.nx2 : 1) * (b.ndim > 2 ? b.nx3 : 1);
+    if (prec == PREC_SPLIT) hipLaunchKernelGGL(divergence_kernel<float>, grid_for(n), dim3(BS), 0, s, b, (const float*)F, dUdt);
+    else hipLaunchKernelGGL(divergence_kernel<double>, grid_for(n), dim3(BS), 0, s, b, (const double*)F, dUdt);
+    return hipGetLastError();
+}
+hipError_t launch_update(const BlockDims& b, const StepConsts& c, int prec, int stage, bool do_dt,
+                         const void* F, const void* emf, const void* cmax, const void* cmin,
+                         const double* Usub, const double* Ufull, const double* Psub,
+                         double* Uout, double* Pout, int* pflag, int* fflag,
+                         int* fail_list, DeviceState* ds, hipStream_t s) {
+    const long n = (long)b.nx1 * (b.ndim > 1 ? b.nx2 : 1) * (b.ndim > 2 ? b.nx3 : 1);
+    const dim3 g = grid_for(n);
+    if (prec == PREC_SPLIT) {
+        auto f = (const float*)F; auto e = (const float*)emf; auto mx = (const float*)cmax; auto mn = (const float*)cmin;
+        if (do_dt) hipLaunchKernelGGL((update_kernel<float, true>), g, dim3(BS), 0, s, b, c, stage, f, e, mx, mn, Usub, Ufull, Psub, Uout, Pout, pflag, fflag, fail_list, ds);
+        else hipLaunchKernelGGL((update_kernel<float, false>), g, dim3(BS), 0, s, b, c, stage, f, e, mx, mn, Usub, Ufull, Psub, Uout, Pout, pflag, fflag, fail_list, ds);
+    } else {
+        auto f = (const double*)F; auto e = (const double*)emf; auto mx = (const double*)cmax; auto mn = (const double*)cmin;
+        if (do_dt) hipLaunchKernelGGL((update_kernel<double, true>), g, dim3(BS), 0, s, b, c, stage, f, e, mx, mn, Usub, Ufull, Psub, Uout, Pout, pflag, fflag, fail_list, ds);
+        else hipLaunchKernelGGL((update_kernel<double, false>), g, dim3(BS), 0, s, b, c, stage, f, e, mx, mn, Usub, Ufull, Psub, Uout, Pout, pflag, fflag, fail_list, ds);
+    }
+    return hipGetLastError();
+}
+hipError_t launch_fixup(const BlockDims& b, const StepConsts& c, int stage, double* U, double* P,
+                        const int* pflag, const int* fail_list, DeviceState* ds, hipStream_t s) {
+    hipLaunchKernelGGL(fixup_list_kernel, dim3(64), dim3(BS), 0, s, b, c, stage, U, P, pflag, fail_list, ds);
+    return hipGetLastError();
+}
+hipError_t launch_finalize(const StepConsts& c, DeviceState* ds, double* dt_hist, int hist_cap, hipStream_t s) {
+    hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(64), 0, s, c, ds, dt_hist, hist_cap);
+    return hipGetLastError();
+}
+hipError_t launch_max_divb(const BlockDims& b, const double* U, DeviceState* ds, hipStream_t s) {
+    if (b.ndim < 2) return hipSuccess;
+    const long n = (long)(b.nx1 + 1) * (b.nx2 + 1) * ((b.ndim > 2) ? b.nx3 + 1 : 1);
+    hipLaunchKernelGGL(max_divb_kernel, grid_for(n), dim3(BS), 0, s, b, U, ds);
+    return hipGetLastError();
+}
+hipError_t launch_reset_state(DeviceState* ds, double dt, hipStream_t s) {
+    hipLaunchKernelGGL(reset_state_kernel, dim3(1), dim3(64), 0, s, ds, dt);
+    return hipGetLastError();
+}
+hipError_t launch_weighted_sum(const BlockDims& b, const double* Usub, const double* Ufull, const double* dUdt,
+                               double gam0, double gam1, double beta_dt, double* Uout, hipStream_t s) {
+    const long n = (long)NV * b.nc;
+    hipLaunchKernelGGL(weighted_sum_kernel, grid_for(n), dim3(BS), 0, s, n, Usub, Ufull, dUdt, gam0, gam1, beta_dt, Uout);
+    return hipGetLastError();
+}
+hipError_t launch_utop(const BlockDims& b, const StepConsts& c, const double* U, double* P, int* pflag, int* fflag,
+                       hipStream_t s) {
+    hipLaunchKernelGGL(utop_kernel, grid_for(b.nc), dim3(BS), 0, s, b, c, U, P, pflag, fflag);
+    return hipGetLastError();
+}
+hipError_t launch_floors(const BlockDims& b, const StepConsts& c, double* U, double* P, int* pflag, int* fflag,
+                         hipStream_t s) {
+    if (!c.floors.on) return hipSuccess;
+    hipLaunchKernelGGL(floors_kernel, grid_for(b.nc), dim3(BS), 0, s, b, c, U, P, pflag, fflag);
+    return hipGetLastError();
+}
+hipError_t launch_fixup_entire(const BlockDims& b, const StepConsts& c, double* U, double* P, const int* pflag,
+                               hipStream_t s) {
+    if (!c.fix_average && !c.fix_atmo) return hipSuccess;
+    hipLaunchKernelGGL(fixup_avg_kernel, grid_for(b.nc), dim3(BS), 0, s, b, c, P, pflag);
+    hipLaunchKernelGGL(fixup_floor_kernel, grid_for(b.nc), dim3(BS), 0, s, b, c, U, P, pflag);
+    return hipGetLastError();
+}
+hipError_t launch_ptou(const BlockDims& b, const StepConsts& c, const double* P, double* U, hipStream_t s) {
+    hipLaunchKernelGGL(ptou_kernel, grid_for(b.nc), dim3(BS), 0, s, b, c, P, U);
+    return hipGetLastError();
+}
+hipError_t launch_sync(const BlockDims& b, double* A, int nvar, hipStream_t s) {
+    hipLaunchKernelGGL(sync_kernel<double>, grid_for(b.nc * nvar), dim3(BS), 0, s, b, A, nvar);
+    return hipGetLastError();
+}
+hipError_t launch_sync_int(const BlockDims& b, int* A, hipStream_t s) {
+    hipLaunchKernelGGL(sync_kernel<int>, grid_for(b.nc), dim3(BS), 0, s, b, A, 1);
+    return hipGetLastError();
+}
+hipError_t launch_estimate_dt(const BlockDims& b, const StepConsts& c, int prec, const void* cmax, const void* cmin,
+                              DeviceState* ds, hipStream_t s) {
+    const long n = (long)b.nx1 * (b.ndim > 1 ? b.nx2 : 1) * (b.ndim > 2 ? b.nx3 : 1);
+    if (prec == PREC_SPLIT) hipLaunchKernelGGL(estimate_dt_kernel<float>, grid_for(n), dim3(BS), 0, s, b, (const float*)cmax, (const float*)cmin, ds);
+    else hipLaunchKernelGGL(estimate_dt_kernel<double>, grid_for(n), dim3(BS), 0, s, b, (const double*)cmax, (const double*)cmin, ds);
+    return hipGetLastError();
+}
+
+}  // namespace kh

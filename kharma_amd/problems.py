@@ -1,0 +1,166 @@
+"""Initial conditions and run configurations for the GRMHD-step workloads.
+
+Each ``*_config`` returns a plain dict describing one meshblock run; each
+``*_prims`` returns the initial primitive array P[8, n3, n2, n1] over the
+entire block (interior + ghosts), variable order rho, u, u1, u2, u3, B1, B2, B3
+(the order of KHARMA's GRMHD + B_FluxCT primitive pack, grmhd/pack.hpp).
+
+Interior values are evaluated at cell centres; ghosts are filled as exact
+periodic images of the interior so that every zone of a periodic block is
+bit-identical to its image from the start.
+"""
+import math
+
+import numpy as np
+
+NV = 8
+RHO, UU, U1, U2, U3, B1, B2, B3 = range(8)
+
+
+def block_shape(cfg):
+    ng = cfg.get("ng", 4)
+    nx1, nx2, nx3 = cfg["nx"]
+    ndim = 3 if nx3 > 1 else (2 if nx2 > 1 else 1)
+    n1 = nx1 + 2 * ng
+    n2 = nx2 + 2 * ng if ndim > 1 else 1
+    n3 = nx3 + 2 * ng if ndim > 2 else 1
+    return ndim, (n3, n2, n1)
+
+
+def cell_centers(cfg):
+    """Interior cell-centre coordinates x1[i], x2[j], x3[k] (parthenon UniformCartesian Xc)."""
+    nx = cfg["nx"]
+    out = []
+    for d in range(3):
+        lo, hi = cfg["bounds"][d]
+        n = nx[d]
+        dx = (hi - lo) / n
+        out.append(lo + (np.arange(n) + 0.5) * dx)
+    return out
+
+
+def periodic_fill(P, cfg):
+    """Fill all ghost zones of P (…, n3, n2, n1) with periodic images of the interior."""
+    ng = cfg.get("ng", 4)
+    ndim, (n3, n2, n1) = block_shape(cfg)
+    nx1, nx2, nx3 = cfg["nx"]
+    ii = (np.arange(n1) - ng) % nx1 + ng
+    P[...] = P[..., ii]
+    if ndim > 1:
+        jj = (np.arange(n2) - ng) % nx2 + ng
+        P[...] = P[..., jj, :]
+    if ndim > 2:
+        kk = (np.arange(n3) - ng) % nx3 + ng
+        P[...] = P[..., kk, :, :]
+    return P
+
+
+def interior(cfg):
+    ng = cfg.get("ng", 4)
+    ndim, _ = block_shape(cfg)
+    nx1, nx2, nx3 = cfg["nx"]
+    si = slice(ng, ng + nx1)
+    sj = slice(ng, ng + nx2) if ndim > 1 else slice(0, 1)
+    sk = slice(ng, ng + nx3) if ndim > 2 else slice(0, 1)
+    return (sk, sj, si)
+
+
+def _base_cfg(**kw):
+    cfg = dict(ng=4, cfl=0.9, dt_min=1e-5, max_dt_increase=2.0, tlim=1e30,
+               recon="weno5", riemann="llf", floors=True, floor_frame="drift",
+               iter_max=8, err_tol=1e-8, fix_average=True, fix_atmo=True,
+               integrator="rk2", coords="cartesian_minkowski")
+    cfg.update(kw)
+    cfg.setdefault("dt_start", cfg["dt_min"])
+    return cfg
+
+
+# ------------------------------------------------------------- Orszag-Tang
+def orszag_tang_config(n=256, steps=100, **kw):
+    """2-D relativistic Orszag-Tang vortex, Minkowski, periodic [-pi, pi]^2, gamma = 5/3.
+
+    Follows the literal workload statement (unscaled u, zero phase) rather than
+    prob/orszag_tang.hpp:35-57; WENO5 + LLF, flux-CT, RK2, CFL 0.9, onedw.
+    """
+    return _base_cfg(name="orszag_tang", nx=(n, n, 1),
+                     bounds=((-math.pi, math.pi), (-math.pi, math.pi), (-0.01, 0.01)),
+                     gamma=5.0 / 3.0, steps=steps, **kw)
+
+
+def orszag_tang_prims(cfg):
+    ndim, shape = block_shape(cfg)
+    P = np.zeros((NV,) + shape)
+    x, y, _ = cell_centers(cfg)
+    X, Y = np.meshgrid(x, y, indexing="xy")   # [j, i]
+    gam = cfg["gamma"]
+    sl = interior(cfg)
+    P[(RHO,) + sl] = 25.0 / 9.0
+    P[(UU,) + sl] = 5.0 / (3.0 * (gam - 1.0))
+    P[(U1,) + sl] = -0.05 * np.sin(Y)
+    P[(U2,) + sl] = 0.05 * np.sin(X)
+    P[(B1,) + sl] = -0.05 * np.sin(Y)
+    P[(B2,) + sl] = 0.05 * np.sin(2.0 * X)
+    return periodic_fill(P, cfg)
+
+
+# ------------------------------------------------------------- MHD modes
+# Eigenvectors and frequencies for the (1,1,0) wave, prob/mhdmodes.hpp:130-191
+# (dir = 3), valid for gamma = 4/3 and background rho = u = 1, B = (1, 0, 0).
+MODES_DIR3 = {
+    "slow": dict(omega=2.41024185339, drho=0.558104461559, du=0.744139282078,
+                 du1=-0.277124827421, du2=0.0630348927707, dB1=-0.164323721928, dB2=0.164323721928),
+    "alfven": dict(omega=3.44144232573, du3=0.480384461415, dB3=0.877058019307),
+    "fast": dict(omega=5.53726217331, drho=0.476395427447, du=0.635193903263,
+                 du1=-0.102965815319, du2=-0.316873207561, dB1=0.359559114174, dB2=-0.359559114174),
+    "entropy": dict(omega=0.0, drho=1.0),
+}
+
+
+def mhdmode_config(n=64, mode="alfven", ndim=3, amp=1e-4, one_period=True, **kw):
+    """Linear MHD eigenmode on the periodic unit cube, k = (2pi, 2pi, 0), gamma = 4/3."""
+    nx = (n, n, n if ndim == 3 else 1)
+    ev = MODES_DIR3[mode]
+    tlim = 2.0 * math.pi / ev["omega"] if (one_period and ev["omega"] > 0) else 1e30
+    return _base_cfg(name="mhdmodes_" + mode, nx=nx, bounds=((0., 1.), (0., 1.), (0., 1.)),
+                     gamma=4.0 / 3.0, mode=mode, amp=amp, tlim=tlim, **kw)
+
+
+def mhdmode_state(cfg, t=0.0, phase=0.0):
+    """Analytic mode at time t on the interior: (rho, u, u1, u2, u3, B1, B2, B3), shape (8, *interior)."""
+    ev = MODES_DIR3[cfg["mode"]]
+    amp = cfg["amp"]
+    x, y, z = cell_centers(cfg)
+    ndim, _ = block_shape(cfg)
+    if ndim == 3:
+        Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    else:
+        Y, X = np.meshgrid(y, x, indexing="ij")
+        X, Y = X[None], Y[None]
+    k1 = k2 = 2.0 * math.pi
+    # mhdmodes.hpp:227 (cos mode), seed_B.hpp:161-169 (B wave); the mode evolves as
+    # Re[exp(i(k.x) + omega t)] with omega imaginary -> cos(k.x - |omega| t) for positive omega.
+    arg = k1 * X + k2 * Y + phase - ev["omega"] * t
+    mode = amp * np.cos(arg)
+    out = np.zeros((NV,) + X.shape)
+    out[RHO] = 1.0 + ev.get("drho", 0.) * mode
+    out[UU] = 1.0 + ev.get("du", 0.) * mode
+    out[U1] = ev.get("du1", 0.) * mode
+    out[U2] = ev.get("du2", 0.) * mode
+    out[U3] = ev.get("du3", 0.) * mode
+    out[B1] = 1.0 + ev.get("dB1", 0.) * mode
+    out[B2] = ev.get("dB2", 0.) * mode
+    out[B3] = ev.get("dB3", 0.) * mode
+    return out
+
+
+def mhdmode_prims(cfg, phase=0.0):
+    ndim, shape = block_shape(cfg)
+    P = np.zeros((NV,) + shape)
+    P[(slice(None),) + interior(cfg)] = mhdmode_state(cfg, 0.0, phase)
+    return periodic_fill(P, cfg)
+
+
+CONFIGS = {
+    "orszag_tang": orszag_tang_config,
+    "mhdmode": mhdmode_config,
+}

@@ -1,0 +1,1119 @@
+/*
+ * kharma_oracle.cpp -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain C++17 FP64 CPU restatement of KHARMA's explicit GRMHD step (the
+ * "kharma" driver, RK2, flux-CT, onedw inverter, floors, fixup), written to be
+ * the parity oracle for the HIP implementation in kharma_amd/.  Nothing in the
+ * product links or calls this file: only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg load it, and only as the checker.
+ *
+ * Every function names the reference file:line it restates.  Where the
+ * reference calls into Parthenon (not vendored in the reference tree), the
+ * restated behaviour is noted "parity unpinned" and is pinned instead by the
+ * closed-form checks in tests/test_oracle_*.py.
+ *
+ * Layout: every field is SoA, [var][k][j][i] over the *entire* block
+ * (interior + ng ghosts per side in each active dimension), i fastest,
+ * exactly like Parthenon's VariablePack(v,k,j,i).  Fluxes are [dir-1][var][k][j][i]
+ * with face index i meaning the left face of cell i (reference convention,
+ * flux/get_flux.hpp, reconstruction.hpp:503-509).
+ *
+ * Build: oracle/Makefile -> oracle/liboracle.so (g++ -O2 -fopenmp -ffp-contract=off).
+ */
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+using Real = double;
+constexpr int NV = 8;   // rho, u, u1, u2, u3, B1, B2, B3
+enum { RHO = 0, UU = 1, U1 = 2, U2 = 3, U3 = 4, B1 = 5, B2 = 6, B3 = 7 };
+// Loci, decs.hpp:103-104 (outer/inner_half are not used on this path)
+enum { FACE1 = 0, FACE2 = 1, FACE3 = 2, CENTER = 3, CORNER = 4, NLOC = 5 };
+
+// FFlag bits, floors/floors.hpp:47-68
+constexpr int F_GEOM_RHO = 32, F_GEOM_U = 64, F_B_RHO = 128, F_B_U = 256, F_TEMP = 512,
+              F_GAMMA = 1024, F_KTOT = 2048, F_GEOM_RHO_FLUX = 4096, F_GEOM_U_FLUX = 8192;
+constexpr int FFLAG_MINIMUM = F_GEOM_RHO;
+// Inverter::Status, inverter/invert_template.hpp:53
+enum { S_SUCCESS = 0, S_NEG_INPUT, S_MAX_ITER, S_BAD_UT, S_BAD_GAMMA, S_NEG_RHO, S_NEG_U, S_NEG_RHOU };
+
+constexpr Real SMALL = 1e-20;  // decs.hpp:90
+
+inline Real SQR(Real x) { return x * x; }
+// kharma_utils.hpp:100-110
+inline Real clip(Real n, Real lower, Real upper) { return std::min(std::max(lower, n), upper); }
+// coordinates/matrix.hpp:51-54
+inline Real dot(const Real a[4], const Real b[4]) { return a[0]*b[0] + a[1]*b[1] + a[2]*b[2] + a[3]*b[3]; }
+
+struct FourVectors { Real ucon[4], ucov[4], bcon[4], bcov[4]; };
+
+}  // namespace
+
+extern "C" {
+// Parameters of one block / one run.  Mirrors the .par options the path reads.
+struct or_params {
+    int nx1, nx2, nx3, ng;            // interior sizes (nx3 == 1 for 2D), ghost depth
+    double x1min, x1max, x2min, x2max, x3min, x3max;
+    double gam;                        // GRMHD/gamma
+    double cfl, dt_min, dt_start, max_dt_increase, tlim;
+    int recon;                         // 0 donor_cell_c, 1 linear_mc, 2 weno5, 3 ppm
+    int use_hlle;                      // flux/type
+    int floors_on;                     // Floors package loaded
+    int recon_floors;                  // flux/reconstruction_floors
+    int floor_frame;                   // 0 fluid, 1 normal, 2 drift
+    double rho_min_geom, u_min_geom, rho_min_const, u_min_const, r_char;
+    double bsq_over_rho_max, bsq_over_u_max, u_over_rho_max, ktot_max, gamma_max;
+    int use_r_char, temp_adjust_u;
+    int iter_max;                      // inverter/iter_max
+    double err_tol;                    // inverter/err_tol
+    int fix_average, fix_atmo;         // inverter/fix_average_neighbors, fix_atmosphere
+    int integrator;                    // 0 rk2, 1 vl2
+    int spherical;                     // coordinates/spherical (Minkowski: 0)
+};
+}
+
+namespace {
+
+struct Block {
+    or_params p;
+    int ndim;
+    int n1, n2, n3;          // entire sizes
+    int is, ie, js, je, ks, ke;
+    size_t nc;               // cells per var
+    Real dx[3];
+    // Geometry cache, gr_coordinates.hpp:86-90: (loc, j, i, mu, nu) over (n2+1) x (n1+1)
+    int gn1, gn2;
+    std::vector<Real> gcov, gcon, gdet, gdet_conn;
+    bool cart_minkowski = true;
+    // Containers: base (full step init / final), s1 (stage 1)
+    std::vector<Real> U0, P0, U1c, P1c;
+    std::vector<Real> F;      // [3][NV][n3][n2][n1] fluxes (OneCopy-style: shared by containers)
+    std::vector<Real> cmax, cmin;  // [3][...]
+    std::vector<Real> Pl, Pr, Ul, Ur, Fl, Fr;  // Flux.* temporaries [NV][...]
+    std::vector<Real> emf;   // [3][...]
+    std::vector<Real> dUdt;  // [NV][...]
+    std::vector<Real> pflag, fflag, rho_floor, u_floor;
+    double time = 0., dt = 0., dt_last = 0.;
+    long ncycle = 0;
+    bool in_loop = false;
+
+    size_t id(int k, int j, int i) const { return ((size_t)k * n2 + j) * n1 + i; }
+    size_t gidx(int loc, int j, int i) const { return ((size_t)loc * gn2 + j) * gn1 + i; }
+    Real gcov_(int loc, int j, int i, int mu, int nu) const { return gcov[gidx(loc, j, i) * 16 + mu * 4 + nu]; }
+    Real gcon_(int loc, int j, int i, int mu, int nu) const { return gcon[gidx(loc, j, i) * 16 + mu * 4 + nu]; }
+    Real gdet_(int loc, int j, int i) const { return gdet[gidx(loc, j, i)]; }
+    Real gdet_conn_(int j, int i, int mu, int nu, int lam) const {
+        return gdet_conn[(((size_t)j * gn1 + i) * 4 + mu) * 16 + nu * 4 + lam];
+    }
+
+    // GRCoordinates::lower/raise, gr_coordinates.hpp:227-238
+    void lower(const Real vcon[4], Real vcov[4], int j, int i, int loc) const {
+        for (int mu = 0; mu < 4; ++mu) vcov[mu] = 0.;
+        for (int mu = 0; mu < 4; ++mu) for (int nu = 0; nu < 4; ++nu) vcov[mu] += gcov_(loc, j, i, mu, nu) * vcon[nu];
+    }
+    void raise(const Real vcov[4], Real vcon[4], int j, int i, int loc) const {
+        for (int mu = 0; mu < 4; ++mu) vcon[mu] = 0.;
+        for (int mu = 0; mu < 4; ++mu) for (int nu = 0; nu < 4; ++nu) vcon[mu] += gcon_(loc, j, i, mu, nu) * vcov[nu];
+    }
+};
+
+// ------------------------------------------------------------------ geometry
+// Cartesian Minkowski: coordinate_systems.hpp:77-86 (gcov = diag(-1,1,1,1)); the
+// cache is filled the way init_GRCoordinates does (gr_coordinates.cpp:120-205):
+// gcon = inverse, gdet = sqrt|det g| = 1, connection = 0.
+void init_geometry_minkowski(Block& b) {
+    b.gn1 = b.n1 + 1; b.gn2 = b.n2 + 1;
+    const size_t ng = (size_t)b.gn1 * b.gn2;
+    b.gcov.assign(NLOC * ng * 16, 0.); b.gcon.assign(NLOC * ng * 16, 0.); b.gdet.assign(NLOC * ng, 1.);
+    b.gdet_conn.assign(ng * 64, 0.);
+    for (size_t c = 0; c < NLOC * ng; ++c) {
+        for (int mu = 0; mu < 4; ++mu) {
+            b.gcov[c * 16 + mu * 5] = (mu == 0) ? -1. : 1.;
+            b.gcon[c * 16 + mu * 5] = (mu == 0) ? -1. : 1.;
+        }
+    }
+    b.cart_minkowski = true;
+}
+
+// --------------------------------------------------------- reconstruction
+// reconstruction.hpp:46 (EPS) and 128-204 (weno5 both sides / left / right)
+constexpr Real WENO_EPS = 1.e-26;
+void weno5(Real x1, Real x2, Real x3, Real x4, Real x5, Real& lout, Real& rout) {
+    Real beta[3], c1, c2;
+    c1 = x1 - 2.*x2 + x3; c2 = x1 - 4.*x2 + 3.*x3;
+    beta[0] = (13./12.)*c1*c1 + (1./4.)*c2*c2;
+    c1 = x2 - 2.*x3 + x4; c2 = x4 - x2;
+    beta[1] = (13./12.)*c1*c1 + (1./4.)*c2*c2;
+    c1 = x3 - 2.*x4 + x5; c2 = x5 - 4.*x4 + 3.*x3;
+    beta[2] = (13./12.)*c1*c1 + (1./4.)*c2*c2;
+    Real den[3] = {WENO_EPS + beta[0], WENO_EPS + beta[1], WENO_EPS + beta[2]};
+    den[0] *= den[0]; den[1] *= den[1]; den[2] *= den[2];
+    Real wtr[3] = {(1./16.)/den[0], (5./8.)/den[1], (5./16.)/den[2]};
+    Real Wr = wtr[0] + wtr[1] + wtr[2];
+    Real wtl[3] = {(1./16.)/den[2], (5./8.)/den[1], (5./16.)/den[0]};
+    Real Wl = wtl[0] + wtl[1] + wtl[2];
+    lout = ((3./8.)*x5 - (5./4.)*x4 + (15./8.)*x3)*(wtl[0] / Wl) +
+           ((-1./8.)*x4 + (3./4.)*x3 + (3./8.)*x2)*(wtl[1] / Wl) +
+           ((3./8.)*x3 + (3./4.)*x2 - (1./8.)*x1)*(wtl[2] / Wl);
+    rout = ((3./8.)*x1 - (5./4.)*x2 + (15./8.)*x3)*(wtr[0] / Wr) +
+           ((-1./8.)*x2 + (3./4.)*x3 + (3./8.)*x4)*(wtr[1] / Wr) +
+           ((3./8.)*x3 + (3./4.)*x4 - (1./8.)*x5)*(wtr[2] / Wr);
+}
+// reconstruction.hpp:52-56
+Real mc_lim(Real dm, Real dp) {
+    const Real r = (std::fabs(dp) > 0. ? dm/dp : 2.0);
+    return std::max(0.0, std::min(2.0, std::min(2*r, 0.5*(1+r))));
+}
+// reconstruction.hpp:107-112
+void linear_mc(Real, Real x2, Real x3, Real x4, Real, Real& lout, Real& rout) {
+    const Real dq = mc_lim(x3 - x2, x4 - x3)*(x4 - x3);
+    rout = x3 + 0.5*dq;
+    lout = x3 - 0.5*dq;
+}
+// reconstruction.hpp:351-382 (PPM, CW84 limiter)
+void ppm(Real q_im2, Real q_im1, Real q_i, Real q_ip1, Real q_ip2, Real& qlv, Real& qrv) {
+    qlv = (7.*(q_i + q_im1) - (q_im2 + q_ip1))/12.0;
+    qrv = (7.*(q_i + q_ip1) - (q_im1 + q_ip2))/12.0;
+    qlv = std::max(qlv, std::min(q_i, q_im1));
+    qlv = std::min(qlv, std::max(q_i, q_im1));
+    qrv = std::max(qrv, std::min(q_i, q_ip1));
+    qrv = std::min(qrv, std::max(q_i, q_ip1));
+    Real qc = qrv - q_i;
+    Real qd = qlv - q_i;
+    if ((qc*qd) >= 0.0) {
+        qlv = q_i; qrv = q_i;
+    } else {
+        if (std::fabs(qc) >= 2.0*std::fabs(qd)) qrv = q_i - 2.0*qd;
+        if (std::fabs(qd) >= 2.0*std::fabs(qc)) qlv = q_i - 2.0*qc;
+    }
+}
+// dispatch: "lout" is the value at the left face of the centre cell, "rout" at its right face
+void reconstruct(int recon, Real x1, Real x2, Real x3, Real x4, Real x5, Real& lout, Real& rout) {
+    switch (recon) {
+    case 0: lout = x3; rout = x3; break;            // donor_cell_c, :92-96
+    case 1: linear_mc(x1, x2, x3, x4, x5, lout, rout); break;
+    case 3: ppm(x1, x2, x3, x4, x5, lout, rout); break;
+    default: weno5(x1, x2, x3, x4, x5, lout, rout); break;
+    }
+}
+
+// ----------------------------------------------------------- GRMHD physics
+// grmhd_functions.hpp:116-129 (Local lorentz_calc)
+Real lorentz_calc(const Block& b, const Real* P, int j, int i, int loc) {
+    const Real qsq = b.gcov_(loc, j, i, 1, 1) * P[U1] * P[U1] +
+                     b.gcov_(loc, j, i, 2, 2) * P[U2] * P[U2] +
+                     b.gcov_(loc, j, i, 3, 3) * P[U3] * P[U3] +
+                     2. * (b.gcov_(loc, j, i, 1, 2) * P[U1] * P[U2] +
+                           b.gcov_(loc, j, i, 1, 3) * P[U1] * P[U3] +
+                           b.gcov_(loc, j, i, 2, 3) * P[U2] * P[U3]);
+    return std::sqrt(1. + qsq);
+}
+// grmhd_functions.hpp:225-246 (Local calc_4vecs, B present)
+void calc_4vecs(const Block& b, const Real* P, int j, int i, int loc, FourVectors& D) {
+    const Real gamma = lorentz_calc(b, P, j, i, loc);
+    const Real alpha = 1. / std::sqrt(-b.gcon_(loc, j, i, 0, 0));
+    D.ucon[0] = gamma / alpha;
+    for (int v = 0; v < 3; ++v) D.ucon[v+1] = P[U1 + v] - gamma * alpha * b.gcon_(loc, j, i, 0, v+1);
+    b.lower(D.ucon, D.ucov, j, i, loc);
+    D.bcon[0] = 0;
+    for (int v = 0; v < 3; ++v) D.bcon[0] += P[B1 + v] * D.ucov[v+1];
+    for (int v = 0; v < 3; ++v) D.bcon[v+1] = (P[B1 + v] + D.bcon[0] * D.ucon[v+1]) / D.ucon[0];
+    b.lower(D.bcon, D.bcov, j, i, loc);
+}
+// grmhd_functions.hpp:142-155
+void calc_tensor_mhd(Real rho, Real u, Real pgas, const FourVectors& D, int dir, Real mhd[4]) {
+    const Real bsq = dot(D.bcon, D.bcov);
+    const Real eta = pgas + rho + u + bsq;
+    const Real ptot = pgas + 0.5 * bsq;
+    for (int mu = 0; mu < 4; ++mu)
+        mhd[mu] = eta * D.ucon[dir] * D.ucov[mu] + ptot * (dir == mu) - D.bcon[dir] * D.bcov[mu];
+}
+// flux/flux_functions.hpp:116-178 (Local prim_to_flux, GRMHD + B_FluxCT variables only)
+void prim_to_flux(const Block& b, const Real* P, const FourVectors& D, int j, int i, int dir,
+                  Real* flux, int loc, bool with_B = true) {
+    const Real gam = b.p.gam;
+    const Real gdet = b.gdet_(loc, j, i);
+    flux[RHO] = P[RHO] * D.ucon[dir] * gdet;
+    Real T[4];
+    calc_tensor_mhd(P[RHO], P[UU], (gam - 1) * P[UU], D, dir, T);
+    flux[UU] = T[0] * gdet + flux[RHO];
+    flux[U1] = T[1] * gdet;
+    flux[U2] = T[2] * gdet;
+    flux[U3] = T[3] * gdet;
+    if (!with_B) return;
+    if (dir == 0) {
+        for (int v = 0; v < 3; ++v) flux[B1 + v] = P[B1 + v] * gdet;
+    } else {
+        for (int v = 0; v < 3; ++v) flux[B1 + v] = (D.bcon[v+1] * D.ucon[dir] - D.bcon[dir] * D.ucon[v+1]) * gdet;
+    }
+}
+// flux/flux_functions.hpp:368-439.  Note both clip() calls are no-ops in the
+// reference (their results are discarded, :378 and :409), so cs2/cms2 are NOT clamped.
+void vchar(const Block& b, const Real* P, const FourVectors& D, int j, int i, int loc, int dir,
+           Real& cmax, Real& cmin) {
+    const Real gam = b.p.gam;
+    const Real ef = P[RHO] + gam * P[UU];
+    const Real cs2 = gam * (gam - 1) * P[UU] / ef;
+    const Real bsq = dot(D.bcon, D.bcov);
+    const Real va2 = bsq / (bsq + ef);
+    const Real cms2 = cs2 + va2 - cs2 * va2;
+    Real A, B, C;
+    {
+        Real Bcov[4] = {1., 0., 0., 0.};
+        Real Acov[4] = {0., 0., 0., 0.}; Acov[dir] = 1.;
+        Real Acon[4], Bcon[4];
+        b.raise(Acov, Acon, j, i, loc);
+        b.raise(Bcov, Bcon, j, i, loc);
+        const Real Asq = dot(Acon, Acov);
+        const Real Bsq = dot(Bcon, Bcov);
+        const Real Au = dot(Acov, D.ucon);
+        const Real Bu = dot(Bcov, D.ucon);
+        const Real AB = dot(Acon, Bcov);
+        A = Bu*Bu - (Bsq + Bu*Bu) * cms2;
+        B = 2. * (Au*Bu - (AB + Au*Bu) * cms2);
+        C = Au*Au - (Asq + Au*Au) * cms2;
+    }
+    Real discr = std::sqrt(std::max(B * B - 4. * A * C, 0.));
+    Real vp = -(-B + discr) / (2. * A);
+    Real vm = -(-B - discr) / (2. * A);
+    cmax = std::max(vp, vm);
+    cmin = std::min(vp, vm);
+}
+// flux/flux.hpp:110-121
+inline Real llf(Real fL, Real fR, Real cmax, Real cmin, Real Ul, Real Ur) {
+    Real ctop = std::max(cmax, cmin);
+    return 0.5 * (fL + fR - ctop * (Ur - Ul));
+}
+inline Real hlle(Real fL, Real fR, Real cmax, Real cmin, Real Ul, Real Ur) {
+    return (cmax*fL + cmin*fR - cmax*cmin*(Ur - Ul)) / (cmax + cmin);
+}
+
+// -------------------------------------------------------------- floors
+// floors/floors_functions.hpp:300-332 (Local apply_geo_floors).  Spherical
+// geometry is not yet part of this oracle; Cartesian uses the constant floors.
+int apply_geo_floors(const Block& b, Real* P) {
+    const Real rhoflr_geom = b.p.rho_min_const;
+    const Real uflr_geom = b.p.u_min_const;
+    int fflag = 0;
+    fflag |= (rhoflr_geom > P[RHO]) * F_GEOM_RHO_FLUX;
+    fflag |= (uflr_geom > P[UU]) * F_GEOM_U_FLUX;
+    P[RHO] += std::max(0., rhoflr_geom - P[RHO]);
+    P[UU]  += std::max(0., uflr_geom - P[UU]);
+    return fflag;
+}
+
+// ---------------------------------------------------------------- helpers
+inline void load_prims(const Block& b, const std::vector<Real>& P, size_t c, Real out[NV]) {
+    for (int v = 0; v < NV; ++v) out[v] = P[v * b.nc + c];
+}
+
+// =================================================================== tasks
+
+// Flux::GetFlux<Recon,dir>, flux/get_flux.hpp:60-373.  Range: GetRange(interior,
+// FaceOf(dir), -1, +1) (:124 + domain.hpp:102-124).  The X1 left state of the
+// first face of that range is left unset by the reference's row
+// reconstruction (ReconstructX1 writes ql(i+1), reconstruction.hpp:511-527)
+// and nothing downstream reads that face; here it is reconstructed from cell is-2.
+void get_flux(Block& b, const std::vector<Real>& P, int dir) {
+    if (b.ndim < 3 && dir == 3) return;
+    if (b.ndim < 2 && dir == 2) return;
+    const int loc = dir - 1;  // face1/2/3
+    int is = b.is - 1, ie = b.ie + 1, js = b.js, je = b.je, ks = b.ks, ke = b.ke;
+    if (b.ndim > 1) { js = b.js - 1; je = b.je + 1; }
+    if (b.ndim > 2) { ks = b.ks - 1; ke = b.ke + 1; }
+    if (dir == 1) ie += 1;  // F1 bounds are [is, ie+1]
+    if (dir == 2) je += 1;
+    if (dir == 3) ke += 1;
+    const int s1 = (dir == 1) ? 1 : 0, s2 = (dir == 2) ? b.n1 : 0, s3 = (dir == 3) ? b.n1 * b.n2 : 0;
+    const long st = s1 + s2 + (long)s3;
+    const Real gam = b.p.gam;
+    (void)gam;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = ks; k <= ke; ++k)
+    for (int j = js; j <= je; ++j) {
+        for (int i = is; i <= ie; ++i) {
+            const size_t c = b.id(k, j, i);
+            Real Pl[NV], Pr[NV];
+            for (int v = 0; v < NV; ++v) {
+                const Real* q = &P[v * b.nc];
+                Real l_unused, r_unused;
+                // left state at this face = right edge of the cell below (c - st)
+                reconstruct(b.p.recon, q[c - 3*st], q[c - 2*st], q[c - st], q[c], q[c + st], l_unused, Pl[v]);
+                // right state = left edge of this cell
+                reconstruct(b.p.recon, q[c - 2*st], q[c - st], q[c], q[c + st], q[c + 2*st], Pr[v], r_unused);
+            }
+            // get_flux.hpp:170-182
+            if (b.p.recon_floors) {
+                apply_geo_floors(b, Pl);
+                apply_geo_floors(b, Pr);
+            }
+            for (int v = 0; v < NV; ++v) { b.Pl[v * b.nc + c] = Pl[v]; b.Pr[v * b.nc + c] = Pr[v]; }
+            Real Ul[NV], Ur[NV], Fl[NV], Fr[NV];
+            FourVectors D;
+            // calc_flux_left, get_flux.hpp:256-277
+            calc_4vecs(b, Pl, j, i, loc, D);
+            prim_to_flux(b, Pl, D, j, i, 0, Ul, loc);
+            prim_to_flux(b, Pl, D, j, i, dir, Fl, loc);
+            Real cmaxL, cminL;
+            vchar(b, Pl, D, j, i, loc, dir, cmaxL, cminL);
+            Real cmx = std::max(0., cmaxL);
+            Real cmn = std::min(0., cminL);
+            // calc_flux_right, get_flux.hpp:313-333
+            calc_4vecs(b, Pr, j, i, loc, D);
+            prim_to_flux(b, Pr, D, j, i, 0, Ur, loc);
+            prim_to_flux(b, Pr, D, j, i, dir, Fr, loc);
+            Real cmaxR, cminR;
+            vchar(b, Pr, D, j, i, loc, dir, cmaxR, cminR);
+            cmx = std::max(cmx, cmaxR);
+            cmn = -std::min(cmn, cminR);
+            b.cmax[(dir-1) * b.nc + c] = cmx;
+            b.cmin[(dir-1) * b.nc + c] = cmn;
+            // flux_llf / flux_hlle, get_flux.hpp:353-367
+            for (int v = 0; v < NV; ++v) {
+                b.Ul[v * b.nc + c] = Ul[v]; b.Ur[v * b.nc + c] = Ur[v];
+                b.Fl[v * b.nc + c] = Fl[v]; b.Fr[v * b.nc + c] = Fr[v];
+                b.F[((size_t)(dir-1) * NV + v) * b.nc + c] = b.p.use_hlle
+                    ? hlle(Fl[v], Fr[v], cmx, cmn, Ul[v], Ur[v])
+                    : llf(Fl[v], Fr[v], cmx, cmn, Ul[v], Ur[v]);
+            }
+        }
+    }
+}
+
+// B_FluxCT::FluxCT, b_flux_ct/b_flux_ct.cpp:307-370
+void flux_ct(Block& b) {
+    if (b.ndim < 2) return;
+    const int nd = b.ndim;
+    auto F = [&](int dir, int v, int k, int j, int i) -> Real& {
+        return b.F[((size_t)(dir-1) * NV + v) * b.nc + b.id(k, j, i)];
+    };
+    auto E = [&](int v, int k, int j, int i) -> Real& { return b.emf[(size_t)v * b.nc + b.id(k, j, i)]; };
+    const int ils = b.is, ile = b.ie + 1, jls = b.js, jle = b.je + 1;
+    const int kls = b.ks, kle = (nd > 2) ? b.ke + 1 : b.ke;
+#pragma omp parallel for collapse(2)
+    for (int k = kls; k <= kle; ++k)
+    for (int j = jls; j <= jle; ++j)
+    for (int i = ils; i <= ile; ++i) {
+        if (nd > 2) {
+            E(0, k, j, i) = 0.25 * (F(2, B3, k, j, i) + F(2, B3, k-1, j, i) -
+                                    F(3, B2, k, j, i) - F(3, B2, k, j-1, i));
+            E(1, k, j, i) = 0.25 * (F(3, B1, k, j, i) + F(3, B1, k, j, i-1) -
+                                    F(1, B3, k, j, i) - F(1, B3, k-1, j, i));
+        }
+        E(2, k, j, i) = 0.25 * (F(1, B2, k, j, i) + F(1, B2, k, j-1, i) -
+                                F(2, B1, k, j, i) - F(2, B1, k, j, i-1));
+    }
+#pragma omp parallel for collapse(2)
+    for (int k = b.ks; k <= b.ke; ++k)
+    for (int j = b.js; j <= b.je; ++j)
+    for (int i = ils; i <= ile; ++i) {
+        F(1, B1, k, j, i) = 0.0;
+        F(1, B2, k, j, i) = 0.5 * (E(2, k, j, i) + E(2, k, j+1, i));
+        if (nd > 2) F(1, B3, k, j, i) = -0.5 * (E(1, k, j, i) + E(1, k+1, j, i));
+    }
+#pragma omp parallel for collapse(2)
+    for (int k = b.ks; k <= b.ke; ++k)
+    for (int j = jls; j <= jle; ++j)
+    for (int i = b.is; i <= b.ie; ++i) {
+        F(2, B1, k, j, i) = -0.5 * (E(2, k, j, i) + E(2, k, j, i+1));
+        F(2, B2, k, j, i) = 0.0;
+        if (nd > 2) F(2, B3, k, j, i) = 0.5 * (E(0, k, j, i) + E(0, k+1, j, i));
+    }
+    if (nd > 2) {
+#pragma omp parallel for collapse(2)
+        for (int k = kls; k <= kle; ++k)
+        for (int j = b.js; j <= b.je; ++j)
+        for (int i = b.is; i <= b.ie; ++i) {
+            F(3, B1, k, j, i) = 0.5 * (E(1, k, j, i) + E(1, k, j, i+1));
+            F(3, B2, k, j, i) = -0.5 * (E(0, k, j, i) + E(0, k, j+1, i));
+            F(3, B3, k, j, i) = 0.0;
+        }
+    }
+}
+
+// KHARMADriver::FluxDivergence, driver/kharma_driver.hpp:208-231 ->
+// parthenon::Update::FluxDivHelper (not in the reference tree; restated from
+// UniformCartesian face areas / cell volume -- summation order parity unpinned).
+void flux_divergence(Block& b) {
+    const Real dx1 = b.dx[0], dx2 = b.dx[1], dx3 = b.dx[2];
+    const Real A1 = dx2 * dx3, A2 = dx1 * dx3, A3 = dx1 * dx2, V = dx1 * dx2 * dx3;
+    const int nd = b.ndim;
+    const long s2 = b.n1, s3 = (long)b.n1 * b.n2;
+#pragma omp parallel for collapse(3)
+    for (int v = 0; v < NV; ++v)
+    for (int k = b.ks; k <= b.ke; ++k)
+    for (int j = b.js; j <= b.je; ++j)
+    for (int i = b.is; i <= b.ie; ++i) {
+        const size_t c = b.id(k, j, i);
+        const Real* f1 = &b.F[((size_t)0 * NV + v) * b.nc];
+        const Real* f2 = &b.F[((size_t)1 * NV + v) * b.nc];
+        const Real* f3 = &b.F[((size_t)2 * NV + v) * b.nc];
+        Real du = (A1 * f1[c + 1] - A1 * f1[c]);
+        if (nd >= 2) du += (A2 * f2[c + s2] - A2 * f2[c]);
+        if (nd == 3) du += (A3 * f3[c + s3] - A3 * f3[c]);
+        b.dUdt[(size_t)v * b.nc + c] = -du / V;
+    }
+}
+
+// Flux::AddGeoSource, flux/flux.cpp:395-444.  Skipped in Cartesian Minkowski (:407).
+void add_geo_source(Block& b, const std::vector<Real>& P) {
+    if (b.cart_minkowski) return;
+    const Real gam = b.p.gam;
+#pragma omp parallel for collapse(2)
+    for (int k = b.ks; k <= b.ke; ++k)
+    for (int j = b.js; j <= b.je; ++j)
+    for (int i = b.is; i <= b.ie; ++i) {
+        const size_t c = b.id(k, j, i);
+        Real Pc[NV]; load_prims(b, P, c, Pc);
+        FourVectors D;
+        calc_4vecs(b, Pc, j, i, CENTER, D);
+        Real Tmu[4] = {0}, new_du[4] = {0};
+        for (int mu = 0; mu < 4; ++mu) {
+            calc_tensor_mhd(Pc[RHO], Pc[UU], (gam - 1) * Pc[UU], D, mu, Tmu);
+            for (int nu = 0; nu < 4; ++nu)
+                for (int lam = 0; lam < 4; ++lam)
+                    new_du[lam] += Tmu[nu] * b.gdet_conn_(j, i, nu, lam, mu);
+        }
+        b.dUdt[(size_t)UU * b.nc + c] += new_du[0];
+        for (int v = 0; v < 3; ++v) b.dUdt[(size_t)(U1 + v) * b.nc + c] += new_du[1 + v];
+    }
+}
+
+// KHARMADriver::AddStateUpdate, driver/kharma_driver.cpp:339-387, via
+// parthenon::Update::WeightedSumData (entire domain; not in the reference tree).
+void state_update(Block& b, const std::vector<Real>& Usub, const std::vector<Real>& Ufull,
+                  const std::vector<Real>& Psub, std::vector<Real>& Uout, std::vector<Real>& Pout,
+                  Real gam0, Real gam1, Real beta_dt) {
+    const size_t n = (size_t)NV * b.nc;
+#pragma omp parallel for
+    for (size_t c = 0; c < n; ++c) Uout[c] = gam0 * Usub[c] + gam1 * Ufull[c];
+#pragma omp parallel for
+    for (size_t c = 0; c < n; ++c) Uout[c] = 1.0 * Uout[c] + beta_dt * b.dUdt[c];
+    // Copy of the HD primitives (rho, u, uvec) sub -> out as the U->P guess (:379-384)
+    if (&Psub != &Pout) {
+#pragma omp parallel for
+        for (size_t c = 0; c < (size_t)5 * b.nc; ++c) Pout[c] = Psub[c];
+    }
+}
+
+// Periodic boundary exchange of one block with itself (the Parthenon ghost
+// exchange, AddBoundaryExchangeTasks at kharma_driver.cpp:151, is not in the
+// reference tree).  Synced variables: those flagged FillGhost = cons (8) and
+// the 5 GRMHD primitives (grmhd.cpp:155-165, kharma_step.cpp:129-138).
+void sync_periodic_var(Block& b, std::vector<Real>& A, int v) {
+    Real* a = &A[(size_t)v * b.nc];
+    const int N1 = b.p.nx1, N2 = b.p.nx2, N3 = b.p.nx3;
+    auto wrap = [](int x, int s, int N) { int r = (x - s) % N; if (r < 0) r += N; return s + r; };
+    // x1 then x2 then x3 sweep over the full extents fills edges and corners
+    for (int k = 0; k < b.n3; ++k) for (int j = 0; j < b.n2; ++j) for (int i = 0; i < b.n1; ++i) {
+        if (i >= b.is && i <= b.ie) continue;
+        a[b.id(k, j, i)] = a[b.id(k, j, wrap(i, b.is, N1))];
+    }
+    if (b.ndim > 1)
+        for (int k = 0; k < b.n3; ++k) for (int j = 0; j < b.n2; ++j) {
+            if (j >= b.js && j <= b.je) continue;
+            for (int i = 0; i < b.n1; ++i) a[b.id(k, j, i)] = a[b.id(k, wrap(j, b.js, N2), i)];
+        }
+    if (b.ndim > 2)
+        for (int k = 0; k < b.n3; ++k) {
+            if (k >= b.ks && k <= b.ke) continue;
+            for (int j = 0; j < b.n2; ++j) for (int i = 0; i < b.n1; ++i)
+                a[b.id(k, j, i)] = a[b.id(wrap(k, b.ks, N3), j, i)];
+        }
+}
+void sync_bounds(Block& b, std::vector<Real>& U, std::vector<Real>& P) {
+#pragma omp parallel for
+    for (int v = 0; v < NV + 5; ++v) {
+        if (v < NV) sync_periodic_var(b, U, v);
+        else sync_periodic_var(b, P, v - NV);
+    }
+}
+
+// Inverter::u_to_p<onedw>, inverter/onedw.hpp:49-233
+Real lorentz_calc_w(Real Bsq, Real D, Real QdB, Real Qtsq, Real Wp) {
+    const Real QdBsq = QdB * QdB;
+    const Real W = Wp + D;
+    const Real W2 = W * W;
+    const Real WB = W + Bsq;
+    const Real utsq = -((W + WB) * QdBsq + W2 * Qtsq) / (QdBsq * (W + WB) + W2 * (Qtsq - WB * WB));
+    if (utsq < -1.e-15 || utsq > 1.e7) return -1.;
+    return std::sqrt(1. + std::fabs(utsq));
+}
+Real err_eqn(Real gam, Real Bsq, Real D, Real Ep, Real QdB, Real Qtsq, Real Wp, int& eflag) {
+    const Real W = Wp + D;
+    const Real gamma = lorentz_calc_w(Bsq, D, QdB, Qtsq, Wp);
+    if (gamma < 1) eflag = S_BAD_UT;
+    const Real w = W / (gamma*gamma);
+    const Real rho = D / gamma;
+    const Real p = (w - rho) * (gam - 1) / gam;
+    return -Ep + Wp - p + 0.5 * Bsq + 0.5 * (Bsq * Qtsq - QdB * QdB) / SQR(Bsq + W);
+}
+int u_to_p_onedw(const Block& b, const Real* U, Real* P, int j, int i, int loc, int max_iterations, Real tol) {
+    const Real gam = b.p.gam;
+    if (U[RHO] <= 0.) return S_NEG_INPUT;
+    const Real alpha = 1./std::sqrt(-b.gcon_(loc, j, i, 0, 0));
+    const Real gdet = b.gdet_(loc, j, i);
+    const Real a_over_g = alpha / gdet;
+    const Real D = U[RHO] * a_over_g;
+    Real Bcon[4] = {0, 0, 0, 0};
+    Bcon[1] = U[B1] * a_over_g;
+    Bcon[2] = U[B2] * a_over_g;
+    Bcon[3] = U[B3] * a_over_g;
+    const Real Qcov[4] = {(U[UU] - U[RHO]) * a_over_g, U[U1] * a_over_g, U[U2] * a_over_g, U[U3] * a_over_g};
+    const Real ncov[4] = {-alpha, 0., 0., 0.};
+    Real Bcov[4], Qcon[4], ncon[4];
+    b.lower(Bcon, Bcov, j, i, loc);
+    b.raise(Qcov, Qcon, j, i, loc);
+    b.raise(ncov, ncon, j, i, loc);
+    const Real Bsq = dot(Bcon, Bcov);
+    const Real QdB = dot(Bcon, Qcov);
+    const Real Qdotn = dot(Qcon, ncov);
+    Real Qtcon[4];
+    for (int mu = 0; mu < 4; ++mu) Qtcon[mu] = Qcon[mu] + ncon[mu] * Qdotn;
+    const Real Qtsq = dot(Qcon, Qcov) + Qdotn*Qdotn;
+    const Real Ep = -Qdotn - D;
+    int eflag = S_SUCCESS;
+    Real Wp, err;
+    {
+        const Real gamma = lorentz_calc(b, P, j, i, loc);
+        if (gamma < 1) return S_BAD_UT;
+        const Real rho = P[RHO], u = P[UU];
+        Wp = (rho + u + (gam - 1) * u) * gamma * gamma - rho * gamma;
+        err = err_eqn(gam, Bsq, D, Ep, QdB, Qtsq, Wp, eflag);
+    }
+    Real dW;
+    {
+        const Real Wpm = (1. - 1.e-5) * Wp;
+        const Real h = Wp - Wpm;
+        const Real Wpp = Wp + h;
+        const Real errm = err_eqn(gam, Bsq, D, Ep, QdB, Qtsq, Wpm, eflag);
+        const Real errp = err_eqn(gam, Bsq, D, Ep, QdB, Qtsq, Wpp, eflag);
+        const Real dedW = (errp - errm) / (Wpp - Wpm);
+        const Real dedW2 = (errp - 2. * err + errm) / (h*h);
+        const Real f = clip(0.5 * err * dedW2 / (dedW*dedW), -0.3, 0.3);
+        dW = clip(-err / dedW / (1. - f), -0.5*Wp, 2.0*Wp);
+    }
+    Real Wp1 = Wp;
+    Real err1 = err;
+    Wp += dW;
+    err = err_eqn(gam, Bsq, D, Ep, QdB, Qtsq, Wp, eflag);
+    int iter = 0;
+    for (iter = 0; iter < max_iterations; iter++) {
+        dW = clip((Wp1 - Wp) * err / (err - err1), -0.5*Wp, 2.0*Wp);
+        Wp1 = Wp;
+        err1 = err;
+        Wp += dW;
+        if (std::fabs(dW / Wp) < tol) break;
+        err = err_eqn(gam, Bsq, D, Ep, QdB, Qtsq, Wp, eflag);
+        if (std::fabs(err / Wp) < tol) break;
+    }
+    if (iter == max_iterations) return S_MAX_ITER;
+    const Real gamma = lorentz_calc_w(Bsq, D, QdB, Qtsq, Wp);
+    if (gamma < 1) return S_BAD_UT;
+    const Real rho = D / gamma;
+    const Real W = Wp + D;
+    const Real w = W / (gamma*gamma);
+    const Real p = (w - rho) * (gam - 1) / gam;
+    const Real u = w - (rho + p);
+    if (rho < 0 && u < 0) return S_NEG_RHOU;
+    else if (rho < 0) return S_NEG_RHO;
+    else if (u < 0) return S_NEG_U;
+    P[RHO] = rho;
+    P[UU] = u;
+    const Real pre = (gamma / (W + Bsq));
+    P[U1] = pre * (Qtcon[1] + QdB * Bcon[1] / W);
+    P[U2] = pre * (Qtcon[2] + QdB * Bcon[2] / W);
+    P[U3] = pre * (Qtcon[3] + QdB * Bcon[3] / W);
+    return S_SUCCESS;
+}
+
+// Packages::MeshUtoP -> Inverter::BlockUtoP (inverter.cpp:131-200, physical
+// range = entire for a periodic block, domain.hpp:161-187) then
+// B_FluxCT::BlockUtoP (b_flux_ct.cpp:195-217, entire).
+void utop(Block& b, std::vector<Real>& U, std::vector<Real>& P) {
+#pragma omp parallel for collapse(2)
+    for (int k = 0; k < b.n3; ++k)
+    for (int j = 0; j < b.n2; ++j)
+    for (int i = 0; i < b.n1; ++i) {
+        const size_t c = b.id(k, j, i);
+        Real Uc[NV], Pc[NV];
+        load_prims(b, U, c, Uc); load_prims(b, P, c, Pc);
+        int pflagl = u_to_p_onedw(b, Uc, Pc, j, i, CENTER, b.p.iter_max, b.p.err_tol);
+        for (int v = 0; v < 5; ++v) P[v * b.nc + c] = Pc[v];
+        b.pflag[c] = pflagl % FFLAG_MINIMUM;
+        b.fflag[c] = (pflagl / FFLAG_MINIMUM) * FFLAG_MINIMUM;
+    }
+#pragma omp parallel for collapse(2)
+    for (int v = 0; v < 3; ++v)
+    for (int k = 0; k < b.n3; ++k)
+    for (int j = 0; j < b.n2; ++j)
+    for (int i = 0; i < b.n1; ++i) {
+        const size_t c = b.id(k, j, i);
+        P[(B1 + v) * b.nc + c] = U[(B1 + v) * b.nc + c] / b.gdet_(CENTER, j, i);
+    }
+}
+
+// floors/floors_functions.hpp:84-160 (Cartesian or spherical geometric floors)
+int determine_floors(const Block& b, const Real* P, int j, int i, Real& rhoflr_max, Real& uflr_max) {
+    const auto& f = b.p;
+    const Real gam = f.gam;
+    Real rhoflr_geom, uflr_geom;
+    rhoflr_geom = f.rho_min_const;
+    uflr_geom = f.u_min_const;
+    FourVectors D;
+    calc_4vecs(b, P, j, i, CENTER, D);
+    Real bsq = dot(D.bcon, D.bcov);
+    Real rhoflr_b = bsq / f.bsq_over_rho_max;
+    Real uflr_b = bsq / f.bsq_over_u_max;
+    uflr_max = std::max(uflr_geom, uflr_b);
+    const Real rho = P[RHO];
+    const Real u = P[UU];
+    int fflag = 0;
+    if (!f.temp_adjust_u) {
+        const double rhoflr_temp = std::max(u, uflr_max) / f.u_over_rho_max;
+        fflag |= (rhoflr_temp > rho) * F_TEMP;
+        rhoflr_max = std::max(std::max(rhoflr_geom, rhoflr_b), rhoflr_temp);
+    } else {
+        rhoflr_max = std::max(rhoflr_geom, rhoflr_b);
+    }
+    if (rhoflr_max > rho || uflr_max > u) {
+        fflag |= (rhoflr_geom > rho) * F_GEOM_RHO;
+        fflag |= (uflr_geom > u) * F_GEOM_U;
+        fflag |= (rhoflr_b > rho) * F_B_RHO;
+        fflag |= (uflr_b > u) * F_B_U;
+    }
+    if (lorentz_calc(b, P, j, i, CENTER) > f.gamma_max) fflag |= F_GAMMA;
+    if ((gam - 1.) * P[UU] / std::pow(P[RHO], gam) > f.ktot_max) fflag |= F_KTOT;
+    if (f.temp_adjust_u && (P[UU] / P[RHO] > f.u_over_rho_max)) fflag |= F_TEMP;
+    return fflag;
+}
+// floors_functions.hpp:186-257 (drift frame)
+int apply_floors_drift(const Block& b, Real* P, int j, int i, Real rhoflr_max, Real uflr_max) {
+    const Real gam = b.p.gam;
+    const Real lapse2 = 1. / (-b.gcon_(CENTER, j, i, 0, 0));
+    double beta[4] = {0};
+    beta[1] = lapse2 * b.gcon_(CENTER, j, i, 0, 1);
+    beta[2] = lapse2 * b.gcon_(CENTER, j, i, 0, 2);
+    beta[3] = lapse2 * b.gcon_(CENTER, j, i, 0, 3);
+    const Real rho = P[RHO];
+    const Real uu = P[UU];
+    const Real pg = (gam - 1.) * uu;
+    const Real w_old = std::max(rho + uu + pg, SMALL);
+    Real Bcon[4] = {0}, Bcov[4] = {0};
+    Bcon[0] = 0; Bcon[1] = P[B1]; Bcon[2] = P[B2]; Bcon[3] = P[B3];
+    for (int mu = 0; mu < 4; ++mu) for (int nu = 0; nu < 4; ++nu) Bcov[mu] += b.gcov_(CENTER, j, i, mu, nu) * Bcon[nu];
+    const Real Bsq = std::max(dot(Bcon, Bcov), SMALL);
+    const Real B_mag = std::sqrt(Bsq);
+    FourVectors D;
+    calc_4vecs(b, P, j, i, CENTER, D);
+    Real Qcov[4] = {0};
+    Qcov[0] = w_old * D.ucon[0] * D.ucov[0] + pg;
+    Qcov[1] = w_old * D.ucon[0] * D.ucov[1];
+    Qcov[2] = w_old * D.ucon[0] * D.ucov[2];
+    Qcov[3] = w_old * D.ucon[0] * D.ucov[3];
+    double QdotB = dot(Bcon, Qcov);
+    Real vpar = QdotB / (B_mag * w_old * D.ucon[0]*D.ucon[0]);
+    Real ucon_dr[4] = {0};
+    ucon_dr[0] = 1. / std::sqrt(1. / (D.ucon[0]*D.ucon[0]) + vpar*vpar);
+    for (int mu = 0; mu < 4; ++mu) ucon_dr[mu] = D.ucon[mu] * (ucon_dr[0] / D.ucon[0]) - (vpar * Bcon[mu] * ucon_dr[0] / B_mag);
+    P[RHO] = std::max(rho, rhoflr_max);
+    P[UU] = std::max(uu, uflr_max);
+    const Real pg_new = (gam - 1.) * P[UU];
+    const Real w_new = P[RHO] + P[UU] + pg_new;
+    const Real x = (2. * QdotB) / (B_mag * w_new * ucon_dr[0]);
+    vpar = x / (1 + std::sqrt(1 + x*x)) * (1. / ucon_dr[0]);
+    D.ucon[0] = 1. / std::sqrt(1/(ucon_dr[0]*ucon_dr[0]) - vpar*vpar);
+    for (int mu = 0; mu < 4; ++mu) D.ucon[mu] = ucon_dr[mu] * (D.ucon[0] / ucon_dr[0]) + (vpar * Bcon[mu] * D.ucon[0] / B_mag);
+    b.lower(D.ucon, D.ucov, j, i, CENTER);
+    P[U1] = D.ucon[1] + (beta[1] * D.ucon[0]);
+    P[U2] = D.ucon[2] + (beta[2] * D.ucon[0]);
+    P[U3] = D.ucon[3] + (beta[3] * D.ucon[0]);
+    return 0;
+}
+// floors_functions.hpp:178-183 (fluid frame)
+int apply_floors_fluid(Real* P, Real rhoflr_max, Real uflr_max) {
+    P[RHO] += std::max(0., rhoflr_max - P[RHO]);
+    P[UU]  += std::max(0., uflr_max - P[UU]);
+    return 0;
+}
+// floors_functions.hpp:50-82
+void apply_ceilings(const Block& b, Real* P, int j, int i) {
+    const auto& f = b.p;
+    const Real gam = f.gam;
+    Real gamma = lorentz_calc(b, P, j, i, CENTER);
+    Real ktot = (gam - 1.) * P[UU] / std::pow(P[RHO], gam);
+    if (gamma > f.gamma_max) {
+        Real fr = std::sqrt((SQR(f.gamma_max) - 1.) / (SQR(gamma) - 1.));
+        for (int v = 0; v < 3; ++v) P[U1 + v] *= fr;
+    }
+    if (ktot > f.ktot_max) P[UU] = f.ktot_max / ktot * P[UU];
+    if (f.temp_adjust_u && P[UU] / P[RHO] > f.u_over_rho_max) P[UU] = f.u_over_rho_max * P[RHO];
+}
+// normal observer frame, floors_functions.hpp:260-289 (onedw recovery, 8 iters, 1e-8)
+int apply_floors_normal(const Block& b, Real* P, Real* U, int j, int i, Real rhoflr_max, Real uflr_max) {
+    const Real gam = b.p.gam;
+    const Real rho_add = std::max(0., rhoflr_max - P[RHO]);
+    const Real u_add = std::max(0., uflr_max - P[UU]);
+    Real Padd[NV] = {rho_add, u_add, 0, 0, 0, 0, 0, 0};
+    // GRMHD::p_to_u_mhd, grmhd_functions.hpp:356-375
+    FourVectors D;
+    calc_4vecs(b, Padd, j, i, CENTER, D);
+    const Real gdet = b.gdet_(CENTER, j, i);
+    Real rho_ut = rho_add * D.ucon[0] * gdet;
+    Real mhd[4];
+    calc_tensor_mhd(rho_add, u_add, (gam - 1) * u_add, D, 0, mhd);
+    Real T[4];
+    T[0] = mhd[0] * gdet + rho_ut;
+    for (int v = 0; v < 3; ++v) T[1 + v] = mhd[1 + v] * gdet;
+    P[RHO] += rho_add;
+    P[UU] += u_add;
+    U[RHO] += rho_ut;
+    U[UU] += T[0];
+    U[U1] += T[1];
+    U[U2] += T[2];
+    U[U3] += T[3];
+    return u_to_p_onedw(b, U, P, j, i, CENTER, 8, 1e-8);
+}
+// Flux::p_to_u_mhd (flux_functions.hpp:321-328): calc_4vecs at centre + 5 conserved
+void p_to_u_mhd(const Block& b, const Real* P, Real* U, int j, int i) {
+    FourVectors D;
+    calc_4vecs(b, P, j, i, CENTER, D);
+    prim_to_flux(b, P, D, j, i, 0, U, CENTER, false);
+}
+// Floors::ApplyGRMHDFloors -> ApplyFloorsInFrame, floors/floors_impl.hpp:45-128 and
+// DetermineGRMHDFloors, floors.cpp:197-231, over the entire domain.
+void apply_floors(Block& b, std::vector<Real>& U, std::vector<Real>& P) {
+    if (!b.p.floors_on) return;
+#pragma omp parallel for collapse(2)
+    for (int k = 0; k < b.n3; ++k)
+    for (int j = 0; j < b.n2; ++j)
+    for (int i = 0; i < b.n1; ++i) {
+        const size_t c = b.id(k, j, i);
+        Real Pc[NV]; load_prims(b, P, c, Pc);
+        Real rf, uf;
+        int ff = determine_floors(b, Pc, j, i, rf, uf);
+        b.fflag[c] = (Real)((int)b.fflag[c] | ff);
+        b.rho_floor[c] = rf; b.u_floor[c] = uf;
+    }
+#pragma omp parallel for collapse(2)
+    for (int k = 0; k < b.n3; ++k)
+    for (int j = 0; j < b.n2; ++j)
+    for (int i = 0; i < b.n1; ++i) {
+        const size_t c = b.id(k, j, i);
+        if (!(int)b.fflag[c]) continue;
+        Real Pc[NV], Uc[NV];
+        load_prims(b, P, c, Pc); load_prims(b, U, c, Uc);
+        int pflag_l = 0;
+        if (b.p.floor_frame == 0) pflag_l = apply_floors_fluid(Pc, b.rho_floor[c], b.u_floor[c]);
+        else if (b.p.floor_frame == 1) pflag_l = apply_floors_normal(b, Pc, Uc, j, i, b.rho_floor[c], b.u_floor[c]);
+        else pflag_l = apply_floors_drift(b, Pc, j, i, b.rho_floor[c], b.u_floor[c]);
+        if (pflag_l) b.pflag[c] = pflag_l;
+        apply_ceilings(b, Pc, j, i);
+        p_to_u_mhd(b, Pc, Uc, j, i);
+        for (int v = 0; v < 5; ++v) { P[v * b.nc + c] = Pc[v]; U[v * b.nc + c] = Uc[v]; }
+    }
+}
+
+// Inverter::FixUtoP, inverter/fixup.cpp:47-156 (physical range = entire here)
+void fix_utop(Block& b, std::vector<Real>& U, std::vector<Real>& P) {
+    if (!b.p.fix_average && !b.p.fix_atmo) return;
+    // First pass reads only non-failed zones and writes only failed ones, so it is
+    // order independent.
+#pragma omp parallel for collapse(2)
+    for (int k = 0; k < b.n3; ++k)
+    for (int j = 0; j < b.n2; ++j)
+    for (int i = 0; i < b.n1; ++i) {
+        const size_t c = b.id(k, j, i);
+        if (!(b.pflag[c] > 0)) continue;
+        double wsum = 0., sum[5] = {0};
+        if (b.p.fix_average) {
+            for (int n = -1; n <= 1; n++) for (int m = -1; m <= 1; m++) for (int l = -1; l <= 1; l++) {
+                int ii = i + l, jj = j + m, kk = k + n;
+                if (ii < 0 || ii >= b.n1 || jj < 0 || jj >= b.n2 || kk < 0 || kk >= b.n3) continue;
+                const size_t cc = b.id(kk, jj, ii);
+                if (!(b.pflag[cc] > 0)) {
+                    double w = 1./(std::abs(l) + std::abs(m) + std::abs(n) + 1);
+                    wsum += w;
+                    for (int p = 0; p < 5; ++p) sum[p] += w * P[p * b.nc + cc];
+                }
+            }
+        }
+        if (wsum < 1.e-10) { for (int p = 0; p < 5; ++p) P[p * b.nc + c] = 0.; }
+        else { for (int p = 0; p < 5; ++p) P[p * b.nc + c] = sum[p]/wsum; }
+    }
+#pragma omp parallel for collapse(2)
+    for (int k = 0; k < b.n3; ++k)
+    for (int j = 0; j < b.n2; ++j)
+    for (int i = 0; i < b.n1; ++i) {
+        const size_t c = b.id(k, j, i);
+        if (!(b.pflag[c] > 0)) continue;
+        Real Pc[NV], Uc[NV];
+        load_prims(b, P, c, Pc); load_prims(b, U, c, Uc);
+        // Floors::apply_geo_floors (Global), floors_functions.hpp:334-367
+        Pc[RHO] += std::max(0., b.p.rho_min_const - Pc[RHO]);
+        Pc[UU]  += std::max(0., b.p.u_min_const - Pc[UU]);
+        // GRMHD::p_to_u, grmhd_functions.hpp:270-298 (5 conserved)
+        FourVectors D;
+        calc_4vecs(b, Pc, j, i, CENTER, D);
+        prim_to_flux(b, Pc, D, j, i, 0, Uc, CENTER, false);
+        for (int v = 0; v < 5; ++v) { P[v * b.nc + c] = Pc[v]; U[v * b.nc + c] = Uc[v]; }
+    }
+}
+
+// Flux::MeshPtoU(entire) -> Flux::p_to_u, flux/flux.cpp:276-322, flux_functions.hpp:343-352
+void ptou(Block& b, const std::vector<Real>& P, std::vector<Real>& U) {
+#pragma omp parallel for collapse(2)
+    for (int k = 0; k < b.n3; ++k)
+    for (int j = 0; j < b.n2; ++j)
+    for (int i = 0; i < b.n1; ++i) {
+        const size_t c = b.id(k, j, i);
+        Real Pc[NV], Uc[NV];
+        load_prims(b, P, c, Pc);
+        FourVectors D;
+        calc_4vecs(b, Pc, j, i, CENTER, D);
+        prim_to_flux(b, Pc, D, j, i, 0, Uc, CENTER, true);
+        for (int v = 0; v < NV; ++v) U[v * b.nc + c] = Uc[v];
+    }
+}
+
+// GRMHD::EstimateTimestep in-loop branch, grmhd/grmhd.cpp:310-339
+double estimate_timestep(const Block& b) {
+    double min_ndt = std::numeric_limits<double>::max();
+    const Real dx1 = b.dx[0], dx2 = b.dx[1], dx3 = b.dx[2];
+#pragma omp parallel for collapse(2) reduction(min:min_ndt)
+    for (int k = b.ks; k <= b.ke; ++k)
+    for (int j = b.js; j <= b.je; ++j)
+    for (int i = b.is; i <= b.ie; ++i) {
+        const size_t c = b.id(k, j, i);
+        double ndt_zone = 1.0 / (1 / (dx1 / std::max(b.cmax[c], b.cmin[c])) +
+                                 1 / (dx2 / std::max(b.cmax[b.nc + c], b.cmin[b.nc + c])) +
+                                 1 / (dx3 / std::max(b.cmax[2 * b.nc + c], b.cmin[2 * b.nc + c])));
+        if (!std::isnan(ndt_zone) && ndt_zone < min_ndt) min_ndt = ndt_zone;
+    }
+    const double dt_max = b.p.max_dt_increase * b.dt_last;
+    return clip(min_ndt * b.p.cfl, b.p.dt_min, dt_max);
+}
+
+// B_FluxCT::MaxDivB with corner_div, b_flux_ct.cpp:566-607, b_flux_ct_functions.hpp:52-75
+double max_divb(const Block& b, const std::vector<Real>& U) {
+    if (b.ndim < 2) return 0.;
+    const bool do3 = b.ndim > 2;
+    const int jsb = b.js, jeb = b.je + 1, ksb = b.ks, keb = b.ke + (do3 ? 1 : 0);
+    const int isb = b.is, ieb = b.ie + 1;
+    const Real* B1v = &U[(size_t)B1 * b.nc];
+    const Real* B2v = &U[(size_t)B2 * b.nc];
+    const Real* B3v = &U[(size_t)B3 * b.nc];
+    double mx = 0.;
+#pragma omp parallel for collapse(2) reduction(max:mx)
+    for (int k = ksb; k <= keb; ++k)
+    for (int j = jsb; j <= jeb; ++j)
+    for (int i = isb; i <= ieb; ++i) {
+        auto id = [&](int kk, int jj, int ii) { return b.id(kk, jj, ii); };
+        const double norm = do3 ? 0.25 : 0.5;
+        double term1 = B1v[id(k, j, i)] - B1v[id(k, j, i-1)] + B1v[id(k, j-1, i)] - B1v[id(k, j-1, i-1)];
+        double term2 = B2v[id(k, j, i)] - B2v[id(k, j-1, i)] + B2v[id(k, j, i-1)] - B2v[id(k, j-1, i-1)];
+        double term3 = 0.;
+        if (do3) {
+            term1 += B1v[id(k-1, j, i)] + B1v[id(k-1, j-1, i)] - B1v[id(k-1, j, i-1)] - B1v[id(k-1, j-1, i-1)];
+            term2 += B2v[id(k-1, j, i)] + B2v[id(k-1, j, i-1)] - B2v[id(k-1, j-1, i)] - B2v[id(k-1, j-1, i-1)];
+            term3 = B3v[id(k, j, i)] + B3v[id(k, j-1, i)] + B3v[id(k, j, i-1)] + B3v[id(k, j-1, i-1)]
+                  - B3v[id(k-1, j, i)] - B3v[id(k-1, j-1, i)] - B3v[id(k-1, j, i-1)] - B3v[id(k-1, j-1, i-1)];
+        }
+        double d = std::fabs(norm*term1/b.dx[0] + norm*term2/b.dx[1] + norm*term3/b.dx[2]);
+        if (d > mx) mx = d;
+    }
+    return mx;
+}
+
+// Integrator weights for parthenon's low-storage "rk2" / "vl2" (not in the
+// reference tree; values restated, parity unpinned).  (gam0, gam1, beta)
+void integrator_weights(int integrator, int stage, Real& gam0, Real& gam1, Real& beta) {
+    if (integrator == 1) {  // vl2
+        gam0 = 0.; gam1 = 1.; beta = (stage == 1) ? 0.5 : 1.0;
+    } else {                // rk2
+        gam0 = (stage == 1) ? 0. : 0.5; gam1 = (stage == 1) ? 1. : 0.5; beta = (stage == 1) ? 1. : 0.5;
+    }
+}
+
+// One stage of KHARMADriver::MakeDefaultTaskCollection, driver/kharma_step.cpp:77-319.
+// Stage containers follow parthenon's MultiStageDriver naming: stage 1 is
+// base -> "1", stage 2 is "1" -> base.
+void run_stage(Block& b, int stage) {
+    std::vector<Real>& Usub = (stage == 1) ? b.U0 : b.U1c;
+    std::vector<Real>& Psub = (stage == 1) ? b.P0 : b.P1c;
+    std::vector<Real>& Uout = (stage == 1) ? b.U1c : b.U0;
+    std::vector<Real>& Pout = (stage == 1) ? b.P1c : b.P0;
+    // Flux region (kharma_step.cpp:141-210)
+    for (int dir = 1; dir <= 3; ++dir) get_flux(b, Psub, dir);
+    flux_ct(b);
+    flux_divergence(b);
+    add_geo_source(b, Psub);
+    Real gam0, gam1, beta;
+    integrator_weights(b.p.integrator, stage, gam0, gam1, beta);
+    state_update(b, Usub, b.U0, Psub, Uout, Pout, gam0, gam1, beta * b.dt);
+    sync_bounds(b, Uout, Pout);
+    // Fix region (kharma_step.cpp:216-288)
+    utop(b, Uout, Pout);
+    apply_floors(b, Uout, Pout);
+    fix_utop(b, Uout, Pout);
+    ptou(b, Pout, Uout);
+    // Second sync (two_sync, kharma_step.cpp:307-314)
+    sync_bounds(b, Uout, Pout);
+}
+
+// EvolutionDriver step + KHARMA dt logic: PreStepWork (kharma.cpp:121-129),
+// stages, EstimateTimestep (grmhd.cpp:252-343), SetGlobalTimeStep
+// (kharma_driver.cpp:439-460).  Returns the dt that was taken.
+double step(Block& b) {
+    b.in_loop = true;
+    b.dt_last = b.dt;
+    for (int stage = 1; stage <= 2; ++stage) run_stage(b, stage);
+    double new_dt = estimate_timestep(b);
+    const double taken = b.dt;
+    b.time += b.dt;
+    b.ncycle++;
+    double dt = b.dt;
+    if (dt < 0.1 * std::numeric_limits<Real>::max()) dt *= 2.0;
+    dt = std::min(dt, new_dt);
+    if (b.time < b.p.tlim && (b.p.tlim - b.time) < dt) dt = b.p.tlim - b.time;
+    b.dt = dt;
+    return taken;
+}
+
+}  // namespace
+
+// ================================================================== C ABI
+extern "C" {
+
+void* or_create(const or_params* p) {
+    Block* b = new Block();
+    b->p = *p;
+    b->ndim = (p->nx3 > 1) ? 3 : ((p->nx2 > 1) ? 2 : 1);
+    const int ng = p->ng;
+    b->n1 = p->nx1 + 2 * ng;
+    b->n2 = (b->ndim > 1) ? p->nx2 + 2 * ng : 1;
+    b->n3 = (b->ndim > 2) ? p->nx3 + 2 * ng : 1;
+    b->is = ng; b->ie = ng + p->nx1 - 1;
+    b->js = (b->ndim > 1) ? ng : 0; b->je = (b->ndim > 1) ? ng + p->nx2 - 1 : 0;
+    b->ks = (b->ndim > 2) ? ng : 0; b->ke = (b->ndim > 2) ? ng + p->nx3 - 1 : 0;
+    b->nc = (size_t)b->n1 * b->n2 * b->n3;
+    b->dx[0] = (p->x1max - p->x1min) / p->nx1;
+    b->dx[1] = (p->x2max - p->x2min) / p->nx2;
+    b->dx[2] = (p->x3max - p->x3min) / p->nx3;
+    init_geometry_minkowski(*b);
+    const size_t nvc = (size_t)NV * b->nc;
+    b->U0.assign(nvc, 0.); b->P0.assign(nvc, 0.); b->U1c.assign(nvc, 0.); b->P1c.assign(nvc, 0.);
+    b->F.assign(3 * nvc, 0.);
+    b->cmax.assign(3 * b->nc, 0.); b->cmin.assign(3 * b->nc, 0.);
+    b->Pl.assign(nvc, 0.); b->Pr.assign(nvc, 0.); b->Ul.assign(nvc, 0.); b->Ur.assign(nvc, 0.);
+    b->Fl.assign(nvc, 0.); b->Fr.assign(nvc, 0.);
+    b->emf.assign(3 * b->nc, 0.); b->dUdt.assign(nvc, 0.);
+    b->pflag.assign(b->nc, 0.); b->fflag.assign(b->nc, 0.);
+    b->rho_floor.assign(b->nc, 0.); b->u_floor.assign(b->nc, 0.);
+    return b;
+}
+
+void or_destroy(void* h) { delete static_cast<Block*>(h); }
+
+size_t or_ncells(void* h) { return static_cast<Block*>(h)->nc; }
+
+// Set the initial primitives (all 8, entire block) and do what
+// KHARMA::PostInitialize does for the fields on this path
+// (post_initialize.cpp:55-193): sync, P->U over the entire domain, sync.
+// dt is dt_start for the first step (grmhd.cpp:283-289).
+void or_init_state(void* h, const double* P) {
+    Block& b = *static_cast<Block*>(h);
+    std::memcpy(b.P0.data(), P, sizeof(double) * NV * b.nc);
+    sync_bounds(b, b.U0, b.P0);
+    ptou(b, b.P0, b.U0);
+    sync_bounds(b, b.U0, b.P0);
+    b.time = 0.; b.ncycle = 0; b.in_loop = false;
+    b.dt = b.p.dt_start;
+    b.dt_last = b.dt;
+}
+
+// Overwrite base-container U and P directly (no P->U), for task-level tests.
+void or_set_state(void* h, const double* P, const double* U) {
+    Block& b = *static_cast<Block*>(h);
+    std::memcpy(b.P0.data(), P, sizeof(double) * NV * b.nc);
+    std::memcpy(b.U0.data(), U, sizeof(double) * NV * b.nc);
+}
+
+void or_get_state(void* h, double* P, double* U) {
+    Block& b = *static_cast<Block*>(h);
+    if (P) std::memcpy(P, b.P0.data(), sizeof(double) * NV * b.nc);
+    if (U) std::memcpy(U, b.U0.data(), sizeof(double) * NV * b.nc);
+}
+
+void or_set_dt(void* h, double dt) { static_cast<Block*>(h)->dt = dt; static_cast<Block*>(h)->dt_last = dt; }
+double or_get_dt(void* h) { return static_cast<Block*>(h)->dt; }
+double or_get_time(void* h) { return static_cast<Block*>(h)->time; }
+
+// Run nsteps full RK2 steps; dts[n] receives the dt of each step (may be null).
+int or_step(void* h, int nsteps, double* dts) {
+    Block& b = *static_cast<Block*>(h);
+    for (int n = 0; n < nsteps; ++n) {
+        double taken = step(b);
+        if (dts) dts[n] = taken;
+    }
+    return 0;
+}
+
+// Task-level entry points on the base container.
+void or_get_flux(void* h, int dir) { Block& b = *static_cast<Block*>(h); get_flux(b, b.P0, dir); }
+void or_flux_ct(void* h) { flux_ct(*static_cast<Block*>(h)); }
+void or_flux_divergence(void* h) { flux_divergence(*static_cast<Block*>(h)); }
+void or_utop(void* h) { Block& b = *static_cast<Block*>(h); utop(b, b.U0, b.P0); }
+void or_apply_floors(void* h) { Block& b = *static_cast<Block*>(h); apply_floors(b, b.U0, b.P0); }
+void or_fix_utop(void* h) { Block& b = *static_cast<Block*>(h); fix_utop(b, b.U0, b.P0); }
+void or_ptou(void* h) { Block& b = *static_cast<Block*>(h); ptou(b, b.P0, b.U0); }
+double or_estimate_timestep(void* h) { return estimate_timestep(*static_cast<Block*>(h)); }
+double or_max_divb(void* h) { Block& b = *static_cast<Block*>(h); return max_divb(b, b.U0); }
+
+// Copy out internal arrays: which = 0 F (3*NV*nc), 1 cmax (3*nc), 2 cmin (3*nc),
+// 3 dUdt (NV*nc), 4 pflag (nc), 5 fflag (nc), 6 emf (3*nc), 7 Pl, 8 Pr (NV*nc)
+void or_get_array(void* h, int which, double* out) {
+    Block& b = *static_cast<Block*>(h);
+    const std::vector<Real>* src = nullptr;
+    switch (which) {
+    case 0: src = &b.F; break; case 1: src = &b.cmax; break; case 2: src = &b.cmin; break;
+    case 3: src = &b.dUdt; break; case 4: src = &b.pflag; break; case 5: src = &b.fflag; break;
+    case 6: src = &b.emf; break; case 7: src = &b.Pl; break; case 8: src = &b.Pr; break;
+    default: return;
+    }
+    std::memcpy(out, src->data(), sizeof(double) * src->size());
+}
+void or_set_array(void* h, int which, const double* in) {
+    Block& b = *static_cast<Block*>(h);
+    std::vector<Real>* dst = nullptr;
+    switch (which) {
+    case 0: dst = &b.F; break; case 1: dst = &b.cmax; break; case 2: dst = &b.cmin; break;
+    case 3: dst = &b.dUdt; break; case 4: dst = &b.pflag; break; case 5: dst = &b.fflag; break;
+    default: return;
+    }
+    std::memcpy(dst->data(), in, sizeof(double) * dst->size());
+}
+
+// Scalar kernels exposed for closed-form tests of the oracle itself.
+void or_weno5(double x1, double x2, double x3, double x4, double x5, double* l, double* r) { weno5(x1, x2, x3, x4, x5, *l, *r); }
+void or_reconstruct(int recon, double x1, double x2, double x3, double x4, double x5, double* l, double* r) {
+    reconstruct(recon, x1, x2, x3, x4, x5, *l, *r);
+}
+int or_num_threads() {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+}  // extern "C"

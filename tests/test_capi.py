@@ -1,0 +1,75 @@
+"""Host-side checks of the C ABI that need no GPU: the library loads, exports
+every function include/kharma_amd.h declares, and rejects bad parameters
+before touching the device."""
+import ctypes
+import os
+
+import pytest
+
+import kharma_amd as kh
+from kharma_amd import problems as pr
+
+
+@pytest.fixture(scope="module")
+def L():
+    if not os.path.exists(kh.LIB_PATH):
+        pytest.fail("libkharma_amd.so not built; run __graft_entry__.build()")
+    return kh.lib()
+
+
+def test_exports_every_declared_symbol(L):
+    names = kh.header_functions()
+    assert len(names) >= 25
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_version(L):
+    assert L.kh_version() == 1
+
+
+def _create(cfg, **over):
+    p = kh.make_params(cfg)
+    for k, v in over.items():
+        setattr(p, k, v)
+    h = ctypes.c_void_p()
+    st = kh.lib().kh_block_create(ctypes.byref(p), ctypes.byref(h))
+    return st, kh.lib().kh_last_error().decode()
+
+
+@pytest.mark.parametrize("over,code", [
+    (dict(ng=2), 1),                 # too few ghosts for a 5-point stencil
+    (dict(nx1=0), 1),
+    (dict(recon=9), 1),
+    (dict(precision=3), 1),
+    (dict(coords=1), 3),             # not implemented -> unsupported, not a crash
+    (dict(periodic1=0), 3),
+    (dict(nx1=3), 1),                # periodic axis shorter than the ghost depth
+])
+def test_invalid_params_rejected_on_host(L, over, code):
+    st, msg = _create(pr.mhdmode_config(n=16, ndim=3), **over)
+    assert st == code and msg
+
+
+def test_null_arguments_rejected(L):
+    assert L.kh_get_flux(None, 1, None, None, None, None) == 1
+    assert L.kh_driver_step(None, 1) == 1
+    assert L.kh_block_create(None, None) == 1
+
+
+def test_header_params_struct_matches_binding():
+    # every field of kh_params appears, in order, in the ctypes mirror
+    import re
+    txt = open(kh.HEADER).read()
+    body = txt[txt.index("typedef struct kh_params {"):txt.index("} kh_params;")]
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    fields = []
+    for decl in body.split(";"):
+        decl = decl.strip()
+        if not decl or decl.startswith("typedef"):
+            decl = decl.split("{", 1)[-1].strip()
+            if not decl:
+                continue
+        names = decl.split(None, 1)[1]
+        fields += [n.strip() for n in names.split(",")]
+    assert fields == [f[0] for f in kh.KhParams._fields_]
