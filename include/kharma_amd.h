@@ -143,6 +143,9 @@ int kh_driver_set_stream(kh_driver* d, void* hip_stream);
  * block, dt = dt_start (KHARMA::PostInitialize, prob/post_initialize.cpp:55-193). */
 int kh_driver_init_prims(kh_driver* d, const double* P);
 int kh_driver_step(kh_driver* d, int nsteps);
+/* Set the clock as a restart does (Parthenon SimTime time / dt / ncycle restored,
+ * KHARMA::ResetGlobals, kharma.cpp:107-119): the next step takes dt. */
+int kh_driver_set_clock(kh_driver* d, double time, double dt, int ncycle);
 /* Copy the base-container state out (device pointers; either may be NULL). */
 int kh_driver_get_state(kh_driver* d, double* P, double* U);
 /* Synchronises; any output may be NULL. */

@@ -95,6 +95,7 @@ def lib():
         "kh_driver_set_stream": (i, [vp, vp]),
         "kh_driver_init_prims": (i, [vp, vp]),
         "kh_driver_step": (i, [vp, i]),
+        "kh_driver_set_clock": (i, [vp, d, d, i]),
         "kh_driver_get_state": (i, [vp, vp, vp]),
         "kh_driver_get_scalars": (i, [vp, ctypes.POINTER(d), ctypes.POINTER(d), ctypes.POINTER(d),
                                       ctypes.POINTER(i), ctypes.POINTER(i)]),
@@ -295,6 +296,10 @@ class Driver:
     def step(self, n=1):
         self._bind()
         _check(lib().kh_driver_step(self.h, n))
+
+    def set_clock(self, time, dt, ncycle=0):
+        self._bind()
+        _check(lib().kh_driver_set_clock(self.h, time, dt, ncycle))
 
     def state(self):
         torch = _torch()

@@ -5,6 +5,7 @@
 // No exception crosses the ABI; every HIP status is checked.
 #include "../../include/kharma_amd.h"
 
+#include <algorithm>
 #include <cfloat>
 #include <cstdio>
 #include <cstring>
@@ -38,14 +39,18 @@ struct kh_block {
     StepConsts consts;
     hipStream_t stream = nullptr;
     DeviceState* scratch = nullptr;  // reductions of the task-level entry points
+    double* partials = nullptr;      // one value per work-group of a reduction
+    long npartials = 0;
 };
 
 struct kh_driver {
     kh_block* blk = nullptr;
     double *U0 = nullptr, *P0 = nullptr, *U1 = nullptr, *P1 = nullptr;
-    void *F = nullptr, *cmax = nullptr, *cmin = nullptr, *emf = nullptr;
+    float *P0f = nullptr, *P1f = nullptr;   // FP32 shadows of P (split precision only)
+    void *F = nullptr, *ctop = nullptr;
     int *pflag = nullptr, *fflag = nullptr, *fail_list = nullptr;
     DeviceState* ds = nullptr;
+    double *dt_part = nullptr, *divb_part = nullptr;
     double* dt_hist = nullptr;
     int hist_cap = 0;
     // kernel timing (kh_driver_set_kernel_timing)
@@ -77,8 +82,8 @@ int validate(const kh_params* p) {
     if (p->coords != 0) return fail(KH_ERR_UNSUPPORTED, "only cartesian_minkowski coordinates are implemented");
     if (!p->periodic1 || (p->nx2 > 1 && !p->periodic2) || (p->nx3 > 1 && !p->periodic3))
         return fail(KH_ERR_UNSUPPORTED, "only periodic boundaries are implemented");
-    if (p->nx1 < p->ng || (p->nx2 > 1 && p->nx2 < p->ng) || (p->nx3 > 1 && p->nx3 < p->ng))
-        return fail(KH_ERR_INVALID, "periodic dimensions need at least ng interior zones");
+    if (p->nx1 < 2 * p->ng || (p->nx2 > 1 && p->nx2 < 2 * p->ng) || (p->nx3 > 1 && p->nx3 < 2 * p->ng))
+        return fail(KH_ERR_INVALID, "periodic dimensions need at least 2*ng interior zones");
     if (p->recon < 0 || p->recon > 3) return fail(KH_ERR_INVALID, "unknown reconstruction");
     if (p->precision != KH_PREC_FP64 && p->precision != KH_PREC_SPLIT) return fail(KH_ERR_INVALID, "unknown precision");
     if (p->integrator != 0 && p->integrator != 1) return fail(KH_ERR_INVALID, "unknown integrator");
@@ -150,8 +155,14 @@ int kh_block_create(const kh_params* p, kh_block** out) {
     b->p = *p;
     fill_dims(p, b->dims);
     fill_consts(p, b->consts);
+    b->npartials = std::max(update_blocks(b->dims), divb_blocks(b->dims)) + 1;
     hipError_t e = hipMalloc(&b->scratch, sizeof(DeviceState));
-    if (e != hipSuccess) { delete b; return fail(KH_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e)); }
+    if (e == hipSuccess) e = hipMalloc(&b->partials, sizeof(double) * b->npartials);
+    if (e != hipSuccess) {
+        if (b->scratch) (void)hipFree(b->scratch);
+        delete b;
+        return fail(KH_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
     *out = b;
     return KH_OK;
 }
@@ -159,6 +170,7 @@ int kh_block_create(const kh_params* p, kh_block** out) {
 void kh_block_destroy(kh_block* b) {
     if (!b) return;
     if (b->scratch) (void)hipFree(b->scratch);
+    if (b->partials) (void)hipFree(b->partials);
     delete b;
 }
 
@@ -236,13 +248,11 @@ int kh_sync_bounds(kh_block* b, double* A, int nvar) {
 
 int kh_estimate_timestep(kh_block* b, const void* cmax, const void* cmin, double dt_last, double* ndt) {
     if (!b || !cmax || !cmin || !ndt) return fail(KH_ERR_INVALID, "null argument");
-    KH_HIP(launch_reset_state(b->scratch, 0., b->stream));
-    KH_HIP(launch_estimate_dt(b->dims, b->consts, b->p.precision, cmax, cmin, b->scratch, b->stream));
-    DeviceState h;
-    KH_HIP(hipMemcpyAsync(&h, b->scratch, sizeof(h), hipMemcpyDeviceToHost, b->stream));
+    KH_HIP(launch_estimate_dt(b->dims, b->consts, b->p.precision, cmax, cmin, b->partials, b->stream));
+    KH_HIP(launch_reduce(b->partials, (int)update_blocks(b->dims), 0, &b->scratch->last_ndt, b->stream));
+    double m = 0.;
+    KH_HIP(hipMemcpyAsync(&m, &b->scratch->last_ndt, sizeof(double), hipMemcpyDeviceToHost, b->stream));
     KH_HIP(hipStreamSynchronize(b->stream));
-    double m;
-    std::memcpy(&m, &h.min_ndt_bits, sizeof(m));
     const double hi = b->p.max_dt_increase * dt_last;
     double v = m * b->p.cfl;
     v = (v < b->p.dt_min) ? b->p.dt_min : v;   // min(max(lower, n), upper)
@@ -253,12 +263,11 @@ int kh_estimate_timestep(kh_block* b, const void* cmax, const void* cmin, double
 
 int kh_max_divb(kh_block* b, const double* U, double* out) {
     if (!b || !U || !out) return fail(KH_ERR_INVALID, "null argument");
-    KH_HIP(launch_reset_state(b->scratch, 0., b->stream));
-    KH_HIP(launch_max_divb(b->dims, U, b->scratch, b->stream));
-    DeviceState h;
-    KH_HIP(hipMemcpyAsync(&h, b->scratch, sizeof(h), hipMemcpyDeviceToHost, b->stream));
+    if (b->dims.ndim < 2) { *out = 0.; return KH_OK; }
+    KH_HIP(launch_max_divb(b->dims, U, b->partials, b->stream));
+    KH_HIP(launch_reduce(b->partials, (int)divb_blocks(b->dims), 1, &b->scratch->last_divb, b->stream));
+    KH_HIP(hipMemcpyAsync(out, &b->scratch->last_divb, sizeof(double), hipMemcpyDeviceToHost, b->stream));
     KH_HIP(hipStreamSynchronize(b->stream));
-    std::memcpy(out, &h.divb_bits, sizeof(double));
     return KH_OK;
 }
 
@@ -285,9 +294,13 @@ int kh_driver_create(const kh_params* p, kh_driver** out) {
     if ((e = hipMalloc(&d->U1, st_bytes)) != hipSuccess) return cleanup(e, "hipMalloc U1");
     if ((e = hipMalloc(&d->P1, st_bytes)) != hipSuccess) return cleanup(e, "hipMalloc P1");
     if ((e = hipMalloc(&d->F, fe * 3 * NV * nc)) != hipSuccess) return cleanup(e, "hipMalloc F");
-    if ((e = hipMalloc(&d->cmax, fe * 3 * nc)) != hipSuccess) return cleanup(e, "hipMalloc cmax");
-    if ((e = hipMalloc(&d->cmin, fe * 3 * nc)) != hipSuccess) return cleanup(e, "hipMalloc cmin");
-    if ((e = hipMalloc(&d->emf, fe * 3 * nc)) != hipSuccess) return cleanup(e, "hipMalloc emf");
+    if ((e = hipMalloc(&d->ctop, fe * 3 * nc)) != hipSuccess) return cleanup(e, "hipMalloc ctop");
+    if (blk->p.precision == KH_PREC_SPLIT) {
+        if ((e = hipMalloc(&d->P0f, sizeof(float) * NV * nc)) != hipSuccess) return cleanup(e, "hipMalloc P0f");
+        if ((e = hipMalloc(&d->P1f, sizeof(float) * NV * nc)) != hipSuccess) return cleanup(e, "hipMalloc P1f");
+    }
+    if ((e = hipMalloc(&d->dt_part, sizeof(double) * (update_blocks(blk->dims) + 1))) != hipSuccess) return cleanup(e, "hipMalloc dt_part");
+    if ((e = hipMalloc(&d->divb_part, sizeof(double) * (divb_blocks(blk->dims) + 1))) != hipSuccess) return cleanup(e, "hipMalloc divb_part");
     if ((e = hipMalloc(&d->pflag, sizeof(int) * nc)) != hipSuccess) return cleanup(e, "hipMalloc pflag");
     if ((e = hipMalloc(&d->fflag, sizeof(int) * nc)) != hipSuccess) return cleanup(e, "hipMalloc fflag");
     if ((e = hipMalloc(&d->fail_list, sizeof(int) * nc)) != hipSuccess) return cleanup(e, "hipMalloc fail_list");
@@ -301,9 +314,8 @@ int kh_driver_create(const kh_params* p, kh_driver** out) {
     if ((e = hipMemset(d->U1, 0, st_bytes)) != hipSuccess) return cleanup(e, "hipMemset");
     if ((e = hipMemset(d->P1, 0, st_bytes)) != hipSuccess) return cleanup(e, "hipMemset");
     if ((e = hipMemset(d->F, 0, fe * 3 * NV * nc)) != hipSuccess) return cleanup(e, "hipMemset");
-    if ((e = hipMemset(d->cmax, 0, fe * 3 * nc)) != hipSuccess) return cleanup(e, "hipMemset");
-    if ((e = hipMemset(d->cmin, 0, fe * 3 * nc)) != hipSuccess) return cleanup(e, "hipMemset");
-    if ((e = hipMemset(d->emf, 0, fe * 3 * nc)) != hipSuccess) return cleanup(e, "hipMemset");
+    if ((e = hipMemset(d->ctop, 0, fe * 3 * nc)) != hipSuccess) return cleanup(e, "hipMemset");
+    if ((e = hipMemset(d->divb_part, 0, sizeof(double) * (divb_blocks(blk->dims) + 1))) != hipSuccess) return cleanup(e, "hipMemset");
     if ((e = hipMemset(d->pflag, 0, sizeof(int) * nc)) != hipSuccess) return cleanup(e, "hipMemset");
     if ((e = hipMemset(d->fflag, 0, sizeof(int) * nc)) != hipSuccess) return cleanup(e, "hipMemset");
     if ((e = hipMemset(d->dt_hist, 0, sizeof(double) * d->hist_cap)) != hipSuccess) return cleanup(e, "hipMemset");
@@ -316,8 +328,9 @@ void kh_driver_destroy(kh_driver* d) {
     if (!d) return;
     (void)hipDeviceSynchronize();
     for (hipEvent_t e : d->ev_pool) (void)hipEventDestroy(e);
-    for (void* ptr : {(void*)d->U0, (void*)d->P0, (void*)d->U1, (void*)d->P1, d->F, d->cmax, d->cmin, d->emf,
-                      (void*)d->pflag, (void*)d->fflag, (void*)d->fail_list, (void*)d->ds, (void*)d->dt_hist})
+    for (void* ptr : {(void*)d->U0, (void*)d->P0, (void*)d->U1, (void*)d->P1, (void*)d->P0f, (void*)d->P1f, d->F, d->ctop,
+                      (void*)d->pflag, (void*)d->fflag, (void*)d->fail_list, (void*)d->ds, (void*)d->dt_hist,
+                      (void*)d->dt_part, (void*)d->divb_part})
         if (ptr) (void)hipFree(ptr);
     kh_block_destroy(d->blk);
     delete d;
@@ -351,6 +364,10 @@ int kh_driver_init_prims(kh_driver* d, const double* P) {
     KH_HIP(launch_sync(b->dims, d->U0, NV, b->stream));
     KH_HIP(hipMemcpyAsync(d->P1, d->P0, st_bytes, hipMemcpyDeviceToDevice, b->stream));
     KH_HIP(hipMemcpyAsync(d->U1, d->U0, st_bytes, hipMemcpyDeviceToDevice, b->stream));
+    if (d->P0f) {
+        KH_HIP(launch_convert_f32(d->P0, d->P0f, NV * b->dims.nc, b->stream));
+        KH_HIP(launch_convert_f32(d->P0, d->P1f, NV * b->dims.nc, b->stream));
+    }
     KH_HIP(launch_reset_state(d->ds, b->p.dt_start, b->stream));
     return KH_OK;
 }
@@ -376,25 +393,29 @@ int kh_driver_step(kh_driver* d, int nsteps) {
     const size_t fe = flux_elem(b);
     hipStream_t s = b->stream;
     char* F = (char*)d->F;
-    char* cmx = (char*)d->cmax;
-    char* cmn = (char*)d->cmin;
+    char* ct = (char*)d->ctop;
+    const int n_dt = (int)update_blocks(bd), n_divb = (int)divb_blocks(bd);
     for (int n = 0; n < nsteps; ++n) {
         for (int stage = 1; stage <= 2; ++stage) {
             const double* Psub = (stage == 1) ? d->P0 : d->P1;
+            const float* Psubf = (stage == 1) ? d->P0f : d->P1f;
             const double* Usub = (stage == 1) ? d->U0 : d->U1;
             double* Pout = (stage == 1) ? d->P1 : d->P0;
+            float* Poutf = (stage == 1) ? d->P1f : d->P0f;
             double* Uout = (stage == 1) ? d->U1 : d->U0;
+            // signal speeds only feed the timestep estimate of the last stage
+            const int speeds = (stage == 2) ? SPEEDS_CTOP : SPEEDS_NONE;
             for (int dir = 1; dir <= bd.ndim; ++dir)
-                KH_TIMED(dir - 1, launch_flux(bd, c, dir, b->p.recon, prec, Psub, F + fe * (dir - 1) * NV * bd.nc,
-                                              cmx + fe * (dir - 1) * bd.nc, cmn + fe * (dir - 1) * bd.nc, s));
-            KH_TIMED(3, launch_emf(bd, prec, d->F, d->emf, s));
-            KH_TIMED(3 + stage, launch_update(bd, c, prec, stage, stage == 2, d->F, d->emf, d->cmax, d->cmin,
-                                              Usub, d->U0, Psub, Uout, Pout, d->pflag, d->fflag, d->fail_list, d->ds, s));
+                KH_TIMED(dir - 1, launch_flux_stage(bd, c, dir, b->p.recon, prec, speeds, Psub, Psubf,
+                                                    F + fe * (dir - 1) * NV * bd.nc, ct + fe * (dir - 1) * bd.nc, s));
+            KH_TIMED(3 + stage, launch_update(bd, c, prec, stage, stage == 2, d->F, d->ctop, Usub, d->U0, Psub,
+                                              Uout, Pout, Poutf, d->pflag, d->fflag, d->fail_list, d->ds, d->dt_part, s));
             if (c.fix_average || c.fix_atmo)
-                KH_TIMED(6, launch_fixup(bd, c, stage, Uout, Pout, d->pflag, d->fail_list, d->ds, s));
+                KH_TIMED(6, launch_fixup(bd, c, stage, Uout, Pout, Poutf, d->pflag, d->fail_list, d->ds, s));
         }
-        KH_TIMED(7, launch_max_divb(bd, d->U0, d->ds, s));
-        KH_TIMED(8, launch_finalize(c, d->ds, d->dt_hist, d->hist_cap, s));
+        KH_TIMED(7, launch_max_divb(bd, d->U0, d->divb_part, s));
+        KH_TIMED(8, launch_finalize(c, d->ds, d->dt_part, n_dt, d->divb_part, bd.ndim > 1 ? n_divb : 0,
+                                    d->dt_hist, d->hist_cap, s));
     }
     return KH_OK;
 }
@@ -418,6 +439,12 @@ int kh_driver_get_scalars(kh_driver* d, double* time, double* dt, double* last_d
     if (last_divb) *last_divb = h.last_divb;
     if (ncycle) *ncycle = h.ncycle;
     if (total_fails) *total_fails = h.total_fails;
+    return KH_OK;
+}
+
+int kh_driver_set_clock(kh_driver* d, double time, double dt, int ncycle) {
+    if (!d || !(dt > 0.) || ncycle < 0) return fail(KH_ERR_INVALID, "bad argument");
+    KH_HIP(launch_set_clock(d->ds, time, dt, ncycle, d->blk->stream));
     return KH_OK;
 }
 

@@ -43,6 +43,72 @@ template <typename R> struct MinkowskiGeom {
     }
 };
 
+// Reciprocal from the hardware estimate plus Newton steps (~1 ulp; not the
+// correctly rounded IEEE quotient, which costs ~12 instructions in FP64).
+__device__ __forceinline__ double rcp(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+__device__ __forceinline__ float rcp(float x) {
+    float r = __builtin_amdgcn_rcpf(x);
+    const float e = fmaf(-x, r, 1.0f);
+    return fmaf(r, e, r);
+}
+
+// Both face values of one cell from its 5-point stencil (x3 is the cell):
+// lout at the left face, rout at the right face, WENO5 weights of
+// flux/reconstruction.hpp:128-204.  The normalised weights
+// (c_k / s_k) / sum_j (c_j / s_j), s_k = (EPS + beta_k)^2, are formed with one
+// reciprocal per side: in FP64 by multiplying through by s0 s1 s2 (s >= 1e-52,
+// so the products stay normal); in FP32 with r_k = min(t)/t_k in (0, 1],
+// which neither overflows nor lets the dominant weight underflow.
+__device__ __forceinline__ void weno5_cell(double x1, double x2, double x3, double x4, double x5,
+                                           double& lout, double& rout) {
+    constexpr double EPS = 1.e-26;
+    double c1 = x1 - 2. * x2 + x3, c2 = x1 - 4. * x2 + 3. * x3;
+    const double t0 = EPS + ((13. / 12.) * c1 * c1 + 0.25 * c2 * c2);
+    c1 = x2 - 2. * x3 + x4; c2 = x4 - x2;
+    const double t1 = EPS + ((13. / 12.) * c1 * c1 + 0.25 * c2 * c2);
+    c1 = x3 - 2. * x4 + x5; c2 = x5 - 4. * x4 + 3. * x3;
+    const double t2 = EPS + ((13. / 12.) * c1 * c1 + 0.25 * c2 * c2);
+    const double s0 = t0 * t0, s1 = t1 * t1, s2 = t2 * t2;
+    const double p12 = s1 * s2, p02 = s0 * s2, p01 = s0 * s1;
+    const double ar0 = (1. / 16.) * p12, ar1 = (5. / 8.) * p02, ar2 = (5. / 16.) * p01;
+    const double al0 = (1. / 16.) * p01, al1 = ar1, al2 = (5. / 16.) * p12;
+    const double iWr = rcp(ar0 + ar1 + ar2), iWl = rcp(al0 + al1 + al2);
+    rout = ((3. / 8.) * x1 - (5. / 4.) * x2 + (15. / 8.) * x3) * (ar0 * iWr) +
+           ((-1. / 8.) * x2 + (3. / 4.) * x3 + (3. / 8.) * x4) * (ar1 * iWr) +
+           ((3. / 8.) * x3 + (3. / 4.) * x4 - (1. / 8.) * x5) * (ar2 * iWr);
+    lout = ((3. / 8.) * x5 - (5. / 4.) * x4 + (15. / 8.) * x3) * (al0 * iWl) +
+           ((-1. / 8.) * x4 + (3. / 4.) * x3 + (3. / 8.) * x2) * (al1 * iWl) +
+           ((3. / 8.) * x3 + (3. / 4.) * x2 - (1. / 8.) * x1) * (al2 * iWl);
+}
+__device__ __forceinline__ void weno5_cell(float x1, float x2, float x3, float x4, float x5,
+                                           float& lout, float& rout) {
+    constexpr float EPS = 1.e-26f;
+    float c1 = x1 - 2.f * x2 + x3, c2 = x1 - 4.f * x2 + 3.f * x3;
+    const float t0 = EPS + ((13.f / 12.f) * c1 * c1 + 0.25f * c2 * c2);
+    c1 = x2 - 2.f * x3 + x4; c2 = x4 - x2;
+    const float t1 = EPS + ((13.f / 12.f) * c1 * c1 + 0.25f * c2 * c2);
+    c1 = x3 - 2.f * x4 + x5; c2 = x5 - 4.f * x4 + 3.f * x3;
+    const float t2 = EPS + ((13.f / 12.f) * c1 * c1 + 0.25f * c2 * c2);
+    const float tm = fminf(fminf(t0, t1), t2);
+    const float r0 = tm * rcp(t0), r1 = tm * rcp(t1), r2 = tm * rcp(t2);
+    const float q0 = r0 * r0, q1 = r1 * r1, q2 = r2 * r2;
+    const float ar0 = (1.f / 16.f) * q0, ar1 = (5.f / 8.f) * q1, ar2 = (5.f / 16.f) * q2;
+    const float al0 = (1.f / 16.f) * q2, al2 = (5.f / 16.f) * q0;
+    const float iWr = rcp(ar0 + ar1 + ar2), iWl = rcp(al0 + ar1 + al2);
+    rout = ((3.f / 8.f) * x1 - (5.f / 4.f) * x2 + (15.f / 8.f) * x3) * (ar0 * iWr) +
+           ((-1.f / 8.f) * x2 + (3.f / 4.f) * x3 + (3.f / 8.f) * x4) * (ar1 * iWr) +
+           ((3.f / 8.f) * x3 + (3.f / 4.f) * x4 - (1.f / 8.f) * x5) * (ar2 * iWr);
+    lout = ((3.f / 8.f) * x5 - (5.f / 4.f) * x4 + (15.f / 8.f) * x3) * (al0 * iWl) +
+           ((-1.f / 8.f) * x4 + (3.f / 4.f) * x3 + (3.f / 8.f) * x2) * (ar1 * iWl) +
+           ((3.f / 8.f) * x3 + (3.f / 4.f) * x2 - (1.f / 8.f) * x1) * (al2 * iWl);
+}
+
 // ------------------------------------------------------------ reconstruction
 // WENO5 of Tchekhovskoy+07 / Shu 2011 as in flux/reconstruction.hpp:128-204.
 // The nonlinear weights (c_i / (EPS+beta_i)^2) / sum are evaluated in the
@@ -143,7 +209,7 @@ __device__ __forceinline__ void calc_4vecs(const G& g, const R P[NV], FourVec<R>
     }
     g.lower(D.ucon, D.ucov);
     D.bcon[0] = P[B1] * D.ucov[1] + P[B2] * D.ucov[2] + P[B3] * D.ucov[3];
-    const R iu0 = R(1) / D.ucon[0];
+    const R iu0 = rcp(D.ucon[0]);
     for (int v = 0; v < 3; ++v) D.bcon[v + 1] = (P[B1 + v] + D.bcon[0] * D.ucon[v + 1]) * iu0;
     g.lower(D.bcon, D.bcov);
 }
@@ -189,9 +255,9 @@ __device__ __forceinline__ void prim_to_u_and_flux(const G& g, const R P[NV], co
 template <typename R, class G, int DIR>
 __device__ __forceinline__ void vchar(const G& g, const R P[NV], const FourVec<R>& D, R gam, R& cmax, R& cmin) {
     const R ef = P[RHO] + gam * P[UU];
-    const R cs2 = gam * (gam - R(1)) * P[UU] / ef;
+    const R cs2 = gam * (gam - R(1)) * P[UU] * rcp(ef);
     const R bsq = dot4(D.bcon, D.bcov);
-    const R va2 = bsq / (bsq + ef);
+    const R va2 = bsq * rcp(bsq + ef);
     const R cms2 = cs2 + va2 - cs2 * va2;
     R A, B, C;
     if constexpr (G::flat) {
@@ -207,7 +273,7 @@ __device__ __forceinline__ void vchar(const G& g, const R P[NV], const FourVec<R
         C = Au * Au - (Asq + Au * Au) * cms2;
     }
     const R discr = sqrt(smax(B * B - R(4) * A * C, R(0)));
-    const R i2A = R(1) / (R(2) * A);
+    const R i2A = rcp(R(2) * A);
     const R vp = -(-B + discr) * i2A;
     const R vm = -(-B - discr) * i2A;
     cmax = smax(vp, vm);
@@ -215,26 +281,30 @@ __device__ __forceinline__ void vchar(const G& g, const R P[NV], const FourVec<R
 }
 
 // ----------------------------------------------------------- inversion (FP64)
-// Inverter::u_to_p<onedw>, inverter/onedw.hpp:49-233.  Always FP64.
+// Inverter::u_to_p<onedw>, inverter/onedw.hpp:49-233.  Always FP64.  Quotients
+// are formed as products with reciprocals (~1 ulp), and the relative
+// convergence tests |x / Wp| < tol as |x| < tol |Wp|.
 __device__ __forceinline__ double lorentz_calc_w(double Bsq, double D, double QdB, double Qtsq, double Wp) {
     const double QdBsq = QdB * QdB;
     const double W = Wp + D;
     const double W2 = W * W;
     const double WB = W + Bsq;
-    const double utsq = -((W + WB) * QdBsq + W2 * Qtsq) / (QdBsq * (W + WB) + W2 * (Qtsq - WB * WB));
+    const double utsq = -((W + WB) * QdBsq + W2 * Qtsq) * rcp(QdBsq * (W + WB) + W2 * (Qtsq - WB * WB));
     if (utsq < -1.e-15 || utsq > 1.e7) return -1.;
     return sqrt(1. + fabs(utsq));
 }
-__device__ __forceinline__ double err_eqn(double gam, double Bsq, double D, double Ep, double QdB, double Qtsq,
+// gm1g = (gam - 1) / gam
+__device__ __forceinline__ double err_eqn(double gm1g, double Bsq, double D, double Ep, double QdB, double Qtsq,
                                           double Wp, int& eflag) {
     const double W = Wp + D;
     const double gamma = lorentz_calc_w(Bsq, D, QdB, Qtsq, Wp);
     if (gamma < 1) eflag = S_BAD_UT;
-    const double w = W / (gamma * gamma);
-    const double rho = D / gamma;
-    const double p = (w - rho) * (gam - 1) / gam;
+    const double ig = rcp(gamma);
+    const double w = W * (ig * ig);
+    const double rho = D * ig;
+    const double p = (w - rho) * gm1g;
     const double bw = Bsq + W;
-    return -Ep + Wp - p + 0.5 * Bsq + 0.5 * (Bsq * Qtsq - QdB * QdB) / (bw * bw);
+    return -Ep + Wp - p + 0.5 * Bsq + 0.5 * (Bsq * Qtsq - QdB * QdB) * rcp(bw * bw);
 }
 // U: 8 conserved at the point; P: in = guess (rho,u,u1..3), out = recovered (only on success)
 template <class G>
@@ -244,6 +314,7 @@ __device__ __forceinline__ int u_to_p_onedw(const G& g, const double U[NV], doub
     double alpha, a_over_g;
     if constexpr (G::flat) { alpha = 1.; a_over_g = 1.; }
     else { alpha = g.alpha(); a_over_g = alpha / g.gdet(); }
+    const double gm1g = (gam - 1) / gam;
     const double D = U[RHO] * a_over_g;
     double Bcon[4] = {0., U[B1] * a_over_g, U[B2] * a_over_g, U[B3] * a_over_g};
     const double Qcov[4] = {(U[UU] - U[RHO]) * a_over_g, U[U1] * a_over_g, U[U2] * a_over_g, U[U3] * a_over_g};
@@ -267,59 +338,61 @@ __device__ __forceinline__ int u_to_p_onedw(const G& g, const double U[NV], doub
         if (gamma < 1) return S_BAD_UT;
         const double rho = P[RHO], u = P[UU];
         Wp = (rho + u + (gam - 1) * u) * gamma * gamma - rho * gamma;
-        err = err_eqn(gam, Bsq, D, Ep, QdB, Qtsq, Wp, eflag);
+        err = err_eqn(gm1g, Bsq, D, Ep, QdB, Qtsq, Wp, eflag);
     }
     double dW;
     {
         const double Wpm = (1. - 1.e-5) * Wp;
         const double h = Wp - Wpm;
         const double Wpp = Wp + h;
-        const double errm = err_eqn(gam, Bsq, D, Ep, QdB, Qtsq, Wpm, eflag);
-        const double errp = err_eqn(gam, Bsq, D, Ep, QdB, Qtsq, Wpp, eflag);
-        const double dedW = (errp - errm) / (Wpp - Wpm);
-        const double dedW2 = (errp - 2. * err + errm) / (h * h);
-        const double f = clip(0.5 * err * dedW2 / (dedW * dedW), -0.3, 0.3);
-        dW = clip(-err / dedW / (1. - f), -0.5 * Wp, 2.0 * Wp);
+        const double errm = err_eqn(gm1g, Bsq, D, Ep, QdB, Qtsq, Wpm, eflag);
+        const double errp = err_eqn(gm1g, Bsq, D, Ep, QdB, Qtsq, Wpp, eflag);
+        const double dedW = (errp - errm) * rcp(Wpp - Wpm);
+        const double dedW2 = (errp - 2. * err + errm) * rcp(h * h);
+        const double f = clip(0.5 * err * dedW2 * rcp(dedW * dedW), -0.3, 0.3);
+        dW = clip(-err * rcp(dedW * (1. - f)), -0.5 * Wp, 2.0 * Wp);
     }
     double Wp1 = Wp;
     double err1 = err;
     Wp += dW;
-    err = err_eqn(gam, Bsq, D, Ep, QdB, Qtsq, Wp, eflag);
+    err = err_eqn(gm1g, Bsq, D, Ep, QdB, Qtsq, Wp, eflag);
     int iter = 0;
     for (iter = 0; iter < max_iterations; iter++) {
-        dW = clip((Wp1 - Wp) * err / (err - err1), -0.5 * Wp, 2.0 * Wp);
+        dW = clip((Wp1 - Wp) * err * rcp(err - err1), -0.5 * Wp, 2.0 * Wp);
         Wp1 = Wp;
         err1 = err;
         Wp += dW;
-        if (fabs(dW / Wp) < tol) break;
-        err = err_eqn(gam, Bsq, D, Ep, QdB, Qtsq, Wp, eflag);
-        if (fabs(err / Wp) < tol) break;
+        if (fabs(dW) < tol * fabs(Wp)) break;
+        err = err_eqn(gm1g, Bsq, D, Ep, QdB, Qtsq, Wp, eflag);
+        if (fabs(err) < tol * fabs(Wp)) break;
     }
     if (iter == max_iterations) return S_MAX_ITER;
     const double gamma = lorentz_calc_w(Bsq, D, QdB, Qtsq, Wp);
     if (gamma < 1) return S_BAD_UT;
-    const double rho = D / gamma;
+    const double ig = rcp(gamma);
+    const double rho = D * ig;
     const double W = Wp + D;
-    const double w = W / (gamma * gamma);
-    const double p = (w - rho) * (gam - 1) / gam;
+    const double w = W * (ig * ig);
+    const double p = (w - rho) * gm1g;
     const double u = w - (rho + p);
     if (rho < 0 && u < 0) return S_NEG_RHOU;
     else if (rho < 0) return S_NEG_RHO;
     else if (u < 0) return S_NEG_U;
     P[RHO] = rho;
     P[UU] = u;
-    const double pre = (gamma / (W + Bsq));
-    P[U1] = pre * (Qtcon[1] + QdB * Bcon[1] / W);
-    P[U2] = pre * (Qtcon[2] + QdB * Bcon[2] / W);
-    P[U3] = pre * (Qtcon[3] + QdB * Bcon[3] / W);
+    const double pre = gamma * rcp(W + Bsq);
+    const double qw = QdB * rcp(W);
+    P[U1] = pre * (Qtcon[1] + qw * Bcon[1]);
+    P[U2] = pre * (Qtcon[2] + qw * Bcon[2]);
+    P[U3] = pre * (Qtcon[3] + qw * Bcon[3]);
     return S_SUCCESS;
 }
 
-// Conserved variables of all 8 fields (Flux::p_to_u, flux_functions.hpp:343-352)
+// Conserved variables of all 8 fields from P and its four-vectors D
+// (Flux::p_to_u, flux_functions.hpp:343-352).
 template <class G>
-__device__ __forceinline__ void p_to_u(const G& g, const double P[NV], double gam, double U[NV]) {
-    FourVec<double> D;
-    calc_4vecs<double>(g, P, D);
+__device__ __forceinline__ void p_to_u_D(const G& g, const double P[NV], const FourVec<double>& D, double gam,
+                                         double U[NV]) {
     const double gdet = g.gdet();
     const double bsq = dot4(D.bcon, D.bcov);
     const double pgas = (gam - 1.) * P[UU];
@@ -334,6 +407,34 @@ __device__ __forceinline__ void p_to_u(const G& g, const double P[NV], double ga
     U[B1] = P[B1] * gdet; U[B2] = P[B2] * gdet; U[B3] = P[B3] * gdet;
 }
 
+// (gam - 1) u / rho^gam > ktot_max (the entropy ceiling test of
+// floors_functions.hpp:147-148), decided in log2 space from hardware log2
+// estimates and falling back to the exact pow() only within 1e-3 of the
+// threshold, so the outcome equals the exact comparison.
+__device__ __forceinline__ double log2_est(double x) {
+    int e;
+    const double m = frexp(x, &e);   // x = m 2^e, m in [0.5, 1)
+    return (double)e + (double)__builtin_amdgcn_logf((float)m);
+}
+__device__ __forceinline__ bool ktot_exceeds(double u, double rho, double gam, double ktot_max) {
+    const double a = (gam - 1.) * u;
+    if (a > 0. && rho > 0. && ktot_max > 0. && a < 1e300 && rho < 1e300 && ktot_max < 1e300 &&
+        a > 1e-300 && rho > 1e-300) {
+        const double d = log2_est(a) - gam * log2_est(rho) - log2_est(ktot_max);
+        if (d > 1e-3) return true;
+        if (d < -1e-3) return false;
+    }
+    return a / pow(rho, gam) > ktot_max;
+}
+
+// Conserved variables of all 8 fields (Flux::p_to_u, flux_functions.hpp:343-352)
+template <class G>
+__device__ __forceinline__ void p_to_u(const G& g, const double P[NV], double gam, double U[NV]) {
+    FourVec<double> D;
+    calc_4vecs<double>(g, P, D);
+    p_to_u_D(g, P, D, gam, U);
+}
+
 // ------------------------------------------------------------------- floors
 struct FloorParams {
     double rho_min_geom, u_min_geom, rho_min_const, u_min_const, r_char;
@@ -342,14 +443,12 @@ struct FloorParams {
 };
 
 // Floors::determine_floors, floors/floors_functions.hpp:84-160 (non-spherical geometric floors)
+// bsq = b^mu b_mu of the zone's current state.
 template <class G>
-__device__ __forceinline__ int determine_floors(const G& g, const double P[NV], double gam, const FloorParams& f,
-                                                double& rhoflr_max, double& uflr_max) {
+__device__ __forceinline__ int determine_floors(const G& g, const double P[NV], double bsq, double gam,
+                                                const FloorParams& f, double& rhoflr_max, double& uflr_max) {
     const double rhoflr_geom = f.rho_min_const;
     const double uflr_geom = f.u_min_const;
-    FourVec<double> D;
-    calc_4vecs<double>(g, P, D);
-    const double bsq = dot4(D.bcon, D.bcov);
     const double rhoflr_b = bsq / f.bsq_over_rho_max;
     const double uflr_b = bsq / f.bsq_over_u_max;
     uflr_max = smax(uflr_geom, uflr_b);
@@ -369,7 +468,7 @@ __device__ __forceinline__ int determine_floors(const G& g, const double P[NV], 
         fflag |= (uflr_b > u) * F_B_U;
     }
     if (g.lorentz(P[U1], P[U2], P[U3]) > f.gamma_max) fflag |= F_GAMMA;
-    if ((gam - 1.) * P[UU] / pow(P[RHO], gam) > f.ktot_max) fflag |= F_KTOT;
+    if (ktot_exceeds(P[UU], P[RHO], gam, f.ktot_max)) fflag |= F_KTOT;
     if (f.temp_adjust_u && (P[UU] / P[RHO] > f.u_over_rho_max)) fflag |= F_TEMP;
     return fflag;
 }
@@ -449,11 +548,12 @@ __device__ __forceinline__ void apply_ceilings(const G& g, double P[NV], double 
 
 // One zone of Floors::ApplyFloorsInFrame (floors_impl.hpp:45-128): determine,
 // apply in the configured frame, ceilings.  Returns fflag; may overwrite pflag.
+// D holds the four-vectors of P on entry and on return (recomputed if P changed).
 template <class G>
-__device__ __forceinline__ int floors_zone(const G& g, double P[NV], double U[NV], double gam, const FloorParams& f,
-                                           int fflag_in, int& pflag) {
+__device__ __forceinline__ int floors_zone(const G& g, double P[NV], double U[NV], FourVec<double>& D, double gam,
+                                           const FloorParams& f, int fflag_in, int& pflag) {
     double rf, uf;
-    const int fflag = fflag_in | determine_floors(g, P, gam, f, rf, uf);
+    const int fflag = fflag_in | determine_floors(g, P, dot4(D.bcon, D.bcov), gam, f, rf, uf);
     if (fflag) {
         int pflag_l = 0;
         if (f.frame == 0) {
@@ -466,6 +566,7 @@ __device__ __forceinline__ int floors_zone(const G& g, double P[NV], double U[NV
         }
         if (pflag_l) pflag = pflag_l;
         apply_ceilings(g, P, gam, f);
+        calc_4vecs<double>(g, P, D);
     }
     return fflag;
 }

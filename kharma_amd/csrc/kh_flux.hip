@@ -1,49 +1,51 @@
-// kh_flux.hip -- fused face-flux kernel: reconstruction + reconstruction floors
+// kh_flux.hip -- fused face-flux kernels: reconstruction + reconstruction floors
 // + prim_to_flux + signal speeds + LLF/HLLE in one pass per direction.
 //
 // Replaces the four kernels of Flux::GetFlux<Recon,dir> (flux/get_flux.hpp:60-373:
 // calc_flux_recon, calc_flux_left, calc_flux_right, flux_llf), which stage
-// Pl/Pr/Ul/Ur/Fl/Fr through HBM scratch.  Here each work-item owns one face:
-// it reads the 6-point stencil of the 8 primitives along `dir` (coalesced
-// across the wavefront along x1), keeps both face states in registers and
-// writes only the 8 fluxes and the two signal speeds.
+// Pl/Pr/Ul/Ur/Fl/Fr through HBM scratch.  Here each work-item reconstructs ONE
+// cell (both of its face values, sharing the smoothness indicators), publishes
+// the right-face value through LDS, and then computes the flux through its
+// cell's left face from its own left-face value and its neighbour's right-face
+// value.  Only the 8 fluxes and 2 signal speeds per face reach HBM.
+//
+// X1: 256-thread blocks walk the cells of consecutive x1 rows (flattened,
+// consecutive blocks overlap by one cell); loads and stores are contiguous.
+// X2/X3: 64 x 8 blocks; x is a flattened transverse index with x1 fastest (so
+// a wavefront reads 64 consecutive doubles), y walks 8 cells along the face
+// normal, consecutive tiles overlapping by one cell.
 #include "kh_internal.hpp"
 
 namespace kh {
 
-template <int DIR, int RECON, typename R>
-__global__ void __launch_bounds__(256)
-flux_kernel(BlockDims b, StepConsts c, const double* __restrict__ P, R* __restrict__ F,
-            R* __restrict__ cmax_out, R* __restrict__ cmin_out,
-            int i0, int ni, int j0, int nj, int k0, int nk) {
-    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
-    const unsigned nij = (unsigned)ni * (unsigned)nj;
-    if (t >= nij * (unsigned)nk) return;
-    const unsigned kk = t / nij;
-    const unsigned rem = t - kk * nij;
-    const unsigned jj = rem / (unsigned)ni;
-    const int i = i0 + (int)(rem - jj * (unsigned)ni);
-    const int j = j0 + (int)jj;
-    const int k = k0 + (int)kk;
-    const long ci = (long)k * b.s3 + (long)j * b.s2 + i;
-    const long st = (DIR == 1) ? 1 : ((DIR == 2) ? b.s2 : b.s3);
-
-    const R gam = R(c.gam);
-    R Pl[NV], Pr[NV];
-#pragma unroll
-    for (int v = 0; v < NV; ++v) {
-        const double* q = P + v * b.nc + ci;
-        R x[6];
-#pragma unroll
-        for (int o = 0; o < 6; ++o) x[o] = R(q[(o - 3) * st]);
-        face_states<RECON, R>(x, Pl[v], Pr[v]);
+template <int RECON, typename R>
+__device__ __forceinline__ void cell_states(R x1, R x2, R x3, R x4, R x5, R& lout, R& rout) {
+    if constexpr (RECON == RECON_WENO5) {
+        weno5_cell(x1, x2, x3, x4, x5, lout, rout);
+    } else if constexpr (RECON == RECON_LINEAR_MC) {
+        const R dq = mc_lim(x3 - x2, x4 - x3) * (x4 - x3);
+        rout = x3 + R(0.5) * dq;
+        lout = x3 - R(0.5) * dq;
+    } else if constexpr (RECON == RECON_PPM) {
+        ppm(x1, x2, x3, x4, x5, lout, rout);
+    } else {
+        lout = x3;
+        rout = x3;
     }
+}
+
+// Flux through one face from its two states (get_flux.hpp:237-367).
+// SPEEDS: SPEEDS_BOTH writes cmax and cmin (Flux.cmax/cmin), SPEEDS_CTOP writes
+// max(cmax, cmin) (all the timestep uses) into cmax_out, SPEEDS_NONE writes none.
+template <int DIR, int SPEEDS, typename R>
+__device__ __forceinline__ void face_flux(const StepConsts& c, const BlockDims& b, R Pl[NV], R Pr[NV], long ci,
+                                          R* __restrict__ F, R* __restrict__ cmax_out, R* __restrict__ cmin_out) {
+    const R gam = R(c.gam);
     // get_flux.hpp:170-182 (Cartesian geometric floors on both face states)
     if (c.recon_floors) {
         apply_geo_floors<R>(Pl, R(c.floors.rho_min_const), R(c.floors.u_min_const));
         apply_geo_floors<R>(Pr, R(c.floors.rho_min_const), R(c.floors.u_min_const));
     }
-
     MinkowskiGeom<R> g;
     R Ul[NV], Fl[NV], Ur[NV], Fr[NV];
     R cmaxL, cminL, cmaxR, cminR;
@@ -62,10 +64,14 @@ flux_kernel(BlockDims b, StepConsts c, const double* __restrict__ P, R* __restri
     // get_flux.hpp:275-276 and 331-332: cmin is stored as a positive speed
     const R cmx = smax(smax(R(0), cmaxL), cmaxR);
     const R cmn = -smin(smin(R(0), cminL), cminR);
-    cmax_out[ci] = cmx;
-    cmin_out[ci] = cmn;
+    if constexpr (SPEEDS == SPEEDS_BOTH) {
+        cmax_out[ci] = cmx;
+        cmin_out[ci] = cmn;
+    } else if constexpr (SPEEDS == SPEEDS_CTOP) {
+        cmax_out[ci] = smax(cmx, cmn);
+    }
     if (c.use_hlle) {
-        const R inv = R(1) / (cmx + cmn);
+        const R inv = rcp(cmx + cmn);
 #pragma unroll
         for (int v = 0; v < NV; ++v)
             F[v * b.nc + ci] = (cmx * Fl[v] + cmn * Fr[v] - cmx * cmn * (Ur[v] - Ul[v])) * inv;
@@ -77,49 +83,143 @@ flux_kernel(BlockDims b, StepConsts c, const double* __restrict__ P, R* __restri
     }
 }
 
-template <int DIR, typename R>
-static hipError_t launch_dir_prec(const BlockDims& b, const StepConsts& c, int recon, const double* P, R* F,
-                                  R* cmax, R* cmin, hipStream_t s) {
-    // Face range consumed by flux-CT and the divergence (see header).
-    int i0 = b.is, i1 = b.ie, j0 = b.js, j1 = b.je, k0 = b.ks, k1 = b.ke;
-    if (DIR == 1) { i1 += 1; } else { i0 -= 1; i1 += 1; }
-    if (b.ndim > 1) { if (DIR == 2) { j1 += 1; } else { j0 -= 1; j1 += 1; } }
-    if (b.ndim > 2) { if (DIR == 3) { k1 += 1; } else { k0 -= 1; k1 += 1; } }
-    const int ni = i1 - i0 + 1, nj = j1 - j0 + 1, nk = k1 - k0 + 1;
-    const long n = (long)ni * nj * nk;
-    const int bs = 256;
-    const dim3 grid((unsigned)((n + bs - 1) / bs));
-    switch (recon) {
-    case RECON_DONOR:
-        hipLaunchKernelGGL((flux_kernel<DIR, RECON_DONOR, R>), grid, dim3(bs), 0, s, b, c, P, F, cmax, cmin, i0, ni, j0, nj, k0, nk);
-        break;
-    case RECON_LINEAR_MC:
-        hipLaunchKernelGGL((flux_kernel<DIR, RECON_LINEAR_MC, R>), grid, dim3(bs), 0, s, b, c, P, F, cmax, cmin, i0, ni, j0, nj, k0, nk);
-        break;
-    case RECON_PPM:
-        hipLaunchKernelGGL((flux_kernel<DIR, RECON_PPM, R>), grid, dim3(bs), 0, s, b, c, P, F, cmax, cmin, i0, ni, j0, nj, k0, nk);
-        break;
-    default:
-        hipLaunchKernelGGL((flux_kernel<DIR, RECON_WENO5, R>), grid, dim3(bs), 0, s, b, c, P, F, cmax, cmin, i0, ni, j0, nj, k0, nk);
-        break;
+// X1 faces i in [is, ie+1] of rows j in [j0, j0+nj), k in [k0, k0+nk).
+// Cells i in [is-1, ie+1] (L per row); block b handles flattened cells
+// [255 b, 255 b + 256) and emits the faces of all but its first cell.
+template <int RECON, int SPEEDS, typename R, typename PT>
+__global__ void __launch_bounds__(256)
+flux_x1_kernel(BlockDims b, StepConsts c, const PT* __restrict__ P, R* __restrict__ F,
+               R* __restrict__ cmax_out, R* __restrict__ cmin_out, int j0, int nj, int k0, int nk) {
+    __shared__ R sr[NV][256];
+    const int L = b.nx1 + 2;
+    const int tid = threadIdx.x;
+    const long g = (long)blockIdx.x * 255 + tid;
+    const bool valid = g < (long)L * nj * nk;
+    int ii = 0;
+    long ci = 0;
+    R lo[NV];
+    if (valid) {
+        const int row = (int)(g / L);
+        ii = (int)(g - (long)row * L);
+        const int jj = row % nj, kk = row / nj;
+        ci = (long)(k0 + kk) * b.s3 + (long)(j0 + jj) * b.s2 + (b.is - 1 + ii);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const PT* q = P + v * b.nc + ci;
+            R ro;
+            cell_states<RECON, R>(R(q[-2]), R(q[-1]), R(q[0]), R(q[1]), R(q[2]), lo[v], ro);
+            sr[v][tid] = ro;
+        }
+    }
+    __syncthreads();
+    if (!valid || tid == 0 || ii == 0) return;
+    R Pl[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) Pl[v] = sr[v][tid - 1];
+    face_flux<1, SPEEDS, R>(c, b, Pl, lo, ci, F, cmax_out, cmin_out);
+}
+
+// X2 / X3 faces.  Transverse columns: flattened (outer, i) with i in [is-1, ie+1]
+// fastest; outer = k (X2) or j (X3) from o0.  Along the normal: faces s in
+// [s0, s1]; tile y covers cells s0-1+7y .. s0+6+7y.
+template <int DIR, int RECON, int SPEEDS, typename R, typename PT>
+__global__ void __launch_bounds__(512)
+flux_xt_kernel(BlockDims b, StepConsts c, const PT* __restrict__ P, R* __restrict__ F,
+               R* __restrict__ cmax_out, R* __restrict__ cmin_out, int o0, int no, int s0, int s1) {
+    __shared__ R sr[NV][8][64];
+    const int ni = b.nx1 + 2;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int col = blockIdx.x * 64 + tx;
+    const int s = s0 - 1 + (int)blockIdx.y * 7 + ty;
+    const bool valid = col < ni * no && s <= s1;
+    const long st = (DIR == 2) ? b.s2 : b.s3;
+    long ci = 0;
+    R lo[NV];
+    if (valid) {
+        const int outer = o0 + col / ni;
+        const int i = b.is - 1 + col % ni;
+        const int j = (DIR == 2) ? s : outer;
+        const int k = (DIR == 2) ? outer : s;
+        ci = (long)k * b.s3 + (long)j * b.s2 + i;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const PT* q = P + v * b.nc + ci;
+            R ro;
+            cell_states<RECON, R>(R(q[-2 * st]), R(q[-st]), R(q[0]), R(q[st]), R(q[2 * st]), lo[v], ro);
+            sr[v][ty][tx] = ro;
+        }
+    }
+    __syncthreads();
+    if (!valid || ty == 0) return;
+    R Pl[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) Pl[v] = sr[v][ty - 1][tx];
+    face_flux<DIR, SPEEDS, R>(c, b, Pl, lo, ci, F, cmax_out, cmin_out);
+}
+
+template <int RECON, int SPEEDS, typename R, typename PT>
+static hipError_t launch_recon(const BlockDims& b, const StepConsts& c, int dir, const PT* P, R* F,
+                               R* cmax, R* cmin, hipStream_t s) {
+    // Face ranges consumed by flux-CT and the divergence: the face-normal index
+    // covers [start, end+1]; transverse indices extend one zone each side.
+    const int jlo = (b.ndim > 1) ? b.js - 1 : b.js, jhi = (b.ndim > 1) ? b.je + 1 : b.je;
+    const int klo = (b.ndim > 2) ? b.ks - 1 : b.ks, khi = (b.ndim > 2) ? b.ke + 1 : b.ke;
+    if (dir == 1) {
+        const int nj = jhi - jlo + 1, nk = khi - klo + 1;
+        const long cells = (long)(b.nx1 + 2) * nj * nk;
+        const dim3 grid((unsigned)((cells - 1 + 254) / 255));
+        hipLaunchKernelGGL((flux_x1_kernel<RECON, SPEEDS, R, PT>), grid, dim3(256), 0, s, b, c, P, F, cmax, cmin, jlo, nj, klo, nk);
+    } else if (dir == 2) {
+        const int no = khi - klo + 1;
+        const int ncol = (b.nx1 + 2) * no;
+        const int nfaces = b.nx2 + 1;
+        const dim3 grid((unsigned)((ncol + 63) / 64), (unsigned)((nfaces + 6) / 7));
+        hipLaunchKernelGGL((flux_xt_kernel<2, RECON, SPEEDS, R, PT>), grid, dim3(64, 8), 0, s, b, c, P, F, cmax, cmin, klo, no, b.js, b.je + 1);
+    } else {
+        const int no = jhi - jlo + 1;
+        const int ncol = (b.nx1 + 2) * no;
+        const int nfaces = b.nx3 + 1;
+        const dim3 grid((unsigned)((ncol + 63) / 64), (unsigned)((nfaces + 6) / 7));
+        hipLaunchKernelGGL((flux_xt_kernel<3, RECON, SPEEDS, R, PT>), grid, dim3(64, 8), 0, s, b, c, P, F, cmax, cmin, jlo, no, b.ks, b.ke + 1);
     }
     return hipGetLastError();
+}
+
+template <int SPEEDS, typename R, typename PT>
+static hipError_t launch_speeds(const BlockDims& b, const StepConsts& c, int dir, int recon, const PT* P, R* F,
+                                R* cmax, R* cmin, hipStream_t s) {
+    switch (recon) {
+    case RECON_DONOR: return launch_recon<RECON_DONOR, SPEEDS, R, PT>(b, c, dir, P, F, cmax, cmin, s);
+    case RECON_LINEAR_MC: return launch_recon<RECON_LINEAR_MC, SPEEDS, R, PT>(b, c, dir, P, F, cmax, cmin, s);
+    case RECON_PPM: return launch_recon<RECON_PPM, SPEEDS, R, PT>(b, c, dir, P, F, cmax, cmin, s);
+    default: return launch_recon<RECON_WENO5, SPEEDS, R, PT>(b, c, dir, P, F, cmax, cmin, s);
+    }
+}
+
+template <typename R, typename PT>
+static hipError_t launch_prec(const BlockDims& b, const StepConsts& c, int dir, int recon, int speeds, const PT* P,
+                              R* F, R* cmax, R* cmin, hipStream_t s) {
+    if (speeds == SPEEDS_CTOP) return launch_speeds<SPEEDS_CTOP, R, PT>(b, c, dir, recon, P, F, cmax, cmin, s);
+    if (speeds == SPEEDS_NONE) return launch_speeds<SPEEDS_NONE, R, PT>(b, c, dir, recon, P, F, cmax, cmin, s);
+    return launch_speeds<SPEEDS_BOTH, R, PT>(b, c, dir, recon, P, F, cmax, cmin, s);
 }
 
 hipError_t launch_flux(const BlockDims& b, const StepConsts& c, int dir, int recon, int prec,
                        const double* P, void* F, void* cmax, void* cmin, hipStream_t s) {
     if (dir == 2 && b.ndim < 2) return hipSuccess;
     if (dir == 3 && b.ndim < 3) return hipSuccess;
-    if (prec == PREC_SPLIT) {
-        float *f = (float*)F, *mx = (float*)cmax, *mn = (float*)cmin;
-        if (dir == 1) return launch_dir_prec<1, float>(b, c, recon, P, f, mx, mn, s);
-        if (dir == 2) return launch_dir_prec<2, float>(b, c, recon, P, f, mx, mn, s);
-        return launch_dir_prec<3, float>(b, c, recon, P, f, mx, mn, s);
-    }
-    double *f = (double*)F, *mx = (double*)cmax, *mn = (double*)cmin;
-    if (dir == 1) return launch_dir_prec<1, double>(b, c, recon, P, f, mx, mn, s);
-    if (dir == 2) return launch_dir_prec<2, double>(b, c, recon, P, f, mx, mn, s);
-    return launch_dir_prec<3, double>(b, c, recon, P, f, mx, mn, s);
+    if (prec == PREC_SPLIT)
+        return launch_prec<float, double>(b, c, dir, recon, SPEEDS_BOTH, P, (float*)F, (float*)cmax, (float*)cmin, s);
+    return launch_prec<double, double>(b, c, dir, recon, SPEEDS_BOTH, P, (double*)F, (double*)cmax, (double*)cmin, s);
+}
+
+hipError_t launch_flux_stage(const BlockDims& b, const StepConsts& c, int dir, int recon, int prec, int speeds,
+                             const double* P, const float* P32, void* F, void* ctop, hipStream_t s) {
+    if (dir == 2 && b.ndim < 2) return hipSuccess;
+    if (dir == 3 && b.ndim < 3) return hipSuccess;
+    if (prec == PREC_SPLIT)
+        return launch_prec<float, float>(b, c, dir, recon, speeds, P32, (float*)F, (float*)ctop, nullptr, s);
+    return launch_prec<double, double>(b, c, dir, recon, speeds, P, (double*)F, (double*)ctop, nullptr, s);
 }
 
 }  // namespace kh

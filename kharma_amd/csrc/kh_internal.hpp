@@ -26,8 +26,6 @@ struct BlockDims {
 // host round trip (dt is produced and consumed on the GPU).
 struct DeviceState {
     double dt, dt_last, time;
-    unsigned long long min_ndt_bits;   // running min of the CFL estimate (non-negative double bits)
-    unsigned long long divb_bits;      // running max of |divB| (non-negative double bits)
     double last_divb;
     double last_ndt;
     int fail_count[2];                 // failed inversions per stage of the current step
@@ -48,6 +46,7 @@ struct StepConsts {
 };
 
 enum Precision { PREC_FP64 = 0, PREC_SPLIT = 1 };
+enum SpeedsOut { SPEEDS_NONE = 0, SPEEDS_BOTH = 1, SPEEDS_CTOP = 2 };
 
 // ---- launchers (kh_flux.hip) ----
 // Face fluxes through the faces normal to `dir` that the step consumes:
@@ -56,21 +55,33 @@ enum Precision { PREC_FP64 = 0, PREC_SPLIT = 1 };
 // direction's arrays ([8][nc] and [nc]); element type float when prec == PREC_SPLIT.
 hipError_t launch_flux(const BlockDims& b, const StepConsts& c, int dir, int recon, int prec,
                        const double* P, void* F, void* cmax, void* cmin, hipStream_t s);
+// Stage-driver variant: speeds = SPEEDS_NONE or SPEEDS_CTOP (ctop = max(cmax, cmin)
+// into ctop[dir-1]); the split form reads the FP32 shadow P32 of the primitives.
+hipError_t launch_flux_stage(const BlockDims& b, const StepConsts& c, int dir, int recon, int prec, int speeds,
+                             const double* P, const float* P32, void* F, void* ctop, hipStream_t s);
 
 // ---- launchers (kh_step.hip) ----
 hipError_t launch_emf(const BlockDims& b, int prec, const void* F, void* emf, hipStream_t s);
 hipError_t launch_flux_ct_rewrite(const BlockDims& b, int prec, void* F, const void* emf, hipStream_t s);
 hipError_t launch_flux_divergence(const BlockDims& b, int prec, const void* F, double* dUdt, hipStream_t s);
+// Fused stage update (flux-CT EMFs, divergence, RK update, U->P, floors, P->U,
+// ghost images).  ctop: [3][nc] max signal speeds, read when do_dt (the last
+// stage), which then writes one CFL minimum per block into dt_part.
 hipError_t launch_update(const BlockDims& b, const StepConsts& c, int prec, int stage, bool do_dt,
-                         const void* F, const void* emf, const void* cmax, const void* cmin,
-                         const double* Usub, const double* Ufull, const double* Psub,
-                         double* Uout, double* Pout, int* pflag, int* fflag,
-                         int* fail_list, DeviceState* ds, hipStream_t s);
-hipError_t launch_fixup(const BlockDims& b, const StepConsts& c, int stage, double* U, double* P,
+                         const void* F, const void* ctop, const double* Usub, const double* Ufull, const double* Psub,
+                         double* Uout, double* Pout, float* Pout32, int* pflag, int* fflag,
+                         int* fail_list, DeviceState* ds, double* dt_part, hipStream_t s);
+long update_blocks(const BlockDims& b);
+long divb_blocks(const BlockDims& b);
+hipError_t launch_fixup(const BlockDims& b, const StepConsts& c, int stage, double* U, double* P, float* P32,
                         const int* pflag, const int* fail_list, DeviceState* ds, hipStream_t s);
-hipError_t launch_finalize(const StepConsts& c, DeviceState* ds, double* dt_hist, int hist_cap, hipStream_t s);
-hipError_t launch_max_divb(const BlockDims& b, const double* U, DeviceState* ds, hipStream_t s);
+hipError_t launch_finalize(const StepConsts& c, DeviceState* ds, const double* dt_part, int n_dt,
+                           const double* divb_part, int n_divb, double* dt_hist, int hist_cap, hipStream_t s);
+hipError_t launch_max_divb(const BlockDims& b, const double* U, double* part, hipStream_t s);
+hipError_t launch_reduce(const double* part, int n, int is_max, double* out, hipStream_t s);
+hipError_t launch_convert_f32(const double* A, float* B, long n, hipStream_t s);
 hipError_t launch_reset_state(DeviceState* ds, double dt, hipStream_t s);
+hipError_t launch_set_clock(DeviceState* ds, double time, double dt, int ncycle, hipStream_t s);
 // Task-level kernels over the entire block (the reference's IndexDomain::entire).
 hipError_t launch_weighted_sum(const BlockDims& b, const double* Usub, const double* Ufull, const double* dUdt,
                                double gam0, double gam1, double beta_dt, double* Uout, hipStream_t s);
@@ -84,6 +95,6 @@ hipError_t launch_ptou(const BlockDims& b, const StepConsts& c, const double* P,
 hipError_t launch_sync(const BlockDims& b, double* A, int nvar, hipStream_t s);
 hipError_t launch_sync_int(const BlockDims& b, int* A, hipStream_t s);
 hipError_t launch_estimate_dt(const BlockDims& b, const StepConsts& c, int prec, const void* cmax, const void* cmin,
-                              DeviceState* ds, hipStream_t s);
+                              double* part, hipStream_t s);
 
 }  // namespace kh

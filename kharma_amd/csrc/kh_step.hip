@@ -20,40 +20,49 @@ namespace kh {
 static constexpr int BS = 256;
 
 static inline dim3 grid_for(long n, int bs = BS) { return dim3((unsigned)((n + bs - 1) / bs)); }
+static inline long ninterior_host(const BlockDims& b) {
+    return (long)b.nx1 * (b.ndim > 1 ? b.nx2 : 1) * (b.ndim > 2 ? b.nx3 : 1);
+}
+
+// FP32 shadow of the primitives read by the split-precision flux kernels.
+__global__ void __launch_bounds__(256) convert_f32_kernel(const double* __restrict__ A, float* __restrict__ B, long n) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) B[t] = (float)A[t];
+}
 
 // ------------------------------------------------------------ ghost images
-// Periodic images of interior zone (k,j,i) inside the ghost region: offsets
-// per axis (0 = the zone itself).  Needs nx >= ng on periodic axes.
+// Periodic image of interior zone (k,j,i) inside the ghost region, as element
+// offsets per axis (0 = none).  With nx >= 2 ng on every periodic axis a zone
+// has at most one image offset per axis, so up to 7 image copies (corners).
 struct ImageSet {
-    int ni, nj, nk;
-    int di[3], dj[3], dk[3];
+    long oi, oj, ok;
 };
 __device__ __forceinline__ void image_set(const BlockDims& b, int k, int j, int i, ImageSet& s) {
-    s.ni = s.nj = s.nk = 1;
-    s.di[0] = s.dj[0] = s.dk[0] = 0;
+    s.oi = s.oj = s.ok = 0;
     if (b.periodic[0]) {
-        if (i - b.is < b.ng) s.di[s.ni++] = b.nx1;
-        if (b.ie - i < b.ng) s.di[s.ni++] = -b.nx1;
+        if (i - b.is < b.ng) s.oi = b.nx1;
+        else if (b.ie - i < b.ng) s.oi = -b.nx1;
     }
     if (b.ndim > 1 && b.periodic[1]) {
-        if (j - b.js < b.ng) s.dj[s.nj++] = b.nx2;
-        if (b.je - j < b.ng) s.dj[s.nj++] = -b.nx2;
+        if (j - b.js < b.ng) s.oj = (long)b.nx2 * b.s2;
+        else if (b.je - j < b.ng) s.oj = -(long)b.nx2 * b.s2;
     }
     if (b.ndim > 2 && b.periodic[2]) {
-        if (k - b.ks < b.ng) s.dk[s.nk++] = b.nx3;
-        if (b.ke - k < b.ng) s.dk[s.nk++] = -b.nx3;
+        if (k - b.ks < b.ng) s.ok = (long)b.nx3 * b.s3;
+        else if (b.ke - k < b.ng) s.ok = -(long)b.nx3 * b.s3;
     }
 }
 template <typename T>
-__device__ __forceinline__ void store_with_images(const BlockDims& b, const ImageSet& s, T* base, long c, T val) {
+__device__ __forceinline__ void store_with_images(const BlockDims&, const ImageSet& s, T* base, long c, T val) {
     base[c] = val;
-    if (s.ni * s.nj * s.nk == 1) return;
-    for (int a = 0; a < s.nk; ++a)
-        for (int e = 0; e < s.nj; ++e)
-            for (int d = 0; d < s.ni; ++d) {
-                if (a == 0 && e == 0 && d == 0) continue;
-                base[c + (long)s.dk[a] * b.s3 + (long)s.dj[e] * b.s2 + s.di[d]] = val;
-            }
+    if ((s.oi | s.oj | s.ok) == 0) return;
+    if (s.oi) base[c + s.oi] = val;
+    if (s.oj) base[c + s.oj] = val;
+    if (s.ok) base[c + s.ok] = val;
+    if (s.oi && s.oj) base[c + s.oi + s.oj] = val;
+    if (s.oi && s.ok) base[c + s.oi + s.ok] = val;
+    if (s.oj && s.ok) base[c + s.oj + s.ok] = val;
+    if (s.oi && s.oj && s.ok) base[c + s.oi + s.oj + s.ok] = val;
 }
 
 __device__ __forceinline__ void interior_index(const BlockDims& b, unsigned t, int& k, int& j, int& i) {
@@ -180,44 +189,124 @@ __device__ __forceinline__ void zone_divergence(const BlockDims& b, const R* __r
     }
 }
 
-__device__ __forceinline__ double zone_ndt(const BlockDims& b, double c1, double n1, double c2, double n2, double c3, double n3) {
-    return 1.0 / (1 / (b.dx1 / smax(c1, n1)) + 1 / (b.dx2 / smax(c2, n2)) + 1 / (b.dx3 / smax(c3, n3)));
+__device__ __forceinline__ double zone_ndt(const BlockDims& b, double c1, double c2, double c3) {
+    // GRMHD::EstimateTimestep, grmhd.cpp:320-322, with c_d = max(cmax_d, cmin_d)
+    return 1.0 / (1 / (b.dx1 / c1) + 1 / (b.dx2 / c2) + 1 / (b.dx3 / c3));
 }
 
-__device__ __forceinline__ void block_min_to(double v, unsigned long long* dst) {
-    __shared__ double red[BS / 64];
+// Block-wide min / max (wavefront shuffles, then LDS across the block's waves).
+__device__ __forceinline__ double block_min(double v) {
+    __shared__ double red[1024 / 64];
     v = wave_min(v);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
     if (lane == 0) red[w] = v;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        double m = red[0];
-        for (int q = 1; q < (int)(blockDim.x >> 6); ++q) m = fmin(m, red[q]);
-        if (m < DBL_MAX) atomic_min_pos(dst, m);
-    }
+    double m = red[0];
+    for (int q = 1; q < (int)(blockDim.x >> 6); ++q) m = fmin(m, red[q]);
+    return m;
 }
-__device__ __forceinline__ void block_max_to(double v, unsigned long long* dst) {
-    __shared__ double red[BS / 64];
+__device__ __forceinline__ double block_max(double v) {
+    __shared__ double red[1024 / 64];
     v = wave_max(v);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
     if (lane == 0) red[w] = v;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        double m = red[0];
-        for (int q = 1; q < (int)(blockDim.x >> 6); ++q) m = fmax(m, red[q]);
-        atomic_max_pos(dst, m);
+    double m = red[0];
+    for (int q = 1; q < (int)(blockDim.x >> 6); ++q) m = fmax(m, red[q]);
+    return m;
+}
+
+// Corner EMFs of flux-CT (b_flux_ct.cpp:331-342) evaluated from the Riemann
+// B fluxes at corner c.
+template <typename R>
+__device__ __forceinline__ R emf3_at(const R* F1B2, const R* F2B1, long c, long s2) {
+    return R(0.25) * (F1B2[c] + F1B2[c - s2] - F2B1[c] - F2B1[c - 1]);
+}
+template <typename R>
+__device__ __forceinline__ R emf1_at(const R* F2B3, const R* F3B2, long c, long s2, long s3) {
+    return R(0.25) * (F2B3[c] + F2B3[c - s3] - F3B2[c] - F3B2[c - s2]);
+}
+template <typename R>
+__device__ __forceinline__ R emf2_at(const R* F3B1, const R* F1B3, long c, long s3) {
+    return R(0.25) * (F3B1[c] + F3B1[c - 1] - F1B3[c] - F1B3[c - s3]);
+}
+
+// Divergence of zone c with the flux-CT B fluxes (b_flux_ct.cpp:347-369) built
+// from EMFs computed here from the Riemann fluxes, so neither the EMFs nor the
+// rewritten B fluxes make a trip through HBM.
+template <typename R>
+__device__ __forceinline__ void zone_divergence_ct(const BlockDims& b, const R* __restrict__ F, long c, double dudt[NV]) {
+    const double A1 = b.dx2 * b.dx3, A2 = b.dx1 * b.dx3, A3 = b.dx1 * b.dx2, V = b.dx1 * b.dx2 * b.dx3;
+    const R* F1 = F;
+    const R* F2 = F + 8 * b.nc;
+    const R* F3 = F + 16 * b.nc;
+    const long s2 = b.s2, s3 = b.s3, nc = b.nc;
+    const int nd = b.ndim;
+    double f1lo[NV], f1hi[NV], f2lo[NV], f2hi[NV], f3lo[NV], f3hi[NV];
+#pragma unroll
+    for (int v = 0; v < B1; ++v) {
+        f1lo[v] = (double)F1[v * nc + c];
+        f1hi[v] = (double)F1[v * nc + c + 1];
+        f2lo[v] = (double)F2[v * nc + c];
+        f2hi[v] = (double)F2[v * nc + c + s2];
+        if (nd == 3) { f3lo[v] = (double)F3[v * nc + c]; f3hi[v] = (double)F3[v * nc + c + s3]; }
+        else { f3lo[v] = 0.; f3hi[v] = 0.; }
+    }
+    const R* F1B2 = F1 + B2 * nc;
+    const R* F2B1 = F2 + B1 * nc;
+    const R e3_00 = emf3_at(F1B2, F2B1, c, s2), e3_10 = emf3_at(F1B2, F2B1, c + 1, s2);
+    const R e3_01 = emf3_at(F1B2, F2B1, c + s2, s2), e3_11 = emf3_at(F1B2, F2B1, c + s2 + 1, s2);
+    f1lo[B1] = 0.; f1hi[B1] = 0.;
+    f1lo[B2] = (double)(R(0.5) * (e3_00 + e3_01));
+    f1hi[B2] = (double)(R(0.5) * (e3_10 + e3_11));
+    f2lo[B1] = (double)(R(-0.5) * (e3_00 + e3_10));
+    f2hi[B1] = (double)(R(-0.5) * (e3_01 + e3_11));
+    f2lo[B2] = 0.; f2hi[B2] = 0.;
+    f3lo[B1] = f3hi[B1] = f3lo[B2] = f3hi[B2] = f3lo[B3] = f3hi[B3] = 0.;
+    if (nd == 3) {
+        const R* F2B3 = F2 + B3 * nc;
+        const R* F3B2 = F3 + B2 * nc;
+        const R* F3B1 = F3 + B1 * nc;
+        const R* F1B3 = F1 + B3 * nc;
+        const R e1_000 = emf1_at(F2B3, F3B2, c, s2, s3), e1_010 = emf1_at(F2B3, F3B2, c + s2, s2, s3);
+        const R e1_001 = emf1_at(F2B3, F3B2, c + s3, s2, s3), e1_011 = emf1_at(F2B3, F3B2, c + s2 + s3, s2, s3);
+        const R e2_000 = emf2_at(F3B1, F1B3, c, s3), e2_100 = emf2_at(F3B1, F1B3, c + 1, s3);
+        const R e2_001 = emf2_at(F3B1, F1B3, c + s3, s3), e2_101 = emf2_at(F3B1, F1B3, c + s3 + 1, s3);
+        f1lo[B3] = (double)(R(-0.5) * (e2_000 + e2_001));
+        f1hi[B3] = (double)(R(-0.5) * (e2_100 + e2_101));
+        f2lo[B3] = (double)(R(0.5) * (e1_000 + e1_001));
+        f2hi[B3] = (double)(R(0.5) * (e1_010 + e1_011));
+        f3lo[B1] = (double)(R(0.5) * (e2_000 + e2_100));
+        f3hi[B1] = (double)(R(0.5) * (e2_001 + e2_101));
+        f3lo[B2] = (double)(R(-0.5) * (e1_000 + e1_010));
+        f3hi[B2] = (double)(R(-0.5) * (e1_001 + e1_011));
+    } else {
+        // 2D: B3 keeps its Riemann fluxes
+        f1lo[B3] = (double)F1[B3 * nc + c];
+        f1hi[B3] = (double)F1[B3 * nc + c + 1];
+        f2lo[B3] = (double)F2[B3 * nc + c];
+        f2hi[B3] = (double)F2[B3 * nc + c + s2];
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        double du = (A1 * f1hi[v] - A1 * f1lo[v]);
+        du += (A2 * f2hi[v] - A2 * f2lo[v]);
+        if (nd == 3) du += (A3 * f3hi[v] - A3 * f3lo[v]);
+        dudt[v] = -du / V;
     }
 }
 
 // ------------------------------------------------------- fused stage update
-template <typename R, bool DO_DT>
-__global__ void __launch_bounds__(BS)
-update_kernel(BlockDims b, StepConsts cst, int stage,
-              const R* __restrict__ F, const R* __restrict__ emf,
-              const R* __restrict__ cmax, const R* __restrict__ cmin,
+// Stage 1 (gam0 = 0 and U_sub == U_full) reads the state once: 0*U + 1*U == U.
+template <typename R, bool DO_DT, bool CT>
+__global__ void __launch_bounds__(BS, 4)
+update_kernel(BlockDims b, StepConsts cst, int stage, const R* __restrict__ F, const R* __restrict__ ctop,
               const double* __restrict__ Usub, const double* Ufull, const double* __restrict__ Psub,
-              double* Uout, double* __restrict__ Pout, int* __restrict__ pflag, int* __restrict__ fflag,
-              int* __restrict__ fail_list, DeviceState* __restrict__ ds) {
+              double* Uout, double* __restrict__ Pout, float* __restrict__ Pout32,
+              int* __restrict__ pflag, int* __restrict__ fflag, int* __restrict__ fail_list,
+              DeviceState* __restrict__ ds, double* __restrict__ dt_part) {
     const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = t < (unsigned)ninterior(b);
     double ndt = DBL_MAX;
@@ -227,13 +316,16 @@ update_kernel(BlockDims b, StepConsts cst, int stage,
         const long c = (long)k * b.s3 + (long)j * b.s2 + i;
         const double gam = cst.gam;
         double dudt[NV];
-        zone_divergence<R>(b, F, emf, c, b.ndim > 1, dudt);
+        if (CT) zone_divergence_ct<R>(b, F, c, dudt);
+        else zone_divergence<R>(b, F, nullptr, c, false, dudt);
         const double g0 = cst.gam0[stage - 1], g1 = cst.gam1[stage - 1];
         const double beta_dt = cst.beta[stage - 1] * ds->dt;
+        const bool same = (g0 == 0.) && (Usub == Ufull);
         double U[NV], P[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-            double u = g0 * Usub[v * b.nc + c] + g1 * Ufull[v * b.nc + c];
+            const double uf = Ufull[v * b.nc + c];
+            const double u = same ? uf : g0 * Usub[v * b.nc + c] + g1 * uf;
             U[v] = 1.0 * u + beta_dt * dudt[v];
         }
         // U->P guess: the stage's starting primitives (kharma_driver.cpp:379-384)
@@ -244,15 +336,18 @@ update_kernel(BlockDims b, StepConsts cst, int stage,
         const int ff0 = (pf / F_GEOM_RHO) * F_GEOM_RHO;
         pf = pf % F_GEOM_RHO;
         P[B1] = U[B1] / g.gdet(); P[B2] = U[B2] / g.gdet(); P[B3] = U[B3] / g.gdet();
+        FourVec<double> D;
+        calc_4vecs<double>(g, P, D);
         int ff = ff0;
-        if (cst.floors.on) ff = floors_zone(g, P, U, gam, cst.floors, ff0, pf);
-        p_to_u(g, P, gam, U);
+        if (cst.floors.on) ff = floors_zone(g, P, U, D, gam, cst.floors, ff0, pf);
+        p_to_u_D(g, P, D, gam, U);
         ImageSet s;
         image_set(b, k, j, i, s);
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             store_with_images(b, s, Uout + v * b.nc, c, U[v]);
             store_with_images(b, s, Pout + v * b.nc, c, P[v]);
+            if (Pout32) store_with_images(b, s, Pout32 + v * b.nc, c, (float)P[v]);
         }
         store_with_images(b, s, pflag, c, pf);
         fflag[c] = ff;
@@ -261,12 +356,15 @@ update_kernel(BlockDims b, StepConsts cst, int stage,
             fail_list[slot] = (int)c;
         }
         if (DO_DT) {
-            const double n = zone_ndt(b, (double)cmax[c], (double)cmin[c], (double)cmax[b.nc + c], (double)cmin[b.nc + c],
-                                      (double)cmax[2 * b.nc + c], (double)cmin[2 * b.nc + c]);
+            const double n = zone_ndt(b, (double)ctop[c], b.ndim > 1 ? (double)ctop[b.nc + c] : 0.,
+                                      b.ndim > 2 ? (double)ctop[2 * b.nc + c] : 0.);
             if (!(n != n)) ndt = n;
         }
     }
-    if (DO_DT) block_min_to(ndt, &ds->min_ndt_bits);
+    if (DO_DT) {
+        const double m = block_min(ndt);
+        if (threadIdx.x == 0) dt_part[blockIdx.x] = m;
+    }
 }
 
 // Inverter::FixUtoP (inverter/fixup.cpp:47-156) on the zones listed by the
@@ -274,7 +372,8 @@ update_kernel(BlockDims b, StepConsts cst, int stage,
 // fallback, geometric floors, then the conserved state of the final P->U.
 __global__ void __launch_bounds__(BS)
 fixup_list_kernel(BlockDims b, StepConsts cst, int stage, double* __restrict__ U, double* __restrict__ P,
-                  const int* __restrict__ pflag, const int* __restrict__ fail_list, const DeviceState* __restrict__ ds) {
+                  float* __restrict__ P32, const int* __restrict__ pflag, const int* __restrict__ fail_list,
+                  const DeviceState* __restrict__ ds) {
     const int n = ds->fail_count[stage - 1];
     for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
         const long c = fail_list[q];
@@ -302,16 +401,36 @@ fixup_list_kernel(BlockDims b, StepConsts cst, int stage, double* __restrict__ U
         p_to_u(g, Pz, cst.gam, Uz);
         ImageSet s;
         image_set(b, k, j, i, s);
-        for (int v = 0; v < 5; ++v) store_with_images(b, s, P + v * b.nc, c, Pz[v]);
+        for (int v = 0; v < 5; ++v) {
+            store_with_images(b, s, P + v * b.nc, c, Pz[v]);
+            if (P32) store_with_images(b, s, P32 + v * b.nc, c, (float)Pz[v]);
+        }
         for (int v = 0; v < NV; ++v) store_with_images(b, s, U + v * b.nc, c, Uz[v]);
     }
 }
 
-// End-of-step scalar work on the device: SetGlobalTimeStep (kharma_driver.cpp:439-460)
+// End-of-step scalar work on the device: the CFL minimum and divB maximum of
+// the per-block partials, then SetGlobalTimeStep (kharma_driver.cpp:439-460)
 // with GRMHD::EstimateTimestep's clip (grmhd.cpp:335-339), time advance, resets.
-__global__ void finalize_kernel(StepConsts cst, DeviceState* ds, double* dt_hist, int hist_cap) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    const double min_ndt = __longlong_as_double((long long)ds->min_ndt_bits);
+__global__ void __launch_bounds__(1024)
+finalize_kernel(StepConsts cst, DeviceState* ds, const double* __restrict__ dt_part, int n_dt,
+                const double* __restrict__ divb_part, int n_divb, double* dt_hist, int hist_cap) {
+    double mn0 = DBL_MAX, mn1 = DBL_MAX, mx0 = 0., mx1 = 0.;
+    const int st = blockDim.x;
+#pragma unroll 4
+    for (int q = threadIdx.x; q < n_dt; q += 2 * st) {
+        mn0 = fmin(mn0, dt_part[q]);
+        if (q + st < n_dt) mn1 = fmin(mn1, dt_part[q + st]);
+    }
+#pragma unroll 4
+    for (int q = threadIdx.x; q < n_divb; q += 2 * st) {
+        mx0 = fmax(mx0, divb_part[q]);
+        if (q + st < n_divb) mx1 = fmax(mx1, divb_part[q + st]);
+    }
+    const double mn = block_min(fmin(mn0, mn1));
+    const double mx = block_max(fmax(mx0, mx1));
+    if (threadIdx.x != 0) return;
+    const double min_ndt = mn;
     const double dt = ds->dt;
     const double dt_max = cst.max_dt_increase * dt;
     const double new_dt = clip(min_ndt * cst.cfl, cst.dt_min, dt_max);
@@ -326,25 +445,34 @@ __global__ void finalize_kernel(StepConsts cst, DeviceState* ds, double* dt_hist
     ds->last_ndt = min_ndt;
     ds->ncycle += 1;
     ds->total_fails += ds->fail_count[0] + ds->fail_count[1];
-    ds->last_divb = __longlong_as_double((long long)ds->divb_bits);
-    ds->divb_bits = 0ull;
-    ds->min_ndt_bits = (unsigned long long)__double_as_longlong(DBL_MAX);
+    ds->last_divb = mx;
     ds->fail_count[0] = 0;
     ds->fail_count[1] = 0;
+}
+
+// One-block reduction of per-block partials into *out (min or max).
+__global__ void __launch_bounds__(BS) reduce_partials_kernel(const double* __restrict__ part, int n, int is_max, double* out) {
+    double v = is_max ? 0. : DBL_MAX;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) v = is_max ? fmax(v, part[q]) : fmin(v, part[q]);
+    v = is_max ? block_max(v) : block_min(v);
+    if (threadIdx.x == 0) *out = v;
 }
 
 __global__ void reset_state_kernel(DeviceState* ds, double dt) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     ds->dt = dt; ds->dt_last = dt; ds->time = 0.;
-    ds->min_ndt_bits = (unsigned long long)__double_as_longlong(DBL_MAX);
-    ds->divb_bits = 0ull;
     ds->last_divb = 0.; ds->last_ndt = 0.;
     ds->fail_count[0] = ds->fail_count[1] = 0;
     ds->ncycle = 0; ds->total_fails = 0;
 }
 
+__global__ void set_clock_kernel(DeviceState* ds, double time, double dt, int ncycle) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    ds->time = time; ds->dt = dt; ds->dt_last = dt; ds->ncycle = ncycle;
+}
+
 // B_FluxCT::MaxDivB with corner_div (b_flux_ct.cpp:566-607, b_flux_ct_functions.hpp:52-75)
-__global__ void __launch_bounds__(BS) max_divb_kernel(BlockDims b, const double* __restrict__ U, DeviceState* ds) {
+__global__ void __launch_bounds__(BS) max_divb_kernel(BlockDims b, const double* __restrict__ U, double* __restrict__ part) {
     const int ni = b.nx1 + 1, nj = b.nx2 + 1, nk = (b.ndim > 2) ? b.nx3 + 1 : 1;
     const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
     double d = 0.;
@@ -369,7 +497,8 @@ __global__ void __launch_bounds__(BS) max_divb_kernel(BlockDims b, const double*
         }
         d = fabs(norm * term1 / b.dx1 + norm * term2 / b.dx2 + norm * term3 / b.dx3);
     }
-    block_max_to(d, &ds->divb_bits);
+    const double m = block_max(d);
+    if (threadIdx.x == 0) part[blockIdx.x] = m;
 }
 
 // ----------------------------------------------------- task-level kernels
@@ -420,12 +549,14 @@ __global__ void __launch_bounds__(BS) floors_kernel(BlockDims b, StepConsts cst,
     for (int v = 0; v < NV; ++v) { Uz[v] = U[v * b.nc + c]; Pz[v] = P[v * b.nc + c]; }
     MinkowskiGeom<double> g;
     int pf = pflag[c];
-    const int ff = floors_zone(g, Pz, Uz, cst.gam, cst.floors, fflag[c], pf);
+    FourVec<double> D;
+    calc_4vecs<double>(g, Pz, D);
+    const int ff = floors_zone(g, Pz, Uz, D, cst.gam, cst.floors, fflag[c], pf);
     fflag[c] = ff;
     pflag[c] = pf;
     if (ff) {
         double Un[NV];
-        p_to_u(g, Pz, cst.gam, Un);
+        p_to_u_D(g, Pz, D, cst.gam, Un);
         for (int v = 0; v < 5; ++v) { P[v * b.nc + c] = Pz[v]; U[v * b.nc + c] = Un[v]; }
     }
 }
@@ -495,18 +626,20 @@ __global__ void __launch_bounds__(BS) sync_kernel(BlockDims b, T* __restrict__ A
 
 template <typename R>
 __global__ void __launch_bounds__(BS) estimate_dt_kernel(BlockDims b, const R* __restrict__ cmax, const R* __restrict__ cmin,
-                                                         DeviceState* ds) {
+                                                         double* __restrict__ part) {
     const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
     double ndt = DBL_MAX;
     if (t < (unsigned)ninterior(b)) {
         int k, j, i;
         interior_index(b, t, k, j, i);
         const long c = (long)k * b.s3 + (long)j * b.s2 + i;
-        const double n = zone_ndt(b, (double)cmax[c], (double)cmin[c], (double)cmax[b.nc + c], (double)cmin[b.nc + c],
-                                  (double)cmax[2 * b.nc + c], (double)cmin[2 * b.nc + c]);
+        const double n = zone_ndt(b, smax((double)cmax[c], (double)cmin[c]),
+                                  smax((double)cmax[b.nc + c], (double)cmin[b.nc + c]),
+                                  smax((double)cmax[2 * b.nc + c], (double)cmin[2 * b.nc + c]));
         if (!(n != n)) ndt = n;
     }
-    block_min_to(ndt, &ds->min_ndt_bits);
+    const double m = block_min(ndt);
+    if (threadIdx.x == 0) part[blockIdx.x] = m;
 }
 
 // ---------------------------------------------------------------- launchers
@@ -531,40 +664,53 @@ hipError_t launch_flux_divergence(const BlockDims& b, int prec, const void* F, d
     return hipGetLastError();
 }
 hipError_t launch_update(const BlockDims& b, const StepConsts& c, int prec, int stage, bool do_dt,
-                         const void* F, const void* emf, const void* cmax, const void* cmin,
-                         const double* Usub, const double* Ufull, const double* Psub,
-                         double* Uout, double* Pout, int* pflag, int* fflag,
-                         int* fail_list, DeviceState* ds, hipStream_t s) {
-    const long n = (long)b.nx1 * (b.ndim > 1 ? b.nx2 : 1) * (b.ndim > 2 ? b.nx3 : 1);
-    const dim3 g = grid_for(n);
+                         const void* F, const void* ctop, const double* Usub, const double* Ufull, const double* Psub,
+                         double* Uout, double* Pout, float* Pout32, int* pflag, int* fflag,
+                         int* fail_list, DeviceState* ds, double* dt_part, hipStream_t s) {
+    const dim3 g = grid_for(ninterior_host(b));
+    const bool ct = b.ndim > 1;
+#define KH_UPD(R, DT, CT)                                                                              \
+    hipLaunchKernelGGL((update_kernel<R, DT, CT>), g, dim3(BS), 0, s, b, c, stage, (const R*)F, (const R*)ctop, \
+                       Usub, Ufull, Psub, Uout, Pout, Pout32, pflag, fflag, fail_list, ds, dt_part)
     if (prec == PREC_SPLIT) {
-        auto f = (const float*)F; auto e = (const float*)emf; auto mx = (const float*)cmax; auto mn = (const float*)cmin;
-        if (do_dt) hipLaunchKernelGGL((update_kernel<float, true>), g, dim3(BS), 0, s, b, c, stage, f, e, mx, mn, Usub, Ufull, Psub, Uout, Pout, pflag, fflag, fail_list, ds);
-        else hipLaunchKernelGGL((update_kernel<float, false>), g, dim3(BS), 0, s, b, c, stage, f, e, mx, mn, Usub, Ufull, Psub, Uout, Pout, pflag, fflag, fail_list, ds);
+        if (do_dt) { if (ct) KH_UPD(float, true, true); else KH_UPD(float, true, false); }
+        else { if (ct) KH_UPD(float, false, true); else KH_UPD(float, false, false); }
     } else {
-        auto f = (const double*)F; auto e = (const double*)emf; auto mx = (const double*)cmax; auto mn = (const double*)cmin;
-        if (do_dt) hipLaunchKernelGGL((update_kernel<double, true>), g, dim3(BS), 0, s, b, c, stage, f, e, mx, mn, Usub, Ufull, Psub, Uout, Pout, pflag, fflag, fail_list, ds);
-        else hipLaunchKernelGGL((update_kernel<double, false>), g, dim3(BS), 0, s, b, c, stage, f, e, mx, mn, Usub, Ufull, Psub, Uout, Pout, pflag, fflag, fail_list, ds);
+        if (do_dt) { if (ct) KH_UPD(double, true, true); else KH_UPD(double, true, false); }
+        else { if (ct) KH_UPD(double, false, true); else KH_UPD(double, false, false); }
     }
+#undef KH_UPD
     return hipGetLastError();
 }
-hipError_t launch_fixup(const BlockDims& b, const StepConsts& c, int stage, double* U, double* P,
+long update_blocks(const BlockDims& b) { return (ninterior_host(b) + BS - 1) / BS; }
+long divb_blocks(const BlockDims& b) {
+    return ((long)(b.nx1 + 1) * (b.nx2 + 1) * ((b.ndim > 2) ? b.nx3 + 1 : 1) + BS - 1) / BS;
+}
+hipError_t launch_fixup(const BlockDims& b, const StepConsts& c, int stage, double* U, double* P, float* P32,
                         const int* pflag, const int* fail_list, DeviceState* ds, hipStream_t s) {
-    hipLaunchKernelGGL(fixup_list_kernel, dim3(64), dim3(BS), 0, s, b, c, stage, U, P, pflag, fail_list, ds);
+    hipLaunchKernelGGL(fixup_list_kernel, dim3(64), dim3(BS), 0, s, b, c, stage, U, P, P32, pflag, fail_list, ds);
     return hipGetLastError();
 }
-hipError_t launch_finalize(const StepConsts& c, DeviceState* ds, double* dt_hist, int hist_cap, hipStream_t s) {
-    hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(64), 0, s, c, ds, dt_hist, hist_cap);
+hipError_t launch_finalize(const StepConsts& c, DeviceState* ds, const double* dt_part, int n_dt,
+                           const double* divb_part, int n_divb, double* dt_hist, int hist_cap, hipStream_t s) {
+    hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(1024), 0, s, c, ds, dt_part, n_dt, divb_part, n_divb, dt_hist, hist_cap);
     return hipGetLastError();
 }
-hipError_t launch_max_divb(const BlockDims& b, const double* U, DeviceState* ds, hipStream_t s) {
+hipError_t launch_max_divb(const BlockDims& b, const double* U, double* part, hipStream_t s) {
     if (b.ndim < 2) return hipSuccess;
-    const long n = (long)(b.nx1 + 1) * (b.nx2 + 1) * ((b.ndim > 2) ? b.nx3 + 1 : 1);
-    hipLaunchKernelGGL(max_divb_kernel, grid_for(n), dim3(BS), 0, s, b, U, ds);
+    hipLaunchKernelGGL(max_divb_kernel, dim3((unsigned)divb_blocks(b)), dim3(BS), 0, s, b, U, part);
+    return hipGetLastError();
+}
+hipError_t launch_reduce(const double* part, int n, int is_max, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(BS), 0, s, part, n, is_max, out);
     return hipGetLastError();
 }
 hipError_t launch_reset_state(DeviceState* ds, double dt, hipStream_t s) {
     hipLaunchKernelGGL(reset_state_kernel, dim3(1), dim3(64), 0, s, ds, dt);
+    return hipGetLastError();
+}
+hipError_t launch_set_clock(DeviceState* ds, double time, double dt, int ncycle, hipStream_t s) {
+    hipLaunchKernelGGL(set_clock_kernel, dim3(1), dim3(64), 0, s, ds, time, dt, ncycle);
     return hipGetLastError();
 }
 hipError_t launch_weighted_sum(const BlockDims& b, const double* Usub, const double* Ufull, const double* dUdt,
@@ -604,10 +750,14 @@ hipError_t launch_sync_int(const BlockDims& b, int* A, hipStream_t s) {
     return hipGetLastError();
 }
 hipError_t launch_estimate_dt(const BlockDims& b, const StepConsts& c, int prec, const void* cmax, const void* cmin,
-                              DeviceState* ds, hipStream_t s) {
-    const long n = (long)b.nx1 * (b.ndim > 1 ? b.nx2 : 1) * (b.ndim > 2 ? b.nx3 : 1);
-    if (prec == PREC_SPLIT) hipLaunchKernelGGL(estimate_dt_kernel<float>, grid_for(n), dim3(BS), 0, s, b, (const float*)cmax, (const float*)cmin, ds);
-    else hipLaunchKernelGGL(estimate_dt_kernel<double>, grid_for(n), dim3(BS), 0, s, b, (const double*)cmax, (const double*)cmin, ds);
+                              double* part, hipStream_t s) {
+    const dim3 g = grid_for(ninterior_host(b));
+    if (prec == PREC_SPLIT) hipLaunchKernelGGL(estimate_dt_kernel<float>, g, dim3(BS), 0, s, b, (const float*)cmax, (const float*)cmin, part);
+    else hipLaunchKernelGGL(estimate_dt_kernel<double>, g, dim3(BS), 0, s, b, (const double*)cmax, (const double*)cmin, part);
+    return hipGetLastError();
+}
+hipError_t launch_convert_f32(const double* A, float* B, long n, hipStream_t s) {
+    hipLaunchKernelGGL(convert_f32_kernel, grid_for(n), dim3(BS), 0, s, A, B, n);
     return hipGetLastError();
 }
 

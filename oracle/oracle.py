@@ -12,6 +12,8 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
+# Same source compiled with FMA contraction: measures a workload's roundoff sensitivity.
+LIB_FMA_PATH = os.path.join(HERE, "liboracle_fma.so")
 
 NV = 8
 
@@ -38,22 +40,23 @@ class OrParams(ctypes.Structure):
     ]
 
 
-_lib = None
+_libs = {}
 
 
 def build(force=False):
-    """Compile liboracle.so from oracle/kharma_oracle.cpp with the committed Makefile."""
-    if force or not os.path.exists(LIB_PATH) or (
-            os.path.getmtime(LIB_PATH) < os.path.getmtime(os.path.join(HERE, "kharma_oracle.cpp"))):
+    """Compile liboracle.so (and liboracle_fma.so) from oracle/kharma_oracle.cpp with the committed Makefile."""
+    src = os.path.getmtime(os.path.join(HERE, "kharma_oracle.cpp"))
+    stale = any(not os.path.exists(p) or os.path.getmtime(p) < src for p in (LIB_PATH, LIB_FMA_PATH))
+    if force or stale:
         subprocess.run(["make", "-s", "-C", HERE] + (["-B"] if force else []), check=True)
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
+def lib(variant=""):
+    if variant not in _libs:
+        path = LIB_FMA_PATH if variant == "fma" else LIB_PATH
+        if not os.path.exists(path):
             build()
-        L = ctypes.CDLL(LIB_PATH)
+        L = ctypes.CDLL(path)
         vp, d, i, dp = ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.POINTER(ctypes.c_double)
         L.or_create.restype = vp
         L.or_create.argtypes = [ctypes.POINTER(OrParams)]
@@ -82,8 +85,8 @@ def lib():
         L.or_weno5.argtypes = [d, d, d, d, d, dp, dp]
         L.or_reconstruct.argtypes = [i, d, d, d, d, d, dp, dp]
         L.or_num_threads.restype = i
-        _lib = L
-    return _lib
+        _libs[variant] = L
+    return _libs[variant]
 
 
 def _ptr(a):
@@ -136,10 +139,10 @@ def make_params(cfg):
 class OracleBlock:
     """One meshblock advanced by the CPU restatement."""
 
-    def __init__(self, cfg):
+    def __init__(self, cfg, variant=""):
         self.cfg = cfg
         self._p = make_params(cfg)
-        self.L = lib()
+        self.L = lib(variant)
         self.h = self.L.or_create(ctypes.byref(self._p))
         ng = self._p.ng
         nx1, nx2, nx3 = cfg["nx"]

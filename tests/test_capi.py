@@ -44,7 +44,7 @@ def _create(cfg, **over):
     (dict(precision=3), 1),
     (dict(coords=1), 3),             # not implemented -> unsupported, not a crash
     (dict(periodic1=0), 3),
-    (dict(nx1=3), 1),                # periodic axis shorter than the ghost depth
+    (dict(nx1=7), 1),                # periodic axis shorter than twice the ghost depth
 ])
 def test_invalid_params_rejected_on_host(L, over, code):
     st, msg = _create(pr.mhdmode_config(n=16, ndim=3), **over)
