@@ -102,6 +102,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--forms", default="fp64,split", help="comma list of fp64,split (profiling runs use one)")
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="after the runs, copy a buffer of this many MiB on the device: a known-size "
+                         "reference for scaling HBM byte counters (FETCH_SIZE / WRITE_SIZE)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on exactly one GPU; multi-GPU runs are not part of this build",
@@ -116,6 +120,16 @@ def main():
     P0 = pr.mhdmode_prims(cfg)
     nz = a.n ** 3
 
+    forms = a.forms.split(",")
+    if forms != ["fp64", "split"]:
+        # profiling mode: run the requested forms only, print their timings, no JSON contract line
+        for f in forms:
+            r = _bench_form(dict(cfg, precision=f), P0, a.steps, a.warmup)
+            print(json.dumps({"form": f, "ms_per_step": r["ms"] / a.steps,
+                              "zone_cycles_per_s": nz * a.steps / (r["ms"] * 1e-3)}))
+        if a.copy_mb:
+            _calibration_copy(a.copy_mb)
+        return
     r64 = _bench_form(dict(cfg, precision="fp64"), P0, a.steps, a.warmup)
     r32 = _bench_form(dict(cfg, precision="split"), P0, a.steps, a.warmup)
     zc64 = nz * a.steps / (r64["ms"] * 1e-3)
@@ -181,6 +195,20 @@ def main():
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))
+    if a.copy_mb:
+        _calibration_copy(a.copy_mb)
+
+
+def _calibration_copy(mb):
+    """One device-to-device copy of `mb` MiB (reads and writes mb MiB of HBM)."""
+    import torch
+    n = mb * (1 << 20) // 8
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration_copy_mib": mb}))
 
 
 if __name__ == "__main__":
