@@ -52,28 +52,28 @@ def _bench_form(cfg, P0, steps, warmup):
     return dict(ms=ms, kernels=ktimes, scalars=s, P=P)
 
 
-def _algorithmic_bytes(kind, cfg, flux_bytes):
-    """Bytes one launch of `kind` must move at minimum (each array element once)."""
-    nx1, nx2, nx3 = cfg["nx"]
-    ng = cfg.get("ng", 4)
-    nz = nx1 * nx2 * nx3
+# Algorithmic HBM bytes per interior zone for each kernel of the step: the FP64
+# state contract of one RK2 step (U and P in and out, 8 FP64 each) apportioned
+# to the kernel that owns it.  Stage 1 reads P0, U0 and writes U1, P1 (4 x 64 B);
+# stage 2 reads P1, U1, U0 and writes U2, P2 (5 x 64 B): 576 B per zone-cycle.
+# Face fluxes, signal speeds, flags and ghost images are implementation
+# intermediates and count only in the measured traffic.
+STATE_BYTES = {"update_s1": 256, "update_s2": 320}
+STEP_STATE_BYTES = 576
+
+
+def _kernel_io_bytes(kind, prec_bytes):
+    """Bytes per interior zone that `kind` must move as this implementation splits the
+    step (every array it reads or writes touched once; ghost images excluded)."""
+    fb = prec_bytes
     if kind.startswith("flux_x"):
-        d = int(kind[-1])
-        faces = [nx1 + 2, nx2 + 2, nx3 + 2]
-        faces[d - 1] -= 1
-        nfaces = faces[0] * faces[1] * faces[2]
-        cells = list(faces)
-        cells[d - 1] += 5                       # 6-point stencil along d
-        ncells = cells[0] * cells[1] * cells[2]
-        return ncells * 8 * 8 + nfaces * (8 + 2) * flux_bytes
-    if kind.startswith("update"):
-        last = kind.endswith("s2")
-        per_zone = (3 * 8 + 3) * flux_bytes + (8 + 8 + 5) * 8 + (8 + 8) * 8 + 2 * 4
-        if last:
-            per_zone += 6 * flux_bytes
-        return nz * per_zone
-    if kind == "emf":
-        return (nx1 + 1) * (nx2 + 1) * (nx3 + 1) * (6 + 3) * flux_bytes
+        # P (FP32 shadow in the split form) in, 8 fluxes out, + 1 signal speed at stage 2
+        return 8 * fb + 8 * fb + (fb if kind.endswith("s2") else 0)
+    if kind == "update_s1":
+        # 3 x 8 face fluxes, U, 5-var P guess in; U, P, FP32 P shadow, 2 flags out
+        return 24 * fb + 64 + 40 + 64 + 64 + (32 if fb == 4 else 0) + 8
+    if kind == "update_s2":
+        return _kernel_io_bytes("update_s1", fb) + 64 + 3 * fb   # + U0, + 3 signal speeds
     return None
 
 
@@ -146,16 +146,20 @@ def main():
     dom = max(kt, key=lambda k: kt[k][0])
     tot_ms, nl = kt[dom]
     avg_ms = tot_ms / nl
-    algo = _algorithmic_bytes(dom, cfg, 4)
+    per_zone = STATE_BYTES.get(dom)
+    algo = per_zone * nz if per_zone else None
     achieved = (algo / (avg_ms * 1e-3) / 1e9) if algo else None
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"traffic_{dom}_split_{a.n}.json")
+    tfile = os.path.join(ROOT, "profiles", f"hbm_traffic_fastmode{a.n}.json")
     if os.path.exists(tfile):
-        traffic = json.load(open(tfile)).get("hbm_bytes_per_launch")
+        traffic = json.load(open(tfile)).get("kernels", {}).get("split", {}).get(dom, {}).get("hbm_bytes_per_launch")
+    io = _kernel_io_bytes(dom, 4)
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic,
-            "kernel": dom, "avg_launch_ms": avg_ms, "algorithmic_bytes_per_launch": algo}
-    step_bytes = 576 * nz  # FP64 state contract per zone-cycle
+            "kernel": dom, "avg_launch_ms": avg_ms, "algorithmic_bytes_per_launch": algo,
+            "algorithmic_bytes_per_zone": per_zone, "zones_per_launch": nz,
+            "kernel_io_bytes_per_launch": io * nz if io else None}
+    step_bytes = STEP_STATE_BYTES * nz
     kernel_ms = {k: {"total_ms": v[0], "launches": v[1], "avg_ms": (v[0] / v[1]) if v[1] else None}
                  for k, v in r32["kernels"].items()}
     kernel_ms64 = {k: {"total_ms": v[0], "launches": v[1], "avg_ms": (v[0] / v[1]) if v[1] else None}
@@ -181,7 +185,7 @@ def main():
         "split_over_fp64": zc32 / zc64,
         "split_rel_l1_vs_fp64": {"max": float(rel.max()), "per_var": [float(x) for x in rel]},
         "adjoint_over_forward": None,
-        "step_state_roofline": {"bytes_per_zone_cycle": 576, "achieved_GBs_fp64": step_bytes * a.steps / (r64["ms"] * 1e-3) / 1e9,
+        "step_state_roofline": {"bytes_per_zone_cycle": STEP_STATE_BYTES, "achieved_GBs_fp64": step_bytes * a.steps / (r64["ms"] * 1e-3) / 1e9,
                                 "achieved_GBs_split": step_bytes * a.steps / (r32["ms"] * 1e-3) / 1e9,
                                 "peak": HBM_PEAK_GBS},
         "roofline": roof,

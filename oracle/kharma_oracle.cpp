@@ -7,10 +7,14 @@
  * product links or calls this file: only tests/, __graft_entry__.smoke() and
  * bench.py's cpu_baseline leg load it, and only as the checker.
  *
- * Every function names the reference file:line it restates.  Where the
- * reference calls into Parthenon (not vendored in the reference tree), the
- * restated behaviour is noted "parity unpinned" and is pinned instead by the
- * closed-form checks in tests/test_oracle_*.py.
+ * Every function names the reference file:line it restates.  The closed-form
+ * and invariant checks in tests/test_oracle.py pin reconstruction, P->U, the
+ * onedw round trip, signal speeds, LLF, flux-CT (divB invariant), the dt ramp,
+ * the fix-up weights, the drift-floor momentum invariant and the whole step
+ * (linear-mode convergence).  Parity unpinned (restated, checked only end to
+ * end): the onedw iterate sequence and exit tests, the drift-frame floor algebra
+ * beyond its invariant, and the Parthenon pieces the reference tree does not
+ * hold (RK2/VL2 weights, FluxDivHelper summation order, periodic fill).
  *
  * Layout: every field is SoA, [var][k][j][i] over the *entire* block
  * (interior + ng ghosts per side in each active dimension), i fastest,
