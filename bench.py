@@ -1,16 +1,23 @@
 """Benchmark of the GRMHD step on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 128] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload torus|fastmode] [--no-cpu]
 
-Workload: 3-D linear MHD fast eigenmode on the periodic unit cube, 128^3 zones
-in one meshblock (4 ghost zones), gamma = 4/3, WENO5 + LLF, flux-CT, RK2,
-onedw inverter, drift-frame floors -- the precision-form configuration.  Both
-forms run back to back on the same input:
+Default workload (torus): 3-D Fishbone-Moncrief torus around a Kerr black hole,
+a = 0.9375, FMKS coordinates, 192 x 128 x 128 zones in one meshblock (4 ghost
+zones), r_out = 1000, gamma = 13/9, SANE seed field normalised to beta = 100,
+internal energy jittered by +-4 %, WENO5 + LLF, flux-CT, RK2, onedw FP64
+inverter, drift-frame floors, outflow / reflecting / periodic boundaries --
+the precision form at production shape.  `--workload fastmode` runs the 3-D
+linear fast mode on the periodic unit cube at --n^3 (default 128).
+
+Both forms run back to back from the same initial state:
   * plain FP64,
-  * split precision (FP32 reconstruction / fluxes / EMFs over FP64 U, inverter, dt).
-A "step" is one full RK2 step; a zone-cycle is one interior zone advanced by
-one step.  `value` is the split form's zone-cycles/s with the state resident in
-HBM before the timed region starts.
+  * split precision (FP32 reconstruction / fluxes / EMFs over FP64 U, inverter,
+    geometry, dt),
+and the line reports the split form's relative L1 deviation from the FP64 run
+after the same number of steps.  A "step" is one full RK2 step; a zone-cycle
+is one interior zone advanced by one step.  `value` is the split form's
+zone-cycles/s with the state resident in HBM before the timed region starts.
 
 Prints one JSON line.
 """
@@ -77,7 +84,7 @@ def _kernel_io_bytes(kind, prec_bytes):
     return None
 
 
-def _cpu_baseline(cfg, P0, budget_s=12.0, max_steps=8):
+def _cpu_baseline(cfg, P0, what, budget_s=12.0, max_steps=8):
     from oracle import oracle
     oracle.build()
     ob = oracle.OracleBlock(cfg)
@@ -91,7 +98,7 @@ def _cpu_baseline(cfg, P0, budget_s=12.0, max_steps=8):
         n += 1
     dt = time.perf_counter() - t0
     return dict(value=nz * n / dt, unit="zone-cycles/s", cores=oracle.num_threads(), kind="port",
-                sample=f"{n} RK2 steps of the same 3-D fast mode {cfg['nx'][0]}^3 block after 1 warm-up step, "
+                sample=f"{n} RK2 steps of the same {what} after 1 warm-up step, "
                        f"CPU restatement of KHARMA's algorithm (C++ FP64, OpenMP over k,j)")
 
 
@@ -100,7 +107,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--workload", default="torus", choices=["torus", "fastmode"])
+    ap.add_argument("--n", type=int, default=128, help="fastmode: zones per side")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="fp64,split", help="comma list of fp64,split (profiling runs use one)")
     ap.add_argument("--copy-mb", type=int, default=0,
@@ -116,9 +124,22 @@ def main():
     import torch
     from kharma_amd import problems as pr
 
-    cfg = pr.mhdmode_config(n=a.n, ndim=3, mode="fast", one_period=False)
-    P0 = pr.mhdmode_prims(cfg)
-    nz = a.n ** 3
+    if a.workload == "torus":
+        cfg = pr.torus_config(192, 128, 128)
+        P0 = pr.torus_prims(cfg)
+        tag = "torus192"
+        what = "FM torus 192x128x128 block"
+        desc = ("3-D Fishbone-Moncrief torus, Kerr a=0.9375, FMKS, 192x128x128 in 1 meshblock, r_out 1000, "
+                "gamma 13/9, SANE field beta 100, u jitter +-4%, WENO5+LLF, flux-CT, RK2, onedw FP64 inverter, "
+                "drift floors, outflow/reflecting/periodic boundaries")
+    else:
+        cfg = pr.mhdmode_config(n=a.n, ndim=3, mode="fast", one_period=False)
+        P0 = pr.mhdmode_prims(cfg)
+        tag = f"fastmode{a.n}"
+        what = f"3-D fast mode {a.n}^3 block"
+        desc = (f"3-D linear MHD fast mode, periodic unit cube {a.n}^3 in 1 meshblock, gamma 4/3, "
+                "WENO5+LLF, flux-CT, RK2, onedw FP64 inverter, drift floors")
+    nz = cfg["nx"][0] * cfg["nx"][1] * cfg["nx"][2]
 
     forms = a.forms.split(",")
     if forms != ["fp64", "split"]:
@@ -150,7 +171,7 @@ def main():
     algo = per_zone * nz if per_zone else None
     achieved = (algo / (avg_ms * 1e-3) / 1e9) if algo else None
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"hbm_traffic_fastmode{a.n}.json")
+    tfile = os.path.join(ROOT, "profiles", f"hbm_traffic_{tag}.json")
     if os.path.exists(tfile):
         traffic = json.load(open(tfile)).get("kernels", {}).get("split", {}).get(dom, {}).get("hbm_bytes_per_launch")
     io = _kernel_io_bytes(dom, 4)
@@ -178,9 +199,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32/f64",
         "data": "synthetic",
-        "config": {"workload": f"3-D linear MHD fast mode, periodic unit cube {a.n}^3 in 1 meshblock, gamma 4/3, "
-                               "WENO5+LLF, flux-CT, RK2, onedw FP64 inverter, drift floors",
-                   "zones": nz, "ghosts": cfg["ng"], "forms": ["fp64", "split"]},
+        "config": {"workload": desc, "zones": nz, "ghosts": cfg["ng"], "forms": ["fp64", "split"]},
         "fp64": {"value": zc64, "ms_per_step": r64["ms"] / a.steps, "dtype": "f64"},
         "split_over_fp64": zc32 / zc64,
         "split_rel_l1_vs_fp64": {"max": float(rel.max()), "per_var": [float(x) for x in rel]},
@@ -195,7 +214,7 @@ def main():
                      "divb_fp64": r64["scalars"]["divb"], "divb_split": r32["scalars"]["divb"]},
     }
     if not a.no_cpu:
-        out["cpu_baseline"] = _cpu_baseline(cfg, P0)
+        out["cpu_baseline"] = _cpu_baseline(cfg, P0, what)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))

@@ -44,6 +44,16 @@ extern "C" {
 #define KH_RECON_WENO5 2
 #define KH_RECON_PPM 3
 
+#define KH_COORDS_CARTESIAN_MINKOWSKI 0
+#define KH_COORDS_KS 1                 /* spherical Kerr-Schild base */
+
+#define KH_TRANSFORM_NULL 0
+#define KH_TRANSFORM_MKS 1             /* "modified" */
+#define KH_TRANSFORM_FMKS 2            /* "funky" */
+
+#define KH_BC_OUTFLOW 1
+#define KH_BC_REFLECT 2
+
 /* Run parameters of one meshblock: the .par options this path reads
  * (GRMHD/*, flux/*, floors/*, inverter/*, parthenon/time/*, parthenon/mesh/*). */
 typedef struct kh_params {
@@ -64,8 +74,19 @@ typedef struct kh_params {
     double err_tol;                 /* inverter/err_tol */
     int fix_average, fix_atmo;      /* inverter/fix_average_neighbors, fix_atmosphere */
     int integrator;                 /* 0 rk2, 1 vl2 (parthenon/time/integrator) */
-    int coords;                     /* 0 cartesian_minkowski */
+    int coords;                     /* KH_COORDS_* (coordinates/base) */
     int precision;                  /* KH_PREC_FP64 or KH_PREC_SPLIT */
+    /* Kerr-Schild geometry (coordinates/a, transform, hslope, mks_smooth, poly_xt, poly_alpha,
+     * fmks_zero_point); x1 is log(r) and x2 the stretched polar angle for MKS/FMKS. */
+    double a;
+    int transform;                  /* KH_TRANSFORM_* */
+    double hslope, mks_smooth, poly_xt, poly_alpha, fmks_zero_point;
+    /* Physical boundaries of non-periodic x1/x2 faces (boundaries/inner_x1 ...): KH_BC_* */
+    int bc_inner_x1, bc_outer_x1, bc_inner_x2, bc_outer_x2;
+    int check_inflow_inner_x1, check_inflow_outer_x1;  /* boundaries/check_inflow_* */
+    int zero_polar_flux;            /* boundaries/zero_polar_flux */
+    int fix_polar_flux;             /* b_field/fix_polar_flux */
+    int fix_corner_inner;           /* boundaries/fix_corner_inner */
 } kh_params;
 
 typedef struct kh_block kh_block;
@@ -91,12 +112,19 @@ int kh_block_shape(const kh_block* b, int* n1, int* n2, int* n3, long* nc);
  * i.e. X1: i in [is, ie+1], j in [js-1, je+1], k in [ks-1, ke+1] (cyclic for X2/X3). */
 int kh_get_flux(kh_block* b, int dir, const double* P, void* F_dir, void* cmax_dir, void* cmin_dir);
 
-/* B_FluxCT::FluxCT (b_flux_ct/b_flux_ct.cpp:307-370): corner EMFs into emf[3][nc]
- * and the rewritten B fluxes in F[3][8][nc]. */
+/* Packages::FixFlux (kharma_package.cpp:43-56): B_FluxCT::FixFlux
+ * (b_flux_ct/b_flux_ct.cpp:264-370: polar ZeroBoundaryFlux, then FluxCT) and
+ * KBoundaries::FixFlux (boundaries/boundaries.cpp:611-690: no mass inflow through
+ * x1 faces, zero flux through polar faces).  Corner EMFs into emf[3][nc] and
+ * the rewritten fluxes in F[3][8][nc]. */
 int kh_flux_ct(kh_block* b, void* F, void* emf);
 
 /* KHARMADriver::FluxDivergence (driver/kharma_driver.hpp:208-231): dUdt[8][nc] on the interior. */
 int kh_flux_divergence(kh_block* b, const void* F, double* dUdt);
+
+/* Flux::AddGeoSource (flux/flux.cpp:395-444): dUdt += sqrt(-g) T^mu_nu Gamma terms
+ * on the interior, from the primitives P[8][nc] (no-op in Cartesian Minkowski). */
+int kh_add_geo_source(kh_block* b, const double* P, double* dUdt);
 
 /* Update::WeightedSumData pair of KHARMADriver::AddStateUpdate
  * (driver/kharma_driver.cpp:339-387), entire block:
@@ -118,6 +146,16 @@ int kh_fix_utop(kh_block* b, double* U, double* P, const int* pflag);
 /* Flux::MeshPtoU (flux/flux.cpp:276-322), entire block. */
 int kh_ptou(kh_block* b, const double* P, double* U);
 
+/* Physical boundaries of a block with outflow / reflecting x1 or x2 faces: the
+ * ghost zones of those faces in U and P[8][nc] from the interior, as
+ * KBoundaries::ApplyBoundary leaves them (boundaries/boundaries.cpp:369-589):
+ * outflow or reflecting fill, check_inflow on x1 faces, B = U.B / sqrt(-g) and
+ * the fluid conserved variables from P.  Faces in the order inner/outer x1,
+ * inner/outer x2, then inner x1 again when fix_corner_inner.  The result is the
+ * state after the stage's repeated boundary passes (kharma_step.cpp:240-314),
+ * whose fluid conserved variables use the updated ghost B. */
+int kh_apply_bounds(kh_block* b, double* U, double* P);
+
 /* Boundary exchange of nvar cell variables of a single periodic block with itself
  * (the parthenon exchange behind KHARMADriver::AddBoundarySync, kharma_driver.cpp:138-183). */
 int kh_sync_bounds(kh_block* b, double* A, int nvar);
@@ -129,6 +167,37 @@ int kh_estimate_timestep(kh_block* b, const void* cmax, const void* cmin, double
 
 /* B_FluxCT::MaxDivB (b_flux_ct.cpp:566-607).  Synchronises the stream. */
 int kh_max_divb(kh_block* b, const double* U, double* out);
+
+/* ---- Problem set-up and geometry (host only; no device work) -------------- */
+
+/* Fishbone-Moncrief torus with a SANE seed field (torus/rin, rmax, kappa;
+ * perturbation/u_jitter, rng_seed; b_field/beta_min, min_A). */
+typedef struct kh_torus_params {
+    double rin, rmax, kappa;
+    double u_jitter;                /* u *= 1 + U(-u_jitter/2, u_jitter/2) */
+    int rng_seed;                   /* std::mt19937 seed (31337 + block gid in the reference) */
+    double beta_min;                /* max(P_gas) / max(P_B) after normalisation */
+    double min_A;
+} kh_torus_params;
+
+/* Initial primitives of the torus problem into the host array P[8][nc]:
+ * InitializeFMTorus (prob/fm_torus.cpp:41-197) with its initial floors, PerturbU
+ * (prob/utils/perturbation.hpp:49-92, serial path), SeedBField<sane> with the
+ * corner-averaged curl and NormalizeBField (prob/seed_B.cpp:81-489).  Interior
+ * zones only; kh_driver_init_prims fills the ghosts. */
+int kh_torus_prims(const kh_params* p, const kh_torus_params* t, double* P);
+
+/* One component of the geometry cache of the block `p` describes
+ * (GRCoordinates cache, coordinates/gr_coordinates.cpp:96-205), evaluated on the
+ * host into out[(n2+1)*(n1+1)] (j-major).  loc: 0 centre, 1 X1 face, 2 X2 face,
+ * 3 corner.  comp: 0..9 gcov (00 01 02 03 11 12 13 22 23 33), 10..19 gcon, 20 gdet,
+ * 21 lapse, 22..85 gdet*Gamma^mu_{nu lam} at centres (index 22 + 16 mu + 4 nu + lam),
+ * 86/87 geometric rho/u floors at centres, 88 radius of the centre. */
+#define KH_GEOM_CONN 22
+#define KH_GEOM_FLOOR_RHO 86
+#define KH_GEOM_FLOOR_U 87
+#define KH_GEOM_RADIUS 88
+int kh_geometry_host(const kh_params* p, int loc, int comp, double* out);
 
 /* ---- Stage driver --------------------------------------------------------
  * KHARMADriver::MakeDefaultTaskCollection (driver/kharma_step.cpp:77-319) for a
@@ -148,6 +217,9 @@ int kh_driver_step(kh_driver* d, int nsteps);
 int kh_driver_set_clock(kh_driver* d, double time, double dt, int ncycle);
 /* Copy the base-container state out (device pointers; either may be NULL). */
 int kh_driver_get_state(kh_driver* d, double* P, double* U);
+/* Inversion / floor flags of the last stage, int32 [nc] device arrays (either may be NULL):
+ * the reference's pflag / fflag fields (inverter/inverter.cpp:104-108, floors/floors.cpp:111-116). */
+int kh_driver_get_flags(kh_driver* d, int* pflag, int* fflag);
 /* Synchronises; any output may be NULL. */
 int kh_driver_get_scalars(kh_driver* d, double* time, double* dt, double* last_divb, int* ncycle, int* total_fails);
 /* dt taken by each of the first n steps since init (host array). Synchronises. */
@@ -158,7 +230,7 @@ int kh_driver_arrays(kh_driver* d, double** U, double** P);
 /* Per-kernel timing with HIP events recorded on the driver's stream around every
  * launch of kh_driver_step while enabled (no counterpart in the reference; it
  * plays the role of the Kokkos-tools kernel timer hooked in run.sh:11-91).
- * Kernel kinds: 0..2 face fluxes X1..X3, 3 corner EMFs, 4 stage-1 update,
+ * Kernel kinds: 0..2 face fluxes X1..X3, 3 physical-boundary flux fixes and ghost fills, 4 stage-1 update,
  * 5 stage-2 update, 6 inversion fix-up, 7 divB, 8 timestep finalize. */
 #define KH_NKERNEL_KINDS 9
 int kh_driver_set_kernel_timing(kh_driver* d, int enable);

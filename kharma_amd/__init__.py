@@ -23,6 +23,9 @@ PREC_FP64 = 0
 PREC_SPLIT = 1
 RECON = {"donor_cell_c": 0, "linear_mc": 1, "weno5": 2, "ppm": 3}
 FRAME = {"fluid": 0, "normal": 1, "drift": 2}
+COORDS = {"cartesian_minkowski": 0, "ks": 1}
+TRANSFORM = {"null": 0, "mks": 1, "fmks": 2}
+BC = {"periodic": 0, "outflow": 1, "reflecting": 2}
 
 
 class KhParams(ctypes.Structure):
@@ -45,6 +48,21 @@ class KhParams(ctypes.Structure):
         ("iter_max", ctypes.c_int), ("err_tol", ctypes.c_double),
         ("fix_average", ctypes.c_int), ("fix_atmo", ctypes.c_int),
         ("integrator", ctypes.c_int), ("coords", ctypes.c_int), ("precision", ctypes.c_int),
+        ("a", ctypes.c_double), ("transform", ctypes.c_int),
+        ("hslope", ctypes.c_double), ("mks_smooth", ctypes.c_double), ("poly_xt", ctypes.c_double),
+        ("poly_alpha", ctypes.c_double), ("fmks_zero_point", ctypes.c_double),
+        ("bc_inner_x1", ctypes.c_int), ("bc_outer_x1", ctypes.c_int),
+        ("bc_inner_x2", ctypes.c_int), ("bc_outer_x2", ctypes.c_int),
+        ("check_inflow_inner_x1", ctypes.c_int), ("check_inflow_outer_x1", ctypes.c_int),
+        ("zero_polar_flux", ctypes.c_int), ("fix_polar_flux", ctypes.c_int), ("fix_corner_inner", ctypes.c_int),
+    ]
+
+
+class KhTorusParams(ctypes.Structure):
+    _fields_ = [
+        ("rin", ctypes.c_double), ("rmax", ctypes.c_double), ("kappa", ctypes.c_double),
+        ("u_jitter", ctypes.c_double), ("rng_seed", ctypes.c_int),
+        ("beta_min", ctypes.c_double), ("min_A", ctypes.c_double),
     ]
 
 
@@ -83,11 +101,15 @@ def lib():
         "kh_flux_ct": (i, [vp, vp, vp]),
         "kh_flux_divergence": (i, [vp, vp, vp]),
         "kh_weighted_sum": (i, [vp, vp, vp, vp, d, d, d, vp]),
+        "kh_add_geo_source": (i, [vp, vp, vp]),
         "kh_utop": (i, [vp, vp, vp, vp, vp]),
         "kh_apply_floors": (i, [vp, vp, vp, vp, vp]),
         "kh_fix_utop": (i, [vp, vp, vp, vp]),
         "kh_ptou": (i, [vp, vp, vp]),
         "kh_sync_bounds": (i, [vp, vp, i]),
+        "kh_apply_bounds": (i, [vp, vp, vp]),
+        "kh_torus_prims": (i, [ctypes.POINTER(KhParams), ctypes.POINTER(KhTorusParams), vp]),
+        "kh_geometry_host": (i, [ctypes.POINTER(KhParams), i, i, vp]),
         "kh_estimate_timestep": (i, [vp, vp, vp, d, ctypes.POINTER(d)]),
         "kh_max_divb": (i, [vp, vp, ctypes.POINTER(d)]),
         "kh_driver_create": (i, [ctypes.POINTER(KhParams), pvp]),
@@ -100,6 +122,7 @@ def lib():
         "kh_driver_get_scalars": (i, [vp, ctypes.POINTER(d), ctypes.POINTER(d), ctypes.POINTER(d),
                                       ctypes.POINTER(i), ctypes.POINTER(i)]),
         "kh_driver_get_dt_history": (i, [vp, ctypes.POINTER(d), i]),
+        "kh_driver_get_flags": (i, [vp, vp, vp]),
         "kh_driver_block": (i, [vp, pvp]),
         "kh_driver_arrays": (i, [vp, pvp, pvp]),
         "kh_driver_set_kernel_timing": (i, [vp, i]),
@@ -124,7 +147,19 @@ def make_params(cfg):
     p.nx1, p.nx2, p.nx3 = cfg["nx"]
     p.ng = cfg.get("ng", 4)
     (p.x1min, p.x1max), (p.x2min, p.x2max), (p.x3min, p.x3max) = cfg["bounds"]
-    p.periodic1 = p.periodic2 = p.periodic3 = 1
+    bcs = cfg.get("boundaries", {})
+    p.bc_inner_x1 = BC[bcs.get("inner_x1", "periodic")]
+    p.bc_outer_x1 = BC[bcs.get("outer_x1", "periodic")]
+    p.bc_inner_x2 = BC[bcs.get("inner_x2", "periodic")]
+    p.bc_outer_x2 = BC[bcs.get("outer_x2", "periodic")]
+    p.periodic1 = int(p.bc_inner_x1 == 0)
+    p.periodic2 = int(p.bc_inner_x2 == 0)
+    p.periodic3 = 1
+    p.check_inflow_inner_x1 = int(cfg.get("check_inflow_inner_x1", False))
+    p.check_inflow_outer_x1 = int(cfg.get("check_inflow_outer_x1", False))
+    p.zero_polar_flux = int(cfg.get("zero_polar_flux", False))
+    p.fix_polar_flux = int(cfg.get("fix_polar_flux", False))
+    p.fix_corner_inner = int(cfg.get("fix_corner_inner", False))
     p.gam = cfg["gamma"]
     p.cfl = cfg.get("cfl", 0.9)
     p.dt_min = cfg.get("dt_min", 1e-5)
@@ -154,9 +189,48 @@ def make_params(cfg):
     p.fix_average = int(cfg.get("fix_average", True))
     p.fix_atmo = int(cfg.get("fix_atmo", True))
     p.integrator = {"rk2": 0, "vl2": 1}[cfg.get("integrator", "rk2")]
-    p.coords = {"cartesian_minkowski": 0}[cfg.get("coords", "cartesian_minkowski")]
+    p.coords = COORDS[cfg.get("coords", "cartesian_minkowski")]
     p.precision = {"fp64": PREC_FP64, "split": PREC_SPLIT}[cfg.get("precision", "fp64")]
+    p.a = cfg.get("a", 0.0)
+    p.transform = TRANSFORM[cfg.get("transform", "null")]
+    p.hslope = cfg.get("hslope", 0.3)
+    p.mks_smooth = cfg.get("mks_smooth", 0.5)
+    p.poly_xt = cfg.get("poly_xt", 0.82)
+    p.poly_alpha = cfg.get("poly_alpha", 14.0)
+    p.fmks_zero_point = cfg.get("fmks_zero_point", 0.0)
     return p
+
+
+GEOM_LOCI = {"center": 0, "face1": 1, "face2": 2, "corner": 3}
+
+
+def geometry_host(cfg, loc, comp):
+    """One component of the block's geometry cache, computed on the host
+    ([n2+1, n1+1] float64; see kh_geometry_host for the component numbering)."""
+    import numpy as np
+    p = make_params(cfg)
+    n1 = p.nx1 + 2 * p.ng
+    n2 = p.nx2 + 2 * p.ng if p.nx2 > 1 else 1
+    out = np.empty((n2 + 1, n1 + 1))
+    _check(lib().kh_geometry_host(ctypes.byref(p), GEOM_LOCI.get(loc, loc), comp,
+                                  ctypes.c_void_p(out.ctypes.data)))
+    return out
+
+
+def torus_prims(cfg):
+    """Initial primitives of the torus problem (host float64 [8, n3, n2, n1], interior filled)."""
+    import numpy as np
+    p = make_params(cfg)
+    t = KhTorusParams()
+    tc = cfg["torus"]
+    t.rin, t.rmax, t.kappa = tc["rin"], tc["rmax"], tc["kappa"]
+    t.u_jitter, t.rng_seed = tc["u_jitter"], tc["rng_seed"]
+    t.beta_min, t.min_A = tc["beta_min"], tc["min_A"]
+    ng = p.ng
+    shape = (p.nx3 + 2 * ng if p.nx3 > 1 else 1, p.nx2 + 2 * ng if p.nx2 > 1 else 1, p.nx1 + 2 * ng)
+    P = np.zeros((NV,) + shape)
+    _check(lib().kh_torus_prims(ctypes.byref(p), ctypes.byref(t), ctypes.c_void_p(P.ctypes.data)))
+    return P
 
 
 def _torch():
@@ -223,6 +297,10 @@ class Block:
         self._bind()
         _check(lib().kh_flux_divergence(self.h, _ptr(F), _ptr(dUdt)))
 
+    def add_geo_source(self, P, dUdt):
+        self._bind()
+        _check(lib().kh_add_geo_source(self.h, _ptr(P), _ptr(dUdt)))
+
     def weighted_sum(self, Usub, Ufull, dUdt, gam0, gam1, beta_dt, Uout):
         self._bind()
         _check(lib().kh_weighted_sum(self.h, _ptr(Usub), _ptr(Ufull), _ptr(dUdt), gam0, gam1, beta_dt, _ptr(Uout)))
@@ -246,6 +324,10 @@ class Block:
     def sync_bounds(self, A, nvar):
         self._bind()
         _check(lib().kh_sync_bounds(self.h, _ptr(A), nvar))
+
+    def apply_bounds(self, U, P):
+        self._bind()
+        _check(lib().kh_apply_bounds(self.h, _ptr(U), _ptr(P)))
 
     def estimate_timestep(self, cmax, cmin, dt_last):
         self._bind()
@@ -309,6 +391,15 @@ class Driver:
         _check(lib().kh_driver_get_state(self.h, _ptr(P), _ptr(U)))
         return P, U
 
+    def flags(self):
+        """(pflag, fflag) of the last stage as int32 device tensors [n3, n2, n1]."""
+        torch = _torch()
+        pf = torch.empty(self.shape, dtype=torch.int32, device="cuda")
+        ff = torch.empty_like(pf)
+        self._bind()
+        _check(lib().kh_driver_get_flags(self.h, _ptr(pf), _ptr(ff)))
+        return pf, ff
+
     def scalars(self):
         t, dt, divb = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
         nc, nf = ctypes.c_int(), ctypes.c_int()
@@ -317,7 +408,7 @@ class Driver:
                                            ctypes.byref(nc), ctypes.byref(nf)))
         return dict(time=t.value, dt=dt.value, divb=divb.value, ncycle=nc.value, fails=nf.value)
 
-    KERNEL_KINDS = ("flux_x1", "flux_x2", "flux_x3", "emf", "update_s1", "update_s2", "fixup", "divb", "finalize")
+    KERNEL_KINDS = ("flux_x1", "flux_x2", "flux_x3", "bounds", "update_s1", "update_s2", "fixup", "divb", "finalize")
 
     def set_kernel_timing(self, enable):
         _check(lib().kh_driver_set_kernel_timing(self.h, int(bool(enable))))

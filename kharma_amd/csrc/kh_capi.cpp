@@ -25,6 +25,12 @@ int fail(int code, const std::string& msg) {
     return code;
 }
 
+}  // namespace
+
+int kh::api_fail(int code, const std::string& msg) { return fail(code, msg); }
+
+namespace {
+
 #define KH_HIP(call)                                                                         \
     do {                                                                                      \
         hipError_t e_ = (call);                                                               \
@@ -37,6 +43,8 @@ struct kh_block {
     kh_params p;
     BlockDims dims;
     StepConsts consts;
+    GeomDev geo;                     // device geometry cache (curvilinear metrics only)
+    std::vector<void*> geo_bufs;     // owned device allocations behind geo
     hipStream_t stream = nullptr;
     DeviceState* scratch = nullptr;  // reductions of the task-level entry points
     double* partials = nullptr;      // one value per work-group of a reduction
@@ -74,16 +82,31 @@ struct kh_driver {
 
 namespace {
 
-int validate(const kh_params* p) {
+bool bc_ok(int t) { return t == KH_BC_OUTFLOW || t == KH_BC_REFLECT; }
+
+}  // namespace
+
+int kh::check_params(const kh_params* p) {
     if (!p) return fail(KH_ERR_INVALID, "null params");
     if (p->nx1 < 1 || p->nx2 < 1 || p->nx3 < 1) return fail(KH_ERR_INVALID, "zone counts must be >= 1");
     if (p->ng < 3) return fail(KH_ERR_INVALID, "need at least 3 ghost zones for a 5-point reconstruction");
     if (p->nx3 > 1 && p->nx2 == 1) return fail(KH_ERR_INVALID, "3D blocks need nx2 > 1");
-    if (p->coords != 0) return fail(KH_ERR_UNSUPPORTED, "only cartesian_minkowski coordinates are implemented");
-    if (!p->periodic1 || (p->nx2 > 1 && !p->periodic2) || (p->nx3 > 1 && !p->periodic3))
-        return fail(KH_ERR_UNSUPPORTED, "only periodic boundaries are implemented");
-    if (p->nx1 < 2 * p->ng || (p->nx2 > 1 && p->nx2 < 2 * p->ng) || (p->nx3 > 1 && p->nx3 < 2 * p->ng))
-        return fail(KH_ERR_INVALID, "periodic dimensions need at least 2*ng interior zones");
+    if (p->coords != KH_COORDS_CARTESIAN_MINKOWSKI && p->coords != KH_COORDS_KS)
+        return fail(KH_ERR_UNSUPPORTED, "coordinates must be cartesian_minkowski or spherical Kerr-Schild");
+    if (p->coords == KH_COORDS_KS) {
+        if (p->transform < KH_TRANSFORM_NULL || p->transform > KH_TRANSFORM_FMKS) return fail(KH_ERR_INVALID, "unknown coordinate transform");
+        if (!(p->a >= 0. && p->a < 1.)) return fail(KH_ERR_INVALID, "black hole spin must be in [0, 1)");
+    }
+    if (p->nx3 > 1 && !p->periodic3) return fail(KH_ERR_UNSUPPORTED, "x3 boundaries must be periodic");
+    if (!p->periodic1 && (!bc_ok(p->bc_inner_x1) || !bc_ok(p->bc_outer_x1)))
+        return fail(KH_ERR_INVALID, "non-periodic x1 needs outflow or reflecting faces");
+    if (p->nx2 > 1 && !p->periodic2 && (!bc_ok(p->bc_inner_x2) || !bc_ok(p->bc_outer_x2)))
+        return fail(KH_ERR_INVALID, "non-periodic x2 needs outflow or reflecting faces");
+    if (p->nx2 == 1 && !p->periodic2) return fail(KH_ERR_UNSUPPORTED, "a 1D block is periodic in x2");
+    auto short_axis = [&](int nx, int periodic) { return nx > 1 && nx < (periodic ? 2 * p->ng : p->ng); };
+    if (short_axis(p->nx1, p->periodic1) || short_axis(p->nx2, p->periodic2) || short_axis(p->nx3, p->periodic3))
+        return fail(KH_ERR_INVALID, "each axis needs nx >= 2 ng zones (periodic) or nx >= ng (physical faces)");
+    if (p->nx1 == 1) return fail(KH_ERR_INVALID, "nx1 must exceed 1");
     if (p->recon < 0 || p->recon > 3) return fail(KH_ERR_INVALID, "unknown reconstruction");
     if (p->precision != KH_PREC_FP64 && p->precision != KH_PREC_SPLIT) return fail(KH_ERR_INVALID, "unknown precision");
     if (p->integrator != 0 && p->integrator != 1) return fail(KH_ERR_INVALID, "unknown integrator");
@@ -94,7 +117,7 @@ int validate(const kh_params* p) {
     return KH_OK;
 }
 
-void fill_dims(const kh_params* p, BlockDims& b) {
+void kh::make_dims(const kh_params* p, BlockDims& b) {
     b.ndim = (p->nx3 > 1) ? 3 : ((p->nx2 > 1) ? 2 : 1);
     b.ng = p->ng;
     b.nx1 = p->nx1; b.nx2 = p->nx2; b.nx3 = p->nx3;
@@ -104,14 +127,29 @@ void fill_dims(const kh_params* p, BlockDims& b) {
     b.is = p->ng; b.ie = p->ng + p->nx1 - 1;
     b.js = (b.ndim > 1) ? p->ng : 0; b.je = (b.ndim > 1) ? p->ng + p->nx2 - 1 : 0;
     b.ks = (b.ndim > 2) ? p->ng : 0; b.ke = (b.ndim > 2) ? p->ng + p->nx3 - 1 : 0;
-    b.periodic[0] = p->periodic1; b.periodic[1] = p->periodic2; b.periodic[2] = p->periodic3;
+    b.periodic[0] = p->periodic1; b.periodic[1] = (b.ndim > 1) ? p->periodic2 : 1; b.periodic[2] = 1;
     b.s2 = b.n1;
     b.s3 = (long)b.n1 * b.n2;
     b.nc = (long)b.n1 * b.n2 * b.n3;
     b.dx1 = (p->x1max - p->x1min) / p->nx1;
     b.dx2 = (p->x2max - p->x2min) / p->nx2;
     b.dx3 = (p->x3max - p->x3min) / p->nx3;
+    b.geo = (p->coords == KH_COORDS_KS);
+    b.bc[0] = b.periodic[0] ? 0 : p->bc_inner_x1;
+    b.bc[1] = b.periodic[0] ? 0 : p->bc_outer_x1;
+    b.bc[2] = b.periodic[1] ? 0 : p->bc_inner_x2;
+    b.bc[3] = b.periodic[1] ? 0 : p->bc_outer_x2;
+    b.check_inflow[0] = !b.periodic[0] && p->check_inflow_inner_x1;
+    b.check_inflow[1] = !b.periodic[0] && p->check_inflow_outer_x1;
+    b.zero_polar_flux = !b.periodic[1] && p->zero_polar_flux;
+    b.fix_polar_flux = !b.periodic[1] && p->fix_polar_flux;
+    b.fix_corner_inner = !b.periodic[0] && !b.periodic[1] && p->fix_corner_inner;
 }
+
+namespace {
+
+int validate(const kh_params* p) { return check_params(p); }
+void fill_dims(const kh_params* p, BlockDims& b) { make_dims(p, b); }
 
 void fill_consts(const kh_params* p, StepConsts& c) {
     c.gam = p->gam; c.cfl = p->cfl; c.dt_min = p->dt_min; c.max_dt_increase = p->max_dt_increase; c.tlim = p->tlim;
@@ -139,6 +177,52 @@ void fill_consts(const kh_params* p, StepConsts& c) {
 
 size_t flux_elem(const kh_block* b) { return b->p.precision == KH_PREC_SPLIT ? sizeof(float) : sizeof(double); }
 
+// Build the block's metric cache on the host and copy it to the device: FP64
+// tables for the centre and the X1/X2 faces (the flux kernels of the split form
+// read FP32 copies), sqrt(-g) Gamma at centres with a mask of its non-zero
+// components, and the geometric floors.
+hipError_t upload_geometry(kh_block* b) {
+    GeomHost G;
+    build_geometry(b->p, b->dims, G);
+    GeomDev& d = b->geo;
+    d.gn1 = G.gn1;
+    d.np = G.np;
+    auto up = [&](const void* src, size_t bytes, void** dst) -> hipError_t {
+        hipError_t e = hipMalloc(dst, bytes);
+        if (e != hipSuccess) return e;
+        b->geo_bufs.push_back(*dst);
+        return hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice);
+    };
+    const size_t ntab = (size_t)GEO_NCOMP * G.np;
+    for (int loc = 0; loc < 3; ++loc) {
+        void* q = nullptr;
+        hipError_t e = up(G.g[loc].data(), sizeof(double) * ntab, &q);
+        if (e != hipSuccess) return e;
+        d.g[loc] = (const double*)q;
+        std::vector<float> f(G.g[loc].begin(), G.g[loc].end());
+        e = up(f.data(), sizeof(float) * ntab, &q);
+        if (e != hipSuccess) return e;
+        d.g32[loc] = (const float*)q;
+    }
+    unsigned long long mask = 0;
+    for (int n = 0; n < 64; ++n)
+        for (long q = 0; q < G.np; ++q)
+            if (G.conn[(size_t)n * G.np + q] != 0.) { mask |= 1ull << n; break; }
+    d.conn_mask = mask;
+    void* q = nullptr;
+    hipError_t e = up(G.conn.data(), sizeof(double) * G.conn.size(), &q);
+    if (e != hipSuccess) return e;
+    d.conn = (const double*)q;
+    e = up(G.flr.data(), sizeof(double) * G.flr.size(), &q);
+    if (e != hipSuccess) return e;
+    d.flr = (const double*)q;
+    std::vector<float> f32(G.flr.begin(), G.flr.end());
+    e = up(f32.data(), sizeof(float) * f32.size(), &q);
+    if (e != hipSuccess) return e;
+    d.flr32 = (const float*)q;
+    return hipSuccess;
+}
+
 }  // namespace
 
 extern "C" {
@@ -158,10 +242,10 @@ int kh_block_create(const kh_params* p, kh_block** out) {
     b->npartials = std::max(update_blocks(b->dims), divb_blocks(b->dims)) + 1;
     hipError_t e = hipMalloc(&b->scratch, sizeof(DeviceState));
     if (e == hipSuccess) e = hipMalloc(&b->partials, sizeof(double) * b->npartials);
+    if (e == hipSuccess && b->dims.geo) e = upload_geometry(b);
     if (e != hipSuccess) {
-        if (b->scratch) (void)hipFree(b->scratch);
-        delete b;
-        return fail(KH_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+        kh_block_destroy(b);
+        return fail(KH_ERR_HIP, std::string("device allocation / copy: ") + hipGetErrorString(e));
     }
     *out = b;
     return KH_OK;
@@ -171,6 +255,7 @@ void kh_block_destroy(kh_block* b) {
     if (!b) return;
     if (b->scratch) (void)hipFree(b->scratch);
     if (b->partials) (void)hipFree(b->partials);
+    for (void* q : b->geo_bufs) (void)hipFree(q);
     delete b;
 }
 
@@ -192,12 +277,13 @@ int kh_block_shape(const kh_block* b, int* n1, int* n2, int* n3, long* nc) {
 int kh_get_flux(kh_block* b, int dir, const double* P, void* F_dir, void* cmax_dir, void* cmin_dir) {
     if (!b || !P || !F_dir || !cmax_dir || !cmin_dir) return fail(KH_ERR_INVALID, "null argument");
     if (dir < 1 || dir > 3) return fail(KH_ERR_INVALID, "dir must be 1, 2 or 3");
-    KH_HIP(launch_flux(b->dims, b->consts, dir, b->p.recon, b->p.precision, P, F_dir, cmax_dir, cmin_dir, b->stream));
+    KH_HIP(launch_flux(b->dims, b->consts, b->geo, dir, b->p.recon, b->p.precision, P, F_dir, cmax_dir, cmin_dir, b->stream));
     return KH_OK;
 }
 
 int kh_flux_ct(kh_block* b, void* F, void* emf) {
     if (!b || !F || !emf) return fail(KH_ERR_INVALID, "null argument");
+    KH_HIP(launch_fix_flux(b->dims, b->p.precision, F, b->stream));
     KH_HIP(launch_emf(b->dims, b->p.precision, F, emf, b->stream));
     KH_HIP(launch_flux_ct_rewrite(b->dims, b->p.precision, F, emf, b->stream));
     return KH_OK;
@@ -206,6 +292,12 @@ int kh_flux_ct(kh_block* b, void* F, void* emf) {
 int kh_flux_divergence(kh_block* b, const void* F, double* dUdt) {
     if (!b || !F || !dUdt) return fail(KH_ERR_INVALID, "null argument");
     KH_HIP(launch_flux_divergence(b->dims, b->p.precision, F, dUdt, b->stream));
+    return KH_OK;
+}
+
+int kh_add_geo_source(kh_block* b, const double* P, double* dUdt) {
+    if (!b || !P || !dUdt) return fail(KH_ERR_INVALID, "null argument");
+    KH_HIP(launch_geo_source(b->dims, b->consts, b->geo, P, dUdt, b->stream));
     return KH_OK;
 }
 
@@ -218,25 +310,31 @@ int kh_weighted_sum(kh_block* b, const double* Usub, const double* Ufull, const 
 
 int kh_utop(kh_block* b, const double* U, double* P, int* pflag, int* fflag) {
     if (!b || !U || !P || !pflag || !fflag) return fail(KH_ERR_INVALID, "null argument");
-    KH_HIP(launch_utop(b->dims, b->consts, U, P, pflag, fflag, b->stream));
+    KH_HIP(launch_utop(b->dims, b->consts, b->geo, U, P, pflag, fflag, b->stream));
     return KH_OK;
 }
 
 int kh_apply_floors(kh_block* b, double* U, double* P, int* pflag, int* fflag) {
     if (!b || !U || !P || !pflag || !fflag) return fail(KH_ERR_INVALID, "null argument");
-    KH_HIP(launch_floors(b->dims, b->consts, U, P, pflag, fflag, b->stream));
+    KH_HIP(launch_floors(b->dims, b->consts, b->geo, U, P, pflag, fflag, b->stream));
     return KH_OK;
 }
 
 int kh_fix_utop(kh_block* b, double* U, double* P, const int* pflag) {
     if (!b || !U || !P || !pflag) return fail(KH_ERR_INVALID, "null argument");
-    KH_HIP(launch_fixup_entire(b->dims, b->consts, U, P, pflag, b->stream));
+    KH_HIP(launch_fixup_entire(b->dims, b->consts, b->geo, U, P, pflag, b->stream));
     return KH_OK;
 }
 
 int kh_ptou(kh_block* b, const double* P, double* U) {
     if (!b || !P || !U) return fail(KH_ERR_INVALID, "null argument");
-    KH_HIP(launch_ptou(b->dims, b->consts, P, U, b->stream));
+    KH_HIP(launch_ptou(b->dims, b->consts, b->geo, P, U, b->stream));
+    return KH_OK;
+}
+
+int kh_apply_bounds(kh_block* b, double* U, double* P) {
+    if (!b || !U || !P) return fail(KH_ERR_INVALID, "null argument");
+    KH_HIP(launch_bounds(b->dims, b->consts, b->geo, U, P, nullptr, b->stream));
     return KH_OK;
 }
 
@@ -360,8 +458,9 @@ int kh_driver_init_prims(kh_driver* d, const double* P) {
     const size_t st_bytes = sizeof(double) * NV * b->dims.nc;
     KH_HIP(hipMemcpyAsync(d->P0, P, st_bytes, hipMemcpyDeviceToDevice, b->stream));
     KH_HIP(launch_sync(b->dims, d->P0, NV, b->stream));
-    KH_HIP(launch_ptou(b->dims, b->consts, d->P0, d->U0, b->stream));
+    KH_HIP(launch_ptou(b->dims, b->consts, b->geo, d->P0, d->U0, b->stream));
     KH_HIP(launch_sync(b->dims, d->U0, NV, b->stream));
+    KH_HIP(launch_bounds(b->dims, b->consts, b->geo, d->U0, d->P0, nullptr, b->stream));
     KH_HIP(hipMemcpyAsync(d->P1, d->P0, st_bytes, hipMemcpyDeviceToDevice, b->stream));
     KH_HIP(hipMemcpyAsync(d->U1, d->U0, st_bytes, hipMemcpyDeviceToDevice, b->stream));
     if (d->P0f) {
@@ -395,6 +494,7 @@ int kh_driver_step(kh_driver* d, int nsteps) {
     char* F = (char*)d->F;
     char* ct = (char*)d->ctop;
     const int n_dt = (int)update_blocks(bd), n_divb = (int)divb_blocks(bd);
+    const bool physical = !bd.periodic[0] || !bd.periodic[1];
     for (int n = 0; n < nsteps; ++n) {
         for (int stage = 1; stage <= 2; ++stage) {
             const double* Psub = (stage == 1) ? d->P0 : d->P1;
@@ -406,12 +506,14 @@ int kh_driver_step(kh_driver* d, int nsteps) {
             // signal speeds only feed the timestep estimate of the last stage
             const int speeds = (stage == 2) ? SPEEDS_CTOP : SPEEDS_NONE;
             for (int dir = 1; dir <= bd.ndim; ++dir)
-                KH_TIMED(dir - 1, launch_flux_stage(bd, c, dir, b->p.recon, prec, speeds, Psub, Psubf,
+                KH_TIMED(dir - 1, launch_flux_stage(bd, c, b->geo, dir, b->p.recon, prec, speeds, Psub, Psubf,
                                                     F + fe * (dir - 1) * NV * bd.nc, ct + fe * (dir - 1) * bd.nc, s));
-            KH_TIMED(3 + stage, launch_update(bd, c, prec, stage, stage == 2, d->F, d->ctop, Usub, d->U0, Psub,
+            if (physical) KH_TIMED(3, launch_fix_flux(bd, prec, d->F, s));
+            KH_TIMED(3 + stage, launch_update(bd, c, b->geo, prec, stage, stage == 2, d->F, d->ctop, Usub, d->U0, Psub,
                                               Uout, Pout, Poutf, d->pflag, d->fflag, d->fail_list, d->ds, d->dt_part, s));
             if (c.fix_average || c.fix_atmo)
-                KH_TIMED(6, launch_fixup(bd, c, stage, Uout, Pout, Poutf, d->pflag, d->fail_list, d->ds, s));
+                KH_TIMED(6, launch_fixup(bd, c, b->geo, stage, Uout, Pout, Poutf, d->pflag, d->fail_list, d->ds, s));
+            if (physical) KH_TIMED(3, launch_bounds(bd, c, b->geo, Uout, Pout, Poutf, s));
         }
         KH_TIMED(7, launch_max_divb(bd, d->U0, d->divb_part, s));
         KH_TIMED(8, launch_finalize(c, d->ds, d->dt_part, n_dt, d->divb_part, bd.ndim > 1 ? n_divb : 0,
@@ -426,6 +528,14 @@ int kh_driver_get_state(kh_driver* d, double* P, double* U) {
     const size_t st_bytes = sizeof(double) * NV * b->dims.nc;
     if (P) KH_HIP(hipMemcpyAsync(P, d->P0, st_bytes, hipMemcpyDeviceToDevice, b->stream));
     if (U) KH_HIP(hipMemcpyAsync(U, d->U0, st_bytes, hipMemcpyDeviceToDevice, b->stream));
+    return KH_OK;
+}
+
+int kh_driver_get_flags(kh_driver* d, int* pflag, int* fflag) {
+    if (!d) return fail(KH_ERR_INVALID, "null driver");
+    const size_t bytes = sizeof(int) * d->blk->dims.nc;
+    if (pflag) KH_HIP(hipMemcpyAsync(pflag, d->pflag, bytes, hipMemcpyDeviceToDevice, d->blk->stream));
+    if (fflag) KH_HIP(hipMemcpyAsync(fflag, d->fflag, bytes, hipMemcpyDeviceToDevice, d->blk->stream));
     return KH_OK;
 }
 

@@ -35,10 +35,50 @@ template <typename R> __device__ __forceinline__ R clip(R n, R lo, R hi) { retur
 template <typename R> struct MinkowskiGeom {
     static constexpr bool flat = true;
     __device__ __forceinline__ R gdet() const { return R(1); }
+    __device__ __forceinline__ R alpha() const { return R(1); }
+    __device__ __forceinline__ R gcov(int a, int b) const { return a != b ? R(0) : (a == 0 ? R(-1) : R(1)); }
+    __device__ __forceinline__ R gcon(int a, int b) const { return a != b ? R(0) : (a == 0 ? R(-1) : R(1)); }
+    __device__ __forceinline__ R gcon0(int v) const { return gcon(0, v); }
     __device__ __forceinline__ void lower(const R v[4], R o[4]) const { o[0] = -v[0]; o[1] = v[1]; o[2] = v[2]; o[3] = v[3]; }
     __device__ __forceinline__ void raise(const R v[4], R o[4]) const { o[0] = -v[0]; o[1] = v[1]; o[2] = v[2]; o[3] = v[3]; }
     __device__ __forceinline__ R lorentz(R u1, R u2, R u3) const {
         const R qsq = u1 * u1 + u2 * u2 + u3 * u3;
+        return sqrt(R(1) + qsq);
+    }
+};
+
+// Metric of one point read from the geometry cache: 10 + 10 symmetric
+// components, sqrt(-g) and the lapse, stored component-major ([22][np]) so a
+// wavefront's loads of one component are contiguous.  Components are read
+// where they are used; the pointer is const and restrict, so the compiler is
+// free to keep or reload them.
+template <typename R> struct CachedGeom {
+    static constexpr bool flat = false;
+    const R* __restrict__ p;
+    long np, q;
+    __device__ __forceinline__ CachedGeom(const R* __restrict__ base, long np_, long q_) : p(base), np(np_), q(q_) {}
+    static __device__ __forceinline__ constexpr int sym(int a, int b) {
+        return (a <= b) ? (a == 0 ? b : a == 1 ? 3 + b : a == 2 ? 5 + b : 9) : sym(b, a);
+    }
+    __device__ __forceinline__ R gcov(int a, int b) const { return p[(long)sym(a, b) * np + q]; }
+    __device__ __forceinline__ R gcon(int a, int b) const { return p[(long)(10 + sym(a, b)) * np + q]; }
+    __device__ __forceinline__ R gcon0(int v) const { return gcon(0, v); }
+    __device__ __forceinline__ R gdet() const { return p[20 * np + q]; }
+    __device__ __forceinline__ R alpha() const { return p[21 * np + q]; }
+    __device__ __forceinline__ void lower(const R v[4], R o[4]) const {
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+            o[m] = gcov(m, 0) * v[0] + gcov(m, 1) * v[1] + gcov(m, 2) * v[2] + gcov(m, 3) * v[3];
+    }
+    __device__ __forceinline__ void raise(const R v[4], R o[4]) const {
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+            o[m] = gcon(m, 0) * v[0] + gcon(m, 1) * v[1] + gcon(m, 2) * v[2] + gcon(m, 3) * v[3];
+    }
+    // GRMHD::lorentz_calc, grmhd_functions.hpp:116-129
+    __device__ __forceinline__ R lorentz(R u1, R u2, R u3) const {
+        const R qsq = gcov(1, 1) * u1 * u1 + gcov(2, 2) * u2 * u2 + gcov(3, 3) * u3 * u3 +
+                      R(2) * (gcov(1, 2) * u1 * u2 + gcov(1, 3) * u1 * u3 + gcov(2, 3) * u2 * u3);
         return sqrt(R(1) + qsq);
     }
 };
@@ -442,13 +482,13 @@ struct FloorParams {
     int use_r_char, temp_adjust_u, frame, on;
 };
 
-// Floors::determine_floors, floors/floors_functions.hpp:84-160 (non-spherical geometric floors)
-// bsq = b^mu b_mu of the zone's current state.
+// Floors::determine_floors, floors/floors_functions.hpp:84-160.  bsq = b^mu b_mu
+// of the zone's current state; rhoflr_geom / uflr_geom are the geometric floors
+// of the zone (constants in Cartesian coordinates, r-dependent in spherical ones).
 template <class G>
 __device__ __forceinline__ int determine_floors(const G& g, const double P[NV], double bsq, double gam,
-                                                const FloorParams& f, double& rhoflr_max, double& uflr_max) {
-    const double rhoflr_geom = f.rho_min_const;
-    const double uflr_geom = f.u_min_const;
+                                                const FloorParams& f, double rhoflr_geom, double uflr_geom,
+                                                double& rhoflr_max, double& uflr_max) {
     const double rhoflr_b = bsq / f.bsq_over_rho_max;
     const double uflr_b = bsq / f.bsq_over_u_max;
     uflr_max = smax(uflr_geom, uflr_b);
@@ -551,9 +591,10 @@ __device__ __forceinline__ void apply_ceilings(const G& g, double P[NV], double 
 // D holds the four-vectors of P on entry and on return (recomputed if P changed).
 template <class G>
 __device__ __forceinline__ int floors_zone(const G& g, double P[NV], double U[NV], FourVec<double>& D, double gam,
-                                           const FloorParams& f, int fflag_in, int& pflag) {
+                                           const FloorParams& f, double rhoflr_geom, double uflr_geom,
+                                           int fflag_in, int& pflag) {
     double rf, uf;
-    const int fflag = fflag_in | determine_floors(g, P, dot4(D.bcon, D.bcov), gam, f, rf, uf);
+    const int fflag = fflag_in | determine_floors(g, P, dot4(D.bcon, D.bcov), gam, f, rhoflr_geom, uflr_geom, rf, uf);
     if (fflag) {
         int pflag_l = 0;
         if (f.frame == 0) {
@@ -569,6 +610,28 @@ __device__ __forceinline__ int floors_zone(const G& g, double P[NV], double U[NV
         calc_4vecs<double>(g, P, D);
     }
     return fflag;
+}
+
+// KBoundaries::check_inflow, boundaries/boundaries.hpp:101-136: velocity into
+// the domain through a physical X1 face is replaced by zero radial 4-velocity.
+// (The clip() of vsq there discards its result, so vsq is used unclamped.)
+template <class G>
+__device__ __forceinline__ void check_inflow(const G& g, double P[NV], bool inner) {
+    double uvec[3] = {P[U1], P[U2], P[U3]};
+    const double gamma0 = g.lorentz(uvec[0], uvec[1], uvec[2]);
+    const double ucon1 = uvec[0] - gamma0 * g.alpha() * g.gcon0(1);
+    if ((ucon1 > 0. && inner) || (ucon1 < 0. && !inner)) {
+        for (int v = 0; v < 3; ++v) uvec[v] /= gamma0;
+        const double alpha = g.alpha();
+        const double beta1 = g.gcon(0, 1) * alpha * alpha;
+        uvec[0] = beta1 / alpha;
+        const double vsq = g.gcov(1, 1) * uvec[0] * uvec[0] + g.gcov(2, 2) * uvec[1] * uvec[1] +
+                           g.gcov(3, 3) * uvec[2] * uvec[2] +
+                           2. * (g.gcov(1, 2) * uvec[0] * uvec[1] + g.gcov(1, 3) * uvec[0] * uvec[2] +
+                                 g.gcov(2, 3) * uvec[1] * uvec[2]);
+        const double gamma = 1. / sqrt(1. - vsq);
+        for (int v = 0; v < 3; ++v) P[U1 + v] = uvec[v] * gamma;
+    }
 }
 
 // Geometric floors on one state (Floors::apply_geo_floors, floors_functions.hpp:300-367)

@@ -37,29 +37,29 @@ __device__ __forceinline__ void cell_states(R x1, R x2, R x3, R x4, R x5, R& lou
 // Flux through one face from its two states (get_flux.hpp:237-367).
 // SPEEDS: SPEEDS_BOTH writes cmax and cmin (Flux.cmax/cmin), SPEEDS_CTOP writes
 // max(cmax, cmin) (all the timestep uses) into cmax_out, SPEEDS_NONE writes none.
-template <int DIR, int SPEEDS, typename R>
-__device__ __forceinline__ void face_flux(const StepConsts& c, const BlockDims& b, R Pl[NV], R Pr[NV], long ci,
-                                          R* __restrict__ F, R* __restrict__ cmax_out, R* __restrict__ cmin_out) {
+template <int DIR, int SPEEDS, typename R, class G>
+__device__ __forceinline__ void face_flux_g(const StepConsts& c, const BlockDims& b, const G& g, R rflr, R uflr,
+                                            R Pl[NV], R Pr[NV], long ci,
+                                            R* __restrict__ F, R* __restrict__ cmax_out, R* __restrict__ cmin_out) {
     const R gam = R(c.gam);
-    // get_flux.hpp:170-182 (Cartesian geometric floors on both face states)
+    // get_flux.hpp:170-182: geometric floors of the face's cell centre on both states
     if (c.recon_floors) {
-        apply_geo_floors<R>(Pl, R(c.floors.rho_min_const), R(c.floors.u_min_const));
-        apply_geo_floors<R>(Pr, R(c.floors.rho_min_const), R(c.floors.u_min_const));
+        apply_geo_floors<R>(Pl, rflr, uflr);
+        apply_geo_floors<R>(Pr, rflr, uflr);
     }
-    MinkowskiGeom<R> g;
     R Ul[NV], Fl[NV], Ur[NV], Fr[NV];
     R cmaxL, cminL, cmaxR, cminR;
     {
         FourVec<R> D;
         calc_4vecs<R>(g, Pl, D);
-        prim_to_u_and_flux<R, MinkowskiGeom<R>, DIR>(g, Pl, D, gam, Ul, Fl);
-        vchar<R, MinkowskiGeom<R>, DIR>(g, Pl, D, gam, cmaxL, cminL);
+        prim_to_u_and_flux<R, G, DIR>(g, Pl, D, gam, Ul, Fl);
+        vchar<R, G, DIR>(g, Pl, D, gam, cmaxL, cminL);
     }
     {
         FourVec<R> D;
         calc_4vecs<R>(g, Pr, D);
-        prim_to_u_and_flux<R, MinkowskiGeom<R>, DIR>(g, Pr, D, gam, Ur, Fr);
-        vchar<R, MinkowskiGeom<R>, DIR>(g, Pr, D, gam, cmaxR, cminR);
+        prim_to_u_and_flux<R, G, DIR>(g, Pr, D, gam, Ur, Fr);
+        vchar<R, G, DIR>(g, Pr, D, gam, cmaxR, cminR);
     }
     // get_flux.hpp:275-276 and 331-332: cmin is stored as a positive speed
     const R cmx = smax(smax(R(0), cmaxL), cmaxR);
@@ -83,25 +83,51 @@ __device__ __forceinline__ void face_flux(const StepConsts& c, const BlockDims& 
     }
 }
 
+template <typename R> __device__ __forceinline__ const R* geo_table(const GeomDev& gd, int loc);
+template <> __device__ __forceinline__ const double* geo_table<double>(const GeomDev& gd, int loc) { return gd.g[loc]; }
+template <> __device__ __forceinline__ const float* geo_table<float>(const GeomDev& gd, int loc) { return gd.g32[loc]; }
+template <typename R> __device__ __forceinline__ R geo_floor(const GeomDev& gd, int which, long q);
+template <> __device__ __forceinline__ double geo_floor<double>(const GeomDev& gd, int which, long q) { return gd.flr[which * gd.np + q]; }
+template <> __device__ __forceinline__ float geo_floor<float>(const GeomDev& gd, int which, long q) { return gd.flr32[which * gd.np + q]; }
+
+// Face at entire-block indices (j, i): flat metric (GEO == 0) or the cached
+// metric of the face locus (X1 face, X2 face, or the centre for X3 faces of the
+// axisymmetric metric) with the floors of cell (j, i).
+template <int DIR, int SPEEDS, typename R, int GEO>
+__device__ __forceinline__ void face_flux(const StepConsts& c, const BlockDims& b, const GeomDev& gd, int j, int i,
+                                          R Pl[NV], R Pr[NV], long ci,
+                                          R* __restrict__ F, R* __restrict__ cmax_out, R* __restrict__ cmin_out) {
+    if constexpr (GEO) {
+        const long q = (long)j * gd.gn1 + i;
+        const int loc = (DIR == 1) ? GLOC_FACE1 : (DIR == 2) ? GLOC_FACE2 : GLOC_CENTER;
+        CachedGeom<R> g(geo_table<R>(gd, loc), gd.np, q);
+        face_flux_g<DIR, SPEEDS, R>(c, b, g, geo_floor<R>(gd, 0, q), geo_floor<R>(gd, 1, q), Pl, Pr, ci, F, cmax_out, cmin_out);
+    } else {
+        MinkowskiGeom<R> g;
+        face_flux_g<DIR, SPEEDS, R>(c, b, g, R(c.floors.rho_min_const), R(c.floors.u_min_const), Pl, Pr, ci, F, cmax_out, cmin_out);
+    }
+}
+
 // X1 faces i in [is, ie+1] of rows j in [j0, j0+nj), k in [k0, k0+nk).
 // Cells i in [is-1, ie+1] (L per row); block b handles flattened cells
 // [255 b, 255 b + 256) and emits the faces of all but its first cell.
-template <int RECON, int SPEEDS, typename R, typename PT>
+template <int RECON, int SPEEDS, typename R, typename PT, int GEO>
 __global__ void __launch_bounds__(256)
-flux_x1_kernel(BlockDims b, StepConsts c, const PT* __restrict__ P, R* __restrict__ F,
+flux_x1_kernel(BlockDims b, StepConsts c, GeomDev gd, const PT* __restrict__ P, R* __restrict__ F,
                R* __restrict__ cmax_out, R* __restrict__ cmin_out, int j0, int nj, int k0, int nk) {
     __shared__ R sr[NV][256];
     const int L = b.nx1 + 2;
     const int tid = threadIdx.x;
     const long g = (long)blockIdx.x * 255 + tid;
     const bool valid = g < (long)L * nj * nk;
-    int ii = 0;
+    int ii = 0, jf = 0;
     long ci = 0;
     R lo[NV];
     if (valid) {
         const int row = (int)(g / L);
         ii = (int)(g - (long)row * L);
         const int jj = row % nj, kk = row / nj;
+        jf = j0 + jj;
         ci = (long)(k0 + kk) * b.s3 + (long)(j0 + jj) * b.s2 + (b.is - 1 + ii);
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
@@ -116,15 +142,15 @@ flux_x1_kernel(BlockDims b, StepConsts c, const PT* __restrict__ P, R* __restric
     R Pl[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) Pl[v] = sr[v][tid - 1];
-    face_flux<1, SPEEDS, R>(c, b, Pl, lo, ci, F, cmax_out, cmin_out);
+    face_flux<1, SPEEDS, R, GEO>(c, b, gd, jf, b.is - 1 + ii, Pl, lo, ci, F, cmax_out, cmin_out);
 }
 
 // X2 / X3 faces.  Transverse columns: flattened (outer, i) with i in [is-1, ie+1]
 // fastest; outer = k (X2) or j (X3) from o0.  Along the normal: faces s in
 // [s0, s1]; tile y covers cells s0-1+7y .. s0+6+7y.
-template <int DIR, int RECON, int SPEEDS, typename R, typename PT>
+template <int DIR, int RECON, int SPEEDS, typename R, typename PT, int GEO>
 __global__ void __launch_bounds__(512)
-flux_xt_kernel(BlockDims b, StepConsts c, const PT* __restrict__ P, R* __restrict__ F,
+flux_xt_kernel(BlockDims b, StepConsts c, GeomDev gd, const PT* __restrict__ P, R* __restrict__ F,
                R* __restrict__ cmax_out, R* __restrict__ cmin_out, int o0, int no, int s0, int s1) {
     __shared__ R sr[NV][8][64];
     const int ni = b.nx1 + 2;
@@ -134,12 +160,14 @@ flux_xt_kernel(BlockDims b, StepConsts c, const PT* __restrict__ P, R* __restric
     const bool valid = col < ni * no && s <= s1;
     const long st = (DIR == 2) ? b.s2 : b.s3;
     long ci = 0;
+    int jf = 0, if_ = 0;
     R lo[NV];
     if (valid) {
         const int outer = o0 + col / ni;
         const int i = b.is - 1 + col % ni;
         const int j = (DIR == 2) ? s : outer;
         const int k = (DIR == 2) ? outer : s;
+        jf = j; if_ = i;
         ci = (long)k * b.s3 + (long)j * b.s2 + i;
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
@@ -154,11 +182,11 @@ flux_xt_kernel(BlockDims b, StepConsts c, const PT* __restrict__ P, R* __restric
     R Pl[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) Pl[v] = sr[v][ty - 1][tx];
-    face_flux<DIR, SPEEDS, R>(c, b, Pl, lo, ci, F, cmax_out, cmin_out);
+    face_flux<DIR, SPEEDS, R, GEO>(c, b, gd, jf, if_, Pl, lo, ci, F, cmax_out, cmin_out);
 }
 
-template <int RECON, int SPEEDS, typename R, typename PT>
-static hipError_t launch_recon(const BlockDims& b, const StepConsts& c, int dir, const PT* P, R* F,
+template <int RECON, int SPEEDS, typename R, typename PT, int GEO>
+static hipError_t launch_recon(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int dir, const PT* P, R* F,
                                R* cmax, R* cmin, hipStream_t s) {
     // Face ranges consumed by flux-CT and the divergence: the face-normal index
     // covers [start, end+1]; transverse indices extend one zone each side.
@@ -168,58 +196,65 @@ static hipError_t launch_recon(const BlockDims& b, const StepConsts& c, int dir,
         const int nj = jhi - jlo + 1, nk = khi - klo + 1;
         const long cells = (long)(b.nx1 + 2) * nj * nk;
         const dim3 grid((unsigned)((cells - 1 + 254) / 255));
-        hipLaunchKernelGGL((flux_x1_kernel<RECON, SPEEDS, R, PT>), grid, dim3(256), 0, s, b, c, P, F, cmax, cmin, jlo, nj, klo, nk);
+        hipLaunchKernelGGL((flux_x1_kernel<RECON, SPEEDS, R, PT, GEO>), grid, dim3(256), 0, s, b, c, gd, P, F, cmax, cmin, jlo, nj, klo, nk);
     } else if (dir == 2) {
         const int no = khi - klo + 1;
         const int ncol = (b.nx1 + 2) * no;
         const int nfaces = b.nx2 + 1;
         const dim3 grid((unsigned)((ncol + 63) / 64), (unsigned)((nfaces + 6) / 7));
-        hipLaunchKernelGGL((flux_xt_kernel<2, RECON, SPEEDS, R, PT>), grid, dim3(64, 8), 0, s, b, c, P, F, cmax, cmin, klo, no, b.js, b.je + 1);
+        hipLaunchKernelGGL((flux_xt_kernel<2, RECON, SPEEDS, R, PT, GEO>), grid, dim3(64, 8), 0, s, b, c, gd, P, F, cmax, cmin, klo, no, b.js, b.je + 1);
     } else {
         const int no = jhi - jlo + 1;
         const int ncol = (b.nx1 + 2) * no;
         const int nfaces = b.nx3 + 1;
         const dim3 grid((unsigned)((ncol + 63) / 64), (unsigned)((nfaces + 6) / 7));
-        hipLaunchKernelGGL((flux_xt_kernel<3, RECON, SPEEDS, R, PT>), grid, dim3(64, 8), 0, s, b, c, P, F, cmax, cmin, jlo, no, b.ks, b.ke + 1);
+        hipLaunchKernelGGL((flux_xt_kernel<3, RECON, SPEEDS, R, PT, GEO>), grid, dim3(64, 8), 0, s, b, c, gd, P, F, cmax, cmin, jlo, no, b.ks, b.ke + 1);
     }
     return hipGetLastError();
 }
 
-template <int SPEEDS, typename R, typename PT>
-static hipError_t launch_speeds(const BlockDims& b, const StepConsts& c, int dir, int recon, const PT* P, R* F,
+template <int SPEEDS, typename R, typename PT, int GEO>
+static hipError_t launch_speeds(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int dir, int recon, const PT* P, R* F,
                                 R* cmax, R* cmin, hipStream_t s) {
     switch (recon) {
-    case RECON_DONOR: return launch_recon<RECON_DONOR, SPEEDS, R, PT>(b, c, dir, P, F, cmax, cmin, s);
-    case RECON_LINEAR_MC: return launch_recon<RECON_LINEAR_MC, SPEEDS, R, PT>(b, c, dir, P, F, cmax, cmin, s);
-    case RECON_PPM: return launch_recon<RECON_PPM, SPEEDS, R, PT>(b, c, dir, P, F, cmax, cmin, s);
-    default: return launch_recon<RECON_WENO5, SPEEDS, R, PT>(b, c, dir, P, F, cmax, cmin, s);
+    case RECON_DONOR: return launch_recon<RECON_DONOR, SPEEDS, R, PT, GEO>(b, c, gd, dir, P, F, cmax, cmin, s);
+    case RECON_LINEAR_MC: return launch_recon<RECON_LINEAR_MC, SPEEDS, R, PT, GEO>(b, c, gd, dir, P, F, cmax, cmin, s);
+    case RECON_PPM: return launch_recon<RECON_PPM, SPEEDS, R, PT, GEO>(b, c, gd, dir, P, F, cmax, cmin, s);
+    default: return launch_recon<RECON_WENO5, SPEEDS, R, PT, GEO>(b, c, gd, dir, P, F, cmax, cmin, s);
     }
 }
 
-template <typename R, typename PT>
-static hipError_t launch_prec(const BlockDims& b, const StepConsts& c, int dir, int recon, int speeds, const PT* P,
-                              R* F, R* cmax, R* cmin, hipStream_t s) {
-    if (speeds == SPEEDS_CTOP) return launch_speeds<SPEEDS_CTOP, R, PT>(b, c, dir, recon, P, F, cmax, cmin, s);
-    if (speeds == SPEEDS_NONE) return launch_speeds<SPEEDS_NONE, R, PT>(b, c, dir, recon, P, F, cmax, cmin, s);
-    return launch_speeds<SPEEDS_BOTH, R, PT>(b, c, dir, recon, P, F, cmax, cmin, s);
+template <typename R, typename PT, int GEO>
+static hipError_t launch_geo(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int dir, int recon, int speeds,
+                             const PT* P, R* F, R* cmax, R* cmin, hipStream_t s) {
+    if (speeds == SPEEDS_CTOP) return launch_speeds<SPEEDS_CTOP, R, PT, GEO>(b, c, gd, dir, recon, P, F, cmax, cmin, s);
+    if (speeds == SPEEDS_NONE) return launch_speeds<SPEEDS_NONE, R, PT, GEO>(b, c, gd, dir, recon, P, F, cmax, cmin, s);
+    return launch_speeds<SPEEDS_BOTH, R, PT, GEO>(b, c, gd, dir, recon, P, F, cmax, cmin, s);
 }
 
-hipError_t launch_flux(const BlockDims& b, const StepConsts& c, int dir, int recon, int prec,
+template <typename R, typename PT>
+static hipError_t launch_prec(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int dir, int recon, int speeds,
+                              const PT* P, R* F, R* cmax, R* cmin, hipStream_t s) {
+    if (b.geo) return launch_geo<R, PT, 1>(b, c, gd, dir, recon, speeds, P, F, cmax, cmin, s);
+    return launch_geo<R, PT, 0>(b, c, gd, dir, recon, speeds, P, F, cmax, cmin, s);
+}
+
+hipError_t launch_flux(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int dir, int recon, int prec,
                        const double* P, void* F, void* cmax, void* cmin, hipStream_t s) {
     if (dir == 2 && b.ndim < 2) return hipSuccess;
     if (dir == 3 && b.ndim < 3) return hipSuccess;
     if (prec == PREC_SPLIT)
-        return launch_prec<float, double>(b, c, dir, recon, SPEEDS_BOTH, P, (float*)F, (float*)cmax, (float*)cmin, s);
-    return launch_prec<double, double>(b, c, dir, recon, SPEEDS_BOTH, P, (double*)F, (double*)cmax, (double*)cmin, s);
+        return launch_prec<float, double>(b, c, gd, dir, recon, SPEEDS_BOTH, P, (float*)F, (float*)cmax, (float*)cmin, s);
+    return launch_prec<double, double>(b, c, gd, dir, recon, SPEEDS_BOTH, P, (double*)F, (double*)cmax, (double*)cmin, s);
 }
 
-hipError_t launch_flux_stage(const BlockDims& b, const StepConsts& c, int dir, int recon, int prec, int speeds,
-                             const double* P, const float* P32, void* F, void* ctop, hipStream_t s) {
+hipError_t launch_flux_stage(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int dir, int recon, int prec,
+                             int speeds, const double* P, const float* P32, void* F, void* ctop, hipStream_t s) {
     if (dir == 2 && b.ndim < 2) return hipSuccess;
     if (dir == 3 && b.ndim < 3) return hipSuccess;
     if (prec == PREC_SPLIT)
-        return launch_prec<float, float>(b, c, dir, recon, speeds, P32, (float*)F, (float*)ctop, nullptr, s);
-    return launch_prec<double, double>(b, c, dir, recon, speeds, P, (double*)F, (double*)ctop, nullptr, s);
+        return launch_prec<float, float>(b, c, gd, dir, recon, speeds, P32, (float*)F, (float*)ctop, nullptr, s);
+    return launch_prec<double, double>(b, c, gd, dir, recon, speeds, P, (double*)F, (double*)ctop, nullptr, s);
 }
 
 }  // namespace kh

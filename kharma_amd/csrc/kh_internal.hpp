@@ -3,7 +3,10 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <string>
+#include <vector>
 
+#include "../../include/kharma_amd.h"
 #include "kh_device.hpp"
 
 namespace kh {
@@ -20,6 +23,10 @@ struct BlockDims {
     long s2, s3;           // strides of j and k
     long nc;               // stride between variables
     double dx1, dx2, dx3;
+    int geo;               // 0: flat Cartesian metric; 1: metric read from the geometry cache
+    int bc[4];             // boundary types of the inner/outer x1 and inner/outer x2 faces (KH_BC_*; 0 if periodic)
+    int check_inflow[2];   // inner/outer x1
+    int zero_polar_flux, fix_polar_flux, fix_corner_inner;
 };
 
 // Scalars that change every step live on the device so a run of steps needs no
@@ -46,6 +53,44 @@ struct StepConsts {
 };
 
 enum Precision { PREC_FP64 = 0, PREC_SPLIT = 1 };
+
+// ---- geometry cache (kh_geom.cpp) ----
+// Loci of the cache; the X3 face of an axisymmetric metric equals the centre.
+enum { GLOC_CENTER = 0, GLOC_FACE1 = 1, GLOC_FACE2 = 2, GLOC_CORNER = 3, GLOC_N = 4 };
+// Components per point: gcov (10, upper triangle row-major), gcon (10), gdet, lapse.
+enum { GEO_GCOV = 0, GEO_GCON = 10, GEO_GDET = 20, GEO_ALPHA = 21, GEO_NCOMP = 22 };
+
+struct GeomHost {
+    int gn1 = 0, gn2 = 0;   // (n1 + 1) x (n2 + 1) points
+    long np = 0;
+    std::vector<double> g[GLOC_N];   // [GEO_NCOMP][gn2][gn1] per locus
+    std::vector<double> conn;        // [64][gn2][gn1]: gdet * Gamma^mu_{nu lam} at centres, index mu*16+nu*4+lam
+    std::vector<double> flr;         // [2][gn2][gn1]: geometric rho and u floors at centres
+    std::vector<double> r;           // [gn2][gn1]: radius of the cell centre
+};
+void build_geometry(const kh_params& p, const BlockDims& b, GeomHost& G);
+// Parameter checks and index ranges shared by every entry point (kh_capi.cpp).
+int check_params(const kh_params* p);
+void make_dims(const kh_params* p, BlockDims& b);
+int api_fail(int code, const std::string& msg);
+void locus_coords(const kh_params& p, const BlockDims& b, int loc, int j, int i, double X[4]);
+double embed_r(const kh_params& p, const double X[4]);
+void embed_coords(const kh_params& p, const double X[4], double Xe[4]);
+void native_dxdX(const kh_params& p, const double X[4], double M[4][4]);
+void native_gcov(const kh_params& p, const double X[4], double g[4][4]);
+double invert_metric(const double g[4][4], double gi[4][4]);
+
+// Device copy of the cache.  The flux kernels of the split form read the FP32 copies.
+struct GeomDev {
+    const double* g[3] = {nullptr, nullptr, nullptr};   // centre, face1, face2
+    const float* g32[3] = {nullptr, nullptr, nullptr};
+    const double* conn = nullptr;
+    unsigned long long conn_mask = 0;   // bit n: component n of conn is non-zero somewhere
+    const double* flr = nullptr;
+    const float* flr32 = nullptr;
+    int gn1 = 0;
+    long np = 0;
+};
 enum SpeedsOut { SPEEDS_NONE = 0, SPEEDS_BOTH = 1, SPEEDS_CTOP = 2 };
 
 // ---- launchers (kh_flux.hip) ----
@@ -53,28 +98,39 @@ enum SpeedsOut { SPEEDS_NONE = 0, SPEEDS_BOTH = 1, SPEEDS_CTOP = 2 };
 // X1 faces i in [is, ie+1], j in [js-1, je+1], k in [ks-1, ke+1] (collapsed in
 // inactive dimensions), cyclically for X2/X3.  F/cmax/cmin point at this
 // direction's arrays ([8][nc] and [nc]); element type float when prec == PREC_SPLIT.
-hipError_t launch_flux(const BlockDims& b, const StepConsts& c, int dir, int recon, int prec,
+hipError_t launch_flux(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int dir, int recon, int prec,
                        const double* P, void* F, void* cmax, void* cmin, hipStream_t s);
 // Stage-driver variant: speeds = SPEEDS_NONE or SPEEDS_CTOP (ctop = max(cmax, cmin)
 // into ctop[dir-1]); the split form reads the FP32 shadow P32 of the primitives.
-hipError_t launch_flux_stage(const BlockDims& b, const StepConsts& c, int dir, int recon, int prec, int speeds,
-                             const double* P, const float* P32, void* F, void* ctop, hipStream_t s);
+hipError_t launch_flux_stage(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int dir, int recon, int prec,
+                             int speeds, const double* P, const float* P32, void* F, void* ctop, hipStream_t s);
 
 // ---- launchers (kh_step.hip) ----
 hipError_t launch_emf(const BlockDims& b, int prec, const void* F, void* emf, hipStream_t s);
 hipError_t launch_flux_ct_rewrite(const BlockDims& b, int prec, void* F, const void* emf, hipStream_t s);
 hipError_t launch_flux_divergence(const BlockDims& b, int prec, const void* F, double* dUdt, hipStream_t s);
+hipError_t launch_geo_source(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const double* P, double* dUdt,
+                             hipStream_t s);
 // Fused stage update (flux-CT EMFs, divergence, RK update, U->P, floors, P->U,
 // ghost images).  ctop: [3][nc] max signal speeds, read when do_dt (the last
 // stage), which then writes one CFL minimum per block into dt_part.
-hipError_t launch_update(const BlockDims& b, const StepConsts& c, int prec, int stage, bool do_dt,
+hipError_t launch_update(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int prec, int stage, bool do_dt,
                          const void* F, const void* ctop, const double* Usub, const double* Ufull, const double* Psub,
                          double* Uout, double* Pout, float* Pout32, int* pflag, int* fflag,
                          int* fail_list, DeviceState* ds, double* dt_part, hipStream_t s);
 long update_blocks(const BlockDims& b);
 long divb_blocks(const BlockDims& b);
-hipError_t launch_fixup(const BlockDims& b, const StepConsts& c, int stage, double* U, double* P, float* P32,
-                        const int* pflag, const int* fail_list, DeviceState* ds, hipStream_t s);
+hipError_t launch_fixup(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int stage, double* U, double* P,
+                        float* P32, const int* pflag, const int* fail_list, DeviceState* ds, hipStream_t s);
+// Flux fixes at physical boundaries, on the Riemann fluxes before the update:
+// B_FluxCT::ZeroBoundaryFlux on polar faces (the update kernel's flux-CT then
+// sees mirrored B2 fluxes), the no-inflow mass-flux clamp on X1 faces and the
+// zeroed polar fluxes of KBoundaries::FixFlux.
+hipError_t launch_fix_flux(const BlockDims& b, int prec, void* F, hipStream_t s);
+// Physical boundary ghost zones of U, P (and the FP32 shadow if non-null) from
+// the final interior state: KBoundaries::ApplyBoundary for each x1/x2 face.
+hipError_t launch_bounds(const BlockDims& b, const StepConsts& c, const GeomDev& gd, double* U, double* P, float* P32,
+                         hipStream_t s);
 hipError_t launch_finalize(const StepConsts& c, DeviceState* ds, const double* dt_part, int n_dt,
                            const double* divb_part, int n_divb, double* dt_hist, int hist_cap, hipStream_t s);
 hipError_t launch_max_divb(const BlockDims& b, const double* U, double* part, hipStream_t s);
@@ -85,13 +141,14 @@ hipError_t launch_set_clock(DeviceState* ds, double time, double dt, int ncycle,
 // Task-level kernels over the entire block (the reference's IndexDomain::entire).
 hipError_t launch_weighted_sum(const BlockDims& b, const double* Usub, const double* Ufull, const double* dUdt,
                                double gam0, double gam1, double beta_dt, double* Uout, hipStream_t s);
-hipError_t launch_utop(const BlockDims& b, const StepConsts& c, const double* U, double* P, int* pflag, int* fflag,
+hipError_t launch_utop(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const double* U, double* P,
+                       int* pflag, int* fflag, hipStream_t s);
+hipError_t launch_floors(const BlockDims& b, const StepConsts& c, const GeomDev& gd, double* U, double* P, int* pflag,
+                         int* fflag, hipStream_t s);
+hipError_t launch_fixup_entire(const BlockDims& b, const StepConsts& c, const GeomDev& gd, double* U, double* P,
+                               const int* pflag, hipStream_t s);
+hipError_t launch_ptou(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const double* P, double* U,
                        hipStream_t s);
-hipError_t launch_floors(const BlockDims& b, const StepConsts& c, double* U, double* P, int* pflag, int* fflag,
-                         hipStream_t s);
-hipError_t launch_fixup_entire(const BlockDims& b, const StepConsts& c, double* U, double* P, const int* pflag,
-                               hipStream_t s);
-hipError_t launch_ptou(const BlockDims& b, const StepConsts& c, const double* P, double* U, hipStream_t s);
 hipError_t launch_sync(const BlockDims& b, double* A, int nvar, hipStream_t s);
 hipError_t launch_sync_int(const BlockDims& b, int* A, hipStream_t s);
 hipError_t launch_estimate_dt(const BlockDims& b, const StepConsts& c, int prec, const void* cmax, const void* cmin,

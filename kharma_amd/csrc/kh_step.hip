@@ -78,6 +78,72 @@ __device__ __forceinline__ long ninterior(const BlockDims& b) {
     return (long)b.nx1 * (b.ndim > 1 ? b.nx2 : 1) * (b.ndim > 2 ? b.nx3 : 1);
 }
 
+// Metric of cell centre (j, i): flat, or read from the geometry cache.
+template <int GEO, class Fn>
+__device__ __forceinline__ void with_center_geom(const GeomDev& gd, int j, int i, Fn&& fn) {
+    if constexpr (GEO) {
+        CachedGeom<double> g(gd.g[GLOC_CENTER], gd.np, (long)j * gd.gn1 + i);
+        fn(g);
+    } else {
+        MinkowskiGeom<double> g;
+        fn(g);
+    }
+}
+// Geometric floors of cell centre (j, i) (floors_functions.hpp:97-108)
+template <int GEO>
+__device__ __forceinline__ void geom_floors(const StepConsts& c, const GeomDev& gd, int j, int i, double& rf, double& uf) {
+    if constexpr (GEO) {
+        const long q = (long)j * gd.gn1 + i;
+        rf = gd.flr[q];
+        uf = gd.flr[gd.np + q];
+    } else {
+        rf = c.floors.rho_min_const;
+        uf = c.floors.u_min_const;
+    }
+}
+// KDomain::GetPhysicalRange (domain.hpp:161-187): interior along physical-boundary
+// dimensions, the entire block along periodic ones.
+__device__ __forceinline__ bool in_physical(const BlockDims& b, int k, int j, int i) {
+    if (i < 0 || i >= b.n1 || j < 0 || j >= b.n2 || k < 0 || k >= b.n3) return false;
+    if (!b.periodic[0] && (i < b.is || i > b.ie)) return false;
+    if (b.ndim > 1 && !b.periodic[1] && (j < b.js || j > b.je)) return false;
+    if (b.ndim > 2 && !b.periodic[2] && (k < b.ks || k > b.ke)) return false;
+    return true;
+}
+
+// Flux::AddGeoSource (flux/flux.cpp:395-444): the T^mu_nu Gamma contraction of
+// zone (j, i) added to the energy and momentum rates.  conn holds
+// sqrt(-g) Gamma^a_{b c} at index 16 a + 4 b + c; mask has bit n set when
+// component n is non-zero anywhere in the block (a uniform branch).
+template <class G>
+__device__ __forceinline__ void add_geo_source(const G& g, const GeomDev& gd, long q, const double Psub[NV], double gam,
+                                               double dudt[NV]) {
+    FourVec<double> D;
+    calc_4vecs<double>(g, Psub, D);
+    const double bsq = dot4(D.bcon, D.bcov);
+    const double pgas = (gam - 1.) * Psub[UU];
+    const double eta = pgas + Psub[RHO] + Psub[UU] + bsq;
+    const double ptot = pgas + 0.5 * bsq;
+    double nd[4] = {0., 0., 0., 0.};
+#pragma unroll
+    for (int mu = 0; mu < 4; ++mu) {
+        double T[4];
+#pragma unroll
+        for (int nu = 0; nu < 4; ++nu) T[nu] = eta * D.ucon[mu] * D.ucov[nu] + (mu == nu ? ptot : 0.) - D.bcon[mu] * D.bcov[nu];
+#pragma unroll
+        for (int nu = 0; nu < 4; ++nu)
+#pragma unroll
+            for (int lam = 0; lam < 4; ++lam) {
+                const int n = nu * 16 + lam * 4 + mu;
+                if ((gd.conn_mask >> n) & 1ull) nd[lam] += T[nu] * gd.conn[(long)n * gd.np + q];
+            }
+    }
+    dudt[UU] += nd[0];
+    dudt[U1] += nd[1];
+    dudt[U2] += nd[2];
+    dudt[U3] += nd[3];
+}
+
 // ----------------------------------------------------------------- flux-CT
 // Corner EMFs, b_flux_ct.cpp:331-342: emf_d at corners i in [is, ie+1],
 // j in [js, je+1], k in [ks, ke(+1 in 3D)].  F = [3][8][nc], emf = [3][nc].
@@ -300,9 +366,9 @@ __device__ __forceinline__ void zone_divergence_ct(const BlockDims& b, const R* 
 
 // ------------------------------------------------------- fused stage update
 // Stage 1 (gam0 = 0 and U_sub == U_full) reads the state once: 0*U + 1*U == U.
-template <typename R, bool DO_DT, bool CT>
+template <typename R, bool DO_DT, bool CT, int GEO>
 __global__ void __launch_bounds__(BS, 4)
-update_kernel(BlockDims b, StepConsts cst, int stage, const R* __restrict__ F, const R* __restrict__ ctop,
+update_kernel(BlockDims b, StepConsts cst, GeomDev gd, int stage, const R* __restrict__ F, const R* __restrict__ ctop,
               const double* __restrict__ Usub, const double* Ufull, const double* __restrict__ Psub,
               double* Uout, double* __restrict__ Pout, float* __restrict__ Pout32,
               int* __restrict__ pflag, int* __restrict__ fflag, int* __restrict__ fail_list,
@@ -318,6 +384,14 @@ update_kernel(BlockDims b, StepConsts cst, int stage, const R* __restrict__ F, c
         double dudt[NV];
         if (CT) zone_divergence_ct<R>(b, F, c, dudt);
         else zone_divergence<R>(b, F, nullptr, c, false, dudt);
+        if constexpr (GEO) {
+            double Ps[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) Ps[v] = Psub[v * b.nc + c];
+            const long q = (long)j * gd.gn1 + i;
+            CachedGeom<double> g(gd.g[GLOC_CENTER], gd.np, q);
+            add_geo_source(g, gd, q, Ps, gam, dudt);
+        }
         const double g0 = cst.gam0[stage - 1], g1 = cst.gam1[stage - 1];
         const double beta_dt = cst.beta[stage - 1] * ds->dt;
         const bool same = (g0 == 0.) && (Usub == Ufull);
@@ -331,16 +405,21 @@ update_kernel(BlockDims b, StepConsts cst, int stage, const R* __restrict__ F, c
         // U->P guess: the stage's starting primitives (kharma_driver.cpp:379-384)
 #pragma unroll
         for (int v = 0; v < 5; ++v) P[v] = Psub[v * b.nc + c];
-        MinkowskiGeom<double> g;
-        int pf = u_to_p_onedw(g, U, P, gam, cst.iter_max, cst.err_tol);
-        const int ff0 = (pf / F_GEOM_RHO) * F_GEOM_RHO;
-        pf = pf % F_GEOM_RHO;
-        P[B1] = U[B1] / g.gdet(); P[B2] = U[B2] / g.gdet(); P[B3] = U[B3] / g.gdet();
-        FourVec<double> D;
-        calc_4vecs<double>(g, P, D);
-        int ff = ff0;
-        if (cst.floors.on) ff = floors_zone(g, P, U, D, gam, cst.floors, ff0, pf);
-        p_to_u_D(g, P, D, gam, U);
+        int pf = 0, ff = 0;
+        double rf, uf;
+        geom_floors<GEO>(cst, gd, j, i, rf, uf);
+        with_center_geom<GEO>(gd, j, i, [&](const auto& g) {
+            pf = u_to_p_onedw(g, U, P, gam, cst.iter_max, cst.err_tol);
+            const int ff0 = (pf / F_GEOM_RHO) * F_GEOM_RHO;
+            pf = pf % F_GEOM_RHO;
+            const double gd_ = g.gdet();
+            P[B1] = U[B1] / gd_; P[B2] = U[B2] / gd_; P[B3] = U[B3] / gd_;
+            FourVec<double> D;
+            calc_4vecs<double>(g, P, D);
+            ff = ff0;
+            if (cst.floors.on) ff = floors_zone(g, P, U, D, gam, cst.floors, rf, uf, ff0, pf);
+            p_to_u_D(g, P, D, gam, U);
+        });
         ImageSet s;
         image_set(b, k, j, i, s);
 #pragma unroll
@@ -370,8 +449,9 @@ update_kernel(BlockDims b, StepConsts cst, int stage, const R* __restrict__ F, c
 // Inverter::FixUtoP (inverter/fixup.cpp:47-156) on the zones listed by the
 // update kernel: distance-weighted average of non-failed neighbours, atmosphere
 // fallback, geometric floors, then the conserved state of the final P->U.
+template <int GEO>
 __global__ void __launch_bounds__(BS)
-fixup_list_kernel(BlockDims b, StepConsts cst, int stage, double* __restrict__ U, double* __restrict__ P,
+fixup_list_kernel(BlockDims b, StepConsts cst, GeomDev gd, int stage, double* __restrict__ U, double* __restrict__ P,
                   float* __restrict__ P32, const int* __restrict__ pflag, const int* __restrict__ fail_list,
                   const DeviceState* __restrict__ ds) {
     const int n = ds->fail_count[stage - 1];
@@ -384,7 +464,7 @@ fixup_list_kernel(BlockDims b, StepConsts cst, int stage, double* __restrict__ U
         if (cst.fix_average) {
             for (int dn = -1; dn <= 1; dn++) for (int dm = -1; dm <= 1; dm++) for (int dl = -1; dl <= 1; dl++) {
                 const int ii = i + dl, jj = j + dm, kk = k + dn;
-                if (ii < 0 || ii >= b.n1 || jj < 0 || jj >= b.n2 || kk < 0 || kk >= b.n3) continue;
+                if (!in_physical(b, kk, jj, ii)) continue;
                 const long cc = (long)kk * b.s3 + (long)jj * b.s2 + ii;
                 if (!(pflag[cc] > 0)) {
                     const double w = 1. / (abs(dl) + abs(dm) + abs(dn) + 1);
@@ -395,10 +475,11 @@ fixup_list_kernel(BlockDims b, StepConsts cst, int stage, double* __restrict__ U
         }
         if (wsum < 1.e-10) { for (int p = 0; p < 5; ++p) Pz[p] = 0.; }
         else { for (int p = 0; p < 5; ++p) Pz[p] = sum[p] / wsum; }
-        apply_geo_floors<double>(Pz, cst.floors.rho_min_const, cst.floors.u_min_const);
-        MinkowskiGeom<double> g;
+        double rf, uf;
+        geom_floors<GEO>(cst, gd, j, i, rf, uf);
+        apply_geo_floors<double>(Pz, rf, uf);
         double Uz[NV];
-        p_to_u(g, Pz, cst.gam, Uz);
+        with_center_geom<GEO>(gd, j, i, [&](const auto& g) { p_to_u(g, Pz, cst.gam, Uz); });
         ImageSet s;
         image_set(b, k, j, i, s);
         for (int v = 0; v < 5; ++v) {
@@ -472,13 +553,16 @@ __global__ void set_clock_kernel(DeviceState* ds, double time, double dt, int nc
 }
 
 // B_FluxCT::MaxDivB with corner_div (b_flux_ct.cpp:566-607, b_flux_ct_functions.hpp:52-75)
+// Corners i in [is, ie+1] for periodic X1; ValidDivBX1 (b_flux_ct.cpp:554-564)
+// skips the corners on physical X1 faces: i in [is+1, ie].
 __global__ void __launch_bounds__(BS) max_divb_kernel(BlockDims b, const double* __restrict__ U, double* __restrict__ part) {
-    const int ni = b.nx1 + 1, nj = b.nx2 + 1, nk = (b.ndim > 2) ? b.nx3 + 1 : 1;
+    const int i0 = b.periodic[0] ? b.is : b.is + 1;
+    const int ni = b.periodic[0] ? b.nx1 + 1 : b.nx1 - 1, nj = b.nx2 + 1, nk = (b.ndim > 2) ? b.nx3 + 1 : 1;
     const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
     double d = 0.;
     if (t < (unsigned)(ni * nj * nk)) {
         const int kk = t / (ni * nj), rem = t - kk * (ni * nj), jj = rem / ni;
-        const int i = b.is + rem - jj * ni, j = b.js + jj, k = b.ks + kk;
+        const int i = i0 + rem - jj * ni, j = b.js + jj, k = b.ks + kk;
         const long c = (long)k * b.s3 + (long)j * b.s2 + i;
         const double* Bx = U + B1 * b.nc;
         const double* By = U + B2 * b.nc;
@@ -501,6 +585,134 @@ __global__ void __launch_bounds__(BS) max_divb_kernel(BlockDims b, const double*
     if (threadIdx.x == 0) part[blockIdx.x] = m;
 }
 
+// ------------------------------------------------- physical boundaries
+// Flux fixes of physical faces on the Riemann fluxes (B_FluxCT::ZeroBoundaryFlux,
+// b_flux_ct.cpp:372-420; KBoundaries::FixFlux, boundaries.cpp:611-690).  One
+// thread per face of the fixed rows; the two kinds of rows are disjoint.
+template <typename R>
+__global__ void __launch_bounds__(BS) fix_flux_kernel(BlockDims b, R* __restrict__ F) {
+    const int nd = b.ndim;
+    const int ni = b.nx1 + 2, nk = (nd > 2) ? b.nx3 + 2 : 1, nj = (nd > 1) ? b.nx2 + 2 : 1;
+    const int i0 = b.is - 1, k0 = (nd > 2) ? b.ks - 1 : b.ks, j0 = (nd > 1) ? b.js - 1 : b.js;
+    const long nc = b.nc;
+    R* F1 = F;
+    R* F2 = F + 8 * nc;
+    R* F3 = F + 16 * nc;
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned npolar = (nd > 1 && !b.periodic[1]) ? 2u * ni * nk : 0u;
+    if (t < npolar) {
+        // polar faces j = js (side 0) and j = je + 1 (side 1), i in [is-1, ie+1], k in [ks-1, ke+1]
+        const int side = t / (ni * nk), rem = t - side * (ni * nk);
+        const int k = k0 + rem / ni, i = i0 + rem % ni;
+        const int j = side == 0 ? b.js : b.je + 1;
+        const long c = (long)k * b.s3 + (long)j * b.s2 + i;
+        const long cg = side == 0 ? c - b.s2 : c;        // the face row that is mirrored into
+        const long cs = side == 0 ? c : c - b.s2;        // and its source
+        if (b.fix_polar_flux) {
+            F2[B1 * nc + c] = R(0);
+            F2[B3 * nc + c] = R(0);
+            F1[B2 * nc + cg] = -F1[B2 * nc + cs];
+            if (nd > 2) F3[B2 * nc + cg] = -F3[B2 * nc + cs];
+        }
+        if (b.zero_polar_flux)
+            for (int v = 0; v < NV; ++v) F2[v * nc + c] = R(0);
+        return;
+    }
+    const unsigned t2 = t - npolar;
+    if (!b.periodic[0] && t2 < 2u * nj * nk) {
+        // X1 faces i = is (side 0) and i = ie + 1 (side 1), j in [js-1, je+1], k in [ks-1, ke+1]
+        const int side = t2 / (nj * nk), rem = t2 - side * (nj * nk);
+        const int k = k0 + rem / nj, j = j0 + rem % nj;
+        if (!b.check_inflow[side]) return;
+        const int i = side == 0 ? b.is : b.ie + 1;
+        const long c = (long)k * b.s3 + (long)j * b.s2 + i;
+        const R f = F1[RHO * nc + c];
+        F1[RHO * nc + c] = side == 0 ? smin(f, R(0)) : smax(f, R(0));
+    }
+}
+
+// One ghost zone of a physical X1 or X2 face, written from the final interior
+// state the way KBoundaries::ApplyBoundary leaves it (boundaries.cpp:369-589):
+// Parthenon's outflow (copy of the first / last interior zone) or reflecting
+// (mirror copy with the face-normal vector components negated) fill of the 8
+// conserved and 5 fluid primitive variables, check_inflow on X1 faces, then
+// BoundaryPtoUElseUtoP (kharma_package.cpp:125-156): B^i = U.B / sqrt(-g) and the
+// 5 fluid conserved variables from the primitives.  Faces apply in the order
+// inner/outer x1, inner/outer x2, then inner x1 again when fix_corner_inner
+// (boundaries.cpp:556-570); corner zones compose the two fills accordingly.
+template <int GEO>
+__device__ __forceinline__ void bc_fill_x1(const BlockDims& b, const GeomDev& gd, const double* __restrict__ Usrc,
+                                           const double* __restrict__ Psrc, int k, int j, int i, double P[NV], double Bu[3]) {
+    const bool inner = i < b.is;
+    const int type = b.bc[inner ? 0 : 1];
+    const int si = (type == KH_BC_OUTFLOW) ? (inner ? b.is : b.ie) : (inner ? 2 * b.is - 1 - i : 2 * b.ie + 1 - i);
+    const long sc = (long)k * b.s3 + (long)j * b.s2 + si;
+    for (int v = 0; v < 5; ++v) P[v] = Psrc[v * b.nc + sc];
+    for (int v = 0; v < 3; ++v) Bu[v] = Usrc[(B1 + v) * b.nc + sc];
+    if (type == KH_BC_REFLECT) { P[U1] = -P[U1]; Bu[0] = -Bu[0]; }
+    if (b.check_inflow[inner ? 0 : 1]) {
+        with_center_geom<GEO>(gd, j, i, [&](const auto& g) { check_inflow(g, P, inner); });
+    }
+}
+template <int GEO>
+__device__ __forceinline__ void bc_fill_x2(const BlockDims& b, const double* __restrict__ Usrc, const double* __restrict__ Psrc,
+                                           int k, int j, int i, double P[NV], double Bu[3]) {
+    const bool inner = j < b.js;
+    const int type = b.bc[inner ? 2 : 3];
+    const int sj = (type == KH_BC_OUTFLOW) ? (inner ? b.js : b.je) : (inner ? 2 * b.js - 1 - j : 2 * b.je + 1 - j);
+    const long sc = (long)k * b.s3 + (long)sj * b.s2 + i;
+    for (int v = 0; v < 5; ++v) P[v] = Psrc[v * b.nc + sc];
+    for (int v = 0; v < 3; ++v) Bu[v] = Usrc[(B1 + v) * b.nc + sc];
+    if (type == KH_BC_REFLECT) { P[U2] = -P[U2]; Bu[1] = -Bu[1]; }
+}
+template <int GEO>
+__global__ void __launch_bounds__(BS) bounds_kernel(BlockDims b, StepConsts cst, GeomDev gd, double* __restrict__ U,
+                                                    double* __restrict__ P, float* __restrict__ P32) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= b.nc) return;
+    const int k = (int)(t / b.s3), j = (int)((t - (long)k * b.s3) / b.s2), i = (int)(t - (long)k * b.s3 - (long)j * b.s2);
+    const bool x1g = !b.periodic[0] && (i < b.is || i > b.ie);
+    const bool x2g = b.ndim > 1 && !b.periodic[1] && (j < b.js || j > b.je);
+    if (!x1g && !x2g) return;
+    double Pz[NV], Bu[3];
+    if (x1g && !x2g) {
+        bc_fill_x1<GEO>(b, gd, U, P, k, j, i, Pz, Bu);
+    } else if (x2g && !x1g) {
+        bc_fill_x2<GEO>(b, U, P, k, j, i, Pz, Bu);
+    } else if (i < b.is && b.fix_corner_inner) {
+        // X2 fill of column is, then the inner-X1 fill over it
+        double Pm[NV], Bm[3];
+        bc_fill_x2<GEO>(b, U, P, k, j, b.is, Pm, Bm);
+        const int type = b.bc[0];
+        for (int v = 0; v < 5; ++v) Pz[v] = Pm[v];
+        for (int v = 0; v < 3; ++v) Bu[v] = Bm[v];
+        if (type == KH_BC_REFLECT) {
+            // a reflecting X1 face would read its mirror column instead
+            const int si = 2 * b.is - 1 - i;
+            bc_fill_x2<GEO>(b, U, P, k, j, si, Pz, Bu);
+            Pz[U1] = -Pz[U1]; Bu[0] = -Bu[0];
+        }
+        if (b.check_inflow[0]) with_center_geom<GEO>(gd, j, i, [&](const auto& g) { check_inflow(g, Pz, true); });
+    } else {
+        // X1 fill of the mirror / edge row, then the X2 fill over it
+        const bool inner = j < b.js;
+        const int type = b.bc[inner ? 2 : 3];
+        const int sj = (type == KH_BC_OUTFLOW) ? (inner ? b.js : b.je) : (inner ? 2 * b.js - 1 - j : 2 * b.je + 1 - j);
+        bc_fill_x1<GEO>(b, gd, U, P, k, sj, i, Pz, Bu);
+        if (type == KH_BC_REFLECT) { Pz[U2] = -Pz[U2]; Bu[1] = -Bu[1]; }
+    }
+    double Uz[NV];
+    with_center_geom<GEO>(gd, j, i, [&](const auto& g) {
+        const double gdet = g.gdet();
+        Pz[B1] = Bu[0] / gdet; Pz[B2] = Bu[1] / gdet; Pz[B3] = Bu[2] / gdet;
+        p_to_u(g, Pz, cst.gam, Uz);
+    });
+    for (int v = 0; v < 5; ++v) U[v * b.nc + t] = Uz[v];
+    for (int v = 0; v < 3; ++v) U[(B1 + v) * b.nc + t] = Bu[v];
+    for (int v = 0; v < NV; ++v) P[v * b.nc + t] = Pz[v];
+    if (P32) for (int v = 0; v < NV; ++v) P32[v * b.nc + t] = (float)Pz[v];
+}
+
 // ----------------------------------------------------- task-level kernels
 template <typename R>
 __global__ void __launch_bounds__(BS) divergence_kernel(BlockDims b, const R* __restrict__ F, double* __restrict__ dUdt) {
@@ -514,6 +726,22 @@ __global__ void __launch_bounds__(BS) divergence_kernel(BlockDims b, const R* __
     for (int v = 0; v < NV; ++v) dUdt[v * b.nc + c] = d[v];
 }
 
+// Flux::AddGeoSource (flux/flux.cpp:395-444) over the interior, from the primitives P
+__global__ void __launch_bounds__(BS) geo_source_kernel(BlockDims b, StepConsts cst, GeomDev gd, const double* __restrict__ P,
+                                                        double* __restrict__ dUdt) {
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (unsigned)ninterior(b)) return;
+    int k, j, i;
+    interior_index(b, t, k, j, i);
+    const long c = (long)k * b.s3 + (long)j * b.s2 + i;
+    double Pz[NV], d[NV];
+    for (int v = 0; v < NV; ++v) { Pz[v] = P[v * b.nc + c]; d[v] = dUdt[v * b.nc + c]; }
+    const long q = (long)j * gd.gn1 + i;
+    CachedGeom<double> g(gd.g[GLOC_CENTER], gd.np, q);
+    add_geo_source(g, gd, q, Pz, cst.gam, d);
+    for (int v = UU; v <= U3; ++v) dUdt[v * b.nc + c] = d[v];
+}
+
 // parthenon::Update::WeightedSumData pair used by AddStateUpdate, entire block
 __global__ void __launch_bounds__(BS) weighted_sum_kernel(long n, const double* __restrict__ Usub, const double* __restrict__ Ufull,
                                                           const double* __restrict__ dUdt, double g0, double g1, double bdt,
@@ -524,55 +752,76 @@ __global__ void __launch_bounds__(BS) weighted_sum_kernel(long n, const double* 
     Uout[t] = 1.0 * u + bdt * dUdt[t];
 }
 
-// Inverter::BlockUtoP + B_FluxCT::BlockUtoP over the entire block
-__global__ void __launch_bounds__(BS) utop_kernel(BlockDims b, StepConsts cst, const double* __restrict__ U, double* __restrict__ P,
-                                                  int* __restrict__ pflag, int* __restrict__ fflag) {
+__device__ __forceinline__ void zone_ijk(const BlockDims& b, long c, int& k, int& j, int& i) {
+    k = (int)(c / b.s3);
+    j = (int)((c - (long)k * b.s3) / b.s2);
+    i = (int)(c - (long)k * b.s3 - (long)j * b.s2);
+}
+
+// Inverter::BlockUtoP over the physical range + B_FluxCT::BlockUtoP over the entire block
+template <int GEO>
+__global__ void __launch_bounds__(BS) utop_kernel(BlockDims b, StepConsts cst, GeomDev gd, const double* __restrict__ U,
+                                                  double* __restrict__ P, int* __restrict__ pflag, int* __restrict__ fflag) {
     const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= b.nc) return;
-    double Uz[NV], Pz[NV];
-    for (int v = 0; v < NV; ++v) { Uz[v] = U[v * b.nc + c]; Pz[v] = P[v * b.nc + c]; }
-    MinkowskiGeom<double> g;
-    const int r = u_to_p_onedw(g, Uz, Pz, cst.gam, cst.iter_max, cst.err_tol);
-    for (int v = 0; v < 5; ++v) P[v * b.nc + c] = Pz[v];
-    for (int v = 0; v < 3; ++v) P[(B1 + v) * b.nc + c] = U[(B1 + v) * b.nc + c] / g.gdet();
-    pflag[c] = r % F_GEOM_RHO;
-    fflag[c] = (r / F_GEOM_RHO) * F_GEOM_RHO;
+    int k, j, i;
+    zone_ijk(b, c, k, j, i);
+    with_center_geom<GEO>(gd, j, i, [&](const auto& g) {
+        if (in_physical(b, k, j, i)) {
+            double Uz[NV], Pz[NV];
+            for (int v = 0; v < NV; ++v) { Uz[v] = U[v * b.nc + c]; Pz[v] = P[v * b.nc + c]; }
+            const int r = u_to_p_onedw(g, Uz, Pz, cst.gam, cst.iter_max, cst.err_tol);
+            for (int v = 0; v < 5; ++v) P[v * b.nc + c] = Pz[v];
+            pflag[c] = r % F_GEOM_RHO;
+            fflag[c] = (r / F_GEOM_RHO) * F_GEOM_RHO;
+        }
+        const double gdet = g.gdet();
+        for (int v = 0; v < 3; ++v) P[(B1 + v) * b.nc + c] = U[(B1 + v) * b.nc + c] / gdet;
+    });
 }
 
 // Floors::ApplyGRMHDFloors over the entire block: floors, ceilings and the
 // 5 hydrodynamic conserved variables of modified zones (p_to_u_mhd).
-__global__ void __launch_bounds__(BS) floors_kernel(BlockDims b, StepConsts cst, double* __restrict__ U, double* __restrict__ P,
-                                                    int* __restrict__ pflag, int* __restrict__ fflag) {
+template <int GEO>
+__global__ void __launch_bounds__(BS) floors_kernel(BlockDims b, StepConsts cst, GeomDev gd, double* __restrict__ U,
+                                                    double* __restrict__ P, int* __restrict__ pflag, int* __restrict__ fflag) {
     const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= b.nc) return;
+    int k, j, i;
+    zone_ijk(b, c, k, j, i);
     double Uz[NV], Pz[NV];
     for (int v = 0; v < NV; ++v) { Uz[v] = U[v * b.nc + c]; Pz[v] = P[v * b.nc + c]; }
-    MinkowskiGeom<double> g;
-    int pf = pflag[c];
-    FourVec<double> D;
-    calc_4vecs<double>(g, Pz, D);
-    const int ff = floors_zone(g, Pz, Uz, D, cst.gam, cst.floors, fflag[c], pf);
-    fflag[c] = ff;
-    pflag[c] = pf;
-    if (ff) {
-        double Un[NV];
-        p_to_u_D(g, Pz, D, cst.gam, Un);
-        for (int v = 0; v < 5; ++v) { P[v * b.nc + c] = Pz[v]; U[v * b.nc + c] = Un[v]; }
-    }
+    double rf, uf;
+    geom_floors<GEO>(cst, gd, j, i, rf, uf);
+    with_center_geom<GEO>(gd, j, i, [&](const auto& g) {
+        int pf = pflag[c];
+        FourVec<double> D;
+        calc_4vecs<double>(g, Pz, D);
+        const int ff = floors_zone(g, Pz, Uz, D, cst.gam, cst.floors, rf, uf, fflag[c], pf);
+        fflag[c] = ff;
+        pflag[c] = pf;
+        if (ff) {
+            double Un[NV];
+            p_to_u_D(g, Pz, D, cst.gam, Un);
+            for (int v = 0; v < 5; ++v) { P[v * b.nc + c] = Pz[v]; U[v * b.nc + c] = Un[v]; }
+        }
+    });
 }
 
-// Inverter::FixUtoP over the entire block (the reference's physical range for
-// a periodic block); two passes so averages only read non-failed zones.
+// Inverter::FixUtoP over the physical range; two passes so averages only read
+// non-failed zones.
 __global__ void __launch_bounds__(BS) fixup_avg_kernel(BlockDims b, StepConsts cst, double* __restrict__ P,
                                                        const int* __restrict__ pflag) {
     const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= b.nc || !(pflag[c] > 0)) return;
-    const int k = (int)(c / b.s3), j = (int)((c - (long)k * b.s3) / b.s2), i = (int)(c - (long)k * b.s3 - (long)j * b.s2);
+    int k, j, i;
+    zone_ijk(b, c, k, j, i);
+    if (!in_physical(b, k, j, i)) return;
     double wsum = 0., sum[5] = {0., 0., 0., 0., 0.};
     if (cst.fix_average) {
         for (int dn = -1; dn <= 1; dn++) for (int dm = -1; dm <= 1; dm++) for (int dl = -1; dl <= 1; dl++) {
             const int ii = i + dl, jj = j + dm, kk = k + dn;
-            if (ii < 0 || ii >= b.n1 || jj < 0 || jj >= b.n2 || kk < 0 || kk >= b.n3) continue;
+            if (!in_physical(b, kk, jj, ii)) continue;
             const long cc = (long)kk * b.s3 + (long)jj * b.s2 + ii;
             if (!(pflag[cc] > 0)) {
                 const double w = 1. / (abs(dl) + abs(dm) + abs(dn) + 1);
@@ -583,26 +832,34 @@ __global__ void __launch_bounds__(BS) fixup_avg_kernel(BlockDims b, StepConsts c
     }
     for (int p = 0; p < 5; ++p) P[p * b.nc + c] = (wsum < 1.e-10) ? 0. : sum[p] / wsum;
 }
-__global__ void __launch_bounds__(BS) fixup_floor_kernel(BlockDims b, StepConsts cst, double* __restrict__ U, double* __restrict__ P,
-                                                         const int* __restrict__ pflag) {
+template <int GEO>
+__global__ void __launch_bounds__(BS) fixup_floor_kernel(BlockDims b, StepConsts cst, GeomDev gd, double* __restrict__ U,
+                                                         double* __restrict__ P, const int* __restrict__ pflag) {
     const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= b.nc || !(pflag[c] > 0)) return;
+    int k, j, i;
+    zone_ijk(b, c, k, j, i);
+    if (!in_physical(b, k, j, i)) return;
     double Pz[NV], Uz[NV];
     for (int v = 0; v < NV; ++v) Pz[v] = P[v * b.nc + c];
-    apply_geo_floors<double>(Pz, cst.floors.rho_min_const, cst.floors.u_min_const);
-    MinkowskiGeom<double> g;
-    p_to_u(g, Pz, cst.gam, Uz);
+    double rf, uf;
+    geom_floors<GEO>(cst, gd, j, i, rf, uf);
+    apply_geo_floors<double>(Pz, rf, uf);
+    with_center_geom<GEO>(gd, j, i, [&](const auto& g) { p_to_u(g, Pz, cst.gam, Uz); });
     for (int v = 0; v < 5; ++v) { P[v * b.nc + c] = Pz[v]; U[v * b.nc + c] = Uz[v]; }
 }
 
 // Flux::MeshPtoU over the entire block
-__global__ void __launch_bounds__(BS) ptou_kernel(BlockDims b, StepConsts cst, const double* __restrict__ P, double* __restrict__ U) {
+template <int GEO>
+__global__ void __launch_bounds__(BS) ptou_kernel(BlockDims b, StepConsts cst, GeomDev gd, const double* __restrict__ P,
+                                                  double* __restrict__ U) {
     const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= b.nc) return;
+    int k, j, i;
+    zone_ijk(b, c, k, j, i);
     double Pz[NV], Uz[NV];
     for (int v = 0; v < NV; ++v) Pz[v] = P[v * b.nc + c];
-    MinkowskiGeom<double> g;
-    p_to_u(g, Pz, cst.gam, Uz);
+    with_center_geom<GEO>(gd, j, i, [&](const auto& g) { p_to_u(g, Pz, cst.gam, Uz); });
     for (int v = 0; v < NV; ++v) U[v * b.nc + c] = Uz[v];
 }
 
@@ -657,38 +914,69 @@ hipError_t launch_flux_ct_rewrite(const BlockDims& b, int prec, void* F, const v
     else hipLaunchKernelGGL(ct_rewrite_kernel<double>, grid_for(n), dim3(BS), 0, s, b, (double*)F, (const double*)emf);
     return hipGetLastError();
 }
+hipError_t launch_geo_source(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const double* P, double* dUdt,
+                             hipStream_t s) {
+    if (!b.geo) return hipSuccess;   // Cartesian Minkowski: no source (flux.cpp:407)
+    hipLaunchKernelGGL(geo_source_kernel, grid_for(ninterior_host(b)), dim3(BS), 0, s, b, c, gd, P, dUdt);
+    return hipGetLastError();
+}
 hipError_t launch_flux_divergence(const BlockDims& b, int prec, const void* F, double* dUdt, hipStream_t s) {
     const long n = (long)b.nx1 * (b.ndim > 1 ? b.nx2 : 1) * (b.ndim > 2 ? b.nx3 : 1);
     if (prec == PREC_SPLIT) hipLaunchKernelGGL(divergence_kernel<float>, grid_for(n), dim3(BS), 0, s, b, (const float*)F, dUdt);
     else hipLaunchKernelGGL(divergence_kernel<double>, grid_for(n), dim3(BS), 0, s, b, (const double*)F, dUdt);
     return hipGetLastError();
 }
-hipError_t launch_update(const BlockDims& b, const StepConsts& c, int prec, int stage, bool do_dt,
+template <typename R, int GEO>
+static void launch_update_t(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int stage, bool do_dt,
+                            const void* F, const void* ctop, const double* Usub, const double* Ufull, const double* Psub,
+                            double* Uout, double* Pout, float* Pout32, int* pflag, int* fflag,
+                            int* fail_list, DeviceState* ds, double* dt_part, hipStream_t s) {
+    const dim3 g = grid_for(ninterior_host(b));
+    const bool ct = b.ndim > 1;
+#define KH_UPD(DT, CT)                                                                                         \
+    hipLaunchKernelGGL((update_kernel<R, DT, CT, GEO>), g, dim3(BS), 0, s, b, c, gd, stage, (const R*)F, (const R*)ctop, \
+                       Usub, Ufull, Psub, Uout, Pout, Pout32, pflag, fflag, fail_list, ds, dt_part)
+    if (do_dt) { if (ct) KH_UPD(true, true); else KH_UPD(true, false); }
+    else { if (ct) KH_UPD(false, true); else KH_UPD(false, false); }
+#undef KH_UPD
+}
+hipError_t launch_update(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int prec, int stage, bool do_dt,
                          const void* F, const void* ctop, const double* Usub, const double* Ufull, const double* Psub,
                          double* Uout, double* Pout, float* Pout32, int* pflag, int* fflag,
                          int* fail_list, DeviceState* ds, double* dt_part, hipStream_t s) {
-    const dim3 g = grid_for(ninterior_host(b));
-    const bool ct = b.ndim > 1;
-#define KH_UPD(R, DT, CT)                                                                              \
-    hipLaunchKernelGGL((update_kernel<R, DT, CT>), g, dim3(BS), 0, s, b, c, stage, (const R*)F, (const R*)ctop, \
-                       Usub, Ufull, Psub, Uout, Pout, Pout32, pflag, fflag, fail_list, ds, dt_part)
     if (prec == PREC_SPLIT) {
-        if (do_dt) { if (ct) KH_UPD(float, true, true); else KH_UPD(float, true, false); }
-        else { if (ct) KH_UPD(float, false, true); else KH_UPD(float, false, false); }
+        if (b.geo) launch_update_t<float, 1>(b, c, gd, stage, do_dt, F, ctop, Usub, Ufull, Psub, Uout, Pout, Pout32, pflag, fflag, fail_list, ds, dt_part, s);
+        else launch_update_t<float, 0>(b, c, gd, stage, do_dt, F, ctop, Usub, Ufull, Psub, Uout, Pout, Pout32, pflag, fflag, fail_list, ds, dt_part, s);
     } else {
-        if (do_dt) { if (ct) KH_UPD(double, true, true); else KH_UPD(double, true, false); }
-        else { if (ct) KH_UPD(double, false, true); else KH_UPD(double, false, false); }
+        if (b.geo) launch_update_t<double, 1>(b, c, gd, stage, do_dt, F, ctop, Usub, Ufull, Psub, Uout, Pout, Pout32, pflag, fflag, fail_list, ds, dt_part, s);
+        else launch_update_t<double, 0>(b, c, gd, stage, do_dt, F, ctop, Usub, Ufull, Psub, Uout, Pout, Pout32, pflag, fflag, fail_list, ds, dt_part, s);
     }
-#undef KH_UPD
     return hipGetLastError();
 }
 long update_blocks(const BlockDims& b) { return (ninterior_host(b) + BS - 1) / BS; }
 long divb_blocks(const BlockDims& b) {
     return ((long)(b.nx1 + 1) * (b.nx2 + 1) * ((b.ndim > 2) ? b.nx3 + 1 : 1) + BS - 1) / BS;
 }
-hipError_t launch_fixup(const BlockDims& b, const StepConsts& c, int stage, double* U, double* P, float* P32,
-                        const int* pflag, const int* fail_list, DeviceState* ds, hipStream_t s) {
-    hipLaunchKernelGGL(fixup_list_kernel, dim3(64), dim3(BS), 0, s, b, c, stage, U, P, P32, pflag, fail_list, ds);
+hipError_t launch_fixup(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int stage, double* U, double* P,
+                        float* P32, const int* pflag, const int* fail_list, DeviceState* ds, hipStream_t s) {
+    if (b.geo) hipLaunchKernelGGL(fixup_list_kernel<1>, dim3(64), dim3(BS), 0, s, b, c, gd, stage, U, P, P32, pflag, fail_list, ds);
+    else hipLaunchKernelGGL(fixup_list_kernel<0>, dim3(64), dim3(BS), 0, s, b, c, gd, stage, U, P, P32, pflag, fail_list, ds);
+    return hipGetLastError();
+}
+hipError_t launch_fix_flux(const BlockDims& b, int prec, void* F, hipStream_t s) {
+    const long npolar = (b.ndim > 1 && !b.periodic[1]) ? 2L * (b.nx1 + 2) * ((b.ndim > 2) ? b.nx3 + 2 : 1) : 0;
+    const long nx1f = !b.periodic[0] ? 2L * ((b.ndim > 1) ? b.nx2 + 2 : 1) * ((b.ndim > 2) ? b.nx3 + 2 : 1) : 0;
+    const long n = npolar + nx1f;
+    if (n == 0) return hipSuccess;
+    if (prec == PREC_SPLIT) hipLaunchKernelGGL(fix_flux_kernel<float>, grid_for(n), dim3(BS), 0, s, b, (float*)F);
+    else hipLaunchKernelGGL(fix_flux_kernel<double>, grid_for(n), dim3(BS), 0, s, b, (double*)F);
+    return hipGetLastError();
+}
+hipError_t launch_bounds(const BlockDims& b, const StepConsts& c, const GeomDev& gd, double* U, double* P, float* P32,
+                         hipStream_t s) {
+    if (b.periodic[0] && (b.ndim < 2 || b.periodic[1])) return hipSuccess;
+    if (b.geo) hipLaunchKernelGGL(bounds_kernel<1>, grid_for(b.nc), dim3(BS), 0, s, b, c, gd, U, P, P32);
+    else hipLaunchKernelGGL(bounds_kernel<0>, grid_for(b.nc), dim3(BS), 0, s, b, c, gd, U, P, P32);
     return hipGetLastError();
 }
 hipError_t launch_finalize(const StepConsts& c, DeviceState* ds, const double* dt_part, int n_dt,
@@ -719,26 +1007,31 @@ hipError_t launch_weighted_sum(const BlockDims& b, const double* Usub, const dou
     hipLaunchKernelGGL(weighted_sum_kernel, grid_for(n), dim3(BS), 0, s, n, Usub, Ufull, dUdt, gam0, gam1, beta_dt, Uout);
     return hipGetLastError();
 }
-hipError_t launch_utop(const BlockDims& b, const StepConsts& c, const double* U, double* P, int* pflag, int* fflag,
-                       hipStream_t s) {
-    hipLaunchKernelGGL(utop_kernel, grid_for(b.nc), dim3(BS), 0, s, b, c, U, P, pflag, fflag);
+hipError_t launch_utop(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const double* U, double* P,
+                       int* pflag, int* fflag, hipStream_t s) {
+    if (b.geo) hipLaunchKernelGGL(utop_kernel<1>, grid_for(b.nc), dim3(BS), 0, s, b, c, gd, U, P, pflag, fflag);
+    else hipLaunchKernelGGL(utop_kernel<0>, grid_for(b.nc), dim3(BS), 0, s, b, c, gd, U, P, pflag, fflag);
     return hipGetLastError();
 }
-hipError_t launch_floors(const BlockDims& b, const StepConsts& c, double* U, double* P, int* pflag, int* fflag,
-                         hipStream_t s) {
+hipError_t launch_floors(const BlockDims& b, const StepConsts& c, const GeomDev& gd, double* U, double* P, int* pflag,
+                         int* fflag, hipStream_t s) {
     if (!c.floors.on) return hipSuccess;
-    hipLaunchKernelGGL(floors_kernel, grid_for(b.nc), dim3(BS), 0, s, b, c, U, P, pflag, fflag);
+    if (b.geo) hipLaunchKernelGGL(floors_kernel<1>, grid_for(b.nc), dim3(BS), 0, s, b, c, gd, U, P, pflag, fflag);
+    else hipLaunchKernelGGL(floors_kernel<0>, grid_for(b.nc), dim3(BS), 0, s, b, c, gd, U, P, pflag, fflag);
     return hipGetLastError();
 }
-hipError_t launch_fixup_entire(const BlockDims& b, const StepConsts& c, double* U, double* P, const int* pflag,
-                               hipStream_t s) {
+hipError_t launch_fixup_entire(const BlockDims& b, const StepConsts& c, const GeomDev& gd, double* U, double* P,
+                               const int* pflag, hipStream_t s) {
     if (!c.fix_average && !c.fix_atmo) return hipSuccess;
     hipLaunchKernelGGL(fixup_avg_kernel, grid_for(b.nc), dim3(BS), 0, s, b, c, P, pflag);
-    hipLaunchKernelGGL(fixup_floor_kernel, grid_for(b.nc), dim3(BS), 0, s, b, c, U, P, pflag);
+    if (b.geo) hipLaunchKernelGGL(fixup_floor_kernel<1>, grid_for(b.nc), dim3(BS), 0, s, b, c, gd, U, P, pflag);
+    else hipLaunchKernelGGL(fixup_floor_kernel<0>, grid_for(b.nc), dim3(BS), 0, s, b, c, gd, U, P, pflag);
     return hipGetLastError();
 }
-hipError_t launch_ptou(const BlockDims& b, const StepConsts& c, const double* P, double* U, hipStream_t s) {
-    hipLaunchKernelGGL(ptou_kernel, grid_for(b.nc), dim3(BS), 0, s, b, c, P, U);
+hipError_t launch_ptou(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const double* P, double* U,
+                       hipStream_t s) {
+    if (b.geo) hipLaunchKernelGGL(ptou_kernel<1>, grid_for(b.nc), dim3(BS), 0, s, b, c, gd, P, U);
+    else hipLaunchKernelGGL(ptou_kernel<0>, grid_for(b.nc), dim3(BS), 0, s, b, c, gd, P, U);
     return hipGetLastError();
 }
 hipError_t launch_sync(const BlockDims& b, double* A, int nvar, hipStream_t s) {

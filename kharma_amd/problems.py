@@ -164,3 +164,50 @@ CONFIGS = {
     "orszag_tang": orszag_tang_config,
     "mhdmode": mhdmode_config,
 }
+
+
+# ------------------------------------------------------------------ torus
+def torus_config(nx1=192, nx2=128, nx3=128, a=0.9375, r_out=1000.0, gamma=13.0 / 9.0,
+                 transform="fmks", u_jitter=0.08, rng_seed=31337, steps=200):
+    """Fishbone-Moncrief torus around a Kerr black hole in Kerr-Schild coordinates
+    (KHARMA's torus problem: prob/fm_torus.cpp, kharma.cpp:182-243).
+
+    x1 = log r from x1min to log(r_out), x1min placing 5.5 zones inside the
+    horizon (kharma.cpp:211-224); x2 in [0, 1] (MKS/FMKS polar coordinate);
+    x3 in [0, 2 pi).  FMKS zero point at x1min.  Outflow x1 / reflecting x2 /
+    periodic x3 boundaries with the spherical defaults: inflow checks on both
+    x1 faces, zero polar fluxes, the polar B-flux fix and the inner corner fix
+    (the domain reaches inside the horizon).  Spherical floor defaults
+    (floors.hpp:123-166) in the drift frame.  One meshblock for the whole mesh.
+    """
+    r_hor = 1.0 + math.sqrt(1.0 - a * a)
+    x1max = math.log(r_out)
+    x1hor = math.log(r_hor)
+    x1min = (nx1 * x1hor / 5.5 - x1max) / (-1.0 + nx1 / 5.5)
+    cfg = dict(
+        nx=(nx1, nx2, nx3), ng=4,
+        bounds=((x1min, x1max), (0.0, 1.0), (0.0, 2.0 * math.pi)),
+        coords="ks", a=a, transform=transform,
+        hslope=0.3, mks_smooth=0.5, poly_xt=0.82, poly_alpha=14.0, fmks_zero_point=x1min,
+        gamma=gamma, cfl=0.9, dt_min=1e-5, dt_start=1e-5, tlim=1e30,
+        recon="weno5", riemann="llf", floors=True, floor_frame="drift",
+        floor_values=dict(rho_min_geom=1e-6, u_min_geom=1e-8, rho_min_const=1e-20, u_min_const=1e-20,
+                          r_char=10.0, bsq_over_rho_max=1e20, bsq_over_u_max=1e20, u_over_rho_max=1e20,
+                          ktot_max=1e20, gamma_max=50.0),
+        boundaries=dict(inner_x1="outflow", outer_x1="outflow", inner_x2="reflecting", outer_x2="reflecting"),
+        check_inflow_inner_x1=True, check_inflow_outer_x1=True,
+        zero_polar_flux=True, fix_polar_flux=True, fix_corner_inner=(math.exp(x1min) < r_hor),
+        torus=dict(rin=6.0, rmax=12.0, kappa=1e-3, u_jitter=u_jitter, rng_seed=rng_seed,
+                   beta_min=100.0, min_A=0.2),
+        steps=steps,
+    )
+    if nx3 == 1:
+        cfg["bounds"] = ((x1min, x1max), (0.0, 1.0), (0.0, 2.0 * math.pi))
+    return cfg
+
+
+def torus_prims(cfg):
+    """Torus initial primitives (interior; the driver fills the ghosts), generated
+    by the library's host-side problem set-up."""
+    import kharma_amd
+    return kharma_amd.torus_prims(cfg)

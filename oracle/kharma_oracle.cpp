@@ -81,7 +81,14 @@ struct or_params {
     double err_tol;                    // inverter/err_tol
     int fix_average, fix_atmo;         // inverter/fix_average_neighbors, fix_atmosphere
     int integrator;                    // 0 rk2, 1 vl2
-    int spherical;                     // coordinates/spherical (Minkowski: 0)
+    int spherical;                     // coordinates/base: 0 cartesian_minkowski, 1 spherical_ks
+    double a;                          // coordinates/a
+    int transform;                     // coordinates/transform: 0 null, 1 mks, 2 fmks
+    double hslope, mks_smooth, poly_xt, poly_alpha, fmks_zero_point;
+    int bc_inner_x1, bc_outer_x1, bc_inner_x2, bc_outer_x2;   // 1 outflow, 2 reflecting
+    int check_inflow_inner_x1, check_inflow_outer_x1;
+    int zero_polar_flux, fix_polar_flux, fix_corner_inner;
+    int periodic1, periodic2, periodic3;
 };
 }
 
@@ -97,6 +104,7 @@ struct Block {
     // Geometry cache, gr_coordinates.hpp:86-90: (loc, j, i, mu, nu) over (n2+1) x (n1+1)
     int gn1, gn2;
     std::vector<Real> gcov, gcon, gdet, gdet_conn;
+    std::vector<Real> rcen;   // embedding radius of each (j, i) cell centre
     bool cart_minkowski = true;
     // Containers: base (full step init / final), s1 (stage 1)
     std::vector<Real> U0, P0, U1c, P1c;
@@ -210,6 +218,145 @@ void reconstruct(int recon, Real x1, Real x2, Real x3, Real x4, Real x5, Real& l
     }
 }
 
+// Spherical Kerr-Schild base with null / "modified" / "funky" transforms:
+// coordinate_systems.hpp:113-151 (gcov_embed), :351-672 (transforms),
+// coordinate_embedding.hpp:461-545 (gcov_native, conn_native), matrix.hpp:57-108
+// (adjoint/determinant inverse), cached as init_GRCoordinates does
+// (gr_coordinates.cpp:96-205, connection_average_points = 1).
+struct KSEmbed {
+    Real a, hslope, smooth, xt, alpha, x0, norm;
+    int transform;
+    void embed(const Real X[4], Real& r, Real& th) const {
+        if (transform == 0) { r = X[1]; th = X[2]; return; }
+        r = std::exp(X[1]);
+        const Real thG = M_PI*X[2] + ((1. - hslope)/2.)*std::sin(2.*M_PI*X[2]);
+        if (transform == 1) { th = thG; return; }
+        const Real y = 2*X[2] - 1.;
+        const Real thJ = norm * y * (1. + std::pow(y/xt, alpha) / (alpha + 1.)) + 0.5 * M_PI;
+        th = thG + std::exp(smooth * (x0 - X[1])) * (thJ - thG);
+    }
+    void dxdX(const Real X[4], Real d[4][4]) const {
+        for (int m = 0; m < 4; ++m) for (int n = 0; n < 4; ++n) d[m][n] = (m == n);
+        if (transform == 0) return;
+        d[1][1] = std::exp(X[1]);
+        if (transform == 1) { d[2][2] = M_PI - (hslope - 1.)*M_PI*std::cos(2.*M_PI*X[2]); return; }
+        const Real e = std::exp(smooth * (x0 - X[1]));
+        d[2][1] = -e * smooth * (M_PI/2. - M_PI*X[2] + norm*(2.*X[2] - 1.)*(1 + std::pow((-1. + 2*X[2])/xt, alpha)/(1 + alpha))
+                                 - 1./2.*(1. - hslope)*std::sin(2.*M_PI*X[2]));
+        d[2][2] = M_PI + (1. - hslope)*M_PI*std::cos(2.*M_PI*X[2])
+                + e * (-M_PI + 2.*norm*(1. + std::pow((2.*X[2] - 1.)/xt, alpha)/(alpha + 1.))
+                       + (2.*alpha*norm*(2.*X[2] - 1.)*std::pow((2.*X[2] - 1.)/xt, alpha - 1.))/((1. + alpha)*xt)
+                       - (1. - hslope)*M_PI*std::cos(2.*M_PI*X[2]));
+    }
+    void gcov_ks(Real r, Real th_in, Real g[4][4]) const {
+        auto excise = [](Real n, Real c, Real w) { return (std::fabs(n - c) > w) ? n : ((n > c) ? c + w : c - w); };
+        const Real th = excise(excise(th_in, 0.0, 1e-20), M_PI, 1e-20);
+        const Real cth = std::cos(th), sth = std::sin(th), sin2 = sth*sth, rho2 = r*r + a*a*cth*cth;
+        g[0][0] = -1. + 2.*r/rho2;  g[0][1] = 2.*r/rho2;  g[0][2] = 0.;  g[0][3] = -2.*a*r*sin2/rho2;
+        g[1][0] = 2.*r/rho2;  g[1][1] = 1. + 2.*r/rho2;  g[1][2] = 0.;  g[1][3] = -a*sin2*(1. + 2.*r/rho2);
+        g[2][0] = 0.;  g[2][1] = 0.;  g[2][2] = rho2;  g[2][3] = 0.;
+        g[3][0] = -2.*a*r*sin2/rho2;  g[3][1] = -a*sin2*(1. + 2.*r/rho2);  g[3][2] = 0.;
+        g[3][3] = sin2*(rho2 + a*a*sin2*(1. + 2.*r/rho2));
+    }
+    void gcov_native(const Real X[4], Real g[4][4]) const {
+        Real r, th, ge[4][4], d[4][4];
+        embed(X, r, th);
+        gcov_ks(r, th, ge);
+        dxdX(X, d);
+        for (int mu = 0; mu < 4; ++mu) for (int nu = 0; nu < 4; ++nu) {
+            g[mu][nu] = 0;
+            for (int lam = 0; lam < 4; ++lam) for (int kap = 0; kap < 4; ++kap)
+                g[mu][nu] += ge[lam][kap]*d[lam][mu]*d[kap][nu];
+        }
+    }
+};
+Real minor3(const Real m[16], int r0, int r1, int r2, int c0, int c1, int c2) {
+    return m[4*r0+c0]*(m[4*r1+c1]*m[4*r2+c2] - m[4*r2+c1]*m[4*r1+c2]) -
+           m[4*r0+c1]*(m[4*r1+c0]*m[4*r2+c2] - m[4*r2+c0]*m[4*r1+c2]) +
+           m[4*r0+c2]*(m[4*r1+c0]*m[4*r2+c1] - m[4*r2+c0]*m[4*r1+c1]);
+}
+// inverse = adjugate / determinant; returns the determinant
+Real invert4(const Real* m, Real* inv) {
+    static const int R[4][3] = {{1, 2, 3}, {0, 2, 3}, {0, 1, 3}, {0, 1, 2}};
+    for (int i = 0; i < 4; ++i) for (int j = 0; j < 4; ++j) {
+        // adj[i][j] = (-1)^(i+j) * minor deleting row j, column i
+        const Real mn = minor3(m, R[j][0], R[j][1], R[j][2], R[i][0], R[i][1], R[i][2]);
+        inv[4*i + j] = ((i + j) % 2) ? -mn : mn;
+    }
+    const Real det = m[0]*minor3(m,1,2,3,1,2,3) - m[1]*minor3(m,1,2,3,0,2,3) +
+                     m[2]*minor3(m,1,2,3,0,1,3) - m[3]*minor3(m,1,2,3,0,1,2);
+    const Real id = 1. / det;
+    for (int c = 0; c < 16; ++c) inv[c] *= id;
+    return det;
+}
+void init_geometry_ks(Block& b) {
+    const or_params& p = b.p;
+    KSEmbed E{p.a, p.hslope, p.mks_smooth, p.poly_xt, p.poly_alpha, p.fmks_zero_point, 0., p.transform};
+    E.norm = 0.5 * M_PI * 1./(1. + 1./(p.poly_alpha + 1.) * 1./std::pow(p.poly_xt, p.poly_alpha));
+    b.gn1 = b.n1 + 1; b.gn2 = b.n2 + 1;
+    const size_t ng = (size_t)b.gn1 * b.gn2;
+    b.gcov.assign(NLOC * ng * 16, 0.); b.gcon.assign(NLOC * ng * 16, 0.); b.gdet.assign(NLOC * ng, 0.);
+    b.gdet_conn.assign(ng * 64, 0.);
+    b.rcen.assign(ng, 0.);
+    const Real d1 = b.dx[0], d2 = b.dx[1];
+    for (int j = 0; j < b.gn2; ++j)
+    for (int i = 0; i < b.gn1; ++i) {
+        for (int loc = 0; loc < NLOC; ++loc) {
+            // GRCoordinates::coord (gr_coordinates.hpp:184-225) at k = 0
+            const bool f1 = (loc == FACE1 || loc == CORNER), f2 = (loc == FACE2 || loc == CORNER);
+            Real X[4] = {0., p.x1min + (i - b.is + (f1 ? 0. : 0.5)) * d1,
+                         b.ndim > 1 ? p.x2min + (j - b.js + (f2 ? 0. : 0.5)) * d2 : 0.5*(p.x2min + p.x2max),
+                         p.x3min + 0.5 * b.dx[2]};
+            Real g[4][4], gi[4][4];
+            E.gcov_native(X, g);
+            const Real det = invert4(&g[0][0], &gi[0][0]);
+            const size_t q = b.gidx(loc, j, i);
+            for (int mu = 0; mu < 4; ++mu) for (int nu = 0; nu < 4; ++nu) {
+                b.gcov[q*16 + mu*4 + nu] = g[mu][nu];
+                b.gcon[q*16 + mu*4 + nu] = gi[mu][nu];
+            }
+            const Real gdet = std::sqrt(std::fabs(det));
+            b.gdet[q] = gdet;
+            if (loc == CENTER) {
+                Real r, th; E.embed(X, r, th);
+                b.rcen[(size_t)j * b.gn1 + i] = r;
+                if (j < b.n2 && i < b.n1) {
+                    // conn_native with delta = 1e-8 (gr_coordinates.cpp:51)
+                    Real conn[4][4][4], tmp[4][4][4];
+                    for (int nu = 0; nu < 4; ++nu) {
+                        Real Xl[4], Xh[4], gh[4][4], gl[4][4];
+                        for (int mu = 0; mu < 4; ++mu) { Xl[mu] = X[mu] - 1e-8*(mu == nu); Xh[mu] = X[mu] + 1e-8*(mu == nu); }
+                        E.gcov_native(Xh, gh); E.gcov_native(Xl, gl);
+                        for (int lam = 0; lam < 4; ++lam) for (int kap = 0; kap < 4; ++kap)
+                            conn[lam][kap][nu] = (gh[lam][kap] - gl[lam][kap]) / (Xh[nu] - Xl[nu]);
+                    }
+                    for (int lam = 0; lam < 4; ++lam) for (int nu = 0; nu < 4; ++nu) for (int mu = 0; mu < 4; ++mu)
+                        tmp[lam][nu][mu] = 0.5 * (conn[nu][lam][mu] + conn[mu][lam][nu] - conn[mu][nu][lam]);
+                    for (int lam = 0; lam < 4; ++lam) for (int nu = 0; nu < 4; ++nu) for (int mu = 0; mu < 4; ++mu) {
+                        Real c = 0.;
+                        for (int kap = 0; kap < 4; ++kap) c += gi[lam][kap] * tmp[kap][nu][mu];
+                        b.gdet_conn[(((size_t)j * b.gn1 + i) * 4 + lam) * 16 + nu * 4 + mu] = gdet * c;
+                    }
+                }
+            }
+        }
+    }
+    b.cart_minkowski = false;
+}
+
+// Geometric floors at cell centre (j, i): floors_functions.hpp:97-108 / 312-323
+void geom_floors(const Block& b, int j, int i, Real& rhoflr, Real& uflr) {
+    if (b.p.spherical) {
+        const Real r = b.rcen[(size_t)j * b.gn1 + i];
+        const Real rhoscal = b.p.use_r_char ? 1. / ((r*r) * (1 + r / b.p.r_char)) : 1. / std::sqrt(r*r*r);
+        rhoflr = std::max(b.p.rho_min_geom * rhoscal, b.p.rho_min_const);
+        uflr = std::max(b.p.u_min_geom * std::pow(rhoscal, b.p.gam), b.p.u_min_const);
+    } else {
+        rhoflr = b.p.rho_min_const;
+        uflr = b.p.u_min_const;
+    }
+}
+
 // ----------------------------------------------------------- GRMHD physics
 // grmhd_functions.hpp:116-129 (Local lorentz_calc)
 Real lorentz_calc(const Block& b, const Real* P, int j, int i, int loc) {
@@ -302,11 +449,11 @@ inline Real hlle(Real fL, Real fR, Real cmax, Real cmin, Real Ul, Real Ur) {
 }
 
 // -------------------------------------------------------------- floors
-// floors/floors_functions.hpp:300-332 (Local apply_geo_floors).  Spherical
-// geometry is not yet part of this oracle; Cartesian uses the constant floors.
-int apply_geo_floors(const Block& b, Real* P) {
-    const Real rhoflr_geom = b.p.rho_min_const;
-    const Real uflr_geom = b.p.u_min_const;
+// floors/floors_functions.hpp:300-332 (Local apply_geo_floors): floors of the
+// cell centre (j, i) whatever the face locus (G.r(0, j, i) at :311).
+int apply_geo_floors(const Block& b, Real* P, int j, int i) {
+    Real rhoflr_geom, uflr_geom;
+    geom_floors(b, j, i, rhoflr_geom, uflr_geom);
     int fflag = 0;
     fflag |= (rhoflr_geom > P[RHO]) * F_GEOM_RHO_FLUX;
     fflag |= (uflr_geom > P[UU]) * F_GEOM_U_FLUX;
@@ -357,8 +504,8 @@ void get_flux(Block& b, const std::vector<Real>& P, int dir) {
             }
             // get_flux.hpp:170-182
             if (b.p.recon_floors) {
-                apply_geo_floors(b, Pl);
-                apply_geo_floors(b, Pr);
+                apply_geo_floors(b, Pl, j, i);
+                apply_geo_floors(b, Pr, j, i);
             }
             for (int v = 0; v < NV; ++v) { b.Pl[v * b.nc + c] = Pl[v]; b.Pr[v * b.nc + c] = Pr[v]; }
             Real Ul[NV], Ur[NV], Fl[NV], Fr[NV];
@@ -518,16 +665,17 @@ void sync_periodic_var(Block& b, std::vector<Real>& A, int v) {
     const int N1 = b.p.nx1, N2 = b.p.nx2, N3 = b.p.nx3;
     auto wrap = [](int x, int s, int N) { int r = (x - s) % N; if (r < 0) r += N; return s + r; };
     // x1 then x2 then x3 sweep over the full extents fills edges and corners
+    if (b.p.periodic1)
     for (int k = 0; k < b.n3; ++k) for (int j = 0; j < b.n2; ++j) for (int i = 0; i < b.n1; ++i) {
         if (i >= b.is && i <= b.ie) continue;
         a[b.id(k, j, i)] = a[b.id(k, j, wrap(i, b.is, N1))];
     }
-    if (b.ndim > 1)
+    if (b.ndim > 1 && b.p.periodic2)
         for (int k = 0; k < b.n3; ++k) for (int j = 0; j < b.n2; ++j) {
             if (j >= b.js && j <= b.je) continue;
             for (int i = 0; i < b.n1; ++i) a[b.id(k, j, i)] = a[b.id(k, wrap(j, b.js, N2), i)];
         }
-    if (b.ndim > 2)
+    if (b.ndim > 2 && b.p.periodic3)
         for (int k = 0; k < b.n3; ++k) {
             if (k >= b.ks && k <= b.ke) continue;
             for (int j = 0; j < b.n2; ++j) for (int i = 0; i < b.n1; ++i)
@@ -640,14 +788,25 @@ int u_to_p_onedw(const Block& b, const Real* U, Real* P, int j, int i, int loc, 
     return S_SUCCESS;
 }
 
-// Packages::MeshUtoP -> Inverter::BlockUtoP (inverter.cpp:131-200, physical
-// range = entire for a periodic block, domain.hpp:161-187) then
-// B_FluxCT::BlockUtoP (b_flux_ct.cpp:195-217, entire).
+// KDomain::GetPhysicalRange (domain.hpp:161-187): the interior along
+// non-periodic (physical-boundary) dimensions, the entire block along periodic ones.
+struct Range3 { int is, ie, js, je, ks, ke; };
+Range3 physical_range(const Block& b) {
+    Range3 r{0, b.n1 - 1, 0, b.n2 - 1, 0, b.n3 - 1};
+    if (!b.p.periodic1) { r.is = b.is; r.ie = b.ie; }
+    if (b.ndim > 1 && !b.p.periodic2) { r.js = b.js; r.je = b.je; }
+    if (b.ndim > 2 && !b.p.periodic3) { r.ks = b.ks; r.ke = b.ke; }
+    return r;
+}
+
+// Packages::MeshUtoP -> Inverter::BlockUtoP (inverter.cpp:131-200, over the
+// physical range) then B_FluxCT::BlockUtoP (b_flux_ct.cpp:195-217, entire).
 void utop(Block& b, std::vector<Real>& U, std::vector<Real>& P) {
+    const Range3 pr = physical_range(b);
 #pragma omp parallel for collapse(2)
-    for (int k = 0; k < b.n3; ++k)
-    for (int j = 0; j < b.n2; ++j)
-    for (int i = 0; i < b.n1; ++i) {
+    for (int k = pr.ks; k <= pr.ke; ++k)
+    for (int j = pr.js; j <= pr.je; ++j)
+    for (int i = pr.is; i <= pr.ie; ++i) {
         const size_t c = b.id(k, j, i);
         Real Uc[NV], Pc[NV];
         load_prims(b, U, c, Uc); load_prims(b, P, c, Pc);
@@ -671,8 +830,7 @@ int determine_floors(const Block& b, const Real* P, int j, int i, Real& rhoflr_m
     const auto& f = b.p;
     const Real gam = f.gam;
     Real rhoflr_geom, uflr_geom;
-    rhoflr_geom = f.rho_min_const;
-    uflr_geom = f.u_min_const;
+    geom_floors(b, j, i, rhoflr_geom, uflr_geom);
     FourVectors D;
     calc_4vecs(b, P, j, i, CENTER, D);
     Real bsq = dot(D.bcon, D.bcov);
@@ -827,22 +985,23 @@ void apply_floors(Block& b, std::vector<Real>& U, std::vector<Real>& P) {
     }
 }
 
-// Inverter::FixUtoP, inverter/fixup.cpp:47-156 (physical range = entire here)
+// Inverter::FixUtoP, inverter/fixup.cpp:47-156 over the physical range
 void fix_utop(Block& b, std::vector<Real>& U, std::vector<Real>& P) {
     if (!b.p.fix_average && !b.p.fix_atmo) return;
+    const Range3 pr = physical_range(b);
     // First pass reads only non-failed zones and writes only failed ones, so it is
     // order independent.
 #pragma omp parallel for collapse(2)
-    for (int k = 0; k < b.n3; ++k)
-    for (int j = 0; j < b.n2; ++j)
-    for (int i = 0; i < b.n1; ++i) {
+    for (int k = pr.ks; k <= pr.ke; ++k)
+    for (int j = pr.js; j <= pr.je; ++j)
+    for (int i = pr.is; i <= pr.ie; ++i) {
         const size_t c = b.id(k, j, i);
         if (!(b.pflag[c] > 0)) continue;
         double wsum = 0., sum[5] = {0};
         if (b.p.fix_average) {
             for (int n = -1; n <= 1; n++) for (int m = -1; m <= 1; m++) for (int l = -1; l <= 1; l++) {
                 int ii = i + l, jj = j + m, kk = k + n;
-                if (ii < 0 || ii >= b.n1 || jj < 0 || jj >= b.n2 || kk < 0 || kk >= b.n3) continue;
+                if (ii < pr.is || ii > pr.ie || jj < pr.js || jj > pr.je || kk < pr.ks || kk > pr.ke) continue;
                 const size_t cc = b.id(kk, jj, ii);
                 if (!(b.pflag[cc] > 0)) {
                     double w = 1./(std::abs(l) + std::abs(m) + std::abs(n) + 1);
@@ -855,16 +1014,18 @@ void fix_utop(Block& b, std::vector<Real>& U, std::vector<Real>& P) {
         else { for (int p = 0; p < 5; ++p) P[p * b.nc + c] = sum[p]/wsum; }
     }
 #pragma omp parallel for collapse(2)
-    for (int k = 0; k < b.n3; ++k)
-    for (int j = 0; j < b.n2; ++j)
-    for (int i = 0; i < b.n1; ++i) {
+    for (int k = pr.ks; k <= pr.ke; ++k)
+    for (int j = pr.js; j <= pr.je; ++j)
+    for (int i = pr.is; i <= pr.ie; ++i) {
         const size_t c = b.id(k, j, i);
         if (!(b.pflag[c] > 0)) continue;
         Real Pc[NV], Uc[NV];
         load_prims(b, P, c, Pc); load_prims(b, U, c, Uc);
         // Floors::apply_geo_floors (Global), floors_functions.hpp:334-367
-        Pc[RHO] += std::max(0., b.p.rho_min_const - Pc[RHO]);
-        Pc[UU]  += std::max(0., b.p.u_min_const - Pc[UU]);
+        Real rf, uf;
+        geom_floors(b, j, i, rf, uf);
+        Pc[RHO] += std::max(0., rf - Pc[RHO]);
+        Pc[UU]  += std::max(0., uf - Pc[UU]);
         // GRMHD::p_to_u, grmhd_functions.hpp:270-298 (5 conserved)
         FourVectors D;
         calc_4vecs(b, Pc, j, i, CENTER, D);
@@ -912,7 +1073,8 @@ double max_divb(const Block& b, const std::vector<Real>& U) {
     if (b.ndim < 2) return 0.;
     const bool do3 = b.ndim > 2;
     const int jsb = b.js, jeb = b.je + 1, ksb = b.ks, keb = b.ke + (do3 ? 1 : 0);
-    const int isb = b.is, ieb = b.ie + 1;
+    // ValidDivBX1 (b_flux_ct.cpp:554-564): skip the corners on physical X1 faces
+    const int isb = b.is + (b.p.periodic1 ? 0 : 1), ieb = b.ie + (b.p.periodic1 ? 1 : 0);
     const Real* B1v = &U[(size_t)B1 * b.nc];
     const Real* B2v = &U[(size_t)B2 * b.nc];
     const Real* B3v = &U[(size_t)B3 * b.nc];
@@ -938,6 +1100,150 @@ double max_divb(const Block& b, const std::vector<Real>& U) {
     return mx;
 }
 
+// ------------------------------------------------------------ boundaries
+// KBoundaries::check_inflow, boundaries.hpp:101-136: remove velocity into the domain.
+// The clip() of vsq there discards its result.
+void check_inflow(const Block& b, Real* Pc, int j, int i, bool inner) {
+    Real uvec[3] = {Pc[U1], Pc[U2], Pc[U3]};
+    const Real gamma0 = lorentz_calc(b, Pc, j, i, CENTER);
+    const Real alpha0 = 1. / std::sqrt(-b.gcon_(CENTER, j, i, 0, 0));
+    const Real ucon1 = uvec[0] - gamma0 * alpha0 * b.gcon_(CENTER, j, i, 0, 1);
+    if ((ucon1 > 0. && inner) || (ucon1 < 0. && !inner)) {
+        for (int v = 0; v < 3; ++v) uvec[v] /= gamma0;
+        const Real alpha = 1. / std::sqrt(-b.gcon_(CENTER, j, i, 0, 0));
+        const Real beta1 = b.gcon_(CENTER, j, i, 0, 1) * alpha * alpha;
+        uvec[0] = beta1 / alpha;
+        const Real vsq = b.gcov_(CENTER, j, i, 1, 1) * uvec[0] * uvec[0] +
+                         b.gcov_(CENTER, j, i, 2, 2) * uvec[1] * uvec[1] +
+                         b.gcov_(CENTER, j, i, 3, 3) * uvec[2] * uvec[2] +
+                         2. * (b.gcov_(CENTER, j, i, 1, 2) * uvec[0] * uvec[1] +
+                               b.gcov_(CENTER, j, i, 1, 3) * uvec[0] * uvec[2] +
+                               b.gcov_(CENTER, j, i, 2, 3) * uvec[1] * uvec[2]);
+        const Real gamma = 1. / std::sqrt(1. - vsq);
+        for (int v = 0; v < 3; ++v) Pc[U1 + v] = uvec[v] * gamma;
+    }
+}
+
+// One physical face, KBoundaries::ApplyBoundary (boundaries.cpp:369-589):
+// Parthenon's outflow (copy of the first/last interior zone) or reflecting
+// (mirror copy, x2 vector components negated) BoundaryFunction over the whole
+// ghost slab -- restated, the functions are not in the reference tree -- on the
+// synced variables (8 conserved + rho, u, uvec), check_inflow on X1 faces, then
+// BoundaryPtoUElseUtoP (kharma_package.cpp:125-156): Flux::BlockPtoUMHD and
+// B_FluxCT::BlockUtoP over the slab.
+enum { INNER_X1 = 0, OUTER_X1 = 1, INNER_X2 = 2, OUTER_X2 = 3 };
+void apply_boundary(Block& b, std::vector<Real>& U, std::vector<Real>& P, int face) {
+    const bool x1 = (face == INNER_X1 || face == OUTER_X1);
+    const bool inner = (face == INNER_X1 || face == INNER_X2);
+    const int type = (face == INNER_X1) ? b.p.bc_inner_x1 : (face == OUTER_X1) ? b.p.bc_outer_x1 :
+                     (face == INNER_X2) ? b.p.bc_inner_x2 : b.p.bc_outer_x2;
+    const int i0 = x1 ? (inner ? 0 : b.ie + 1) : 0, i1 = x1 ? (inner ? b.is - 1 : b.n1 - 1) : b.n1 - 1;
+    const int j0 = x1 ? 0 : (inner ? 0 : b.je + 1), j1 = x1 ? b.n2 - 1 : (inner ? b.js - 1 : b.n2 - 1);
+    const bool inflow = x1 && (inner ? b.p.check_inflow_inner_x1 : b.p.check_inflow_outer_x1);
+#pragma omp parallel for collapse(2)
+    for (int k = 0; k < b.n3; ++k)
+    for (int j = j0; j <= j1; ++j)
+    for (int i = i0; i <= i1; ++i) {
+        int si = i, sj = j;
+        Real sign2 = 1.;
+        if (x1) {
+            si = (type == 1) ? (inner ? b.is : b.ie) : (inner ? 2 * b.is - 1 - i : 2 * b.ie + 1 - i);
+        } else {
+            sj = (type == 1) ? (inner ? b.js : b.je) : (inner ? 2 * b.js - 1 - j : 2 * b.je + 1 - j);
+        }
+        if (type == 2) sign2 = -1.;
+        const size_t c = b.id(k, j, i), sc = b.id(k, sj, si);
+        for (int v = 0; v < NV; ++v) {
+            const Real flip = (type == 2 && ((x1 && (v == U1 || v == B1)) || (!x1 && (v == U2 || v == B2)))) ? sign2 : 1.;
+            U[v * b.nc + c] = flip * U[v * b.nc + sc];
+        }
+        for (int v = 0; v < 5; ++v) {
+            const Real flip = (type == 2 && ((x1 && v == U1) || (!x1 && v == U2))) ? sign2 : 1.;
+            P[v * b.nc + c] = flip * P[v * b.nc + sc];
+        }
+    }
+    if (inflow) {
+#pragma omp parallel for collapse(2)
+        for (int k = 0; k < b.n3; ++k)
+        for (int j = j0; j <= j1; ++j)
+        for (int i = i0; i <= i1; ++i) {
+            const size_t c = b.id(k, j, i);
+            Real Pc[NV]; load_prims(b, P, c, Pc);
+            check_inflow(b, Pc, j, i, inner);
+            for (int v = U1; v <= U3; ++v) P[v * b.nc + c] = Pc[v];
+        }
+    }
+#pragma omp parallel for collapse(2)
+    for (int k = 0; k < b.n3; ++k)
+    for (int j = j0; j <= j1; ++j)
+    for (int i = i0; i <= i1; ++i) {
+        const size_t c = b.id(k, j, i);
+        Real Pc[NV], Uc[NV];
+        load_prims(b, P, c, Pc); load_prims(b, U, c, Uc);
+        p_to_u_mhd(b, Pc, Uc, j, i);
+        for (int v = 0; v < 5; ++v) U[v * b.nc + c] = Uc[v];
+        for (int v = 0; v < 3; ++v) P[(B1 + v) * b.nc + c] = U[(B1 + v) * b.nc + c] / b.gdet_(CENTER, j, i);
+    }
+}
+// ApplyBoundaryConditions over the faces in Parthenon's order, plus KHARMA's
+// inner-X1 corner fix after the X2 faces (boundaries.cpp:556-570).
+void apply_physical_bcs(Block& b, std::vector<Real>& U, std::vector<Real>& P) {
+    if (!b.p.periodic1) { apply_boundary(b, U, P, INNER_X1); apply_boundary(b, U, P, OUTER_X1); }
+    if (b.ndim > 1 && !b.p.periodic2) {
+        apply_boundary(b, U, P, INNER_X2); apply_boundary(b, U, P, OUTER_X2);
+        if (b.p.fix_corner_inner && !b.p.periodic1) apply_boundary(b, U, P, INNER_X1);
+    }
+}
+
+// B_FluxCT::ZeroBoundaryFlux on both X2 faces (b_flux_ct.cpp:372-420): no B
+// flux through the pole, and the mirrored B2 fluxes that make the polar EMFs vanish.
+void zero_polar_b_flux(Block& b) {
+    if (b.ndim < 2 || b.p.periodic2 || !b.p.fix_polar_flux) return;
+    auto F = [&](int dir, int v, int k, int j, int i) -> Real& {
+        return b.F[((size_t)(dir-1) * NV + v) * b.nc + b.id(k, j, i)];
+    };
+    const int ks = b.ks - (b.ndim > 2), ke = b.ke + (b.ndim > 2);
+    for (int k = ks; k <= ke; ++k)
+    for (int i = b.is - 1; i <= b.ie + 1; ++i) {
+        int j = b.js;
+        F(2, B1, k, j, i) = 0.; F(2, B3, k, j, i) = 0.;
+        F(1, B2, k, j-1, i) = -F(1, B2, k, j, i);
+        if (b.ndim > 2) F(3, B2, k, j-1, i) = -F(3, B2, k, j, i);
+        j = b.je + 1;
+        F(2, B1, k, j, i) = 0.; F(2, B3, k, j, i) = 0.;
+        F(1, B2, k, j, i) = -F(1, B2, k, j-1, i);
+        if (b.ndim > 2) F(3, B2, k, j, i) = -F(3, B2, k, j-1, i);
+    }
+}
+// KBoundaries::FixFlux (boundaries.cpp:611-690): no mass inflow through
+// physical X1 faces, and zero fluxes through the X2 (polar) faces.
+void kboundaries_fix_flux(Block& b) {
+    auto F = [&](int dir, int v, int k, int j, int i) -> Real& {
+        return b.F[((size_t)(dir-1) * NV + v) * b.nc + b.id(k, j, i)];
+    };
+    const int is = b.is - 1, ie = b.ie + 1;
+    const int js = b.js - (b.ndim > 1), je = b.je + (b.ndim > 1);
+    const int ks = b.ks - (b.ndim > 2), ke = b.ke + (b.ndim > 2);
+    if (!b.p.periodic1) {
+        for (int side = 0; side < 2; ++side) {
+            const bool on = side == 0 ? b.p.check_inflow_inner_x1 : b.p.check_inflow_outer_x1;
+            if (!on) continue;
+            const int i = side == 0 ? b.is : b.ie + 1;
+            for (int k = ks; k <= ke; ++k) for (int j = js; j <= je; ++j) {
+                Real& f = F(1, RHO, k, j, i);
+                f = side == 0 ? std::min(f, 0.) : std::max(f, 0.);
+            }
+        }
+    }
+    if (b.ndim > 1 && !b.p.periodic2 && b.p.zero_polar_flux) {
+        for (int side = 0; side < 2; ++side) {
+            const int j = side == 0 ? b.js : b.je + 1;
+            for (int v = 0; v < NV; ++v) for (int k = ks; k <= ke; ++k) for (int i = is; i <= ie; ++i)
+                F(2, v, k, j, i) = 0.;
+        }
+    }
+}
+
 // Integrator weights for parthenon's low-storage "rk2" / "vl2" (not in the
 // reference tree; values restated, parity unpinned).  (gam0, gam1, beta)
 void integrator_weights(int integrator, int stage, Real& gam0, Real& gam1, Real& beta) {
@@ -958,20 +1264,27 @@ void run_stage(Block& b, int stage) {
     std::vector<Real>& Pout = (stage == 1) ? b.P1c : b.P0;
     // Flux region (kharma_step.cpp:141-210)
     for (int dir = 1; dir <= 3; ++dir) get_flux(b, Psub, dir);
+    // Packages::FixFlux: B_FluxCT (ZeroBoundaryFlux, FluxCT), then KBoundaries
+    zero_polar_b_flux(b);
     flux_ct(b);
+    kboundaries_fix_flux(b);
     flux_divergence(b);
     add_geo_source(b, Psub);
     Real gam0, gam1, beta;
     integrator_weights(b.p.integrator, stage, gam0, gam1, beta);
     state_update(b, Usub, b.U0, Psub, Uout, Pout, gam0, gam1, beta * b.dt);
+    // boundary exchange, which applies the physical boundaries too (kharma_driver.cpp:148-150)
     sync_bounds(b, Uout, Pout);
+    apply_physical_bcs(b, Uout, Pout);
     // Fix region (kharma_step.cpp:216-288)
     utop(b, Uout, Pout);
     apply_floors(b, Uout, Pout);
     fix_utop(b, Uout, Pout);
+    apply_physical_bcs(b, Uout, Pout);
     ptou(b, Pout, Uout);
     // Second sync (two_sync, kharma_step.cpp:307-314)
     sync_bounds(b, Uout, Pout);
+    apply_physical_bcs(b, Uout, Pout);
 }
 
 // EvolutionDriver step + KHARMA dt logic: PreStepWork (kharma.cpp:121-129),
@@ -1013,7 +1326,8 @@ void* or_create(const or_params* p) {
     b->dx[0] = (p->x1max - p->x1min) / p->nx1;
     b->dx[1] = (p->x2max - p->x2min) / p->nx2;
     b->dx[2] = (p->x3max - p->x3min) / p->nx3;
-    init_geometry_minkowski(*b);
+    if (p->spherical) init_geometry_ks(*b);
+    else init_geometry_minkowski(*b);
     const size_t nvc = (size_t)NV * b->nc;
     b->U0.assign(nvc, 0.); b->P0.assign(nvc, 0.); b->U1c.assign(nvc, 0.); b->P1c.assign(nvc, 0.);
     b->F.assign(3 * nvc, 0.);
@@ -1037,9 +1351,14 @@ size_t or_ncells(void* h) { return static_cast<Block*>(h)->nc; }
 void or_init_state(void* h, const double* P) {
     Block& b = *static_cast<Block*>(h);
     std::memcpy(b.P0.data(), P, sizeof(double) * NV * b.nc);
-    sync_bounds(b, b.U0, b.P0);
+    // KHARMA::PostInitialize (prob/post_initialize.cpp:55-193): P->U over the
+    // entire block, then the boundary sync with physical boundaries.  The input
+    // may hold the interior only; its periodic ghosts (all 8 primitives, B
+    // included, as a field seeded over the entire block would have) come first.
+    for (int v = 0; v < NV; ++v) sync_periodic_var(b, b.P0, v);
     ptou(b, b.P0, b.U0);
     sync_bounds(b, b.U0, b.P0);
+    apply_physical_bcs(b, b.U0, b.P0);
     b.time = 0.; b.ncycle = 0; b.in_loop = false;
     b.dt = b.p.dt_start;
     b.dt_last = b.dt;
@@ -1057,6 +1376,28 @@ void or_get_state(void* h, double* P, double* U) {
     if (P) std::memcpy(P, b.P0.data(), sizeof(double) * NV * b.nc);
     if (U) std::memcpy(U, b.U0.data(), sizeof(double) * NV * b.nc);
 }
+
+// Geometry cache: which = 0 gcov [NLOC][gn2][gn1][4][4], 1 gcon, 2 gdet [NLOC][gn2][gn1],
+// 3 gdet*Gamma [gn2][gn1][4][4][4], 4 centre radius [gn2][gn1].  Returns the element count.
+long or_get_geometry(void* h, int which, double* out) {
+    Block& b = *static_cast<Block*>(h);
+    const std::vector<Real>* v = which == 0 ? &b.gcov : which == 1 ? &b.gcon : which == 2 ? &b.gdet :
+                                 which == 3 ? &b.gdet_conn : &b.rcen;
+    if (out) std::memcpy(out, v->data(), sizeof(double) * v->size());
+    return (long)v->size();
+}
+// Replace the geometry cache (same layouts as or_get_geometry), e.g. to run the
+// step on exactly the metric another implementation uses.
+void or_set_geometry(void* h, int which, const double* in) {
+    Block& b = *static_cast<Block*>(h);
+    std::vector<Real>* v = which == 0 ? &b.gcov : which == 1 ? &b.gcon : which == 2 ? &b.gdet :
+                           which == 3 ? &b.gdet_conn : &b.rcen;
+    std::memcpy(v->data(), in, sizeof(double) * v->size());
+}
+// Physical boundaries on the base state (KBoundaries::ApplyBoundary for each face).
+void or_apply_bcs(void* h) { Block& b = *static_cast<Block*>(h); apply_physical_bcs(b, b.U0, b.P0); }
+// Flux fixes on the stored fluxes (ZeroBoundaryFlux, FluxCT, KBoundaries::FixFlux).
+void or_fix_flux(void* h) { Block& b = *static_cast<Block*>(h); zero_polar_b_flux(b); flux_ct(b); kboundaries_fix_flux(b); }
 
 void or_set_dt(void* h, double dt) { static_cast<Block*>(h)->dt = dt; static_cast<Block*>(h)->dt_last = dt; }
 double or_get_dt(void* h) { return static_cast<Block*>(h)->dt; }
@@ -1076,6 +1417,7 @@ int or_step(void* h, int nsteps, double* dts) {
 void or_get_flux(void* h, int dir) { Block& b = *static_cast<Block*>(h); get_flux(b, b.P0, dir); }
 void or_flux_ct(void* h) { flux_ct(*static_cast<Block*>(h)); }
 void or_flux_divergence(void* h) { flux_divergence(*static_cast<Block*>(h)); }
+void or_add_geo_source(void* h) { Block& b = *static_cast<Block*>(h); add_geo_source(b, b.P0); }
 void or_utop(void* h) { Block& b = *static_cast<Block*>(h); utop(b, b.U0, b.P0); }
 void or_apply_floors(void* h) { Block& b = *static_cast<Block*>(h); apply_floors(b, b.U0, b.P0); }
 void or_fix_utop(void* h) { Block& b = *static_cast<Block*>(h); fix_utop(b, b.U0, b.P0); }

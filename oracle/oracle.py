@@ -37,6 +37,14 @@ class OrParams(ctypes.Structure):
         ("iter_max", ctypes.c_int), ("err_tol", ctypes.c_double),
         ("fix_average", ctypes.c_int), ("fix_atmo", ctypes.c_int),
         ("integrator", ctypes.c_int), ("spherical", ctypes.c_int),
+        ("a", ctypes.c_double), ("transform", ctypes.c_int),
+        ("hslope", ctypes.c_double), ("mks_smooth", ctypes.c_double), ("poly_xt", ctypes.c_double),
+        ("poly_alpha", ctypes.c_double), ("fmks_zero_point", ctypes.c_double),
+        ("bc_inner_x1", ctypes.c_int), ("bc_outer_x1", ctypes.c_int),
+        ("bc_inner_x2", ctypes.c_int), ("bc_outer_x2", ctypes.c_int),
+        ("check_inflow_inner_x1", ctypes.c_int), ("check_inflow_outer_x1", ctypes.c_int),
+        ("zero_polar_flux", ctypes.c_int), ("fix_polar_flux", ctypes.c_int), ("fix_corner_inner", ctypes.c_int),
+        ("periodic1", ctypes.c_int), ("periodic2", ctypes.c_int), ("periodic3", ctypes.c_int),
     ]
 
 
@@ -74,7 +82,8 @@ def lib(variant=""):
         L.or_step.restype = i
         L.or_step.argtypes = [vp, i, dp]
         L.or_get_flux.argtypes = [vp, i]
-        for name in ("or_flux_ct", "or_flux_divergence", "or_utop", "or_apply_floors", "or_fix_utop", "or_ptou"):
+        for name in ("or_flux_ct", "or_flux_divergence", "or_utop", "or_apply_floors", "or_fix_utop", "or_ptou",
+                     "or_add_geo_source"):
             getattr(L, name).argtypes = [vp]
         L.or_estimate_timestep.restype = d
         L.or_estimate_timestep.argtypes = [vp]
@@ -85,6 +94,11 @@ def lib(variant=""):
         L.or_weno5.argtypes = [d, d, d, d, d, dp, dp]
         L.or_reconstruct.argtypes = [i, d, d, d, d, d, dp, dp]
         L.or_num_threads.restype = i
+        L.or_get_geometry.restype = ctypes.c_long
+        L.or_get_geometry.argtypes = [vp, i, dp]
+        L.or_apply_bcs.argtypes = [vp]
+        L.or_set_geometry.argtypes = [vp, i, dp]
+        L.or_fix_flux.argtypes = [vp]
         _libs[variant] = L
     return _libs[variant]
 
@@ -95,6 +109,8 @@ def _ptr(a):
 
 RECON = {"donor_cell_c": 0, "linear_mc": 1, "weno5": 2, "ppm": 3}
 FRAME = {"fluid": 0, "normal": 1, "drift": 2}
+TRANSFORM = {"null": 0, "mks": 1, "fmks": 2}
+BC = {"periodic": 0, "outflow": 1, "reflecting": 2}
 
 
 def make_params(cfg):
@@ -132,7 +148,27 @@ def make_params(cfg):
     p.fix_average = int(cfg.get("fix_average", True))
     p.fix_atmo = int(cfg.get("fix_atmo", True))
     p.integrator = {"rk2": 0, "vl2": 1}[cfg.get("integrator", "rk2")]
-    p.spherical = 0
+    p.spherical = int(cfg.get("coords", "cartesian_minkowski") == "ks")
+    p.a = cfg.get("a", 0.0)
+    p.transform = TRANSFORM[cfg.get("transform", "null")]
+    p.hslope = cfg.get("hslope", 0.3)
+    p.mks_smooth = cfg.get("mks_smooth", 0.5)
+    p.poly_xt = cfg.get("poly_xt", 0.82)
+    p.poly_alpha = cfg.get("poly_alpha", 14.0)
+    p.fmks_zero_point = cfg.get("fmks_zero_point", 0.0)
+    bcs = cfg.get("boundaries", {})
+    p.bc_inner_x1 = BC[bcs.get("inner_x1", "periodic")]
+    p.bc_outer_x1 = BC[bcs.get("outer_x1", "periodic")]
+    p.bc_inner_x2 = BC[bcs.get("inner_x2", "periodic")]
+    p.bc_outer_x2 = BC[bcs.get("outer_x2", "periodic")]
+    p.periodic1 = int(p.bc_inner_x1 == 0)
+    p.periodic2 = int(p.bc_inner_x2 == 0)
+    p.periodic3 = 1
+    p.check_inflow_inner_x1 = int(cfg.get("check_inflow_inner_x1", False))
+    p.check_inflow_outer_x1 = int(cfg.get("check_inflow_outer_x1", False))
+    p.zero_polar_flux = int(cfg.get("zero_polar_flux", False))
+    p.fix_polar_flux = int(cfg.get("fix_polar_flux", False))
+    p.fix_corner_inner = int(cfg.get("fix_corner_inner", False))
     return p
 
 
@@ -211,6 +247,9 @@ class OracleBlock:
     def flux_divergence(self):
         self.L.or_flux_divergence(self.h)
 
+    def add_geo_source(self):
+        self.L.or_add_geo_source(self.h)
+
     def utop(self):
         self.L.or_utop(self.h)
 
@@ -228,6 +267,28 @@ class OracleBlock:
 
     def max_divb(self):
         return self.L.or_max_divb(self.h)
+
+    def apply_bcs(self):
+        self.L.or_apply_bcs(self.h)
+
+    def fix_flux(self):
+        self.L.or_fix_flux(self.h)
+
+    def set_geometry(self, which, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        assert a.size == self.L.or_get_geometry(self.h, which, None)
+        self.L.or_set_geometry(self.h, which, _ptr(a))
+
+    def geometry(self, which):
+        """Geometry cache: 0 gcov, 1 gcon, 2 gdet, 3 gdet*Gamma, 4 centre radius (see or_get_geometry)."""
+        n = self.L.or_get_geometry(self.h, which, None)
+        out = np.empty(n)
+        self.L.or_get_geometry(self.h, which, _ptr(out))
+        gn2, gn1 = self.shape[1] + 1, self.shape[2] + 1
+        nloc = 5
+        shapes = {0: (nloc, gn2, gn1, 4, 4), 1: (nloc, gn2, gn1, 4, 4), 2: (nloc, gn2, gn1),
+                  3: (gn2, gn1, 4, 4, 4), 4: (gn2, gn1)}
+        return out.reshape(shapes[which])
 
 
 def weno5(x):

@@ -42,13 +42,26 @@ def _create(cfg, **over):
     (dict(nx1=0), 1),
     (dict(recon=9), 1),
     (dict(precision=3), 1),
-    (dict(coords=1), 3),             # not implemented -> unsupported, not a crash
-    (dict(periodic1=0), 3),
+    (dict(coords=2), 3),             # unknown coordinate system -> unsupported, not a crash
+    (dict(periodic1=0), 1),          # physical x1 faces without a boundary type
+    (dict(periodic3=0), 3),          # x3 must be periodic
+    (dict(coords=1, a=1.0), 1),      # extremal spin
+    (dict(coords=1, transform=7), 1),
     (dict(nx1=7), 1),                # periodic axis shorter than twice the ghost depth
 ])
 def test_invalid_params_rejected_on_host(L, over, code):
     st, msg = _create(pr.mhdmode_config(n=16, ndim=3), **over)
     assert st == code and msg
+
+
+def test_torus_block_accepted_and_small_physical_axis_rejected(L):
+    cfg = pr.torus_config(16, 8, 8)
+    st, msg = _create(cfg)
+    # creation needs a device for its buffers; on a host without one the
+    # parameters must still pass validation (no KH_ERR_INVALID / UNSUPPORTED)
+    assert st in (0, 2), msg
+    st, msg = _create(cfg, nx2=3)    # physical x2 faces need nx2 >= ng
+    assert st == 1 and msg
 
 
 def test_null_arguments_rejected(L):
