@@ -18,6 +18,9 @@
 namespace kh {
 
 static constexpr int BS = 256;
+// Work-groups per CU the update kernel is compiled for: 4 (128 VGPRs) with the
+// flat metric; the cached-metric variant needs ~240 VGPRs and would spill at 4.
+template <int GEO> struct UpdateOcc { static constexpr int min_blocks = GEO ? 2 : 4; };
 
 static inline dim3 grid_for(long n, int bs = BS) { return dim3((unsigned)((n + bs - 1) / bs)); }
 static inline long ninterior_host(const BlockDims& b) {
@@ -367,7 +370,7 @@ __device__ __forceinline__ void zone_divergence_ct(const BlockDims& b, const R* 
 // ------------------------------------------------------- fused stage update
 // Stage 1 (gam0 = 0 and U_sub == U_full) reads the state once: 0*U + 1*U == U.
 template <typename R, bool DO_DT, bool CT, int GEO>
-__global__ void __launch_bounds__(BS, 4)
+__global__ void __launch_bounds__(BS, UpdateOcc<GEO>::min_blocks)
 update_kernel(BlockDims b, StepConsts cst, GeomDev gd, int stage, const R* __restrict__ F, const R* __restrict__ ctop,
               const double* __restrict__ Usub, const double* Ufull, const double* __restrict__ Psub,
               double* Uout, double* __restrict__ Pout, float* __restrict__ Pout32,
