@@ -3,7 +3,7 @@
 // Host code only: it validates arguments, fills the index/constant structs,
 // and enqueues the gfx950 kernels of kh_flux.hip / kh_step.hip on one stream.
 // No exception crosses the ABI; every HIP status is checked.
-#include "../../include/kharma_amd.h"
+#include "kharma_amd.h"
 
 #include <algorithm>
 #include <cfloat>

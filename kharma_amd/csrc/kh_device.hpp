@@ -589,7 +589,11 @@ __device__ __forceinline__ void apply_ceilings(const G& g, double P[NV], double 
 // One zone of Floors::ApplyFloorsInFrame (floors_impl.hpp:45-128): determine,
 // apply in the configured frame, ceilings.  Returns fflag; may overwrite pflag.
 // D holds the four-vectors of P on entry and on return (recomputed if P changed).
-template <class G>
+// FRAME (0 fluid, 1 normal, 2 drift) fixes the frame at compile time, which
+// keeps the other frames' code and registers out of the kernel; FRAME_RUNTIME
+// reads it from f.frame.
+constexpr int FRAME_RUNTIME = -1;
+template <int FRAME = FRAME_RUNTIME, class G>
 __device__ __forceinline__ int floors_zone(const G& g, double P[NV], double U[NV], FourVec<double>& D, double gam,
                                            const FloorParams& f, double rhoflr_geom, double uflr_geom,
                                            int fflag_in, int& pflag) {
@@ -597,10 +601,11 @@ __device__ __forceinline__ int floors_zone(const G& g, double P[NV], double U[NV
     const int fflag = fflag_in | determine_floors(g, P, dot4(D.bcon, D.bcov), gam, f, rhoflr_geom, uflr_geom, rf, uf);
     if (fflag) {
         int pflag_l = 0;
-        if (f.frame == 0) {
+        const int frame = (FRAME == FRAME_RUNTIME) ? f.frame : FRAME;
+        if (frame == 0) {
             P[RHO] += smax(0., rf - P[RHO]);
             P[UU] += smax(0., uf - P[UU]);
-        } else if (f.frame == 1) {
+        } else if (frame == 1) {
             pflag_l = apply_floors_normal(g, P, U, gam, rf, uf);
         } else {
             apply_floors_drift(g, P, gam, rf, uf);

@@ -6,7 +6,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/kharma_amd.h"
+#include "kharma_amd.h"
 #include "kh_device.hpp"
 
 namespace kh {

@@ -116,8 +116,9 @@ __device__ __forceinline__ bool in_physical(const BlockDims& b, int k, int j, in
 
 // Flux::AddGeoSource (flux/flux.cpp:395-444): the T^mu_nu Gamma contraction of
 // zone (j, i) added to the energy and momentum rates.  conn holds
-// sqrt(-g) Gamma^a_{b c} at index 16 a + 4 b + c; mask has bit n set when
-// component n is non-zero anywhere in the block (a uniform branch).
+// sqrt(-g) Gamma^a_{b c} at index 16 a + 4 b + c.  All 64 terms are summed,
+// in the order mu, nu, lam, with no test for zero components: a branch per
+// component would make every load wait for the one before it.
 template <class G>
 __device__ __forceinline__ void add_geo_source(const G& g, const GeomDev& gd, long q, const double Psub[NV], double gam,
                                                double dudt[NV]) {
@@ -133,13 +134,13 @@ __device__ __forceinline__ void add_geo_source(const G& g, const GeomDev& gd, lo
         double T[4];
 #pragma unroll
         for (int nu = 0; nu < 4; ++nu) T[nu] = eta * D.ucon[mu] * D.ucov[nu] + (mu == nu ? ptot : 0.) - D.bcon[mu] * D.bcov[nu];
+        double cm[16];   // this mu's 16 components, loaded together
+#pragma unroll
+        for (int nl = 0; nl < 16; ++nl) cm[nl] = gd.conn[(long)(nl * 4 + mu) * gd.np + q];
 #pragma unroll
         for (int nu = 0; nu < 4; ++nu)
 #pragma unroll
-            for (int lam = 0; lam < 4; ++lam) {
-                const int n = nu * 16 + lam * 4 + mu;
-                if ((gd.conn_mask >> n) & 1ull) nd[lam] += T[nu] * gd.conn[(long)n * gd.np + q];
-            }
+            for (int lam = 0; lam < 4; ++lam) nd[lam] += T[nu] * cm[nu * 4 + lam];
     }
     dudt[UU] += nd[0];
     dudt[U1] += nd[1];
@@ -305,14 +306,15 @@ __device__ __forceinline__ R emf2_at(const R* F3B1, const R* F1B3, long c, long 
 // Divergence of zone c with the flux-CT B fluxes (b_flux_ct.cpp:347-369) built
 // from EMFs computed here from the Riemann fluxes, so neither the EMFs nor the
 // rewritten B fluxes make a trip through HBM.
-template <typename R>
+// ND (2 or 3) fixes the dimensionality at compile time; ND = 0 reads b.ndim.
+template <typename R, int ND = 0>
 __device__ __forceinline__ void zone_divergence_ct(const BlockDims& b, const R* __restrict__ F, long c, double dudt[NV]) {
     const double A1 = b.dx2 * b.dx3, A2 = b.dx1 * b.dx3, A3 = b.dx1 * b.dx2, V = b.dx1 * b.dx2 * b.dx3;
     const R* F1 = F;
     const R* F2 = F + 8 * b.nc;
     const R* F3 = F + 16 * b.nc;
     const long s2 = b.s2, s3 = b.s3, nc = b.nc;
-    const int nd = b.ndim;
+    const int nd = ND ? ND : b.ndim;
     double f1lo[NV], f1hi[NV], f2lo[NV], f2hi[NV], f3lo[NV], f3hi[NV];
 #pragma unroll
     for (int v = 0; v < B1; ++v) {
@@ -446,6 +448,135 @@ update_kernel(BlockDims b, StepConsts cst, GeomDev gd, int stage, const R* __res
     if (DO_DT) {
         const double m = block_min(ndt);
         if (threadIdx.x == 0) dt_part[blockIdx.x] = m;
+    }
+}
+
+// ------------------------------------------- curved-space stage update, split
+// In Kerr-Schild geometry the fused update needs ~240 VGPRs (two waves per
+// SIMD) and spends most of its time waiting.  It runs as two kernels instead:
+// combine_kernel (flux divergence, geometric source, RK combination -> U) and
+// recover_kernel (U->P, floors, P->U, flags).  The arithmetic is the same as
+// the fused kernel's, so the results are bit-identical; the cost is one extra
+// write and read of U (128 bytes per zone).
+//
+// Both walk the interior in (j, k, i) order, i fastest, with the workgroups
+// renumbered so each XCD (workgroup w runs on XCD w % 8) owns one contiguous
+// range of j: the axisymmetric geometry tables ([comp][j][i]) a workgroup
+// reads then stay within one eighth of the table per XCD L2.
+#ifndef KH_CMB_OCC
+#define KH_CMB_OCC 1
+#endif
+#ifndef KH_REC_OCC
+#define KH_REC_OCC 1
+#endif
+constexpr unsigned N_XCD = 8;
+__device__ __forceinline__ unsigned xcd_logical_block(unsigned b, unsigned nb) {
+    const unsigned x = b % N_XCD, r = b / N_XCD, q = nb / N_XCD, m = nb % N_XCD;
+    return x * q + (x < m ? x : m) + r;
+}
+__device__ __forceinline__ void interior_index_jki(const BlockDims& b, unsigned t, int& k, int& j, int& i) {
+    const unsigned n3 = (unsigned)(b.ndim > 2 ? b.nx3 : 1);
+    const unsigned row = t / (unsigned)b.nx1;
+    const unsigned jj = row / n3;
+    i = b.is + (int)(t - row * (unsigned)b.nx1);
+    k = b.ks + (int)(row - jj * n3);
+    j = b.js + (int)jj;
+}
+
+// ND: dimensionality (flux-CT for 2 and 3); SAME: stage with gam0 = 0 and
+// U_sub == U_full, which reads the state once (0*U + 1*U == U).
+template <typename R, bool DO_DT, int ND, bool SAME>
+__global__ void __launch_bounds__(BS, KH_CMB_OCC)
+combine_kernel(BlockDims b, StepConsts cst, GeomDev gd, int stage, const R* __restrict__ F, const R* __restrict__ ctop,
+               const double* __restrict__ Usub, const double* Ufull, const double* __restrict__ Psub, double* Uout,
+               const DeviceState* __restrict__ ds, double* __restrict__ dt_part) {
+    const unsigned t = xcd_logical_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const bool active = t < (unsigned)ninterior(b);
+    double ndt = DBL_MAX;
+    if (active) {
+        int k, j, i;
+        interior_index_jki(b, t, k, j, i);
+        const long c = (long)k * b.s3 + (long)j * b.s2 + i;
+        const double g0 = cst.gam0[stage - 1], g1 = cst.gam1[stage - 1];
+        double Ub[NV];   // the stage's base state, loaded first so its latency overlaps the rest
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const double uf = Ufull[v * b.nc + c];
+            Ub[v] = SAME ? uf : g0 * Usub[v * b.nc + c] + g1 * uf;
+        }
+        double dudt[NV];
+        if constexpr (ND > 1) zone_divergence_ct<R, ND>(b, F, c, dudt);
+        else zone_divergence<R>(b, F, nullptr, c, false, dudt);
+        {
+            double Ps[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) Ps[v] = Psub[v * b.nc + c];
+            const long q = (long)j * gd.gn1 + i;
+            CachedGeom<double> g(gd.g[GLOC_CENTER], gd.np, q);
+            add_geo_source(g, gd, q, Ps, cst.gam, dudt);
+        }
+        const double beta_dt = cst.beta[stage - 1] * ds->dt;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) Uout[v * b.nc + c] = 1.0 * Ub[v] + beta_dt * dudt[v];
+        if (DO_DT) {
+            const double n = zone_ndt(b, (double)ctop[c], ND > 1 ? (double)ctop[b.nc + c] : 0.,
+                                      ND > 2 ? (double)ctop[2 * b.nc + c] : 0.);
+            if (!(n != n)) ndt = n;
+        }
+    }
+    if (DO_DT) {
+        const double m = block_min(ndt);
+        if (threadIdx.x == 0) dt_part[blockIdx.x] = m;
+    }
+}
+
+// U->P with the stage's starting primitives as guess (kharma_driver.cpp:379-384),
+// B from U, floors in frame FRAME, P->U; U is read from and written back to Uout.
+template <int FRAME>
+__global__ void __launch_bounds__(BS, KH_REC_OCC)
+recover_kernel(BlockDims b, StepConsts cst, GeomDev gd, int stage, const double* __restrict__ Psub, double* Uout,
+               double* __restrict__ Pout, float* __restrict__ Pout32, int* __restrict__ pflag,
+               int* __restrict__ fflag, int* __restrict__ fail_list, DeviceState* __restrict__ ds) {
+    const unsigned t = xcd_logical_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (t >= (unsigned)ninterior(b)) return;
+    int k, j, i;
+    interior_index_jki(b, t, k, j, i);
+    const long c = (long)k * b.s3 + (long)j * b.s2 + i;
+    const double gam = cst.gam;
+    double U[NV], P[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) U[v] = Uout[v * b.nc + c];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) P[v] = Psub[v * b.nc + c];
+    int pf = 0, ff = 0;
+    double rf, uf;
+    geom_floors<1>(cst, gd, j, i, rf, uf);
+    {
+        CachedGeom<double> g(gd.g[GLOC_CENTER], gd.np, (long)j * gd.gn1 + i);
+        pf = u_to_p_onedw(g, U, P, gam, cst.iter_max, cst.err_tol);
+        const int ff0 = (pf / F_GEOM_RHO) * F_GEOM_RHO;
+        pf = pf % F_GEOM_RHO;
+        const double gd_ = g.gdet();
+        P[B1] = U[B1] / gd_; P[B2] = U[B2] / gd_; P[B3] = U[B3] / gd_;
+        FourVec<double> D;
+        calc_4vecs<double>(g, P, D);
+        ff = ff0;
+        if (cst.floors.on) ff = floors_zone<FRAME>(g, P, U, D, gam, cst.floors, rf, uf, ff0, pf);
+        p_to_u_D(g, P, D, gam, U);
+    }
+    ImageSet s;
+    image_set(b, k, j, i, s);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        store_with_images(b, s, Uout + v * b.nc, c, U[v]);
+        store_with_images(b, s, Pout + v * b.nc, c, P[v]);
+        if (Pout32) store_with_images(b, s, Pout32 + v * b.nc, c, (float)P[v]);
+    }
+    store_with_images(b, s, pflag, c, pf);
+    fflag[c] = ff;
+    if (pf > 0) {
+        const int slot = atomicAdd(&ds->fail_count[stage - 1], 1);
+        fail_list[slot] = (int)c;
     }
 }
 
@@ -668,15 +799,43 @@ __device__ __forceinline__ void bc_fill_x2(const BlockDims& b, const double* __r
     for (int v = 0; v < 3; ++v) Bu[v] = Usrc[(B1 + v) * b.nc + sc];
     if (type == KH_BC_REFLECT) { P[U2] = -P[U2]; Bu[1] = -Bu[1]; }
 }
+// The zones the physical boundaries fill, enumerated without the interior:
+// slab 1 is every zone of the 2 ng X1 ghost columns (all j, k) when X1 is
+// physical; slab 2 is the 2 ng X2 ghost rows over the X1 interior (or over all
+// i when X1 is periodic) when X2 is physical.
+__host__ __device__ inline long bounds_slab1(const BlockDims& b) {
+    return b.periodic[0] ? 0 : (long)b.n3 * b.n2 * 2 * b.ng;
+}
+__host__ __device__ inline long bounds_slab2(const BlockDims& b) {
+    if (b.ndim < 2 || b.periodic[1]) return 0;
+    return (long)b.n3 * 2 * b.ng * (b.periodic[0] ? b.n1 : b.nx1);
+}
 template <int GEO>
 __global__ void __launch_bounds__(BS) bounds_kernel(BlockDims b, StepConsts cst, GeomDev gd, double* __restrict__ U,
                                                     double* __restrict__ P, float* __restrict__ P32) {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= b.nc) return;
-    const int k = (int)(t / b.s3), j = (int)((t - (long)k * b.s3) / b.s2), i = (int)(t - (long)k * b.s3 - (long)j * b.s2);
+    long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long n1s = bounds_slab1(b);
+    int k, j, i;
+    if (r < n1s) {
+        const int g = (int)(r % (2 * b.ng));
+        r /= 2 * b.ng;
+        j = (int)(r % b.n2);
+        k = (int)(r / b.n2);
+        i = (g < b.ng) ? g : b.ie + 1 + (g - b.ng);
+    } else {
+        r -= n1s;
+        if (r >= bounds_slab2(b)) return;
+        const int w = b.periodic[0] ? b.n1 : b.nx1;
+        const int ii = (int)(r % w);
+        r /= w;
+        const int g = (int)(r % (2 * b.ng));
+        k = (int)(r / (2 * b.ng));
+        j = (g < b.ng) ? g : b.je + 1 + (g - b.ng);
+        i = (b.periodic[0] ? 0 : b.is) + ii;
+    }
+    const long t = (long)k * b.s3 + (long)j * b.s2 + i;
     const bool x1g = !b.periodic[0] && (i < b.is || i > b.ie);
     const bool x2g = b.ndim > 1 && !b.periodic[1] && (j < b.js || j > b.je);
-    if (!x1g && !x2g) return;
     double Pz[NV], Bu[3];
     if (x1g && !x2g) {
         bc_fill_x1<GEO>(b, gd, U, P, k, j, i, Pz, Bu);
@@ -929,6 +1088,29 @@ hipError_t launch_flux_divergence(const BlockDims& b, int prec, const void* F, d
     else hipLaunchKernelGGL(divergence_kernel<double>, grid_for(n), dim3(BS), 0, s, b, (const double*)F, dUdt);
     return hipGetLastError();
 }
+template <typename R>
+static void launch_update_geo(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int stage, bool do_dt,
+                              const void* F, const void* ctop, const double* Usub, const double* Ufull,
+                              const double* Psub, double* Uout, double* Pout, float* Pout32, int* pflag, int* fflag,
+                              int* fail_list, DeviceState* ds, double* dt_part, hipStream_t s) {
+    const dim3 g = grid_for(ninterior_host(b));
+    const bool same = (c.gam0[stage - 1] == 0.) && (Usub == Ufull);
+#define KH_CMB(DT, ND, SAME)                                                                                          \
+    hipLaunchKernelGGL((combine_kernel<R, DT, ND, SAME>), g, dim3(BS), 0, s, b, c, gd, stage, (const R*)F, (const R*)ctop, \
+                       Usub, Ufull, Psub, Uout, ds, dt_part)
+#define KH_CMB_ND(DT, SAME) \
+    { if (b.ndim == 3) KH_CMB(DT, 3, SAME); else if (b.ndim == 2) KH_CMB(DT, 2, SAME); else KH_CMB(DT, 1, SAME); }
+    if (do_dt) { if (same) KH_CMB_ND(true, true) else KH_CMB_ND(true, false) }
+    else { if (same) KH_CMB_ND(false, true) else KH_CMB_ND(false, false) }
+#undef KH_CMB_ND
+#undef KH_CMB
+#define KH_REC(FR) \
+    hipLaunchKernelGGL((recover_kernel<FR>), g, dim3(BS), 0, s, b, c, gd, stage, Psub, Uout, Pout, Pout32, pflag, fflag, fail_list, ds)
+    if (c.floors.frame == 0) KH_REC(0);
+    else if (c.floors.frame == 1) KH_REC(1);
+    else KH_REC(2);
+#undef KH_REC
+}
 template <typename R, int GEO>
 static void launch_update_t(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int stage, bool do_dt,
                             const void* F, const void* ctop, const double* Usub, const double* Ufull, const double* Psub,
@@ -948,10 +1130,10 @@ hipError_t launch_update(const BlockDims& b, const StepConsts& c, const GeomDev&
                          double* Uout, double* Pout, float* Pout32, int* pflag, int* fflag,
                          int* fail_list, DeviceState* ds, double* dt_part, hipStream_t s) {
     if (prec == PREC_SPLIT) {
-        if (b.geo) launch_update_t<float, 1>(b, c, gd, stage, do_dt, F, ctop, Usub, Ufull, Psub, Uout, Pout, Pout32, pflag, fflag, fail_list, ds, dt_part, s);
+        if (b.geo) launch_update_geo<float>(b, c, gd, stage, do_dt, F, ctop, Usub, Ufull, Psub, Uout, Pout, Pout32, pflag, fflag, fail_list, ds, dt_part, s);
         else launch_update_t<float, 0>(b, c, gd, stage, do_dt, F, ctop, Usub, Ufull, Psub, Uout, Pout, Pout32, pflag, fflag, fail_list, ds, dt_part, s);
     } else {
-        if (b.geo) launch_update_t<double, 1>(b, c, gd, stage, do_dt, F, ctop, Usub, Ufull, Psub, Uout, Pout, Pout32, pflag, fflag, fail_list, ds, dt_part, s);
+        if (b.geo) launch_update_geo<double>(b, c, gd, stage, do_dt, F, ctop, Usub, Ufull, Psub, Uout, Pout, Pout32, pflag, fflag, fail_list, ds, dt_part, s);
         else launch_update_t<double, 0>(b, c, gd, stage, do_dt, F, ctop, Usub, Ufull, Psub, Uout, Pout, Pout32, pflag, fflag, fail_list, ds, dt_part, s);
     }
     return hipGetLastError();
@@ -978,8 +1160,9 @@ hipError_t launch_fix_flux(const BlockDims& b, int prec, void* F, hipStream_t s)
 hipError_t launch_bounds(const BlockDims& b, const StepConsts& c, const GeomDev& gd, double* U, double* P, float* P32,
                          hipStream_t s) {
     if (b.periodic[0] && (b.ndim < 2 || b.periodic[1])) return hipSuccess;
-    if (b.geo) hipLaunchKernelGGL(bounds_kernel<1>, grid_for(b.nc), dim3(BS), 0, s, b, c, gd, U, P, P32);
-    else hipLaunchKernelGGL(bounds_kernel<0>, grid_for(b.nc), dim3(BS), 0, s, b, c, gd, U, P, P32);
+    const long n = bounds_slab1(b) + bounds_slab2(b);
+    if (b.geo) hipLaunchKernelGGL(bounds_kernel<1>, grid_for(n), dim3(BS), 0, s, b, c, gd, U, P, P32);
+    else hipLaunchKernelGGL(bounds_kernel<0>, grid_for(n), dim3(BS), 0, s, b, c, gd, U, P, P32);
     return hipGetLastError();
 }
 hipError_t launch_finalize(const StepConsts& c, DeviceState* ds, const double* dt_part, int n_dt,
