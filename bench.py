@@ -110,6 +110,7 @@ def main():
     ap.add_argument("--workload", default="torus", choices=["torus", "fastmode"])
     ap.add_argument("--n", type=int, default=128, help="fastmode: zones per side")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiation-form measurement")
     ap.add_argument("--forms", default="fp64,split", help="comma list of fp64,split (profiling runs use one)")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="after the runs, copy a buffer of this many MiB on the device: a known-size "
@@ -122,6 +123,7 @@ def main():
 
     import numpy as np
     import torch
+    import kharma_amd as kh
     from kharma_amd import problems as pr
 
     if a.workload == "torus":
@@ -153,6 +155,7 @@ def main():
         return
     r64 = _bench_form(dict(cfg, precision="fp64"), P0, a.steps, a.warmup)
     r32 = _bench_form(dict(cfg, precision="split"), P0, a.steps, a.warmup)
+    adj = None if a.no_adjoint else _bench_adjoint()
     zc64 = nz * a.steps / (r64["ms"] * 1e-3)
     zc32 = nz * a.steps / (r32["ms"] * 1e-3)
 
@@ -161,6 +164,19 @@ def main():
     P64 = r64["P"].double().cpu().numpy()[sl]
     P32 = r32["P"].double().cpu().numpy()[sl]
     rel = (np.abs(P32 - P64).sum(axis=(1, 2, 3)) / np.abs(P64).sum(axis=(1, 2, 3)).clip(1e-300))
+
+    # one-step deviation: one split step and one FP64 step from the same (FP64) state
+    dt_last = r64["scalars"]["dt"]
+    one = {}
+    for f in ("fp64", "split"):
+        drv = kh.Driver(dict(cfg, precision=f))
+        drv.init_prims(r64["P"])
+        drv.set_clock(r64["scalars"]["time"], dt_last, r64["scalars"]["ncycle"])
+        drv.step(1)
+        one[f] = drv.state()[0].double().cpu().numpy()[sl]
+        del drv
+    rel1 = (np.abs(one["split"] - one["fp64"]).sum(axis=(1, 2, 3)) /
+            np.abs(one["fp64"]).sum(axis=(1, 2, 3)).clip(1e-300))
 
     # dominant kernel of the reported (split) form
     kt = {k: v for k, v in r32["kernels"].items() if v[1] > 0}
@@ -202,8 +218,11 @@ def main():
         "config": {"workload": desc, "zones": nz, "ghosts": cfg["ng"], "forms": ["fp64", "split"]},
         "fp64": {"value": zc64, "ms_per_step": r64["ms"] / a.steps, "dtype": "f64"},
         "split_over_fp64": zc32 / zc64,
-        "split_rel_l1_vs_fp64": {"max": float(rel.max()), "per_var": [float(x) for x in rel]},
-        "adjoint_over_forward": None,
+        "split_rel_l1_vs_fp64": {"after_steps": a.steps + a.warmup, "max": float(rel.max()),
+                                 "per_var": [float(x) for x in rel],
+                                 "one_step_max": float(rel1.max()), "one_step_per_var": [float(x) for x in rel1]},
+        "adjoint_over_forward": adj["adjoint_over_forward"] if adj else None,
+        "adjoint": adj,
         "step_state_roofline": {"bytes_per_zone_cycle": STEP_STATE_BYTES, "achieved_GBs_fp64": step_bytes * a.steps / (r64["ms"] * 1e-3) / 1e9,
                                 "achieved_GBs_split": step_bytes * a.steps / (r32["ms"] * 1e-3) / 1e9,
                                 "peak": HBM_PEAK_GBS},
@@ -220,6 +239,92 @@ def main():
     print(json.dumps(out))
     if a.copy_mb:
         _calibration_copy(a.copy_mb)
+
+
+def _bench_adjoint(n=64, nsteps=20, ndirs=64, seed=20261202, h=1e-7):
+    """Differentiation form on the Alfven mode: gradient of L = |P_N - P_target|^2
+    after nsteps FP64 RK2 steps (P_target: the mode at phase + pi/4) with respect
+    to the initial primitive field, by the adjoint sweep (per-step checkpoints);
+    adjoint/forward GPU time per step; check against central differences (step h).
+
+    Directions: the mode is uniform along x3 and rho, u, B1 are uniform
+    everywhere, where WENO5's smoothness indicators are exactly zero and any
+    finite perturbation moves its nonlinear weights by O(1), so differences of
+    the literal state are taken in ndirs x3-independent white-noise directions in
+    the wave's fields u3, B3 (the directions along which the map is smooth at
+    this state).  The same objective is then checked from a generic smooth state
+    (the mode plus a 1e-3 Fourier mode in every variable, uniform nowhere) in
+    smooth random and white-noise directions over all 8 variables."""
+    import math
+    import numpy as np
+    import torch
+    from kharma_amd import problems as pr
+    from kharma_amd.adjoint import Evolution, generic_state, noise_direction, smooth_direction
+
+    cfg = pr.mhdmode_config(n=n, ndim=3, mode="alfven", amp=1e-4, one_period=False)
+    # start at the CFL step of the fast speed (~0.66 c for this background) instead of dt_min
+    cfg["dt_start"] = 0.9 / (n * 3 * 0.7)
+    evo = Evolution(cfg)
+    inner = (slice(None),) + pr.interior(cfg)
+    Pt = torch.as_tensor(pr.mhdmode_state(cfg, 0.0, phase=math.pi / 4), device="cuda")
+    rng = np.random.default_rng(seed)
+
+    def grad_and_check(P0, dirs):
+        _, dts, _ = evo.run(P0, nsteps)          # the CFL timestep sequence, then held fixed
+        PN, _, ck = evo.run(P0, nsteps, dts, checkpoints=True)
+        lam = evo._embed(2.0 * (PN - Pt))
+        rec = adj = 0.0
+        for k in reversed(range(nsteps)):
+            evo.adj.step(ck[k], dts[k], lam)
+            f, g_ = evo.adj.last_times()
+            rec += f
+            adj += g_
+        g = lam[inner].cpu().numpy()
+        loss = lambda P: float(((evo.run(P, nsteps, dts)[0] - Pt) ** 2).sum())
+        fd, ad = [], []
+        for v in dirs:
+            vt = torch.as_tensor(v, device="cuda")
+            fd.append((loss(P0 + h * vt) - loss(P0 - h * vt)) / (2 * h))
+            ad.append(float((g * v).sum()))
+        fd, ad = np.array(fd), np.array(ad)
+        err = np.abs(fd - ad)
+        chk = {"directions": len(fd), "max_abs_err_over_max_fd": float(err.max() / np.abs(fd).max()),
+               "median_rel_err": float(np.median(err / np.abs(fd).clip(1e-300)))}
+        return dts, rec / nsteps, adj / nsteps, chk
+
+    # literal C4 state
+    P0 = torch.as_tensor(pr.mhdmode_prims(cfg)[inner].copy(), device="cuda")
+    dirs = []
+    for _ in range(ndirs):
+        v = noise_direction(cfg, rng, (pr.U3, pr.B3))
+        v[:] = v[:, :1]
+        dirs.append(v / np.linalg.norm(v))
+    dts, rec_ms, adj_ms, chk_c4 = grad_and_check(P0, dirs)
+    # forward GPU time per step (events on the stream the driver runs on)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    evo.drv.init_prims(evo._embed(P0))
+    t = 0.0
+    for k in range(nsteps):
+        evo.drv.set_clock(t, dts[k], k)
+        evo.drv.step(1)
+        t += dts[k]
+    e1.record()
+    torch.cuda.synchronize()
+    fwd_ms = e0.elapsed_time(e1) / nsteps
+    # generic smooth state
+    Pg = torch.as_tensor(generic_state(cfg, rng), device="cuda")
+    dirs = [smooth_direction(cfg, rng) for _ in range(16)] + [noise_direction(cfg, rng, range(pr.NV)) for _ in range(8)]
+    _, _, _, chk_gen = grad_and_check(Pg, dirs)
+    back_ms = rec_ms + adj_ms
+    return {"workload": f"Alfven mode {n}^3, amp 1e-4, {nsteps} RK2 steps, FP64, per-step checkpoints",
+            "loss": "|P_N - P_target|^2, P_target = mode at phase + pi/4",
+            "forward_ms_per_step": fwd_ms, "backward_ms_per_step": back_ms,
+            "backward_recompute_ms_per_step": rec_ms, "backward_transpose_ms_per_step": adj_ms,
+            "adjoint_over_forward": back_ms / fwd_ms, "fd_step": h, "seed": seed,
+            "grad_check": {"c4_state_x3_independent_u3_B3_noise": chk_c4,
+                           "generic_state_smooth_and_noise_all_vars": chk_gen}}
 
 
 def _calibration_copy(mb):

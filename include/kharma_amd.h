@@ -238,6 +238,42 @@ int kh_driver_set_kernel_timing(kh_driver* d, int enable);
  * synchronises and resets the accumulators. */
 int kh_driver_kernel_times(kh_driver* d, double* total_ms, int* launches, int nkinds);
 
+/* ------------------------------------------------------------------------
+ * Differentiation form: the reverse-mode adjoint of one driver step.  No
+ * counterpart in the reference (KHARMA has no adjoint); it transposes the
+ * forward step of kh_driver_step stage by stage: WENO5 + LLF face fluxes
+ * (flux/get_flux.hpp:60-373), flux-CT and divergence (b_flux_ct.cpp:307-370,
+ * kharma_driver.hpp:208-231), the RK update (kharma_driver.cpp:339-387) and the
+ * onedw recovery (inverter/onedw.hpp:89-233) by the implicit-function theorem
+ * (lam_U = (dU/dP)^-T lam_P at the converged primitives).  The timestep is a
+ * fixed input of each step.  Supported: FP64 form, WENO5, LLF, every axis
+ * periodic, Cartesian Minkowski or Kerr-Schild metric; a step in which any
+ * zone is floored or fails to invert returns KH_ERR_UNSUPPORTED.
+ * ---------------------------------------------------------------------- */
+typedef struct kh_adjoint kh_adjoint;
+int kh_adjoint_create(const kh_params* p, kh_adjoint** out);
+void kh_adjoint_destroy(kh_adjoint* a);
+int kh_adjoint_set_stream(kh_adjoint* a, void* hip_stream);
+/* lam_P <- (dP_{n+1}/dP_n)^T lam_P for the step P_n -> P_{n+1} taken with timestep dt.
+ * P: the step's starting primitives ([8][nc] doubles, device; the interior is
+ * used, ghosts are refilled).  lam_P ([8][nc], device): in, dL/dP_{n+1} on the
+ * interior (ghost entries ignored); out, dL/dP_n on the interior, zero ghosts.
+ * The step is recomputed from P (per-step checkpointing).  Synchronises. */
+int kh_adjoint_step(kh_adjoint* a, const double* P, double dt, double* lam_P);
+/* Task-level transposes (device arrays over the entire block, FP64):
+ *   kh_adjoint_flux:      lam_P  = sum_d (dF_d/dP)^T lam_F_d   (kh_get_flux, all dirs; lam_F [3][8][nc])
+ *   kh_adjoint_flux_div:  lam_F  = (d dUdt/dF)^T lam_dUdt      (kh_flux_ct + kh_flux_divergence)
+ *   kh_adjoint_utop:      lam_U  = (dU/dP)^-T lam_P            (kh_utop at converged P, interior)
+ *   kh_adjoint_ptou:      lam_P += (dU/dP)^T lam_U             (kh_ptou, interior)
+ * Asynchronous on the adjoint's stream; outputs are overwritten except where noted. */
+int kh_adjoint_flux(kh_adjoint* a, const double* P, const double* lam_F, double* lam_P);
+int kh_adjoint_flux_div(kh_adjoint* a, const double* lam_dUdt, double* lam_F);
+int kh_adjoint_utop(kh_adjoint* a, const double* P, const double* lam_P, double* lam_U);
+int kh_adjoint_ptou(kh_adjoint* a, const double* P, const double* lam_U, double* lam_P);
+/* Milliseconds of the last kh_adjoint_step spent recomputing the forward step
+ * and in the adjoint kernels (HIP events on the adjoint's stream). */
+int kh_adjoint_last_times(kh_adjoint* a, double* forward_ms, double* adjoint_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -127,6 +127,15 @@ def lib():
         "kh_driver_arrays": (i, [vp, pvp, pvp]),
         "kh_driver_set_kernel_timing": (i, [vp, i]),
         "kh_driver_kernel_times": (i, [vp, ctypes.POINTER(d), ctypes.POINTER(i), i]),
+        "kh_adjoint_create": (i, [ctypes.POINTER(KhParams), pvp]),
+        "kh_adjoint_destroy": (None, [vp]),
+        "kh_adjoint_set_stream": (i, [vp, vp]),
+        "kh_adjoint_step": (i, [vp, vp, d, vp]),
+        "kh_adjoint_last_times": (i, [vp, ctypes.POINTER(d), ctypes.POINTER(d)]),
+        "kh_adjoint_flux": (i, [vp, vp, vp, vp]),
+        "kh_adjoint_flux_div": (i, [vp, vp, vp]),
+        "kh_adjoint_utop": (i, [vp, vp, vp, vp]),
+        "kh_adjoint_ptou": (i, [vp, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -426,3 +435,52 @@ class Driver:
         self._bind()
         _check(lib().kh_driver_get_dt_history(self.h, arr, n))
         return list(arr)
+
+
+class AdjointStep:
+    """Reverse-mode adjoint of one driver step (kh_adjoint_step): maps dL/dP_{n+1}
+    to dL/dP_n for the step taken from P_n with timestep dt, recomputing the
+    forward step from P_n.  FP64, WENO5 + LLF, periodic blocks."""
+
+    def __init__(self, cfg):
+        self.cfg = cfg
+        self._p = make_params(dict(cfg, precision="fp64"))
+        h = ctypes.c_void_p()
+        _check(lib().kh_adjoint_create(ctypes.byref(self._p), ctypes.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        try:
+            lib().kh_adjoint_destroy(self.h)
+        except Exception:
+            pass
+
+    def step(self, P, dt, lamP):
+        """In place: lamP (float64 CUDA tensor [8, n3, n2, n1]) <- dL/dP_n."""
+        _check(lib().kh_adjoint_set_stream(self.h, _stream()))
+        _check(lib().kh_adjoint_step(self.h, _ptr(P), float(dt), _ptr(lamP)))
+
+    def _call(self, fn, *args):
+        _check(lib().kh_adjoint_set_stream(self.h, _stream()))
+        _check(getattr(lib(), fn)(self.h, *[_ptr(x) for x in args]))
+
+    def flux(self, P, lamF, lamP):
+        """lamP = sum_d (dF_d/dP)^T lamF[d] (entire block)."""
+        self._call("kh_adjoint_flux", P, lamF, lamP)
+
+    def flux_div(self, lam_dUdt, lamF):
+        """lamF = (d dUdt / dF)^T lam_dUdt, flux-CT included."""
+        self._call("kh_adjoint_flux_div", lam_dUdt, lamF)
+
+    def utop(self, P, lamP, lamU):
+        """lamU = (dU/dP)^-T lamP at each interior zone."""
+        self._call("kh_adjoint_utop", P, lamP, lamU)
+
+    def ptou(self, P, lamU, lamP):
+        """lamP += (dU/dP)^T lamU at each interior zone."""
+        self._call("kh_adjoint_ptou", P, lamU, lamP)
+
+    def last_times(self):
+        f, a = ctypes.c_double(), ctypes.c_double()
+        _check(lib().kh_adjoint_last_times(self.h, ctypes.byref(f), ctypes.byref(a)))
+        return f.value, a.value

@@ -1,0 +1,386 @@
+// kh_adjoint.hpp -- vector-Jacobian products (reverse mode) of the pointwise
+// functions of the GRMHD step, FP64.
+//
+// Each *_vjp function recomputes the forward intermediates it needs from its
+// inputs and returns lambda^T J for the Jacobian J of the forward function at
+// that point.  The forward functions are those of kh_device.hpp:
+//   weno5_cell (flux/reconstruction.hpp:128-204), calc_4vecs
+//   (grmhd/grmhd_functions.hpp:225-246), prim_to_u_and_flux
+//   (flux/flux_functions.hpp:116-178), vchar (flux_functions.hpp:368-439), the
+//   LLF flux (flux/flux.hpp:110-121), apply_geo_floors (floors_functions.hpp:300-332)
+// and p_to_u (flux_functions.hpp:343-352).  Derivatives are those of the
+// mathematical functions the forward code evaluates (its reciprocal estimates
+// and reassociated sums differ from them by rounding only).  At the kinks of
+// max/min the branch the forward code takes carries the whole derivative.
+#pragma once
+#include "kh_device.hpp"
+
+namespace kh {
+
+// ------------------------------------------------------------------ WENO5
+// Reverse of weno5_cell(x1..x5) -> (lout, rout): g[k] = lam_l dl/dx_k + lam_r dr/dx_k.
+// With t_k = EPS + beta_k and normalised weights w_k = (c_k / t_k^2) / sum_j (c_j / t_j^2):
+//   d(out)/d(q_k) = w_k,   d(out)/d(t_k) = -2 w_k (q_k - out) / t_k.
+__device__ __forceinline__ void weno5_cell_vjp(double x1, double x2, double x3, double x4, double x5,
+                                               double lam_l, double lam_r, double g[5]) {
+    constexpr double EPS = 1.e-26;
+    const double a1 = x1 - 2. * x2 + x3, a2 = x1 - 4. * x2 + 3. * x3;
+    const double b1 = x2 - 2. * x3 + x4, b2 = x4 - x2;
+    const double c1 = x3 - 2. * x4 + x5, c2 = x5 - 4. * x4 + 3. * x3;
+    const double t0 = EPS + ((13. / 12.) * a1 * a1 + 0.25 * a2 * a2);
+    const double t1 = EPS + ((13. / 12.) * b1 * b1 + 0.25 * b2 * b2);
+    const double t2 = EPS + ((13. / 12.) * c1 * c1 + 0.25 * c2 * c2);
+    // weights in the product form of the forward code (no under/overflow for t >= EPS)
+    const double s0 = t0 * t0, s1 = t1 * t1, s2 = t2 * t2;
+    const double p12 = s1 * s2, p02 = s0 * s2, p01 = s0 * s1;
+    const double ar0 = (1. / 16.) * p12, ar1 = (5. / 8.) * p02, ar2 = (5. / 16.) * p01;
+    const double al0 = (1. / 16.) * p01, al1 = ar1, al2 = (5. / 16.) * p12;
+    const double iWr = 1. / (ar0 + ar1 + ar2), iWl = 1. / (al0 + al1 + al2);
+    const double wr0 = ar0 * iWr, wr1 = ar1 * iWr, wr2 = ar2 * iWr;
+    const double wl0 = al0 * iWl, wl1 = al1 * iWl, wl2 = al2 * iWl;
+    const double qr0 = (3. / 8.) * x1 - (5. / 4.) * x2 + (15. / 8.) * x3;
+    const double qr1 = (-1. / 8.) * x2 + (3. / 4.) * x3 + (3. / 8.) * x4;
+    const double qr2 = (3. / 8.) * x3 + (3. / 4.) * x4 - (1. / 8.) * x5;
+    const double ql0 = (3. / 8.) * x5 - (5. / 4.) * x4 + (15. / 8.) * x3;   // weight of t2
+    const double ql1 = (-1. / 8.) * x4 + (3. / 4.) * x3 + (3. / 8.) * x2;   // weight of t1
+    const double ql2 = (3. / 8.) * x3 + (3. / 4.) * x2 - (1. / 8.) * x1;    // weight of t0
+    const double rout = qr0 * wr0 + qr1 * wr1 + qr2 * wr2;
+    const double lout = ql0 * wl0 + ql1 * wl1 + ql2 * wl2;
+    // adjoints of the candidate values
+    const double gr0 = lam_r * wr0, gr1 = lam_r * wr1, gr2 = lam_r * wr2;
+    const double gl0 = lam_l * wl0, gl1 = lam_l * wl1, gl2 = lam_l * wl2;
+    // adjoints of t_k
+    const double gt0 = -2. * (gr0 * (qr0 - rout) + gl2 * (ql2 - lout)) / t0;
+    const double gt1 = -2. * (gr1 * (qr1 - rout) + gl1 * (ql1 - lout)) / t1;
+    const double gt2 = -2. * (gr2 * (qr2 - rout) + gl0 * (ql0 - lout)) / t2;
+    // through the candidate values
+    g[0] = (3. / 8.) * gr0 - (1. / 8.) * gl2;
+    g[1] = (-5. / 4.) * gr0 - (1. / 8.) * gr1 + (3. / 8.) * gl1 + (3. / 4.) * gl2;
+    g[2] = (15. / 8.) * gr0 + (3. / 4.) * gr1 + (3. / 8.) * gr2 + (15. / 8.) * gl0 + (3. / 4.) * gl1 + (3. / 8.) * gl2;
+    g[3] = (3. / 8.) * gr1 + (3. / 4.) * gr2 - (5. / 4.) * gl0 - (1. / 8.) * gl1;
+    g[4] = (-1. / 8.) * gr2 + (3. / 8.) * gl0;
+    // through the smoothness indicators: dt/dx = (13/6) c1 dc1/dx + (1/2) c2 dc2/dx
+    const double e0a = (13. / 6.) * a1 * gt0, e0b = 0.5 * a2 * gt0;
+    g[0] += e0a + e0b;
+    g[1] += -2. * e0a - 4. * e0b;
+    g[2] += e0a + 3. * e0b;
+    const double e1a = (13. / 6.) * b1 * gt1, e1b = 0.5 * b2 * gt1;
+    g[1] += e1a - e1b;
+    g[2] += -2. * e1a;
+    g[3] += e1a + e1b;
+    const double e2a = (13. / 6.) * c1 * gt2, e2b = 0.5 * c2 * gt2;
+    g[2] += e2a + 3. * e2b;
+    g[3] += -2. * e2a - 4. * e2b;
+    g[4] += e2a + e2b;
+}
+
+// ------------------------------------------------------------ one face side
+// Reverse of P -> (U = prim_to_u(P), F^DIR = prim_to_flux(P), cmax, cmin = vchar(P))
+// at one point of geometry g.  lU, lF, lcmax, lcmin are the adjoints of the
+// outputs; lP receives the adjoint of P.  DIR = 0 skips F and the speeds (the
+// conserved variables alone, for the inverter and P->U adjoints).
+template <int DIR, class G>
+__device__ __forceinline__ void side_vjp(const G& g, const double P[NV], double gam, const double lU[NV],
+                                         const double lF[NV], double lcmax, double lcmin, double lP[NV]) {
+    // ---- forward intermediates (calc_4vecs, grmhd_functions.hpp:225-246)
+    const double ut1 = P[U1], ut2 = P[U2], ut3 = P[U3];
+    const double gamma = g.lorentz(ut1, ut2, ut3);
+    double alpha = 1., ucon[4];
+    if constexpr (G::flat) {
+        ucon[0] = gamma; ucon[1] = ut1; ucon[2] = ut2; ucon[3] = ut3;
+    } else {
+        alpha = g.alpha();
+        ucon[0] = gamma / alpha;
+        for (int v = 0; v < 3; ++v) ucon[v + 1] = P[U1 + v] - gamma * alpha * g.gcon0(v + 1);
+    }
+    double ucov[4], bcon[4], bcov[4];
+    g.lower(ucon, ucov);
+    bcon[0] = P[B1] * ucov[1] + P[B2] * ucov[2] + P[B3] * ucov[3];
+    const double iu0 = 1. / ucon[0];
+    for (int v = 0; v < 3; ++v) bcon[v + 1] = (P[B1 + v] + bcon[0] * ucon[v + 1]) * iu0;
+    g.lower(bcon, bcov);
+    const double gdet = g.gdet();
+    const double bsq = dot4(bcon, bcov);
+    const double pgas = (gam - 1.) * P[UU];
+    const double eta = pgas + P[RHO] + P[UU] + bsq;
+
+    // ---- reverse
+    double l_ucon[4] = {0., 0., 0., 0.}, l_ucov[4] = {0., 0., 0., 0.};
+    double l_bcon[4] = {0., 0., 0., 0.}, l_bcov[4] = {0., 0., 0., 0.};
+    double l_rho = 0., l_u = 0., l_eta = 0., l_ptot = 0., l_bsq = 0.;
+    double l_B[3] = {0., 0., 0.};
+
+    if constexpr (DIR > 0) {
+        // vchar (flux_functions.hpp:368-439)
+        const double ef = P[RHO] + gam * P[UU];
+        const double ief = 1. / ef;
+        const double cs2 = gam * (gam - 1.) * P[UU] * ief;
+        const double ibef = 1. / (bsq + ef);
+        const double va2 = bsq * ibef;
+        const double cms2 = cs2 + va2 - cs2 * va2;
+        double Asq, Bsq, AB;
+        if constexpr (G::flat) { Asq = 1.; Bsq = -1.; AB = 0.; }
+        else { Asq = g.gcon(DIR, DIR); Bsq = g.gcon(0, 0); AB = g.gcon(0, DIR); }
+        const double Au = ucon[DIR], Bu = ucon[0];
+        const double A = Bu * Bu - (Bsq + Bu * Bu) * cms2;
+        const double B = 2. * (Au * Bu - (AB + Au * Bu) * cms2);
+        const double C = Au * Au - (Asq + Au * Au) * cms2;
+        const double Dd = B * B - 4. * A * C;
+        const double discr = sqrt(smax(Dd, 0.));
+        const double i2A = 1. / (2. * A);
+        const double vp = (B - discr) * i2A, vm = (B + discr) * i2A;
+        // cmax = smax(vp, vm), cmin = smin(vp, vm): ties take vp for both
+        double l_vp, l_vm;
+        if (vp < vm) { l_vp = lcmin; l_vm = lcmax; }
+        else if (vm < vp) { l_vp = lcmax; l_vm = lcmin; }
+        else { l_vp = lcmax + lcmin; l_vm = 0.; }
+        double l_B2 = (l_vp + l_vm) * i2A;
+        const double l_discr = (l_vm - l_vp) * i2A;
+        double l_A = -(l_vp * vp + l_vm * vm) / A;
+        const double l_Dd = (Dd > 0.) ? 0.5 * l_discr / discr : 0.;
+        l_B2 += 2. * B * l_Dd;
+        l_A += -4. * C * l_Dd;
+        const double l_C = -4. * A * l_Dd;
+        const double om = 1. - cms2;
+        l_ucon[DIR] += 2. * om * (l_B2 * Bu + l_C * Au);
+        l_ucon[0] += 2. * om * (l_A * Bu + l_B2 * Au);
+        const double l_cms2 = -l_A * (Bsq + Bu * Bu) - 2. * l_B2 * (AB + Au * Bu) - l_C * (Asq + Au * Au);
+        const double l_cs2 = l_cms2 * (1. - va2), l_va2 = l_cms2 * (1. - cs2);
+        l_bsq += l_va2 * ef * ibef * ibef;
+        double l_ef = -l_va2 * bsq * ibef * ibef;
+        l_u += l_cs2 * gam * (gam - 1.) * ief;
+        l_ef += -l_cs2 * cs2 * ief;
+        l_rho += l_ef;
+        l_u += gam * l_ef;
+    }
+
+    // prim_to_u / prim_to_flux (flux_functions.hpp:116-178): U[UU] and F[UU] carry
+    // + U[RHO] / + F[RHO].
+    {
+        const double lU0 = lU[RHO] + lU[UU];
+        l_rho += lU0 * ucon[0] * gdet;
+        l_ucon[0] += lU0 * P[RHO] * gdet;
+        if constexpr (DIR > 0) {
+            const double lF0 = lF[RHO] + lF[UU];
+            l_rho += lF0 * ucon[DIR] * gdet;
+            l_ucon[DIR] += lF0 * P[RHO] * gdet;
+        }
+#pragma unroll
+        for (int mu = 0; mu < 4; ++mu) {
+            // U[UU+mu] = (eta u^0 u_mu + delta^0_mu ptot - b^0 b_mu) gdet
+            const double a = lU[UU + mu] * gdet;
+            l_eta += a * ucon[0] * ucov[mu];
+            l_ucon[0] += a * eta * ucov[mu];
+            l_ucov[mu] += a * eta * ucon[0];
+            if (mu == 0) l_ptot += a;
+            l_bcon[0] -= a * bcov[mu];
+            l_bcov[mu] -= a * bcon[0];
+            if constexpr (DIR > 0) {
+                // F[UU+mu] = (eta u^d u_mu + delta^d_mu ptot - b^d b_mu) gdet
+                const double f = lF[UU + mu] * gdet;
+                l_eta += f * ucon[DIR] * ucov[mu];
+                l_ucon[DIR] += f * eta * ucov[mu];
+                l_ucov[mu] += f * eta * ucon[DIR];
+                if (mu == DIR) l_ptot += f;
+                l_bcon[DIR] -= f * bcov[mu];
+                l_bcov[mu] -= f * bcon[DIR];
+            }
+        }
+#pragma unroll
+        for (int i = 1; i <= 3; ++i) {
+            l_B[i - 1] += lU[B1 + i - 1] * gdet;
+            if constexpr (DIR > 0) {
+                // F[Bi] = (b^i u^d - b^d u^i) gdet
+                const double f = lF[B1 + i - 1] * gdet;
+                l_bcon[i] += f * ucon[DIR];
+                l_ucon[DIR] += f * bcon[i];
+                l_bcon[DIR] -= f * ucon[i];
+                l_ucon[i] -= f * bcon[DIR];
+            }
+        }
+    }
+    // eta = pgas + rho + u + bsq, ptot = pgas + bsq / 2, pgas = (gam - 1) u
+    l_rho += l_eta;
+    l_u += l_eta + (gam - 1.) * (l_eta + l_ptot);
+    l_bsq += l_eta + 0.5 * l_ptot;
+    // bsq = b^mu b_mu
+#pragma unroll
+    for (int mu = 0; mu < 4; ++mu) {
+        l_bcon[mu] += l_bsq * bcov[mu];
+        l_bcov[mu] += l_bsq * bcon[mu];
+    }
+    // b_mu = g_{mu nu} b^nu (g symmetric)
+    {
+        double t[4];
+        g.lower(l_bcov, t);
+#pragma unroll
+        for (int mu = 0; mu < 4; ++mu) l_bcon[mu] += t[mu];
+    }
+    // b^i = (B^i + b^0 u^i) / u^0
+    double l_iu0 = 0.;
+#pragma unroll
+    for (int i = 1; i <= 3; ++i) {
+        l_B[i - 1] += l_bcon[i] * iu0;
+        l_bcon[0] += l_bcon[i] * ucon[i] * iu0;
+        l_ucon[i] += l_bcon[i] * bcon[0] * iu0;
+        l_iu0 += l_bcon[i] * (P[B1 + i - 1] + bcon[0] * ucon[i]);
+    }
+    l_ucon[0] -= l_iu0 * iu0 * iu0;
+    // b^0 = B^i u_i
+#pragma unroll
+    for (int i = 1; i <= 3; ++i) {
+        l_B[i - 1] += l_bcon[0] * ucov[i];
+        l_ucov[i] += l_bcon[0] * P[B1 + i - 1];
+    }
+    // u_mu = g_{mu nu} u^nu
+    {
+        double t[4];
+        g.lower(l_ucov, t);
+#pragma unroll
+        for (int mu = 0; mu < 4; ++mu) l_ucon[mu] += t[mu];
+    }
+    // u^0 = gamma / alpha, u^i = ut^i - gamma alpha g^{0i}
+    double l_gamma;
+    if constexpr (G::flat) l_gamma = l_ucon[0];
+    else l_gamma = l_ucon[0] / alpha - alpha * (l_ucon[1] * g.gcon0(1) + l_ucon[2] * g.gcon0(2) + l_ucon[3] * g.gcon0(3));
+    // gamma = sqrt(1 + g_ij ut^i ut^j)
+    const double l_q = l_gamma / gamma;   // 2 * d(gamma)/d(qsq) = 1 / gamma
+    double lut[3];
+#pragma unroll
+    for (int i = 1; i <= 3; ++i)
+        lut[i - 1] = l_ucon[i] + l_q * (g.gcov(i, 1) * ut1 + g.gcov(i, 2) * ut2 + g.gcov(i, 3) * ut3);
+    lP[RHO] = l_rho;
+    lP[UU] = l_u;
+    lP[U1] = lut[0]; lP[U2] = lut[1]; lP[U3] = lut[2];
+    lP[B1] = l_B[0]; lP[B2] = l_B[1]; lP[B3] = l_B[2];
+}
+
+// -------------------------------------------------------------- face (LLF)
+// Reverse of the LLF face flux of face_flux_g (kh_flux.hip; get_flux.hpp:237-367)
+// from the two reconstructed states, including the reconstruction floors.
+template <int DIR, class G>
+__device__ __forceinline__ void face_vjp_llf(const G& g, double gam, bool recon_floors, double rflr, double uflr,
+                                             const double Pl_in[NV], const double Pr_in[NV], const double lF[NV],
+                                             double lPl[NV], double lPr[NV]) {
+    double Pl[NV], Pr[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) { Pl[v] = Pl_in[v]; Pr[v] = Pr_in[v]; }
+    // P += max(0, floor - P): the state moves only where it is below the floor
+    bool kl_rho = true, kl_u = true, kr_rho = true, kr_u = true;
+    if (recon_floors) {
+        kl_rho = !(rflr - Pl[RHO] > 0.); kl_u = !(uflr - Pl[UU] > 0.);
+        kr_rho = !(rflr - Pr[RHO] > 0.); kr_u = !(uflr - Pr[UU] > 0.);
+        apply_geo_floors<double>(Pl, rflr, uflr);
+        apply_geo_floors<double>(Pr, rflr, uflr);
+    }
+    double Ul[NV], Fl[NV], Ur[NV], Fr[NV], cmaxL, cminL, cmaxR, cminR;
+    {
+        FourVec<double> D;
+        calc_4vecs<double>(g, Pl, D);
+        prim_to_u_and_flux<double, G, DIR>(g, Pl, D, gam, Ul, Fl);
+        vchar<double, G, DIR>(g, Pl, D, gam, cmaxL, cminL);
+    }
+    {
+        FourVec<double> D;
+        calc_4vecs<double>(g, Pr, D);
+        prim_to_u_and_flux<double, G, DIR>(g, Pr, D, gam, Ur, Fr);
+        vchar<double, G, DIR>(g, Pr, D, gam, cmaxR, cminR);
+    }
+    // cmx = smax(smax(0, cmaxL), cmaxR); cmn = -smin(smin(0, cminL), cminR); ctop = smax(cmx, cmn)
+    const double t1 = smax(0., cmaxL);
+    const bool mx_R = t1 < cmaxR, mx_L = !mx_R && (0. < cmaxL);
+    const double cmx = mx_R ? cmaxR : t1;
+    const double s1 = smin(0., cminL);
+    const bool mn_R = cminR < s1, mn_L = !mn_R && (cminL < 0.);
+    const double cmn = -(mn_R ? cminR : s1);
+    const bool use_cmn = cmx < cmn;
+    const double ctop = use_cmn ? cmn : cmx;
+    double lUl[NV], lUr[NV], lFh[NV], l_ctop = 0.;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        lFh[v] = 0.5 * lF[v];
+        lUl[v] = 0.5 * ctop * lF[v];
+        lUr[v] = -0.5 * ctop * lF[v];
+        l_ctop -= 0.5 * lF[v] * (Ur[v] - Ul[v]);
+    }
+    double lcmaxL = 0., lcmaxR = 0., lcminL = 0., lcminR = 0.;
+    if (use_cmn) {
+        if (mn_R) lcminR = -l_ctop;
+        else if (mn_L) lcminL = -l_ctop;
+    } else {
+        if (mx_R) lcmaxR = l_ctop;
+        else if (mx_L) lcmaxL = l_ctop;
+    }
+    side_vjp<DIR>(g, Pl, gam, lUl, lFh, lcmaxL, lcminL, lPl);
+    side_vjp<DIR>(g, Pr, gam, lUr, lFh, lcmaxR, lcminR, lPr);
+    if (!kl_rho) lPl[RHO] = 0.;
+    if (!kl_u) lPl[UU] = 0.;
+    if (!kr_rho) lPr[RHO] = 0.;
+    if (!kr_u) lPr[UU] = 0.;
+}
+
+// ------------------------------------------------------- inverter (IFT)
+// Adjoint of the converged U->P recovery at P (Inverter::u_to_p<onedw>,
+// inverter/onedw.hpp:89-233, with P.B = U.B / gdet, b_flux_ct.cpp:172-217): by the
+// implicit-function theorem dP/dU = (dU/dP)^-1 for U = p_to_u(P), so
+// lU = (dU/dP)^-T lP.  dU/dP has the block form [[J_hh, J_hB], [0, gdet I]]
+// (hydro rows rho, u, u^1..3; B rows); J_hh^T x = lP_h is solved by Gaussian
+// elimination with partial pivoting, then lU_B = (lP_B - J_hB^T x) / gdet.
+// Returns false if J_hh is singular to working precision.
+template <class G>
+__device__ __forceinline__ bool inverter_vjp(const G& g, const double P[NV], double gam, const double lP[NV],
+                                             double lU[NV]) {
+    double M[5][6];   // [row][col] of J_hh^T with lP_h appended
+    double N[3][5];   // J_hB^T
+    const double zero[NV] = {0., 0., 0., 0., 0., 0., 0., 0.};
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        double e[NV] = {0., 0., 0., 0., 0., 0., 0., 0.};
+        e[k] = 1.;
+        double row[NV];   // (dU_k/dP)^T: row k of dU/dP
+        side_vjp<0>(g, P, gam, e, zero, 0., 0., row);
+#pragma unroll
+        for (int j = 0; j < 5; ++j) M[j][k] = row[j];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) N[j][k] = row[5 + j];
+    }
+#pragma unroll
+    for (int j = 0; j < 5; ++j) M[j][5] = lP[j];
+#pragma unroll
+    for (int col = 0; col < 5; ++col) {
+        int piv = col;
+        double best = fabs(M[col][col]);
+#pragma unroll
+        for (int r = col + 1; r < 5; ++r)
+            if (fabs(M[r][col]) > best) { best = fabs(M[r][col]); piv = r; }
+        if (!(best > 0.)) return false;
+        if (piv != col) {
+#pragma unroll
+            for (int q = 0; q < 6; ++q) { const double t = M[col][q]; M[col][q] = M[piv][q]; M[piv][q] = t; }
+        }
+        const double ip = 1. / M[col][col];
+#pragma unroll
+        for (int r = col + 1; r < 5; ++r) {
+            const double f = M[r][col] * ip;
+#pragma unroll
+            for (int q = col; q < 6; ++q) M[r][q] -= f * M[col][q];
+        }
+    }
+    double x[5];
+#pragma unroll
+    for (int r = 4; r >= 0; --r) {
+        double s = M[r][5];
+#pragma unroll
+        for (int q = r + 1; q < 5; ++q) s -= M[r][q] * x[q];
+        x[r] = s / M[r][r];
+    }
+    const double igd = 1. / g.gdet();
+#pragma unroll
+    for (int j = 0; j < 5; ++j) lU[j] = x[j];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) lU[5 + j] = (lP[5 + j] - (N[j][0] * x[0] + N[j][1] * x[1] + N[j][2] * x[2] +
+                                                           N[j][3] * x[3] + N[j][4] * x[4])) * igd;
+    return true;
+}
+
+}  // namespace kh

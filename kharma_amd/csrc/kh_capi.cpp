@@ -12,7 +12,7 @@
 #include <string>
 #include <vector>
 
-#include "kh_internal.hpp"
+#include "kh_host.hpp"
 
 using namespace kh;
 
@@ -39,46 +39,7 @@ namespace {
 
 }  // namespace
 
-struct kh_block {
-    kh_params p;
-    BlockDims dims;
-    StepConsts consts;
-    GeomDev geo;                     // device geometry cache (curvilinear metrics only)
-    std::vector<void*> geo_bufs;     // owned device allocations behind geo
-    hipStream_t stream = nullptr;
-    DeviceState* scratch = nullptr;  // reductions of the task-level entry points
-    double* partials = nullptr;      // one value per work-group of a reduction
-    long npartials = 0;
-};
 
-struct kh_driver {
-    kh_block* blk = nullptr;
-    double *U0 = nullptr, *P0 = nullptr, *U1 = nullptr, *P1 = nullptr;
-    float *P0f = nullptr, *P1f = nullptr;   // FP32 shadows of P (split precision only)
-    void *F = nullptr, *ctop = nullptr;
-    int *pflag = nullptr, *fflag = nullptr, *fail_list = nullptr;
-    DeviceState* ds = nullptr;
-    double *dt_part = nullptr, *divb_part = nullptr;
-    double* dt_hist = nullptr;
-    int hist_cap = 0;
-    // kernel timing (kh_driver_set_kernel_timing)
-    bool timing = false;
-    std::vector<hipEvent_t> ev_pool;
-    size_t ev_used = 0;
-    struct Rec { int kind; size_t a, b; };
-    std::vector<Rec> recs;
-
-    hipError_t mark(hipEvent_t* out) {
-        if (ev_used == ev_pool.size()) {
-            hipEvent_t e;
-            hipError_t st = hipEventCreate(&e);
-            if (st != hipSuccess) return st;
-            ev_pool.push_back(e);
-        }
-        *out = ev_pool[ev_used++];
-        return hipEventRecord(*out, blk->stream);
-    }
-};
 
 namespace {
 
@@ -175,7 +136,6 @@ void fill_consts(const kh_params* p, StepConsts& c) {
     f.on = p->floors_on;
 }
 
-size_t flux_elem(const kh_block* b) { return b->p.precision == KH_PREC_SPLIT ? sizeof(float) : sizeof(double); }
 
 // Build the block's metric cache on the host and copy it to the device: FP64
 // tables for the centre and the X1/X2 faces (the flux kernels of the split form
@@ -483,8 +443,11 @@ int kh_driver_init_prims(kh_driver* d, const double* P) {
         if (d->timing) { KH_HIP(d->mark(&ev_)); d->recs.push_back({(kind), ia_, d->ev_used - 1}); } \
     } while (0)
 
-int kh_driver_step(kh_driver* d, int nsteps) {
-    if (!d) return fail(KH_ERR_INVALID, "null driver");
+}  // extern "C"
+
+// One stage of the default task collection on the driver's buffers: fluxes,
+// flux fixes, the fused update (+ inversion, floors, P->U), fixups, boundaries.
+int kh::driver_stage(kh_driver* d, int stage) {
     kh_block* b = d->blk;
     const BlockDims& bd = b->dims;
     const StepConsts& c = b->consts;
@@ -493,27 +456,40 @@ int kh_driver_step(kh_driver* d, int nsteps) {
     hipStream_t s = b->stream;
     char* F = (char*)d->F;
     char* ct = (char*)d->ctop;
-    const int n_dt = (int)update_blocks(bd), n_divb = (int)divb_blocks(bd);
     const bool physical = !bd.periodic[0] || !bd.periodic[1];
+    const double* Psub = (stage == 1) ? d->P0 : d->P1;
+    const float* Psubf = (stage == 1) ? d->P0f : d->P1f;
+    const double* Usub = (stage == 1) ? d->U0 : d->U1;
+    double* Pout = (stage == 1) ? d->P1 : d->P0;
+    float* Poutf = (stage == 1) ? d->P1f : d->P0f;
+    double* Uout = (stage == 1) ? d->U1 : d->U0;
+    // signal speeds only feed the timestep estimate of the last stage
+    const int speeds = (stage == 2) ? SPEEDS_CTOP : SPEEDS_NONE;
+    for (int dir = 1; dir <= bd.ndim; ++dir)
+        KH_TIMED(dir - 1, launch_flux_stage(bd, c, b->geo, dir, b->p.recon, prec, speeds, Psub, Psubf,
+                                            F + fe * (dir - 1) * NV * bd.nc, ct + fe * (dir - 1) * bd.nc, s));
+    if (physical) KH_TIMED(3, launch_fix_flux(bd, prec, d->F, s));
+    KH_TIMED(3 + stage, launch_update(bd, c, b->geo, prec, stage, stage == 2, d->F, d->ctop, Usub, d->U0, Psub,
+                                      Uout, Pout, Poutf, d->pflag, d->fflag, d->fail_list, d->ds, d->dt_part, s));
+    if (c.fix_average || c.fix_atmo)
+        KH_TIMED(6, launch_fixup(bd, c, b->geo, stage, Uout, Pout, Poutf, d->pflag, d->fail_list, d->ds, s));
+    if (physical) KH_TIMED(3, launch_bounds(bd, c, b->geo, Uout, Pout, Poutf, s));
+    return KH_OK;
+}
+
+extern "C" {
+
+int kh_driver_step(kh_driver* d, int nsteps) {
+    if (!d) return fail(KH_ERR_INVALID, "null driver");
+    kh_block* b = d->blk;
+    const BlockDims& bd = b->dims;
+    const StepConsts& c = b->consts;
+    hipStream_t s = b->stream;
+    const int n_dt = (int)update_blocks(bd), n_divb = (int)divb_blocks(bd);
     for (int n = 0; n < nsteps; ++n) {
         for (int stage = 1; stage <= 2; ++stage) {
-            const double* Psub = (stage == 1) ? d->P0 : d->P1;
-            const float* Psubf = (stage == 1) ? d->P0f : d->P1f;
-            const double* Usub = (stage == 1) ? d->U0 : d->U1;
-            double* Pout = (stage == 1) ? d->P1 : d->P0;
-            float* Poutf = (stage == 1) ? d->P1f : d->P0f;
-            double* Uout = (stage == 1) ? d->U1 : d->U0;
-            // signal speeds only feed the timestep estimate of the last stage
-            const int speeds = (stage == 2) ? SPEEDS_CTOP : SPEEDS_NONE;
-            for (int dir = 1; dir <= bd.ndim; ++dir)
-                KH_TIMED(dir - 1, launch_flux_stage(bd, c, b->geo, dir, b->p.recon, prec, speeds, Psub, Psubf,
-                                                    F + fe * (dir - 1) * NV * bd.nc, ct + fe * (dir - 1) * bd.nc, s));
-            if (physical) KH_TIMED(3, launch_fix_flux(bd, prec, d->F, s));
-            KH_TIMED(3 + stage, launch_update(bd, c, b->geo, prec, stage, stage == 2, d->F, d->ctop, Usub, d->U0, Psub,
-                                              Uout, Pout, Poutf, d->pflag, d->fflag, d->fail_list, d->ds, d->dt_part, s));
-            if (c.fix_average || c.fix_atmo)
-                KH_TIMED(6, launch_fixup(bd, c, b->geo, stage, Uout, Pout, Poutf, d->pflag, d->fail_list, d->ds, s));
-            if (physical) KH_TIMED(3, launch_bounds(bd, c, b->geo, Uout, Pout, Poutf, s));
+            const int st = driver_stage(d, stage);
+            if (st != KH_OK) return st;
         }
         KH_TIMED(7, launch_max_divb(bd, d->U0, d->divb_part, s));
         KH_TIMED(8, launch_finalize(c, d->ds, d->dt_part, n_dt, d->divb_part, bd.ndim > 1 ? n_divb : 0,

@@ -1,0 +1,54 @@
+// kh_host.hpp -- host-side objects shared by the C ABI translation units:
+// the block context and the stage driver (include/kharma_amd.h).
+#pragma once
+#include <vector>
+
+#include "kh_internal.hpp"
+
+struct kh_block {
+    kh_params p;
+    kh::BlockDims dims;
+    kh::StepConsts consts;
+    kh::GeomDev geo;                     // device geometry cache (curvilinear metrics only)
+    std::vector<void*> geo_bufs;     // owned device allocations behind geo
+    hipStream_t stream = nullptr;
+    kh::DeviceState* scratch = nullptr;  // reductions of the task-level entry points
+    double* partials = nullptr;      // one value per work-group of a reduction
+    long npartials = 0;
+};
+
+struct kh_driver {
+    kh_block* blk = nullptr;
+    double *U0 = nullptr, *P0 = nullptr, *U1 = nullptr, *P1 = nullptr;
+    float *P0f = nullptr, *P1f = nullptr;   // FP32 shadows of P (split precision only)
+    void *F = nullptr, *ctop = nullptr;
+    int *pflag = nullptr, *fflag = nullptr, *fail_list = nullptr;
+    kh::DeviceState* ds = nullptr;
+    double *dt_part = nullptr, *divb_part = nullptr;
+    double* dt_hist = nullptr;
+    int hist_cap = 0;
+    // kernel timing (kh_driver_set_kernel_timing)
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    struct Rec { int kind; size_t a, b; };
+    std::vector<Rec> recs;
+
+    hipError_t mark(hipEvent_t* out) {
+        if (ev_used == ev_pool.size()) {
+            hipEvent_t e;
+            hipError_t st = hipEventCreate(&e);
+            if (st != hipSuccess) return st;
+            ev_pool.push_back(e);
+        }
+        *out = ev_pool[ev_used++];
+        return hipEventRecord(*out, blk->stream);
+    }
+};
+
+namespace kh {
+// Bytes per element of the face-flux arrays (FP32 in the split form).
+inline size_t flux_elem(const kh_block* b) { return b->p.precision == KH_PREC_SPLIT ? sizeof(float) : sizeof(double); }
+// One RK stage (1 or 2) of the driver's step on its current buffers.
+int driver_stage(kh_driver* d, int stage);
+}  // namespace kh

@@ -151,6 +151,21 @@ hipError_t launch_ptou(const BlockDims& b, const StepConsts& c, const GeomDev& g
                        hipStream_t s);
 hipError_t launch_sync(const BlockDims& b, double* A, int nvar, hipStream_t s);
 hipError_t launch_sync_int(const BlockDims& b, int* A, hipStream_t s);
+
+// ---- adjoint (kh_adjoint.hip) ----
+hipError_t launch_adj_fill(double* a, long n, double v, hipStream_t s);
+// lamP += (d F / d P)^T lF over all directions; lamLR is [2][8][nc] scratch.
+hipError_t launch_adj_flux(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const double* P,
+                           const double* lF, double* lamLR, double* lamP, hipStream_t s);
+// lF ([3][8][nc]) = (d dudt / d F)^T ld; lemf is [3][nc] scratch.
+hipError_t launch_adj_divct(const BlockDims& b, const double* ld, double* lF, double* lemf, hipStream_t s);
+hipError_t launch_adj_stage(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const double* Pout,
+                            const double* lPout, const double* lUout, double g0, double g1, double beta_dt, bool same,
+                            double* lUsub, double* lUfull, double* ld, int* err, hipStream_t s);
+hipError_t launch_adj_ptou(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const double* P,
+                           const double* lU, double* lP, hipStream_t s);
+hipError_t launch_adj_fold(const BlockDims& b, double* A, int nvar, hipStream_t s);
+hipError_t launch_adj_count_flags(const BlockDims& b, const int* pflag, const int* fflag, int* err, hipStream_t s);
 hipError_t launch_estimate_dt(const BlockDims& b, const StepConsts& c, int prec, const void* cmax, const void* cmin,
                               double* part, hipStream_t s);
 
