@@ -275,7 +275,7 @@ def _bench_adjoint(n=64, nsteps=20, ndirs=64, seed=20261202, h=1e-7):
         lam = evo._embed(2.0 * (PN - Pt))
         rec = adj = 0.0
         for k in reversed(range(nsteps)):
-            evo.adj.step(ck[k], dts[k], lam)
+            evo.adj.step(ck[k], dts[k], lam, ck[k + 1])
             f, g_ = evo.adj.last_times()
             rec += f
             adj += g_

@@ -256,10 +256,14 @@ void kh_adjoint_destroy(kh_adjoint* a);
 int kh_adjoint_set_stream(kh_adjoint* a, void* hip_stream);
 /* lam_P <- (dP_{n+1}/dP_n)^T lam_P for the step P_n -> P_{n+1} taken with timestep dt.
  * P: the step's starting primitives ([8][nc] doubles, device; the interior is
- * used, ghosts are refilled).  lam_P ([8][nc], device): in, dL/dP_{n+1} on the
- * interior (ghost entries ignored); out, dL/dP_n on the interior, zero ghosts.
- * The step is recomputed from P (per-step checkpointing).  Synchronises. */
-int kh_adjoint_step(kh_adjoint* a, const double* P, double dt, double* lam_P);
+ * used, ghosts are refilled).  P_next: the step's result P_{n+1} (the next
+ * checkpoint; interior used), or NULL to recompute it.  lam_P ([8][nc],
+ * device): in, dL/dP_{n+1} on the interior (ghost entries ignored); out,
+ * dL/dP_n on the interior, zero ghosts.  Stage 1 is recomputed from P
+ * (per-step checkpointing); with P_next given, the caller vouches that no zone
+ * of the step's second stage was floored or failed to invert (the driver's
+ * flags after the step).  Synchronises. */
+int kh_adjoint_step(kh_adjoint* a, const double* P, const double* P_next, double dt, double* lam_P);
 /* Task-level transposes (device arrays over the entire block, FP64):
  *   kh_adjoint_flux:      lam_P  = sum_d (dF_d/dP)^T lam_F_d   (kh_get_flux, all dirs; lam_F [3][8][nc])
  *   kh_adjoint_flux_div:  lam_F  = (d dUdt/dF)^T lam_dUdt      (kh_flux_ct + kh_flux_divergence)

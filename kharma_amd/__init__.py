@@ -130,7 +130,7 @@ def lib():
         "kh_adjoint_create": (i, [ctypes.POINTER(KhParams), pvp]),
         "kh_adjoint_destroy": (None, [vp]),
         "kh_adjoint_set_stream": (i, [vp, vp]),
-        "kh_adjoint_step": (i, [vp, vp, d, vp]),
+        "kh_adjoint_step": (i, [vp, vp, vp, d, vp]),
         "kh_adjoint_last_times": (i, [vp, ctypes.POINTER(d), ctypes.POINTER(d)]),
         "kh_adjoint_flux": (i, [vp, vp, vp, vp]),
         "kh_adjoint_flux_div": (i, [vp, vp, vp]),
@@ -455,10 +455,12 @@ class AdjointStep:
         except Exception:
             pass
 
-    def step(self, P, dt, lamP):
-        """In place: lamP (float64 CUDA tensor [8, n3, n2, n1]) <- dL/dP_n."""
+    def step(self, P, dt, lamP, P_next=None):
+        """In place: lamP (float64 CUDA tensor [8, n3, n2, n1]) <- dL/dP_n for the
+        step from P taken with dt.  P_next, the step's result, spares the
+        recomputation of its second stage (its flags must have been checked)."""
         _check(lib().kh_adjoint_set_stream(self.h, _stream()))
-        _check(lib().kh_adjoint_step(self.h, _ptr(P), float(dt), _ptr(lamP)))
+        _check(lib().kh_adjoint_step(self.h, _ptr(P), _ptr(P_next), float(dt), _ptr(lamP)))
 
     def _call(self, fn, *args):
         _check(lib().kh_adjoint_set_stream(self.h, _stream()))

@@ -42,9 +42,12 @@ class Evolution:
 
     def run(self, P_int, nsteps, dts=None, checkpoints=False):
         """Advance the interior field P_int by nsteps.  Returns (P_N interior,
-        timesteps used, list of per-step starting states or None)."""
+        timesteps used, checkpoints: the nsteps + 1 states P_0 .. P_N, or None).
+        With checkpoints the second stage of every step is checked for floored
+        or failed zones (the adjoint step recomputes and checks the first)."""
         self.drv.init_prims(self._embed(P_int.detach()))
         ckpts = [] if checkpoints else None
+        flagged = torch.zeros((), dtype=torch.int64, device="cuda")
         used = []
         time = 0.0
         for n in range(nsteps):
@@ -54,17 +57,25 @@ class Evolution:
                 P, _ = self.drv.state()
                 ckpts.append(P)
             self.drv.step(1)
+            if checkpoints:
+                pf, ff = self.drv.flags()
+                flagged += ((pf[self.inner[1:]] != 0) | (ff[self.inner[1:]] != 0)).sum()
             dt = dts[n] if dts is not None else self.drv.dt_history(n + 1)[n]
             used.append(float(dt))
             time += float(dt)
         P, _ = self.drv.state()
+        if checkpoints:
+            ckpts.append(P)
+            if int(flagged) > 0:
+                raise kh.KharmaError(f"{int(flagged)} zone-steps were floored or failed to invert; "
+                                     "the adjoint covers unfloored, converged runs only")
         return P[self.inner].contiguous(), used, ckpts
 
     def backward(self, lam_int, ckpts, dts):
         """dL/dP0 (interior) from dL/dP_N (interior) by the adjoint sweep."""
         lam = self._embed(lam_int.detach())
-        for n in reversed(range(len(ckpts))):
-            self.adj.step(ckpts[n], dts[n], lam)
+        for n in reversed(range(len(dts))):
+            self.adj.step(ckpts[n], dts[n], lam, ckpts[n + 1])
         return lam[self.inner].contiguous()
 
 

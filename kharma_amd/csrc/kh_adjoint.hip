@@ -15,7 +15,8 @@
 // write, per cell m, lamL[v][m] (adjoint of the value at m's left face, from
 // face m) and lamR[v][m] (at m's right face, from face m + 1).  adj_recon then
 // applies the WENO5 transpose per cell and gathers the 5-point contributions
-// into lamP through LDS.
+// into lamP through LDS; it reads lamL / lamR only where those faces exist, so
+// the scratch needs no clearing.
 #include "kh_internal.hpp"
 #include "kh_adjoint.hpp"
 
@@ -185,7 +186,10 @@ adj_recon_kernel(BlockDims b, const double* __restrict__ P, const double* __rest
         double g5[5] = {0., 0., 0., 0., 0.};
         if (recon) {
             const double* q = P + v * b.nc + c;
-            weno5_cell_vjp(q[-2 * st], q[-st], q[0], q[st], q[2 * st], lamL[v * b.nc + c], lamR[v * b.nc + c], g5);
+            // the left face of cell n is face n (exists for n >= s), its right face is n + 1 (n <= e)
+            const double ll = (n >= s) ? lamL[v * b.nc + c] : 0.;
+            const double lr = (n <= e) ? lamR[v * b.nc + c] : 0.;
+            weno5_cell_vjp(q[-2 * st], q[-st], q[0], q[st], q[2 * st], ll, lr, g5);
         }
 #pragma unroll
         for (int o = 0; o < 5; ++o) G[o][tn][tt] = g5[o];
@@ -279,8 +283,8 @@ template <int GEO>
 __global__ void __launch_bounds__(ABS)
 adj_stage_kernel(BlockDims b, StepConsts c, GeomDev gd, const double* __restrict__ Pout,
                  const double* __restrict__ lPout, const double* __restrict__ lUout, double g0, double g1,
-                 double beta_dt, int same, double* lUsub, double* lUfull, double* __restrict__ ld,
-                 int* __restrict__ err) {
+                 double beta_dt, int same, int accumulate, double* lUsub, double* lUfull,
+                 double* __restrict__ ld, int* __restrict__ err) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long n = (long)b.nx1 * (b.ndim > 1 ? b.nx2 : 1) * (b.ndim > 2 ? b.nx3 : 1);
     if (t >= n) return;
@@ -308,10 +312,10 @@ adj_stage_kernel(BlockDims b, StepConsts c, GeomDev gd, const double* __restrict
         const double l = lU[v] + (lUout ? lUout[v * b.nc + x] : 0.);
         ld[v * b.nc + x] = beta_dt * l;
         if (same) {
-            lUfull[v * b.nc + x] += l;
+            lUfull[v * b.nc + x] = (accumulate ? lUfull[v * b.nc + x] : 0.) + l;
         } else {
-            lUsub[v * b.nc + x] += g0 * l;
-            lUfull[v * b.nc + x] += g1 * l;
+            lUsub[v * b.nc + x] = (accumulate ? lUsub[v * b.nc + x] : 0.) + g0 * l;
+            lUfull[v * b.nc + x] = (accumulate ? lUfull[v * b.nc + x] : 0.) + g1 * l;
         }
     }
 }
@@ -365,6 +369,7 @@ __global__ void __launch_bounds__(ABS) adj_fold_kernel(BlockDims b, double* __re
         if (k - b.nx3 >= 0) ok[nk++] = -b.nx3;
         if (k + b.nx3 < b.n3) ok[nk++] = b.nx3;
     }
+    if (ni == 1 && nj == 1 && nk == 1) return;   // no periodic image
     const long x = (long)k * b.s3 + (long)j * b.s2 + i;
     for (int v = 0; v < nvar; ++v) {
         double s = 0.;
@@ -414,7 +419,6 @@ hipError_t launch_adj_flux_t(const BlockDims& b, const StepConsts& c, const Geom
     const int klo = (b.ndim > 2) ? b.ks - 1 : b.ks, khi = (b.ndim > 2) ? b.ke + 1 : b.ke;
     const int nj = jhi - jlo + 1, nk = khi - klo + 1;
     for (int dir = 1; dir <= b.ndim; ++dir) {
-        hipLaunchKernelGGL(fill_kernel, agrid(2L * NV * b.nc), dim3(ABS), 0, s, lamL, 2L * NV * b.nc, 0.);
         const double* lFd = lF + (long)(dir - 1) * NV * b.nc;
         if (dir == 1) {
             const long cells = (long)(b.nx1 + 2) * nj * nk;
@@ -473,14 +477,14 @@ hipError_t launch_adj_divct(const BlockDims& b, const double* ld, double* lF, do
 
 hipError_t launch_adj_stage(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const double* Pout,
                             const double* lPout, const double* lUout, double g0, double g1, double beta_dt, bool same,
-                            double* lUsub, double* lUfull, double* ld, int* err, hipStream_t s) {
+                            bool accumulate, double* lUsub, double* lUfull, double* ld, int* err, hipStream_t s) {
     const dim3 g = agrid(ninterior_l(b));
     if (b.geo)
         hipLaunchKernelGGL(adj_stage_kernel<1>, g, dim3(ABS), 0, s, b, c, gd, Pout, lPout, lUout, g0, g1, beta_dt,
-                           (int)same, lUsub, lUfull, ld, err);
+                           (int)same, (int)accumulate, lUsub, lUfull, ld, err);
     else
         hipLaunchKernelGGL(adj_stage_kernel<0>, g, dim3(ABS), 0, s, b, c, gd, Pout, lPout, lUout, g0, g1, beta_dt,
-                           (int)same, lUsub, lUfull, ld, err);
+                           (int)same, (int)accumulate, lUsub, lUfull, ld, err);
     return hipGetLastError();
 }
 

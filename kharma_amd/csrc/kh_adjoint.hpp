@@ -10,7 +10,8 @@
 //   LLF flux (flux/flux.hpp:110-121), apply_geo_floors (floors_functions.hpp:300-332)
 // and p_to_u (flux_functions.hpp:343-352).  Derivatives are those of the
 // mathematical functions the forward code evaluates (its reciprocal estimates
-// and reassociated sums differ from them by rounding only).  At the kinks of
+// and reassociated sums differ from them by rounding only); quotients here are
+// products with rcp() (~1 ulp).  At the kinks of
 // max/min the branch the forward code takes carries the whole derivative.
 #pragma once
 #include "kh_device.hpp"
@@ -35,7 +36,7 @@ __device__ __forceinline__ void weno5_cell_vjp(double x1, double x2, double x3, 
     const double p12 = s1 * s2, p02 = s0 * s2, p01 = s0 * s1;
     const double ar0 = (1. / 16.) * p12, ar1 = (5. / 8.) * p02, ar2 = (5. / 16.) * p01;
     const double al0 = (1. / 16.) * p01, al1 = ar1, al2 = (5. / 16.) * p12;
-    const double iWr = 1. / (ar0 + ar1 + ar2), iWl = 1. / (al0 + al1 + al2);
+    const double iWr = rcp(ar0 + ar1 + ar2), iWl = rcp(al0 + al1 + al2);
     const double wr0 = ar0 * iWr, wr1 = ar1 * iWr, wr2 = ar2 * iWr;
     const double wl0 = al0 * iWl, wl1 = al1 * iWl, wl2 = al2 * iWl;
     const double qr0 = (3. / 8.) * x1 - (5. / 4.) * x2 + (15. / 8.) * x3;
@@ -50,9 +51,9 @@ __device__ __forceinline__ void weno5_cell_vjp(double x1, double x2, double x3, 
     const double gr0 = lam_r * wr0, gr1 = lam_r * wr1, gr2 = lam_r * wr2;
     const double gl0 = lam_l * wl0, gl1 = lam_l * wl1, gl2 = lam_l * wl2;
     // adjoints of t_k
-    const double gt0 = -2. * (gr0 * (qr0 - rout) + gl2 * (ql2 - lout)) / t0;
-    const double gt1 = -2. * (gr1 * (qr1 - rout) + gl1 * (ql1 - lout)) / t1;
-    const double gt2 = -2. * (gr2 * (qr2 - rout) + gl0 * (ql0 - lout)) / t2;
+    const double gt0 = -2. * (gr0 * (qr0 - rout) + gl2 * (ql2 - lout)) * rcp(t0);
+    const double gt1 = -2. * (gr1 * (qr1 - rout) + gl1 * (ql1 - lout)) * rcp(t1);
+    const double gt2 = -2. * (gr2 * (qr2 - rout) + gl0 * (ql0 - lout)) * rcp(t2);
     // through the candidate values
     g[0] = (3. / 8.) * gr0 - (1. / 8.) * gl2;
     g[1] = (-5. / 4.) * gr0 - (1. / 8.) * gr1 + (3. / 8.) * gl1 + (3. / 4.) * gl2;
@@ -75,35 +76,39 @@ __device__ __forceinline__ void weno5_cell_vjp(double x1, double x2, double x3, 
 }
 
 // ------------------------------------------------------------ one face side
-// Reverse of P -> (U = prim_to_u(P), F^DIR = prim_to_flux(P), cmax, cmin = vchar(P))
-// at one point of geometry g.  lU, lF, lcmax, lcmin are the adjoints of the
-// outputs; lP receives the adjoint of P.  DIR = 0 skips F and the speeds (the
-// conserved variables alone, for the inverter and P->U adjoints).
-template <int DIR, class G>
-__device__ __forceinline__ void side_vjp(const G& g, const double P[NV], double gam, const double lU[NV],
-                                         const double lF[NV], double lcmax, double lcmin, double lP[NV]) {
-    // ---- forward intermediates (calc_4vecs, grmhd_functions.hpp:225-246)
-    const double ut1 = P[U1], ut2 = P[U2], ut3 = P[U3];
-    const double gamma = g.lorentz(ut1, ut2, ut3);
-    double alpha = 1., ucon[4];
-    if constexpr (G::flat) {
-        ucon[0] = gamma; ucon[1] = ut1; ucon[2] = ut2; ucon[3] = ut3;
-    } else {
-        alpha = g.alpha();
-        ucon[0] = gamma / alpha;
-        for (int v = 0; v < 3; ++v) ucon[v + 1] = P[U1 + v] - gamma * alpha * g.gcon0(v + 1);
-    }
-    double ucov[4], bcon[4], bcov[4];
-    g.lower(ucon, ucov);
-    bcon[0] = P[B1] * ucov[1] + P[B2] * ucov[2] + P[B3] * ucov[3];
-    const double iu0 = 1. / ucon[0];
-    for (int v = 0; v < 3; ++v) bcon[v + 1] = (P[B1 + v] + bcon[0] * ucon[v + 1]) * iu0;
-    g.lower(bcon, bcov);
-    const double gdet = g.gdet();
-    const double bsq = dot4(bcon, bcov);
-    const double pgas = (gam - 1.) * P[UU];
-    const double eta = pgas + P[RHO] + P[UU] + bsq;
+// Forward intermediates of one point shared by the reverse sweeps below
+// (calc_4vecs, grmhd_functions.hpp:225-246, and the tensor invariants).
+struct SideState {
+    FourVec<double> D;
+    double gamma, alpha, iu0, gdet, bsq, eta;
+};
+template <class G>
+__device__ __forceinline__ void side_forward(const G& g, const double P[NV], double gam, SideState& S) {
+    calc_4vecs<double>(g, P, S.D);
+    S.gamma = g.lorentz(P[U1], P[U2], P[U3]);
+    if constexpr (G::flat) S.alpha = 1.;
+    else S.alpha = g.alpha();
+    S.iu0 = rcp(S.D.ucon[0]);
+    S.gdet = g.gdet();
+    S.bsq = dot4(S.D.bcon, S.D.bcov);
+    S.eta = (gam - 1.) * P[UU] + P[RHO] + P[UU] + S.bsq;
+}
 
+// Reverse of P -> (U = prim_to_u(P), F^DIR = prim_to_flux(P), cmax, cmin = vchar(P))
+// at one point of geometry g, from the forward intermediates S of side_forward.
+// lU, lF, lcmax, lcmin are the adjoints of the outputs; lP receives the
+// adjoint of P.  DIR = 0 skips F and the speeds (the conserved variables
+// alone, for the inverter and P->U adjoints).
+template <int DIR, class G>
+__device__ __forceinline__ void side_reverse(const G& g, const double P[NV], double gam, const SideState& S,
+                                             const double lU[NV], const double lF[NV], double lcmax, double lcmin,
+                                             double lP[NV]) {
+    const double ut1 = P[U1], ut2 = P[U2], ut3 = P[U3];
+    const double* ucon = S.D.ucon;
+    const double* ucov = S.D.ucov;
+    const double* bcon = S.D.bcon;
+    const double* bcov = S.D.bcov;
+    const double gamma = S.gamma, alpha = S.alpha, iu0 = S.iu0, gdet = S.gdet, bsq = S.bsq, eta = S.eta;
     // ---- reverse
     double l_ucon[4] = {0., 0., 0., 0.}, l_ucov[4] = {0., 0., 0., 0.};
     double l_bcon[4] = {0., 0., 0., 0.}, l_bcov[4] = {0., 0., 0., 0.};
@@ -113,9 +118,9 @@ __device__ __forceinline__ void side_vjp(const G& g, const double P[NV], double 
     if constexpr (DIR > 0) {
         // vchar (flux_functions.hpp:368-439)
         const double ef = P[RHO] + gam * P[UU];
-        const double ief = 1. / ef;
+        const double ief = rcp(ef);
         const double cs2 = gam * (gam - 1.) * P[UU] * ief;
-        const double ibef = 1. / (bsq + ef);
+        const double ibef = rcp(bsq + ef);
         const double va2 = bsq * ibef;
         const double cms2 = cs2 + va2 - cs2 * va2;
         double Asq, Bsq, AB;
@@ -127,7 +132,7 @@ __device__ __forceinline__ void side_vjp(const G& g, const double P[NV], double 
         const double C = Au * Au - (Asq + Au * Au) * cms2;
         const double Dd = B * B - 4. * A * C;
         const double discr = sqrt(smax(Dd, 0.));
-        const double i2A = 1. / (2. * A);
+        const double i2A = rcp(2. * A);
         const double vp = (B - discr) * i2A, vm = (B + discr) * i2A;
         // cmax = smax(vp, vm), cmin = smin(vp, vm): ties take vp for both
         double l_vp, l_vm;
@@ -136,8 +141,8 @@ __device__ __forceinline__ void side_vjp(const G& g, const double P[NV], double 
         else { l_vp = lcmax + lcmin; l_vm = 0.; }
         double l_B2 = (l_vp + l_vm) * i2A;
         const double l_discr = (l_vm - l_vp) * i2A;
-        double l_A = -(l_vp * vp + l_vm * vm) / A;
-        const double l_Dd = (Dd > 0.) ? 0.5 * l_discr / discr : 0.;
+        double l_A = -2. * (l_vp * vp + l_vm * vm) * i2A;
+        const double l_Dd = (Dd > 0.) ? 0.5 * l_discr * rcp(discr) : 0.;
         l_B2 += 2. * B * l_Dd;
         l_A += -4. * C * l_Dd;
         const double l_C = -4. * A * l_Dd;
@@ -242,9 +247,9 @@ __device__ __forceinline__ void side_vjp(const G& g, const double P[NV], double 
     // u^0 = gamma / alpha, u^i = ut^i - gamma alpha g^{0i}
     double l_gamma;
     if constexpr (G::flat) l_gamma = l_ucon[0];
-    else l_gamma = l_ucon[0] / alpha - alpha * (l_ucon[1] * g.gcon0(1) + l_ucon[2] * g.gcon0(2) + l_ucon[3] * g.gcon0(3));
+    else l_gamma = l_ucon[0] * rcp(alpha) - alpha * (l_ucon[1] * g.gcon0(1) + l_ucon[2] * g.gcon0(2) + l_ucon[3] * g.gcon0(3));
     // gamma = sqrt(1 + g_ij ut^i ut^j)
-    const double l_q = l_gamma / gamma;   // 2 * d(gamma)/d(qsq) = 1 / gamma
+    const double l_q = l_gamma * rcp(gamma);   // 2 * d(gamma)/d(qsq) = 1 / gamma
     double lut[3];
 #pragma unroll
     for (int i = 1; i <= 3; ++i)
@@ -253,6 +258,13 @@ __device__ __forceinline__ void side_vjp(const G& g, const double P[NV], double 
     lP[UU] = l_u;
     lP[U1] = lut[0]; lP[U2] = lut[1]; lP[U3] = lut[2];
     lP[B1] = l_B[0]; lP[B2] = l_B[1]; lP[B3] = l_B[2];
+}
+template <int DIR, class G>
+__device__ __forceinline__ void side_vjp(const G& g, const double P[NV], double gam, const double lU[NV],
+                                         const double lF[NV], double lcmax, double lcmin, double lP[NV]) {
+    SideState S;
+    side_forward(g, P, gam, S);
+    side_reverse<DIR>(g, P, gam, S, lU, lF, lcmax, lcmin, lP);
 }
 
 // -------------------------------------------------------------- face (LLF)
@@ -274,18 +286,13 @@ __device__ __forceinline__ void face_vjp_llf(const G& g, double gam, bool recon_
         apply_geo_floors<double>(Pr, rflr, uflr);
     }
     double Ul[NV], Fl[NV], Ur[NV], Fr[NV], cmaxL, cminL, cmaxR, cminR;
-    {
-        FourVec<double> D;
-        calc_4vecs<double>(g, Pl, D);
-        prim_to_u_and_flux<double, G, DIR>(g, Pl, D, gam, Ul, Fl);
-        vchar<double, G, DIR>(g, Pl, D, gam, cmaxL, cminL);
-    }
-    {
-        FourVec<double> D;
-        calc_4vecs<double>(g, Pr, D);
-        prim_to_u_and_flux<double, G, DIR>(g, Pr, D, gam, Ur, Fr);
-        vchar<double, G, DIR>(g, Pr, D, gam, cmaxR, cminR);
-    }
+    SideState SL, SR;
+    side_forward(g, Pl, gam, SL);
+    prim_to_u_and_flux<double, G, DIR>(g, Pl, SL.D, gam, Ul, Fl);
+    vchar<double, G, DIR>(g, Pl, SL.D, gam, cmaxL, cminL);
+    side_forward(g, Pr, gam, SR);
+    prim_to_u_and_flux<double, G, DIR>(g, Pr, SR.D, gam, Ur, Fr);
+    vchar<double, G, DIR>(g, Pr, SR.D, gam, cmaxR, cminR);
     // cmx = smax(smax(0, cmaxL), cmaxR); cmn = -smin(smin(0, cminL), cminR); ctop = smax(cmx, cmn)
     const double t1 = smax(0., cmaxL);
     const bool mx_R = t1 < cmaxR, mx_L = !mx_R && (0. < cmaxL);
@@ -311,8 +318,8 @@ __device__ __forceinline__ void face_vjp_llf(const G& g, double gam, bool recon_
         if (mx_R) lcmaxR = l_ctop;
         else if (mx_L) lcmaxL = l_ctop;
     }
-    side_vjp<DIR>(g, Pl, gam, lUl, lFh, lcmaxL, lcminL, lPl);
-    side_vjp<DIR>(g, Pr, gam, lUr, lFh, lcmaxR, lcminR, lPr);
+    side_reverse<DIR>(g, Pl, gam, SL, lUl, lFh, lcmaxL, lcminL, lPl);
+    side_reverse<DIR>(g, Pr, gam, SR, lUr, lFh, lcmaxR, lcminR, lPr);
     if (!kl_rho) lPl[RHO] = 0.;
     if (!kl_u) lPl[UU] = 0.;
     if (!kr_rho) lPr[RHO] = 0.;
@@ -333,12 +340,14 @@ __device__ __forceinline__ bool inverter_vjp(const G& g, const double P[NV], dou
     double M[5][6];   // [row][col] of J_hh^T with lP_h appended
     double N[3][5];   // J_hB^T
     const double zero[NV] = {0., 0., 0., 0., 0., 0., 0., 0.};
+    SideState S;
+    side_forward(g, P, gam, S);
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
         double e[NV] = {0., 0., 0., 0., 0., 0., 0., 0.};
         e[k] = 1.;
         double row[NV];   // (dU_k/dP)^T: row k of dU/dP
-        side_vjp<0>(g, P, gam, e, zero, 0., 0., row);
+        side_reverse<0>(g, P, gam, S, e, zero, 0., 0., row);
 #pragma unroll
         for (int j = 0; j < 5; ++j) M[j][k] = row[j];
 #pragma unroll
@@ -346,19 +355,22 @@ __device__ __forceinline__ bool inverter_vjp(const G& g, const double P[NV], dou
     }
 #pragma unroll
     for (int j = 0; j < 5; ++j) M[j][5] = lP[j];
+    // Partial pivoting by conditional row swaps with static indices only (a
+    // runtime-indexed row would send M to scratch memory).
 #pragma unroll
     for (int col = 0; col < 5; ++col) {
-        int piv = col;
-        double best = fabs(M[col][col]);
 #pragma unroll
-        for (int r = col + 1; r < 5; ++r)
-            if (fabs(M[r][col]) > best) { best = fabs(M[r][col]); piv = r; }
-        if (!(best > 0.)) return false;
-        if (piv != col) {
+        for (int r = col + 1; r < 5; ++r) {
+            const bool sw = fabs(M[r][col]) > fabs(M[col][col]);
 #pragma unroll
-            for (int q = 0; q < 6; ++q) { const double t = M[col][q]; M[col][q] = M[piv][q]; M[piv][q] = t; }
+            for (int q = col; q < 6; ++q) {
+                const double a = M[col][q], c = M[r][q];
+                M[col][q] = sw ? c : a;
+                M[r][q] = sw ? a : c;
+            }
         }
-        const double ip = 1. / M[col][col];
+        if (!(fabs(M[col][col]) > 0.)) return false;
+        const double ip = rcp(M[col][col]);
 #pragma unroll
         for (int r = col + 1; r < 5; ++r) {
             const double f = M[r][col] * ip;
@@ -372,9 +384,9 @@ __device__ __forceinline__ bool inverter_vjp(const G& g, const double P[NV], dou
         double s = M[r][5];
 #pragma unroll
         for (int q = r + 1; q < 5; ++q) s -= M[r][q] * x[q];
-        x[r] = s / M[r][r];
+        x[r] = s * rcp(M[r][r]);
     }
-    const double igd = 1. / g.gdet();
+    const double igd = rcp(g.gdet());
 #pragma unroll
     for (int j = 0; j < 5; ++j) lU[j] = x[j];
 #pragma unroll

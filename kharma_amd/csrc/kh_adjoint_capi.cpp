@@ -20,9 +20,8 @@ using namespace kh;
 
 struct kh_adjoint {
     kh_driver* drv = nullptr;
-    double* P0 = nullptr;      // the step's starting primitives, ghosts refilled
-    double* lP = nullptr;      // adjoint of the stage's output primitives
-    double* lPin = nullptr;    // adjoint of the stage's input primitives
+    double* P0 = nullptr;      // copy of the step's starting primitives when stage 2 is recomputed
+    double* lPin = nullptr;    // adjoint of the stage-2 input primitives (P1)
     double* lU0 = nullptr;     // adjoint of U_0 (the step's starting conserved state)
     double* lU1 = nullptr;     // adjoint of U_1
     double* ld = nullptr;      // adjoint of dU/dt
@@ -42,21 +41,24 @@ namespace {
         if (e_ != hipSuccess) return api_fail(KH_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-// Transpose of one stage.  P_in, P_out: the stage's input and output primitives;
-// lP holds lam(P_out) on entry; on return lPin holds lam(P_in) (interior, ghosts
-// zero).  lUout may be null (no adjoint on the stage's output conserved state).
-int stage_transpose(kh_adjoint* a, int stage, const double* Pin, const double* Pout, const double* lUout,
-                    double* lUsub, double* lUfull, bool same, double dt) {
+// Transpose of one stage.  Pin, Pout: the stage's input and output primitives.
+// lPout holds lam(P_out) (interior); lPin receives lam(P_in) (interior, ghosts
+// zero).  lUout may be null (no adjoint on the stage's output conserved
+// state).  accumulate = false: lUsub / lUfull are overwritten, not added to.
+int stage_transpose(kh_adjoint* a, int stage, const double* Pin, const double* Pout, const double* lPout,
+                    const double* lUout, double* lUsub, double* lUfull, bool same, bool accumulate, double dt,
+                    double* lPin) {
     kh_block* b = a->drv->blk;
     const BlockDims& bd = b->dims;
     const StepConsts& c = b->consts;
     hipStream_t s = b->stream;
     const double g0 = c.gam0[stage - 1], g1 = c.gam1[stage - 1], beta_dt = c.beta[stage - 1] * dt;
-    KA_HIP(launch_adj_stage(bd, c, b->geo, Pout, a->lP, lUout, g0, g1, beta_dt, same, lUsub, lUfull, a->ld, a->err, s));
+    KA_HIP(launch_adj_stage(bd, c, b->geo, Pout, lPout, lUout, g0, g1, beta_dt, same, accumulate, lUsub, lUfull,
+                            a->ld, a->err, s));
     KA_HIP(launch_adj_divct(bd, a->ld, a->lF, a->lemf, s));
-    KA_HIP(launch_adj_fill(a->lPin, (long)NV * bd.nc, 0., s));
-    KA_HIP(launch_adj_flux(bd, c, b->geo, Pin, a->lF, a->lLR, a->lPin, s));
-    KA_HIP(launch_adj_fold(bd, a->lPin, NV, s));
+    KA_HIP(launch_adj_fill(lPin, (long)NV * bd.nc, 0., s));
+    KA_HIP(launch_adj_flux(bd, c, b->geo, Pin, a->lF, a->lLR, lPin, s));
+    KA_HIP(launch_adj_fold(bd, lPin, NV, s));
     return KH_OK;
 }
 
@@ -85,7 +87,6 @@ int kh_adjoint_create(const kh_params* p, kh_adjoint** out) {
         if (e == hipSuccess) e = hipMemset(*q, 0, bytes);
     };
     alloc(&a->P0, sb);
-    alloc(&a->lP, sb);
     alloc(&a->lPin, sb);
     alloc(&a->lU0, sb);
     alloc(&a->lU1, sb);
@@ -107,7 +108,7 @@ int kh_adjoint_create(const kh_params* p, kh_adjoint** out) {
 void kh_adjoint_destroy(kh_adjoint* a) {
     if (!a) return;
     if (a->drv) (void)hipStreamSynchronize(a->drv->blk->stream);
-    for (double* q : {a->P0, a->lP, a->lPin, a->lU0, a->lU1, a->ld, a->lF, a->lemf, a->lLR})
+    for (double* q : {a->P0, a->lPin, a->lU0, a->lU1, a->ld, a->lF, a->lemf, a->lLR})
         if (q) (void)hipFree(q);
     if (a->err) (void)hipFree(a->err);
     for (hipEvent_t ev : a->ev)
@@ -121,7 +122,7 @@ int kh_adjoint_set_stream(kh_adjoint* a, void* stream) {
     return kh_driver_set_stream(a->drv, stream);
 }
 
-int kh_adjoint_step(kh_adjoint* a, const double* P, double dt, double* lam_P) {
+int kh_adjoint_step(kh_adjoint* a, const double* P, const double* P_next, double dt, double* lam_P) {
     if (!a || !P || !lam_P) return api_fail(KH_ERR_INVALID, "null argument");
     if (!(dt > 0.)) return api_fail(KH_ERR_INVALID, "dt must be positive");
     kh_driver* d = a->drv;
@@ -132,33 +133,33 @@ int kh_adjoint_step(kh_adjoint* a, const double* P, double dt, double* lam_P) {
     const size_t sb = sizeof(double) * NV * bd.nc;
     KA_HIP(hipMemsetAsync(a->err, 0, 2 * sizeof(int), s));
     KA_HIP(hipEventRecord(a->ev[0], s));
-    // ---- forward recompute from the checkpoint
-    KA_HIP(hipMemcpyAsync(a->P0, P, sb, hipMemcpyDeviceToDevice, s));
-    KA_HIP(launch_sync(bd, a->P0, NV, s));
-    int st = kh_driver_init_prims(d, a->P0);
-    if (st != KH_OK) return st;
-    st = kh_driver_set_clock(d, 0., dt, 0);
-    if (st != KH_OK) return st;
-    st = driver_stage(d, 1);                                 // -> P1, U1
+    // ---- forward recompute from the checkpoint: P_0 (ghosts refilled), U_0 = p_to_u(P_0), stage 1
+    KA_HIP(hipMemcpyAsync(d->P0, P, sb, hipMemcpyDeviceToDevice, s));
+    KA_HIP(launch_sync(bd, d->P0, NV, s));
+    KA_HIP(launch_ptou(bd, c, b->geo, d->P0, d->U0, s));
+    KA_HIP(launch_reset_state(d->ds, dt, s));
+    int st = driver_stage(d, 1);                             // -> P1, U1
     if (st != KH_OK) return st;
     KA_HIP(launch_adj_count_flags(bd, d->pflag, d->fflag, a->err, s));
-    // stage 2 writes P2, U2 over the driver's P0, U0; P_0 stays in a->P0
-    st = driver_stage(d, 2);
-    if (st != KH_OK) return st;
-    KA_HIP(launch_adj_count_flags(bd, d->pflag, d->fflag, a->err, s));
+    const double* P0 = d->P0;
+    const double* P2 = P_next;
+    if (!P2) {
+        // stage 2 writes P2, U2 over the driver's P0, U0: keep P_0
+        KA_HIP(hipMemcpyAsync(a->P0, d->P0, sb, hipMemcpyDeviceToDevice, s));
+        P0 = a->P0;
+        if ((st = driver_stage(d, 2)) != KH_OK) return st;
+        KA_HIP(launch_adj_count_flags(bd, d->pflag, d->fflag, a->err, s));
+        P2 = d->P0;
+    }
     KA_HIP(hipEventRecord(a->ev[1], s));
-    // ---- transpose
-    KA_HIP(hipMemcpyAsync(a->lP, lam_P, sb, hipMemcpyDeviceToDevice, s));
-    KA_HIP(launch_adj_fill(a->lU0, (long)NV * bd.nc, 0., s));
-    KA_HIP(launch_adj_fill(a->lU1, (long)NV * bd.nc, 0., s));
-    // stage 2: P1 -> P2 with U_sub = U1, U_full = U0
-    if ((st = stage_transpose(a, 2, d->P1, d->P0, nullptr, a->lU1, a->lU0, false, dt)) != KH_OK) return st;
-    // stage 1: P0 -> P1 with U_sub = U_full = U0
-    KA_HIP(hipMemcpyAsync(a->lP, a->lPin, sb, hipMemcpyDeviceToDevice, s));
-    if ((st = stage_transpose(a, 1, a->P0, d->P1, a->lU1, a->lU0, a->lU0, true, dt)) != KH_OK) return st;
+    // ---- transpose.  stage 2: P1 -> P2 with U_sub = U1, U_full = U0 (writes lU1, lU0)
+    if ((st = stage_transpose(a, 2, d->P1, P2, lam_P, nullptr, a->lU1, a->lU0, false, false, dt, a->lPin)) != KH_OK)
+        return st;
+    // stage 1: P0 -> P1 with U_sub = U_full = U0 (adds to lU0); lam(P_0) lands in lam_P
+    if ((st = stage_transpose(a, 1, P0, d->P1, a->lPin, a->lU1, a->lU0, a->lU0, true, true, dt, lam_P)) != KH_OK)
+        return st;
     // U_0 = p_to_u(P_0)
-    KA_HIP(launch_adj_ptou(bd, c, b->geo, a->P0, a->lU0, a->lPin, s));
-    KA_HIP(hipMemcpyAsync(lam_P, a->lPin, sb, hipMemcpyDeviceToDevice, s));
+    KA_HIP(launch_adj_ptou(bd, c, b->geo, P0, a->lU0, lam_P, s));
     KA_HIP(hipEventRecord(a->ev[2], s));
     int err[2] = {0, 0};
     KA_HIP(hipMemcpyAsync(err, a->err, sizeof(err), hipMemcpyDeviceToHost, s));
@@ -195,8 +196,8 @@ int kh_adjoint_utop(kh_adjoint* a, const double* P, const double* lam_P, double*
     // g0 = 0, g1 = 1, same = true accumulates lam_U* into lam_U (cleared first).
     KA_HIP(launch_adj_fill(lam_U, (long)NV * b->dims.nc, 0., b->stream));
     KA_HIP(hipMemsetAsync(a->err, 0, 2 * sizeof(int), b->stream));
-    KA_HIP(launch_adj_stage(b->dims, b->consts, b->geo, P, lam_P, nullptr, 0., 1., 0., true, lam_U, lam_U, a->ld,
-                            a->err, b->stream));
+    KA_HIP(launch_adj_stage(b->dims, b->consts, b->geo, P, lam_P, nullptr, 0., 1., 0., true, true, lam_U, lam_U,
+                            a->ld, a->err, b->stream));
     return KH_OK;
 }
 

@@ -159,9 +159,10 @@ hipError_t launch_adj_flux(const BlockDims& b, const StepConsts& c, const GeomDe
                            const double* lF, double* lamLR, double* lamP, hipStream_t s);
 // lF ([3][8][nc]) = (d dudt / d F)^T ld; lemf is [3][nc] scratch.
 hipError_t launch_adj_divct(const BlockDims& b, const double* ld, double* lF, double* lemf, hipStream_t s);
+// accumulate = false overwrites lUsub / lUfull on the interior instead of adding to them.
 hipError_t launch_adj_stage(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const double* Pout,
                             const double* lPout, const double* lUout, double g0, double g1, double beta_dt, bool same,
-                            double* lUsub, double* lUfull, double* ld, int* err, hipStream_t s);
+                            bool accumulate, double* lUsub, double* lUfull, double* ld, int* err, hipStream_t s);
 hipError_t launch_adj_ptou(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const double* P,
                            const double* lU, double* lP, hipStream_t s);
 hipError_t launch_adj_fold(const BlockDims& b, double* A, int nvar, hipStream_t s);
