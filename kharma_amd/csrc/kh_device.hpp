@@ -353,7 +353,7 @@ __device__ __forceinline__ int u_to_p_onedw(const G& g, const double U[NV], doub
     if (U[RHO] <= 0.) return S_NEG_INPUT;
     double alpha, a_over_g;
     if constexpr (G::flat) { alpha = 1.; a_over_g = 1.; }
-    else { alpha = g.alpha(); a_over_g = alpha / g.gdet(); }
+    else { alpha = g.alpha(); a_over_g = alpha * rcp(g.gdet()); }
     const double gm1g = (gam - 1) / gam;
     const double D = U[RHO] * a_over_g;
     double Bcon[4] = {0., U[B1] * a_over_g, U[B2] * a_over_g, U[B3] * a_over_g};
@@ -518,11 +518,12 @@ template <class G>
 __device__ __forceinline__ void apply_floors_drift(const G& g, double P[NV], double gam, double rhoflr_max, double uflr_max) {
     double beta[4] = {0., 0., 0., 0.};
     if constexpr (!G::flat) {
-        const double lapse2 = 1. / (-g.gcon(0, 0));
+        const double lapse2 = rcp(-g.gcon(0, 0));
         beta[1] = lapse2 * g.gcon(0, 1);
         beta[2] = lapse2 * g.gcon(0, 2);
         beta[3] = lapse2 * g.gcon(0, 3);
     }
+    // Quotients are products with rcp() (~1 ulp) rather than IEEE divisions.
     const double rho = P[RHO], uu = P[UU];
     const double pg = (gam - 1.) * uu;
     const double w_old = smax(rho + uu + pg, 1e-20);
@@ -531,28 +532,36 @@ __device__ __forceinline__ void apply_floors_drift(const G& g, double P[NV], dou
     g.lower(Bcon, Bcov);
     const double Bsq = smax(dot4(Bcon, Bcov), 1e-20);
     const double B_mag = sqrt(Bsq);
+    const double iBm = rcp(B_mag);
     FourVec<double> D;
     calc_4vecs<double>(g, P, D);
+    const double u0 = D.ucon[0], iu0 = rcp(u0);
     double Qcov[4];
-    Qcov[0] = w_old * D.ucon[0] * D.ucov[0] + pg;
-    Qcov[1] = w_old * D.ucon[0] * D.ucov[1];
-    Qcov[2] = w_old * D.ucon[0] * D.ucov[2];
-    Qcov[3] = w_old * D.ucon[0] * D.ucov[3];
+    Qcov[0] = w_old * u0 * D.ucov[0] + pg;
+    Qcov[1] = w_old * u0 * D.ucov[1];
+    Qcov[2] = w_old * u0 * D.ucov[2];
+    Qcov[3] = w_old * u0 * D.ucov[3];
     const double QdotB = dot4(Bcon, Qcov);
-    double vpar = QdotB / (B_mag * w_old * D.ucon[0] * D.ucon[0]);
+    double vpar = QdotB * rcp(B_mag * w_old * u0 * u0);
     double ucon_dr[4];
-    ucon_dr[0] = 1. / sqrt(1. / (D.ucon[0] * D.ucon[0]) + vpar * vpar);
-    for (int mu = 0; mu < 4; ++mu)
-        ucon_dr[mu] = D.ucon[mu] * (ucon_dr[0] / D.ucon[0]) - (vpar * Bcon[mu] * ucon_dr[0] / B_mag);
+    const double udr0 = rcp(sqrt(iu0 * iu0 + vpar * vpar));
+    {
+        const double a = udr0 * iu0, c = vpar * udr0 * iBm;
+        for (int mu = 0; mu < 4; ++mu) ucon_dr[mu] = D.ucon[mu] * a - Bcon[mu] * c;
+    }
     P[RHO] = smax(rho, rhoflr_max);
     P[UU] = smax(uu, uflr_max);
     const double pg_new = (gam - 1.) * P[UU];
     const double w_new = P[RHO] + P[UU] + pg_new;
-    const double x = (2. * QdotB) / (B_mag * w_new * ucon_dr[0]);
-    vpar = x / (1 + sqrt(1 + x * x)) * (1. / ucon_dr[0]);
+    const double iudr0 = rcp(ucon_dr[0]);
+    const double x = (2. * QdotB) * rcp(B_mag * w_new * ucon_dr[0]);
+    vpar = x * rcp(1 + sqrt(1 + x * x)) * iudr0;
     double ucon[4];
-    ucon[0] = 1. / sqrt(1 / (ucon_dr[0] * ucon_dr[0]) - vpar * vpar);
-    for (int mu = 0; mu < 4; ++mu) ucon[mu] = ucon_dr[mu] * (ucon[0] / ucon_dr[0]) + (vpar * Bcon[mu] * ucon[0] / B_mag);
+    ucon[0] = rcp(sqrt(iudr0 * iudr0 - vpar * vpar));
+    {
+        const double a = ucon[0] * iudr0, c = vpar * ucon[0] * iBm;
+        for (int mu = 1; mu < 4; ++mu) ucon[mu] = ucon_dr[mu] * a + Bcon[mu] * c;
+    }
     P[U1] = ucon[1] + (beta[1] * ucon[0]);
     P[U2] = ucon[2] + (beta[2] * ucon[0]);
     P[U3] = ucon[3] + (beta[3] * ucon[0]);
@@ -574,15 +583,20 @@ __device__ __forceinline__ int apply_floors_normal(const G& g, double P[NV], dou
 }
 
 // Floors::apply_ceilings, floors_functions.hpp:50-82
+// ktot = (gam - 1) u / rho^gam is formed (with its pow) only where the entropy
+// ceiling applies; ktot_exceeds decides that exactly from log2 estimates.
 template <class G>
 __device__ __forceinline__ void apply_ceilings(const G& g, double P[NV], double gam, const FloorParams& f) {
     const double gamma = g.lorentz(P[U1], P[U2], P[U3]);
-    const double ktot = (gam - 1.) * P[UU] / pow(P[RHO], gam);
+    const double u0 = P[UU], rho0 = P[RHO];
     if (gamma > f.gamma_max) {
         const double fr = sqrt((f.gamma_max * f.gamma_max - 1.) / (gamma * gamma - 1.));
         P[U1] *= fr; P[U2] *= fr; P[U3] *= fr;
     }
-    if (ktot > f.ktot_max) P[UU] = f.ktot_max / ktot * P[UU];
+    if (ktot_exceeds(u0, rho0, gam, f.ktot_max)) {
+        const double ktot = (gam - 1.) * u0 / pow(rho0, gam);
+        P[UU] = f.ktot_max / ktot * P[UU];
+    }
     if (f.temp_adjust_u && P[UU] / P[RHO] > f.u_over_rho_max) P[UU] = f.u_over_rho_max * P[RHO];
 }
 

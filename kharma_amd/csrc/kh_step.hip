@@ -210,6 +210,7 @@ template <typename R>
 __device__ __forceinline__ void zone_divergence(const BlockDims& b, const R* __restrict__ F, const R* __restrict__ emf,
                                                 long c, bool ct, double dudt[NV]) {
     const double A1 = b.dx2 * b.dx3, A2 = b.dx1 * b.dx3, A3 = b.dx1 * b.dx2, V = b.dx1 * b.dx2 * b.dx3;
+    const double iV = 1. / V;   // one division per zone instead of eight
     const R* F1 = F;
     const R* F2 = F + 8 * b.nc;
     const R* F3 = F + 16 * b.nc;
@@ -255,7 +256,7 @@ __device__ __forceinline__ void zone_divergence(const BlockDims& b, const R* __r
         double du = (A1 * f1hi[v] - A1 * f1lo[v]);
         if (nd >= 2) du += (A2 * f2hi[v] - A2 * f2lo[v]);
         if (nd == 3) du += (A3 * f3hi[v] - A3 * f3lo[v]);
-        dudt[v] = -du / V;
+        dudt[v] = -du * iV;
     }
 }
 
@@ -310,6 +311,7 @@ __device__ __forceinline__ R emf2_at(const R* F3B1, const R* F1B3, long c, long 
 template <typename R, int ND = 0>
 __device__ __forceinline__ void zone_divergence_ct(const BlockDims& b, const R* __restrict__ F, long c, double dudt[NV]) {
     const double A1 = b.dx2 * b.dx3, A2 = b.dx1 * b.dx3, A3 = b.dx1 * b.dx2, V = b.dx1 * b.dx2 * b.dx3;
+    const double iV = 1. / V;   // one division per zone instead of eight
     const R* F1 = F;
     const R* F2 = F + 8 * b.nc;
     const R* F3 = F + 16 * b.nc;
@@ -365,7 +367,7 @@ __device__ __forceinline__ void zone_divergence_ct(const BlockDims& b, const R* 
         double du = (A1 * f1hi[v] - A1 * f1lo[v]);
         du += (A2 * f2hi[v] - A2 * f2lo[v]);
         if (nd == 3) du += (A3 * f3hi[v] - A3 * f3lo[v]);
-        dudt[v] = -du / V;
+        dudt[v] = -du * iV;
     }
 }
 
@@ -418,7 +420,8 @@ update_kernel(BlockDims b, StepConsts cst, GeomDev gd, int stage, const R* __res
             const int ff0 = (pf / F_GEOM_RHO) * F_GEOM_RHO;
             pf = pf % F_GEOM_RHO;
             const double gd_ = g.gdet();
-            P[B1] = U[B1] / gd_; P[B2] = U[B2] / gd_; P[B3] = U[B3] / gd_;
+            const double igd = rcp(gd_);
+            P[B1] = U[B1] * igd; P[B2] = U[B2] * igd; P[B3] = U[B3] * igd;
             FourVec<double> D;
             calc_4vecs<double>(g, P, D);
             ff = ff0;
@@ -557,7 +560,8 @@ recover_kernel(BlockDims b, StepConsts cst, GeomDev gd, int stage, const double*
         const int ff0 = (pf / F_GEOM_RHO) * F_GEOM_RHO;
         pf = pf % F_GEOM_RHO;
         const double gd_ = g.gdet();
-        P[B1] = U[B1] / gd_; P[B2] = U[B2] / gd_; P[B3] = U[B3] / gd_;
+        const double igd = rcp(gd_);
+            P[B1] = U[B1] * igd; P[B2] = U[B2] * igd; P[B3] = U[B3] * igd;
         FourVec<double> D;
         calc_4vecs<double>(g, P, D);
         ff = ff0;
