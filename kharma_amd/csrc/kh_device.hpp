@@ -92,11 +92,8 @@ __device__ __forceinline__ double rcp(double x) {
     e = fma(-x, r, 1.0);
     return fma(r, e, r);
 }
-__device__ __forceinline__ float rcp(float x) {
-    float r = __builtin_amdgcn_rcpf(x);
-    const float e = fmaf(-x, r, 1.0f);
-    return fmaf(r, e, r);
-}
+// FP32 (split-form fluxes): the hardware estimate alone, which is within 1 ulp.
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
 // Both face values of one cell from its 5-point stencil (x3 is the cell):
 // lout at the left face, rout at the right face, WENO5 weights of
