@@ -234,6 +234,17 @@ int kh_driver_arrays(kh_driver* d, double** U, double** P);
  * 5 stage-2 update, 6 inversion fix-up, 7 divB, 8 timestep finalize. */
 #define KH_NKERNEL_KINDS 9
 int kh_driver_set_kernel_timing(kh_driver* d, int enable);
+/* Face-flux objective accumulated by kh_driver_step (no counterpart in the
+ * reference; the differentiation form's time-integrated flux, cf. the horizon
+ * fluxes of grmhd.cpp:221-233): J += weight * sum_stages w_s dt sum F_dir[var] dA
+ * over the faces of index `face` (entire-block index along dir) and the interior
+ * transverse range, from the fluxes after the physical-boundary fixes; w_s are
+ * the RK weights of the stages' fluxes in the step (1/2, 1/2 for rk2; 0, 1 for
+ * vl2), dA the product of the two transverse zone widths.  var in 0..4;
+ * dir = 0 switches it off. */
+int kh_driver_set_face_flux_objective(kh_driver* d, int dir, int var, int face, double weight);
+/* The accumulated objective (0 if never set); reset != 0 zeroes it.  Synchronises. */
+int kh_driver_objective(kh_driver* d, double* value, int reset);
 /* Sum of launch durations (ms) and launch counts per kind since the last call;
  * synchronises and resets the accumulators. */
 int kh_driver_kernel_times(kh_driver* d, double* total_ms, int* launches, int nkinds);
@@ -245,10 +256,16 @@ int kh_driver_kernel_times(kh_driver* d, double* total_ms, int* launches, int nk
  * (flux/get_flux.hpp:60-373), flux-CT and divergence (b_flux_ct.cpp:307-370,
  * kharma_driver.hpp:208-231), the RK update (kharma_driver.cpp:339-387) and the
  * onedw recovery (inverter/onedw.hpp:89-233) by the implicit-function theorem
- * (lam_U = (dU/dP)^-T lam_P at the converged primitives).  The timestep is a
- * fixed input of each step.  Supported: FP64 form, WENO5, LLF, every axis
- * periodic, Cartesian Minkowski or Kerr-Schild metric; a step in which any
- * zone is floored or fails to invert returns KH_ERR_UNSUPPORTED.
+ * (lam_U = (dU/dP)^-T lam_P at the converged primitives), the geometric source
+ * (flux/flux.cpp:395-444), the floors (floors/floors_impl.hpp:45-128; fluid and
+ * drift frames), the physical-boundary flux fixes (b_flux_ct.cpp:372-420,
+ * boundaries.cpp:611-690) and ghost fills with the inflow check
+ * (boundaries.cpp:369-589, boundaries.hpp:101-136).  The timestep is a fixed
+ * input of each step.  Zones that fail to invert are covered through the
+ * transpose of their fixup (inverter/fixup.cpp:47-156).  Supported: FP64 form,
+ * WENO5, LLF; Cartesian Minkowski (floored zones unsupported there) or
+ * Kerr-Schild metric; any boundaries of the forward step.  A step with a zone
+ * outside this returns KH_ERR_UNSUPPORTED.
  * ---------------------------------------------------------------------- */
 typedef struct kh_adjoint kh_adjoint;
 int kh_adjoint_create(const kh_params* p, kh_adjoint** out);
@@ -260,20 +277,42 @@ int kh_adjoint_set_stream(kh_adjoint* a, void* hip_stream);
  * checkpoint; interior used), or NULL to recompute it.  lam_P ([8][nc],
  * device): in, dL/dP_{n+1} on the interior (ghost entries ignored); out,
  * dL/dP_n on the interior, zero ghosts.  Stage 1 is recomputed from P
- * (per-step checkpointing); with P_next given, the caller vouches that no zone
- * of the step's second stage was floored or failed to invert (the driver's
- * flags after the step).  Synchronises. */
+ * (per-step checkpointing).  P_next is used on periodic Minkowski blocks only,
+ * where the caller vouches that no zone of the step's second stage was floored
+ * or failed to invert (the driver's flags after the step); otherwise stage 2 is
+ * recomputed too.  With a face-flux objective set, lam_P also receives
+ * dJ_n/dP_n for the step's share J_n of the objective.  Synchronises. */
 int kh_adjoint_step(kh_adjoint* a, const double* P, const double* P_next, double dt, double* lam_P);
+/* The objective of kh_driver_set_face_flux_objective, same arguments: its
+ * derivative is added by every following kh_adjoint_step (dir = 0: none). */
+int kh_adjoint_set_face_flux_objective(kh_adjoint* a, int dir, int var, int face, double weight);
 /* Task-level transposes (device arrays over the entire block, FP64):
  *   kh_adjoint_flux:      lam_P  = sum_d (dF_d/dP)^T lam_F_d   (kh_get_flux, all dirs; lam_F [3][8][nc])
  *   kh_adjoint_flux_div:  lam_F  = (d dUdt/dF)^T lam_dUdt      (kh_flux_ct + kh_flux_divergence)
  *   kh_adjoint_utop:      lam_U  = (dU/dP)^-T lam_P            (kh_utop at converged P, interior)
  *   kh_adjoint_ptou:      lam_P += (dU/dP)^T lam_U             (kh_ptou, interior)
+ *   kh_adjoint_bounds:    lam_P on ghost zones moved onto the zones the ghost fill reads
+ *                         (physical faces as kh_apply_bounds fills P, with P.B from U.B =
+ *                         gdet P.B, then periodic images); ghosts zeroed, in place
+ *   kh_adjoint_fix_flux:  lam_F = (dF_fixed/dF)^T lam_F, in place, for the physical-boundary
+ *                         flux fixes kh_flux_ct applies before its CT; F_fixed: the fixed
+ *                         fluxes [3][8][nc] (their X1 mass fluxes decide the inflow clamp)
+ *   kh_adjoint_geo_source: lam_P += (dS/dP)^T lam_dUdt          (kh_add_geo_source, interior)
+ *   kh_adjoint_floors:    lam_P_in = (dP_out/dP_in)^T lam_P_out (kh_apply_floors' floors and
+ *                         ceilings of each interior zone at P_in, fluid or drift frame)
+ *   kh_adjoint_fixup:     lam_P <- (dP_fixed/dP)^T lam_P on the fluid primitives, in place, for
+ *                         kh_fix_utop's replacement of the zones with pflag > 0 (int32 [nc])
+ *                         by the weighted average of their neighbours; P_fixed: its output
  * Asynchronous on the adjoint's stream; outputs are overwritten except where noted. */
 int kh_adjoint_flux(kh_adjoint* a, const double* P, const double* lam_F, double* lam_P);
 int kh_adjoint_flux_div(kh_adjoint* a, const double* lam_dUdt, double* lam_F);
 int kh_adjoint_utop(kh_adjoint* a, const double* P, const double* lam_P, double* lam_U);
 int kh_adjoint_ptou(kh_adjoint* a, const double* P, const double* lam_U, double* lam_P);
+int kh_adjoint_bounds(kh_adjoint* a, const double* P, double* lam_P);
+int kh_adjoint_fix_flux(kh_adjoint* a, const double* F_fixed, double* lam_F);
+int kh_adjoint_geo_source(kh_adjoint* a, const double* P, const double* lam_dUdt, double* lam_P);
+int kh_adjoint_floors(kh_adjoint* a, const double* P_in, const double* lam_P_out, double* lam_P_in);
+int kh_adjoint_fixup(kh_adjoint* a, const double* P_fixed, const int* pflag, double* lam_P);
 /* Milliseconds of the last kh_adjoint_step spent recomputing the forward step
  * and in the adjoint kernels (HIP events on the adjoint's stream). */
 int kh_adjoint_last_times(kh_adjoint* a, double* forward_ms, double* adjoint_ms);

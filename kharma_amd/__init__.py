@@ -136,6 +136,14 @@ def lib():
         "kh_adjoint_flux_div": (i, [vp, vp, vp]),
         "kh_adjoint_utop": (i, [vp, vp, vp, vp]),
         "kh_adjoint_ptou": (i, [vp, vp, vp, vp]),
+        "kh_adjoint_bounds": (i, [vp, vp, vp]),
+        "kh_adjoint_fix_flux": (i, [vp, vp, vp]),
+        "kh_adjoint_geo_source": (i, [vp, vp, vp, vp]),
+        "kh_adjoint_floors": (i, [vp, vp, vp, vp]),
+        "kh_adjoint_fixup": (i, [vp, vp, vp, vp]),
+        "kh_adjoint_set_face_flux_objective": (i, [vp, i, i, i, d]),
+        "kh_driver_set_face_flux_objective": (i, [vp, i, i, i, d]),
+        "kh_driver_objective": (i, [vp, ctypes.POINTER(d), i]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -430,6 +438,17 @@ class Driver:
         _check(lib().kh_driver_kernel_times(self.h, ms, cnt, n))
         return {k: (ms[q], cnt[q]) for q, k in enumerate(self.KERNEL_KINDS)}
 
+    def set_face_flux_objective(self, dir, var, face, weight=1.0):
+        """Accumulate J += weight sum_stages w_s dt sum F_dir[var](face) dA during steps
+        (kh_driver_set_face_flux_objective; dir = 0 switches it off)."""
+        _check(lib().kh_driver_set_face_flux_objective(self.h, int(dir), int(var), int(face), float(weight)))
+
+    def objective(self, reset=False):
+        v = ctypes.c_double()
+        self._bind()
+        _check(lib().kh_driver_objective(self.h, ctypes.byref(v), int(bool(reset))))
+        return v.value
+
     def dt_history(self, n):
         arr = (ctypes.c_double * n)()
         self._bind()
@@ -440,7 +459,8 @@ class Driver:
 class AdjointStep:
     """Reverse-mode adjoint of one driver step (kh_adjoint_step): maps dL/dP_{n+1}
     to dL/dP_n for the step taken from P_n with timestep dt, recomputing the
-    forward step from P_n.  FP64, WENO5 + LLF, periodic blocks."""
+    forward step from P_n.  FP64, WENO5 + LLF; Minkowski or Kerr-Schild blocks
+    with the forward step's boundaries (floors on the Kerr-Schild path)."""
 
     def __init__(self, cfg):
         self.cfg = cfg
@@ -481,6 +501,30 @@ class AdjointStep:
     def ptou(self, P, lamU, lamP):
         """lamP += (dU/dP)^T lamU at each interior zone."""
         self._call("kh_adjoint_ptou", P, lamU, lamP)
+
+    def bounds(self, P, lamP):
+        """In place: ghost-zone adjoints of lamP moved onto the zones the ghost fill reads."""
+        self._call("kh_adjoint_bounds", P, lamP)
+
+    def fix_flux(self, F_fixed, lamF):
+        """In place: lamF <- (dF_fixed/dF)^T lamF for the physical-boundary flux fixes."""
+        self._call("kh_adjoint_fix_flux", F_fixed, lamF)
+
+    def geo_source(self, P, lam_dUdt, lamP):
+        """lamP += (dS/dP)^T lam_dUdt at each interior zone (geometric source)."""
+        self._call("kh_adjoint_geo_source", P, lam_dUdt, lamP)
+
+    def floors(self, P_in, lamP_out, lamP_in):
+        """lamP_in = (dP_out/dP_in)^T lamP_out for the floors of each interior zone."""
+        self._call("kh_adjoint_floors", P_in, lamP_out, lamP_in)
+
+    def fixup(self, P_fixed, pflag, lamP):
+        """In place: lamP <- (dP_fixed/dP)^T lamP for the inversion fixup of the zones pflag > 0."""
+        self._call("kh_adjoint_fixup", P_fixed, pflag, lamP)
+
+    def set_face_flux_objective(self, dir, var, face, weight=1.0):
+        """Seed every following step with the derivative of the driver's face-flux objective."""
+        _check(lib().kh_adjoint_set_face_flux_objective(self.h, int(dir), int(var), int(face), float(weight)))
 
     def last_times(self):
         f, a = ctypes.c_double(), ctypes.c_double()

@@ -6,15 +6,22 @@ libkharma_amd (kh_adjoint_step).
     loss = ((P_N - P_target) ** 2).sum()
     loss.backward()                         # P0.grad via the adjoint sweep
 
-The forward pass runs the FP64 driver step by step and keeps one checkpoint
-(the starting primitives) per step; the backward pass walks the steps in
-reverse, each adjoint step recomputing its forward stages from its checkpoint.
-The timestep sequence is an input of the map: by default it is the CFL sequence
-the driver picks from P0 (recorded in the forward pass) and it is then held
-fixed, so the gradient is that of the map P0 -> P_N at those timesteps.
+    P_N, J = evolve_objective(P0, evo, nsteps)   # J: a face-flux objective of the run
+    J.backward()
 
-Supported blocks are those of kh_adjoint_create: FP64, WENO5 + LLF, every axis
-periodic, with no zone floored or failing to invert during the run.
+The forward pass runs the FP64 driver and keeps checkpoints of the primitives:
+every step (every=1), or every `every` steps, in which case the backward pass
+first re-runs each segment from its checkpoint to recover the per-step states
+(two-level, revolve-style checkpointing: N / every + every states held instead
+of N).  The backward pass walks the steps in reverse, each adjoint step
+recomputing its RK stages from its starting state.  The timestep sequence is an
+input of the map: by default the CFL sequence the driver picks from P0
+(recorded in the forward pass), then held fixed, so the gradient is that of the
+map P0 -> P_N at those timesteps.
+
+Supported blocks are those of kh_adjoint_create: FP64, WENO5 + LLF, Minkowski
+(no zone floored) or Kerr-Schild (floors included); zones failing to invert are
+covered through the transpose of their fixup.
 """
 import math
 
@@ -26,77 +33,143 @@ from kharma_amd import problems as pr
 
 
 class Evolution:
-    """FP64 driver + adjoint context for one block configuration."""
+    """FP64 driver + adjoint context for one block configuration.  objective:
+    None, or dict(dir, var, face, weight=1) of the face-flux objective
+    J = weight sum_steps sum_stages w_s dt sum F_dir[var](face) dA
+    (kh_driver_set_face_flux_objective)."""
 
-    def __init__(self, cfg):
+    def __init__(self, cfg, objective=None):
         self.cfg = dict(cfg, precision="fp64")
         self.drv = kh.Driver(self.cfg)
         self.adj = kh.AdjointStep(self.cfg)
         self.ndim, self.shape = pr.block_shape(self.cfg)
         self.inner = (slice(None),) + pr.interior(self.cfg)
+        self.objective = dict(objective) if objective else None
+        if self.objective:
+            o = self.objective
+            self.drv.set_face_flux_objective(o["dir"], o["var"], o["face"], o.get("weight", 1.0))
 
     def _embed(self, P_int):
         P = torch.zeros((pr.NV,) + self.shape, dtype=torch.float64, device="cuda")
         P[self.inner] = P_int
         return P
 
-    def run(self, P_int, nsteps, dts=None, checkpoints=False):
+    def _check_flags(self, flagged):
+        if int(flagged) > 0:
+            raise kh.KharmaError(f"{int(flagged)} zone-steps were floored on a flat-space block or failed to "
+                                 "invert with the fixup off; the adjoint does not cover them")
+
+    def _count(self, flagged, floors_ok, fixup_ok):
+        pf, ff = self.drv.flags()
+        bad = torch.zeros_like(pf[self.inner[1:]], dtype=torch.bool)
+        if not fixup_ok:
+            bad = bad | (pf[self.inner[1:]] != 0)
+        if not floors_ok:
+            bad = bad | (ff[self.inner[1:]] != 0)
+        flagged += bad.sum()
+
+    def run(self, P_int, nsteps, dts=None, checkpoints=False, every=1):
         """Advance the interior field P_int by nsteps.  Returns (P_N interior,
-        timesteps used, checkpoints: the nsteps + 1 states P_0 .. P_N, or None).
-        With checkpoints the second stage of every step is checked for floored
-        or failed zones (the adjoint step recomputes and checks the first)."""
+        timesteps used, checkpoints {step: full P} every `every` steps and at
+        nsteps, or None, objective J (0 without one)).  With checkpoints the
+        second stage of every step is checked for zones the adjoint does not
+        cover (floored in flat space; failed to invert with the fixup off); the
+        adjoint step recomputes and checks the first."""
         self.drv.init_prims(self._embed(P_int.detach()))
-        ckpts = [] if checkpoints else None
+        if self.objective:
+            self.drv.objective(reset=True)
+        ckpts = {} if checkpoints else None
         flagged = torch.zeros((), dtype=torch.int64, device="cuda")
+        floors_ok = self.cfg.get("coords", "cartesian_minkowski") != "cartesian_minkowski"
+        # the adjoint step recomputes stage 2 (and transposes its fixup) except on
+        # periodic flat-space blocks, where it takes stage 2 from the next checkpoint
+        recomputes = floors_ok or bool(self.cfg.get("boundaries"))
+        fixup_ok = recomputes and (self.cfg.get("fix_average", True) or self.cfg.get("fix_atmo", True))
         used = []
         time = 0.0
         for n in range(nsteps):
             if dts is not None:
                 self.drv.set_clock(time, float(dts[n]), n)
-            if checkpoints:
-                P, _ = self.drv.state()
-                ckpts.append(P)
+            if checkpoints and n % every == 0:
+                ckpts[n] = self.drv.state()[0]
             self.drv.step(1)
             if checkpoints:
-                pf, ff = self.drv.flags()
-                flagged += ((pf[self.inner[1:]] != 0) | (ff[self.inner[1:]] != 0)).sum()
+                self._count(flagged, floors_ok, fixup_ok)
             dt = dts[n] if dts is not None else self.drv.dt_history(n + 1)[n]
             used.append(float(dt))
             time += float(dt)
         P, _ = self.drv.state()
+        J = self.drv.objective() if self.objective else 0.0
         if checkpoints:
-            ckpts.append(P)
-            if int(flagged) > 0:
-                raise kh.KharmaError(f"{int(flagged)} zone-steps were floored or failed to invert; "
-                                     "the adjoint covers unfloored, converged runs only")
-        return P[self.inner].contiguous(), used, ckpts
+            ckpts[nsteps] = P
+            self._check_flags(flagged)
+        return P[self.inner].contiguous(), used, ckpts, J
 
-    def backward(self, lam_int, ckpts, dts):
-        """dL/dP0 (interior) from dL/dP_N (interior) by the adjoint sweep."""
-        lam = self._embed(lam_int.detach())
-        for n in reversed(range(len(dts))):
-            self.adj.step(ckpts[n], dts[n], lam, ckpts[n + 1])
+    def _segment_states(self, ckpts, dts, a, b):
+        """States P_a .. P_b (full), re-run from the checkpoint at step a unless all are held."""
+        if all(n in ckpts for n in range(a, b + 1)):
+            return [ckpts[n] for n in range(a, b + 1)]
+        self.drv.init_prims(ckpts[a])
+        states = [ckpts[a]]
+        time = float(sum(dts[:a]))
+        for n in range(a, b - 1):
+            self.drv.set_clock(time, float(dts[n]), n)
+            self.drv.step(1)
+            time += float(dts[n])
+            states.append(self.drv.state()[0])
+        states.append(ckpts[b])
+        return states
+
+    def backward(self, lam_int, ckpts, dts, gJ=0.0):
+        """dL/dP0 (interior) from dL/dP_N (interior) and dL/dJ by the adjoint sweep."""
+        lam = self._embed(lam_int.detach()) if lam_int is not None else \
+            torch.zeros((pr.NV,) + self.shape, dtype=torch.float64, device="cuda")
+        if self.objective:
+            o = self.objective
+            self.adj.set_face_flux_objective(o["dir"], o["var"], o["face"], o.get("weight", 1.0) * float(gJ))
+        marks = sorted(ckpts)
+        for q in reversed(range(len(marks) - 1)):
+            a, b = marks[q], marks[q + 1]
+            states = self._segment_states(ckpts, dts, a, b)
+            for n in reversed(range(a, b)):
+                self.adj.step(states[n - a], dts[n], lam, states[n - a + 1])
+        if self.objective:
+            self.adj.set_face_flux_objective(0, 0, 0, 0.0)
         return lam[self.inner].contiguous()
 
 
 class _Evolve(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, P0, evo, nsteps, dts):
-        PN, used, ckpts = evo.run(P0, nsteps, dts, checkpoints=True)
+    def forward(ctx, P0, evo, nsteps, dts, every):
+        PN, used, ckpts, J = evo.run(P0, nsteps, dts, checkpoints=True, every=every)
         ctx.evo, ctx.ckpts, ctx.dts = evo, ckpts, used
-        return PN
+        return PN, torch.tensor(J, dtype=torch.float64, device=P0.device)
 
     @staticmethod
-    def backward(ctx, gPN):
-        g = ctx.evo.backward(gPN.to(torch.float64), ctx.ckpts, ctx.dts)
+    def backward(ctx, gPN, gJ):
+        g = ctx.evo.backward(gPN.to(torch.float64) if gPN is not None else None, ctx.ckpts, ctx.dts,
+                             float(gJ) if gJ is not None else 0.0)
         ctx.ckpts = None
-        return g, None, None, None
+        return g, None, None, None, None
 
 
-def evolve(P0, cfg_or_evo, nsteps, dts=None):
+def evolve(P0, cfg_or_evo, nsteps, dts=None, every=1):
     """P_N (interior) after nsteps from the interior field P0, differentiable in P0."""
     evo = cfg_or_evo if isinstance(cfg_or_evo, Evolution) else Evolution(cfg_or_evo)
-    return _Evolve.apply(P0, evo, nsteps, dts)
+    return _Evolve.apply(P0, evo, nsteps, dts, every)[0]
+
+
+def evolve_objective(P0, evo, nsteps, dts=None, every=1):
+    """(P_N interior, J) after nsteps, both differentiable in P0; evo carries the objective."""
+    return _Evolve.apply(P0, evo, nsteps, dts, every)
+
+
+def face_index(cfg, dir, x):
+    """Entire-block index of the face along dir nearest the native coordinate x."""
+    lo, hi = cfg["bounds"][dir - 1]
+    n = cfg["nx"][dir - 1]
+    ng = cfg.get("ng", 4)
+    return ng + int(round((x - lo) / ((hi - lo) / n)))
 
 
 def smooth_direction(cfg, rng, kmax=3, modes=4):

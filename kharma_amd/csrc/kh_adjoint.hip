@@ -101,15 +101,20 @@ adj_face_x1_kernel(BlockDims b, StepConsts c, GeomDev gd, const double* __restri
     face_adjoint<1, GEO>(b, c, gd, jf, b.is - 1 + ii, ci, 1, Pl, lo, lF, lamL, lamR);
 }
 
-// X2 / X3 faces: the 64 x 8 tiles of flux_xt_kernel.
+// X2 / X3 faces: TX x 8 tiles as in flux_xt_kernel (TX = 64 in flat space; 32
+// with the curved metric, whose face adjoint needs more than the 256 registers
+// a wave gets with two waves per SIMD).
 template <int DIR, int GEO>
-__global__ void __launch_bounds__(512)
+constexpr int adj_xt_tx() { return GEO ? 32 : 64; }
+template <int DIR, int GEO>
+__global__ void __launch_bounds__(GEO ? 256 : 512)
 adj_face_xt_kernel(BlockDims b, StepConsts c, GeomDev gd, const double* __restrict__ P, const double* __restrict__ lF,
                    double* __restrict__ lamL, double* __restrict__ lamR, int o0, int no, int s0, int s1) {
-    __shared__ double sr[NV][8][64];
+    constexpr int TX = adj_xt_tx<DIR, GEO>();
+    __shared__ double sr[NV][8][TX];
     const int ni = b.nx1 + 2;
     const int tx = threadIdx.x, ty = threadIdx.y;
-    const int col = blockIdx.x * 64 + tx;
+    const int col = blockIdx.x * TX + tx;
     const int s = s0 - 1 + (int)blockIdx.y * 7 + ty;
     const bool valid = col < ni * no && s <= s1;
     const long st = (DIR == 2) ? b.s2 : b.s3;
@@ -277,45 +282,194 @@ __device__ __forceinline__ void interior_zone(const BlockDims& b, long t, int& k
     k = b.ks + (int)kk;
 }
 
-// Stage transpose at each interior zone: lU* = (dU/dP)^-T lP_out + lU_out, then
-// lam dudt = beta dt lU* and the base-state adjoints (same: U* = U_full).
+// KDomain::GetPhysicalRange membership (domain.hpp:161-187), as the forward fixup uses it.
+__device__ __forceinline__ bool phys_zone(const BlockDims& b, int k, int j, int i) {
+    if (i < 0 || i >= b.n1 || j < 0 || j >= b.n2 || k < 0 || k >= b.n3) return false;
+    if (!b.periodic[0] && (i < b.is || i > b.ie)) return false;
+    if (b.ndim > 1 && !b.periodic[1] && (j < b.js || j > b.je)) return false;
+    if (b.ndim > 2 && !b.periodic[2] && (k < b.ks || k > b.ke)) return false;
+    return true;
+}
+
+// Transpose of fixup_list_kernel (Inverter::FixUtoP, inverter/fixup.cpp:47-156)
+// for the fluid primitives, gathered at each zone y of the physical range: a
+// zone f that failed to invert took P_f = geo_floors(sum_m w_fm P_m / sum_m w_fm)
+// over its non-failed neighbours m (w = 1 / (|l| + |m| + |n| + 1)), and
+// U_f = p_to_u(P_f).  y receives (w_fy / W_f) lam_f for each failed interior
+// neighbour f, with lam_f = lP_out(f) + (dU/dP at P_f)^T lU_out(f) (fluid part,
+// zero where the geometric floor set the value).  lPfx is written on every
+// zone (periodic images are folded afterwards).
 template <int GEO>
+__global__ void __launch_bounds__(ABS)
+adj_fixup_kernel(BlockDims b, StepConsts c, GeomDev gd, const double* __restrict__ Pout,
+                 const double* __restrict__ lPout, const double* __restrict__ lUout, const int* __restrict__ pf,
+                 double* __restrict__ lPfx) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= b.nc) return;
+    const int k = (int)(t / b.s3), j = (int)((t - (long)k * b.s3) / b.s2), i = (int)(t - (long)k * b.s3 - (long)j * b.s2);
+    double acc[5] = {0., 0., 0., 0., 0.};
+    const int nd = b.ndim;
+    if (c.fix_average && phys_zone(b, k, j, i) && !(pf[t] > 0)) {
+        for (int dn = (nd > 2 ? -1 : 0); dn <= (nd > 2 ? 1 : 0); ++dn)
+            for (int dm = (nd > 1 ? -1 : 0); dm <= (nd > 1 ? 1 : 0); ++dm)
+                for (int dl = -1; dl <= 1; ++dl) {
+                    const int fk = k + dn, fj = j + dm, fi = i + dl;
+                    if (fi < b.is || fi > b.ie || fj < b.js || fj > b.je || fk < b.ks || fk > b.ke) continue;
+                    const long f = (long)fk * b.s3 + (long)fj * b.s2 + fi;
+                    if (!(pf[f] > 0)) continue;
+                    // the failed zone's average over its non-failed neighbours
+                    double wsum = 0., sum[2] = {0., 0.};
+                    for (int en = -1; en <= 1; ++en)
+                        for (int em = -1; em <= 1; ++em)
+                            for (int el = -1; el <= 1; ++el) {
+                                const int mk = fk + en, mj = fj + em, mi = fi + el;
+                                if (!phys_zone(b, mk, mj, mi)) continue;
+                                const long m = (long)mk * b.s3 + (long)mj * b.s2 + mi;
+                                if (pf[m] > 0) continue;
+                                const double w = 1. / (abs(el) + abs(em) + abs(en) + 1);
+                                wsum += w;
+                                sum[0] += w * Pout[RHO * b.nc + m];
+                                sum[1] += w * Pout[UU * b.nc + m];
+                            }
+                    if (wsum < 1.e-10) continue;   // atmosphere fallback: no dependence
+                    double rf, uf;
+                    if constexpr (GEO) {
+                        const long q = (long)fj * gd.gn1 + fi;
+                        rf = gd.flr[q]; uf = gd.flr[gd.np + q];
+                    } else {
+                        rf = c.floors.rho_min_const; uf = c.floors.u_min_const;
+                    }
+                    double Pf[NV], lu[NV], lt[NV];
+                    const double zero[NV] = {0., 0., 0., 0., 0., 0., 0., 0.};
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) { Pf[v] = Pout[v * b.nc + f]; lu[v] = lUout ? lUout[v * b.nc + f] : 0.; }
+                    if constexpr (GEO) {
+                        CachedGeom<double> g(gd.g[GLOC_CENTER], gd.np, (long)fj * gd.gn1 + fi);
+                        side_vjp<0>(g, Pf, c.gam, lu, zero, 0., 0., lt);
+                    } else {
+                        MinkowskiGeom<double> g;
+                        side_vjp<0>(g, Pf, c.gam, lu, zero, 0., 0., lt);
+                    }
+#pragma unroll
+                    for (int v = 0; v < 5; ++v) lt[v] += lPout[v * b.nc + f];
+                    if (rf - sum[0] / wsum > 0.) lt[RHO] = 0.;
+                    if (uf - sum[1] / wsum > 0.) lt[UU] = 0.;
+                    const double cw = 1. / ((abs(dl) + abs(dm) + abs(dn) + 1) * wsum);
+#pragma unroll
+                    for (int v = 0; v < 5; ++v) acc[v] += cw * lt[v];
+                }
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) lPfx[v * b.nc + t] = (v < 5) ? acc[v] : 0.;
+}
+
+// Stage transpose at each interior zone.  Unfloored zones: lU* = (dU/dP)^-T
+// lP_out + lU_out at P_out (U_out = p_to_u(P_out) and P_out = u_to_p(U*)).
+// Zones the stage floored (ffs != 0; FRAME >= 0): P_inv = u_to_p(U*) is
+// recovered again from the stage's U* and guess Psub exactly as the forward
+// update does, then lP_inv = (dfloors/dP)^T (lP_out + (dU/dP at P_out)^T lU_out)
+// and lU* = (dU/dP at P_inv)^-T lP_inv.  Then lam dudt = beta dt lU* and the
+// base-state adjoints (same: U* = U_full).  With lPgeo (curved geometry), the
+// geometric source's transpose at the stage input Psub is written (not added)
+// to lPgeo at the zone.
+template <int GEO, int FRAME>
 __global__ void __launch_bounds__(ABS)
 adj_stage_kernel(BlockDims b, StepConsts c, GeomDev gd, const double* __restrict__ Pout,
                  const double* __restrict__ lPout, const double* __restrict__ lUout, double g0, double g1,
                  double beta_dt, int same, int accumulate, double* lUsub, double* lUfull,
-                 double* __restrict__ ld, int* __restrict__ err) {
+                 double* __restrict__ ld, int* __restrict__ err, const double* __restrict__ Ustar,
+                 const double* __restrict__ Psub, const int* __restrict__ ffs, double* __restrict__ lPgeo,
+                 const int* __restrict__ pfs, const double* __restrict__ lPfx) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long n = (long)b.nx1 * (b.ndim > 1 ? b.nx2 : 1) * (b.ndim > 2 ? b.nx3 : 1);
     if (t >= n) return;
     int k, j, i;
     interior_zone(b, t, k, j, i);
     const long x = (long)k * b.s3 + (long)j * b.s2 + i;
-    double P[NV], lP[NV], lU[NV];
+    double P[NV], lP[NV], lU[NV], lUo[NV];
 #pragma unroll
-    for (int v = 0; v < NV; ++v) { P[v] = Pout[v * b.nc + x]; lP[v] = lPout[v * b.nc + x]; }
-    bool ok;
+    for (int v = 0; v < NV; ++v) {
+        P[v] = Pout[v * b.nc + x];
+        lP[v] = lPout[v * b.nc + x] + (lPfx ? lPfx[v * b.nc + x] : 0.);
+        lUo[v] = lUout ? lUout[v * b.nc + x] : 0.;
+    }
+    const bool failed = pfs && pfs[x] > 0;
+    bool ok = true;
+    auto body = [&](const auto& g) {
+        if (failed) {
+            // fixed up from its neighbours (adj_fixup_kernel carries the fluid part);
+            // only P.B = U*.B / gdet depends on this zone's U*
+            double t8[NV];
+            const double zero[NV] = {0., 0., 0., 0., 0., 0., 0., 0.};
+            side_vjp<0>(g, P, c.gam, lUo, zero, 0., 0., t8);
+            const double igd = rcp(g.gdet());
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                lU[v] = (v >= B1) ? (lP[v] + t8[v]) * igd : 0.;
+                lUo[v] = 0.;
+            }
+            return;
+        }
+        if constexpr (FRAME >= 0) {
+            if (ffs[x] != 0) {
+                double t8[NV];
+                const double zero[NV] = {0., 0., 0., 0., 0., 0., 0., 0.};
+                side_vjp<0>(g, P, c.gam, lUo, zero, 0., 0., t8);
+                double Us[NV], Pi[NV], lPi[NV];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) { lP[v] += t8[v]; Us[v] = Ustar[v * b.nc + x]; }
+#pragma unroll
+                for (int v = 0; v < 5; ++v) Pi[v] = Psub[v * b.nc + x];
+                const int st = u_to_p_onedw(g, Us, Pi, c.gam, c.iter_max, c.err_tol);
+                const double igd = rcp(g.gdet());
+                Pi[B1] = Us[B1] * igd; Pi[B2] = Us[B2] * igd; Pi[B3] = Us[B3] * igd;
+                double rf, uf;
+                if constexpr (GEO) { const long q = (long)j * gd.gn1 + i; rf = gd.flr[q]; uf = gd.flr[gd.np + q]; }
+                else { rf = c.floors.rho_min_const; uf = c.floors.u_min_const; }
+                floors_vjp<FRAME>(g, Pi, c.gam, c.floors, rf, uf, lP, lPi);
+                ok = (st % F_GEOM_RHO == 0) && inverter_vjp(g, Pi, c.gam, lPi, lU);
+#pragma unroll
+                for (int v = 0; v < NV; ++v) lUo[v] = 0.;   // already carried through P_out
+                return;
+            }
+        }
+        ok = inverter_vjp(g, P, c.gam, lP, lU);
+    };
     if constexpr (GEO) {
         CachedGeom<double> g(gd.g[GLOC_CENTER], gd.np, (long)j * gd.gn1 + i);
-        ok = inverter_vjp(g, P, c.gam, lP, lU);
+        body(g);
     } else {
         MinkowskiGeom<double> g;
-        ok = inverter_vjp(g, P, c.gam, lP, lU);
+        body(g);
     }
     if (!ok) {
         atomicAdd(&err[1], 1);
 #pragma unroll
         for (int v = 0; v < NV; ++v) lU[v] = 0.;
     }
+    double lS[4];
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
-        const double l = lU[v] + (lUout ? lUout[v * b.nc + x] : 0.);
+        const double l = lU[v] + lUo[v];
         ld[v * b.nc + x] = beta_dt * l;
+        if (v >= UU && v <= U3) lS[v - UU] = beta_dt * l;
         if (same) {
             lUfull[v * b.nc + x] = (accumulate ? lUfull[v * b.nc + x] : 0.) + l;
         } else {
             lUsub[v * b.nc + x] = (accumulate ? lUsub[v * b.nc + x] : 0.) + g0 * l;
             lUfull[v * b.nc + x] = (accumulate ? lUfull[v * b.nc + x] : 0.) + g1 * l;
+        }
+    }
+    if constexpr (GEO) {
+        if (lPgeo) {
+            double Pz[NV], out[NV] = {0., 0., 0., 0., 0., 0., 0., 0.};
+#pragma unroll
+            for (int v = 0; v < NV; ++v) Pz[v] = Psub[v * b.nc + x];
+            const long q = (long)j * gd.gn1 + i;
+            CachedGeom<double> g(gd.g[GLOC_CENTER], gd.np, q);
+            geo_source_vjp(g, gd, q, Pz, c.gam, lS, out);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) lPgeo[v * b.nc + x] = out[v];
         }
     }
 }
@@ -344,6 +498,197 @@ adj_ptou_kernel(BlockDims b, StepConsts c, GeomDev gd, const double* __restrict_
     }
 #pragma unroll
     for (int v = 0; v < NV; ++v) lP[v * b.nc + x] += out[v];
+}
+
+// ------------------------------------------------ physical boundaries
+// The interior zone a physical ghost zone (j, i) is filled from, and how
+// (bounds_kernel, kh_step.hip; KBoundaries::ApplyBoundary, boundaries.cpp:369-589):
+// P_ghost[0..4] = post(inflow(pre(P_src[0..4]))) and U_ghost.B = pre.post(U_src.B),
+// P_ghost.B = U_ghost.B / gdet(ghost), with pre / post the reflections (bit 0:
+// U1, B1 negated; bit 1: U2, B2) before and after the inflow check, which runs
+// (inflow = 1 inner, 2 outer side) with the geometry of zone (gj, gi).
+struct GhostSrc { int j, i, pre, post, inflow, gj, gi; };
+__device__ __forceinline__ bool phys_ghost(const BlockDims& b, int j, int i) {
+    const bool x1g = !b.periodic[0] && (i < b.is || i > b.ie);
+    const bool x2g = b.ndim > 1 && !b.periodic[1] && (j < b.js || j > b.je);
+    return x1g || x2g;
+}
+__device__ __forceinline__ GhostSrc ghost_source(const BlockDims& b, int j, int i) {
+    const bool x1g = !b.periodic[0] && (i < b.is || i > b.ie);
+    const bool x2g = b.ndim > 1 && !b.periodic[1] && (j < b.js || j > b.je);
+    const bool in1 = i < b.is, in2 = j < b.js;
+    const bool refl1 = b.bc[in1 ? 0 : 1] == KH_BC_REFLECT, refl2 = b.bc[in2 ? 2 : 3] == KH_BC_REFLECT;
+    const int si = refl1 ? (in1 ? 2 * b.is - 1 - i : 2 * b.ie + 1 - i) : (in1 ? b.is : b.ie);
+    const int sj = refl2 ? (in2 ? 2 * b.js - 1 - j : 2 * b.je + 1 - j) : (in2 ? b.js : b.je);
+    const int infl = b.check_inflow[in1 ? 0 : 1] ? (in1 ? 1 : 2) : 0;
+    GhostSrc r;
+    if (x1g && !x2g) {
+        r = {j, si, refl1 ? 1 : 0, 0, infl, j, i};
+    } else if (x2g && !x1g) {
+        r = {sj, i, refl2 ? 2 : 0, 0, 0, j, i};
+    } else if (in1 && b.fix_corner_inner) {
+        // the X2 fill of column is (or of the mirror column), then the inner X1 face over it
+        const bool r1 = b.bc[0] == KH_BC_REFLECT;
+        r = {sj, r1 ? 2 * b.is - 1 - i : b.is, (refl2 ? 2 : 0) | (r1 ? 1 : 0), 0, b.check_inflow[0] ? 1 : 0, j, i};
+    } else {
+        // the X1 fill of the mirror / edge row (inflow check with that row's geometry), then the X2 reflection
+        r = {sj, si, refl1 ? 1 : 0, refl2 ? 2 : 0, infl, sj, i};
+    }
+    return r;
+}
+
+// Transpose of the physical ghost fill, gathered at each zone the fill reads
+// (within ng of a physical face): the adjoints held by the ghost zones filled
+// from it are mapped back through the fill and added.  Ghosts are cleared
+// afterwards (adj_fold).
+template <int GEO>
+__global__ void __launch_bounds__(ABS)
+adj_bounds_kernel(BlockDims b, GeomDev gd, const double* __restrict__ P, double* __restrict__ A) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= b.nc) return;
+    const int k = (int)(t / b.s3), j = (int)((t - (long)k * b.s3) / b.s2), i = (int)(t - (long)k * b.s3 - (long)j * b.s2);
+    if (phys_ghost(b, j, i)) return;
+    const bool p1 = !b.periodic[0], p2 = b.ndim > 1 && !b.periodic[1];
+    const bool near1 = p1 && (i < b.is + b.ng || i > b.ie - b.ng);
+    const bool near2 = p2 && (j < b.js + b.ng || j > b.je - b.ng);
+    if (!near1 && !near2) return;
+    // candidate ghosts: this row / column and the ghost rows / columns of each physical axis
+    int ic[9], jc[9], ni = 0, nj = 0;
+    ic[ni++] = i;
+    jc[nj++] = j;
+    if (near1)
+        for (int g = 0; g < 2 * b.ng; ++g) ic[ni++] = (g < b.ng) ? g : b.ie + 1 + (g - b.ng);
+    if (near2)
+        for (int g = 0; g < 2 * b.ng; ++g) jc[nj++] = (g < b.ng) ? g : b.je + 1 + (g - b.ng);
+    double src[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) src[v] = P[v * b.nc + t];
+    double acc[NV] = {0., 0., 0., 0., 0., 0., 0., 0.};
+    bool any = false;
+    for (int qj = 0; qj < nj; ++qj)
+        for (int qi = 0; qi < ni; ++qi) {
+            const int gj = jc[qj], gi = ic[qi];
+            if (!phys_ghost(b, gj, gi)) continue;
+            const GhostSrc m = ghost_source(b, gj, gi);
+            if (m.j != j || m.i != i) continue;
+            any = true;
+            const long gx = (long)k * b.s3 + (long)gj * b.s2 + gi;
+            double l[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) l[v] = A[v * b.nc + gx];
+            // B: P_g.B = flip(U_src.B) / gdet_g = flip(P_src.B) gdet_src / gdet_g
+            double ratio = 1.;
+            if constexpr (GEO) ratio = gd.g[GLOC_CENTER][20 * gd.np + (long)j * gd.gn1 + i] /
+                                       gd.g[GLOC_CENTER][20 * gd.np + (long)gj * gd.gn1 + gi];
+            const int fl = m.pre ^ m.post;
+            acc[B1] += ((fl & 1) ? -l[B1] : l[B1]) * ratio;
+            acc[B2] += ((fl & 2) ? -l[B2] : l[B2]) * ratio;
+            acc[B3] += l[B3] * ratio;
+            // fluid: post flips, inflow transpose at the pre-flipped source state, pre flips
+            if (m.post & 1) l[U1] = -l[U1];
+            if (m.post & 2) l[U2] = -l[U2];
+            if (m.inflow) {
+                double s1[NV];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) s1[v] = src[v];
+                if (m.pre & 1) s1[U1] = -s1[U1];
+                if (m.pre & 2) s1[U2] = -s1[U2];
+                if constexpr (GEO) {
+                    CachedGeom<double> g(gd.g[GLOC_CENTER], gd.np, (long)m.gj * gd.gn1 + m.gi);
+                    inflow_vjp(g, s1, m.inflow == 1, l);
+                } else {
+                    MinkowskiGeom<double> g;
+                    inflow_vjp(g, s1, m.inflow == 1, l);
+                }
+            }
+            if (m.pre & 1) l[U1] = -l[U1];
+            if (m.pre & 2) l[U2] = -l[U2];
+#pragma unroll
+            for (int v = 0; v < 5; ++v) acc[v] += l[v];
+        }
+    if (!any) return;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) A[v * b.nc + t] += acc[v];
+}
+
+// Transpose of fix_flux_kernel (kh_step.hip) on the flux adjoints lF, in
+// place: zeroed polar fluxes get zero adjoint, the mirrored polar B2 fluxes pass
+// theirs (negated) to the mirrored source row, and the X1 mass-flux clamp
+// passes the adjoint where the clamped flux is the Riemann flux.  x1f holds the
+// clamped X1 mass fluxes of the forward stage (save_x1_flux_kernel).
+__global__ void __launch_bounds__(ABS) adj_fix_flux_kernel(BlockDims b, double* __restrict__ lF,
+                                                           const double* __restrict__ x1f) {
+    const int nd = b.ndim;
+    const int ni = b.nx1 + 2, nk = (nd > 2) ? b.nx3 + 2 : 1, nj = (nd > 1) ? b.nx2 + 2 : 1;
+    const int i0 = b.is - 1, k0 = (nd > 2) ? b.ks - 1 : b.ks, j0 = (nd > 1) ? b.js - 1 : b.js;
+    const long nc = b.nc;
+    double* F1 = lF;
+    double* F2 = lF + 8 * nc;
+    double* F3 = lF + 16 * nc;
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned npolar = (nd > 1 && !b.periodic[1]) ? 2u * ni * nk : 0u;
+    if (t < npolar) {
+        const int side = t / (ni * nk), rem = t - side * (ni * nk);
+        const int k = k0 + rem / ni, i = i0 + rem % ni;
+        const int j = side == 0 ? b.js : b.je + 1;
+        const long c = (long)k * b.s3 + (long)j * b.s2 + i;
+        const long cg = side == 0 ? c - b.s2 : c;
+        const long cs = side == 0 ? c : c - b.s2;
+        if (b.zero_polar_flux)
+            for (int v = 0; v < NV; ++v) F2[v * nc + c] = 0.;
+        if (b.fix_polar_flux) {
+            F1[B2 * nc + cs] -= F1[B2 * nc + cg];
+            F1[B2 * nc + cg] = 0.;
+            if (nd > 2) {
+                F3[B2 * nc + cs] -= F3[B2 * nc + cg];
+                F3[B2 * nc + cg] = 0.;
+            }
+            F2[B1 * nc + c] = 0.;
+            F2[B3 * nc + c] = 0.;
+        }
+        return;
+    }
+    const unsigned t2 = t - npolar;
+    if (!b.periodic[0] && t2 < 2u * nj * nk) {
+        const int side = t2 / (nj * nk), rem = t2 - side * (nj * nk);
+        const int k = k0 + rem / nj, j = j0 + rem % nj;
+        if (!b.check_inflow[side]) return;
+        const int i = side == 0 ? b.is : b.ie + 1;
+        const long c = (long)k * b.s3 + (long)j * b.s2 + i;
+        const double f = x1f[t2];
+        const bool pass = side == 0 ? (f < 0.) : (f > 0.);
+        if (!pass) F1[RHO * nc + c] = 0.;
+    }
+}
+
+// The clamped X1 mass fluxes of fix_flux_kernel's X1 rows (same enumeration).
+__global__ void __launch_bounds__(ABS) save_x1_flux_kernel(BlockDims b, const double* __restrict__ F,
+                                                           double* __restrict__ x1f) {
+    const int nd = b.ndim;
+    const int nk = (nd > 2) ? b.nx3 + 2 : 1, nj = (nd > 1) ? b.nx2 + 2 : 1;
+    const int k0 = (nd > 2) ? b.ks - 1 : b.ks, j0 = (nd > 1) ? b.js - 1 : b.js;
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 2u * nj * nk) return;
+    const int side = t / (nj * nk), rem = t - side * (nj * nk);
+    const int k = k0 + rem / nj, j = j0 + rem % nj;
+    const int i = side == 0 ? b.is : b.ie + 1;
+    x1f[t] = F[RHO * b.nc + (long)k * b.s3 + (long)j * b.s2 + i];
+}
+
+// Adjoint seed of a face-flux objective: lF_dir[var] += w on the faces of
+// index `face` along dir, over the interior transverse range.
+__global__ void __launch_bounds__(ABS) adj_seed_kernel(BlockDims b, int dir, int var, int face, double w,
+                                                       double* __restrict__ lF) {
+    const int na = (dir == 1) ? (b.ndim > 1 ? b.nx2 : 1) : b.nx1;
+    const int nb = (dir == 3) ? (b.ndim > 1 ? b.nx2 : 1) : (b.ndim > 2 ? b.nx3 : 1);
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= na * nb) return;
+    const int ta = t % na, tb = t / na;
+    int i, j, k;
+    if (dir == 1) { i = face; j = b.js + ta; k = b.ks + tb; }
+    else if (dir == 2) { i = b.is + ta; j = face; k = b.ks + tb; }
+    else { i = b.is + ta; k = face; j = b.js + tb; }
+    lF[((long)(dir - 1) * NV + var) * b.nc + (long)k * b.s3 + (long)j * b.s2 + i] += w;
 }
 
 // Transpose of the periodic ghost fill: each interior zone collects the
@@ -392,10 +737,12 @@ __global__ void __launch_bounds__(ABS) adj_clear_ghosts_kernel(BlockDims b, doub
     for (int v = 0; v < nvar; ++v) A[v * b.nc + t] = 0.;
 }
 
-// Count of zones with a non-zero inversion or floor flag (the adjoint assumes
-// an unfloored, converged recovery everywhere).
+// Count of zones with a non-zero inversion flag, or floor flag where fflag is
+// given (the adjoint covers converged recoveries, and floors only where the
+// stage's U* is kept).
 __global__ void __launch_bounds__(ABS) adj_count_flags_kernel(BlockDims b, const int* __restrict__ pflag,
                                                               const int* __restrict__ fflag, int* __restrict__ err) {
+    // fflag null: floored zones are supported and only failed inversions count
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long n = (long)b.nx1 * (b.ndim > 1 ? b.nx2 : 1) * (b.ndim > 2 ? b.nx3 : 1);
     int bad = 0;
@@ -403,9 +750,73 @@ __global__ void __launch_bounds__(ABS) adj_count_flags_kernel(BlockDims b, const
         int k, j, i;
         interior_zone(b, t, k, j, i);
         const long x = (long)k * b.s3 + (long)j * b.s2 + i;
-        bad = (pflag[x] != 0) || (fflag[x] != 0);
+        bad = (pflag[x] != 0) || (fflag && fflag[x] != 0);
     }
     if (bad) atomicAdd(&err[0], 1);
+}
+
+// lP += (dS/dP)^T lS at each interior zone (geometric source; lS = lam dudt).
+__global__ void __launch_bounds__(ABS)
+adj_geo_kernel(BlockDims b, StepConsts c, GeomDev gd, const double* __restrict__ P, const double* __restrict__ ld,
+               double* __restrict__ lP) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long n = (long)b.nx1 * (b.ndim > 1 ? b.nx2 : 1) * (b.ndim > 2 ? b.nx3 : 1);
+    if (t >= n) return;
+    int k, j, i;
+    interior_zone(b, t, k, j, i);
+    const long x = (long)k * b.s3 + (long)j * b.s2 + i;
+    double Pz[NV], out[NV] = {0., 0., 0., 0., 0., 0., 0., 0.}, lS[4];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) Pz[v] = P[v * b.nc + x];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) lS[v] = ld[(UU + v) * b.nc + x];
+    const long q = (long)j * gd.gn1 + i;
+    CachedGeom<double> g(gd.g[GLOC_CENTER], gd.np, q);
+    geo_source_vjp(g, gd, q, Pz, c.gam, lS, out);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) lP[v * b.nc + x] += out[v];
+}
+
+// lPin = (dfloors/dP at Pin)^T lPout at each interior zone (B passes through).
+template <int GEO, int FRAME>
+__global__ void __launch_bounds__(ABS)
+adj_floors_kernel(BlockDims b, StepConsts c, GeomDev gd, const double* __restrict__ Pin,
+                  const double* __restrict__ lPout, double* __restrict__ lPin) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long n = (long)b.nx1 * (b.ndim > 1 ? b.nx2 : 1) * (b.ndim > 2 ? b.nx3 : 1);
+    if (t >= n) return;
+    int k, j, i;
+    interior_zone(b, t, k, j, i);
+    const long x = (long)k * b.s3 + (long)j * b.s2 + i;
+    double Pz[NV], lo[NV], li[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) { Pz[v] = Pin[v * b.nc + x]; lo[v] = lPout[v * b.nc + x]; }
+    if constexpr (GEO) {
+        const long q = (long)j * gd.gn1 + i;
+        CachedGeom<double> g(gd.g[GLOC_CENTER], gd.np, q);
+        floors_vjp<FRAME>(g, Pz, c.gam, c.floors, gd.flr[q], gd.flr[gd.np + q], lo, li);
+    } else {
+        MinkowskiGeom<double> g;
+        floors_vjp<FRAME>(g, Pz, c.gam, c.floors, c.floors.rho_min_const, c.floors.u_min_const, lo, li);
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) lPin[v * b.nc + x] = li[v];
+}
+
+// lP <- (dP_fixed/dP)^T lP for the fluid primitives of the fixup alone (task-level
+// check of adj_fixup_kernel): failed zones lose their fluid adjoint to lPfx's
+// neighbours, the others add lPfx.
+__global__ void __launch_bounds__(ABS)
+adj_fixup_apply_kernel(BlockDims b, const int* __restrict__ pf, const double* __restrict__ lPfx, double* __restrict__ lP) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long n = (long)b.nx1 * (b.ndim > 1 ? b.nx2 : 1) * (b.ndim > 2 ? b.nx3 : 1);
+    if (t >= n) return;
+    int k, j, i;
+    interior_zone(b, t, k, j, i);
+    const long x = (long)k * b.s3 + (long)j * b.s2 + i;
+    const bool failed = pf[x] > 0;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) lP[v * b.nc + x] = failed ? 0. : lP[v * b.nc + x] + lPfx[v * b.nc + x];
 }
 
 long ninterior_l(const BlockDims& b) {
@@ -429,15 +840,17 @@ hipError_t launch_adj_flux_t(const BlockDims& b, const StepConsts& c, const Geom
                                dim3(64, 4), 0, s, b, P, lamL, lamR, lamP, jlo, nj, klo, nk);
         } else if (dir == 2) {
             const int ncol = (b.nx1 + 2) * nk;
-            hipLaunchKernelGGL((adj_face_xt_kernel<2, GEO>), dim3((unsigned)((ncol + 63) / 64), (unsigned)((b.nx2 + 1 + 6) / 7)),
-                               dim3(64, 8), 0, s, b, c, gd, P, lFd, lamL, lamR, klo, nk, b.js, b.je + 1);
+            constexpr int TX = adj_xt_tx<2, GEO>();
+            hipLaunchKernelGGL((adj_face_xt_kernel<2, GEO>), dim3((unsigned)((ncol + TX - 1) / TX), (unsigned)((b.nx2 + 1 + 6) / 7)),
+                               dim3(TX, 8), 0, s, b, c, gd, P, lFd, lamL, lamR, klo, nk, b.js, b.je + 1);
             const int nout = b.nx2 + 6;
             hipLaunchKernelGGL((adj_recon_kernel<2>), dim3((unsigned)((ncol + 63) / 64), (unsigned)((nout + 11) / 12)),
                                dim3(64, 16), 0, s, b, P, lamL, lamR, lamP, jlo, nj, klo, nk);
         } else {
             const int ncol = (b.nx1 + 2) * nj;
-            hipLaunchKernelGGL((adj_face_xt_kernel<3, GEO>), dim3((unsigned)((ncol + 63) / 64), (unsigned)((b.nx3 + 1 + 6) / 7)),
-                               dim3(64, 8), 0, s, b, c, gd, P, lFd, lamL, lamR, jlo, nj, b.ks, b.ke + 1);
+            constexpr int TX = adj_xt_tx<3, GEO>();
+            hipLaunchKernelGGL((adj_face_xt_kernel<3, GEO>), dim3((unsigned)((ncol + TX - 1) / TX), (unsigned)((b.nx3 + 1 + 6) / 7)),
+                               dim3(TX, 8), 0, s, b, c, gd, P, lFd, lamL, lamR, jlo, nj, b.ks, b.ke + 1);
             const int nout = b.nx3 + 6;
             hipLaunchKernelGGL((adj_recon_kernel<3>), dim3((unsigned)((ncol + 63) / 64), (unsigned)((nout + 11) / 12)),
                                dim3(64, 16), 0, s, b, P, lamL, lamR, lamP, jlo, nj, klo, nk);
@@ -477,14 +890,72 @@ hipError_t launch_adj_divct(const BlockDims& b, const double* ld, double* lF, do
 
 hipError_t launch_adj_stage(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const double* Pout,
                             const double* lPout, const double* lUout, double g0, double g1, double beta_dt, bool same,
-                            bool accumulate, double* lUsub, double* lUfull, double* ld, int* err, hipStream_t s) {
+                            bool accumulate, double* lUsub, double* lUfull, double* ld, int* err, const double* Ustar,
+                            const double* Psub, const int* ffs, double* lPgeo, const int* pfs, const double* lPfx,
+                            hipStream_t s) {
     const dim3 g = agrid(ninterior_l(b));
-    if (b.geo)
-        hipLaunchKernelGGL(adj_stage_kernel<1>, g, dim3(ABS), 0, s, b, c, gd, Pout, lPout, lUout, g0, g1, beta_dt,
-                           (int)same, (int)accumulate, lUsub, lUfull, ld, err);
-    else
-        hipLaunchKernelGGL(adj_stage_kernel<0>, g, dim3(ABS), 0, s, b, c, gd, Pout, lPout, lUout, g0, g1, beta_dt,
-                           (int)same, (int)accumulate, lUsub, lUfull, ld, err);
+    const int frame = ffs ? c.floors.frame : -1;
+    if (frame == 1) return hipErrorInvalidValue;
+#define KA_STAGE(GEO, FR)                                                                                      \
+    hipLaunchKernelGGL((adj_stage_kernel<GEO, FR>), g, dim3(ABS), 0, s, b, c, gd, Pout, lPout, lUout, g0, g1,   \
+                       beta_dt, (int)same, (int)accumulate, lUsub, lUfull, ld, err, Ustar, Psub, ffs, lPgeo, pfs, lPfx)
+    if (b.geo) {
+        if (frame == 0) KA_STAGE(1, 0);
+        else if (frame == 2) KA_STAGE(1, 2);
+        else KA_STAGE(1, -1);
+    } else {
+        if (frame == 0) KA_STAGE(0, 0);
+        else if (frame == 2) KA_STAGE(0, 2);
+        else KA_STAGE(0, -1);
+    }
+#undef KA_STAGE
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_fixup(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const double* Pout,
+                            const double* lPout, const double* lUout, const int* pfs, double* lPfx, hipStream_t s) {
+    if (b.geo) hipLaunchKernelGGL(adj_fixup_kernel<1>, agrid(b.nc), dim3(ABS), 0, s, b, c, gd, Pout, lPout, lUout, pfs, lPfx);
+    else hipLaunchKernelGGL(adj_fixup_kernel<0>, agrid(b.nc), dim3(ABS), 0, s, b, c, gd, Pout, lPout, lUout, pfs, lPfx);
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_fixup_apply(const BlockDims& b, const int* pfs, const double* lPfx, double* lP, hipStream_t s) {
+    hipLaunchKernelGGL(adj_fixup_apply_kernel, agrid(ninterior_l(b)), dim3(ABS), 0, s, b, pfs, lPfx, lP);
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_bounds(const BlockDims& b, const GeomDev& gd, const double* P, double* A, hipStream_t s) {
+    if (b.periodic[0] && (b.ndim < 2 || b.periodic[1])) return hipSuccess;
+    if (b.ng > 4) return hipErrorInvalidValue;
+    if (b.geo) hipLaunchKernelGGL(adj_bounds_kernel<1>, agrid(b.nc), dim3(ABS), 0, s, b, gd, P, A);
+    else hipLaunchKernelGGL(adj_bounds_kernel<0>, agrid(b.nc), dim3(ABS), 0, s, b, gd, P, A);
+    return hipGetLastError();
+}
+
+long x1_flux_rows(const BlockDims& b) {
+    if (b.periodic[0]) return 0;
+    return 2L * ((b.ndim > 1) ? b.nx2 + 2 : 1) * ((b.ndim > 2) ? b.nx3 + 2 : 1);
+}
+
+hipError_t launch_save_x1_flux(const BlockDims& b, const double* F, double* x1f, hipStream_t s) {
+    const long n = x1_flux_rows(b);
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(save_x1_flux_kernel, agrid(n), dim3(ABS), 0, s, b, F, x1f);
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_fix_flux(const BlockDims& b, double* lF, const double* x1f, hipStream_t s) {
+    const long npolar = (b.ndim > 1 && !b.periodic[1]) ? 2L * (b.nx1 + 2) * ((b.ndim > 2) ? b.nx3 + 2 : 1) : 0;
+    const long n = npolar + x1_flux_rows(b);
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(adj_fix_flux_kernel, agrid(n), dim3(ABS), 0, s, b, lF, x1f);
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_seed(const BlockDims& b, int dir, int var, int face, double w, double* lF, hipStream_t s) {
+    const int na = (dir == 1) ? (b.ndim > 1 ? b.nx2 : 1) : b.nx1;
+    const int nb = (dir == 3) ? (b.ndim > 1 ? b.nx2 : 1) : (b.ndim > 2 ? b.nx3 : 1);
+    hipLaunchKernelGGL(adj_seed_kernel, agrid((long)na * nb), dim3(ABS), 0, s, b, dir, var, face, w, lF);
     return hipGetLastError();
 }
 
@@ -499,6 +970,28 @@ hipError_t launch_adj_ptou(const BlockDims& b, const StepConsts& c, const GeomDe
 hipError_t launch_adj_fold(const BlockDims& b, double* A, int nvar, hipStream_t s) {
     hipLaunchKernelGGL(adj_fold_kernel, agrid(ninterior_l(b)), dim3(ABS), 0, s, b, A, nvar);
     hipLaunchKernelGGL(adj_clear_ghosts_kernel, agrid(b.nc), dim3(ABS), 0, s, b, A, nvar);
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_geo(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const double* P, const double* ld,
+                          double* lP, hipStream_t s) {
+    if (!b.geo) return hipSuccess;
+    hipLaunchKernelGGL(adj_geo_kernel, agrid(ninterior_l(b)), dim3(ABS), 0, s, b, c, gd, P, ld, lP);
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_floors(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const double* Pin,
+                             const double* lPout, double* lPin, hipStream_t s) {
+    const dim3 g = agrid(ninterior_l(b));
+    const int fr = c.floors.frame;
+    if (fr == 1) return hipErrorInvalidValue;
+    if (b.geo) {
+        if (fr == 0) hipLaunchKernelGGL((adj_floors_kernel<1, 0>), g, dim3(ABS), 0, s, b, c, gd, Pin, lPout, lPin);
+        else hipLaunchKernelGGL((adj_floors_kernel<1, 2>), g, dim3(ABS), 0, s, b, c, gd, Pin, lPout, lPin);
+    } else {
+        if (fr == 0) hipLaunchKernelGGL((adj_floors_kernel<0, 0>), g, dim3(ABS), 0, s, b, c, gd, Pin, lPout, lPin);
+        else hipLaunchKernelGGL((adj_floors_kernel<0, 2>), g, dim3(ABS), 0, s, b, c, gd, Pin, lPout, lPin);
+    }
     return hipGetLastError();
 }
 

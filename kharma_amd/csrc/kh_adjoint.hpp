@@ -99,21 +99,93 @@ __device__ __forceinline__ void side_forward(const G& g, const double P[NV], dou
 // lU, lF, lcmax, lcmin are the adjoints of the outputs; lP receives the
 // adjoint of P.  DIR = 0 skips F and the speeds (the conserved variables
 // alone, for the inverter and P->U adjoints).
-template <int DIR, class G>
-__device__ __forceinline__ void side_reverse(const G& g, const double P[NV], double gam, const SideState& S,
-                                             const double lU[NV], const double lF[NV], double lcmax, double lcmin,
-                                             double lP[NV]) {
+// Adjoint accumulators of one point's forward intermediates (SideState).
+struct SideAdj {
+    double ucon[4] = {0., 0., 0., 0.}, ucov[4] = {0., 0., 0., 0.};
+    double bcon[4] = {0., 0., 0., 0.}, bcov[4] = {0., 0., 0., 0.};
+    double rho = 0., u = 0., eta = 0., ptot = 0., bsq = 0.;
+    double B[3] = {0., 0., 0.};
+};
+
+// Shared tail of the reverse sweeps of one point: from the adjoints of the
+// four-vectors, of rho, u, B^i and of the tensor invariants eta, ptot, bsq (a)
+// to the adjoint of P, through calc_4vecs (grmhd_functions.hpp:225-246).
+template <class G>
+__device__ __forceinline__ void side_tail(const G& g, const double P[NV], double gam, const SideState& S, SideAdj& a,
+                                          double lP[NV]) {
     const double ut1 = P[U1], ut2 = P[U2], ut3 = P[U3];
     const double* ucon = S.D.ucon;
     const double* ucov = S.D.ucov;
     const double* bcon = S.D.bcon;
     const double* bcov = S.D.bcov;
-    const double gamma = S.gamma, alpha = S.alpha, iu0 = S.iu0, gdet = S.gdet, bsq = S.bsq, eta = S.eta;
+    const double gamma = S.gamma, alpha = S.alpha, iu0 = S.iu0;
+    // eta = pgas + rho + u + bsq, ptot = pgas + bsq / 2, pgas = (gam - 1) u
+    a.rho += a.eta;
+    a.u += a.eta + (gam - 1.) * (a.eta + a.ptot);
+    a.bsq += a.eta + 0.5 * a.ptot;
+    // bsq = b^mu b_mu
+#pragma unroll
+    for (int mu = 0; mu < 4; ++mu) {
+        a.bcon[mu] += a.bsq * bcov[mu];
+        a.bcov[mu] += a.bsq * bcon[mu];
+    }
+    // b_mu = g_{mu nu} b^nu (g symmetric)
+    {
+        double t[4];
+        g.lower(a.bcov, t);
+#pragma unroll
+        for (int mu = 0; mu < 4; ++mu) a.bcon[mu] += t[mu];
+    }
+    // b^i = (B^i + b^0 u^i) / u^0
+    double l_iu0 = 0.;
+#pragma unroll
+    for (int i = 1; i <= 3; ++i) {
+        a.B[i - 1] += a.bcon[i] * iu0;
+        a.bcon[0] += a.bcon[i] * ucon[i] * iu0;
+        a.ucon[i] += a.bcon[i] * bcon[0] * iu0;
+        l_iu0 += a.bcon[i] * (P[B1 + i - 1] + bcon[0] * ucon[i]);
+    }
+    a.ucon[0] -= l_iu0 * iu0 * iu0;
+    // b^0 = B^i u_i
+#pragma unroll
+    for (int i = 1; i <= 3; ++i) {
+        a.B[i - 1] += a.bcon[0] * ucov[i];
+        a.ucov[i] += a.bcon[0] * P[B1 + i - 1];
+    }
+    // u_mu = g_{mu nu} u^nu
+    {
+        double t[4];
+        g.lower(a.ucov, t);
+#pragma unroll
+        for (int mu = 0; mu < 4; ++mu) a.ucon[mu] += t[mu];
+    }
+    // u^0 = gamma / alpha, u^i = ut^i - gamma alpha g^{0i}
+    double l_gamma;
+    if constexpr (G::flat) l_gamma = a.ucon[0];
+    else l_gamma = a.ucon[0] * rcp(alpha) - alpha * (a.ucon[1] * g.gcon0(1) + a.ucon[2] * g.gcon0(2) + a.ucon[3] * g.gcon0(3));
+    // gamma = sqrt(1 + g_ij ut^i ut^j)
+    const double l_q = l_gamma * rcp(gamma);   // 2 * d(gamma)/d(qsq) = 1 / gamma
+    double lut[3];
+#pragma unroll
+    for (int i = 1; i <= 3; ++i)
+        lut[i - 1] = a.ucon[i] + l_q * (g.gcov(i, 1) * ut1 + g.gcov(i, 2) * ut2 + g.gcov(i, 3) * ut3);
+    lP[RHO] = a.rho;
+    lP[UU] = a.u;
+    lP[U1] = lut[0]; lP[U2] = lut[1]; lP[U3] = lut[2];
+    lP[B1] = a.B[0]; lP[B2] = a.B[1]; lP[B3] = a.B[2];
+}
+
+template <int DIR, class G>
+__device__ __forceinline__ void side_reverse(const G& g, const double P[NV], double gam, const SideState& S,
+                                             const double lU[NV], const double lF[NV], double lcmax, double lcmin,
+                                             double lP[NV]) {
+    const double* ucon = S.D.ucon;
+    const double* ucov = S.D.ucov;
+    const double* bcon = S.D.bcon;
+    const double* bcov = S.D.bcov;
+    const double gdet = S.gdet, bsq = S.bsq, eta = S.eta;
     // ---- reverse
-    double l_ucon[4] = {0., 0., 0., 0.}, l_ucov[4] = {0., 0., 0., 0.};
-    double l_bcon[4] = {0., 0., 0., 0.}, l_bcov[4] = {0., 0., 0., 0.};
-    double l_rho = 0., l_u = 0., l_eta = 0., l_ptot = 0., l_bsq = 0.;
-    double l_B[3] = {0., 0., 0.};
+    SideAdj acc;
 
     if constexpr (DIR > 0) {
         // vchar (flux_functions.hpp:368-439)
@@ -147,118 +219,66 @@ __device__ __forceinline__ void side_reverse(const G& g, const double P[NV], dou
         l_A += -4. * C * l_Dd;
         const double l_C = -4. * A * l_Dd;
         const double om = 1. - cms2;
-        l_ucon[DIR] += 2. * om * (l_B2 * Bu + l_C * Au);
-        l_ucon[0] += 2. * om * (l_A * Bu + l_B2 * Au);
+        acc.ucon[DIR] += 2. * om * (l_B2 * Bu + l_C * Au);
+        acc.ucon[0] += 2. * om * (l_A * Bu + l_B2 * Au);
         const double l_cms2 = -l_A * (Bsq + Bu * Bu) - 2. * l_B2 * (AB + Au * Bu) - l_C * (Asq + Au * Au);
         const double l_cs2 = l_cms2 * (1. - va2), l_va2 = l_cms2 * (1. - cs2);
-        l_bsq += l_va2 * ef * ibef * ibef;
+        acc.bsq += l_va2 * ef * ibef * ibef;
         double l_ef = -l_va2 * bsq * ibef * ibef;
-        l_u += l_cs2 * gam * (gam - 1.) * ief;
+        acc.u += l_cs2 * gam * (gam - 1.) * ief;
         l_ef += -l_cs2 * cs2 * ief;
-        l_rho += l_ef;
-        l_u += gam * l_ef;
+        acc.rho += l_ef;
+        acc.u += gam * l_ef;
     }
 
     // prim_to_u / prim_to_flux (flux_functions.hpp:116-178): U[UU] and F[UU] carry
     // + U[RHO] / + F[RHO].
     {
         const double lU0 = lU[RHO] + lU[UU];
-        l_rho += lU0 * ucon[0] * gdet;
-        l_ucon[0] += lU0 * P[RHO] * gdet;
+        acc.rho += lU0 * ucon[0] * gdet;
+        acc.ucon[0] += lU0 * P[RHO] * gdet;
         if constexpr (DIR > 0) {
             const double lF0 = lF[RHO] + lF[UU];
-            l_rho += lF0 * ucon[DIR] * gdet;
-            l_ucon[DIR] += lF0 * P[RHO] * gdet;
+            acc.rho += lF0 * ucon[DIR] * gdet;
+            acc.ucon[DIR] += lF0 * P[RHO] * gdet;
         }
 #pragma unroll
         for (int mu = 0; mu < 4; ++mu) {
             // U[UU+mu] = (eta u^0 u_mu + delta^0_mu ptot - b^0 b_mu) gdet
             const double a = lU[UU + mu] * gdet;
-            l_eta += a * ucon[0] * ucov[mu];
-            l_ucon[0] += a * eta * ucov[mu];
-            l_ucov[mu] += a * eta * ucon[0];
-            if (mu == 0) l_ptot += a;
-            l_bcon[0] -= a * bcov[mu];
-            l_bcov[mu] -= a * bcon[0];
+            acc.eta += a * ucon[0] * ucov[mu];
+            acc.ucon[0] += a * eta * ucov[mu];
+            acc.ucov[mu] += a * eta * ucon[0];
+            if (mu == 0) acc.ptot += a;
+            acc.bcon[0] -= a * bcov[mu];
+            acc.bcov[mu] -= a * bcon[0];
             if constexpr (DIR > 0) {
                 // F[UU+mu] = (eta u^d u_mu + delta^d_mu ptot - b^d b_mu) gdet
                 const double f = lF[UU + mu] * gdet;
-                l_eta += f * ucon[DIR] * ucov[mu];
-                l_ucon[DIR] += f * eta * ucov[mu];
-                l_ucov[mu] += f * eta * ucon[DIR];
-                if (mu == DIR) l_ptot += f;
-                l_bcon[DIR] -= f * bcov[mu];
-                l_bcov[mu] -= f * bcon[DIR];
+                acc.eta += f * ucon[DIR] * ucov[mu];
+                acc.ucon[DIR] += f * eta * ucov[mu];
+                acc.ucov[mu] += f * eta * ucon[DIR];
+                if (mu == DIR) acc.ptot += f;
+                acc.bcon[DIR] -= f * bcov[mu];
+                acc.bcov[mu] -= f * bcon[DIR];
             }
         }
 #pragma unroll
         for (int i = 1; i <= 3; ++i) {
-            l_B[i - 1] += lU[B1 + i - 1] * gdet;
+            acc.B[i - 1] += lU[B1 + i - 1] * gdet;
             if constexpr (DIR > 0) {
                 // F[Bi] = (b^i u^d - b^d u^i) gdet
                 const double f = lF[B1 + i - 1] * gdet;
-                l_bcon[i] += f * ucon[DIR];
-                l_ucon[DIR] += f * bcon[i];
-                l_bcon[DIR] -= f * ucon[i];
-                l_ucon[i] -= f * bcon[DIR];
+                acc.bcon[i] += f * ucon[DIR];
+                acc.ucon[DIR] += f * bcon[i];
+                acc.bcon[DIR] -= f * ucon[i];
+                acc.ucon[i] -= f * bcon[DIR];
             }
         }
     }
-    // eta = pgas + rho + u + bsq, ptot = pgas + bsq / 2, pgas = (gam - 1) u
-    l_rho += l_eta;
-    l_u += l_eta + (gam - 1.) * (l_eta + l_ptot);
-    l_bsq += l_eta + 0.5 * l_ptot;
-    // bsq = b^mu b_mu
-#pragma unroll
-    for (int mu = 0; mu < 4; ++mu) {
-        l_bcon[mu] += l_bsq * bcov[mu];
-        l_bcov[mu] += l_bsq * bcon[mu];
-    }
-    // b_mu = g_{mu nu} b^nu (g symmetric)
-    {
-        double t[4];
-        g.lower(l_bcov, t);
-#pragma unroll
-        for (int mu = 0; mu < 4; ++mu) l_bcon[mu] += t[mu];
-    }
-    // b^i = (B^i + b^0 u^i) / u^0
-    double l_iu0 = 0.;
-#pragma unroll
-    for (int i = 1; i <= 3; ++i) {
-        l_B[i - 1] += l_bcon[i] * iu0;
-        l_bcon[0] += l_bcon[i] * ucon[i] * iu0;
-        l_ucon[i] += l_bcon[i] * bcon[0] * iu0;
-        l_iu0 += l_bcon[i] * (P[B1 + i - 1] + bcon[0] * ucon[i]);
-    }
-    l_ucon[0] -= l_iu0 * iu0 * iu0;
-    // b^0 = B^i u_i
-#pragma unroll
-    for (int i = 1; i <= 3; ++i) {
-        l_B[i - 1] += l_bcon[0] * ucov[i];
-        l_ucov[i] += l_bcon[0] * P[B1 + i - 1];
-    }
-    // u_mu = g_{mu nu} u^nu
-    {
-        double t[4];
-        g.lower(l_ucov, t);
-#pragma unroll
-        for (int mu = 0; mu < 4; ++mu) l_ucon[mu] += t[mu];
-    }
-    // u^0 = gamma / alpha, u^i = ut^i - gamma alpha g^{0i}
-    double l_gamma;
-    if constexpr (G::flat) l_gamma = l_ucon[0];
-    else l_gamma = l_ucon[0] * rcp(alpha) - alpha * (l_ucon[1] * g.gcon0(1) + l_ucon[2] * g.gcon0(2) + l_ucon[3] * g.gcon0(3));
-    // gamma = sqrt(1 + g_ij ut^i ut^j)
-    const double l_q = l_gamma * rcp(gamma);   // 2 * d(gamma)/d(qsq) = 1 / gamma
-    double lut[3];
-#pragma unroll
-    for (int i = 1; i <= 3; ++i)
-        lut[i - 1] = l_ucon[i] + l_q * (g.gcov(i, 1) * ut1 + g.gcov(i, 2) * ut2 + g.gcov(i, 3) * ut3);
-    lP[RHO] = l_rho;
-    lP[UU] = l_u;
-    lP[U1] = lut[0]; lP[U2] = lut[1]; lP[U3] = lut[2];
-    lP[B1] = l_B[0]; lP[B2] = l_B[1]; lP[B3] = l_B[2];
+    side_tail(g, P, gam, S, acc, lP);
 }
+
 template <int DIR, class G>
 __device__ __forceinline__ void side_vjp(const G& g, const double P[NV], double gam, const double lU[NV],
                                          const double lF[NV], double lcmax, double lcmin, double lP[NV]) {
@@ -393,6 +413,139 @@ __device__ __forceinline__ bool inverter_vjp(const G& g, const double P[NV], dou
     for (int j = 0; j < 3; ++j) lU[5 + j] = (lP[5 + j] - (N[j][0] * x[0] + N[j][1] * x[1] + N[j][2] * x[2] +
                                                            N[j][3] * x[3] + N[j][4] * x[4])) * igd;
     return true;
+}
+
+// ------------------------------------------------------ dual numbers
+// Forward-mode dual numbers v + d e (e^2 = 0): one directional derivative
+// carried through the templated floor and inflow functions of kh_device.hpp,
+// whose Jacobians (a few branches, no fixed structure) are taken column by
+// column this way.  Values follow the forward arithmetic operation for
+// operation (IEEE quotients where the forward code divides), so the branches
+// taken are the forward ones.  They live in their own namespace and are found
+// by argument-dependent lookup, so double-typed calls never see them.
+namespace ad {
+struct Dual {
+    double v, d;
+    __host__ __device__ constexpr Dual(double x = 0., double dx = 0.) : v(x), d(dx) {}
+};
+__device__ __forceinline__ Dual operator+(const Dual& a, const Dual& b) { return Dual(a.v + b.v, a.d + b.d); }
+__device__ __forceinline__ Dual operator-(const Dual& a, const Dual& b) { return Dual(a.v - b.v, a.d - b.d); }
+__device__ __forceinline__ Dual operator-(const Dual& a) { return Dual(-a.v, -a.d); }
+__device__ __forceinline__ Dual operator*(const Dual& a, const Dual& b) { return Dual(a.v * b.v, a.d * b.v + a.v * b.d); }
+__device__ __forceinline__ Dual operator/(const Dual& a, const Dual& b) {
+    const double q = a.v / b.v;
+    return Dual(q, (a.d - q * b.d) / b.v);
+}
+__device__ __forceinline__ bool operator<(const Dual& a, const Dual& b) { return a.v < b.v; }
+__device__ __forceinline__ bool operator>(const Dual& a, const Dual& b) { return a.v > b.v; }
+__device__ __forceinline__ bool operator<=(const Dual& a, const Dual& b) { return a.v <= b.v; }
+__device__ __forceinline__ bool operator>=(const Dual& a, const Dual& b) { return a.v >= b.v; }
+__device__ __forceinline__ Dual sqrt(const Dual& a) {
+    const double r = ::sqrt(a.v);
+    return Dual(r, 0.5 * a.d / r);
+}
+__device__ __forceinline__ Dual rcp(const Dual& a) {
+    const double r = kh::rcp(a.v);
+    return Dual(r, -a.d * r * r);
+}
+__device__ __forceinline__ Dual pow(const Dual& a, double e) {
+    const double r = ::pow(a.v, e);
+    return Dual(r, e * r / a.v * a.d);
+}
+__device__ __forceinline__ double val(const Dual& a) { return a.v; }
+}  // namespace ad
+
+// -------------------------------------------------- geometric source
+// Reverse of add_geo_source (Flux::AddGeoSource, flux/flux.cpp:395-444) at one
+// zone: S_lam = sum_{mu,nu} T^mu_nu (gdet Gamma^nu_{lam mu}) for lam = 0..3
+// (dudt[UU..U3]), T^mu_nu = eta u^mu u_nu + ptot delta^mu_nu - b^mu b_nu.
+// lP += (dS/dP)^T lS.
+template <class G>
+__device__ __forceinline__ void geo_source_vjp(const G& g, const GeomDev& gd, long q, const double P[NV], double gam,
+                                               const double lS[4], double lP[NV]) {
+    SideState S;
+    side_forward(g, P, gam, S);
+    SideAdj a;
+    const double* ucon = S.D.ucon;
+    const double* ucov = S.D.ucov;
+    const double* bcon = S.D.bcon;
+    const double* bcov = S.D.bcov;
+#pragma unroll
+    for (int mu = 0; mu < 4; ++mu) {
+        double cm[16];
+#pragma unroll
+        for (int nl = 0; nl < 16; ++nl) cm[nl] = gd.conn[(long)(nl * 4 + mu) * gd.np + q];
+#pragma unroll
+        for (int nu = 0; nu < 4; ++nu) {
+            const double lT = lS[0] * cm[nu * 4] + lS[1] * cm[nu * 4 + 1] + lS[2] * cm[nu * 4 + 2] + lS[3] * cm[nu * 4 + 3];
+            a.eta += lT * ucon[mu] * ucov[nu];
+            a.ucon[mu] += lT * S.eta * ucov[nu];
+            a.ucov[nu] += lT * S.eta * ucon[mu];
+            if (mu == nu) a.ptot += lT;
+            a.bcon[mu] -= lT * bcov[nu];
+            a.bcov[nu] -= lT * bcon[mu];
+        }
+    }
+    double out[NV];
+    side_tail(g, P, gam, S, a, out);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) lP[v] += out[v];
+}
+
+// ------------------------------------------------------------- floors
+// Reverse of the floors of one zone (floors_zone: determine_floors, the frame's
+// floors, apply_ceilings; floors/floors_impl.hpp:45-128) at the inverted state
+// Pin: lPin = (dP_out / dPin)^T lPout.  The B components pass through.
+// FRAME 0 (fluid) or 2 (drift).  One dual-number pass per input component.
+template <int FRAME, class G>
+__device__ __forceinline__ void floors_vjp(const G& g, const double Pin[NV], double gam, const FloorParams& f,
+                                           double rflr, double uflr, const double lPout[NV], double lPin[NV]) {
+    using ad::Dual;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) lPin[v] = (v >= B1) ? lPout[v] : 0.;
+#pragma unroll 1
+    for (int k = 0; k < NV; ++k) {
+        Dual P[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) P[v] = Dual(Pin[v], v == k ? 1. : 0.);
+        FourVec<Dual> D;
+        calc_4vecs<Dual>(g, P, D);
+        Dual rf, uf;
+        const int ff = determine_floors(g, P, dot4(D.bcon, D.bcov), gam, f, rflr, uflr, rf, uf);
+        if (ff) {
+            if constexpr (FRAME == 0) {
+                P[RHO] = P[RHO] + smax(Dual(0.), rf - P[RHO]);
+                P[UU] = P[UU] + smax(Dual(0.), uf - P[UU]);
+            } else {
+                apply_floors_drift(g, P, gam, rf, uf);
+            }
+            apply_ceilings(g, P, gam, f);
+        }
+        double dk = 0.;
+#pragma unroll
+        for (int m = 0; m < 5; ++m) dk += lPout[m] * P[m].d;
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+            if (v == k) lPin[v] += dk;
+    }
+}
+
+// ------------------------------------------------------- inflow check
+// Reverse of check_inflow (boundaries/boundaries.hpp:101-136) at the state Pin:
+// l[U1..U3] <- (dP_out / dPin)^T l[U1..U3] (the other components pass through).
+template <class G>
+__device__ __forceinline__ void inflow_vjp(const G& g, const double Pin[NV], bool inner, double l[NV]) {
+    using ad::Dual;
+    double out[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        Dual P[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) P[v] = Dual(Pin[v], v == U1 + k ? 1. : 0.);
+        check_inflow(g, P, inner);
+        out[k] = l[U1] * P[U1].d + l[U2] * P[U2].d + l[U3] * P[U3].d;
+    }
+    l[U1] = out[0]; l[U2] = out[1]; l[U3] = out[2];
 }
 
 }  // namespace kh

@@ -1,15 +1,18 @@
 // kh_adjoint_capi.cpp -- C ABI of the adjoint step (include/kharma_amd.h).
 //
 // kh_adjoint_step recomputes the forward step from its checkpointed starting
-// primitives on an internal driver (stage 1 -> P1, U1; stage 2 -> P2, U2), then
-// runs the stage transposes of kh_adjoint.hip in reverse order:
-//   stage 2: lU*_2 = (dU/dP)^-T lP_2;  lU_1 += g0 lU*_2, lU_0 += g1 lU*_2;
-//            lP_1 = (dF/dP at P_1)^T (ddudt/dF)^T (beta dt lU*_2)
-//   stage 1: lU*_1 = (dU/dP)^-T lP_1 + lU_1;  lU_0 += lU*_1;
-//            lP_0 = (dF/dP at P_0)^T (ddudt/dF)^T (beta dt lU*_1)
+// primitives on an internal driver (stage 1 -> P1, U1; stage 2 -> P2, U2),
+// keeping what the transposes need: each stage's U* (the update before
+// recovery) and floor flags on the curved-space path, and the clamped X1 mass
+// fluxes where X1 is physical.  Then it runs the stage transposes of
+// kh_adjoint.hip in reverse order:
+//   stage 2: lU*_2 = (dU/dP)^-T (dfloors/dP)^T (...) lP_2;  lU_1 += g0 lU*_2, lU_0 += g1 lU*_2;
+//            lP_1 = (dF/dP at P_1)^T (dfix/dF)^T (ddudt/dF)^T (beta dt lU*_2) + (dS/dP_1)^T beta dt lU*_2
+//   stage 1: likewise from lP_1 and lU_1 into lP_0, lU_0
 //   then     lP_0 += (dU/dP at P_0)^T lU_0        (U_0 = p_to_u(P_0))
-// with the periodic ghost images folded back onto their interior zones after
-// each flux transpose.
+// with the ghost zones' adjoints mapped back through the physical-boundary
+// fill and the periodic images after each flux transpose, and a face-flux
+// objective's seed added to the flux adjoints of each stage.
 #include "kharma_amd.h"
 
 #include <string>
@@ -29,6 +32,14 @@ struct kh_adjoint {
     double* lemf = nullptr;    // [3][nc] adjoint of the corner EMFs
     double* lLR = nullptr;     // [2][8][nc] adjoints of the reconstructed face values
     int* err = nullptr;        // [0] flagged zones, [1] singular Jacobians (device)
+    double* Ustar[2] = {nullptr, nullptr};   // U* of each stage (curved-space path)
+    int* ffs[2] = {nullptr, nullptr};        // floor flags of each stage (curved-space path with floors)
+    double* x1f[2] = {nullptr, nullptr};     // clamped X1 mass fluxes of each stage (physical X1)
+    int* pfs[2] = {nullptr, nullptr};        // inversion flags of each stage (with the fixup)
+    double* lPfx = nullptr;                  // [8][nc] fixup transpose
+    bool geo = false, physical = false, floors = false, fixup = false;
+    int obj_dir = 0, obj_var = 0, obj_face = 0;   // face-flux objective (kh_adjoint_set_face_flux_objective)
+    double obj_weight = 0.;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     float fwd_ms = 0.f, adj_ms = 0.f;
 };
@@ -40,6 +51,51 @@ namespace {
         hipError_t e_ = (call);                                                                        \
         if (e_ != hipSuccess) return api_fail(KH_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
     } while (0)
+
+// One forward stage on the driver's buffers, as kh::driver_stage runs it,
+// keeping U* (between the update's combine and recovery halves), the floor
+// flags and the clamped X1 mass fluxes of the stage.
+int recompute_stage(kh_adjoint* a, int stage) {
+    kh_driver* d = a->drv;
+    kh_block* b = d->blk;
+    const BlockDims& bd = b->dims;
+    const StepConsts& c = b->consts;
+    hipStream_t s = b->stream;
+    const double* Psub = (stage == 1) ? d->P0 : d->P1;
+    const double* Usub = (stage == 1) ? d->U0 : d->U1;
+    double* Pout = (stage == 1) ? d->P1 : d->P0;
+    double* Uout = (stage == 1) ? d->U1 : d->U0;
+    double* F = (double*)d->F;
+    double* ct = (double*)d->ctop;
+    for (int dir = 1; dir <= bd.ndim; ++dir)
+        KA_HIP(launch_flux_stage(bd, c, b->geo, dir, b->p.recon, b->p.precision, SPEEDS_NONE, Psub, nullptr,
+                                 F + (long)(dir - 1) * NV * bd.nc, ct + (long)(dir - 1) * bd.nc, s));
+    if (a->physical) {
+        KA_HIP(launch_fix_flux(bd, b->p.precision, F, s));
+        KA_HIP(launch_save_x1_flux(bd, F, a->x1f[stage - 1], s));
+    }
+    if (a->geo) {
+        KA_HIP(launch_update_parts(bd, c, b->geo, stage, 1, F, Usub, d->U0, Psub, Uout, Pout, d->pflag, d->fflag,
+                                   d->fail_list, d->ds, s));
+        KA_HIP(hipMemcpyAsync(a->Ustar[stage - 1], Uout, sizeof(double) * NV * bd.nc, hipMemcpyDeviceToDevice, s));
+        KA_HIP(launch_update_parts(bd, c, b->geo, stage, 2, F, Usub, d->U0, Psub, Uout, Pout, d->pflag, d->fflag,
+                                   d->fail_list, d->ds, s));
+        if (a->floors)
+            KA_HIP(hipMemcpyAsync(a->ffs[stage - 1], d->fflag, sizeof(int) * bd.nc, hipMemcpyDeviceToDevice, s));
+    } else {
+        KA_HIP(launch_update(bd, c, b->geo, b->p.precision, stage, false, F, ct, Usub, d->U0, Psub, Uout, Pout,
+                             nullptr, d->pflag, d->fflag, d->fail_list, d->ds, d->dt_part, s));
+    }
+    if (a->fixup) {
+        KA_HIP(hipMemcpyAsync(a->pfs[stage - 1], d->pflag, sizeof(int) * bd.nc, hipMemcpyDeviceToDevice, s));
+        KA_HIP(launch_fixup(bd, c, b->geo, stage, Uout, Pout, nullptr, d->pflag, d->fail_list, d->ds, s));
+        if (!a->floors) KA_HIP(launch_adj_count_flags(bd, d->fflag, nullptr, a->err, s));
+    } else {
+        KA_HIP(launch_adj_count_flags(bd, d->pflag, a->floors ? nullptr : d->fflag, a->err, s));
+    }
+    if (a->physical) KA_HIP(launch_bounds(bd, c, b->geo, Uout, Pout, nullptr, s));
+    return KH_OK;
+}
 
 // Transpose of one stage.  Pin, Pout: the stage's input and output primitives.
 // lPout holds lam(P_out) (interior); lPin receives lam(P_in) (interior, ghosts
@@ -53,11 +109,25 @@ int stage_transpose(kh_adjoint* a, int stage, const double* Pin, const double* P
     const StepConsts& c = b->consts;
     hipStream_t s = b->stream;
     const double g0 = c.gam0[stage - 1], g1 = c.gam1[stage - 1], beta_dt = c.beta[stage - 1] * dt;
-    KA_HIP(launch_adj_stage(bd, c, b->geo, Pout, lPout, lUout, g0, g1, beta_dt, same, accumulate, lUsub, lUfull,
-                            a->ld, a->err, s));
-    KA_HIP(launch_adj_divct(bd, a->ld, a->lF, a->lemf, s));
+    if (a->fixup) {
+        KA_HIP(launch_adj_fixup(bd, c, b->geo, Pout, lPout, lUout, a->pfs[stage - 1], a->lPfx, s));
+        KA_HIP(launch_adj_fold(bd, a->lPfx, NV, s));
+    }
     KA_HIP(launch_adj_fill(lPin, (long)NV * bd.nc, 0., s));
+    KA_HIP(launch_adj_stage(bd, c, b->geo, Pout, lPout, lUout, g0, g1, beta_dt, same, accumulate, lUsub, lUfull,
+                            a->ld, a->err, a->geo ? a->Ustar[stage - 1] : nullptr, Pin,
+                            a->floors ? a->ffs[stage - 1] : nullptr, a->geo ? lPin : nullptr,
+                            a->fixup ? a->pfs[stage - 1] : nullptr, a->fixup ? a->lPfx : nullptr, s));
+    KA_HIP(launch_adj_divct(bd, a->ld, a->lF, a->lemf, s));
+    if (a->obj_dir) {
+        const int dir = a->obj_dir;
+        const double dA = (dir == 1) ? bd.dx2 * bd.dx3 : (dir == 2) ? bd.dx1 * bd.dx3 : bd.dx1 * bd.dx2;
+        KA_HIP(launch_adj_seed(bd, dir, a->obj_var, a->obj_face, a->obj_weight * stage_flux_weight(c, stage) * dt * dA,
+                               a->lF, s));
+    }
+    if (a->physical) KA_HIP(launch_adj_fix_flux(bd, a->lF, a->x1f[stage - 1], s));
     KA_HIP(launch_adj_flux(bd, c, b->geo, Pin, a->lF, a->lLR, lPin, s));
+    if (a->physical) KA_HIP(launch_adj_bounds(bd, b->geo, Pin, lPin, s));
     KA_HIP(launch_adj_fold(bd, lPin, NV, s));
     return KH_OK;
 }
@@ -74,12 +144,17 @@ int kh_adjoint_create(const kh_params* p, kh_adjoint** out) {
     if (p->precision != KH_PREC_FP64) return api_fail(KH_ERR_UNSUPPORTED, "the adjoint runs in the FP64 form");
     if (p->recon != KH_RECON_WENO5) return api_fail(KH_ERR_UNSUPPORTED, "the adjoint supports WENO5 reconstruction");
     if (p->use_hlle) return api_fail(KH_ERR_UNSUPPORTED, "the adjoint supports the LLF flux");
-    if (!p->periodic1 || (p->nx2 > 1 && !p->periodic2))
-        return api_fail(KH_ERR_UNSUPPORTED, "the adjoint needs every axis periodic");
+    if (p->floors_on && p->floor_frame == 1)
+        return api_fail(KH_ERR_UNSUPPORTED, "the adjoint supports the fluid and drift floor frames");
     kh_adjoint* a = new kh_adjoint();
     st = kh_driver_create(p, &a->drv);
     if (st != KH_OK) { delete a; return st; }
-    const long nc = a->drv->blk->dims.nc;
+    const BlockDims& bd = a->drv->blk->dims;
+    const long nc = bd.nc;
+    a->geo = bd.geo != 0;
+    a->physical = !bd.periodic[0] || (bd.ndim > 1 && !bd.periodic[1]);
+    a->floors = a->geo && a->drv->blk->consts.floors.on;
+    a->fixup = a->drv->blk->consts.fix_average || a->drv->blk->consts.fix_atmo;
     const size_t sb = sizeof(double) * NV * nc;
     hipError_t e = hipSuccess;
     auto alloc = [&](double** q, size_t bytes) {
@@ -94,6 +169,14 @@ int kh_adjoint_create(const kh_params* p, kh_adjoint** out) {
     alloc(&a->lF, 3 * sb);
     alloc(&a->lemf, sizeof(double) * 3 * nc);
     alloc(&a->lLR, 2 * sb);
+    for (int q = 0; q < 2; ++q) {
+        if (a->geo) alloc(&a->Ustar[q], sb);
+        if (a->floors && e == hipSuccess) e = hipMalloc(&a->ffs[q], sizeof(int) * nc);
+        if (a->physical && x1_flux_rows(bd) > 0) alloc(&a->x1f[q], sizeof(double) * x1_flux_rows(bd));
+        if (a->fixup && e == hipSuccess) e = hipMalloc(&a->pfs[q], sizeof(int) * nc);
+        if (a->fixup && e == hipSuccess) e = hipMemset(a->pfs[q], 0, sizeof(int) * nc);
+    }
+    if (a->fixup) alloc(&a->lPfx, sb);
     if (e == hipSuccess) e = hipMalloc(&a->err, 2 * sizeof(int));
     for (int q = 0; q < 3 && e == hipSuccess; ++q) e = hipEventCreate(&a->ev[q]);
     if (e == hipSuccess) e = hipDeviceSynchronize();
@@ -108,7 +191,10 @@ int kh_adjoint_create(const kh_params* p, kh_adjoint** out) {
 void kh_adjoint_destroy(kh_adjoint* a) {
     if (!a) return;
     if (a->drv) (void)hipStreamSynchronize(a->drv->blk->stream);
-    for (double* q : {a->P0, a->lPin, a->lU0, a->lU1, a->ld, a->lF, a->lemf, a->lLR})
+    for (double* q : {a->P0, a->lPin, a->lU0, a->lU1, a->ld, a->lF, a->lemf, a->lLR, a->Ustar[0], a->Ustar[1],
+                      a->x1f[0], a->x1f[1], a->lPfx})
+        if (q) (void)hipFree(q);
+    for (int* q : {a->ffs[0], a->ffs[1], a->pfs[0], a->pfs[1]})
         if (q) (void)hipFree(q);
     if (a->err) (void)hipFree(a->err);
     for (hipEvent_t ev : a->ev)
@@ -133,23 +219,27 @@ int kh_adjoint_step(kh_adjoint* a, const double* P, const double* P_next, double
     const size_t sb = sizeof(double) * NV * bd.nc;
     KA_HIP(hipMemsetAsync(a->err, 0, 2 * sizeof(int), s));
     KA_HIP(hipEventRecord(a->ev[0], s));
-    // ---- forward recompute from the checkpoint: P_0 (ghosts refilled), U_0 = p_to_u(P_0), stage 1
+    // ---- forward recompute from the checkpoint: P_0 (ghosts as checkpointed; periodic
+    // images refilled), U_0 = p_to_u(P_0), stage 1
     KA_HIP(hipMemcpyAsync(d->P0, P, sb, hipMemcpyDeviceToDevice, s));
     KA_HIP(launch_sync(bd, d->P0, NV, s));
     KA_HIP(launch_ptou(bd, c, b->geo, d->P0, d->U0, s));
     KA_HIP(launch_reset_state(d->ds, dt, s));
-    int st = driver_stage(d, 1);                             // -> P1, U1
+    int st = recompute_stage(a, 1);                          // -> P1, U1
     if (st != KH_OK) return st;
-    KA_HIP(launch_adj_count_flags(bd, d->pflag, d->fflag, a->err, s));
     const double* P0 = d->P0;
     const double* P2 = P_next;
-    if (!P2) {
+    // the curved-space and physical-boundary transposes need stage 2's U*, flags
+    // and clamped fluxes: recompute it unless a plain periodic step can use P_next
+    if (!P2 || a->geo || a->physical) {
         // stage 2 writes P2, U2 over the driver's P0, U0: keep P_0
         KA_HIP(hipMemcpyAsync(a->P0, d->P0, sb, hipMemcpyDeviceToDevice, s));
         P0 = a->P0;
-        if ((st = driver_stage(d, 2)) != KH_OK) return st;
-        KA_HIP(launch_adj_count_flags(bd, d->pflag, d->fflag, a->err, s));
+        if ((st = recompute_stage(a, 2)) != KH_OK) return st;
         P2 = d->P0;
+    } else if (a->fixup) {
+        // stage 2 taken from P_next: the caller vouches that no zone of it failed
+        KA_HIP(hipMemsetAsync(a->pfs[1], 0, sizeof(int) * bd.nc, s));
     }
     KA_HIP(hipEventRecord(a->ev[1], s));
     // ---- transpose.  stage 2: P1 -> P2 with U_sub = U1, U_full = U0 (writes lU1, lU0)
@@ -167,8 +257,11 @@ int kh_adjoint_step(kh_adjoint* a, const double* P, const double* P_next, double
     KA_HIP(hipEventElapsedTime(&a->fwd_ms, a->ev[0], a->ev[1]));
     KA_HIP(hipEventElapsedTime(&a->adj_ms, a->ev[1], a->ev[2]));
     if (err[0] > 0)
-        return api_fail(KH_ERR_UNSUPPORTED, std::to_string(err[0]) + " zone-stages were floored or failed to invert; "
-                                            "the adjoint covers unfloored, converged steps only");
+        return api_fail(KH_ERR_UNSUPPORTED,
+                        std::to_string(err[0]) + " zone-stages " +
+                            (a->floors ? "failed to invert with the fixup off" : a->fixup ? "were floored" :
+                             "were floored or failed to invert with the fixup off") +
+                            "; the adjoint does not cover them on this block");
     if (err[1] > 0)
         return api_fail(KH_ERR_UNSUPPORTED, std::to_string(err[1]) + " zones have a singular dU/dP");
     return KH_OK;
@@ -197,7 +290,7 @@ int kh_adjoint_utop(kh_adjoint* a, const double* P, const double* lam_P, double*
     KA_HIP(launch_adj_fill(lam_U, (long)NV * b->dims.nc, 0., b->stream));
     KA_HIP(hipMemsetAsync(a->err, 0, 2 * sizeof(int), b->stream));
     KA_HIP(launch_adj_stage(b->dims, b->consts, b->geo, P, lam_P, nullptr, 0., 1., 0., true, true, lam_U, lam_U,
-                            a->ld, a->err, b->stream));
+                            a->ld, a->err, nullptr, P, nullptr, nullptr, nullptr, nullptr, b->stream));
     return KH_OK;
 }
 
@@ -205,6 +298,60 @@ int kh_adjoint_ptou(kh_adjoint* a, const double* P, const double* lam_U, double*
     if (!a || !P || !lam_U || !lam_P) return api_fail(KH_ERR_INVALID, "null argument");
     kh_block* b = a->drv->blk;
     KA_HIP(launch_adj_ptou(b->dims, b->consts, b->geo, P, lam_U, lam_P, b->stream));
+    return KH_OK;
+}
+
+int kh_adjoint_set_face_flux_objective(kh_adjoint* a, int dir, int var, int face, double weight) {
+    if (!a) return api_fail(KH_ERR_INVALID, "null adjoint");
+    if (dir == 0) { a->obj_dir = 0; return KH_OK; }
+    const int st = check_face_objective(a->drv->blk->dims, dir, var, face);
+    if (st != KH_OK) return st;
+    a->obj_dir = dir; a->obj_var = var; a->obj_face = face; a->obj_weight = weight;
+    return KH_OK;
+}
+
+int kh_adjoint_bounds(kh_adjoint* a, const double* P, double* lam_P) {
+    if (!a || !P || !lam_P) return api_fail(KH_ERR_INVALID, "null argument");
+    kh_block* b = a->drv->blk;
+    KA_HIP(launch_adj_bounds(b->dims, b->geo, P, lam_P, b->stream));
+    KA_HIP(launch_adj_fold(b->dims, lam_P, NV, b->stream));
+    return KH_OK;
+}
+
+int kh_adjoint_fix_flux(kh_adjoint* a, const double* F_fixed, double* lam_F) {
+    if (!a || !F_fixed || !lam_F) return api_fail(KH_ERR_INVALID, "null argument");
+    kh_block* b = a->drv->blk;
+    double* x1f = a->x1f[0];
+    if (x1_flux_rows(b->dims) > 0) {
+        if (!x1f) return api_fail(KH_ERR_INVALID, "no X1 flux rows on this block");
+        KA_HIP(launch_save_x1_flux(b->dims, F_fixed, x1f, b->stream));
+    }
+    KA_HIP(launch_adj_fix_flux(b->dims, lam_F, x1f, b->stream));
+    return KH_OK;
+}
+
+int kh_adjoint_geo_source(kh_adjoint* a, const double* P, const double* lam_dUdt, double* lam_P) {
+    if (!a || !P || !lam_dUdt || !lam_P) return api_fail(KH_ERR_INVALID, "null argument");
+    kh_block* b = a->drv->blk;
+    KA_HIP(launch_adj_geo(b->dims, b->consts, b->geo, P, lam_dUdt, lam_P, b->stream));
+    return KH_OK;
+}
+
+int kh_adjoint_floors(kh_adjoint* a, const double* P_in, const double* lam_P_out, double* lam_P_in) {
+    if (!a || !P_in || !lam_P_out || !lam_P_in) return api_fail(KH_ERR_INVALID, "null argument");
+    kh_block* b = a->drv->blk;
+    if (!b->consts.floors.on) return api_fail(KH_ERR_INVALID, "floors are off on this block");
+    KA_HIP(launch_adj_floors(b->dims, b->consts, b->geo, P_in, lam_P_out, lam_P_in, b->stream));
+    return KH_OK;
+}
+
+int kh_adjoint_fixup(kh_adjoint* a, const double* P_fixed, const int* pflag, double* lam_P) {
+    if (!a || !P_fixed || !pflag || !lam_P) return api_fail(KH_ERR_INVALID, "null argument");
+    if (!a->fixup) return api_fail(KH_ERR_INVALID, "the fixup is off on this block");
+    kh_block* b = a->drv->blk;
+    KA_HIP(launch_adj_fixup(b->dims, b->consts, b->geo, P_fixed, lam_P, nullptr, pflag, a->lPfx, b->stream));
+    KA_HIP(launch_adj_fold(b->dims, a->lPfx, NV, b->stream));
+    KA_HIP(launch_adj_fixup_apply(b->dims, pflag, a->lPfx, lam_P, b->stream));
     return KH_OK;
 }
 

@@ -388,7 +388,7 @@ void kh_driver_destroy(kh_driver* d) {
     for (hipEvent_t e : d->ev_pool) (void)hipEventDestroy(e);
     for (void* ptr : {(void*)d->U0, (void*)d->P0, (void*)d->U1, (void*)d->P1, (void*)d->P0f, (void*)d->P1f, d->F, d->ctop,
                       (void*)d->pflag, (void*)d->fflag, (void*)d->fail_list, (void*)d->ds, (void*)d->dt_hist,
-                      (void*)d->dt_part, (void*)d->divb_part})
+                      (void*)d->dt_part, (void*)d->divb_part, (void*)d->obj})
         if (ptr) (void)hipFree(ptr);
     kh_block_destroy(d->blk);
     delete d;
@@ -445,6 +445,15 @@ int kh_driver_init_prims(kh_driver* d, const double* P) {
 
 }  // extern "C"
 
+int kh::check_face_objective(const BlockDims& b, int dir, int var, int face) {
+    if (dir < 1 || dir > b.ndim) return fail(KH_ERR_INVALID, "objective direction outside the block's dimensions");
+    if (var < 0 || var >= B1) return fail(KH_ERR_INVALID, "objective variable must be a fluid variable (0..4)");
+    const int lo = (dir == 1) ? b.is : (dir == 2) ? b.js : b.ks;
+    const int hi = (dir == 1) ? b.ie + 1 : (dir == 2) ? b.je + 1 : b.ke + 1;
+    if (face < lo || face > hi) return fail(KH_ERR_INVALID, "objective face outside the block's faces");
+    return KH_OK;
+}
+
 // One stage of the default task collection on the driver's buffers: fluxes,
 // flux fixes, the fused update (+ inversion, floors, P->U), fixups, boundaries.
 int kh::driver_stage(kh_driver* d, int stage) {
@@ -469,6 +478,9 @@ int kh::driver_stage(kh_driver* d, int stage) {
         KH_TIMED(dir - 1, launch_flux_stage(bd, c, b->geo, dir, b->p.recon, prec, speeds, Psub, Psubf,
                                             F + fe * (dir - 1) * NV * bd.nc, ct + fe * (dir - 1) * bd.nc, s));
     if (physical) KH_TIMED(3, launch_fix_flux(bd, prec, d->F, s));
+    if (d->obj_dir)
+        KH_HIP(launch_face_flux_objective(bd, prec, d->F, d->obj_dir, d->obj_var, d->obj_face,
+                                          d->obj_weight * stage_flux_weight(c, stage), d->ds, d->obj, s));
     KH_TIMED(3 + stage, launch_update(bd, c, b->geo, prec, stage, stage == 2, d->F, d->ctop, Usub, d->U0, Psub,
                                       Uout, Pout, Poutf, d->pflag, d->fflag, d->fail_list, d->ds, d->dt_part, s));
     if (c.fix_average || c.fix_atmo)
@@ -531,6 +543,29 @@ int kh_driver_get_scalars(kh_driver* d, double* time, double* dt, double* last_d
 int kh_driver_set_clock(kh_driver* d, double time, double dt, int ncycle) {
     if (!d || !(dt > 0.) || ncycle < 0) return fail(KH_ERR_INVALID, "bad argument");
     KH_HIP(launch_set_clock(d->ds, time, dt, ncycle, d->blk->stream));
+    return KH_OK;
+}
+
+int kh_driver_set_face_flux_objective(kh_driver* d, int dir, int var, int face, double weight) {
+    if (!d) return fail(KH_ERR_INVALID, "null driver");
+    if (dir == 0) { d->obj_dir = 0; return KH_OK; }
+    const int st = check_face_objective(d->blk->dims, dir, var, face);
+    if (st != KH_OK) return st;
+    if (!d->obj) {
+        KH_HIP(hipMalloc(&d->obj, sizeof(double)));
+        KH_HIP(hipMemsetAsync(d->obj, 0, sizeof(double), d->blk->stream));
+    }
+    d->obj_dir = dir; d->obj_var = var; d->obj_face = face; d->obj_weight = weight;
+    return KH_OK;
+}
+
+int kh_driver_objective(kh_driver* d, double* value, int reset) {
+    if (!d || !value) return fail(KH_ERR_INVALID, "null argument");
+    *value = 0.;
+    if (!d->obj) return KH_OK;
+    KH_HIP(hipMemcpyAsync(value, d->obj, sizeof(double), hipMemcpyDeviceToHost, d->blk->stream));
+    if (reset) KH_HIP(hipMemsetAsync(d->obj, 0, sizeof(double), d->blk->stream));
+    KH_HIP(hipStreamSynchronize(d->blk->stream));
     return KH_OK;
 }
 

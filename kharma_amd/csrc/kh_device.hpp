@@ -8,6 +8,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <type_traits>
 
 namespace kh {
 
@@ -31,6 +32,13 @@ template <typename R> __device__ __forceinline__ R smin(R a, R b) { return (b < 
 // kharma_utils.hpp:100-110: min(max(lower, n), upper)
 template <typename R> __device__ __forceinline__ R clip(R n, R lo, R hi) { return smin(smax(lo, n), hi); }
 
+// Enables the geometry overloads below for non-arithmetic scalar types only, so
+// that calls with float or double arguments resolve exactly as without them.
+template <class T> using NonArith = std::enable_if_t<!std::is_arithmetic<T>::value>;
+// Value of a scalar (the identity for double; the adjoint's dual numbers
+// overload it).
+__device__ __forceinline__ double val(double x) { return x; }
+
 // Flat space, Cartesian coordinates.
 template <typename R> struct MinkowskiGeom {
     static constexpr bool flat = true;
@@ -44,6 +52,16 @@ template <typename R> struct MinkowskiGeom {
     __device__ __forceinline__ R lorentz(R u1, R u2, R u3) const {
         const R qsq = u1 * u1 + u2 * u2 + u3 * u3;
         return sqrt(R(1) + qsq);
+    }
+    // the same for non-arithmetic scalars (the adjoint's dual numbers)
+    template <class T, class = NonArith<T>>
+    __device__ __forceinline__ void lower(const T v[4], T o[4]) const { o[0] = -v[0]; o[1] = v[1]; o[2] = v[2]; o[3] = v[3]; }
+    template <class T, class = NonArith<T>>
+    __device__ __forceinline__ void raise(const T v[4], T o[4]) const { o[0] = -v[0]; o[1] = v[1]; o[2] = v[2]; o[3] = v[3]; }
+    template <class T, class = NonArith<T>>
+    __device__ __forceinline__ T lorentz(T u1, T u2, T u3) const {
+        const T qsq = u1 * u1 + u2 * u2 + u3 * u3;
+        return sqrt(T(1.) + qsq);
     }
 };
 
@@ -80,6 +98,25 @@ template <typename R> struct CachedGeom {
         const R qsq = gcov(1, 1) * u1 * u1 + gcov(2, 2) * u2 * u2 + gcov(3, 3) * u3 * u3 +
                       R(2) * (gcov(1, 2) * u1 * u2 + gcov(1, 3) * u1 * u3 + gcov(2, 3) * u2 * u3);
         return sqrt(R(1) + qsq);
+    }
+    // the same for non-arithmetic scalars (the adjoint's dual numbers)
+    template <class T, class = NonArith<T>>
+    __device__ __forceinline__ void lower(const T v[4], T o[4]) const {
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+            o[m] = gcov(m, 0) * v[0] + gcov(m, 1) * v[1] + gcov(m, 2) * v[2] + gcov(m, 3) * v[3];
+    }
+    template <class T, class = NonArith<T>>
+    __device__ __forceinline__ void raise(const T v[4], T o[4]) const {
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+            o[m] = gcon(m, 0) * v[0] + gcon(m, 1) * v[1] + gcon(m, 2) * v[2] + gcon(m, 3) * v[3];
+    }
+    template <class T, class = NonArith<T>>
+    __device__ __forceinline__ T lorentz(T u1, T u2, T u3) const {
+        const T qsq = gcov(1, 1) * u1 * u1 + gcov(2, 2) * u2 * u2 + gcov(3, 3) * u3 * u3 +
+                      R(2) * (gcov(1, 2) * u1 * u2 + gcov(1, 3) * u1 * u3 + gcov(2, 3) * u2 * u3);
+        return sqrt(T(1.) + qsq);
     }
 };
 
@@ -482,21 +519,22 @@ struct FloorParams {
 // Floors::determine_floors, floors/floors_functions.hpp:84-160.  bsq = b^mu b_mu
 // of the zone's current state; rhoflr_geom / uflr_geom are the geometric floors
 // of the zone (constants in Cartesian coordinates, r-dependent in spherical ones).
-template <class G>
-__device__ __forceinline__ int determine_floors(const G& g, const double P[NV], double bsq, double gam,
+// T is double, or the adjoint's dual numbers (its floor Jacobians).
+template <class G, class T>
+__device__ __forceinline__ int determine_floors(const G& g, const T P[NV], T bsq, double gam,
                                                 const FloorParams& f, double rhoflr_geom, double uflr_geom,
-                                                double& rhoflr_max, double& uflr_max) {
-    const double rhoflr_b = bsq / f.bsq_over_rho_max;
-    const double uflr_b = bsq / f.bsq_over_u_max;
-    uflr_max = smax(uflr_geom, uflr_b);
-    const double rho = P[RHO], u = P[UU];
+                                                T& rhoflr_max, T& uflr_max) {
+    const T rhoflr_b = bsq / f.bsq_over_rho_max;
+    const T uflr_b = bsq / f.bsq_over_u_max;
+    uflr_max = smax(T(uflr_geom), uflr_b);
+    const T rho = P[RHO], u = P[UU];
     int fflag = 0;
     if (!f.temp_adjust_u) {
-        const double rhoflr_temp = smax(u, uflr_max) / f.u_over_rho_max;
+        const T rhoflr_temp = smax(u, uflr_max) / f.u_over_rho_max;
         fflag |= (rhoflr_temp > rho) * F_TEMP;
-        rhoflr_max = smax(smax(rhoflr_geom, rhoflr_b), rhoflr_temp);
+        rhoflr_max = smax(smax(T(rhoflr_geom), rhoflr_b), rhoflr_temp);
     } else {
-        rhoflr_max = smax(rhoflr_geom, rhoflr_b);
+        rhoflr_max = smax(T(rhoflr_geom), rhoflr_b);
     }
     if (rhoflr_max > rho || uflr_max > u) {
         fflag |= (rhoflr_geom > rho) * F_GEOM_RHO;
@@ -505,14 +543,14 @@ __device__ __forceinline__ int determine_floors(const G& g, const double P[NV], 
         fflag |= (uflr_b > u) * F_B_U;
     }
     if (g.lorentz(P[U1], P[U2], P[U3]) > f.gamma_max) fflag |= F_GAMMA;
-    if (ktot_exceeds(P[UU], P[RHO], gam, f.ktot_max)) fflag |= F_KTOT;
+    if (ktot_exceeds(val(P[UU]), val(P[RHO]), gam, f.ktot_max)) fflag |= F_KTOT;
     if (f.temp_adjust_u && (P[UU] / P[RHO] > f.u_over_rho_max)) fflag |= F_TEMP;
     return fflag;
 }
 
 // Drift-frame floors, floors_functions.hpp:186-257
-template <class G>
-__device__ __forceinline__ void apply_floors_drift(const G& g, double P[NV], double gam, double rhoflr_max, double uflr_max) {
+template <class G, class T>
+__device__ __forceinline__ void apply_floors_drift(const G& g, T P[NV], double gam, T rhoflr_max, T uflr_max) {
     double beta[4] = {0., 0., 0., 0.};
     if constexpr (!G::flat) {
         const double lapse2 = rcp(-g.gcon(0, 0));
@@ -521,42 +559,42 @@ __device__ __forceinline__ void apply_floors_drift(const G& g, double P[NV], dou
         beta[3] = lapse2 * g.gcon(0, 3);
     }
     // Quotients are products with rcp() (~1 ulp) rather than IEEE divisions.
-    const double rho = P[RHO], uu = P[UU];
-    const double pg = (gam - 1.) * uu;
-    const double w_old = smax(rho + uu + pg, 1e-20);
-    double Bcon[4] = {0., P[B1], P[B2], P[B3]};
-    double Bcov[4];
+    const T rho = P[RHO], uu = P[UU];
+    const T pg = (gam - 1.) * uu;
+    const T w_old = smax(rho + uu + pg, T(1e-20));
+    T Bcon[4] = {T(0.), P[B1], P[B2], P[B3]};
+    T Bcov[4];
     g.lower(Bcon, Bcov);
-    const double Bsq = smax(dot4(Bcon, Bcov), 1e-20);
-    const double B_mag = sqrt(Bsq);
-    const double iBm = rcp(B_mag);
-    FourVec<double> D;
-    calc_4vecs<double>(g, P, D);
-    const double u0 = D.ucon[0], iu0 = rcp(u0);
-    double Qcov[4];
+    const T Bsq = smax(dot4(Bcon, Bcov), T(1e-20));
+    const T B_mag = sqrt(Bsq);
+    const T iBm = rcp(B_mag);
+    FourVec<T> D;
+    calc_4vecs<T>(g, P, D);
+    const T u0 = D.ucon[0], iu0 = rcp(u0);
+    T Qcov[4];
     Qcov[0] = w_old * u0 * D.ucov[0] + pg;
     Qcov[1] = w_old * u0 * D.ucov[1];
     Qcov[2] = w_old * u0 * D.ucov[2];
     Qcov[3] = w_old * u0 * D.ucov[3];
-    const double QdotB = dot4(Bcon, Qcov);
-    double vpar = QdotB * rcp(B_mag * w_old * u0 * u0);
-    double ucon_dr[4];
-    const double udr0 = rcp(sqrt(iu0 * iu0 + vpar * vpar));
+    const T QdotB = dot4(Bcon, Qcov);
+    T vpar = QdotB * rcp(B_mag * w_old * u0 * u0);
+    T ucon_dr[4];
+    const T udr0 = rcp(sqrt(iu0 * iu0 + vpar * vpar));
     {
-        const double a = udr0 * iu0, c = vpar * udr0 * iBm;
+        const T a = udr0 * iu0, c = vpar * udr0 * iBm;
         for (int mu = 0; mu < 4; ++mu) ucon_dr[mu] = D.ucon[mu] * a - Bcon[mu] * c;
     }
     P[RHO] = smax(rho, rhoflr_max);
     P[UU] = smax(uu, uflr_max);
-    const double pg_new = (gam - 1.) * P[UU];
-    const double w_new = P[RHO] + P[UU] + pg_new;
-    const double iudr0 = rcp(ucon_dr[0]);
-    const double x = (2. * QdotB) * rcp(B_mag * w_new * ucon_dr[0]);
-    vpar = x * rcp(1 + sqrt(1 + x * x)) * iudr0;
-    double ucon[4];
+    const T pg_new = (gam - 1.) * P[UU];
+    const T w_new = P[RHO] + P[UU] + pg_new;
+    const T iudr0 = rcp(ucon_dr[0]);
+    const T x = (2. * QdotB) * rcp(B_mag * w_new * ucon_dr[0]);
+    vpar = x * rcp(1. + sqrt(1. + x * x)) * iudr0;
+    T ucon[4];
     ucon[0] = rcp(sqrt(iudr0 * iudr0 - vpar * vpar));
     {
-        const double a = ucon[0] * iudr0, c = vpar * ucon[0] * iBm;
+        const T a = ucon[0] * iudr0, c = vpar * ucon[0] * iBm;
         for (int mu = 1; mu < 4; ++mu) ucon[mu] = ucon_dr[mu] * a + Bcon[mu] * c;
     }
     P[U1] = ucon[1] + (beta[1] * ucon[0]);
@@ -582,16 +620,16 @@ __device__ __forceinline__ int apply_floors_normal(const G& g, double P[NV], dou
 // Floors::apply_ceilings, floors_functions.hpp:50-82
 // ktot = (gam - 1) u / rho^gam is formed (with its pow) only where the entropy
 // ceiling applies; ktot_exceeds decides that exactly from log2 estimates.
-template <class G>
-__device__ __forceinline__ void apply_ceilings(const G& g, double P[NV], double gam, const FloorParams& f) {
-    const double gamma = g.lorentz(P[U1], P[U2], P[U3]);
-    const double u0 = P[UU], rho0 = P[RHO];
+template <class G, class T>
+__device__ __forceinline__ void apply_ceilings(const G& g, T P[NV], double gam, const FloorParams& f) {
+    const T gamma = g.lorentz(P[U1], P[U2], P[U3]);
+    const T u0 = P[UU], rho0 = P[RHO];
     if (gamma > f.gamma_max) {
-        const double fr = sqrt((f.gamma_max * f.gamma_max - 1.) / (gamma * gamma - 1.));
-        P[U1] *= fr; P[U2] *= fr; P[U3] *= fr;
+        const T fr = sqrt((f.gamma_max * f.gamma_max - 1.) / (gamma * gamma - 1.));
+        P[U1] = P[U1] * fr; P[U2] = P[U2] * fr; P[U3] = P[U3] * fr;
     }
-    if (ktot_exceeds(u0, rho0, gam, f.ktot_max)) {
-        const double ktot = (gam - 1.) * u0 / pow(rho0, gam);
+    if (ktot_exceeds(val(u0), val(rho0), gam, f.ktot_max)) {
+        const T ktot = (gam - 1.) * u0 / pow(rho0, gam);
         P[UU] = f.ktot_max / ktot * P[UU];
     }
     if (f.temp_adjust_u && P[UU] / P[RHO] > f.u_over_rho_max) P[UU] = f.u_over_rho_max * P[RHO];
@@ -631,21 +669,21 @@ __device__ __forceinline__ int floors_zone(const G& g, double P[NV], double U[NV
 // KBoundaries::check_inflow, boundaries/boundaries.hpp:101-136: velocity into
 // the domain through a physical X1 face is replaced by zero radial 4-velocity.
 // (The clip() of vsq there discards its result, so vsq is used unclamped.)
-template <class G>
-__device__ __forceinline__ void check_inflow(const G& g, double P[NV], bool inner) {
-    double uvec[3] = {P[U1], P[U2], P[U3]};
-    const double gamma0 = g.lorentz(uvec[0], uvec[1], uvec[2]);
-    const double ucon1 = uvec[0] - gamma0 * g.alpha() * g.gcon0(1);
+template <class G, class T>
+__device__ __forceinline__ void check_inflow(const G& g, T P[NV], bool inner) {
+    T uvec[3] = {P[U1], P[U2], P[U3]};
+    const T gamma0 = g.lorentz(uvec[0], uvec[1], uvec[2]);
+    const T ucon1 = uvec[0] - gamma0 * g.alpha() * g.gcon0(1);
     if ((ucon1 > 0. && inner) || (ucon1 < 0. && !inner)) {
-        for (int v = 0; v < 3; ++v) uvec[v] /= gamma0;
+        for (int v = 0; v < 3; ++v) uvec[v] = uvec[v] / gamma0;
         const double alpha = g.alpha();
         const double beta1 = g.gcon(0, 1) * alpha * alpha;
         uvec[0] = beta1 / alpha;
-        const double vsq = g.gcov(1, 1) * uvec[0] * uvec[0] + g.gcov(2, 2) * uvec[1] * uvec[1] +
+        const T vsq = g.gcov(1, 1) * uvec[0] * uvec[0] + g.gcov(2, 2) * uvec[1] * uvec[1] +
                            g.gcov(3, 3) * uvec[2] * uvec[2] +
                            2. * (g.gcov(1, 2) * uvec[0] * uvec[1] + g.gcov(1, 3) * uvec[0] * uvec[2] +
                                  g.gcov(2, 3) * uvec[1] * uvec[2]);
-        const double gamma = 1. / sqrt(1. - vsq);
+        const T gamma = 1. / sqrt(1. - vsq);
         for (int v = 0; v < 3; ++v) P[U1 + v] = uvec[v] * gamma;
     }
 }

@@ -27,6 +27,10 @@ struct kh_driver {
     double *dt_part = nullptr, *divb_part = nullptr;
     double* dt_hist = nullptr;
     int hist_cap = 0;
+    // face-flux objective (kh_driver_set_face_flux_objective): dir 0 = off
+    int obj_dir = 0, obj_var = 0, obj_face = 0;
+    double obj_weight = 0.;
+    double* obj = nullptr;   // device accumulator
     // kernel timing (kh_driver_set_kernel_timing)
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
@@ -51,4 +55,11 @@ namespace kh {
 inline size_t flux_elem(const kh_block* b) { return b->p.precision == KH_PREC_SPLIT ? sizeof(float) : sizeof(double); }
 // One RK stage (1 or 2) of the driver's step on its current buffers.
 int driver_stage(kh_driver* d, int stage);
+// Weight of stage `stage`'s fluxes in the step's update U_n -> U_{n+1}
+// (the RK weights: beta_1 gam0_2 for stage 1, beta_2 for stage 2).
+inline double stage_flux_weight(const StepConsts& c, int stage) {
+    return stage == 1 ? c.beta[0] * c.gam0[1] : c.beta[1];
+}
+// Checks a face-flux objective's (dir, var, face) against the block.
+int check_face_objective(const BlockDims& b, int dir, int var, int face);
 }  // namespace kh

@@ -118,6 +118,15 @@ hipError_t launch_update(const BlockDims& b, const StepConsts& c, const GeomDev&
                          const void* F, const void* ctop, const double* Usub, const double* Ufull, const double* Psub,
                          double* Uout, double* Pout, float* Pout32, int* pflag, int* fflag,
                          int* fail_list, DeviceState* ds, double* dt_part, hipStream_t s);
+// The two halves of launch_update on the curved-space FP64 path: parts bit 1
+// runs the combine (U* of the stage into Uout), bit 2 the recovery (U* -> P,
+// floors, P -> U).  The adjoint's recompute reads U* in between.
+hipError_t launch_update_parts(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int stage, int parts,
+                               const double* F, const double* Usub, const double* Ufull, const double* Psub, double* Uout,
+                               double* Pout, int* pflag, int* fflag, int* fail_list, DeviceState* ds, hipStream_t s);
+// acc += w dt sum_faces F_dir[var] dA (one stage of a face-flux objective).
+hipError_t launch_face_flux_objective(const BlockDims& b, int prec, const void* F, int dir, int var, int face, double w,
+                                      const DeviceState* ds, double* acc, hipStream_t s);
 long update_blocks(const BlockDims& b);
 long divb_blocks(const BlockDims& b);
 hipError_t launch_fixup(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int stage, double* U, double* P,
@@ -160,12 +169,41 @@ hipError_t launch_adj_flux(const BlockDims& b, const StepConsts& c, const GeomDe
 // lF ([3][8][nc]) = (d dudt / d F)^T ld; lemf is [3][nc] scratch.
 hipError_t launch_adj_divct(const BlockDims& b, const double* ld, double* lF, double* lemf, hipStream_t s);
 // accumulate = false overwrites lUsub / lUfull on the interior instead of adding to them.
+// ffs (per-zone floor flags of the stage, or null: no zone floored) with Ustar
+// (the stage's U before recovery) and Psub (its input primitives, the recovery's
+// guess); lPgeo (or null) receives the geometric source's transpose.
 hipError_t launch_adj_stage(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const double* Pout,
                             const double* lPout, const double* lUout, double g0, double g1, double beta_dt, bool same,
-                            bool accumulate, double* lUsub, double* lUfull, double* ld, int* err, hipStream_t s);
+                            bool accumulate, double* lUsub, double* lUfull, double* ld, int* err, const double* Ustar,
+                            const double* Psub, const int* ffs, double* lPgeo, const int* pfs, const double* lPfx,
+                            hipStream_t s);
+// Transpose of the inversion fixup (fixup_list_kernel): lPfx[8][nc] (every zone;
+// fold periodic images afterwards) = the fluid adjoints that zones failing to
+// invert (pfs > 0) pass to the neighbours they were averaged from.  The stage
+// transpose then adds lPfx (pfs, lPfx non-null) and gives failed zones only
+// their P.B = U*.B / gdet dependence.
+hipError_t launch_adj_fixup(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const double* Pout,
+                            const double* lPout, const double* lUout, const int* pfs, double* lPfx, hipStream_t s);
+// lP (interior fluid part) <- failed zones 0, others + lPfx.
+hipError_t launch_adj_fixup_apply(const BlockDims& b, const int* pfs, const double* lPfx, double* lP, hipStream_t s);
+// Transpose of the physical-boundary ghost fill of P (bounds_kernel), gathered
+// into the zones it reads; ghosts are left for launch_adj_fold to clear.
+hipError_t launch_adj_bounds(const BlockDims& b, const GeomDev& gd, const double* P, double* A, hipStream_t s);
+// X1 rows of fix_flux: x1_flux_rows(b) saved mass fluxes per stage.
+long x1_flux_rows(const BlockDims& b);
+hipError_t launch_save_x1_flux(const BlockDims& b, const double* F, double* x1f, hipStream_t s);
+hipError_t launch_adj_fix_flux(const BlockDims& b, double* lF, const double* x1f, hipStream_t s);
+hipError_t launch_adj_seed(const BlockDims& b, int dir, int var, int face, double w, double* lF, hipStream_t s);
 hipError_t launch_adj_ptou(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const double* P,
                            const double* lU, double* lP, hipStream_t s);
 hipError_t launch_adj_fold(const BlockDims& b, double* A, int nvar, hipStream_t s);
+// lP += (dS/dP)^T ld for the geometric source (curved geometry; interior).
+hipError_t launch_adj_geo(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const double* P, const double* ld,
+                          double* lP, hipStream_t s);
+// lPin = (dfloors/dP at Pin)^T lPout (interior; frames 0 and 2).
+hipError_t launch_adj_floors(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const double* Pin,
+                             const double* lPout, double* lPin, hipStream_t s);
+// err[0] += zones with pflag != 0, or (fflag non-null) fflag != 0.
 hipError_t launch_adj_count_flags(const BlockDims& b, const int* pflag, const int* fflag, int* err, hipStream_t s);
 hipError_t launch_estimate_dt(const BlockDims& b, const StepConsts& c, int prec, const void* cmax, const void* cmin,
                               double* part, hipStream_t s);

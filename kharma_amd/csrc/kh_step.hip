@@ -1065,7 +1065,47 @@ __global__ void __launch_bounds__(BS) estimate_dt_kernel(BlockDims b, const R* _
     if (threadIdx.x == 0) part[blockIdx.x] = m;
 }
 
+// Face-flux objective of one stage: acc += w dt sum F_dir[var] dA over the
+// faces of index `face` along dir and the interior transverse range (dA: the
+// product of the two transverse zone widths).  One work-group, fixed order.
+template <typename R>
+__global__ void __launch_bounds__(BS) face_flux_objective_kernel(BlockDims b, const R* __restrict__ F, int dir, int var,
+                                                                 int face, double w, const DeviceState* __restrict__ ds,
+                                                                 double* __restrict__ acc) {
+    const int na = (dir == 1) ? (b.ndim > 1 ? b.nx2 : 1) : b.nx1;
+    const int nb = (dir == 3) ? (b.ndim > 1 ? b.nx2 : 1) : (b.ndim > 2 ? b.nx3 : 1);
+    const R* Fd = F + ((long)(dir - 1) * NV + var) * b.nc;
+    double sum = 0.;
+    for (int t = threadIdx.x; t < na * nb; t += blockDim.x) {
+        const int ta = t % na, tb = t / na;
+        int i, j, k;
+        if (dir == 1) { i = face; j = b.js + ta; k = b.ks + tb; }
+        else if (dir == 2) { i = b.is + ta; j = face; k = b.ks + tb; }
+        else { i = b.is + ta; k = face; j = b.js + tb; }
+        sum += (double)Fd[(long)k * b.s3 + (long)j * b.s2 + i];
+    }
+    __shared__ double red[BS];
+    red[threadIdx.x] = sum;
+    __syncthreads();
+    for (int h = BS / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double dA = (dir == 1) ? b.dx2 * b.dx3 : (dir == 2) ? b.dx1 * b.dx3 : b.dx1 * b.dx2;
+        acc[0] += w * ds->dt * red[0] * dA;
+    }
+}
+
 // ---------------------------------------------------------------- launchers
+hipError_t launch_face_flux_objective(const BlockDims& b, int prec, const void* F, int dir, int var, int face, double w,
+                                      const DeviceState* ds, double* acc, hipStream_t s) {
+    if (prec == PREC_SPLIT)
+        hipLaunchKernelGGL(face_flux_objective_kernel<float>, dim3(1), dim3(BS), 0, s, b, (const float*)F, dir, var, face, w, ds, acc);
+    else
+        hipLaunchKernelGGL(face_flux_objective_kernel<double>, dim3(1), dim3(BS), 0, s, b, (const double*)F, dir, var, face, w, ds, acc);
+    return hipGetLastError();
+}
 hipError_t launch_emf(const BlockDims& b, int prec, const void* F, void* emf, hipStream_t s) {
     if (b.ndim < 2) return hipSuccess;
     const long n = (long)(b.nx1 + 1) * (b.nx2 + 1) * ((b.ndim > 2) ? b.nx3 + 1 : 1);
@@ -1096,9 +1136,10 @@ template <typename R>
 static void launch_update_geo(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int stage, bool do_dt,
                               const void* F, const void* ctop, const double* Usub, const double* Ufull,
                               const double* Psub, double* Uout, double* Pout, float* Pout32, int* pflag, int* fflag,
-                              int* fail_list, DeviceState* ds, double* dt_part, hipStream_t s) {
+                              int* fail_list, DeviceState* ds, double* dt_part, hipStream_t s, int parts = 3) {
     const dim3 g = grid_for(ninterior_host(b));
     const bool same = (c.gam0[stage - 1] == 0.) && (Usub == Ufull);
+    if (parts & 1) {
 #define KH_CMB(DT, ND, SAME)                                                                                          \
     hipLaunchKernelGGL((combine_kernel<R, DT, ND, SAME>), g, dim3(BS), 0, s, b, c, gd, stage, (const R*)F, (const R*)ctop, \
                        Usub, Ufull, Psub, Uout, ds, dt_part)
@@ -1108,6 +1149,8 @@ static void launch_update_geo(const BlockDims& b, const StepConsts& c, const Geo
     else { if (same) KH_CMB_ND(false, true) else KH_CMB_ND(false, false) }
 #undef KH_CMB_ND
 #undef KH_CMB
+    }
+    if (!(parts & 2)) return;
 #define KH_REC(FR) \
     hipLaunchKernelGGL((recover_kernel<FR>), g, dim3(BS), 0, s, b, c, gd, stage, Psub, Uout, Pout, Pout32, pflag, fflag, fail_list, ds)
     if (c.floors.frame == 0) KH_REC(0);
@@ -1140,6 +1183,14 @@ hipError_t launch_update(const BlockDims& b, const StepConsts& c, const GeomDev&
         if (b.geo) launch_update_geo<double>(b, c, gd, stage, do_dt, F, ctop, Usub, Ufull, Psub, Uout, Pout, Pout32, pflag, fflag, fail_list, ds, dt_part, s);
         else launch_update_t<double, 0>(b, c, gd, stage, do_dt, F, ctop, Usub, Ufull, Psub, Uout, Pout, Pout32, pflag, fflag, fail_list, ds, dt_part, s);
     }
+    return hipGetLastError();
+}
+hipError_t launch_update_parts(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int stage, int parts,
+                               const double* F, const double* Usub, const double* Ufull, const double* Psub, double* Uout,
+                               double* Pout, int* pflag, int* fflag, int* fail_list, DeviceState* ds, hipStream_t s) {
+    if (!b.geo) return hipErrorInvalidValue;
+    launch_update_geo<double>(b, c, gd, stage, false, F, nullptr, Usub, Ufull, Psub, Uout, Pout, nullptr, pflag, fflag,
+                              fail_list, ds, nullptr, s, parts);
     return hipGetLastError();
 }
 long update_blocks(const BlockDims& b) { return (ninterior_host(b) + BS - 1) / BS; }

@@ -84,15 +84,15 @@ def test_adjoint_step_vs_fd(case):
     evo = Evolution(cfg)
     rng = np.random.default_rng(7)
     P0 = _cuda(generic_state(cfg, rng))
-    _, dts, ck = evo.run(P0, 1, checkpoints=True)
+    _, dts, ck, _ = evo.run(P0, 1, checkpoints=True)
     lam = _cuda(rng.normal(size=P0.shape))
     g = evo.backward(lam, ck, dts).cpu().numpy()
     lam_np = lam.cpu().numpy()
     dirs = [smooth_direction(cfg, rng) for _ in range(3)] + [noise_direction(cfg, rng, range(pr.NV)) for _ in range(2)]
     fd, ad = [], []
     for v in dirs:
-        Pp, _, _ = evo.run(P0 + H * _cuda(v), 1, dts)
-        Pm, _, _ = evo.run(P0 - H * _cuda(v), 1, dts)
+        Pp = evo.run(P0 + H * _cuda(v), 1, dts)[0]
+        Pm = evo.run(P0 - H * _cuda(v), 1, dts)[0]
         fd.append(float((lam_np * (Pp - Pm).cpu().numpy()).sum() / (2 * H)))
         ad.append(float((g * v).sum()))
     _check(fd, ad, f"{case} one step")
@@ -144,7 +144,7 @@ def test_adjoint_rejects_unsupported():
         kh.AdjointStep(dict(cfg, riemann="hlle"))
     tcfg = pr.torus_config(nx1=32, nx2=16, nx3=1)
     with pytest.raises(kh.KharmaError):
-        kh.AdjointStep(tcfg)
+        kh.AdjointStep(dict(tcfg, floor_frame="normal"))
 
 
 # ------------------------------------------------------- task-level transposes
