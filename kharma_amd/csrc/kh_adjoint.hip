@@ -149,13 +149,13 @@ adj_face_xt_kernel(BlockDims b, StepConsts c, GeomDev gd, const double* __restri
 // stencils, and so the gather, cover cells [s-3, e+3].  Tiles hold TN cells along DIR and emit the
 // TN - 4 cells whose whole +-2 neighbourhood lies inside the tile.
 // DIR = 1: block (64, 4), x along i, y over transverse rows (j, k).
-// DIR = 2, 3: block (64, 16), x over transverse columns (outer, i), y along DIR.
+// DIR = 2, 3: block (16, 16), x over transverse columns (outer, i), y along DIR.
 template <int DIR>
-__global__ void __launch_bounds__(1024)
+__global__ void __launch_bounds__(256)
 adj_recon_kernel(BlockDims b, const double* __restrict__ P, const double* __restrict__ lamL,
                  const double* __restrict__ lamR, double* __restrict__ lamP, int j0, int nj, int k0, int nk) {
     constexpr int TN = (DIR == 1) ? 64 : 16;
-    constexpr int TT = (DIR == 1) ? 4 : 64;
+    constexpr int TT = (DIR == 1) ? 4 : 16;
     __shared__ double G[5][TN][TT];
     int tn, tt, n, i = 0, j = 0, k = 0;
     bool tvalid;
@@ -213,7 +213,8 @@ adj_recon_kernel(BlockDims b, const double* __restrict__ P, const double* __rest
 // corner EMF adjoints; ld (= lam dudt) is zero outside the interior.
 template <int ND>
 __global__ void __launch_bounds__(ABS)
-adj_divct_kernel(BlockDims b, const double* __restrict__ ld, double* __restrict__ lF, double* __restrict__ lemf) {
+adj_divct_kernel(BlockDims b, const double* __restrict__ ld, double* __restrict__ lF, double* __restrict__ lemf,
+                 FluxSeed seed) {
     const long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= b.nc) return;
     const double V = b.dx1 * b.dx2 * b.dx3;
@@ -232,6 +233,15 @@ adj_divct_kernel(BlockDims b, const double* __restrict__ ld, double* __restrict_
             if (direct) val = ad[d] * (L(v, x) - L(v, x - st[d]));
             lF[(d * NV + v) * nc + x] = val;
         }
+    }
+    if (seed.dir) {
+        // a face-flux objective's seed (adj_seed_kernel's faces, fluid variables)
+        const int k = (int)(x / s3), j = (int)((x - (long)k * s3) / s2), i = (int)(x - (long)k * s3 - (long)j * s2);
+        const int along = (seed.dir == 1) ? i : (seed.dir == 2) ? j : k;
+        const bool t1 = (seed.dir == 1) || (i >= b.is && i <= b.ie);
+        const bool t2 = (seed.dir == 2) || (b.ndim < 2) || (j >= b.js && j <= b.je);
+        const bool t3 = (seed.dir == 3) || (b.ndim < 3) || (k >= b.ks && k <= b.ke);
+        if (along == seed.face && t1 && t2 && t3) lF[((seed.dir - 1) * NV + seed.var) * nc + x] += seed.w;
     }
     if constexpr (ND >= 2) {
         // e3 (b_flux_ct.cpp:331-342) enters dudt[B1] through F2 and dudt[B2] through F1
@@ -303,9 +313,9 @@ template <int GEO>
 __global__ void __launch_bounds__(ABS)
 adj_fixup_kernel(BlockDims b, StepConsts c, GeomDev gd, const double* __restrict__ Pout,
                  const double* __restrict__ lPout, const double* __restrict__ lUout, const int* __restrict__ pf,
-                 double* __restrict__ lPfx) {
+                 const int* __restrict__ gate, double* __restrict__ lPfx) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= b.nc) return;
+    if (t >= b.nc || (gate && *gate == 0)) return;   // no failed inversion in the stage
     const int k = (int)(t / b.s3), j = (int)((t - (long)k * b.s3) / b.s2), i = (int)(t - (long)k * b.s3 - (long)j * b.s2);
     double acc[5] = {0., 0., 0., 0., 0.};
     const int nd = b.ndim;
@@ -363,12 +373,72 @@ adj_fixup_kernel(BlockDims b, StepConsts c, GeomDev gd, const double* __restrict
     for (int v = 0; v < NV; ++v) lPfx[v * b.nc + t] = (v < 5) ? acc[v] : 0.;
 }
 
+// Floors of the stage transpose, eight threads per floored zone (one per
+// Jacobian column): P_inv = u_to_p(U*) recovered again from the stage's U* and
+// guess Psub as the forward update does, then
+// lPinv = (dfloors/dP at P_inv)^T (lP_out + (dU/dP at P_out)^T lU_out)
+// (+ the fixup transpose lPfx when the stage had failed inversions).  Thread 0
+// of each zone also stores P_inv.
+template <int GEO, int FRAME>
+__global__ void __launch_bounds__(ABS)
+adj_floors_prep_kernel(BlockDims b, StepConsts c, GeomDev gd, const double* __restrict__ Pout,
+                       const double* __restrict__ lPout, const double* __restrict__ lUout,
+                       const double* __restrict__ Ustar, const double* __restrict__ Psub, const int* __restrict__ ffs,
+                       const int* __restrict__ pfs, const int* __restrict__ gate, const double* __restrict__ lPfx,
+                       double* __restrict__ Pinv, double* __restrict__ lPinv, int* __restrict__ err) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long n = (long)b.nx1 * (b.ndim > 1 ? b.nx2 : 1) * (b.ndim > 2 ? b.nx3 : 1);
+    const long z = t >> 3;
+    const int col = (int)(t & 7);
+    if (z >= n) return;
+    int k, j, i;
+    interior_zone(b, z, k, j, i);
+    const long x = (long)k * b.s3 + (long)j * b.s2 + i;
+    if (ffs[x] == 0 || (pfs && pfs[x] > 0)) return;
+    const bool fx = lPfx && gate && *gate > 0;
+    double P[NV], lP[NV], lUo[NV], Us[NV], Pi[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        P[v] = Pout[v * b.nc + x];
+        lP[v] = lPout[v * b.nc + x] + (fx ? lPfx[v * b.nc + x] : 0.);
+        lUo[v] = lUout ? lUout[v * b.nc + x] : 0.;
+        Us[v] = Ustar[v * b.nc + x];
+    }
+#pragma unroll
+    for (int v = 0; v < 5; ++v) Pi[v] = Psub[v * b.nc + x];
+    auto body = [&](const auto& g, double rf, double uf) {
+        double t8[NV];
+        const double zero[NV] = {0., 0., 0., 0., 0., 0., 0., 0.};
+        side_vjp<0>(g, P, c.gam, lUo, zero, 0., 0., t8);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) lP[v] += t8[v];
+        const int st = u_to_p_onedw(g, Us, Pi, c.gam, c.iter_max, c.err_tol);
+        const double igd = rcp(g.gdet());
+        Pi[B1] = Us[B1] * igd; Pi[B2] = Us[B2] * igd; Pi[B3] = Us[B3] * igd;
+        lPinv[col * b.nc + x] = floors_vjp_col<FRAME>(g, Pi, c.gam, c.floors, rf, uf, lP, col);
+        if (col == 0) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) Pinv[v * b.nc + x] = Pi[v];
+            if (st % F_GEOM_RHO != 0) atomicAdd(&err[1], 1);
+        }
+    };
+    if constexpr (GEO) {
+        const long q = (long)j * gd.gn1 + i;
+        CachedGeom<double> g(gd.g[GLOC_CENTER], gd.np, q);
+        body(g, gd.flr[q], gd.flr[gd.np + q]);
+    } else {
+        MinkowskiGeom<double> g;
+        body(g, c.floors.rho_min_const, c.floors.u_min_const);
+    }
+}
+
 // Stage transpose at each interior zone.  Unfloored zones: lU* = (dU/dP)^-T
 // lP_out + lU_out at P_out (U_out = p_to_u(P_out) and P_out = u_to_p(U*)).
-// Zones the stage floored (ffs != 0; FRAME >= 0): P_inv = u_to_p(U*) is
-// recovered again from the stage's U* and guess Psub exactly as the forward
-// update does, then lP_inv = (dfloors/dP)^T (lP_out + (dU/dP at P_out)^T lU_out)
-// and lU* = (dU/dP at P_inv)^-T lP_inv.  Then lam dudt = beta dt lU* and the
+// Zones the stage floored (ffs != 0; FRAME >= 0): lU* = (dU/dP at P_inv)^-T
+// lP_inv from adj_floors_prep_kernel's P_inv and lP_inv.  Zones that failed to
+// invert (pfs > 0) pass only their B adjoint to U*.  With failed inversions in
+// the stage (*gate > 0) every zone first adds the fixup transpose lPfx.  Then
+// lam dudt = beta dt lU* and the
 // base-state adjoints (same: U* = U_full).  With lPgeo (curved geometry), the
 // geometric source's transpose at the stage input Psub is written (not added)
 // to lPgeo at the zone.
@@ -377,9 +447,10 @@ __global__ void __launch_bounds__(ABS)
 adj_stage_kernel(BlockDims b, StepConsts c, GeomDev gd, const double* __restrict__ Pout,
                  const double* __restrict__ lPout, const double* __restrict__ lUout, double g0, double g1,
                  double beta_dt, int same, int accumulate, double* lUsub, double* lUfull,
-                 double* __restrict__ ld, int* __restrict__ err, const double* __restrict__ Ustar,
-                 const double* __restrict__ Psub, const int* __restrict__ ffs, double* __restrict__ lPgeo,
-                 const int* __restrict__ pfs, const double* __restrict__ lPfx) {
+                 double* __restrict__ ld, int* __restrict__ err, const double* __restrict__ Pinv,
+                 const double* __restrict__ lPinv, const double* __restrict__ Psub, const int* __restrict__ ffs,
+                 double* __restrict__ lPgeo, const int* __restrict__ pfs, const int* __restrict__ gate,
+                 const double* __restrict__ lPfx) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long n = (long)b.nx1 * (b.ndim > 1 ? b.nx2 : 1) * (b.ndim > 2 ? b.nx3 : 1);
     if (t >= n) return;
@@ -387,10 +458,11 @@ adj_stage_kernel(BlockDims b, StepConsts c, GeomDev gd, const double* __restrict
     interior_zone(b, t, k, j, i);
     const long x = (long)k * b.s3 + (long)j * b.s2 + i;
     double P[NV], lP[NV], lU[NV], lUo[NV];
+    const bool fx = lPfx && gate && *gate > 0;
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
         P[v] = Pout[v * b.nc + x];
-        lP[v] = lPout[v * b.nc + x] + (lPfx ? lPfx[v * b.nc + x] : 0.);
+        lP[v] = lPout[v * b.nc + x] + (fx ? lPfx[v * b.nc + x] : 0.);
         lUo[v] = lUout ? lUout[v * b.nc + x] : 0.;
     }
     const bool failed = pfs && pfs[x] > 0;
@@ -412,22 +484,11 @@ adj_stage_kernel(BlockDims b, StepConsts c, GeomDev gd, const double* __restrict
         }
         if constexpr (FRAME >= 0) {
             if (ffs[x] != 0) {
-                double t8[NV];
-                const double zero[NV] = {0., 0., 0., 0., 0., 0., 0., 0.};
-                side_vjp<0>(g, P, c.gam, lUo, zero, 0., 0., t8);
-                double Us[NV], Pi[NV], lPi[NV];
+                // floors carried through by adj_floors_prep_kernel: lP_inv at P_inv
+                double Pi[NV], lPi[NV];
 #pragma unroll
-                for (int v = 0; v < NV; ++v) { lP[v] += t8[v]; Us[v] = Ustar[v * b.nc + x]; }
-#pragma unroll
-                for (int v = 0; v < 5; ++v) Pi[v] = Psub[v * b.nc + x];
-                const int st = u_to_p_onedw(g, Us, Pi, c.gam, c.iter_max, c.err_tol);
-                const double igd = rcp(g.gdet());
-                Pi[B1] = Us[B1] * igd; Pi[B2] = Us[B2] * igd; Pi[B3] = Us[B3] * igd;
-                double rf, uf;
-                if constexpr (GEO) { const long q = (long)j * gd.gn1 + i; rf = gd.flr[q]; uf = gd.flr[gd.np + q]; }
-                else { rf = c.floors.rho_min_const; uf = c.floors.u_min_const; }
-                floors_vjp<FRAME>(g, Pi, c.gam, c.floors, rf, uf, lP, lPi);
-                ok = (st % F_GEOM_RHO == 0) && inverter_vjp(g, Pi, c.gam, lPi, lU);
+                for (int v = 0; v < NV; ++v) { Pi[v] = Pinv[v * b.nc + x]; lPi[v] = lPinv[v * b.nc + x]; }
+                ok = inverter_vjp(g, Pi, c.gam, lPi, lU);
 #pragma unroll
                 for (int v = 0; v < NV; ++v) lUo[v] = 0.;   // already carried through P_out
                 return;
@@ -552,14 +613,30 @@ adj_bounds_kernel(BlockDims b, GeomDev gd, const double* __restrict__ P, double*
     const bool near1 = p1 && (i < b.is + b.ng || i > b.ie - b.ng);
     const bool near2 = p2 && (j < b.js + b.ng || j > b.je - b.ng);
     if (!near1 && !near2) return;
-    // candidate ghosts: this row / column and the ghost rows / columns of each physical axis
+    // candidate ghosts: this row / column and the ghost columns / rows whose fill
+    // reads this column / row (every composite fill reads its source column from
+    // the X1 rule and its source row from the X2 rule, or column is for the
+    // inner corner fix; ghost_source confirms each candidate)
     int ic[9], jc[9], ni = 0, nj = 0;
     ic[ni++] = i;
     jc[nj++] = j;
     if (near1)
-        for (int g = 0; g < 2 * b.ng; ++g) ic[ni++] = (g < b.ng) ? g : b.ie + 1 + (g - b.ng);
+        for (int g = 0; g < 2 * b.ng; ++g) {
+            const int gi = (g < b.ng) ? g : b.ie + 1 + (g - b.ng);
+            const bool in1 = gi < b.is;
+            const bool refl1 = b.bc[in1 ? 0 : 1] == KH_BC_REFLECT;
+            const int si = refl1 ? (in1 ? 2 * b.is - 1 - gi : 2 * b.ie + 1 - gi) : (in1 ? b.is : b.ie);
+            if (si == i) ic[ni++] = gi;
+        }
     if (near2)
-        for (int g = 0; g < 2 * b.ng; ++g) jc[nj++] = (g < b.ng) ? g : b.je + 1 + (g - b.ng);
+        for (int g = 0; g < 2 * b.ng; ++g) {
+            const int gj = (g < b.ng) ? g : b.je + 1 + (g - b.ng);
+            const bool in2 = gj < b.js;
+            const bool refl2 = b.bc[in2 ? 2 : 3] == KH_BC_REFLECT;
+            const int sj = refl2 ? (in2 ? 2 * b.js - 1 - gj : 2 * b.je + 1 - gj) : (in2 ? b.js : b.je);
+            if (sj == j) jc[nj++] = gj;
+        }
+    if (ni == 1 && nj == 1) return;
     double src[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) src[v] = P[v * b.nc + t];
@@ -694,10 +771,11 @@ __global__ void __launch_bounds__(ABS) adj_seed_kernel(BlockDims b, int dir, int
 // Transpose of the periodic ghost fill: each interior zone collects the
 // adjoints held by its periodic images (up to 3 per axis offset, corners
 // included); ghosts are cleared by adj_clear_ghosts afterwards.
-__global__ void __launch_bounds__(ABS) adj_fold_kernel(BlockDims b, double* __restrict__ A, int nvar) {
+__global__ void __launch_bounds__(ABS) adj_fold_kernel(BlockDims b, double* __restrict__ A, int nvar,
+                                                       const int* __restrict__ gate) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long n = (long)b.nx1 * (b.ndim > 1 ? b.nx2 : 1) * (b.ndim > 2 ? b.nx3 : 1);
-    if (t >= n) return;
+    if (t >= n || (gate && *gate == 0)) return;
     int k, j, i;
     interior_zone(b, t, k, j, i);
     int ni = 1, nj = 1, nk = 1;
@@ -844,16 +922,16 @@ hipError_t launch_adj_flux_t(const BlockDims& b, const StepConsts& c, const Geom
             hipLaunchKernelGGL((adj_face_xt_kernel<2, GEO>), dim3((unsigned)((ncol + TX - 1) / TX), (unsigned)((b.nx2 + 1 + 6) / 7)),
                                dim3(TX, 8), 0, s, b, c, gd, P, lFd, lamL, lamR, klo, nk, b.js, b.je + 1);
             const int nout = b.nx2 + 6;
-            hipLaunchKernelGGL((adj_recon_kernel<2>), dim3((unsigned)((ncol + 63) / 64), (unsigned)((nout + 11) / 12)),
-                               dim3(64, 16), 0, s, b, P, lamL, lamR, lamP, jlo, nj, klo, nk);
+            hipLaunchKernelGGL((adj_recon_kernel<2>), dim3((unsigned)((ncol + 15) / 16), (unsigned)((nout + 11) / 12)),
+                               dim3(16, 16), 0, s, b, P, lamL, lamR, lamP, jlo, nj, klo, nk);
         } else {
             const int ncol = (b.nx1 + 2) * nj;
             constexpr int TX = adj_xt_tx<3, GEO>();
             hipLaunchKernelGGL((adj_face_xt_kernel<3, GEO>), dim3((unsigned)((ncol + TX - 1) / TX), (unsigned)((b.nx3 + 1 + 6) / 7)),
                                dim3(TX, 8), 0, s, b, c, gd, P, lFd, lamL, lamR, jlo, nj, b.ks, b.ke + 1);
             const int nout = b.nx3 + 6;
-            hipLaunchKernelGGL((adj_recon_kernel<3>), dim3((unsigned)((ncol + 63) / 64), (unsigned)((nout + 11) / 12)),
-                               dim3(64, 16), 0, s, b, P, lamL, lamR, lamP, jlo, nj, klo, nk);
+            hipLaunchKernelGGL((adj_recon_kernel<3>), dim3((unsigned)((ncol + 15) / 16), (unsigned)((nout + 11) / 12)),
+                               dim3(16, 16), 0, s, b, P, lamL, lamR, lamP, jlo, nj, klo, nk);
         }
     }
     return hipGetLastError();
@@ -874,16 +952,17 @@ hipError_t launch_adj_flux(const BlockDims& b, const StepConsts& c, const GeomDe
     return launch_adj_flux_t<0>(b, c, gd, P, lF, lamL, lamR, lamP, s);
 }
 
-hipError_t launch_adj_divct(const BlockDims& b, const double* ld, double* lF, double* lemf, hipStream_t s) {
+hipError_t launch_adj_divct(const BlockDims& b, const double* ld, double* lF, double* lemf, hipStream_t s,
+                            FluxSeed seed) {
     const dim3 g = agrid(b.nc);
     if (b.ndim == 3) {
-        hipLaunchKernelGGL(adj_divct_kernel<3>, g, dim3(ABS), 0, s, b, ld, lF, lemf);
+        hipLaunchKernelGGL(adj_divct_kernel<3>, g, dim3(ABS), 0, s, b, ld, lF, lemf, seed);
         hipLaunchKernelGGL(adj_emf_kernel<3>, g, dim3(ABS), 0, s, b, lemf, lF);
     } else if (b.ndim == 2) {
-        hipLaunchKernelGGL(adj_divct_kernel<2>, g, dim3(ABS), 0, s, b, ld, lF, lemf);
+        hipLaunchKernelGGL(adj_divct_kernel<2>, g, dim3(ABS), 0, s, b, ld, lF, lemf, seed);
         hipLaunchKernelGGL(adj_emf_kernel<2>, g, dim3(ABS), 0, s, b, lemf, lF);
     } else {
-        hipLaunchKernelGGL(adj_divct_kernel<1>, g, dim3(ABS), 0, s, b, ld, lF, lemf);
+        hipLaunchKernelGGL(adj_divct_kernel<1>, g, dim3(ABS), 0, s, b, ld, lF, lemf, seed);
     }
     return hipGetLastError();
 }
@@ -891,31 +970,38 @@ hipError_t launch_adj_divct(const BlockDims& b, const double* ld, double* lF, do
 hipError_t launch_adj_stage(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const double* Pout,
                             const double* lPout, const double* lUout, double g0, double g1, double beta_dt, bool same,
                             bool accumulate, double* lUsub, double* lUfull, double* ld, int* err, const double* Ustar,
-                            const double* Psub, const int* ffs, double* lPgeo, const int* pfs, const double* lPfx,
-                            hipStream_t s) {
+                            const double* Psub, const int* ffs, double* lPgeo, const int* pfs, const int* gate,
+                            const double* lPfx, double* Pinv, double* lPinv, hipStream_t s) {
     const dim3 g = agrid(ninterior_l(b));
     const int frame = ffs ? c.floors.frame : -1;
-    if (frame == 1) return hipErrorInvalidValue;
+    if (frame == 1 || (ffs && (!Ustar || !Pinv || !lPinv))) return hipErrorInvalidValue;
+    const dim3 g8 = agrid(8 * ninterior_l(b));
+#define KA_PREP(GEO, FR)                                                                                     \
+    hipLaunchKernelGGL((adj_floors_prep_kernel<GEO, FR>), g8, dim3(ABS), 0, s, b, c, gd, Pout, lPout, lUout, \
+                       Ustar, Psub, ffs, pfs, gate, lPfx, Pinv, lPinv, err)
 #define KA_STAGE(GEO, FR)                                                                                      \
     hipLaunchKernelGGL((adj_stage_kernel<GEO, FR>), g, dim3(ABS), 0, s, b, c, gd, Pout, lPout, lUout, g0, g1,   \
-                       beta_dt, (int)same, (int)accumulate, lUsub, lUfull, ld, err, Ustar, Psub, ffs, lPgeo, pfs, lPfx)
+                       beta_dt, (int)same, (int)accumulate, lUsub, lUfull, ld, err, Pinv, lPinv, Psub, ffs, lPgeo, \
+                       pfs, gate, lPfx)
     if (b.geo) {
-        if (frame == 0) KA_STAGE(1, 0);
-        else if (frame == 2) KA_STAGE(1, 2);
+        if (frame == 0) { KA_PREP(1, 0); KA_STAGE(1, 0); }
+        else if (frame == 2) { KA_PREP(1, 2); KA_STAGE(1, 2); }
         else KA_STAGE(1, -1);
     } else {
-        if (frame == 0) KA_STAGE(0, 0);
-        else if (frame == 2) KA_STAGE(0, 2);
+        if (frame == 0) { KA_PREP(0, 0); KA_STAGE(0, 0); }
+        else if (frame == 2) { KA_PREP(0, 2); KA_STAGE(0, 2); }
         else KA_STAGE(0, -1);
     }
 #undef KA_STAGE
+#undef KA_PREP
     return hipGetLastError();
 }
 
 hipError_t launch_adj_fixup(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const double* Pout,
-                            const double* lPout, const double* lUout, const int* pfs, double* lPfx, hipStream_t s) {
-    if (b.geo) hipLaunchKernelGGL(adj_fixup_kernel<1>, agrid(b.nc), dim3(ABS), 0, s, b, c, gd, Pout, lPout, lUout, pfs, lPfx);
-    else hipLaunchKernelGGL(adj_fixup_kernel<0>, agrid(b.nc), dim3(ABS), 0, s, b, c, gd, Pout, lPout, lUout, pfs, lPfx);
+                            const double* lPout, const double* lUout, const int* pfs, const int* gate, double* lPfx,
+                            hipStream_t s) {
+    if (b.geo) hipLaunchKernelGGL(adj_fixup_kernel<1>, agrid(b.nc), dim3(ABS), 0, s, b, c, gd, Pout, lPout, lUout, pfs, gate, lPfx);
+    else hipLaunchKernelGGL(adj_fixup_kernel<0>, agrid(b.nc), dim3(ABS), 0, s, b, c, gd, Pout, lPout, lUout, pfs, gate, lPfx);
     return hipGetLastError();
 }
 
@@ -968,8 +1054,17 @@ hipError_t launch_adj_ptou(const BlockDims& b, const StepConsts& c, const GeomDe
 }
 
 hipError_t launch_adj_fold(const BlockDims& b, double* A, int nvar, hipStream_t s) {
-    hipLaunchKernelGGL(adj_fold_kernel, agrid(ninterior_l(b)), dim3(ABS), 0, s, b, A, nvar);
+    const bool periodic = b.periodic[0] || (b.ndim > 1 && b.periodic[1]) || (b.ndim > 2 && b.periodic[2]);
+    if (periodic) hipLaunchKernelGGL(adj_fold_kernel, agrid(ninterior_l(b)), dim3(ABS), 0, s, b, A, nvar, nullptr);
     hipLaunchKernelGGL(adj_clear_ghosts_kernel, agrid(b.nc), dim3(ABS), 0, s, b, A, nvar);
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_fold_interior(const BlockDims& b, double* A, int nvar, const int* gate, hipStream_t s) {
+    // periodic axes with images (collapsed axes have none)
+    const bool periodic = b.periodic[0] || (b.ndim > 1 && b.periodic[1]) || (b.ndim > 2 && b.periodic[2]);
+    if (!periodic) return hipSuccess;
+    hipLaunchKernelGGL(adj_fold_kernel, agrid(ninterior_l(b)), dim3(ABS), 0, s, b, A, nvar, gate);
     return hipGetLastError();
 }
 

@@ -497,36 +497,44 @@ __device__ __forceinline__ void geo_source_vjp(const G& g, const GeomDev& gd, lo
 // floors, apply_ceilings; floors/floors_impl.hpp:45-128) at the inverted state
 // Pin: lPin = (dP_out / dPin)^T lPout.  The B components pass through.
 // FRAME 0 (fluid) or 2 (drift).  One dual-number pass per input component.
+// Component k of lPin (one column of the Jacobian, one dual-number pass).
+template <int FRAME, class G>
+__device__ __forceinline__ double floors_vjp_col(const G& g, const double Pin[NV], double gam, const FloorParams& f,
+                                                 double rflr, double uflr, const double lPout[NV], int k) {
+    using ad::Dual;
+    Dual P[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) P[v] = Dual(Pin[v], v == k ? 1. : 0.);
+    FourVec<Dual> D;
+    calc_4vecs<Dual>(g, P, D);
+    Dual rf, uf;
+    const int ff = determine_floors(g, P, dot4(D.bcon, D.bcov), gam, f, rflr, uflr, rf, uf);
+    if (ff) {
+        if constexpr (FRAME == 0) {
+            P[RHO] = P[RHO] + smax(Dual(0.), rf - P[RHO]);
+            P[UU] = P[UU] + smax(Dual(0.), uf - P[UU]);
+        } else {
+            apply_floors_drift(g, P, gam, rf, uf);
+        }
+        apply_ceilings(g, P, gam, f);
+    }
+    double dk = 0.;
+#pragma unroll
+    for (int m = 0; m < 5; ++m) dk += lPout[m] * P[m].d;
+#pragma unroll
+    for (int v = B1; v < NV; ++v)
+        if (v == k) dk += lPout[v];
+    return dk;
+}
 template <int FRAME, class G>
 __device__ __forceinline__ void floors_vjp(const G& g, const double Pin[NV], double gam, const FloorParams& f,
                                            double rflr, double uflr, const double lPout[NV], double lPin[NV]) {
-    using ad::Dual;
-#pragma unroll
-    for (int v = 0; v < NV; ++v) lPin[v] = (v >= B1) ? lPout[v] : 0.;
 #pragma unroll 1
     for (int k = 0; k < NV; ++k) {
-        Dual P[NV];
-#pragma unroll
-        for (int v = 0; v < NV; ++v) P[v] = Dual(Pin[v], v == k ? 1. : 0.);
-        FourVec<Dual> D;
-        calc_4vecs<Dual>(g, P, D);
-        Dual rf, uf;
-        const int ff = determine_floors(g, P, dot4(D.bcon, D.bcov), gam, f, rflr, uflr, rf, uf);
-        if (ff) {
-            if constexpr (FRAME == 0) {
-                P[RHO] = P[RHO] + smax(Dual(0.), rf - P[RHO]);
-                P[UU] = P[UU] + smax(Dual(0.), uf - P[UU]);
-            } else {
-                apply_floors_drift(g, P, gam, rf, uf);
-            }
-            apply_ceilings(g, P, gam, f);
-        }
-        double dk = 0.;
-#pragma unroll
-        for (int m = 0; m < 5; ++m) dk += lPout[m] * P[m].d;
+        const double dk = floors_vjp_col<FRAME>(g, Pin, gam, f, rflr, uflr, lPout, k);
 #pragma unroll
         for (int v = 0; v < NV; ++v)
-            if (v == k) lPin[v] += dk;
+            if (v == k) lPin[v] = dk;
     }
 }
 

@@ -15,6 +15,7 @@
 // objective's seed added to the flux adjoints of each stage.
 #include "kharma_amd.h"
 
+#include <cstddef>
 #include <string>
 
 #include "kh_host.hpp"
@@ -37,6 +38,8 @@ struct kh_adjoint {
     double* x1f[2] = {nullptr, nullptr};     // clamped X1 mass fluxes of each stage (physical X1)
     int* pfs[2] = {nullptr, nullptr};        // inversion flags of each stage (with the fixup)
     double* lPfx = nullptr;                  // [8][nc] fixup transpose
+    double* Pinv = nullptr;                  // [8][nc] recovered primitives of floored zones
+    double* lPinv = nullptr;                 // [8][nc] their adjoints through the floors
     bool geo = false, physical = false, floors = false, fixup = false;
     int obj_dir = 0, obj_var = 0, obj_face = 0;   // face-flux objective (kh_adjoint_set_face_flux_objective)
     double obj_weight = 0.;
@@ -101,34 +104,45 @@ int recompute_stage(kh_adjoint* a, int stage) {
 // lPout holds lam(P_out) (interior); lPin receives lam(P_in) (interior, ghosts
 // zero).  lUout may be null (no adjoint on the stage's output conserved
 // state).  accumulate = false: lUsub / lUfull are overwritten, not added to.
+// clear_ghosts = false leaves ghost entries of lPin stale (interior consumers).
 int stage_transpose(kh_adjoint* a, int stage, const double* Pin, const double* Pout, const double* lPout,
                     const double* lUout, double* lUsub, double* lUfull, bool same, bool accumulate, double dt,
-                    double* lPin) {
+                    double* lPin, bool clear_ghosts) {
     kh_block* b = a->drv->blk;
     const BlockDims& bd = b->dims;
     const StepConsts& c = b->consts;
     hipStream_t s = b->stream;
     const double g0 = c.gam0[stage - 1], g1 = c.gam1[stage - 1], beta_dt = c.beta[stage - 1] * dt;
+    // the stage's count of failed inversions (device), which gates the fixup transpose
+    const int* gate = reinterpret_cast<const int*>(reinterpret_cast<const char*>(a->drv->ds) +
+                                                   offsetof(DeviceState, fail_count)) + (stage - 1);
     if (a->fixup) {
-        KA_HIP(launch_adj_fixup(bd, c, b->geo, Pout, lPout, lUout, a->pfs[stage - 1], a->lPfx, s));
-        KA_HIP(launch_adj_fold(bd, a->lPfx, NV, s));
+        KA_HIP(launch_adj_fixup(bd, c, b->geo, Pout, lPout, lUout, a->pfs[stage - 1], gate, a->lPfx, s));
+        KA_HIP(launch_adj_fold_interior(bd, a->lPfx, NV, gate, s));
     }
     KA_HIP(launch_adj_fill(lPin, (long)NV * bd.nc, 0., s));
     KA_HIP(launch_adj_stage(bd, c, b->geo, Pout, lPout, lUout, g0, g1, beta_dt, same, accumulate, lUsub, lUfull,
                             a->ld, a->err, a->geo ? a->Ustar[stage - 1] : nullptr, Pin,
                             a->floors ? a->ffs[stage - 1] : nullptr, a->geo ? lPin : nullptr,
-                            a->fixup ? a->pfs[stage - 1] : nullptr, a->fixup ? a->lPfx : nullptr, s));
-    KA_HIP(launch_adj_divct(bd, a->ld, a->lF, a->lemf, s));
+                            a->fixup ? a->pfs[stage - 1] : nullptr, a->fixup ? gate : nullptr,
+                            a->fixup ? a->lPfx : nullptr, a->Pinv, a->lPinv, s));
+    FluxSeed seed;
     if (a->obj_dir) {
         const int dir = a->obj_dir;
         const double dA = (dir == 1) ? bd.dx2 * bd.dx3 : (dir == 2) ? bd.dx1 * bd.dx3 : bd.dx1 * bd.dx2;
-        KA_HIP(launch_adj_seed(bd, dir, a->obj_var, a->obj_face, a->obj_weight * stage_flux_weight(c, stage) * dt * dA,
-                               a->lF, s));
+        seed.dir = dir; seed.var = a->obj_var; seed.face = a->obj_face;
+        seed.w = a->obj_weight * stage_flux_weight(c, stage) * dt * dA;
     }
+    KA_HIP(launch_adj_divct(bd, a->ld, a->lF, a->lemf, s, seed));
     if (a->physical) KA_HIP(launch_adj_fix_flux(bd, a->lF, a->x1f[stage - 1], s));
     KA_HIP(launch_adj_flux(bd, c, b->geo, Pin, a->lF, a->lLR, lPin, s));
     if (a->physical) KA_HIP(launch_adj_bounds(bd, b->geo, Pin, lPin, s));
-    KA_HIP(launch_adj_fold(bd, lPin, NV, s));
+    if (clear_ghosts) {
+        KA_HIP(launch_adj_fold(bd, lPin, NV, s));
+    } else {
+        // the next consumer reads the interior only
+        KA_HIP(launch_adj_fold_interior(bd, lPin, NV, nullptr, s));
+    }
     return KH_OK;
 }
 
@@ -177,6 +191,10 @@ int kh_adjoint_create(const kh_params* p, kh_adjoint** out) {
         if (a->fixup && e == hipSuccess) e = hipMemset(a->pfs[q], 0, sizeof(int) * nc);
     }
     if (a->fixup) alloc(&a->lPfx, sb);
+    if (a->floors) {
+        alloc(&a->Pinv, sb);
+        alloc(&a->lPinv, sb);
+    }
     if (e == hipSuccess) e = hipMalloc(&a->err, 2 * sizeof(int));
     for (int q = 0; q < 3 && e == hipSuccess; ++q) e = hipEventCreate(&a->ev[q]);
     if (e == hipSuccess) e = hipDeviceSynchronize();
@@ -192,7 +210,7 @@ void kh_adjoint_destroy(kh_adjoint* a) {
     if (!a) return;
     if (a->drv) (void)hipStreamSynchronize(a->drv->blk->stream);
     for (double* q : {a->P0, a->lPin, a->lU0, a->lU1, a->ld, a->lF, a->lemf, a->lLR, a->Ustar[0], a->Ustar[1],
-                      a->x1f[0], a->x1f[1], a->lPfx})
+                      a->x1f[0], a->x1f[1], a->lPfx, a->Pinv, a->lPinv})
         if (q) (void)hipFree(q);
     for (int* q : {a->ffs[0], a->ffs[1], a->pfs[0], a->pfs[1]})
         if (q) (void)hipFree(q);
@@ -243,10 +261,11 @@ int kh_adjoint_step(kh_adjoint* a, const double* P, const double* P_next, double
     }
     KA_HIP(hipEventRecord(a->ev[1], s));
     // ---- transpose.  stage 2: P1 -> P2 with U_sub = U1, U_full = U0 (writes lU1, lU0)
-    if ((st = stage_transpose(a, 2, d->P1, P2, lam_P, nullptr, a->lU1, a->lU0, false, false, dt, a->lPin)) != KH_OK)
+    if ((st = stage_transpose(a, 2, d->P1, P2, lam_P, nullptr, a->lU1, a->lU0, false, false, dt, a->lPin, false)) !=
+        KH_OK)
         return st;
     // stage 1: P0 -> P1 with U_sub = U_full = U0 (adds to lU0); lam(P_0) lands in lam_P
-    if ((st = stage_transpose(a, 1, P0, d->P1, a->lPin, a->lU1, a->lU0, a->lU0, true, true, dt, lam_P)) != KH_OK)
+    if ((st = stage_transpose(a, 1, P0, d->P1, a->lPin, a->lU1, a->lU0, a->lU0, true, true, dt, lam_P, true)) != KH_OK)
         return st;
     // U_0 = p_to_u(P_0)
     KA_HIP(launch_adj_ptou(bd, c, b->geo, P0, a->lU0, lam_P, s));
@@ -290,7 +309,8 @@ int kh_adjoint_utop(kh_adjoint* a, const double* P, const double* lam_P, double*
     KA_HIP(launch_adj_fill(lam_U, (long)NV * b->dims.nc, 0., b->stream));
     KA_HIP(hipMemsetAsync(a->err, 0, 2 * sizeof(int), b->stream));
     KA_HIP(launch_adj_stage(b->dims, b->consts, b->geo, P, lam_P, nullptr, 0., 1., 0., true, true, lam_U, lam_U,
-                            a->ld, a->err, nullptr, P, nullptr, nullptr, nullptr, nullptr, b->stream));
+                            a->ld, a->err, nullptr, P, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                            b->stream));
     return KH_OK;
 }
 
@@ -349,7 +369,7 @@ int kh_adjoint_fixup(kh_adjoint* a, const double* P_fixed, const int* pflag, dou
     if (!a || !P_fixed || !pflag || !lam_P) return api_fail(KH_ERR_INVALID, "null argument");
     if (!a->fixup) return api_fail(KH_ERR_INVALID, "the fixup is off on this block");
     kh_block* b = a->drv->blk;
-    KA_HIP(launch_adj_fixup(b->dims, b->consts, b->geo, P_fixed, lam_P, nullptr, pflag, a->lPfx, b->stream));
+    KA_HIP(launch_adj_fixup(b->dims, b->consts, b->geo, P_fixed, lam_P, nullptr, pflag, nullptr, a->lPfx, b->stream));
     KA_HIP(launch_adj_fold(b->dims, a->lPfx, NV, b->stream));
     KA_HIP(launch_adj_fixup_apply(b->dims, pflag, a->lPfx, lam_P, b->stream));
     return KH_OK;

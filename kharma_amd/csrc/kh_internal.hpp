@@ -166,24 +166,34 @@ hipError_t launch_adj_fill(double* a, long n, double v, hipStream_t s);
 // lamP += (d F / d P)^T lF over all directions; lamLR is [2][8][nc] scratch.
 hipError_t launch_adj_flux(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const double* P,
                            const double* lF, double* lamLR, double* lamP, hipStream_t s);
-// lF ([3][8][nc]) = (d dudt / d F)^T ld; lemf is [3][nc] scratch.
-hipError_t launch_adj_divct(const BlockDims& b, const double* ld, double* lF, double* lemf, hipStream_t s);
+// A face-flux objective's seed: lF_dir[var] += w on the faces of index `face`
+// along dir over the interior transverse range (dir 0: none).
+struct FluxSeed { int dir = 0, var = 0, face = 0; double w = 0.; };
+// lF ([3][8][nc]) = (d dudt / d F)^T ld (+ seed); lemf is [3][nc] scratch.
+hipError_t launch_adj_divct(const BlockDims& b, const double* ld, double* lF, double* lemf, hipStream_t s,
+                            FluxSeed seed = FluxSeed());
 // accumulate = false overwrites lUsub / lUfull on the interior instead of adding to them.
 // ffs (per-zone floor flags of the stage, or null: no zone floored) with Ustar
-// (the stage's U before recovery) and Psub (its input primitives, the recovery's
-// guess); lPgeo (or null) receives the geometric source's transpose.
+// (the stage's U before recovery), Psub (its input primitives, the recovery's
+// guess) and scratch Pinv, lPinv ([8][nc]); lPgeo (or null) receives the
+// geometric source's transpose; pfs (inversion flags, or null), gate (device
+// count of the stage's failed inversions) and lPfx (launch_adj_fixup's output).
 hipError_t launch_adj_stage(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const double* Pout,
                             const double* lPout, const double* lUout, double g0, double g1, double beta_dt, bool same,
                             bool accumulate, double* lUsub, double* lUfull, double* ld, int* err, const double* Ustar,
-                            const double* Psub, const int* ffs, double* lPgeo, const int* pfs, const double* lPfx,
-                            hipStream_t s);
+                            const double* Psub, const int* ffs, double* lPgeo, const int* pfs, const int* gate,
+                            const double* lPfx, double* Pinv, double* lPinv, hipStream_t s);
 // Transpose of the inversion fixup (fixup_list_kernel): lPfx[8][nc] (every zone;
 // fold periodic images afterwards) = the fluid adjoints that zones failing to
 // invert (pfs > 0) pass to the neighbours they were averaged from.  The stage
 // transpose then adds lPfx (pfs, lPfx non-null) and gives failed zones only
 // their P.B = U*.B / gdet dependence.
 hipError_t launch_adj_fixup(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const double* Pout,
-                            const double* lPout, const double* lUout, const int* pfs, double* lPfx, hipStream_t s);
+                            const double* lPout, const double* lUout, const int* pfs, const int* gate, double* lPfx,
+                            hipStream_t s);
+// Periodic images of A folded onto the interior (no ghost clearing), skipped
+// when *gate == 0 (gate may be null).
+hipError_t launch_adj_fold_interior(const BlockDims& b, double* A, int nvar, const int* gate, hipStream_t s);
 // lP (interior fluid part) <- failed zones 0, others + lPfx.
 hipError_t launch_adj_fixup_apply(const BlockDims& b, const int* pfs, const double* lPfx, double* lP, hipStream_t s);
 // Transpose of the physical-boundary ghost fill of P (bounds_kernel), gathered
