@@ -373,26 +373,22 @@ adj_fixup_kernel(BlockDims b, StepConsts c, GeomDev gd, const double* __restrict
     for (int v = 0; v < NV; ++v) lPfx[v * b.nc + t] = (v < 5) ? acc[v] : 0.;
 }
 
-// Floors of the stage transpose, eight threads per floored zone (one per
-// Jacobian column): P_inv = u_to_p(U*) recovered again from the stage's U* and
-// guess Psub as the forward update does, then
-// lPinv = (dfloors/dP at P_inv)^T (lP_out + (dU/dP at P_out)^T lU_out)
-// (+ the fixup transpose lPfx when the stage had failed inversions).  Thread 0
-// of each zone also stores P_inv.
-template <int GEO, int FRAME>
+// Floors of the stage transpose, first half (one thread per floored zone):
+// P_inv = u_to_p(U*) recovered again from the stage's U* and guess Psub as the
+// forward update does, and lPtot = lP_out + (dU/dP at P_out)^T lU_out (+ the
+// fixup transpose lPfx when the stage had failed inversions), into Pinv, lPtot.
+template <int GEO>
 __global__ void __launch_bounds__(ABS)
 adj_floors_prep_kernel(BlockDims b, StepConsts c, GeomDev gd, const double* __restrict__ Pout,
                        const double* __restrict__ lPout, const double* __restrict__ lUout,
                        const double* __restrict__ Ustar, const double* __restrict__ Psub, const int* __restrict__ ffs,
                        const int* __restrict__ pfs, const int* __restrict__ gate, const double* __restrict__ lPfx,
-                       double* __restrict__ Pinv, double* __restrict__ lPinv, int* __restrict__ err) {
+                       double* __restrict__ Pinv, double* __restrict__ lPtot, int* __restrict__ err) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long n = (long)b.nx1 * (b.ndim > 1 ? b.nx2 : 1) * (b.ndim > 2 ? b.nx3 : 1);
-    const long z = t >> 3;
-    const int col = (int)(t & 7);
-    if (z >= n) return;
+    if (t >= n) return;
     int k, j, i;
-    interior_zone(b, z, k, j, i);
+    interior_zone(b, t, k, j, i);
     const long x = (long)k * b.s3 + (long)j * b.s2 + i;
     if (ffs[x] == 0 || (pfs && pfs[x] > 0)) return;
     const bool fx = lPfx && gate && *gate > 0;
@@ -406,29 +402,55 @@ adj_floors_prep_kernel(BlockDims b, StepConsts c, GeomDev gd, const double* __re
     }
 #pragma unroll
     for (int v = 0; v < 5; ++v) Pi[v] = Psub[v * b.nc + x];
-    auto body = [&](const auto& g, double rf, double uf) {
+    auto body = [&](const auto& g) {
         double t8[NV];
         const double zero[NV] = {0., 0., 0., 0., 0., 0., 0., 0.};
         side_vjp<0>(g, P, c.gam, lUo, zero, 0., 0., t8);
-#pragma unroll
-        for (int v = 0; v < NV; ++v) lP[v] += t8[v];
         const int st = u_to_p_onedw(g, Us, Pi, c.gam, c.iter_max, c.err_tol);
         const double igd = rcp(g.gdet());
         Pi[B1] = Us[B1] * igd; Pi[B2] = Us[B2] * igd; Pi[B3] = Us[B3] * igd;
-        lPinv[col * b.nc + x] = floors_vjp_col<FRAME>(g, Pi, c.gam, c.floors, rf, uf, lP, col);
-        if (col == 0) {
 #pragma unroll
-            for (int v = 0; v < NV; ++v) Pinv[v * b.nc + x] = Pi[v];
-            if (st % F_GEOM_RHO != 0) atomicAdd(&err[1], 1);
+        for (int v = 0; v < NV; ++v) {
+            Pinv[v * b.nc + x] = Pi[v];
+            lPtot[v * b.nc + x] = lP[v] + t8[v];
         }
+        if (st % F_GEOM_RHO != 0) atomicAdd(&err[1], 1);
     };
+    if constexpr (GEO) {
+        CachedGeom<double> g(gd.g[GLOC_CENTER], gd.np, (long)j * gd.gn1 + i);
+        body(g);
+    } else {
+        MinkowskiGeom<double> g;
+        body(g);
+    }
+}
+
+// Second half, eight threads per floored zone (one per Jacobian column):
+// lPinv = (dfloors/dP at P_inv)^T lPtot.
+template <int GEO, int FRAME>
+__global__ void __launch_bounds__(ABS)
+adj_floors_cols_kernel(BlockDims b, StepConsts c, GeomDev gd, const int* __restrict__ ffs, const int* __restrict__ pfs,
+                       const double* __restrict__ Pinv, const double* __restrict__ lPtot, double* __restrict__ lPinv) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long n = (long)b.nx1 * (b.ndim > 1 ? b.nx2 : 1) * (b.ndim > 2 ? b.nx3 : 1);
+    const long z = t >> 3;
+    const int col = (int)(t & 7);
+    if (z >= n) return;
+    int k, j, i;
+    interior_zone(b, z, k, j, i);
+    const long x = (long)k * b.s3 + (long)j * b.s2 + i;
+    if (ffs[x] == 0 || (pfs && pfs[x] > 0)) return;
+    double Pi[NV], lP[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) { Pi[v] = Pinv[v * b.nc + x]; lP[v] = lPtot[v * b.nc + x]; }
     if constexpr (GEO) {
         const long q = (long)j * gd.gn1 + i;
         CachedGeom<double> g(gd.g[GLOC_CENTER], gd.np, q);
-        body(g, gd.flr[q], gd.flr[gd.np + q]);
+        lPinv[col * b.nc + x] = floors_vjp_col<FRAME>(g, Pi, c.gam, c.floors, gd.flr[q], gd.flr[gd.np + q], lP, col);
     } else {
         MinkowskiGeom<double> g;
-        body(g, c.floors.rho_min_const, c.floors.u_min_const);
+        lPinv[col * b.nc + x] =
+            floors_vjp_col<FRAME>(g, Pi, c.gam, c.floors, c.floors.rho_min_const, c.floors.u_min_const, lP, col);
     }
 }
 
@@ -602,12 +624,42 @@ __device__ __forceinline__ GhostSrc ghost_source(const BlockDims& b, int j, int 
 // (within ng of a physical face): the adjoints held by the ghost zones filled
 // from it are mapped back through the fill and added.  Ghosts are cleared
 // afterwards (adj_fold).
+// Zones within ng of a physical face, enumerated: band 1 is the 2 ng interior
+// columns next to the X1 faces (every j, k) when X1 is physical; band 2 the
+// 2 ng interior rows next to the X2 faces over the other columns when X2 is.
+__host__ __device__ inline int x1_band_cols(const BlockDims& b) { return b.periodic[0] ? 0 : 2 * b.ng; }
+__host__ __device__ inline long bounds_band1(const BlockDims& b) { return (long)b.n3 * b.n2 * x1_band_cols(b); }
+__host__ __device__ inline long bounds_band2(const BlockDims& b) {
+    if (b.ndim < 2 || b.periodic[1]) return 0;
+    return (long)b.n3 * 2 * b.ng * (b.n1 - x1_band_cols(b));
+}
 template <int GEO>
 __global__ void __launch_bounds__(ABS)
 adj_bounds_kernel(BlockDims b, GeomDev gd, const double* __restrict__ P, double* __restrict__ A) {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= b.nc) return;
-    const int k = (int)(t / b.s3), j = (int)((t - (long)k * b.s3) / b.s2), i = (int)(t - (long)k * b.s3 - (long)j * b.s2);
+    long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long nb1 = bounds_band1(b);
+    int k, j, i;
+    if (r < nb1) {
+        const int c = (int)(r % x1_band_cols(b));
+        r /= x1_band_cols(b);
+        j = (int)(r % b.n2);
+        k = (int)(r / b.n2);
+        i = (c < b.ng) ? b.is + c : b.ie - (c - b.ng);
+    } else {
+        r -= nb1;
+        if (r >= bounds_band2(b)) return;
+        const int w = b.n1 - x1_band_cols(b);
+        int c = (int)(r % w);
+        r /= w;
+        const int g = (int)(r % (2 * b.ng));
+        k = (int)(r / (2 * b.ng));
+        j = (g < b.ng) ? b.js + g : b.je - (g - b.ng);
+        // the columns outside band 1, in order
+        if (!b.periodic[0] && c >= b.is) c += b.ng;                 // skip is .. is+ng-1
+        if (!b.periodic[0] && c > b.ie - b.ng) c += b.ng;           // skip ie-ng+1 .. ie
+        i = c;
+    }
+    const long t = (long)k * b.s3 + (long)j * b.s2 + i;
     if (phys_ghost(b, j, i)) return;
     const bool p1 = !b.periodic[0], p2 = b.ndim > 1 && !b.periodic[1];
     const bool near1 = p1 && (i < b.is + b.ng || i > b.ie - b.ng);
@@ -971,14 +1023,16 @@ hipError_t launch_adj_stage(const BlockDims& b, const StepConsts& c, const GeomD
                             const double* lPout, const double* lUout, double g0, double g1, double beta_dt, bool same,
                             bool accumulate, double* lUsub, double* lUfull, double* ld, int* err, const double* Ustar,
                             const double* Psub, const int* ffs, double* lPgeo, const int* pfs, const int* gate,
-                            const double* lPfx, double* Pinv, double* lPinv, hipStream_t s) {
+                            const double* lPfx, double* Pinv, double* lPinv, double* lPtot, hipStream_t s) {
     const dim3 g = agrid(ninterior_l(b));
     const int frame = ffs ? c.floors.frame : -1;
-    if (frame == 1 || (ffs && (!Ustar || !Pinv || !lPinv))) return hipErrorInvalidValue;
+    if (frame == 1 || (ffs && (!Ustar || !Pinv || !lPinv || !lPtot))) return hipErrorInvalidValue;
     const dim3 g8 = agrid(8 * ninterior_l(b));
 #define KA_PREP(GEO, FR)                                                                                     \
-    hipLaunchKernelGGL((adj_floors_prep_kernel<GEO, FR>), g8, dim3(ABS), 0, s, b, c, gd, Pout, lPout, lUout, \
-                       Ustar, Psub, ffs, pfs, gate, lPfx, Pinv, lPinv, err)
+    hipLaunchKernelGGL((adj_floors_prep_kernel<GEO>), g, dim3(ABS), 0, s, b, c, gd, Pout, lPout, lUout,       \
+                       Ustar, Psub, ffs, pfs, gate, lPfx, Pinv, lPtot, err);                                 \
+    hipLaunchKernelGGL((adj_floors_cols_kernel<GEO, FR>), g8, dim3(ABS), 0, s, b, c, gd, ffs, pfs, Pinv,     \
+                       lPtot, lPinv)
 #define KA_STAGE(GEO, FR)                                                                                      \
     hipLaunchKernelGGL((adj_stage_kernel<GEO, FR>), g, dim3(ABS), 0, s, b, c, gd, Pout, lPout, lUout, g0, g1,   \
                        beta_dt, (int)same, (int)accumulate, lUsub, lUfull, ld, err, Pinv, lPinv, Psub, ffs, lPgeo, \
@@ -1013,8 +1067,9 @@ hipError_t launch_adj_fixup_apply(const BlockDims& b, const int* pfs, const doub
 hipError_t launch_adj_bounds(const BlockDims& b, const GeomDev& gd, const double* P, double* A, hipStream_t s) {
     if (b.periodic[0] && (b.ndim < 2 || b.periodic[1])) return hipSuccess;
     if (b.ng > 4) return hipErrorInvalidValue;
-    if (b.geo) hipLaunchKernelGGL(adj_bounds_kernel<1>, agrid(b.nc), dim3(ABS), 0, s, b, gd, P, A);
-    else hipLaunchKernelGGL(adj_bounds_kernel<0>, agrid(b.nc), dim3(ABS), 0, s, b, gd, P, A);
+    const long n = bounds_band1(b) + bounds_band2(b);
+    if (b.geo) hipLaunchKernelGGL(adj_bounds_kernel<1>, agrid(n), dim3(ABS), 0, s, b, gd, P, A);
+    else hipLaunchKernelGGL(adj_bounds_kernel<0>, agrid(n), dim3(ABS), 0, s, b, gd, P, A);
     return hipGetLastError();
 }
 

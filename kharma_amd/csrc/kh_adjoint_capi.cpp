@@ -1,7 +1,8 @@
 // kh_adjoint_capi.cpp -- C ABI of the adjoint step (include/kharma_amd.h).
 //
 // kh_adjoint_step recomputes the forward step from its checkpointed starting
-// primitives on an internal driver (stage 1 -> P1, U1; stage 2 -> P2, U2),
+// primitives on an internal driver (stage 1 -> P1, U1; stage 2 -> P2, U2 of
+// the adjoint's own buffers),
 // keeping what the transposes need: each stage's U* (the update before
 // recovery) and floor flags on the curved-space path, and the clamped X1 mass
 // fluxes where X1 is physical.  Then it runs the stage transposes of
@@ -24,7 +25,8 @@ using namespace kh;
 
 struct kh_adjoint {
     kh_driver* drv = nullptr;
-    double* P0 = nullptr;      // copy of the step's starting primitives when stage 2 is recomputed
+    double* P2 = nullptr;      // stage 2's recomputed primitives and conserved state (the driver's
+    double* U2 = nullptr;      // P0, U0 keep the step's starting state)
     double* lPin = nullptr;    // adjoint of the stage-2 input primitives (P1)
     double* lU0 = nullptr;     // adjoint of U_0 (the step's starting conserved state)
     double* lU1 = nullptr;     // adjoint of U_1
@@ -40,6 +42,7 @@ struct kh_adjoint {
     double* lPfx = nullptr;                  // [8][nc] fixup transpose
     double* Pinv = nullptr;                  // [8][nc] recovered primitives of floored zones
     double* lPinv = nullptr;                 // [8][nc] their adjoints through the floors
+    double* lPtot = nullptr;                 // [8][nc] their output adjoints, P->U included
     bool geo = false, physical = false, floors = false, fixup = false;
     int obj_dir = 0, obj_var = 0, obj_face = 0;   // face-flux objective (kh_adjoint_set_face_flux_objective)
     double obj_weight = 0.;
@@ -66,8 +69,11 @@ int recompute_stage(kh_adjoint* a, int stage) {
     hipStream_t s = b->stream;
     const double* Psub = (stage == 1) ? d->P0 : d->P1;
     const double* Usub = (stage == 1) ? d->U0 : d->U1;
-    double* Pout = (stage == 1) ? d->P1 : d->P0;
-    double* Uout = (stage == 1) ? d->U1 : d->U0;
+    double* Pout = (stage == 1) ? d->P1 : a->P2;
+    double* Uout = (stage == 1) ? d->U1 : a->U2;
+    // the stage's flags go straight to the buffers the transpose reads
+    int* pflag = a->fixup ? a->pfs[stage - 1] : d->pflag;
+    int* fflag = a->floors ? a->ffs[stage - 1] : d->fflag;
     double* F = (double*)d->F;
     double* ct = (double*)d->ctop;
     for (int dir = 1; dir <= bd.ndim; ++dir)
@@ -78,23 +84,20 @@ int recompute_stage(kh_adjoint* a, int stage) {
         KA_HIP(launch_save_x1_flux(bd, F, a->x1f[stage - 1], s));
     }
     if (a->geo) {
-        KA_HIP(launch_update_parts(bd, c, b->geo, stage, 1, F, Usub, d->U0, Psub, Uout, Pout, d->pflag, d->fflag,
+        KA_HIP(launch_update_parts(bd, c, b->geo, stage, 1, F, Usub, d->U0, Psub, Uout, Pout, pflag, fflag,
                                    d->fail_list, d->ds, s));
         KA_HIP(hipMemcpyAsync(a->Ustar[stage - 1], Uout, sizeof(double) * NV * bd.nc, hipMemcpyDeviceToDevice, s));
-        KA_HIP(launch_update_parts(bd, c, b->geo, stage, 2, F, Usub, d->U0, Psub, Uout, Pout, d->pflag, d->fflag,
+        KA_HIP(launch_update_parts(bd, c, b->geo, stage, 2, F, Usub, d->U0, Psub, Uout, Pout, pflag, fflag,
                                    d->fail_list, d->ds, s));
-        if (a->floors)
-            KA_HIP(hipMemcpyAsync(a->ffs[stage - 1], d->fflag, sizeof(int) * bd.nc, hipMemcpyDeviceToDevice, s));
     } else {
         KA_HIP(launch_update(bd, c, b->geo, b->p.precision, stage, false, F, ct, Usub, d->U0, Psub, Uout, Pout,
-                             nullptr, d->pflag, d->fflag, d->fail_list, d->ds, d->dt_part, s));
+                             nullptr, pflag, fflag, d->fail_list, d->ds, d->dt_part, s));
     }
     if (a->fixup) {
-        KA_HIP(hipMemcpyAsync(a->pfs[stage - 1], d->pflag, sizeof(int) * bd.nc, hipMemcpyDeviceToDevice, s));
-        KA_HIP(launch_fixup(bd, c, b->geo, stage, Uout, Pout, nullptr, d->pflag, d->fail_list, d->ds, s));
-        if (!a->floors) KA_HIP(launch_adj_count_flags(bd, d->fflag, nullptr, a->err, s));
+        KA_HIP(launch_fixup(bd, c, b->geo, stage, Uout, Pout, nullptr, pflag, d->fail_list, d->ds, s));
+        if (!a->floors) KA_HIP(launch_adj_count_flags(bd, fflag, nullptr, a->err, s));
     } else {
-        KA_HIP(launch_adj_count_flags(bd, d->pflag, a->floors ? nullptr : d->fflag, a->err, s));
+        KA_HIP(launch_adj_count_flags(bd, pflag, a->floors ? nullptr : fflag, a->err, s));
     }
     if (a->physical) KA_HIP(launch_bounds(bd, c, b->geo, Uout, Pout, nullptr, s));
     return KH_OK;
@@ -125,7 +128,7 @@ int stage_transpose(kh_adjoint* a, int stage, const double* Pin, const double* P
                             a->ld, a->err, a->geo ? a->Ustar[stage - 1] : nullptr, Pin,
                             a->floors ? a->ffs[stage - 1] : nullptr, a->geo ? lPin : nullptr,
                             a->fixup ? a->pfs[stage - 1] : nullptr, a->fixup ? gate : nullptr,
-                            a->fixup ? a->lPfx : nullptr, a->Pinv, a->lPinv, s));
+                            a->fixup ? a->lPfx : nullptr, a->Pinv, a->lPinv, a->lPtot, s));
     FluxSeed seed;
     if (a->obj_dir) {
         const int dir = a->obj_dir;
@@ -175,7 +178,8 @@ int kh_adjoint_create(const kh_params* p, kh_adjoint** out) {
         if (e == hipSuccess) e = hipMalloc(q, bytes);
         if (e == hipSuccess) e = hipMemset(*q, 0, bytes);
     };
-    alloc(&a->P0, sb);
+    alloc(&a->P2, sb);
+    alloc(&a->U2, sb);
     alloc(&a->lPin, sb);
     alloc(&a->lU0, sb);
     alloc(&a->lU1, sb);
@@ -194,6 +198,7 @@ int kh_adjoint_create(const kh_params* p, kh_adjoint** out) {
     if (a->floors) {
         alloc(&a->Pinv, sb);
         alloc(&a->lPinv, sb);
+        alloc(&a->lPtot, sb);
     }
     if (e == hipSuccess) e = hipMalloc(&a->err, 2 * sizeof(int));
     for (int q = 0; q < 3 && e == hipSuccess; ++q) e = hipEventCreate(&a->ev[q]);
@@ -209,8 +214,8 @@ int kh_adjoint_create(const kh_params* p, kh_adjoint** out) {
 void kh_adjoint_destroy(kh_adjoint* a) {
     if (!a) return;
     if (a->drv) (void)hipStreamSynchronize(a->drv->blk->stream);
-    for (double* q : {a->P0, a->lPin, a->lU0, a->lU1, a->ld, a->lF, a->lemf, a->lLR, a->Ustar[0], a->Ustar[1],
-                      a->x1f[0], a->x1f[1], a->lPfx, a->Pinv, a->lPinv})
+    for (double* q : {a->P2, a->U2, a->lPin, a->lU0, a->lU1, a->ld, a->lF, a->lemf, a->lLR, a->Ustar[0], a->Ustar[1],
+                      a->x1f[0], a->x1f[1], a->lPfx, a->Pinv, a->lPinv, a->lPtot})
         if (q) (void)hipFree(q);
     for (int* q : {a->ffs[0], a->ffs[1], a->pfs[0], a->pfs[1]})
         if (q) (void)hipFree(q);
@@ -240,7 +245,8 @@ int kh_adjoint_step(kh_adjoint* a, const double* P, const double* P_next, double
     // ---- forward recompute from the checkpoint: P_0 (ghosts as checkpointed; periodic
     // images refilled), U_0 = p_to_u(P_0), stage 1
     KA_HIP(hipMemcpyAsync(d->P0, P, sb, hipMemcpyDeviceToDevice, s));
-    KA_HIP(launch_sync(bd, d->P0, NV, s));
+    if (bd.periodic[0] || (bd.ndim > 1 && bd.periodic[1]) || (bd.ndim > 2 && bd.periodic[2]))
+        KA_HIP(launch_sync(bd, d->P0, NV, s));
     KA_HIP(launch_ptou(bd, c, b->geo, d->P0, d->U0, s));
     KA_HIP(launch_reset_state(d->ds, dt, s));
     int st = recompute_stage(a, 1);                          // -> P1, U1
@@ -250,11 +256,8 @@ int kh_adjoint_step(kh_adjoint* a, const double* P, const double* P_next, double
     // the curved-space and physical-boundary transposes need stage 2's U*, flags
     // and clamped fluxes: recompute it unless a plain periodic step can use P_next
     if (!P2 || a->geo || a->physical) {
-        // stage 2 writes P2, U2 over the driver's P0, U0: keep P_0
-        KA_HIP(hipMemcpyAsync(a->P0, d->P0, sb, hipMemcpyDeviceToDevice, s));
-        P0 = a->P0;
-        if ((st = recompute_stage(a, 2)) != KH_OK) return st;
-        P2 = d->P0;
+        if ((st = recompute_stage(a, 2)) != KH_OK) return st;   // -> a->P2, a->U2
+        P2 = a->P2;
     } else if (a->fixup) {
         // stage 2 taken from P_next: the caller vouches that no zone of it failed
         KA_HIP(hipMemsetAsync(a->pfs[1], 0, sizeof(int) * bd.nc, s));
@@ -310,7 +313,7 @@ int kh_adjoint_utop(kh_adjoint* a, const double* P, const double* lam_P, double*
     KA_HIP(hipMemsetAsync(a->err, 0, 2 * sizeof(int), b->stream));
     KA_HIP(launch_adj_stage(b->dims, b->consts, b->geo, P, lam_P, nullptr, 0., 1., 0., true, true, lam_U, lam_U,
                             a->ld, a->err, nullptr, P, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
-                            b->stream));
+                            nullptr, b->stream));
     return KH_OK;
 }
 

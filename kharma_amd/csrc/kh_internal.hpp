@@ -175,14 +175,14 @@ hipError_t launch_adj_divct(const BlockDims& b, const double* ld, double* lF, do
 // accumulate = false overwrites lUsub / lUfull on the interior instead of adding to them.
 // ffs (per-zone floor flags of the stage, or null: no zone floored) with Ustar
 // (the stage's U before recovery), Psub (its input primitives, the recovery's
-// guess) and scratch Pinv, lPinv ([8][nc]); lPgeo (or null) receives the
+// guess) and scratch Pinv, lPinv, lPtot ([8][nc]); lPgeo (or null) receives the
 // geometric source's transpose; pfs (inversion flags, or null), gate (device
 // count of the stage's failed inversions) and lPfx (launch_adj_fixup's output).
 hipError_t launch_adj_stage(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const double* Pout,
                             const double* lPout, const double* lUout, double g0, double g1, double beta_dt, bool same,
                             bool accumulate, double* lUsub, double* lUfull, double* ld, int* err, const double* Ustar,
                             const double* Psub, const int* ffs, double* lPgeo, const int* pfs, const int* gate,
-                            const double* lPfx, double* Pinv, double* lPinv, hipStream_t s);
+                            const double* lPfx, double* Pinv, double* lPinv, double* lPtot, hipStream_t s);
 // Transpose of the inversion fixup (fixup_list_kernel): lPfx[8][nc] (every zone;
 // fold periodic images afterwards) = the fluid adjoints that zones failing to
 // invert (pfs > 0) pass to the neighbours they were averaged from.  The stage
