@@ -11,12 +11,10 @@
 // held fixed.  Each kernel writes every output element it owns exactly once
 // (gathers, no atomics), so the adjoint is deterministic.
 //
-// Face adjoints:  adj_face_* recompute the face states of the forward tiles and
-// write, per cell m, lamL[v][m] (adjoint of the value at m's left face, from
-// face m) and lamR[v][m] (at m's right face, from face m + 1).  adj_recon then
-// applies the WENO5 transpose per cell and gathers the 5-point contributions
-// into lamP through LDS; it reads lamL / lamR only where those faces exist, so
-// the scratch needs no clearing.
+// Face adjoints:  adj_flux_*_fused recompute the face states of a tile of
+// cells along one direction, take the adjoint of each face, keep the two face
+// adjoints of every cell in LDS and apply the WENO5 transpose, gathering the
+// 5-point contributions into lamP; no face-state adjoints go through HBM.
 #include "kh_internal.hpp"
 #include "kh_adjoint.hpp"
 
@@ -47,161 +45,169 @@ __device__ __forceinline__ void with_face_geom(const StepConsts& c, const GeomDe
     }
 }
 
-template <int DIR, int GEO>
-__device__ __forceinline__ void face_adjoint(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int j, int i,
-                                             long ci, long st, const double Pl[NV], const double Pr[NV],
-                                             const double* __restrict__ lF, double* __restrict__ lamL,
-                                             double* __restrict__ lamR) {
-    double lf[NV], lPl[NV], lPr[NV];
-#pragma unroll
-    for (int v = 0; v < NV; ++v) lf[v] = lF[v * b.nc + ci];
-    with_face_geom<DIR, GEO>(c, gd, j, i, [&](const auto& g, double rflr, double uflr) {
-        face_vjp_llf<DIR>(g, c.gam, c.recon_floors != 0, rflr, uflr, Pl, Pr, lf, lPl, lPr);
-    });
-#pragma unroll
-    for (int v = 0; v < NV; ++v) {
-        lamL[v * b.nc + ci] = lPr[v];
-        lamR[v * b.nc + ci - st] = lPl[v];
-    }
-}
-
-// X1 faces: the tile scheme of flux_x1_kernel (flattened rows, 256 cells per
-// block, consecutive blocks overlapping by one cell).
+// Face adjoints and WENO5 transpose fused (no face-state adjoint arrays), X1.
+// Rows (j, k) are laid end to end as padded segments of cells [is-5, ie+5]
+// (length nx1 + 10); a workgroup of 256 threads covers 256 consecutive slots
+// (from 3 before its 250 output slots), one cell each.  Per slot:
+// reconstruct the cell (cells [is-1, ie+1]), take the adjoint of the face at its
+// left (faces [is, ie+1]; the left state comes from the neighbour slot through
+// LDS), apply the WENO5 transpose with the adjoints of both its faces, and
+// gather the 5-point contributions into lamP for the output cells [is-3, ie+3]
+// whose whole neighbourhood lies in the workgroup (slots 3 .. 252).
 template <int GEO>
 __global__ void __launch_bounds__(256)
-adj_face_x1_kernel(BlockDims b, StepConsts c, GeomDev gd, const double* __restrict__ P, const double* __restrict__ lF,
-                   double* __restrict__ lamL, double* __restrict__ lamR, int j0, int nj, int k0, int nk) {
-    __shared__ double sr[NV][256];
-    const int L = b.nx1 + 2;
+adj_flux_x1_fused_kernel(BlockDims b, StepConsts c, GeomDev gd, const double* __restrict__ P,
+                         const double* __restrict__ lF, double* __restrict__ lamP, int j0, int nj, int k0, int nk) {
+    __shared__ double sro[NV][256];   // right face value of each slot's cell, then the adjoint of it
+    __shared__ double G[5][256];
+    const int Lp = b.nx1 + 10;
     const int tid = threadIdx.x;
-    const long g = (long)blockIdx.x * 255 + tid;
-    const bool valid = g < (long)L * nj * nk;
-    int ii = 0, jf = 0;
+    const long g = (long)blockIdx.x * 250 + tid - 3;   // workgroup b outputs slots [250 b, 250 b + 250)
+    const long nslots = (long)Lp * nj * nk;
+    const bool valid = g >= 0 && g < nslots;
+    int m = 0, jf = 0;
     long ci = 0;
-    double lo[NV];
     if (valid) {
-        const int row = (int)(g / L);
-        ii = (int)(g - (long)row * L);
+        const int row = (int)(g / Lp);
+        m = b.is - 5 + (int)(g - (long)row * Lp);
         const int jj = row % nj, kk = row / nj;
         jf = j0 + jj;
-        ci = (long)(k0 + kk) * b.s3 + (long)(j0 + jj) * b.s2 + (b.is - 1 + ii);
+        ci = (long)(k0 + kk) * b.s3 + (long)jf * b.s2 + m;
+    }
+    const bool recon = valid && m >= b.is - 1 && m <= b.ie + 1;
+    double lo[NV], ll[NV];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) {
+    for (int v = 0; v < NV; ++v) {
+        double ro = 0.;
+        lo[v] = 0.;
+        if (recon) {
             const double* q = P + v * b.nc + ci;
-            double ro;
             weno5_cell(q[-2], q[-1], q[0], q[1], q[2], lo[v], ro);
-            sr[v][tid] = ro;
         }
+        sro[v][tid] = ro;
+        ll[v] = 0.;
     }
     __syncthreads();
-    if (!valid || tid == 0 || ii == 0) return;
-    double Pl[NV];
+    // the face at this cell's left: its adjoint splits into lamL (this cell) and
+    // lamR (the left neighbour, written back into sro after every read of it)
+    const bool face = valid && tid > 0 && m >= b.is && m <= b.ie + 1;
+    double lPl[NV];
+    if (face) {
+        double Pl[NV];
 #pragma unroll
-    for (int v = 0; v < NV; ++v) Pl[v] = sr[v][tid - 1];
-    face_adjoint<1, GEO>(b, c, gd, jf, b.is - 1 + ii, ci, 1, Pl, lo, lF, lamL, lamR);
-}
-
-// X2 / X3 faces: TX x 8 tiles as in flux_xt_kernel (TX = 64 in flat space; 32
-// with the curved metric, whose face adjoint needs more than the 256 registers
-// a wave gets with two waves per SIMD).
-template <int DIR, int GEO>
-constexpr int adj_xt_tx() { return GEO ? 32 : 64; }
-template <int DIR, int GEO>
-__global__ void __launch_bounds__(GEO ? 256 : 512)
-adj_face_xt_kernel(BlockDims b, StepConsts c, GeomDev gd, const double* __restrict__ P, const double* __restrict__ lF,
-                   double* __restrict__ lamL, double* __restrict__ lamR, int o0, int no, int s0, int s1) {
-    constexpr int TX = adj_xt_tx<DIR, GEO>();
-    __shared__ double sr[NV][8][TX];
-    const int ni = b.nx1 + 2;
-    const int tx = threadIdx.x, ty = threadIdx.y;
-    const int col = blockIdx.x * TX + tx;
-    const int s = s0 - 1 + (int)blockIdx.y * 7 + ty;
-    const bool valid = col < ni * no && s <= s1;
-    const long st = (DIR == 2) ? b.s2 : b.s3;
-    long ci = 0;
-    int jf = 0, if_ = 0;
-    double lo[NV];
-    if (valid) {
-        const int outer = o0 + col / ni;
-        const int i = b.is - 1 + col % ni;
-        const int j = (DIR == 2) ? s : outer;
-        const int k = (DIR == 2) ? outer : s;
-        jf = j; if_ = i;
-        ci = (long)k * b.s3 + (long)j * b.s2 + i;
+        for (int v = 0; v < NV; ++v) Pl[v] = sro[v][tid - 1];
+        double lf[NV];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            const double* q = P + v * b.nc + ci;
-            double ro;
-            weno5_cell(q[-2 * st], q[-st], q[0], q[st], q[2 * st], lo[v], ro);
-            sr[v][ty][tx] = ro;
-        }
+        for (int v = 0; v < NV; ++v) lf[v] = lF[v * b.nc + ci];
+        with_face_geom<1, GEO>(c, gd, jf, m, [&](const auto& gg, double rflr, double uflr) {
+            face_vjp_llf<1>(gg, c.gam, c.recon_floors != 0, rflr, uflr, Pl, lo, lf, lPl, ll);
+        });
     }
     __syncthreads();
-    if (!valid || ty == 0) return;
-    double Pl[NV];
 #pragma unroll
-    for (int v = 0; v < NV; ++v) Pl[v] = sr[v][ty - 1][tx];
-    face_adjoint<DIR, GEO>(b, c, gd, jf, if_, ci, st, Pl, lo, lF, lamL, lamR);
-}
-
-// WENO5 transpose and 5-point gather along DIR, accumulated into lamP.
-// Cells m along DIR in [s-1, e+1] were reconstructed by the forward pass; their
-// stencils, and so the gather, cover cells [s-3, e+3].  Tiles hold TN cells along DIR and emit the
-// TN - 4 cells whose whole +-2 neighbourhood lies inside the tile.
-// DIR = 1: block (64, 4), x along i, y over transverse rows (j, k).
-// DIR = 2, 3: block (16, 16), x over transverse columns (outer, i), y along DIR.
-template <int DIR>
-__global__ void __launch_bounds__(256)
-adj_recon_kernel(BlockDims b, const double* __restrict__ P, const double* __restrict__ lamL,
-                 const double* __restrict__ lamR, double* __restrict__ lamP, int j0, int nj, int k0, int nk) {
-    constexpr int TN = (DIR == 1) ? 64 : 16;
-    constexpr int TT = (DIR == 1) ? 4 : 16;
-    __shared__ double G[5][TN][TT];
-    int tn, tt, n, i = 0, j = 0, k = 0;
-    bool tvalid;
-    long st;
-    int s, e;
-    if (DIR == 1) {
-        tn = threadIdx.x; tt = threadIdx.y;
-        s = b.is; e = b.ie; st = 1;
-        n = s - 3 - 2 + (int)blockIdx.x * (TN - 4) + tn;
-        const int row = (int)blockIdx.y * TT + tt;
-        tvalid = row < nj * nk;
-        j = j0 + (tvalid ? row % nj : 0);
-        k = k0 + (tvalid ? row / nj : 0);
-        i = n;
-    } else {
-        tt = threadIdx.x; tn = threadIdx.y;
-        const int ni = b.nx1 + 2;
-        const int col = (int)blockIdx.x * TT + tt;
-        const int no = (DIR == 2) ? nk : nj;
-        tvalid = col < ni * no;
-        const int outer = (DIR == 2 ? k0 : j0) + (tvalid ? col / ni : 0);
-        i = b.is - 1 + (tvalid ? col % ni : 0);
-        if (DIR == 2) { s = b.js; e = b.je; st = b.s2; k = outer; }
-        else { s = b.ks; e = b.ke; st = b.s3; j = outer; }
-        n = s - 3 - 2 + (int)blockIdx.y * (TN - 4) + tn;
-        if (DIR == 2) j = n; else k = n;
+    for (int v = 0; v < NV; ++v) sro[v][tid] = 0.;
+    __syncthreads();
+    if (face) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) sro[v][tid - 1] = lPl[v];
     }
-    const bool recon = tvalid && n >= s - 1 && n <= e + 1;
-    const bool out = tvalid && tn >= 2 && tn < TN - 2 && n >= s - 3 && n <= e + 3;
-    const long c = recon || out ? (long)k * b.s3 + (long)j * b.s2 + i : 0;
-#pragma unroll 1
+    __syncthreads();
+    const bool out = valid && tid >= 3 && tid < 253 && m >= b.is - 3 && m <= b.ie + 3;
+#pragma unroll
     for (int v = 0; v < NV; ++v) {
         double g5[5] = {0., 0., 0., 0., 0.};
         if (recon) {
-            const double* q = P + v * b.nc + c;
-            // the left face of cell n is face n (exists for n >= s), its right face is n + 1 (n <= e)
-            const double ll = (n >= s) ? lamL[v * b.nc + c] : 0.;
-            const double lr = (n <= e) ? lamR[v * b.nc + c] : 0.;
-            weno5_cell_vjp(q[-2 * st], q[-st], q[0], q[st], q[2 * st], ll, lr, g5);
+            const double* q = P + v * b.nc + ci;
+            weno5_cell_vjp(q[-2], q[-1], q[0], q[1], q[2], ll[v], sro[v][tid], g5);
         }
 #pragma unroll
-        for (int o = 0; o < 5; ++o) G[o][tn][tt] = g5[o];
+        for (int o = 0; o < 5; ++o) G[o][tid] = g5[o];
         __syncthreads();
         if (out) {
-            const double a = G[4][tn - 2][tt] + G[3][tn - 1][tt] + G[2][tn][tt] + G[1][tn + 1][tt] + G[0][tn + 2][tt];
-            lamP[v * b.nc + c] += a;
+            const double a = G[4][tid - 2] + G[3][tid - 1] + G[2][tid] + G[1][tid + 1] + G[0][tid + 2];
+            lamP[v * b.nc + ci] += a;
+        }
+        __syncthreads();
+    }
+}
+
+// The same for X2 / X3: tiles of 8 transverse columns (outer, i) x 32 slots
+// along DIR; the slots of a column are the cells [s-5, e+5] along DIR, and a
+// tile outputs its 26 slots 3 .. 28 (cells [s-3, e+3]).
+template <int DIR, int GEO>
+__global__ void __launch_bounds__(256)
+adj_flux_xt_fused_kernel(BlockDims b, StepConsts c, GeomDev gd, const double* __restrict__ P,
+                         const double* __restrict__ lF, double* __restrict__ lamP, int o0, int no) {
+    constexpr int TX = 8, TY = 32;
+    __shared__ double sro[NV][TY][TX];
+    __shared__ double G[5][TY][TX];
+    const int ni = b.nx1 + 2;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int col = blockIdx.x * TX + tx;
+    const int s0 = (DIR == 2) ? b.js : b.ks, e0 = (DIR == 2) ? b.je : b.ke;
+    const int n = s0 - 5 + (int)blockIdx.y * (TY - 6) + ty - 3;
+    const bool valid = col < ni * no && n >= s0 - 5 && n <= e0 + 5;
+    const long st = (DIR == 2) ? b.s2 : b.s3;
+    int jf = 0, if_ = 0;
+    long ci = 0;
+    if (valid) {
+        const int outer = o0 + col / ni;
+        const int i = b.is - 1 + col % ni;
+        const int j = (DIR == 2) ? n : outer;
+        const int k = (DIR == 2) ? outer : n;
+        jf = j; if_ = i;
+        ci = (long)k * b.s3 + (long)j * b.s2 + i;
+    }
+    const bool recon = valid && n >= s0 - 1 && n <= e0 + 1;
+    double lo[NV], ll[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        double ro = 0.;
+        lo[v] = 0.;
+        if (recon) {
+            const double* q = P + v * b.nc + ci;
+            weno5_cell(q[-2 * st], q[-st], q[0], q[st], q[2 * st], lo[v], ro);
+        }
+        sro[v][ty][tx] = ro;
+        ll[v] = 0.;
+    }
+    __syncthreads();
+    const bool face = valid && ty > 0 && n >= s0 && n <= e0 + 1;
+    double lPl[NV];
+    if (face) {
+        double Pl[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) Pl[v] = sro[v][ty - 1][tx];
+        double lf[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) lf[v] = lF[v * b.nc + ci];
+        with_face_geom<DIR, GEO>(c, gd, jf, if_, [&](const auto& gg, double rflr, double uflr) {
+            face_vjp_llf<DIR>(gg, c.gam, c.recon_floors != 0, rflr, uflr, Pl, lo, lf, lPl, ll);
+        });
+    }
+    __syncthreads();
+#pragma unroll
+    for (int v = 0; v < NV; ++v) sro[v][ty][tx] = 0.;
+    __syncthreads();
+    if (face) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) sro[v][ty - 1][tx] = lPl[v];
+    }
+    __syncthreads();
+    const bool out = valid && ty >= 3 && ty < TY - 3 && n >= s0 - 3 && n <= e0 + 3;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        double g5[5] = {0., 0., 0., 0., 0.};
+        if (recon) {
+            const double* q = P + v * b.nc + ci;
+            weno5_cell_vjp(q[-2 * st], q[-st], q[0], q[st], q[2 * st], ll[v], sro[v][ty][tx], g5);
+        }
+#pragma unroll
+        for (int o = 0; o < 5; ++o) G[o][ty][tx] = g5[o];
+        __syncthreads();
+        if (out) {
+            const double a = G[4][ty - 2][tx] + G[3][ty - 1][tx] + G[2][ty][tx] + G[1][ty + 1][tx] + G[0][ty + 2][tx];
+            lamP[v * b.nc + ci] += a;
         }
         __syncthreads();
     }
@@ -823,13 +829,62 @@ __global__ void __launch_bounds__(ABS) adj_seed_kernel(BlockDims b, int dir, int
 // Transpose of the periodic ghost fill: each interior zone collects the
 // adjoints held by its periodic images (up to 3 per axis offset, corners
 // included); ghosts are cleared by adj_clear_ghosts afterwards.
+// The interior zones that have periodic images: those within ng of a periodic
+// face, enumerated by the outermost axis at which they are (k, then j, then i).
+// Valid when every periodic axis holds at least 2 ng zones (else the two edges
+// overlap and adj_fold_kernel walks the whole interior).
+struct FoldShell {
+    int p0, p1, p2;             // periodic axes with images
+    long n1, n2, n3;            // zones in regions 1 (k edge), 2 (j edge), 3 (i edge)
+};
+inline FoldShell fold_shell(const BlockDims& b) {
+    FoldShell f;
+    f.p0 = b.periodic[0];
+    f.p1 = b.ndim > 1 && b.periodic[1];
+    f.p2 = b.ndim > 2 && b.periodic[2];
+    const long nx1 = b.nx1, nx2 = b.ndim > 1 ? b.nx2 : 1, nx3 = b.ndim > 2 ? b.nx3 : 1;
+    const long mk = nx3 - (f.p2 ? 2 * b.ng : 0), mj = nx2 - (f.p1 ? 2 * b.ng : 0);
+    f.n1 = f.p2 ? 2L * b.ng * nx2 * nx1 : 0;
+    f.n2 = f.p1 ? mk * 2L * b.ng * nx1 : 0;
+    f.n3 = f.p0 ? mk * mj * 2L * b.ng : 0;
+    return f;
+}
+inline bool fold_shell_ok(const BlockDims& b) {
+    const FoldShell f = fold_shell(b);
+    return (!f.p0 || b.nx1 >= 2 * b.ng) && (!f.p1 || b.nx2 >= 2 * b.ng) && (!f.p2 || b.nx3 >= 2 * b.ng);
+}
+__device__ __forceinline__ void fold_shell_zone(const BlockDims& b, const FoldShell& f, long t, int& k, int& j, int& i) {
+    const int nx1 = b.nx1, nx2 = b.ndim > 1 ? b.nx2 : 1;
+    auto edge = [&](int e, int s0, int e0) { return e < b.ng ? s0 + e : e0 - (e - b.ng); };
+    const int k0 = b.ks + (f.p2 ? b.ng : 0), j0 = b.js + (f.p1 ? b.ng : 0);
+    const int mj = nx2 - (f.p1 ? 2 * b.ng : 0);
+    if (t < f.n1) {
+        i = b.is + (int)(t % nx1); t /= nx1;
+        j = b.js + (int)(t % nx2); t /= nx2;
+        k = edge((int)t, b.ks, b.ke);
+        return;
+    }
+    t -= f.n1;
+    if (t < f.n2) {
+        i = b.is + (int)(t % nx1); t /= nx1;
+        j = edge((int)(t % (2 * b.ng)), b.js, b.je); t /= 2 * b.ng;
+        k = k0 + (int)t;
+        return;
+    }
+    t -= f.n2;
+    i = edge((int)(t % (2 * b.ng)), b.is, b.ie); t /= 2 * b.ng;
+    j = j0 + (int)(t % mj); t /= mj;
+    k = k0 + (int)t;
+}
+
 __global__ void __launch_bounds__(ABS) adj_fold_kernel(BlockDims b, double* __restrict__ A, int nvar,
-                                                       const int* __restrict__ gate) {
+                                                       const int* __restrict__ gate, FoldShell f, int shell) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long n = (long)b.nx1 * (b.ndim > 1 ? b.nx2 : 1) * (b.ndim > 2 ? b.nx3 : 1);
+    const long n = shell ? f.n1 + f.n2 + f.n3 : (long)b.nx1 * (b.ndim > 1 ? b.nx2 : 1) * (b.ndim > 2 ? b.nx3 : 1);
     if (t >= n || (gate && *gate == 0)) return;
     int k, j, i;
-    interior_zone(b, t, k, j, i);
+    if (shell) fold_shell_zone(b, f, t, k, j, i);
+    else interior_zone(b, t, k, j, i);
     int ni = 1, nj = 1, nk = 1;
     int oi[3] = {0, 0, 0}, oj[3] = {0, 0, 0}, ok[3] = {0, 0, 0};
     if (b.periodic[0]) {
@@ -955,35 +1010,26 @@ long ninterior_l(const BlockDims& b) {
 
 template <int GEO>
 hipError_t launch_adj_flux_t(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const double* P,
-                             const double* lF, double* lamL, double* lamR, double* lamP, hipStream_t s) {
+                             const double* lF, double* lamP, hipStream_t s) {
     const int jlo = (b.ndim > 1) ? b.js - 1 : b.js, jhi = (b.ndim > 1) ? b.je + 1 : b.je;
     const int klo = (b.ndim > 2) ? b.ks - 1 : b.ks, khi = (b.ndim > 2) ? b.ke + 1 : b.ke;
     const int nj = jhi - jlo + 1, nk = khi - klo + 1;
     for (int dir = 1; dir <= b.ndim; ++dir) {
         const double* lFd = lF + (long)(dir - 1) * NV * b.nc;
         if (dir == 1) {
-            const long cells = (long)(b.nx1 + 2) * nj * nk;
-            hipLaunchKernelGGL((adj_face_x1_kernel<GEO>), dim3((unsigned)((cells - 1 + 254) / 255)), dim3(256), 0, s,
-                               b, c, gd, P, lFd, lamL, lamR, jlo, nj, klo, nk);
-            const int nout = b.nx1 + 6;
-            hipLaunchKernelGGL((adj_recon_kernel<1>), dim3((unsigned)((nout + 59) / 60), (unsigned)((nj * nk + 3) / 4)),
-                               dim3(64, 4), 0, s, b, P, lamL, lamR, lamP, jlo, nj, klo, nk);
-        } else if (dir == 2) {
-            const int ncol = (b.nx1 + 2) * nk;
-            constexpr int TX = adj_xt_tx<2, GEO>();
-            hipLaunchKernelGGL((adj_face_xt_kernel<2, GEO>), dim3((unsigned)((ncol + TX - 1) / TX), (unsigned)((b.nx2 + 1 + 6) / 7)),
-                               dim3(TX, 8), 0, s, b, c, gd, P, lFd, lamL, lamR, klo, nk, b.js, b.je + 1);
-            const int nout = b.nx2 + 6;
-            hipLaunchKernelGGL((adj_recon_kernel<2>), dim3((unsigned)((ncol + 15) / 16), (unsigned)((nout + 11) / 12)),
-                               dim3(16, 16), 0, s, b, P, lamL, lamR, lamP, jlo, nj, klo, nk);
+            const long slots = (long)(b.nx1 + 10) * nj * nk;
+            hipLaunchKernelGGL((adj_flux_x1_fused_kernel<GEO>), dim3((unsigned)((slots + 249) / 250)), dim3(256), 0,
+                               s, b, c, gd, P, lFd, lamP, jlo, nj, klo, nk);
         } else {
-            const int ncol = (b.nx1 + 2) * nj;
-            constexpr int TX = adj_xt_tx<3, GEO>();
-            hipLaunchKernelGGL((adj_face_xt_kernel<3, GEO>), dim3((unsigned)((ncol + TX - 1) / TX), (unsigned)((b.nx3 + 1 + 6) / 7)),
-                               dim3(TX, 8), 0, s, b, c, gd, P, lFd, lamL, lamR, jlo, nj, b.ks, b.ke + 1);
-            const int nout = b.nx3 + 6;
-            hipLaunchKernelGGL((adj_recon_kernel<3>), dim3((unsigned)((ncol + 15) / 16), (unsigned)((nout + 11) / 12)),
-                               dim3(16, 16), 0, s, b, P, lamL, lamR, lamP, jlo, nj, klo, nk);
+            const int no = (dir == 2) ? nk : nj;
+            const int o0 = (dir == 2) ? klo : jlo;
+            const int ncol = (b.nx1 + 2) * no;
+            const int nslots = ((dir == 2) ? b.nx2 : b.nx3) + 10;
+            const dim3 grid((unsigned)((ncol + 7) / 8), (unsigned)((nslots + 25) / 26));
+            if (dir == 2)
+                hipLaunchKernelGGL((adj_flux_xt_fused_kernel<2, GEO>), grid, dim3(8, 32), 0, s, b, c, gd, P, lFd, lamP, o0, no);
+            else
+                hipLaunchKernelGGL((adj_flux_xt_fused_kernel<3, GEO>), grid, dim3(8, 32), 0, s, b, c, gd, P, lFd, lamP, o0, no);
         }
     }
     return hipGetLastError();
@@ -997,11 +1043,9 @@ hipError_t launch_adj_fill(double* a, long n, double v, hipStream_t s) {
 }
 
 hipError_t launch_adj_flux(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const double* P,
-                           const double* lF, double* lamLR, double* lamP, hipStream_t s) {
-    double* lamL = lamLR;
-    double* lamR = lamLR + (long)NV * b.nc;
-    if (b.geo) return launch_adj_flux_t<1>(b, c, gd, P, lF, lamL, lamR, lamP, s);
-    return launch_adj_flux_t<0>(b, c, gd, P, lF, lamL, lamR, lamP, s);
+                           const double* lF, double* lamP, hipStream_t s) {
+    if (b.geo) return launch_adj_flux_t<1>(b, c, gd, P, lF, lamP, s);
+    return launch_adj_flux_t<0>(b, c, gd, P, lF, lamP, s);
 }
 
 hipError_t launch_adj_divct(const BlockDims& b, const double* ld, double* lF, double* lemf, hipStream_t s,
@@ -1108,9 +1152,17 @@ hipError_t launch_adj_ptou(const BlockDims& b, const StepConsts& c, const GeomDe
     return hipGetLastError();
 }
 
+// Fold over the periodic shell (or the whole interior when an axis is too short).
+static void fold_launch(const BlockDims& b, double* A, int nvar, const int* gate, hipStream_t s) {
+    const FoldShell f = fold_shell(b);
+    const bool shell = fold_shell_ok(b);
+    const long n = shell ? f.n1 + f.n2 + f.n3 : ninterior_l(b);
+    if (n > 0) hipLaunchKernelGGL(adj_fold_kernel, agrid(n), dim3(ABS), 0, s, b, A, nvar, gate, f, (int)shell);
+}
+
 hipError_t launch_adj_fold(const BlockDims& b, double* A, int nvar, hipStream_t s) {
     const bool periodic = b.periodic[0] || (b.ndim > 1 && b.periodic[1]) || (b.ndim > 2 && b.periodic[2]);
-    if (periodic) hipLaunchKernelGGL(adj_fold_kernel, agrid(ninterior_l(b)), dim3(ABS), 0, s, b, A, nvar, nullptr);
+    if (periodic) fold_launch(b, A, nvar, nullptr, s);
     hipLaunchKernelGGL(adj_clear_ghosts_kernel, agrid(b.nc), dim3(ABS), 0, s, b, A, nvar);
     return hipGetLastError();
 }
@@ -1119,7 +1171,7 @@ hipError_t launch_adj_fold_interior(const BlockDims& b, double* A, int nvar, con
     // periodic axes with images (collapsed axes have none)
     const bool periodic = b.periodic[0] || (b.ndim > 1 && b.periodic[1]) || (b.ndim > 2 && b.periodic[2]);
     if (!periodic) return hipSuccess;
-    hipLaunchKernelGGL(adj_fold_kernel, agrid(ninterior_l(b)), dim3(ABS), 0, s, b, A, nvar, gate);
+    fold_launch(b, A, nvar, gate, s);
     return hipGetLastError();
 }
 

@@ -33,7 +33,6 @@ struct kh_adjoint {
     double* ld = nullptr;      // adjoint of dU/dt
     double* lF = nullptr;      // [3][8][nc] adjoint of the face fluxes
     double* lemf = nullptr;    // [3][nc] adjoint of the corner EMFs
-    double* lLR = nullptr;     // [2][8][nc] adjoints of the reconstructed face values
     int* err = nullptr;        // [0] flagged zones, [1] singular Jacobians (device)
     double* Ustar[2] = {nullptr, nullptr};   // U* of each stage (curved-space path)
     int* ffs[2] = {nullptr, nullptr};        // floor flags of each stage (curved-space path with floors)
@@ -110,7 +109,7 @@ int recompute_stage(kh_adjoint* a, int stage) {
 // clear_ghosts = false leaves ghost entries of lPin stale (interior consumers).
 int stage_transpose(kh_adjoint* a, int stage, const double* Pin, const double* Pout, const double* lPout,
                     const double* lUout, double* lUsub, double* lUfull, bool same, bool accumulate, double dt,
-                    double* lPin, bool clear_ghosts) {
+                    double* lPin, bool clear_ghosts, bool fixup) {
     kh_block* b = a->drv->blk;
     const BlockDims& bd = b->dims;
     const StepConsts& c = b->consts;
@@ -119,7 +118,7 @@ int stage_transpose(kh_adjoint* a, int stage, const double* Pin, const double* P
     // the stage's count of failed inversions (device), which gates the fixup transpose
     const int* gate = reinterpret_cast<const int*>(reinterpret_cast<const char*>(a->drv->ds) +
                                                    offsetof(DeviceState, fail_count)) + (stage - 1);
-    if (a->fixup) {
+    if (fixup) {
         KA_HIP(launch_adj_fixup(bd, c, b->geo, Pout, lPout, lUout, a->pfs[stage - 1], gate, a->lPfx, s));
         KA_HIP(launch_adj_fold_interior(bd, a->lPfx, NV, gate, s));
     }
@@ -127,8 +126,8 @@ int stage_transpose(kh_adjoint* a, int stage, const double* Pin, const double* P
     KA_HIP(launch_adj_stage(bd, c, b->geo, Pout, lPout, lUout, g0, g1, beta_dt, same, accumulate, lUsub, lUfull,
                             a->ld, a->err, a->geo ? a->Ustar[stage - 1] : nullptr, Pin,
                             a->floors ? a->ffs[stage - 1] : nullptr, a->geo ? lPin : nullptr,
-                            a->fixup ? a->pfs[stage - 1] : nullptr, a->fixup ? gate : nullptr,
-                            a->fixup ? a->lPfx : nullptr, a->Pinv, a->lPinv, a->lPtot, s));
+                            fixup ? a->pfs[stage - 1] : nullptr, fixup ? gate : nullptr,
+                            fixup ? a->lPfx : nullptr, a->Pinv, a->lPinv, a->lPtot, s));
     FluxSeed seed;
     if (a->obj_dir) {
         const int dir = a->obj_dir;
@@ -138,7 +137,7 @@ int stage_transpose(kh_adjoint* a, int stage, const double* Pin, const double* P
     }
     KA_HIP(launch_adj_divct(bd, a->ld, a->lF, a->lemf, s, seed));
     if (a->physical) KA_HIP(launch_adj_fix_flux(bd, a->lF, a->x1f[stage - 1], s));
-    KA_HIP(launch_adj_flux(bd, c, b->geo, Pin, a->lF, a->lLR, lPin, s));
+    KA_HIP(launch_adj_flux(bd, c, b->geo, Pin, a->lF, lPin, s));
     if (a->physical) KA_HIP(launch_adj_bounds(bd, b->geo, Pin, lPin, s));
     if (clear_ghosts) {
         KA_HIP(launch_adj_fold(bd, lPin, NV, s));
@@ -186,7 +185,6 @@ int kh_adjoint_create(const kh_params* p, kh_adjoint** out) {
     alloc(&a->ld, sb);
     alloc(&a->lF, 3 * sb);
     alloc(&a->lemf, sizeof(double) * 3 * nc);
-    alloc(&a->lLR, 2 * sb);
     for (int q = 0; q < 2; ++q) {
         if (a->geo) alloc(&a->Ustar[q], sb);
         if (a->floors && e == hipSuccess) e = hipMalloc(&a->ffs[q], sizeof(int) * nc);
@@ -214,7 +212,7 @@ int kh_adjoint_create(const kh_params* p, kh_adjoint** out) {
 void kh_adjoint_destroy(kh_adjoint* a) {
     if (!a) return;
     if (a->drv) (void)hipStreamSynchronize(a->drv->blk->stream);
-    for (double* q : {a->P2, a->U2, a->lPin, a->lU0, a->lU1, a->ld, a->lF, a->lemf, a->lLR, a->Ustar[0], a->Ustar[1],
+    for (double* q : {a->P2, a->U2, a->lPin, a->lU0, a->lU1, a->ld, a->lF, a->lemf, a->Ustar[0], a->Ustar[1],
                       a->x1f[0], a->x1f[1], a->lPfx, a->Pinv, a->lPinv, a->lPtot})
         if (q) (void)hipFree(q);
     for (int* q : {a->ffs[0], a->ffs[1], a->pfs[0], a->pfs[1]})
@@ -255,20 +253,21 @@ int kh_adjoint_step(kh_adjoint* a, const double* P, const double* P_next, double
     const double* P2 = P_next;
     // the curved-space and physical-boundary transposes need stage 2's U*, flags
     // and clamped fluxes: recompute it unless a plain periodic step can use P_next
-    if (!P2 || a->geo || a->physical) {
+    // stage 2 taken from P_next: the caller vouches that no zone of it failed, so
+    // its transpose has no fixup to undo
+    const bool recompute2 = !P2 || a->geo || a->physical;
+    if (recompute2) {
         if ((st = recompute_stage(a, 2)) != KH_OK) return st;   // -> a->P2, a->U2
         P2 = a->P2;
-    } else if (a->fixup) {
-        // stage 2 taken from P_next: the caller vouches that no zone of it failed
-        KA_HIP(hipMemsetAsync(a->pfs[1], 0, sizeof(int) * bd.nc, s));
     }
     KA_HIP(hipEventRecord(a->ev[1], s));
     // ---- transpose.  stage 2: P1 -> P2 with U_sub = U1, U_full = U0 (writes lU1, lU0)
-    if ((st = stage_transpose(a, 2, d->P1, P2, lam_P, nullptr, a->lU1, a->lU0, false, false, dt, a->lPin, false)) !=
-        KH_OK)
+    if ((st = stage_transpose(a, 2, d->P1, P2, lam_P, nullptr, a->lU1, a->lU0, false, false, dt, a->lPin, false,
+                              a->fixup && recompute2)) != KH_OK)
         return st;
     // stage 1: P0 -> P1 with U_sub = U_full = U0 (adds to lU0); lam(P_0) lands in lam_P
-    if ((st = stage_transpose(a, 1, P0, d->P1, a->lPin, a->lU1, a->lU0, a->lU0, true, true, dt, lam_P, true)) != KH_OK)
+    if ((st = stage_transpose(a, 1, P0, d->P1, a->lPin, a->lU1, a->lU0, a->lU0, true, true, dt, lam_P, true,
+                              a->fixup)) != KH_OK)
         return st;
     // U_0 = p_to_u(P_0)
     KA_HIP(launch_adj_ptou(bd, c, b->geo, P0, a->lU0, lam_P, s));
@@ -293,7 +292,7 @@ int kh_adjoint_flux(kh_adjoint* a, const double* P, const double* lam_F, double*
     if (!a || !P || !lam_F || !lam_P) return api_fail(KH_ERR_INVALID, "null argument");
     kh_block* b = a->drv->blk;
     KA_HIP(launch_adj_fill(lam_P, (long)NV * b->dims.nc, 0., b->stream));
-    KA_HIP(launch_adj_flux(b->dims, b->consts, b->geo, P, lam_F, a->lLR, lam_P, b->stream));
+    KA_HIP(launch_adj_flux(b->dims, b->consts, b->geo, P, lam_F, lam_P, b->stream));
     return KH_OK;
 }
 

@@ -163,9 +163,9 @@ hipError_t launch_sync_int(const BlockDims& b, int* A, hipStream_t s);
 
 // ---- adjoint (kh_adjoint.hip) ----
 hipError_t launch_adj_fill(double* a, long n, double v, hipStream_t s);
-// lamP += (d F / d P)^T lF over all directions; lamLR is [2][8][nc] scratch.
+// lamP += (d F / d P)^T lF over all directions.
 hipError_t launch_adj_flux(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const double* P,
-                           const double* lF, double* lamLR, double* lamP, hipStream_t s);
+                           const double* lF, double* lamP, hipStream_t s);
 // A face-flux objective's seed: lF_dir[var] += w on the faces of index `face`
 // along dir over the interior transverse range (dir 0: none).
 struct FluxSeed { int dir = 0, var = 0, face = 0; double w = 0.; };
