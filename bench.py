@@ -111,6 +111,9 @@ def main():
     ap.add_argument("--n", type=int, default=128, help="fastmode: zones per side")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiation-form measurement")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="run only the differentiation-form measurements (Alfven 64^3 and 2-D torus) and print "
+                         "their results (profiling runs)")
     ap.add_argument("--forms", default="fp64,split", help="comma list of fp64,split (profiling runs use one)")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="after the runs, copy a buffer of this many MiB on the device: a known-size "
@@ -125,6 +128,10 @@ def main():
     import torch
     import kharma_amd as kh
     from kharma_amd import problems as pr
+
+    if a.adjoint_only:
+        print(json.dumps({"adjoint": _bench_adjoint(ndirs=8), "adjoint_torus": _bench_adjoint_torus(ndirs=2)}))
+        return
 
     if a.workload == "torus":
         cfg = pr.torus_config(192, 128, 128)

@@ -626,10 +626,83 @@ __device__ __forceinline__ GhostSrc ghost_source(const BlockDims& b, int j, int 
     return r;
 }
 
-// Transpose of the physical ghost fill, gathered at each zone the fill reads
-// (within ng of a physical face): the adjoints held by the ghost zones filled
-// from it are mapped back through the fill and added.  Ghosts are cleared
-// afterwards (adj_fold).
+// Transpose of the physical ghost fill, in two passes.  adj_bounds_map_kernel,
+// one thread per physical ghost zone (the slabs of bounds_kernel), maps the
+// ghost's adjoint back through its fill to an adjoint of its source zone and
+// stores it at the ghost's own position of C; adj_bounds_kernel then gathers,
+// at each zone within ng of a physical face, the C entries of the ghosts filled
+// from it.  Ghosts are cleared afterwards (adj_fold).
+__host__ __device__ inline long ghost_slab1(const BlockDims& b) {
+    return b.periodic[0] ? 0 : (long)b.n3 * b.n2 * 2 * b.ng;
+}
+__host__ __device__ inline long ghost_slab2(const BlockDims& b) {
+    if (b.ndim < 2 || b.periodic[1]) return 0;
+    return (long)b.n3 * 2 * b.ng * (b.periodic[0] ? b.n1 : b.nx1);
+}
+template <int GEO>
+__global__ void __launch_bounds__(ABS)
+adj_bounds_map_kernel(BlockDims b, GeomDev gd, const double* __restrict__ P, const double* __restrict__ A,
+                      double* __restrict__ C) {
+    long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long n1s = ghost_slab1(b);
+    int k, j, i;
+    if (r < n1s) {
+        const int g = (int)(r % (2 * b.ng));
+        r /= 2 * b.ng;
+        j = (int)(r % b.n2);
+        k = (int)(r / b.n2);
+        i = (g < b.ng) ? g : b.ie + 1 + (g - b.ng);
+    } else {
+        r -= n1s;
+        if (r >= ghost_slab2(b)) return;
+        const int w = b.periodic[0] ? b.n1 : b.nx1;
+        const int ii = (int)(r % w);
+        r /= w;
+        const int g = (int)(r % (2 * b.ng));
+        k = (int)(r / (2 * b.ng));
+        j = (g < b.ng) ? g : b.je + 1 + (g - b.ng);
+        i = (b.periodic[0] ? 0 : b.is) + ii;
+    }
+    const long gx = (long)k * b.s3 + (long)j * b.s2 + i;
+    const GhostSrc m = ghost_source(b, j, i);
+    const long sx = (long)k * b.s3 + (long)m.j * b.s2 + m.i;
+    double l[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) l[v] = A[v * b.nc + gx];
+    // B: P_g.B = flip(U_src.B) / gdet_g = flip(P_src.B) gdet_src / gdet_g
+    double ratio = 1.;
+    if constexpr (GEO) ratio = gd.g[GLOC_CENTER][20 * gd.np + (long)m.j * gd.gn1 + m.i] /
+                               gd.g[GLOC_CENTER][20 * gd.np + (long)j * gd.gn1 + i];
+    const int fl = m.pre ^ m.post;
+    double c[NV];
+    c[B1] = ((fl & 1) ? -l[B1] : l[B1]) * ratio;
+    c[B2] = ((fl & 2) ? -l[B2] : l[B2]) * ratio;
+    c[B3] = l[B3] * ratio;
+    // fluid: post flips, inflow transpose at the pre-flipped source state, pre flips
+    if (m.post & 1) l[U1] = -l[U1];
+    if (m.post & 2) l[U2] = -l[U2];
+    if (m.inflow) {
+        double s1[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) s1[v] = P[v * b.nc + sx];
+        if (m.pre & 1) s1[U1] = -s1[U1];
+        if (m.pre & 2) s1[U2] = -s1[U2];
+        if constexpr (GEO) {
+            CachedGeom<double> g(gd.g[GLOC_CENTER], gd.np, (long)m.gj * gd.gn1 + m.gi);
+            inflow_vjp(g, s1, m.inflow == 1, l);
+        } else {
+            MinkowskiGeom<double> g;
+            inflow_vjp(g, s1, m.inflow == 1, l);
+        }
+    }
+    if (m.pre & 1) l[U1] = -l[U1];
+    if (m.pre & 2) l[U2] = -l[U2];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) c[v] = l[v];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) C[v * b.nc + gx] = c[v];
+}
+
 // Zones within ng of a physical face, enumerated: band 1 is the 2 ng interior
 // columns next to the X1 faces (every j, k) when X1 is physical; band 2 the
 // 2 ng interior rows next to the X2 faces over the other columns when X2 is.
@@ -639,9 +712,8 @@ __host__ __device__ inline long bounds_band2(const BlockDims& b) {
     if (b.ndim < 2 || b.periodic[1]) return 0;
     return (long)b.n3 * 2 * b.ng * (b.n1 - x1_band_cols(b));
 }
-template <int GEO>
 __global__ void __launch_bounds__(ABS)
-adj_bounds_kernel(BlockDims b, GeomDev gd, const double* __restrict__ P, double* __restrict__ A) {
+adj_bounds_kernel(BlockDims b, const double* __restrict__ C, double* __restrict__ A) {
     long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long nb1 = bounds_band1(b);
     int k, j, i;
@@ -695,9 +767,6 @@ adj_bounds_kernel(BlockDims b, GeomDev gd, const double* __restrict__ P, double*
             if (sj == j) jc[nj++] = gj;
         }
     if (ni == 1 && nj == 1) return;
-    double src[NV];
-#pragma unroll
-    for (int v = 0; v < NV; ++v) src[v] = P[v * b.nc + t];
     double acc[NV] = {0., 0., 0., 0., 0., 0., 0., 0.};
     bool any = false;
     for (int qj = 0; qj < nj; ++qj)
@@ -708,38 +777,8 @@ adj_bounds_kernel(BlockDims b, GeomDev gd, const double* __restrict__ P, double*
             if (m.j != j || m.i != i) continue;
             any = true;
             const long gx = (long)k * b.s3 + (long)gj * b.s2 + gi;
-            double l[NV];
 #pragma unroll
-            for (int v = 0; v < NV; ++v) l[v] = A[v * b.nc + gx];
-            // B: P_g.B = flip(U_src.B) / gdet_g = flip(P_src.B) gdet_src / gdet_g
-            double ratio = 1.;
-            if constexpr (GEO) ratio = gd.g[GLOC_CENTER][20 * gd.np + (long)j * gd.gn1 + i] /
-                                       gd.g[GLOC_CENTER][20 * gd.np + (long)gj * gd.gn1 + gi];
-            const int fl = m.pre ^ m.post;
-            acc[B1] += ((fl & 1) ? -l[B1] : l[B1]) * ratio;
-            acc[B2] += ((fl & 2) ? -l[B2] : l[B2]) * ratio;
-            acc[B3] += l[B3] * ratio;
-            // fluid: post flips, inflow transpose at the pre-flipped source state, pre flips
-            if (m.post & 1) l[U1] = -l[U1];
-            if (m.post & 2) l[U2] = -l[U2];
-            if (m.inflow) {
-                double s1[NV];
-#pragma unroll
-                for (int v = 0; v < NV; ++v) s1[v] = src[v];
-                if (m.pre & 1) s1[U1] = -s1[U1];
-                if (m.pre & 2) s1[U2] = -s1[U2];
-                if constexpr (GEO) {
-                    CachedGeom<double> g(gd.g[GLOC_CENTER], gd.np, (long)m.gj * gd.gn1 + m.gi);
-                    inflow_vjp(g, s1, m.inflow == 1, l);
-                } else {
-                    MinkowskiGeom<double> g;
-                    inflow_vjp(g, s1, m.inflow == 1, l);
-                }
-            }
-            if (m.pre & 1) l[U1] = -l[U1];
-            if (m.pre & 2) l[U2] = -l[U2];
-#pragma unroll
-            for (int v = 0; v < 5; ++v) acc[v] += l[v];
+            for (int v = 0; v < NV; ++v) acc[v] += C[v * b.nc + gx];
         }
     if (!any) return;
 #pragma unroll
@@ -1108,12 +1147,15 @@ hipError_t launch_adj_fixup_apply(const BlockDims& b, const int* pfs, const doub
     return hipGetLastError();
 }
 
-hipError_t launch_adj_bounds(const BlockDims& b, const GeomDev& gd, const double* P, double* A, hipStream_t s) {
+hipError_t launch_adj_bounds(const BlockDims& b, const GeomDev& gd, const double* P, double* A, double* C,
+                             hipStream_t s) {
     if (b.periodic[0] && (b.ndim < 2 || b.periodic[1])) return hipSuccess;
     if (b.ng > 4) return hipErrorInvalidValue;
+    const long ng = ghost_slab1(b) + ghost_slab2(b);
+    if (b.geo) hipLaunchKernelGGL(adj_bounds_map_kernel<1>, agrid(ng), dim3(ABS), 0, s, b, gd, P, A, C);
+    else hipLaunchKernelGGL(adj_bounds_map_kernel<0>, agrid(ng), dim3(ABS), 0, s, b, gd, P, A, C);
     const long n = bounds_band1(b) + bounds_band2(b);
-    if (b.geo) hipLaunchKernelGGL(adj_bounds_kernel<1>, agrid(n), dim3(ABS), 0, s, b, gd, P, A);
-    else hipLaunchKernelGGL(adj_bounds_kernel<0>, agrid(n), dim3(ABS), 0, s, b, gd, P, A);
+    hipLaunchKernelGGL(adj_bounds_kernel, agrid(n), dim3(ABS), 0, s, b, C, A);
     return hipGetLastError();
 }
 

@@ -42,6 +42,7 @@ struct kh_adjoint {
     double* Pinv = nullptr;                  // [8][nc] recovered primitives of floored zones
     double* lPinv = nullptr;                 // [8][nc] their adjoints through the floors
     double* lPtot = nullptr;                 // [8][nc] their output adjoints, P->U included
+    double* gC = nullptr;                    // [8][nc] ghost-fill transpose scratch (physical boundaries)
     bool geo = false, physical = false, floors = false, fixup = false;
     int obj_dir = 0, obj_var = 0, obj_face = 0;   // face-flux objective (kh_adjoint_set_face_flux_objective)
     double obj_weight = 0.;
@@ -138,7 +139,7 @@ int stage_transpose(kh_adjoint* a, int stage, const double* Pin, const double* P
     KA_HIP(launch_adj_divct(bd, a->ld, a->lF, a->lemf, s, seed));
     if (a->physical) KA_HIP(launch_adj_fix_flux(bd, a->lF, a->x1f[stage - 1], s));
     KA_HIP(launch_adj_flux(bd, c, b->geo, Pin, a->lF, lPin, s));
-    if (a->physical) KA_HIP(launch_adj_bounds(bd, b->geo, Pin, lPin, s));
+    if (a->physical) KA_HIP(launch_adj_bounds(bd, b->geo, Pin, lPin, a->gC, s));
     if (clear_ghosts) {
         KA_HIP(launch_adj_fold(bd, lPin, NV, s));
     } else {
@@ -193,6 +194,7 @@ int kh_adjoint_create(const kh_params* p, kh_adjoint** out) {
         if (a->fixup && e == hipSuccess) e = hipMemset(a->pfs[q], 0, sizeof(int) * nc);
     }
     if (a->fixup) alloc(&a->lPfx, sb);
+    if (a->physical) alloc(&a->gC, sb);
     if (a->floors) {
         alloc(&a->Pinv, sb);
         alloc(&a->lPinv, sb);
@@ -213,7 +215,7 @@ void kh_adjoint_destroy(kh_adjoint* a) {
     if (!a) return;
     if (a->drv) (void)hipStreamSynchronize(a->drv->blk->stream);
     for (double* q : {a->P2, a->U2, a->lPin, a->lU0, a->lU1, a->ld, a->lF, a->lemf, a->Ustar[0], a->Ustar[1],
-                      a->x1f[0], a->x1f[1], a->lPfx, a->Pinv, a->lPinv, a->lPtot})
+                      a->x1f[0], a->x1f[1], a->lPfx, a->Pinv, a->lPinv, a->lPtot, a->gC})
         if (q) (void)hipFree(q);
     for (int* q : {a->ffs[0], a->ffs[1], a->pfs[0], a->pfs[1]})
         if (q) (void)hipFree(q);
@@ -335,7 +337,7 @@ int kh_adjoint_set_face_flux_objective(kh_adjoint* a, int dir, int var, int face
 int kh_adjoint_bounds(kh_adjoint* a, const double* P, double* lam_P) {
     if (!a || !P || !lam_P) return api_fail(KH_ERR_INVALID, "null argument");
     kh_block* b = a->drv->blk;
-    KA_HIP(launch_adj_bounds(b->dims, b->geo, P, lam_P, b->stream));
+    if (a->gC) KA_HIP(launch_adj_bounds(b->dims, b->geo, P, lam_P, a->gC, b->stream));
     KA_HIP(launch_adj_fold(b->dims, lam_P, NV, b->stream));
     return KH_OK;
 }

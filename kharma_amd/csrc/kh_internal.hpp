@@ -197,8 +197,10 @@ hipError_t launch_adj_fold_interior(const BlockDims& b, double* A, int nvar, con
 // lP (interior fluid part) <- failed zones 0, others + lPfx.
 hipError_t launch_adj_fixup_apply(const BlockDims& b, const int* pfs, const double* lPfx, double* lP, hipStream_t s);
 // Transpose of the physical-boundary ghost fill of P (bounds_kernel), gathered
-// into the zones it reads; ghosts are left for launch_adj_fold to clear.
-hipError_t launch_adj_bounds(const BlockDims& b, const GeomDev& gd, const double* P, double* A, hipStream_t s);
+// into the zones it reads; ghosts are left for launch_adj_fold to clear.  C is
+// [8][nc] scratch (its ghost entries are written and read).
+hipError_t launch_adj_bounds(const BlockDims& b, const GeomDev& gd, const double* P, double* A, double* C,
+                             hipStream_t s);
 // X1 rows of fix_flux: x1_flux_rows(b) saved mass fluxes per stage.
 long x1_flux_rows(const BlockDims& b);
 hipError_t launch_save_x1_flux(const BlockDims& b, const double* F, double* x1f, hipStream_t s);
