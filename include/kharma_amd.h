@@ -54,6 +54,13 @@ extern "C" {
 #define KH_BC_OUTFLOW 1
 #define KH_BC_REFLECT 2
 
+/* driver/type (driver/kharma_driver.cpp:60-70): the task ordering of a stage.
+ * "kharma" is MakeDefaultTaskCollection (driver/kharma_step.cpp:77-319), "simple"
+ * MakeSimpleTaskCollection (driver/simple_step.cpp:39-168).  On one block both
+ * orderings give the same state bit for bit; the value is checked and kept. */
+#define KH_DRIVER_KHARMA 0
+#define KH_DRIVER_SIMPLE 1
+
 /* Run parameters of one meshblock: the .par options this path reads
  * (GRMHD/*, flux/*, floors/*, inverter/*, parthenon/time/*, parthenon/mesh/*). */
 typedef struct kh_params {
@@ -87,6 +94,7 @@ typedef struct kh_params {
     int zero_polar_flux;            /* boundaries/zero_polar_flux */
     int fix_polar_flux;             /* b_field/fix_polar_flux */
     int fix_corner_inner;           /* boundaries/fix_corner_inner */
+    int driver_type;                /* KH_DRIVER_* (driver/type) */
 } kh_params;
 
 typedef struct kh_block kh_block;
@@ -153,7 +161,10 @@ int kh_ptou(kh_block* b, const double* P, double* U);
  * the fluid conserved variables from P.  Faces in the order inner/outer x1,
  * inner/outer x2, then inner x1 again when fix_corner_inner.  The result is the
  * state after the stage's repeated boundary passes (kharma_step.cpp:240-314),
- * whose fluid conserved variables use the updated ghost B. */
+ * whose fluid conserved variables use the updated ghost B.  With driver_type
+ * KH_DRIVER_SIMPLE the fill is that of the synced primitives
+ * (boundaries.cpp:561-571): all 8 conserved variables of a filled zone come
+ * from its primitives, so U.B = (U.B_src / sqrt(-g)) sqrt(-g) there. */
 int kh_apply_bounds(kh_block* b, double* U, double* P);
 
 /* Boundary exchange of nvar cell variables of a single periodic block with itself

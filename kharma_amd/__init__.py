@@ -26,6 +26,7 @@ FRAME = {"fluid": 0, "normal": 1, "drift": 2}
 COORDS = {"cartesian_minkowski": 0, "ks": 1}
 TRANSFORM = {"null": 0, "mks": 1, "fmks": 2}
 BC = {"periodic": 0, "outflow": 1, "reflecting": 2}
+DRIVER = {"kharma": 0, "simple": 1}
 
 
 class KhParams(ctypes.Structure):
@@ -55,6 +56,7 @@ class KhParams(ctypes.Structure):
         ("bc_inner_x2", ctypes.c_int), ("bc_outer_x2", ctypes.c_int),
         ("check_inflow_inner_x1", ctypes.c_int), ("check_inflow_outer_x1", ctypes.c_int),
         ("zero_polar_flux", ctypes.c_int), ("fix_polar_flux", ctypes.c_int), ("fix_corner_inner", ctypes.c_int),
+        ("driver_type", ctypes.c_int),
     ]
 
 
@@ -177,6 +179,7 @@ def make_params(cfg):
     p.zero_polar_flux = int(cfg.get("zero_polar_flux", False))
     p.fix_polar_flux = int(cfg.get("fix_polar_flux", False))
     p.fix_corner_inner = int(cfg.get("fix_corner_inner", False))
+    p.driver_type = DRIVER[cfg.get("driver", "kharma")]
     p.gam = cfg["gamma"]
     p.cfl = cfg.get("cfl", 0.9)
     p.dt_min = cfg.get("dt_min", 1e-5)

@@ -71,6 +71,8 @@ int kh::check_params(const kh_params* p) {
     if (p->recon < 0 || p->recon > 3) return fail(KH_ERR_INVALID, "unknown reconstruction");
     if (p->precision != KH_PREC_FP64 && p->precision != KH_PREC_SPLIT) return fail(KH_ERR_INVALID, "unknown precision");
     if (p->integrator != 0 && p->integrator != 1) return fail(KH_ERR_INVALID, "unknown integrator");
+    if (p->driver_type != KH_DRIVER_KHARMA && p->driver_type != KH_DRIVER_SIMPLE)
+        return fail(KH_ERR_INVALID, "unknown driver type");
     if (p->floor_frame < 0 || p->floor_frame > 2) return fail(KH_ERR_INVALID, "unknown floor frame");
     if (p->iter_max < 0) return fail(KH_ERR_INVALID, "iter_max < 0");
     const double n = (double)(p->nx1 + 2 * p->ng) * (p->nx2 + 2 * p->ng) * (p->nx3 + 2 * p->ng);
@@ -105,6 +107,7 @@ void kh::make_dims(const kh_params* p, BlockDims& b) {
     b.zero_polar_flux = !b.periodic[1] && p->zero_polar_flux;
     b.fix_polar_flux = !b.periodic[1] && p->fix_polar_flux;
     b.fix_corner_inner = !b.periodic[0] && !b.periodic[1] && p->fix_corner_inner;
+    b.prim_fill = p->driver_type == KH_DRIVER_SIMPLE;
 }
 
 namespace {

@@ -27,6 +27,9 @@ struct BlockDims {
     int bc[4];             // boundary types of the inner/outer x1 and inner/outer x2 faces (KH_BC_*; 0 if periodic)
     int check_inflow[2];   // inner/outer x1
     int zero_polar_flux, fix_polar_flux, fix_corner_inner;
+    // Physical boundary fill of driver "simple" (primitives are the synced
+    // fields): each filled zone's U.B is recomputed from its P.B = U.B / sqrt(-g).
+    int prim_fill;
 };
 
 // Scalars that change every step live on the device so a run of steps needs no

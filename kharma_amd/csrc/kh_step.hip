@@ -778,6 +778,10 @@ __global__ void __launch_bounds__(BS) fix_flux_kernel(BlockDims b, R* __restrict
 // 5 fluid conserved variables from the primitives.  Faces apply in the order
 // inner/outer x1, inner/outer x2, then inner x1 again when fix_corner_inner
 // (boundaries.cpp:556-570); corner zones compose the two fills accordingly.
+// With b.prim_fill (driver "simple", boundaries.cpp:561-571) the fill copies
+// the primitives and U.B, then sets B^i = U.B / sqrt(-g) and all 8 conserved
+// variables from the primitives, so every filled zone -- a corner's
+// intermediate zone included -- holds U.B = (U.B_src / sqrt(-g)) sqrt(-g).
 template <int GEO>
 __device__ __forceinline__ void bc_fill_x1(const BlockDims& b, const GeomDev& gd, const double* __restrict__ Usrc,
                                            const double* __restrict__ Psrc, int k, int j, int i, double P[NV], double Bu[3]) {
@@ -841,6 +845,14 @@ __global__ void __launch_bounds__(BS) bounds_kernel(BlockDims b, StepConsts cst,
     const bool x1g = !b.periodic[0] && (i < b.is || i > b.ie);
     const bool x2g = b.ndim > 1 && !b.periodic[1] && (j < b.js || j > b.je);
     double Pz[NV], Bu[3];
+    // U.B that a filled zone (jj, ii) ends with, under b.prim_fill
+    auto settle = [&](double B[3], int jj, int ii) {
+        if (!b.prim_fill) return;
+        with_center_geom<GEO>(gd, jj, ii, [&](const auto& g) {
+            const double gdet = g.gdet();
+            for (int v = 0; v < 3; ++v) B[v] = (B[v] / gdet) * gdet;
+        });
+    };
     if (x1g && !x2g) {
         bc_fill_x1<GEO>(b, gd, U, P, k, j, i, Pz, Bu);
     } else if (x2g && !x1g) {
@@ -849,6 +861,7 @@ __global__ void __launch_bounds__(BS) bounds_kernel(BlockDims b, StepConsts cst,
         // X2 fill of column is, then the inner-X1 fill over it
         double Pm[NV], Bm[3];
         bc_fill_x2<GEO>(b, U, P, k, j, b.is, Pm, Bm);
+        settle(Bm, j, b.is);
         const int type = b.bc[0];
         for (int v = 0; v < 5; ++v) Pz[v] = Pm[v];
         for (int v = 0; v < 3; ++v) Bu[v] = Bm[v];
@@ -856,6 +869,7 @@ __global__ void __launch_bounds__(BS) bounds_kernel(BlockDims b, StepConsts cst,
             // a reflecting X1 face would read its mirror column instead
             const int si = 2 * b.is - 1 - i;
             bc_fill_x2<GEO>(b, U, P, k, j, si, Pz, Bu);
+            settle(Bu, j, si);
             Pz[U1] = -Pz[U1]; Bu[0] = -Bu[0];
         }
         if (b.check_inflow[0]) with_center_geom<GEO>(gd, j, i, [&](const auto& g) { check_inflow(g, Pz, true); });
@@ -865,6 +879,7 @@ __global__ void __launch_bounds__(BS) bounds_kernel(BlockDims b, StepConsts cst,
         const int type = b.bc[inner ? 2 : 3];
         const int sj = (type == KH_BC_OUTFLOW) ? (inner ? b.js : b.je) : (inner ? 2 * b.js - 1 - j : 2 * b.je + 1 - j);
         bc_fill_x1<GEO>(b, gd, U, P, k, sj, i, Pz, Bu);
+        settle(Bu, sj, i);
         if (type == KH_BC_REFLECT) { Pz[U2] = -Pz[U2]; Bu[1] = -Bu[1]; }
     }
     double Uz[NV];
@@ -874,7 +889,7 @@ __global__ void __launch_bounds__(BS) bounds_kernel(BlockDims b, StepConsts cst,
         p_to_u(g, Pz, cst.gam, Uz);
     });
     for (int v = 0; v < 5; ++v) U[v * b.nc + t] = Uz[v];
-    for (int v = 0; v < 3; ++v) U[(B1 + v) * b.nc + t] = Bu[v];
+    for (int v = 0; v < 3; ++v) U[(B1 + v) * b.nc + t] = b.prim_fill ? Uz[B1 + v] : Bu[v];
     for (int v = 0; v < NV; ++v) P[v * b.nc + t] = Pz[v];
     if (P32) for (int v = 0; v < NV; ++v) P32[v * b.nc + t] = (float)Pz[v];
 }

@@ -117,7 +117,11 @@ MODES_DIR3 = {
 
 
 def mhdmode_config(n=64, mode="alfven", ndim=3, amp=1e-4, one_period=True, **kw):
-    """Linear MHD eigenmode on the periodic unit cube, k = (2pi, 2pi, 0), gamma = 4/3."""
+    """Linear MHD eigenmode on the periodic unit cube, k = (2pi, 2pi, 0), gamma = 4/3.
+
+    Uses the "simple" driver ordering, as pars/tests/mhdmodes.par:92-97 does.
+    """
+    kw.setdefault("driver", "simple")
     nx = (n, n, n if ndim == 3 else 1)
     ev = MODES_DIR3[mode]
     tlim = 2.0 * math.pi / ev["omega"] if (one_period and ev["omega"] > 0) else 1e30

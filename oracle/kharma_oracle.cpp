@@ -89,6 +89,7 @@ struct or_params {
     int check_inflow_inner_x1, check_inflow_outer_x1;
     int zero_polar_flux, fix_polar_flux, fix_corner_inner;
     int periodic1, periodic2, periodic3;
+    int driver;                        // driver/type: 0 kharma, 1 simple
 };
 }
 
@@ -690,6 +691,31 @@ void sync_bounds(Block& b, std::vector<Real>& U, std::vector<Real>& P) {
     }
 }
 
+// The exchange of driver "simple" (sync_prims, kharma_driver.cpp:109-121): the
+// eight primitives, the conserved B and pflag (FillGhost when the fix-up
+// averages neighbours, inverter.cpp:97-104), then B_FluxCT::BlockUtoP over each
+// periodic boundary's ghost slab (kharma_driver.cpp:155-180).  Fluid conserved
+// variables are not exchanged.
+void sync_bounds_prims(Block& b, std::vector<Real>& U, std::vector<Real>& P) {
+#pragma omp parallel for
+    for (int v = 0; v < NV + 3; ++v) {
+        if (v < NV) sync_periodic_var(b, P, v);
+        else sync_periodic_var(b, U, B1 + v - NV);
+    }
+    if (b.p.fix_average) sync_periodic_var(b, b.pflag, 0);
+#pragma omp parallel for collapse(2)
+    for (int k = 0; k < b.n3; ++k)
+    for (int j = 0; j < b.n2; ++j)
+    for (int i = 0; i < b.n1; ++i) {
+        const bool g1 = b.p.periodic1 && (i < b.is || i > b.ie);
+        const bool g2 = b.ndim > 1 && b.p.periodic2 && (j < b.js || j > b.je);
+        const bool g3 = b.ndim > 2 && b.p.periodic3 && (k < b.ks || k > b.ke);
+        if (!(g1 || g2 || g3)) continue;
+        const size_t c = b.id(k, j, i);
+        for (int v = 0; v < 3; ++v) P[(B1 + v) * b.nc + c] = U[(B1 + v) * b.nc + c] / b.gdet_(CENTER, j, i);
+    }
+}
+
 // Inverter::u_to_p<onedw>, inverter/onedw.hpp:49-233
 Real lorentz_calc_w(Real Bsq, Real D, Real QdB, Real Qtsq, Real Wp) {
     const Real QdBsq = QdB * QdB;
@@ -800,8 +826,10 @@ Range3 physical_range(const Block& b) {
 }
 
 // Packages::MeshUtoP -> Inverter::BlockUtoP (inverter.cpp:131-200, over the
-// physical range) then B_FluxCT::BlockUtoP (b_flux_ct.cpp:195-217, entire).
-void utop(Block& b, std::vector<Real>& U, std::vector<Real>& P) {
+// physical range whatever the domain) then B_FluxCT::BlockUtoP
+// (b_flux_ct.cpp:195-217, over the domain: entire, or the interior when
+// b_interior is set).
+void utop(Block& b, std::vector<Real>& U, std::vector<Real>& P, bool b_interior = false) {
     const Range3 pr = physical_range(b);
 #pragma omp parallel for collapse(2)
     for (int k = pr.ks; k <= pr.ke; ++k)
@@ -815,11 +843,13 @@ void utop(Block& b, std::vector<Real>& U, std::vector<Real>& P) {
         b.pflag[c] = pflagl % FFLAG_MINIMUM;
         b.fflag[c] = (pflagl / FFLAG_MINIMUM) * FFLAG_MINIMUM;
     }
+    const Range3 br = b_interior ? Range3{b.is, b.ie, b.js, b.je, b.ks, b.ke}
+                                 : Range3{0, b.n1 - 1, 0, b.n2 - 1, 0, b.n3 - 1};
 #pragma omp parallel for collapse(2)
     for (int v = 0; v < 3; ++v)
-    for (int k = 0; k < b.n3; ++k)
-    for (int j = 0; j < b.n2; ++j)
-    for (int i = 0; i < b.n1; ++i) {
+    for (int k = br.ks; k <= br.ke; ++k)
+    for (int j = br.js; j <= br.je; ++j)
+    for (int i = br.is; i <= br.ie; ++i) {
         const size_t c = b.id(k, j, i);
         P[(B1 + v) * b.nc + c] = U[(B1 + v) * b.nc + c] / b.gdet_(CENTER, j, i);
     }
@@ -952,13 +982,16 @@ void p_to_u_mhd(const Block& b, const Real* P, Real* U, int j, int i) {
     prim_to_flux(b, P, D, j, i, 0, U, CENTER, false);
 }
 // Floors::ApplyGRMHDFloors -> ApplyFloorsInFrame, floors/floors_impl.hpp:45-128 and
-// DetermineGRMHDFloors, floors.cpp:197-231, over the entire domain.
-void apply_floors(Block& b, std::vector<Real>& U, std::vector<Real>& P) {
+// DetermineGRMHDFloors, floors.cpp:197-231, over the entire domain (or the
+// interior, when `interior` is set).
+void apply_floors(Block& b, std::vector<Real>& U, std::vector<Real>& P, bool interior = false) {
     if (!b.p.floors_on) return;
+    const Range3 r = interior ? Range3{b.is, b.ie, b.js, b.je, b.ks, b.ke}
+                              : Range3{0, b.n1 - 1, 0, b.n2 - 1, 0, b.n3 - 1};
 #pragma omp parallel for collapse(2)
-    for (int k = 0; k < b.n3; ++k)
-    for (int j = 0; j < b.n2; ++j)
-    for (int i = 0; i < b.n1; ++i) {
+    for (int k = r.ks; k <= r.ke; ++k)
+    for (int j = r.js; j <= r.je; ++j)
+    for (int i = r.is; i <= r.ie; ++i) {
         const size_t c = b.id(k, j, i);
         Real Pc[NV]; load_prims(b, P, c, Pc);
         Real rf, uf;
@@ -967,9 +1000,9 @@ void apply_floors(Block& b, std::vector<Real>& U, std::vector<Real>& P) {
         b.rho_floor[c] = rf; b.u_floor[c] = uf;
     }
 #pragma omp parallel for collapse(2)
-    for (int k = 0; k < b.n3; ++k)
-    for (int j = 0; j < b.n2; ++j)
-    for (int i = 0; i < b.n1; ++i) {
+    for (int k = r.ks; k <= r.ke; ++k)
+    for (int j = r.js; j <= r.je; ++j)
+    for (int i = r.is; i <= r.ie; ++i) {
         const size_t c = b.id(k, j, i);
         if (!(int)b.fflag[c]) continue;
         Real Pc[NV], Uc[NV];
@@ -1132,7 +1165,12 @@ void check_inflow(const Block& b, Real* Pc, int j, int i, bool inner) {
 // BoundaryPtoUElseUtoP (kharma_package.cpp:125-156): Flux::BlockPtoUMHD and
 // B_FluxCT::BlockUtoP over the slab.
 enum { INNER_X1 = 0, OUTER_X1 = 1, INNER_X2 = 2, OUTER_X2 = 3 };
-void apply_boundary(Block& b, std::vector<Real>& U, std::vector<Real>& P, int face) {
+//
+// With `prims` set (driver "simple": the primitives, the conserved B and pflag
+// are the FillGhost fields, kharma_driver.cpp:109-121, inverter.cpp:97-104) the
+// copy is of those fields, and the slab then gets B_FluxCT::BlockUtoP and
+// Flux::BlockPtoU of all eight variables (boundaries.cpp:561-571).
+void apply_boundary(Block& b, std::vector<Real>& U, std::vector<Real>& P, int face, bool prims = false) {
     const bool x1 = (face == INNER_X1 || face == OUTER_X1);
     const bool inner = (face == INNER_X1 || face == INNER_X2);
     const int type = (face == INNER_X1) ? b.p.bc_inner_x1 : (face == OUTER_X1) ? b.p.bc_outer_x1 :
@@ -1153,14 +1191,12 @@ void apply_boundary(Block& b, std::vector<Real>& U, std::vector<Real>& P, int fa
         }
         if (type == 2) sign2 = -1.;
         const size_t c = b.id(k, j, i), sc = b.id(k, sj, si);
-        for (int v = 0; v < NV; ++v) {
-            const Real flip = (type == 2 && ((x1 && (v == U1 || v == B1)) || (!x1 && (v == U2 || v == B2)))) ? sign2 : 1.;
-            U[v * b.nc + c] = flip * U[v * b.nc + sc];
-        }
-        for (int v = 0; v < 5; ++v) {
-            const Real flip = (type == 2 && ((x1 && v == U1) || (!x1 && v == U2))) ? sign2 : 1.;
-            P[v * b.nc + c] = flip * P[v * b.nc + sc];
-        }
+        auto flip = [&](int v) {
+            return (type == 2 && ((x1 && (v == U1 || v == B1)) || (!x1 && (v == U2 || v == B2)))) ? sign2 : 1.;
+        };
+        for (int v = prims ? B1 : 0; v < NV; ++v) U[v * b.nc + c] = flip(v) * U[v * b.nc + sc];
+        for (int v = 0; v < (prims ? NV : 5); ++v) P[v * b.nc + c] = flip(v) * P[v * b.nc + sc];
+        if (prims) b.pflag[c] = b.pflag[sc];
     }
     if (inflow) {
 #pragma omp parallel for collapse(2)
@@ -1179,6 +1215,15 @@ void apply_boundary(Block& b, std::vector<Real>& U, std::vector<Real>& P, int fa
     for (int i = i0; i <= i1; ++i) {
         const size_t c = b.id(k, j, i);
         Real Pc[NV], Uc[NV];
+        if (prims) {
+            for (int v = 0; v < 3; ++v) P[(B1 + v) * b.nc + c] = U[(B1 + v) * b.nc + c] / b.gdet_(CENTER, j, i);
+            load_prims(b, P, c, Pc);
+            FourVectors D;
+            calc_4vecs(b, Pc, j, i, CENTER, D);
+            prim_to_flux(b, Pc, D, j, i, 0, Uc, CENTER, true);
+            for (int v = 0; v < NV; ++v) U[v * b.nc + c] = Uc[v];
+            continue;
+        }
         load_prims(b, P, c, Pc); load_prims(b, U, c, Uc);
         p_to_u_mhd(b, Pc, Uc, j, i);
         for (int v = 0; v < 5; ++v) U[v * b.nc + c] = Uc[v];
@@ -1187,11 +1232,11 @@ void apply_boundary(Block& b, std::vector<Real>& U, std::vector<Real>& P, int fa
 }
 // ApplyBoundaryConditions over the faces in Parthenon's order, plus KHARMA's
 // inner-X1 corner fix after the X2 faces (boundaries.cpp:556-570).
-void apply_physical_bcs(Block& b, std::vector<Real>& U, std::vector<Real>& P) {
-    if (!b.p.periodic1) { apply_boundary(b, U, P, INNER_X1); apply_boundary(b, U, P, OUTER_X1); }
+void apply_physical_bcs(Block& b, std::vector<Real>& U, std::vector<Real>& P, bool prims = false) {
+    if (!b.p.periodic1) { apply_boundary(b, U, P, INNER_X1, prims); apply_boundary(b, U, P, OUTER_X1, prims); }
     if (b.ndim > 1 && !b.p.periodic2) {
-        apply_boundary(b, U, P, INNER_X2); apply_boundary(b, U, P, OUTER_X2);
-        if (b.p.fix_corner_inner && !b.p.periodic1) apply_boundary(b, U, P, INNER_X1);
+        apply_boundary(b, U, P, INNER_X2, prims); apply_boundary(b, U, P, OUTER_X2, prims);
+        if (b.p.fix_corner_inner && !b.p.periodic1) apply_boundary(b, U, P, INNER_X1, prims);
     }
 }
 
@@ -1287,13 +1332,45 @@ void run_stage(Block& b, int stage) {
     apply_physical_bcs(b, Uout, Pout);
 }
 
+// One stage of KHARMADriver::MakeSimpleTaskCollection, driver/simple_step.cpp:39-168
+// (driver/type = simple): inversion and floors on the interior before the
+// exchange (:122-128), the fix-up after it (:141), then the boundary fill and
+// P->U over the entire block; the exchanged fields are the primitives.
+void run_stage_simple(Block& b, int stage) {
+    std::vector<Real>& Usub = (stage == 1) ? b.U0 : b.U1c;
+    std::vector<Real>& Psub = (stage == 1) ? b.P0 : b.P1c;
+    std::vector<Real>& Uout = (stage == 1) ? b.U1c : b.U0;
+    std::vector<Real>& Pout = (stage == 1) ? b.P1c : b.P0;
+    for (int dir = 1; dir <= 3; ++dir) get_flux(b, Psub, dir);          // :86
+    zero_polar_b_flux(b);                                                // :93 Packages::FixFlux
+    flux_ct(b);
+    kboundaries_fix_flux(b);
+    flux_divergence(b);                                                  // :96
+    add_geo_source(b, Psub);                                             // :100
+    Real gam0, gam1, beta;
+    integrator_weights(b.p.integrator, stage, gam0, gam1, beta);
+    state_update(b, Usub, b.U0, Psub, Uout, Pout, gam0, gam1, beta * b.dt);  // :104-119
+    utop(b, Uout, Pout, true);                                           // :123 (interior)
+    apply_floors(b, Uout, Pout, true);                                   // :126 (interior)
+    sync_bounds_prims(b, Uout, Pout);                                    // :129
+    apply_physical_bcs(b, Uout, Pout, true);
+    fix_utop(b, Uout, Pout);                                             // :141
+    apply_physical_bcs(b, Uout, Pout, true);                             // :143
+    ptou(b, Pout, Uout);                                                 // :146 (entire)
+    sync_bounds_prims(b, Uout, Pout);                                    // :160-165 two_sync
+    apply_physical_bcs(b, Uout, Pout, true);
+}
+
 // EvolutionDriver step + KHARMA dt logic: PreStepWork (kharma.cpp:121-129),
 // stages, EstimateTimestep (grmhd.cpp:252-343), SetGlobalTimeStep
 // (kharma_driver.cpp:439-460).  Returns the dt that was taken.
 double step(Block& b) {
     b.in_loop = true;
     b.dt_last = b.dt;
-    for (int stage = 1; stage <= 2; ++stage) run_stage(b, stage);
+    for (int stage = 1; stage <= 2; ++stage) {
+        if (b.p.driver == 1) run_stage_simple(b, stage);
+        else run_stage(b, stage);
+    }
     double new_dt = estimate_timestep(b);
     const double taken = b.dt;
     b.time += b.dt;
@@ -1395,7 +1472,9 @@ void or_set_geometry(void* h, int which, const double* in) {
     std::memcpy(v->data(), in, sizeof(double) * v->size());
 }
 // Physical boundaries on the base state (KBoundaries::ApplyBoundary for each face).
-void or_apply_bcs(void* h) { Block& b = *static_cast<Block*>(h); apply_physical_bcs(b, b.U0, b.P0); }
+// The physical boundary fill of the block's driver type (the synced-primitive
+// fill for driver "simple").
+void or_apply_bcs(void* h) { Block& b = *static_cast<Block*>(h); apply_physical_bcs(b, b.U0, b.P0, b.p.driver == 1); }
 // Flux fixes on the stored fluxes (ZeroBoundaryFlux, FluxCT, KBoundaries::FixFlux).
 void or_fix_flux(void* h) { Block& b = *static_cast<Block*>(h); zero_polar_b_flux(b); flux_ct(b); kboundaries_fix_flux(b); }
 

@@ -45,6 +45,7 @@ class OrParams(ctypes.Structure):
         ("check_inflow_inner_x1", ctypes.c_int), ("check_inflow_outer_x1", ctypes.c_int),
         ("zero_polar_flux", ctypes.c_int), ("fix_polar_flux", ctypes.c_int), ("fix_corner_inner", ctypes.c_int),
         ("periodic1", ctypes.c_int), ("periodic2", ctypes.c_int), ("periodic3", ctypes.c_int),
+        ("driver", ctypes.c_int),
     ]
 
 
@@ -111,6 +112,7 @@ RECON = {"donor_cell_c": 0, "linear_mc": 1, "weno5": 2, "ppm": 3}
 FRAME = {"fluid": 0, "normal": 1, "drift": 2}
 TRANSFORM = {"null": 0, "mks": 1, "fmks": 2}
 BC = {"periodic": 0, "outflow": 1, "reflecting": 2}
+DRIVER = {"kharma": 0, "simple": 1}
 
 
 def make_params(cfg):
@@ -169,6 +171,7 @@ def make_params(cfg):
     p.zero_polar_flux = int(cfg.get("zero_polar_flux", False))
     p.fix_polar_flux = int(cfg.get("fix_polar_flux", False))
     p.fix_corner_inner = int(cfg.get("fix_corner_inner", False))
+    p.driver = DRIVER[cfg.get("driver", "kharma")]
     return p
 
 

@@ -297,3 +297,61 @@ def test_mhdmode_convergence_bounds(oracle_lib, mode):
         slope = np.polyfit(np.log(res), np.log(L1[:, v]), 1)[0]
         lo = -2.3 if v in (U2, U3) else -2.15
         assert lo <= slope <= -1.94, (mode, key, slope, L1[:, v])
+
+
+# ------------------------------------------------------ driver/type = simple
+def _both_orderings(oracle_lib, cfg, P0, nsteps):
+    out = {}
+    for drv in ("kharma", "simple"):
+        b = _block(oracle_lib, dict(cfg, driver=drv))
+        b.init_state(P0)
+        b.step(nsteps)
+        out[drv] = b.state()
+    return out
+
+
+def test_simple_ordering_periodic_block_bitwise(oracle_lib):
+    """driver/type simple (driver/simple_step.cpp:39-168: inversion and floors on
+    the interior, exchange of the primitives, then the fix-up) against the default
+    ordering (driver/kharma_step.cpp:77-319) on a periodic block with failed
+    inversions and floored zones.  Both invert, floor and fix the same zones from
+    the same values, and every ghost is an exact periodic image, so the whole
+    state agrees bit for bit."""
+    cfg = pr.mhdmode_config(n=16, mode="fast", ndim=3, amp=1e-1)
+    P0 = pr.mhdmode_prims(cfg)
+    rng = np.random.default_rng(3)
+    Pi = P0[(slice(None),) + pr.interior(cfg)]
+    for k, j, i in rng.integers(0, 16, size=(12, 3)):
+        Pi[U1:U3 + 1, k, j, i] = 30.0 * rng.normal(size=3)   # near-luminal, light zones
+        Pi[RHO, k, j, i] = 1e-3
+    P0[(slice(None),) + pr.interior(cfg)] = Pi
+    P0 = pr.periodic_fill(P0, cfg)
+    b = _block(oracle_lib, cfg)
+    b.init_state(P0)
+    fails = floored = 0
+    for _ in range(6):
+        b.step(1)
+        fails += int((interior_of(b.array(4)[None], cfg) > 0).sum())
+        floored += int((interior_of(b.array(5)[None], cfg) > 0).sum())
+    assert fails > 0 and floored > 0                     # fix-ups and floors act
+    out = _both_orderings(oracle_lib, cfg, P0, 6)
+    for a, s in zip(out["kharma"], out["simple"]):
+        assert np.array_equal(a, s)
+
+
+def test_simple_ordering_torus_physical_boundaries(oracle_lib):
+    """The same comparison on a torus block (outflow x1 with the inflow check,
+    reflecting x2 with the polar fixes, floored atmosphere).  The interiors agree;
+    in the ghost zones the simple fill recomputes U.B from P.B = U.B / sqrt(-g)
+    (boundaries.cpp:561-571), which moves it by at most an ulp or two."""
+    cfg = pr.torus_config(48, 32, 1)
+    P0 = pr.torus_prims(cfg)
+    out = _both_orderings(oracle_lib, cfg, P0, 10)
+    (Pk, Uk), (Ps, Us) = out["kharma"], out["simple"]
+    sl = (slice(None),) + pr.interior(cfg)
+    assert np.array_equal(Pk[sl], Ps[sl]) and np.array_equal(Uk[sl], Us[sl])
+    dB = np.abs(Uk[B1:] - Us[B1:])
+    assert dB.max() > 0                                   # the fills do differ ...
+    assert np.all(dB <= 4.5e-16 * np.abs(Uk[B1:]))        # ... by rounding only
+    assert np.all(np.abs(Uk[:B1] - Us[:B1]) <= 4.5e-16 * np.abs(Uk[:B1]))
+    assert np.all(np.abs(Pk - Ps) <= 4.5e-16 * np.abs(Pk))
