@@ -42,6 +42,9 @@ CASES = {
     "torus3d": lambda: pr.torus_config(48, 32, 16),
     # driver/type = simple (driver/simple_step.cpp:39-168): the synced-primitive boundary fill
     "torus2d_simple": lambda: dict(pr.torus_config(64, 48, 1), driver="simple"),
+    # floors/frame (floors.cpp:68): fluid-frame and normal-observer-frame injection
+    "torus2d_fluid": lambda: dict(pr.torus_config(64, 48, 1), floor_frame="fluid"),
+    "torus2d_normal": lambda: dict(pr.torus_config(64, 48, 1), floor_frame="normal"),
 }
 
 
@@ -131,7 +134,7 @@ def test_torus_flux_fixes_ct_divergence_source(oracle_lib, case):
     assert np.all(rel_l1(a, b)[:5] < 1e-12), rel_l1(a, b)
 
 
-@pytest.mark.parametrize("case", ["torus2d", "torus3d", "torus2d_simple"])
+@pytest.mark.parametrize("case", ["torus2d", "torus3d", "torus2d_simple", "torus2d_fluid", "torus2d_normal"])
 def test_torus_utop_floors_ptou_bounds(oracle_lib, case):
     import kharma_amd as kh
     cfg, _, P, U, _ = _evolved(oracle_lib, case, 5)
@@ -176,7 +179,7 @@ def test_torus_utop_floors_ptou_bounds(oracle_lib, case):
         assert np.array_equal(Ug[B1:], Uo[B1:]) and np.array_equal(Pg[B1:], Po[B1:])
 
 
-@pytest.mark.parametrize("case", ["torus2d", "torus3d", "torus2d_simple"])
+@pytest.mark.parametrize("case", ["torus2d", "torus3d", "torus2d_simple", "torus2d_fluid", "torus2d_normal"])
 def test_torus_step_parity_fp64_per_step(oracle_lib, case):
     import kharma_amd as kh
     cfg = CASES[case]()
