@@ -283,8 +283,9 @@ def _bench_adjoint(n=64, nsteps=20, ndirs=64, seed=20261202, h=1e-7):
         PN, _, ck, _ = evo.run(P0, nsteps, dts, checkpoints=True)
         lam = evo._embed(2.0 * (PN - Pt))
         rec = adj = 0.0
+        s1 = ck["stage1"]   # stage 1 of every step, recorded by the forward run
         for k in reversed(range(nsteps)):
-            evo.adj.step(ck[k], dts[k], lam, ck[k + 1])
+            evo.adj.step_recorded(ck[k], s1[k][0], s1[k][1], dts[k], lam, ck[k + 1])
             f, g_ = evo.adj.last_times()
             rec += f
             adj += g_
@@ -308,6 +309,7 @@ def _bench_adjoint(n=64, nsteps=20, ndirs=64, seed=20261202, h=1e-7):
     _l = evo._embed(torch.zeros_like(P0))
     for k in reversed(range(2)):
         evo.adj.step(_w[2][k], _w[1][k], _l, _w[2][k + 1])
+        evo.adj.step_recorded(_w[2][k], _w[2]["stage1"][k][0], _w[2]["stage1"][k][1], _w[1][k], _l, _w[2][k + 1])
     dirs = []
     for _ in range(ndirs):
         v = noise_direction(cfg, rng, (pr.U3, pr.B3))
@@ -315,31 +317,43 @@ def _bench_adjoint(n=64, nsteps=20, ndirs=64, seed=20261202, h=1e-7):
         dirs.append(v / np.linalg.norm(v))
     dts, rec_ms, adj_ms, chk_c4 = grad_and_check(P0, dirs)
     # forward GPU time per step (events on the stream the driver runs on, around
-    # the steps alone; median of three runs)
+    # the steps alone; median of three runs), plain and with the stage-1 record
+    # the differentiable forward run keeps (one state copy + a flag count per step)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    runs = []
-    for _ in range(3):
-        evo.drv.init_prims(evo._embed(P0))
-        torch.cuda.synchronize()
-        e0.record()
-        t = 0.0
-        for k in range(nsteps):
-            evo.drv.set_clock(t, dts[k], k)
-            evo.drv.step(1)
-            t += dts[k]
-        e1.record()
-        torch.cuda.synchronize()
-        runs.append(e0.elapsed_time(e1) / nsteps)
-    fwd_ms = sorted(runs)[1]
+    rec_buf = (torch.empty((pr.NV,) + evo.shape, dtype=torch.float64, device="cuda"),
+               torch.empty(1, dtype=torch.int32, device="cuda"))
+
+    def fwd_time(record):
+        runs = []
+        for _ in range(3):
+            evo.drv.init_prims(evo._embed(P0))
+            if record:
+                evo.drv.set_stage_record(*rec_buf)
+            torch.cuda.synchronize()
+            e0.record()
+            t = 0.0
+            for k in range(nsteps):
+                evo.drv.set_clock(t, dts[k], k)
+                evo.drv.step(1)
+                t += dts[k]
+            e1.record()
+            torch.cuda.synchronize()
+            evo.drv.set_stage_record()
+            runs.append(e0.elapsed_time(e1) / nsteps)
+        return sorted(runs)[1]
+    fwd_plain_ms = fwd_time(False)
+    fwd_ms = fwd_time(True)
     # generic smooth state
     Pg = torch.as_tensor(generic_state(cfg, rng), device="cuda")
     dirs = [smooth_direction(cfg, rng) for _ in range(16)] + [noise_direction(cfg, rng, range(pr.NV)) for _ in range(8)]
     _, _, _, chk_gen = grad_and_check(Pg, dirs)
     back_ms = rec_ms + adj_ms
-    return {"workload": f"Alfven mode {n}^3, amp 1e-4, {nsteps} RK2 steps, FP64, per-step checkpoints",
+    return {"workload": f"Alfven mode {n}^3, amp 1e-4, {nsteps} RK2 steps, FP64, per-step checkpoints "
+                        "with the first stage of each step recorded",
             "loss": "|P_N - P_target|^2, P_target = mode at phase + pi/4",
-            "forward_ms_per_step": fwd_ms, "backward_ms_per_step": back_ms,
-            "backward_recompute_ms_per_step": rec_ms, "backward_transpose_ms_per_step": adj_ms,
+            "forward_ms_per_step": fwd_ms, "forward_plain_ms_per_step": fwd_plain_ms,
+            "backward_ms_per_step": back_ms,
+            "backward_checkpoint_load_ms_per_step": rec_ms, "backward_transpose_ms_per_step": adj_ms,
             "adjoint_over_forward": back_ms / fwd_ms, "fd_step": h, "seed": seed,
             "grad_check": {"c4_state_x3_independent_u3_B3_noise": chk_c4,
                            "generic_state_smooth_and_noise_all_vars": chk_gen}}

@@ -57,7 +57,9 @@ extern "C" {
 /* driver/type (driver/kharma_driver.cpp:60-70): the task ordering of a stage.
  * "kharma" is MakeDefaultTaskCollection (driver/kharma_step.cpp:77-319), "simple"
  * MakeSimpleTaskCollection (driver/simple_step.cpp:39-168).  On one block both
- * orderings give the same state bit for bit; the value is checked and kept. */
+ * orderings invert, floor and fix the same zones from the same values; they
+ * differ only in the physical boundary fill, which "simple" makes from the
+ * synced primitives (see kh_apply_bounds). */
 #define KH_DRIVER_KHARMA 0
 #define KH_DRIVER_SIMPLE 1
 
@@ -254,6 +256,13 @@ int kh_driver_set_kernel_timing(kh_driver* d, int enable);
  * vl2), dA the product of the two transverse zone widths.  var in 0..4;
  * dir = 0 switches it off. */
 int kh_driver_set_face_flux_objective(kh_driver* d, int dir, int var, int face, double weight);
+/* Stage-1 record for the differentiation form (no counterpart in the reference,
+ * whose tasks keep no per-stage history): after stage 1 of every following
+ * kh_driver_step step, P1 (device, [8][nc]) receives the stage's output
+ * primitives, periodic images included, and *flagged (device int) the number of
+ * interior zones the stage floored or failed to invert.  Each step overwrites
+ * them.  Null, null switches recording off. */
+int kh_driver_set_stage_record(kh_driver* d, double* P1, int* flagged);
 /* The accumulated objective (0 if never set); reset != 0 zeroes it.  Synchronises. */
 int kh_driver_objective(kh_driver* d, double* value, int reset);
 /* Sum of launch durations (ms) and launch counts per kind since the last call;
@@ -294,6 +303,12 @@ int kh_adjoint_set_stream(kh_adjoint* a, void* hip_stream);
  * recomputed too.  With a face-flux objective set, lam_P also receives
  * dJ_n/dP_n for the step's share J_n of the objective.  Synchronises. */
 int kh_adjoint_step(kh_adjoint* a, const double* P, const double* P_next, double dt, double* lam_P);
+/* kh_adjoint_step from a recorded stage 1 (kh_driver_set_stage_record of the
+ * step from P to P_next) instead of a recompute: P1 and flagged as recorded.
+ * Flat-space periodic blocks only (KH_ERR_UNSUPPORTED otherwise); returns
+ * KH_ERR_UNSUPPORTED if *flagged is nonzero.  Synchronises. */
+int kh_adjoint_step_recorded(kh_adjoint* a, const double* P, const double* P1, const int* flagged,
+                             const double* P_next, double dt, double* lam_P);
 /* The objective of kh_driver_set_face_flux_objective, same arguments: its
  * derivative is added by every following kh_adjoint_step (dir = 0: none). */
 int kh_adjoint_set_face_flux_objective(kh_adjoint* a, int dir, int var, int face, double weight);

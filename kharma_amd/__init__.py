@@ -133,6 +133,7 @@ def lib():
         "kh_adjoint_destroy": (None, [vp]),
         "kh_adjoint_set_stream": (i, [vp, vp]),
         "kh_adjoint_step": (i, [vp, vp, vp, d, vp]),
+        "kh_adjoint_step_recorded": (i, [vp, vp, vp, vp, vp, d, vp]),
         "kh_adjoint_last_times": (i, [vp, ctypes.POINTER(d), ctypes.POINTER(d)]),
         "kh_adjoint_flux": (i, [vp, vp, vp, vp]),
         "kh_adjoint_flux_div": (i, [vp, vp, vp]),
@@ -145,6 +146,7 @@ def lib():
         "kh_adjoint_fixup": (i, [vp, vp, vp, vp]),
         "kh_adjoint_set_face_flux_objective": (i, [vp, i, i, i, d]),
         "kh_driver_set_face_flux_objective": (i, [vp, i, i, i, d]),
+        "kh_driver_set_stage_record": (i, [vp, vp, vp]),
         "kh_driver_objective": (i, [vp, ctypes.POINTER(d), i]),
     }
     for name, (res, args) in sig.items():
@@ -441,6 +443,15 @@ class Driver:
         _check(lib().kh_driver_kernel_times(self.h, ms, cnt, n))
         return {k: (ms[q], cnt[q]) for q, k in enumerate(self.KERNEL_KINDS)}
 
+    def set_stage_record(self, P1=None, flagged=None):
+        """Record stage 1 of every following step (kh_driver_set_stage_record):
+        P1 (float64 CUDA tensor [8, n3, n2, n1]) gets the stage's output and
+        flagged (int32 CUDA tensor, 1 element) its count of floored or failed
+        interior zones.  No arguments: recording off.  The tensors must outlive
+        the recording."""
+        self._bind()
+        _check(lib().kh_driver_set_stage_record(self.h, _ptr(P1), _ptr(flagged)))
+
     def set_face_flux_objective(self, dir, var, face, weight=1.0):
         """Accumulate J += weight sum_stages w_s dt sum F_dir[var](face) dA during steps
         (kh_driver_set_face_flux_objective; dir = 0 switches it off)."""
@@ -484,6 +495,13 @@ class AdjointStep:
         recomputation of its second stage (its flags must have been checked)."""
         _check(lib().kh_adjoint_set_stream(self.h, _stream()))
         _check(lib().kh_adjoint_step(self.h, _ptr(P), _ptr(P_next), float(dt), _ptr(lamP)))
+
+    def step_recorded(self, P, P1, flagged, dt, lamP, P_next):
+        """step() from the stage 1 recorded by Driver.set_stage_record for the
+        step from P to P_next, without the recompute (flat-space periodic blocks)."""
+        _check(lib().kh_adjoint_set_stream(self.h, _stream()))
+        _check(lib().kh_adjoint_step_recorded(self.h, _ptr(P), _ptr(P1), _ptr(flagged), _ptr(P_next), float(dt),
+                                              _ptr(lamP)))
 
     def _call(self, fn, *args):
         _check(lib().kh_adjoint_set_stream(self.h, _stream()))

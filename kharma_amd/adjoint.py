@@ -74,7 +74,10 @@ class Evolution:
         nsteps, or None, objective J (0 without one)).  With checkpoints the
         second stage of every step is checked for zones the adjoint does not
         cover (floored in flat space; failed to invert with the fixup off); the
-        adjoint step recomputes and checks the first."""
+        adjoint step recomputes and checks the first -- except on flat-space
+        periodic blocks with every = 1, where the driver records each step's
+        first stage and its count of such zones (checkpoints["stage1"]:
+        {step: (P1, count)}) and the adjoint step uses the record instead."""
         self.drv.init_prims(self._embed(P_int.detach()))
         if self.objective:
             self.drv.objective(reset=True)
@@ -85,6 +88,9 @@ class Evolution:
         # periodic flat-space blocks, where it takes stage 2 from the next checkpoint
         recomputes = floors_ok or bool(self.cfg.get("boundaries"))
         fixup_ok = recomputes and (self.cfg.get("fix_average", True) or self.cfg.get("fix_atmo", True))
+        record = checkpoints and every == 1 and not recomputes
+        if record:
+            ckpts["stage1"] = {}
         used = []
         time = 0.0
         for n in range(nsteps):
@@ -92,12 +98,19 @@ class Evolution:
                 self.drv.set_clock(time, float(dts[n]), n)
             if checkpoints and n % every == 0:
                 ckpts[n] = self.drv.state()[0]
+            if record:
+                rec = (torch.empty((pr.NV,) + self.shape, dtype=torch.float64, device="cuda"),
+                       torch.empty(1, dtype=torch.int32, device="cuda"))
+                ckpts["stage1"][n] = rec
+                self.drv.set_stage_record(*rec)
             self.drv.step(1)
             if checkpoints:
                 self._count(flagged, floors_ok, fixup_ok)
             dt = dts[n] if dts is not None else self.drv.dt_history(n + 1)[n]
             used.append(float(dt))
             time += float(dt)
+        if record:
+            self.drv.set_stage_record()
         P, _ = self.drv.state()
         J = self.drv.objective() if self.objective else 0.0
         if checkpoints:
@@ -127,12 +140,16 @@ class Evolution:
         if self.objective:
             o = self.objective
             self.adj.set_face_flux_objective(o["dir"], o["var"], o["face"], o.get("weight", 1.0) * float(gJ))
-        marks = sorted(ckpts)
+        marks = sorted(k for k in ckpts if isinstance(k, int))
+        rec = ckpts.get("stage1", {})
         for q in reversed(range(len(marks) - 1)):
             a, b = marks[q], marks[q + 1]
             states = self._segment_states(ckpts, dts, a, b)
             for n in reversed(range(a, b)):
-                self.adj.step(states[n - a], dts[n], lam, states[n - a + 1])
+                if n in rec:
+                    self.adj.step_recorded(states[n - a], rec[n][0], rec[n][1], dts[n], lam, states[n - a + 1])
+                else:
+                    self.adj.step(states[n - a], dts[n], lam, states[n - a + 1])
         if self.objective:
             self.adj.set_face_flux_objective(0, 0, 0, 0.0)
         return lam[self.inner].contiguous()

@@ -231,6 +231,26 @@ int kh_adjoint_set_stream(kh_adjoint* a, void* stream) {
     return kh_driver_set_stream(a->drv, stream);
 }
 
+// End of an adjoint step: timing marks, then the device error counts.
+static int finish_step(kh_adjoint* a) {
+    hipStream_t s = a->drv->blk->stream;
+    KA_HIP(hipEventRecord(a->ev[2], s));
+    int err[2] = {0, 0};
+    KA_HIP(hipMemcpyAsync(err, a->err, sizeof(err), hipMemcpyDeviceToHost, s));
+    KA_HIP(hipStreamSynchronize(s));
+    KA_HIP(hipEventElapsedTime(&a->fwd_ms, a->ev[0], a->ev[1]));
+    KA_HIP(hipEventElapsedTime(&a->adj_ms, a->ev[1], a->ev[2]));
+    if (err[0] > 0)
+        return api_fail(KH_ERR_UNSUPPORTED,
+                        std::to_string(err[0]) + " zone-stages " +
+                            (a->floors ? "failed to invert with the fixup off" : a->fixup ? "were floored" :
+                             "were floored or failed to invert with the fixup off") +
+                            "; the adjoint does not cover them on this block");
+    if (err[1] > 0)
+        return api_fail(KH_ERR_UNSUPPORTED, std::to_string(err[1]) + " zones have a singular dU/dP");
+    return KH_OK;
+}
+
 int kh_adjoint_step(kh_adjoint* a, const double* P, const double* P_next, double dt, double* lam_P) {
     if (!a || !P || !lam_P) return api_fail(KH_ERR_INVALID, "null argument");
     if (!(dt > 0.)) return api_fail(KH_ERR_INVALID, "dt must be positive");
@@ -273,21 +293,36 @@ int kh_adjoint_step(kh_adjoint* a, const double* P, const double* P_next, double
         return st;
     // U_0 = p_to_u(P_0)
     KA_HIP(launch_adj_ptou(bd, c, b->geo, P0, a->lU0, lam_P, s));
-    KA_HIP(hipEventRecord(a->ev[2], s));
-    int err[2] = {0, 0};
-    KA_HIP(hipMemcpyAsync(err, a->err, sizeof(err), hipMemcpyDeviceToHost, s));
-    KA_HIP(hipStreamSynchronize(s));
-    KA_HIP(hipEventElapsedTime(&a->fwd_ms, a->ev[0], a->ev[1]));
-    KA_HIP(hipEventElapsedTime(&a->adj_ms, a->ev[1], a->ev[2]));
-    if (err[0] > 0)
-        return api_fail(KH_ERR_UNSUPPORTED,
-                        std::to_string(err[0]) + " zone-stages " +
-                            (a->floors ? "failed to invert with the fixup off" : a->fixup ? "were floored" :
-                             "were floored or failed to invert with the fixup off") +
-                            "; the adjoint does not cover them on this block");
-    if (err[1] > 0)
-        return api_fail(KH_ERR_UNSUPPORTED, std::to_string(err[1]) + " zones have a singular dU/dP");
-    return KH_OK;
+    return finish_step(a);
+}
+
+int kh_adjoint_step_recorded(kh_adjoint* a, const double* P, const double* P1, const int* flagged,
+                             const double* P_next, double dt, double* lam_P) {
+    if (!a || !P || !P1 || !flagged || !P_next || !lam_P) return api_fail(KH_ERR_INVALID, "null argument");
+    if (!(dt > 0.)) return api_fail(KH_ERR_INVALID, "dt must be positive");
+    if (a->geo || a->physical)
+        return api_fail(KH_ERR_UNSUPPORTED, "a recorded stage 1 replaces the recompute on flat-space periodic blocks only");
+    kh_driver* d = a->drv;
+    kh_block* b = d->blk;
+    const BlockDims& bd = b->dims;
+    const StepConsts& c = b->consts;
+    hipStream_t s = b->stream;
+    KA_HIP(hipMemsetAsync(a->err, 0, 2 * sizeof(int), s));
+    // the recorded count of stage-1 zones the adjoint does not cover
+    KA_HIP(hipMemcpyAsync(a->err, flagged, sizeof(int), hipMemcpyDeviceToDevice, s));
+    KA_HIP(hipEventRecord(a->ev[0], s));
+    KA_HIP(hipMemcpyAsync(d->P0, P, sizeof(double) * NV * bd.nc, hipMemcpyDeviceToDevice, s));
+    KA_HIP(launch_sync(bd, d->P0, NV, s));
+    KA_HIP(hipEventRecord(a->ev[1], s));
+    int st;
+    if ((st = stage_transpose(a, 2, P1, P_next, lam_P, nullptr, a->lU1, a->lU0, false, false, dt, a->lPin, false,
+                              false)) != KH_OK)
+        return st;
+    if ((st = stage_transpose(a, 1, d->P0, P1, a->lPin, a->lU1, a->lU0, a->lU0, true, true, dt, lam_P, true,
+                              false)) != KH_OK)
+        return st;
+    KA_HIP(launch_adj_ptou(bd, c, b->geo, d->P0, a->lU0, lam_P, s));
+    return finish_step(a);
 }
 
 int kh_adjoint_flux(kh_adjoint* a, const double* P, const double* lam_F, double* lam_P) {

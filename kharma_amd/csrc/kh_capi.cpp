@@ -505,6 +505,12 @@ int kh_driver_step(kh_driver* d, int nsteps) {
         for (int stage = 1; stage <= 2; ++stage) {
             const int st = driver_stage(d, stage);
             if (st != KH_OK) return st;
+            if (stage == 1 && d->rec_P1) {
+                // the stage's output and its count of floored or failed interior zones
+                KH_HIP(hipMemcpyAsync(d->rec_P1, d->P1, sizeof(double) * NV * bd.nc, hipMemcpyDeviceToDevice, s));
+                KH_HIP(launch_fill_int(d->rec_flags, 1, 0, s));
+                KH_HIP(launch_adj_count_flags(bd, d->pflag, d->fflag, d->rec_flags, s));
+            }
         }
         KH_TIMED(7, launch_max_divb(bd, d->U0, d->divb_part, s));
         KH_TIMED(8, launch_finalize(c, d->ds, d->dt_part, n_dt, d->divb_part, bd.ndim > 1 ? n_divb : 0,
@@ -546,6 +552,14 @@ int kh_driver_get_scalars(kh_driver* d, double* time, double* dt, double* last_d
 int kh_driver_set_clock(kh_driver* d, double time, double dt, int ncycle) {
     if (!d || !(dt > 0.) || ncycle < 0) return fail(KH_ERR_INVALID, "bad argument");
     KH_HIP(launch_set_clock(d->ds, time, dt, ncycle, d->blk->stream));
+    return KH_OK;
+}
+
+int kh_driver_set_stage_record(kh_driver* d, double* P1, int* flagged) {
+    if (!d) return fail(KH_ERR_INVALID, "null driver");
+    if ((P1 == nullptr) != (flagged == nullptr)) return fail(KH_ERR_INVALID, "P1 and flagged are set or cleared together");
+    d->rec_P1 = P1;
+    d->rec_flags = flagged;
     return KH_OK;
 }
 

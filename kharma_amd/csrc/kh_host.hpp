@@ -31,6 +31,9 @@ struct kh_driver {
     int obj_dir = 0, obj_var = 0, obj_face = 0;
     double obj_weight = 0.;
     double* obj = nullptr;   // device accumulator
+    // stage-1 record (kh_driver_set_stage_record): caller's device buffers
+    double* rec_P1 = nullptr;
+    int* rec_flags = nullptr;
     // kernel timing (kh_driver_set_kernel_timing)
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;

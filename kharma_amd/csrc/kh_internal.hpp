@@ -220,6 +220,7 @@ hipError_t launch_adj_floors(const BlockDims& b, const StepConsts& c, const Geom
                              const double* lPout, double* lPin, hipStream_t s);
 // err[0] += zones with pflag != 0, or (fflag non-null) fflag != 0.
 hipError_t launch_adj_count_flags(const BlockDims& b, const int* pflag, const int* fflag, int* err, hipStream_t s);
+hipError_t launch_fill_int(int* p, long n, int v, hipStream_t s);
 hipError_t launch_estimate_dt(const BlockDims& b, const StepConsts& c, int prec, const void* cmax, const void* cmin,
                               double* part, hipStream_t s);
 
