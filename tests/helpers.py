@@ -43,26 +43,37 @@ def face_slices(cfg, d):
 SYM = [(0, 0), (0, 1), (0, 2), (0, 3), (1, 1), (1, 2), (1, 3), (2, 2), (2, 3), (3, 3)]
 
 
+_GEOMETRY = {}
+
+
+def _library_geometry(ob, cfg):
+    """The library's host geometry in the oracle's layout, built once per
+    configuration (each kh_geometry_host call builds the whole cache)."""
+    import kharma_amd as kh
+    key = repr(sorted((k, repr(v)) for k, v in cfg.items()))
+    if key not in _GEOMETRY:
+        _GEOMETRY.clear()
+        gcov = ob.geometry(0)
+        gcon = ob.geometry(1)
+        gdet = ob.geometry(2)
+        for ol, loc in ((0, "face1"), (1, "face2"), (2, "center"), (3, "center"), (4, "corner")):
+            for c, (a, b) in enumerate(SYM):
+                v = kh.geometry_host(cfg, loc, c)
+                gcov[ol, ..., a, b] = v
+                gcov[ol, ..., b, a] = v
+                w = kh.geometry_host(cfg, loc, 10 + c)
+                gcon[ol, ..., a, b] = w
+                gcon[ol, ..., b, a] = w
+            gdet[ol] = kh.geometry_host(cfg, loc, 20)
+        conn = np.stack([kh.geometry_host(cfg, "center", 22 + n) for n in range(64)], axis=-1)
+        _GEOMETRY[key] = (gcov, gcon, gdet, conn.reshape(conn.shape[:2] + (4, 4, 4)),
+                          kh.geometry_host(cfg, "center", 88))
+    return _GEOMETRY[key]
+
+
 def install_library_geometry(ob, cfg):
     """Give the oracle block the library's geometry cache, so that a step
     comparison sees identical metric inputs (the caches themselves are compared
     in test_geometry_torus.py).  Oracle loci: face1, face2, face3, centre, corner."""
-    import kharma_amd as kh
-    gcov = ob.geometry(0)
-    gcon = ob.geometry(1)
-    gdet = ob.geometry(2)
-    for ol, loc in ((0, "face1"), (1, "face2"), (2, "center"), (3, "center"), (4, "corner")):
-        for c, (a, b) in enumerate(SYM):
-            v = kh.geometry_host(cfg, loc, c)
-            gcov[ol, ..., a, b] = v
-            gcov[ol, ..., b, a] = v
-            w = kh.geometry_host(cfg, loc, 10 + c)
-            gcon[ol, ..., a, b] = w
-            gcon[ol, ..., b, a] = w
-        gdet[ol] = kh.geometry_host(cfg, loc, 20)
-    conn = np.stack([kh.geometry_host(cfg, "center", 22 + n) for n in range(64)], axis=-1)
-    ob.set_geometry(0, gcov)
-    ob.set_geometry(1, gcon)
-    ob.set_geometry(2, gdet)
-    ob.set_geometry(3, conn.reshape(conn.shape[:2] + (4, 4, 4)))
-    ob.set_geometry(4, kh.geometry_host(cfg, "center", 88))
+    for which, a in enumerate(_library_geometry(ob, cfg)):
+        ob.set_geometry(which, a)

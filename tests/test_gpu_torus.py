@@ -271,3 +271,49 @@ def test_torus_split_precision_against_fp64(oracle_lib):
     print("split vs fp64 oracle rel L1 per variable:", r)
     assert np.all(r[:5] < 1e-6), r
     assert np.isfinite(Pg).all()
+
+
+def test_torus_full_size_one_step(oracle_lib):
+    """The torus workload at its full size (192 x 128 x 128, one block): one FP64
+    driver step against the oracle at the per-step bar (rho, u, B to 1e-12; the
+    velocities and momenta to 10x the oracle's own FMA sensitivity), corner divB
+    at roundoff, and one split-precision step within 1e-6 of the FP64 one."""
+    import kharma_amd as kh
+    cfg = pr.torus_config()
+    P0 = pr.torus_prims(cfg)
+    print("[torus full size] oracle geometry (library cache)", flush=True)
+    ob = _oracle(oracle_lib, cfg)
+    print("[torus full size] oracle steps", flush=True)
+    ob.init_state(P0)
+    ob.step(2)
+    P, U = ob.state()
+    t, dt = ob.time, ob.dt
+    ob.step(1)
+    Po, Uo = ob.state()
+    of = _oracle(oracle_lib, cfg, variant="fma")
+    of.set_state(P, U)
+    of.dt = dt
+    of.step(1)
+    Pf, _ = of.state()
+    out = {}
+    for prec in ("fp64", "split"):
+        print(f"[torus full size] GPU step, {prec}", flush=True)
+        drv = kh.Driver(dict(cfg, precision=prec))
+        drv.init_prims(_cuda(P))
+        drv.set_clock(t, dt, 2)
+        drv.step(1)
+        out[prec] = _np(drv.state()[0])
+        if prec == "fp64":
+            assert drv.scalars()["divb"] < 1e-10
+        del drv
+    sens = rel_l1(interior_of(Pf, cfg), interior_of(Po, cfg))
+    r = rel_l1(interior_of(out["fp64"], cfg), interior_of(Po, cfg))
+    assert np.all(r < np.maximum(1e-12, 10 * sens)), (r, sens)
+    assert np.all(r[[0, 1, 5, 6, 7]] < 1e-12), r
+    Ps, Pd = interior_of(out["split"], cfg), interior_of(out["fp64"], cfg)
+    rs = rel_l1(Ps, Pd)
+    assert np.all(rs[:5] < 1e-6), rs
+    # B3 starts at zero (poloidal seed field), so one step's B3 is an O(dt)
+    # field of EMF differences: the field error is measured against |B| as a whole
+    rb = np.abs(Ps[5:] - Pd[5:]).sum(axis=(1, 2, 3)) / np.abs(Pd[5:]).sum()
+    assert np.all(rb < 1e-6), rb
