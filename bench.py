@@ -283,9 +283,9 @@ def _bench_adjoint(n=64, nsteps=20, ndirs=64, seed=20261202, h=1e-7):
         PN, _, ck, _ = evo.run(P0, nsteps, dts, checkpoints=True)
         lam = evo._embed(2.0 * (PN - Pt))
         rec = adj = 0.0
-        s1 = ck["stage1"]   # stage 1 of every step, recorded by the forward run
+        recs = ck["records"]   # every step's stage intermediates, recorded by the forward run
         for k in reversed(range(nsteps)):
-            evo.adj.step_recorded(ck[k], s1[k][0], s1[k][1], dts[k], lam, ck[k + 1])
+            evo.adj.step_recorded(ck[k], recs[k], dts[k], lam, ck[k + 1])
             f, g_ = evo.adj.last_times()
             rec += f
             adj += g_
@@ -309,7 +309,7 @@ def _bench_adjoint(n=64, nsteps=20, ndirs=64, seed=20261202, h=1e-7):
     _l = evo._embed(torch.zeros_like(P0))
     for k in reversed(range(2)):
         evo.adj.step(_w[2][k], _w[1][k], _l, _w[2][k + 1])
-        evo.adj.step_recorded(_w[2][k], _w[2]["stage1"][k][0], _w[2]["stage1"][k][1], _w[1][k], _l, _w[2][k + 1])
+        evo.adj.step_recorded(_w[2][k], _w[2]["records"][k], _w[1][k], _l, _w[2][k + 1])
     dirs = []
     for _ in range(ndirs):
         v = noise_direction(cfg, rng, (pr.U3, pr.B3))
@@ -318,17 +318,16 @@ def _bench_adjoint(n=64, nsteps=20, ndirs=64, seed=20261202, h=1e-7):
     dts, rec_ms, adj_ms, chk_c4 = grad_and_check(P0, dirs)
     # forward GPU time per step (events on the stream the driver runs on, around
     # the steps alone; median of three runs), plain and with the stage-1 record
-    # the differentiable forward run keeps (one state copy + a flag count per step)
+    # the differentiable forward run keeps (a copy of the stage-1 state and flags per step)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    rec_buf = (torch.empty((pr.NV,) + evo.shape, dtype=torch.float64, device="cuda"),
-               torch.empty(1, dtype=torch.int32, device="cuda"))
+    rec_buf = evo.drv.new_stage_record()
 
     def fwd_time(record):
         runs = []
         for _ in range(3):
             evo.drv.init_prims(evo._embed(P0))
             if record:
-                evo.drv.set_stage_record(*rec_buf)
+                evo.drv.set_stage_record(rec_buf)
             torch.cuda.synchronize()
             e0.record()
             t = 0.0
@@ -349,7 +348,7 @@ def _bench_adjoint(n=64, nsteps=20, ndirs=64, seed=20261202, h=1e-7):
     _, _, _, chk_gen = grad_and_check(Pg, dirs)
     back_ms = rec_ms + adj_ms
     return {"workload": f"Alfven mode {n}^3, amp 1e-4, {nsteps} RK2 steps, FP64, per-step checkpoints "
-                        "with the first stage of each step recorded",
+                        "with each step's stage intermediates recorded",
             "loss": "|P_N - P_target|^2, P_target = mode at phase + pi/4",
             "forward_ms_per_step": fwd_ms, "forward_plain_ms_per_step": fwd_plain_ms,
             "backward_ms_per_step": back_ms,
@@ -367,8 +366,9 @@ def _bench_adjoint_torus(nx1=256, nx2=128, nsteps=500, every=25, ndirs=8, seed=2
     (u = u_0 (1 + delta), the seeded +-4 % jitter) by the adjoint sweep with
     two-level (revolve-style) checkpointing every `every` steps.  Timed with
     HIP events: the plain forward run, the forward run with checkpoints, the
-    backward sweep (segment re-runs + adjoint steps), and the adjoint steps'
-    own recompute / transpose times.  Checked against central differences of
+    backward sweep (segment re-runs that record each step's stage
+    intermediates, then adjoint steps from those records), and the adjoint
+    steps' own record-load / transpose times.  Checked against central differences of
     the driver along `ndirs` random directions of delta (relative step h)."""
     import math
     import numpy as np
@@ -413,28 +413,39 @@ def _bench_adjoint_torus(nx1=256, nx2=128, nsteps=500, every=25, ndirs=8, seed=2
 
     plain_forward()
     fwd_ms = sorted(timed_forward() for _ in range(3))[1]
-    # gradient: forward with checkpoints, backward sweep
+    # gradient: forward with checkpoints, backward sweep (coverage checked once at its end)
     Pg = P0.clone().requires_grad_(True)
     torch.cuda.synchronize()
     ev[0].record()
     _, J = evolve_objective(Pg, evo, nsteps, dts, every=every)
     ev[1].record()
-    rec = adj = 0.0
-    orig_step = evo.adj.step
-
-    def timed_step(*args):
-        nonlocal rec, adj
-        orig_step(*args)
-        f, g_ = evo.adj.last_times()
-        rec += f
-        adj += g_
-    evo.adj.step = timed_step
     J.backward()
-    evo.adj.step = orig_step
     ev[2].record()
     torch.cuda.synchronize()
     ckpt_fwd_ms = ev[0].elapsed_time(ev[1])
     back_ms = ev[1].elapsed_time(ev[2])
+    # the adjoint steps' own record-load / transpose times: a second sweep whose
+    # steps synchronise and time themselves
+    rec = adj = 0.0
+    orig = {name: getattr(evo.adj, name) for name in ("step", "step_recorded")}
+
+    def timed(name):
+        def call(*args):
+            nonlocal rec, adj
+            orig[name](*args)
+            f, g_ = evo.adj.last_times()
+            rec += f
+            adj += g_
+        return call
+    for name in orig:
+        setattr(evo.adj, name, timed(name))
+    evo.deferred_checks = False
+    P2 = P0.clone().requires_grad_(True)
+    evolve_objective(P2, evo, nsteps, dts, every=every)[1].backward()
+    evo.deferred_checks = True
+    for name, fn in orig.items():
+        setattr(evo.adj, name, fn)
+    assert torch.equal(P2.grad, Pg.grad)
     g_delta = (Pg.grad[pr.UU] * u0).cpu().numpy()      # dJ/d delta
     Jv = float(J.detach())
     # central differences along random directions of delta (unit max norm)
@@ -463,7 +474,7 @@ def _bench_adjoint_torus(nx1=256, nx2=128, nsteps=500, every=25, ndirs=8, seed=2
             "gradient_over_forward": (ckpt_fwd_ms + back_ms) / fwd_ms,
             "backward_ms_per_step": back_ms / nsteps,
             "backward_over_forward": back_ms / fwd_ms,
-            "adjoint_step_ms": (rec + adj) / nsteps, "adjoint_step_recompute_ms": rec / nsteps,
+            "adjoint_step_ms": (rec + adj) / nsteps, "adjoint_step_record_load_ms": rec / nsteps,
             "adjoint_step_transpose_ms": adj / nsteps,
             "adjoint_step_over_forward_step": (rec + adj) / nsteps / fwd_step,
             "grad_check": {"directions": int(ndirs), "fd_rel_step": h, "seed": seed,

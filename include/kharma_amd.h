@@ -256,13 +256,16 @@ int kh_driver_set_kernel_timing(kh_driver* d, int enable);
  * vl2), dA the product of the two transverse zone widths.  var in 0..4;
  * dir = 0 switches it off. */
 int kh_driver_set_face_flux_objective(kh_driver* d, int dir, int var, int face, double weight);
-/* Stage-1 record for the differentiation form (no counterpart in the reference,
- * whose tasks keep no per-stage history): after stage 1 of every following
- * kh_driver_step step, P1 (device, [8][nc]) receives the stage's output
- * primitives, periodic images included, and *flagged (device int) the number of
- * interior zones the stage floored or failed to invert.  Each step overwrites
- * them.  Null, null switches recording off. */
-int kh_driver_set_stage_record(kh_driver* d, double* P1, int* flagged);
+/* Step records for the differentiation form (no counterpart in the reference,
+ * whose tasks keep no per-stage history).  kh_driver_stage_record_bytes gives
+ * the size of one step's record.  With a record set (a device buffer of that
+ * size, 256-byte aligned), every following kh_driver_step step of the FP64
+ * driver writes into it what kh_adjoint_step_recorded needs in place of a
+ * recompute: stage 1's output primitives, each stage's U* (curved space),
+ * inversion and floor flags, clamped X1 mass fluxes and failed-inversion
+ * count.  Each step overwrites the record; NULL switches recording off. */
+int kh_driver_stage_record_bytes(kh_driver* d, size_t* bytes);
+int kh_driver_set_stage_record(kh_driver* d, void* rec);
 /* The accumulated objective (0 if never set); reset != 0 zeroes it.  Synchronises. */
 int kh_driver_objective(kh_driver* d, double* value, int reset);
 /* Sum of launch durations (ms) and launch counts per kind since the last call;
@@ -303,12 +306,19 @@ int kh_adjoint_set_stream(kh_adjoint* a, void* hip_stream);
  * recomputed too.  With a face-flux objective set, lam_P also receives
  * dJ_n/dP_n for the step's share J_n of the objective.  Synchronises. */
 int kh_adjoint_step(kh_adjoint* a, const double* P, const double* P_next, double dt, double* lam_P);
-/* kh_adjoint_step from a recorded stage 1 (kh_driver_set_stage_record of the
- * step from P to P_next) instead of a recompute: P1 and flagged as recorded.
- * Flat-space periodic blocks only (KH_ERR_UNSUPPORTED otherwise); returns
- * KH_ERR_UNSUPPORTED if *flagged is nonzero.  Synchronises. */
-int kh_adjoint_step_recorded(kh_adjoint* a, const double* P, const double* P1, const int* flagged,
-                             const double* P_next, double dt, double* lam_P);
+/* kh_adjoint_step from the record of the step from P to P_next
+ * (kh_driver_set_stage_record, a driver with the same parameters) instead of
+ * recomputing its stages.  Same coverage and errors as kh_adjoint_step.
+ * Synchronises. */
+int kh_adjoint_step_recorded(kh_adjoint* a, const double* P, const void* rec, const double* P_next, double dt,
+                             double* lam_P);
+/* Deferred checks (on != 0): the following adjoint steps neither synchronise nor
+ * report zones they do not cover; the counts accumulate on the device until
+ * kh_adjoint_check, which synchronises, returns the status a step would have
+ * returned for them, and clears them.  kh_adjoint_last_times is not updated
+ * while deferred.  Switching either way clears the counts. */
+int kh_adjoint_set_deferred_checks(kh_adjoint* a, int on);
+int kh_adjoint_check(kh_adjoint* a);
 /* The objective of kh_driver_set_face_flux_objective, same arguments: its
  * derivative is added by every following kh_adjoint_step (dir = 0: none). */
 int kh_adjoint_set_face_flux_objective(kh_adjoint* a, int dir, int var, int face, double weight);

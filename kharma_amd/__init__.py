@@ -133,7 +133,9 @@ def lib():
         "kh_adjoint_destroy": (None, [vp]),
         "kh_adjoint_set_stream": (i, [vp, vp]),
         "kh_adjoint_step": (i, [vp, vp, vp, d, vp]),
-        "kh_adjoint_step_recorded": (i, [vp, vp, vp, vp, vp, d, vp]),
+        "kh_adjoint_step_recorded": (i, [vp, vp, vp, vp, d, vp]),
+        "kh_adjoint_set_deferred_checks": (i, [vp, i]),
+        "kh_adjoint_check": (i, [vp]),
         "kh_adjoint_last_times": (i, [vp, ctypes.POINTER(d), ctypes.POINTER(d)]),
         "kh_adjoint_flux": (i, [vp, vp, vp, vp]),
         "kh_adjoint_flux_div": (i, [vp, vp, vp]),
@@ -146,7 +148,8 @@ def lib():
         "kh_adjoint_fixup": (i, [vp, vp, vp, vp]),
         "kh_adjoint_set_face_flux_objective": (i, [vp, i, i, i, d]),
         "kh_driver_set_face_flux_objective": (i, [vp, i, i, i, d]),
-        "kh_driver_set_stage_record": (i, [vp, vp, vp]),
+        "kh_driver_set_stage_record": (i, [vp, vp]),
+        "kh_driver_stage_record_bytes": (i, [vp, ctypes.POINTER(ctypes.c_size_t)]),
         "kh_driver_objective": (i, [vp, ctypes.POINTER(d), i]),
     }
     for name, (res, args) in sig.items():
@@ -443,14 +446,17 @@ class Driver:
         _check(lib().kh_driver_kernel_times(self.h, ms, cnt, n))
         return {k: (ms[q], cnt[q]) for q, k in enumerate(self.KERNEL_KINDS)}
 
-    def set_stage_record(self, P1=None, flagged=None):
-        """Record stage 1 of every following step (kh_driver_set_stage_record):
-        P1 (float64 CUDA tensor [8, n3, n2, n1]) gets the stage's output and
-        flagged (int32 CUDA tensor, 1 element) its count of floored or failed
-        interior zones.  No arguments: recording off.  The tensors must outlive
-        the recording."""
+    def new_stage_record(self):
+        """A device buffer (uint8 CUDA tensor) for one step's record."""
+        n = ctypes.c_size_t()
+        _check(lib().kh_driver_stage_record_bytes(self.h, ctypes.byref(n)))
+        return _torch().empty(n.value, dtype=_torch().uint8, device="cuda")
+
+    def set_stage_record(self, rec=None):
+        """Record every following step into rec (kh_driver_set_stage_record);
+        None: recording off.  rec must outlive the recording."""
         self._bind()
-        _check(lib().kh_driver_set_stage_record(self.h, _ptr(P1), _ptr(flagged)))
+        _check(lib().kh_driver_set_stage_record(self.h, _ptr(rec)))
 
     def set_face_flux_objective(self, dir, var, face, weight=1.0):
         """Accumulate J += weight sum_stages w_s dt sum F_dir[var](face) dA during steps
@@ -496,12 +502,22 @@ class AdjointStep:
         _check(lib().kh_adjoint_set_stream(self.h, _stream()))
         _check(lib().kh_adjoint_step(self.h, _ptr(P), _ptr(P_next), float(dt), _ptr(lamP)))
 
-    def step_recorded(self, P, P1, flagged, dt, lamP, P_next):
-        """step() from the stage 1 recorded by Driver.set_stage_record for the
-        step from P to P_next, without the recompute (flat-space periodic blocks)."""
+    def step_recorded(self, P, rec, dt, lamP, P_next):
+        """step() from the record the driver kept of the step from P to P_next
+        (Driver.set_stage_record), without recomputing its stages."""
         _check(lib().kh_adjoint_set_stream(self.h, _stream()))
-        _check(lib().kh_adjoint_step_recorded(self.h, _ptr(P), _ptr(P1), _ptr(flagged), _ptr(P_next), float(dt),
-                                              _ptr(lamP)))
+        _check(lib().kh_adjoint_step_recorded(self.h, _ptr(P), _ptr(rec), _ptr(P_next), float(dt), _ptr(lamP)))
+
+    def set_deferred_checks(self, on):
+        """Deferred checks (kh_adjoint_set_deferred_checks): steps stop
+        synchronising; check() reports what they found."""
+        _check(lib().kh_adjoint_set_stream(self.h, _stream()))
+        _check(lib().kh_adjoint_set_deferred_checks(self.h, int(bool(on))))
+
+    def check(self):
+        """Raise KharmaError for zones the deferred steps did not cover (kh_adjoint_check)."""
+        _check(lib().kh_adjoint_set_stream(self.h, _stream()))
+        _check(lib().kh_adjoint_check(self.h))
 
     def _call(self, fn, *args):
         _check(lib().kh_adjoint_set_stream(self.h, _stream()))

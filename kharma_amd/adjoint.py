@@ -44,6 +44,8 @@ class Evolution:
         self.adj = kh.AdjointStep(self.cfg)
         self.ndim, self.shape = pr.block_shape(self.cfg)
         self.inner = (slice(None),) + pr.interior(self.cfg)
+        # the sweep checks the steps' coverage once, at its end (no per-step sync)
+        self.deferred_checks = True
         self.objective = dict(objective) if objective else None
         if self.objective:
             o = self.objective
@@ -74,10 +76,10 @@ class Evolution:
         nsteps, or None, objective J (0 without one)).  With checkpoints the
         second stage of every step is checked for zones the adjoint does not
         cover (floored in flat space; failed to invert with the fixup off); the
-        adjoint step recomputes and checks the first -- except on flat-space
-        periodic blocks with every = 1, where the driver records each step's
-        first stage and its count of such zones (checkpoints["stage1"]:
-        {step: (P1, count)}) and the adjoint step uses the record instead."""
+        adjoint step recomputes and checks the first.  With every = 1 the
+        driver also records each step's stage intermediates
+        (checkpoints["records"]: {step: record}), which the adjoint step then
+        uses instead of recomputing the stages."""
         self.drv.init_prims(self._embed(P_int.detach()))
         if self.objective:
             self.drv.objective(reset=True)
@@ -88,9 +90,9 @@ class Evolution:
         # periodic flat-space blocks, where it takes stage 2 from the next checkpoint
         recomputes = floors_ok or bool(self.cfg.get("boundaries"))
         fixup_ok = recomputes and (self.cfg.get("fix_average", True) or self.cfg.get("fix_atmo", True))
-        record = checkpoints and every == 1 and not recomputes
+        record = checkpoints and every == 1
         if record:
-            ckpts["stage1"] = {}
+            ckpts["records"] = {}
         used = []
         time = 0.0
         for n in range(nsteps):
@@ -99,10 +101,8 @@ class Evolution:
             if checkpoints and n % every == 0:
                 ckpts[n] = self.drv.state()[0]
             if record:
-                rec = (torch.empty((pr.NV,) + self.shape, dtype=torch.float64, device="cuda"),
-                       torch.empty(1, dtype=torch.int32, device="cuda"))
-                ckpts["stage1"][n] = rec
-                self.drv.set_stage_record(*rec)
+                ckpts["records"][n] = self.drv.new_stage_record()
+                self.drv.set_stage_record(ckpts["records"][n])
             self.drv.step(1)
             if checkpoints:
                 self._count(flagged, floors_ok, fixup_ok)
@@ -119,19 +119,26 @@ class Evolution:
         return P[self.inner].contiguous(), used, ckpts, J
 
     def _segment_states(self, ckpts, dts, a, b):
-        """States P_a .. P_b (full), re-run from the checkpoint at step a unless all are held."""
+        """States P_a .. P_b (full) and the records of steps a .. b-1 (None where
+        not held): re-run from the checkpoint at step a, recording, unless all
+        states are held."""
+        held = ckpts.get("records", {})
         if all(n in ckpts for n in range(a, b + 1)):
-            return [ckpts[n] for n in range(a, b + 1)]
+            return [ckpts[n] for n in range(a, b + 1)], [held.get(n) for n in range(a, b)]
         self.drv.init_prims(ckpts[a])
-        states = [ckpts[a]]
+        states, recs = [ckpts[a]], []
         time = float(sum(dts[:a]))
-        for n in range(a, b - 1):
+        for n in range(a, b):
             self.drv.set_clock(time, float(dts[n]), n)
+            recs.append(self.drv.new_stage_record())
+            self.drv.set_stage_record(recs[-1])
             self.drv.step(1)
             time += float(dts[n])
-            states.append(self.drv.state()[0])
+            if n < b - 1:
+                states.append(self.drv.state()[0])
+        self.drv.set_stage_record()
         states.append(ckpts[b])
-        return states
+        return states, recs
 
     def backward(self, lam_int, ckpts, dts, gJ=0.0):
         """dL/dP0 (interior) from dL/dP_N (interior) and dL/dJ by the adjoint sweep."""
@@ -141,17 +148,22 @@ class Evolution:
             o = self.objective
             self.adj.set_face_flux_objective(o["dir"], o["var"], o["face"], o.get("weight", 1.0) * float(gJ))
         marks = sorted(k for k in ckpts if isinstance(k, int))
-        rec = ckpts.get("stage1", {})
-        for q in reversed(range(len(marks) - 1)):
-            a, b = marks[q], marks[q + 1]
-            states = self._segment_states(ckpts, dts, a, b)
-            for n in reversed(range(a, b)):
-                if n in rec:
-                    self.adj.step_recorded(states[n - a], rec[n][0], rec[n][1], dts[n], lam, states[n - a + 1])
-                else:
-                    self.adj.step(states[n - a], dts[n], lam, states[n - a + 1])
-        if self.objective:
-            self.adj.set_face_flux_objective(0, 0, 0, 0.0)
+        self.adj.set_deferred_checks(self.deferred_checks)
+        try:
+            for q in reversed(range(len(marks) - 1)):
+                a, b = marks[q], marks[q + 1]
+                states, recs = self._segment_states(ckpts, dts, a, b)
+                for n in reversed(range(a, b)):
+                    if recs[n - a] is not None:
+                        self.adj.step_recorded(states[n - a], recs[n - a], dts[n], lam, states[n - a + 1])
+                    else:
+                        self.adj.step(states[n - a], dts[n], lam, states[n - a + 1])
+            if self.deferred_checks:
+                self.adj.check()
+        finally:
+            self.adj.set_deferred_checks(False)
+            if self.objective:
+                self.adj.set_face_flux_objective(0, 0, 0, 0.0)
         return lam[self.inner].contiguous()
 
 

@@ -1239,15 +1239,6 @@ hipError_t launch_adj_floors(const BlockDims& b, const StepConsts& c, const Geom
     return hipGetLastError();
 }
 
-__global__ void __launch_bounds__(ABS) fill_int_kernel(int* __restrict__ p, long n, int v) {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n) p[t] = v;
-}
-hipError_t launch_fill_int(int* p, long n, int v, hipStream_t s) {
-    hipLaunchKernelGGL(fill_int_kernel, agrid(n), dim3(ABS), 0, s, p, n, v);
-    return hipGetLastError();
-}
-
 hipError_t launch_adj_count_flags(const BlockDims& b, const int* pflag, const int* fflag, int* err, hipStream_t s) {
     hipLaunchKernelGGL(adj_count_flags_kernel, agrid(ninterior_l(b)), dim3(ABS), 0, s, b, pflag, fflag, err);
     return hipGetLastError();

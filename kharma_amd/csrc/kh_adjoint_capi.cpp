@@ -17,6 +17,7 @@
 #include "kharma_amd.h"
 
 #include <cstddef>
+#include <cstdint>
 #include <string>
 
 #include "kh_host.hpp"
@@ -48,6 +49,7 @@ struct kh_adjoint {
     double obj_weight = 0.;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     float fwd_ms = 0.f, adj_ms = 0.f;
+    bool deferred = false;   // kh_adjoint_set_deferred_checks: counts accumulate, no per-step sync
 };
 
 namespace {
@@ -231,15 +233,8 @@ int kh_adjoint_set_stream(kh_adjoint* a, void* stream) {
     return kh_driver_set_stream(a->drv, stream);
 }
 
-// End of an adjoint step: timing marks, then the device error counts.
-static int finish_step(kh_adjoint* a) {
-    hipStream_t s = a->drv->blk->stream;
-    KA_HIP(hipEventRecord(a->ev[2], s));
-    int err[2] = {0, 0};
-    KA_HIP(hipMemcpyAsync(err, a->err, sizeof(err), hipMemcpyDeviceToHost, s));
-    KA_HIP(hipStreamSynchronize(s));
-    KA_HIP(hipEventElapsedTime(&a->fwd_ms, a->ev[0], a->ev[1]));
-    KA_HIP(hipEventElapsedTime(&a->adj_ms, a->ev[1], a->ev[2]));
+// The device error counts (after a synchronisation) as a status.
+static int report_errors(kh_adjoint* a, const int err[2]) {
     if (err[0] > 0)
         return api_fail(KH_ERR_UNSUPPORTED,
                         std::to_string(err[0]) + " zone-stages " +
@@ -251,6 +246,43 @@ static int finish_step(kh_adjoint* a) {
     return KH_OK;
 }
 
+// Start of an adjoint step: clear the error counts unless they accumulate.
+static int begin_step(kh_adjoint* a) {
+    if (!a->deferred) KA_HIP(hipMemsetAsync(a->err, 0, 2 * sizeof(int), a->drv->blk->stream));
+    return KH_OK;
+}
+
+// End of an adjoint step: timing marks, then the device error counts (deferred
+// mode: left on the device for kh_adjoint_check).
+static int finish_step(kh_adjoint* a) {
+    hipStream_t s = a->drv->blk->stream;
+    KA_HIP(hipEventRecord(a->ev[2], s));
+    if (a->deferred) return KH_OK;
+    int err[2] = {0, 0};
+    KA_HIP(hipMemcpyAsync(err, a->err, sizeof(err), hipMemcpyDeviceToHost, s));
+    KA_HIP(hipStreamSynchronize(s));
+    KA_HIP(hipEventElapsedTime(&a->fwd_ms, a->ev[0], a->ev[1]));
+    KA_HIP(hipEventElapsedTime(&a->adj_ms, a->ev[1], a->ev[2]));
+    return report_errors(a, err);
+}
+
+int kh_adjoint_set_deferred_checks(kh_adjoint* a, int on) {
+    if (!a) return api_fail(KH_ERR_INVALID, "null adjoint");
+    a->deferred = on != 0;
+    KA_HIP(hipMemsetAsync(a->err, 0, 2 * sizeof(int), a->drv->blk->stream));
+    return KH_OK;
+}
+
+int kh_adjoint_check(kh_adjoint* a) {
+    if (!a) return api_fail(KH_ERR_INVALID, "null adjoint");
+    hipStream_t s = a->drv->blk->stream;
+    int err[2] = {0, 0};
+    KA_HIP(hipMemcpyAsync(err, a->err, sizeof(err), hipMemcpyDeviceToHost, s));
+    KA_HIP(hipStreamSynchronize(s));
+    KA_HIP(hipMemsetAsync(a->err, 0, 2 * sizeof(int), s));
+    return report_errors(a, err);
+}
+
 int kh_adjoint_step(kh_adjoint* a, const double* P, const double* P_next, double dt, double* lam_P) {
     if (!a || !P || !lam_P) return api_fail(KH_ERR_INVALID, "null argument");
     if (!(dt > 0.)) return api_fail(KH_ERR_INVALID, "dt must be positive");
@@ -260,7 +292,7 @@ int kh_adjoint_step(kh_adjoint* a, const double* P, const double* P_next, double
     const StepConsts& c = b->consts;
     hipStream_t s = b->stream;
     const size_t sb = sizeof(double) * NV * bd.nc;
-    KA_HIP(hipMemsetAsync(a->err, 0, 2 * sizeof(int), s));
+    if (const int st0 = begin_step(a)) return st0;
     KA_HIP(hipEventRecord(a->ev[0], s));
     // ---- forward recompute from the checkpoint: P_0 (ghosts as checkpointed; periodic
     // images refilled), U_0 = p_to_u(P_0), stage 1
@@ -296,30 +328,59 @@ int kh_adjoint_step(kh_adjoint* a, const double* P, const double* P_next, double
     return finish_step(a);
 }
 
-int kh_adjoint_step_recorded(kh_adjoint* a, const double* P, const double* P1, const int* flagged,
-                             const double* P_next, double dt, double* lam_P) {
-    if (!a || !P || !P1 || !flagged || !P_next || !lam_P) return api_fail(KH_ERR_INVALID, "null argument");
+int kh_adjoint_step_recorded(kh_adjoint* a, const double* P, const void* rec, const double* P_next, double dt,
+                             double* lam_P) {
+    if (!a || !P || !rec || !P_next || !lam_P) return api_fail(KH_ERR_INVALID, "null argument");
     if (!(dt > 0.)) return api_fail(KH_ERR_INVALID, "dt must be positive");
-    if (a->geo || a->physical)
-        return api_fail(KH_ERR_UNSUPPORTED, "a recorded stage 1 replaces the recompute on flat-space periodic blocks only");
+    if (reinterpret_cast<uintptr_t>(rec) & 255) return api_fail(KH_ERR_INVALID, "a step record must be 256-byte aligned");
     kh_driver* d = a->drv;
     kh_block* b = d->blk;
     const BlockDims& bd = b->dims;
     const StepConsts& c = b->consts;
     hipStream_t s = b->stream;
-    KA_HIP(hipMemsetAsync(a->err, 0, 2 * sizeof(int), s));
-    // the recorded count of stage-1 zones the adjoint does not cover
-    KA_HIP(hipMemcpyAsync(a->err, flagged, sizeof(int), hipMemcpyDeviceToDevice, s));
+    const RecordLayout L = record_layout(bd);
+    const char* r = static_cast<const char*>(rec);
+    auto at_d = [&](size_t off) { return reinterpret_cast<double*>(const_cast<char*>(r + off)); };
+    auto at_i = [&](size_t off) { return reinterpret_cast<int*>(const_cast<char*>(r + off)); };
+    if (const int st0 = begin_step(a)) return st0;
     KA_HIP(hipEventRecord(a->ev[0], s));
+    // P_0 with its periodic images, the stages' failed-inversion counts (the
+    // fix-up transposes' gates), and the count of zones the adjoint does not
+    // cover, by the rule of the recompute
     KA_HIP(hipMemcpyAsync(d->P0, P, sizeof(double) * NV * bd.nc, hipMemcpyDeviceToDevice, s));
-    KA_HIP(launch_sync(bd, d->P0, NV, s));
+    if (bd.periodic[0] || (bd.ndim > 1 && bd.periodic[1]) || (bd.ndim > 2 && bd.periodic[2]))
+        KA_HIP(launch_sync(bd, d->P0, NV, s));
+    KA_HIP(hipMemcpyAsync(reinterpret_cast<char*>(d->ds) + offsetof(DeviceState, fail_count), r + L.fails,
+                          2 * sizeof(int), hipMemcpyDeviceToDevice, s));
+    for (int q = 0; q < 2; ++q) {
+        if (a->fixup) {
+            if (!a->floors) KA_HIP(launch_adj_count_flags(bd, at_i(L.fflag[q]), nullptr, a->err, s));
+        } else {
+            KA_HIP(launch_adj_count_flags(bd, at_i(L.pflag[q]), a->floors ? nullptr : at_i(L.fflag[q]), a->err, s));
+        }
+    }
     KA_HIP(hipEventRecord(a->ev[1], s));
+    // the transposes read the recorded intermediates in place of the recompute's
+    struct Swap {
+        kh_adjoint* a;
+        double* Ustar[2]; int* pfs[2]; int* ffs[2]; double* x1f[2];
+        ~Swap() {
+            for (int q = 0; q < 2; ++q) { a->Ustar[q] = Ustar[q]; a->pfs[q] = pfs[q]; a->ffs[q] = ffs[q]; a->x1f[q] = x1f[q]; }
+        }
+    } keep{a, {a->Ustar[0], a->Ustar[1]}, {a->pfs[0], a->pfs[1]}, {a->ffs[0], a->ffs[1]}, {a->x1f[0], a->x1f[1]}};
+    for (int q = 0; q < 2; ++q) {
+        if (a->geo) a->Ustar[q] = at_d(L.Ustar[q]);
+        if (a->fixup) a->pfs[q] = at_i(L.pflag[q]);
+        if (a->floors) a->ffs[q] = at_i(L.fflag[q]);
+        if (a->physical) a->x1f[q] = at_d(L.x1f[q]);
+    }
+    const double* P1 = at_d(L.P1);
     int st;
     if ((st = stage_transpose(a, 2, P1, P_next, lam_P, nullptr, a->lU1, a->lU0, false, false, dt, a->lPin, false,
-                              false)) != KH_OK)
+                              a->fixup)) != KH_OK)
         return st;
     if ((st = stage_transpose(a, 1, d->P0, P1, a->lPin, a->lU1, a->lU0, a->lU0, true, true, dt, lam_P, true,
-                              false)) != KH_OK)
+                              a->fixup)) != KH_OK)
         return st;
     KA_HIP(launch_adj_ptou(bd, c, b->geo, d->P0, a->lU0, lam_P, s));
     return finish_step(a);
@@ -346,7 +407,7 @@ int kh_adjoint_utop(kh_adjoint* a, const double* P, const double* lam_P, double*
     // lam_U* with no base-state or dU/dt consumers: the stage transpose with
     // g0 = 0, g1 = 1, same = true accumulates lam_U* into lam_U (cleared first).
     KA_HIP(launch_adj_fill(lam_U, (long)NV * b->dims.nc, 0., b->stream));
-    KA_HIP(hipMemsetAsync(a->err, 0, 2 * sizeof(int), b->stream));
+    if (const int st0 = begin_step(a)) return st0;
     KA_HIP(launch_adj_stage(b->dims, b->consts, b->geo, P, lam_P, nullptr, 0., 1., 0., true, true, lam_U, lam_U,
                             a->ld, a->err, nullptr, P, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
                             nullptr, b->stream));

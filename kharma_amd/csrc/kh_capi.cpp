@@ -7,6 +7,8 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <cstddef>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -484,11 +486,32 @@ int kh::driver_stage(kh_driver* d, int stage) {
     if (d->obj_dir)
         KH_HIP(launch_face_flux_objective(bd, prec, d->F, d->obj_dir, d->obj_var, d->obj_face,
                                           d->obj_weight * stage_flux_weight(c, stage), d->ds, d->obj, s));
-    KH_TIMED(3 + stage, launch_update(bd, c, b->geo, prec, stage, stage == 2, d->F, d->ctop, Usub, d->U0, Psub,
-                                      Uout, Pout, Poutf, d->pflag, d->fflag, d->fail_list, d->ds, d->dt_part, s));
+    const RecordLayout L = record_layout(bd);
+    const int q = stage - 1;
+    if (d->rec && physical)
+        KH_HIP(launch_save_x1_flux(bd, (const double*)d->F, (double*)(d->rec + L.x1f[q]), s));
+    if (d->rec) {
+        KH_TIMED(3 + stage, launch_update_recorded(bd, c, b->geo, stage, stage == 2, d->F, d->ctop, Usub, d->U0, Psub,
+                                                   Uout, Pout, d->pflag, d->fflag, d->fail_list, d->ds, d->dt_part,
+                                                   (double*)(d->rec + L.Ustar[q]), s));
+    } else {
+        KH_TIMED(3 + stage, launch_update(bd, c, b->geo, prec, stage, stage == 2, d->F, d->ctop, Usub, d->U0, Psub,
+                                          Uout, Pout, Poutf, d->pflag, d->fflag, d->fail_list, d->ds, d->dt_part, s));
+    }
     if (c.fix_average || c.fix_atmo)
         KH_TIMED(6, launch_fixup(bd, c, b->geo, stage, Uout, Pout, Poutf, d->pflag, d->fail_list, d->ds, s));
     if (physical) KH_TIMED(3, launch_bounds(bd, c, b->geo, Uout, Pout, Poutf, s));
+    if (d->rec) {
+        // the stage's flags (after the fix-up), failed-inversion count and, for stage 1, output
+        const size_t fb = sizeof(int) * bd.nc;
+        KH_HIP(hipMemcpyAsync(d->rec + L.pflag[q], d->pflag, fb, hipMemcpyDeviceToDevice, s));
+        KH_HIP(hipMemcpyAsync(d->rec + L.fflag[q], d->fflag, fb, hipMemcpyDeviceToDevice, s));
+        KH_HIP(hipMemcpyAsync(d->rec + L.fails + sizeof(int) * q,
+                              reinterpret_cast<const char*>(d->ds) + offsetof(DeviceState, fail_count) + sizeof(int) * q,
+                              sizeof(int), hipMemcpyDeviceToDevice, s));
+        if (stage == 1)
+            KH_HIP(hipMemcpyAsync(d->rec + L.P1, Pout, sizeof(double) * NV * bd.nc, hipMemcpyDeviceToDevice, s));
+    }
     return KH_OK;
 }
 
@@ -505,12 +528,6 @@ int kh_driver_step(kh_driver* d, int nsteps) {
         for (int stage = 1; stage <= 2; ++stage) {
             const int st = driver_stage(d, stage);
             if (st != KH_OK) return st;
-            if (stage == 1 && d->rec_P1) {
-                // the stage's output and its count of floored or failed interior zones
-                KH_HIP(hipMemcpyAsync(d->rec_P1, d->P1, sizeof(double) * NV * bd.nc, hipMemcpyDeviceToDevice, s));
-                KH_HIP(launch_fill_int(d->rec_flags, 1, 0, s));
-                KH_HIP(launch_adj_count_flags(bd, d->pflag, d->fflag, d->rec_flags, s));
-            }
         }
         KH_TIMED(7, launch_max_divb(bd, d->U0, d->divb_part, s));
         KH_TIMED(8, launch_finalize(c, d->ds, d->dt_part, n_dt, d->divb_part, bd.ndim > 1 ? n_divb : 0,
@@ -555,11 +572,19 @@ int kh_driver_set_clock(kh_driver* d, double time, double dt, int ncycle) {
     return KH_OK;
 }
 
-int kh_driver_set_stage_record(kh_driver* d, double* P1, int* flagged) {
+int kh_driver_stage_record_bytes(kh_driver* d, size_t* bytes) {
+    if (!d || !bytes) return fail(KH_ERR_INVALID, "null argument");
+    *bytes = record_layout(d->blk->dims).total;
+    return KH_OK;
+}
+
+int kh_driver_set_stage_record(kh_driver* d, void* rec) {
     if (!d) return fail(KH_ERR_INVALID, "null driver");
-    if ((P1 == nullptr) != (flagged == nullptr)) return fail(KH_ERR_INVALID, "P1 and flagged are set or cleared together");
-    d->rec_P1 = P1;
-    d->rec_flags = flagged;
+    if (rec && d->blk->p.precision != KH_PREC_FP64)
+        return fail(KH_ERR_UNSUPPORTED, "step records are kept by the FP64 driver (the adjoint's form)");
+    if (rec && (reinterpret_cast<uintptr_t>(rec) & 255))
+        return fail(KH_ERR_INVALID, "a step record must be 256-byte aligned");
+    d->rec = static_cast<char*>(rec);
     return KH_OK;
 }
 

@@ -31,9 +31,8 @@ struct kh_driver {
     int obj_dir = 0, obj_var = 0, obj_face = 0;
     double obj_weight = 0.;
     double* obj = nullptr;   // device accumulator
-    // stage-1 record (kh_driver_set_stage_record): caller's device buffers
-    double* rec_P1 = nullptr;
-    int* rec_flags = nullptr;
+    // step record (kh_driver_set_stage_record): caller's device buffer, or null
+    char* rec = nullptr;
     // kernel timing (kh_driver_set_kernel_timing)
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
@@ -65,4 +64,26 @@ inline double stage_flux_weight(const StepConsts& c, int stage) {
 }
 // Checks a face-flux objective's (dir, var, face) against the block.
 int check_face_objective(const BlockDims& b, int dir, int var, int face);
+
+// Byte offsets of one step's record (kh_driver_set_stage_record) in its buffer:
+// stage 1's output primitives, each stage's U* (curved-space blocks), inversion
+// and floor flags, clamped X1 mass fluxes (physical X1 faces) and failed-
+// inversion count.  Sections start on 256-byte boundaries.
+struct RecordLayout {
+    size_t P1, Ustar[2], pflag[2], fflag[2], x1f[2], fails, total;
+};
+inline RecordLayout record_layout(const BlockDims& b) {
+    RecordLayout L{};
+    size_t o = 0;
+    auto take = [&](size_t bytes) { const size_t at = o; o += (bytes + 255) / 256 * 256; return at; };
+    const size_t sb = sizeof(double) * NV * b.nc;
+    L.P1 = take(sb);
+    for (int q = 0; q < 2; ++q) L.Ustar[q] = take(b.geo ? sb : 0);
+    for (int q = 0; q < 2; ++q) L.pflag[q] = take(sizeof(int) * b.nc);
+    for (int q = 0; q < 2; ++q) L.fflag[q] = take(sizeof(int) * b.nc);
+    for (int q = 0; q < 2; ++q) L.x1f[q] = take(sizeof(double) * x1_flux_rows(b));
+    L.fails = take(2 * sizeof(int));
+    L.total = o;
+    return L;
+}
 }  // namespace kh

@@ -127,6 +127,13 @@ hipError_t launch_update(const BlockDims& b, const StepConsts& c, const GeomDev&
 hipError_t launch_update_parts(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int stage, int parts,
                                const double* F, const double* Usub, const double* Ufull, const double* Psub, double* Uout,
                                double* Pout, int* pflag, int* fflag, int* fail_list, DeviceState* ds, hipStream_t s);
+// launch_update on the FP64 path with the stage's U* also copied to Ustar
+// (curved space: the combine, the copy, the recovery; flat space has no
+// separate U* and runs the fused update).
+hipError_t launch_update_recorded(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int stage, bool do_dt,
+                                  const void* F, const void* ctop, const double* Usub, const double* Ufull,
+                                  const double* Psub, double* Uout, double* Pout, int* pflag, int* fflag,
+                                  int* fail_list, DeviceState* ds, double* dt_part, double* Ustar, hipStream_t s);
 // acc += w dt sum_faces F_dir[var] dA (one stage of a face-flux objective).
 hipError_t launch_face_flux_objective(const BlockDims& b, int prec, const void* F, int dir, int var, int face, double w,
                                       const DeviceState* ds, double* acc, hipStream_t s);
@@ -220,7 +227,6 @@ hipError_t launch_adj_floors(const BlockDims& b, const StepConsts& c, const Geom
                              const double* lPout, double* lPin, hipStream_t s);
 // err[0] += zones with pflag != 0, or (fflag non-null) fflag != 0.
 hipError_t launch_adj_count_flags(const BlockDims& b, const int* pflag, const int* fflag, int* err, hipStream_t s);
-hipError_t launch_fill_int(int* p, long n, int v, hipStream_t s);
 hipError_t launch_estimate_dt(const BlockDims& b, const StepConsts& c, int prec, const void* cmax, const void* cmin,
                               double* part, hipStream_t s);
 

@@ -1208,6 +1208,21 @@ hipError_t launch_update_parts(const BlockDims& b, const StepConsts& c, const Ge
                               fail_list, ds, nullptr, s, parts);
     return hipGetLastError();
 }
+hipError_t launch_update_recorded(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int stage, bool do_dt,
+                                  const void* F, const void* ctop, const double* Usub, const double* Ufull,
+                                  const double* Psub, double* Uout, double* Pout, int* pflag, int* fflag,
+                                  int* fail_list, DeviceState* ds, double* dt_part, double* Ustar, hipStream_t s) {
+    if (!b.geo)
+        return launch_update(b, c, gd, PREC_FP64, stage, do_dt, F, ctop, Usub, Ufull, Psub, Uout, Pout, nullptr, pflag,
+                             fflag, fail_list, ds, dt_part, s);
+    launch_update_geo<double>(b, c, gd, stage, do_dt, F, ctop, Usub, Ufull, Psub, Uout, Pout, nullptr, pflag, fflag,
+                              fail_list, ds, dt_part, s, 1);
+    hipError_t e = hipMemcpyAsync(Ustar, Uout, sizeof(double) * NV * b.nc, hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) return e;
+    launch_update_geo<double>(b, c, gd, stage, do_dt, F, ctop, Usub, Ufull, Psub, Uout, Pout, nullptr, pflag, fflag,
+                              fail_list, ds, dt_part, s, 2);
+    return hipGetLastError();
+}
 long update_blocks(const BlockDims& b) { return (ninterior_host(b) + BS - 1) / BS; }
 long divb_blocks(const BlockDims& b) {
     return ((long)(b.nx1 + 1) * (b.nx2 + 1) * ((b.ndim > 2) ? b.nx3 + 1 : 1) + BS - 1) / BS;

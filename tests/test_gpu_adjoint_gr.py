@@ -400,6 +400,28 @@ def test_floored_step_velocity_and_field_directions():
             assert errs[1e-5] < 5e-6
 
 
+def test_recorded_steps_match_recompute(smooth_state):
+    """kh_adjoint_step_recorded (each stage's U*, flags and clamped X1 fluxes
+    recorded by the forward driver) against kh_adjoint_step (both stages
+    recomputed from the checkpoint) on the torus.  The recorded stages are the
+    forward trajectory's own; the recompute starts from U_n = p_to_u(P_n), a
+    last bit away from the forward's U_n after step 0, and its recoveries
+    converge to 1e-14, so the adjoints agree far inside the step bar."""
+    cfg, evo, Ps = smooth_state
+    _, dts, ck, _ = evo.run(Ps, 3, checkpoints=True)
+    recs = ck["records"]
+    rng = np.random.default_rng(21)
+    lam = evo._embed(_cuda(rng.normal(size=tuple(Ps.shape))))
+    for n in (0, 1, 2):
+        la, lb = lam.clone(), lam.clone()
+        evo.adj.step(ck[n], dts[n], la, ck[n + 1])
+        evo.adj.step_recorded(ck[n], recs[n], dts[n], lb, ck[n + 1])
+        a, b = _np(la[evo.inner]), _np(lb[evo.inner])
+        rel = np.abs(a - b).max() / np.abs(a).max()
+        print(f"[gr recorded] step {n}: max rel diff {rel:.2e}", flush=True)
+        assert rel < 1e-9, (n, rel)
+
+
 def test_mass_flux_objective_two_level_checkpoints(smooth_state):
     """J = time-integrated mass flux through the face nearest r = 2.5 over 6 steps,
     gradient with checkpoints every 3 steps (segments re-run) and every step."""
