@@ -53,6 +53,7 @@ def _bench_form(cfg, P0, steps, warmup):
     ms = start.elapsed_time(stop)
     ktimes = drv.kernel_times()
     drv.set_kernel_timing(False)
+    drv.check()   # the reference's stop on a large divB (b_field/kill_on_large_divb)
     s = drv.scalars()
     P, _ = drv.state()
     torch.cuda.synchronize()

@@ -35,6 +35,7 @@ extern "C" {
 #define KH_ERR_INVALID 1
 #define KH_ERR_HIP 2
 #define KH_ERR_UNSUPPORTED 3
+#define KH_ERR_DIVB 4                  /* corner divB above b_field/kill_on_divb_over (kh_driver_check) */
 
 #define KH_PREC_FP64 0
 #define KH_PREC_SPLIT 1
@@ -97,6 +98,8 @@ typedef struct kh_params {
     int fix_polar_flux;             /* b_field/fix_polar_flux */
     int fix_corner_inner;           /* boundaries/fix_corner_inner */
     int driver_type;                /* KH_DRIVER_* (driver/type) */
+    int kill_on_large_divb;         /* b_field/kill_on_large_divb (the reference's default: on) */
+    double kill_on_divb_over;       /* b_field/kill_on_divb_over (default 1e-3) */
 } kh_params;
 
 typedef struct kh_block kh_block;
@@ -234,6 +237,13 @@ int kh_driver_get_state(kh_driver* d, double* P, double* U);
  * the reference's pflag / fflag fields (inverter/inverter.cpp:104-108, floors/floors.cpp:111-116). */
 int kh_driver_get_flags(kh_driver* d, int* pflag, int* fflag);
 /* Synchronises; any output may be NULL. */
+/* PrintGlobalMaxDivB's stop (b_flux_ct/b_flux_ct.cpp:620-643): with
+ * kill_on_large_divb, each step's maximum corner divB is compared on the device
+ * with kill_on_divb_over and the first step above it is latched, so steps stay
+ * asynchronous.  kh_driver_check synchronises and returns KH_ERR_DIVB (with the
+ * cycle and value in kh_last_error) once a step has exceeded it, KH_OK
+ * otherwise.  kh_driver_init_prims clears the latch. */
+int kh_driver_check(kh_driver* d);
 int kh_driver_get_scalars(kh_driver* d, double* time, double* dt, double* last_divb, int* ncycle, int* total_fails);
 /* dt taken by each of the first n steps since init (host array). Synchronises. */
 int kh_driver_get_dt_history(kh_driver* d, double* out, int n);

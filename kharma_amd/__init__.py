@@ -57,6 +57,7 @@ class KhParams(ctypes.Structure):
         ("check_inflow_inner_x1", ctypes.c_int), ("check_inflow_outer_x1", ctypes.c_int),
         ("zero_polar_flux", ctypes.c_int), ("fix_polar_flux", ctypes.c_int), ("fix_corner_inner", ctypes.c_int),
         ("driver_type", ctypes.c_int),
+        ("kill_on_large_divb", ctypes.c_int), ("kill_on_divb_over", ctypes.c_double),
     ]
 
 
@@ -150,6 +151,7 @@ def lib():
         "kh_driver_set_face_flux_objective": (i, [vp, i, i, i, d]),
         "kh_driver_set_stage_record": (i, [vp, vp]),
         "kh_driver_stage_record_bytes": (i, [vp, ctypes.POINTER(ctypes.c_size_t)]),
+        "kh_driver_check": (i, [vp]),
         "kh_driver_objective": (i, [vp, ctypes.POINTER(d), i]),
     }
     for name, (res, args) in sig.items():
@@ -185,6 +187,8 @@ def make_params(cfg):
     p.fix_polar_flux = int(cfg.get("fix_polar_flux", False))
     p.fix_corner_inner = int(cfg.get("fix_corner_inner", False))
     p.driver_type = DRIVER[cfg.get("driver", "kharma")]
+    p.kill_on_large_divb = int(cfg.get("kill_on_large_divb", True))
+    p.kill_on_divb_over = cfg.get("kill_on_divb_over", 1e-3)
     p.gam = cfg["gamma"]
     p.cfl = cfg.get("cfl", 0.9)
     p.dt_min = cfg.get("dt_min", 1e-5)
@@ -445,6 +449,12 @@ class Driver:
         cnt = (ctypes.c_int * n)()
         _check(lib().kh_driver_kernel_times(self.h, ms, cnt, n))
         return {k: (ms[q], cnt[q]) for q, k in enumerate(self.KERNEL_KINDS)}
+
+    def check(self):
+        """Raise KharmaError if a step's corner divB exceeded kill_on_divb_over
+        (kh_driver_check; synchronises)."""
+        self._bind()
+        _check(lib().kh_driver_check(self.h))
 
     def new_stage_record(self):
         """A device buffer (uint8 CUDA tensor) for one step's record."""

@@ -75,6 +75,8 @@ int kh::check_params(const kh_params* p) {
     if (p->integrator != 0 && p->integrator != 1) return fail(KH_ERR_INVALID, "unknown integrator");
     if (p->driver_type != KH_DRIVER_KHARMA && p->driver_type != KH_DRIVER_SIMPLE)
         return fail(KH_ERR_INVALID, "unknown driver type");
+    if (p->kill_on_large_divb && !(p->kill_on_divb_over > 0.))
+        return fail(KH_ERR_INVALID, "kill_on_divb_over must be positive when kill_on_large_divb is set");
     if (p->floor_frame < 0 || p->floor_frame > 2) return fail(KH_ERR_INVALID, "unknown floor frame");
     if (p->iter_max < 0) return fail(KH_ERR_INVALID, "iter_max < 0");
     const double n = (double)(p->nx1 + 2 * p->ng) * (p->nx2 + 2 * p->ng) * (p->nx3 + 2 * p->ng);
@@ -132,6 +134,8 @@ void fill_consts(const kh_params* p, StepConsts& c) {
     c.fix_average = p->fix_average;
     c.fix_atmo = p->fix_atmo;
     c.use_hlle = p->use_hlle;
+    c.kill_divb = p->kill_on_large_divb;
+    c.divb_over = p->kill_on_divb_over;
     FloorParams& f = c.floors;
     f.rho_min_geom = p->rho_min_geom; f.u_min_geom = p->u_min_geom;
     f.rho_min_const = p->rho_min_const; f.u_min_const = p->u_min_const; f.r_char = p->r_char;
@@ -550,6 +554,20 @@ int kh_driver_get_flags(kh_driver* d, int* pflag, int* fflag) {
     const size_t bytes = sizeof(int) * d->blk->dims.nc;
     if (pflag) KH_HIP(hipMemcpyAsync(pflag, d->pflag, bytes, hipMemcpyDeviceToDevice, d->blk->stream));
     if (fflag) KH_HIP(hipMemcpyAsync(fflag, d->fflag, bytes, hipMemcpyDeviceToDevice, d->blk->stream));
+    return KH_OK;
+}
+
+int kh_driver_check(kh_driver* d) {
+    if (!d) return fail(KH_ERR_INVALID, "null driver");
+    DeviceState h;
+    KH_HIP(hipMemcpyAsync(&h, d->ds, sizeof(h), hipMemcpyDeviceToHost, d->blk->stream));
+    KH_HIP(hipStreamSynchronize(d->blk->stream));
+    if (h.divb_over_cycle > 0) {
+        char msg[160];
+        std::snprintf(msg, sizeof(msg), "DivB exceeds maximum! max corner divB %.6g > %.6g after cycle %d",
+                      h.divb_over_value, d->blk->consts.divb_over, h.divb_over_cycle);
+        return fail(KH_ERR_DIVB, msg);
+    }
     return KH_OK;
 }
 

@@ -41,6 +41,8 @@ struct DeviceState {
     int fail_count[2];                 // failed inversions per stage of the current step
     int ncycle;
     int total_fails;
+    int divb_over_cycle;               // first cycle whose divB exceeded kill_on_divb_over (0: none)
+    double divb_over_value;            // its divB
 };
 
 // Run-wide constants of the step.
@@ -53,6 +55,8 @@ struct StepConsts {
     int fix_average, fix_atmo;
     int use_hlle;
     FloorParams floors;
+    int kill_divb;                      // b_field/kill_on_large_divb
+    double divb_over;                   // b_field/kill_on_divb_over
 };
 
 enum Precision { PREC_FP64 = 0, PREC_SPLIT = 1 };

@@ -665,6 +665,10 @@ finalize_kernel(StepConsts cst, DeviceState* ds, const double* __restrict__ dt_p
     ds->ncycle += 1;
     ds->total_fails += ds->fail_count[0] + ds->fail_count[1];
     ds->last_divb = mx;
+    if (cst.kill_divb && mx > cst.divb_over && ds->divb_over_cycle == 0) {
+        ds->divb_over_cycle = ds->ncycle;
+        ds->divb_over_value = mx;
+    }
     ds->fail_count[0] = 0;
     ds->fail_count[1] = 0;
 }
@@ -683,6 +687,7 @@ __global__ void reset_state_kernel(DeviceState* ds, double dt) {
     ds->last_divb = 0.; ds->last_ndt = 0.;
     ds->fail_count[0] = ds->fail_count[1] = 0;
     ds->ncycle = 0; ds->total_fails = 0;
+    ds->divb_over_cycle = 0; ds->divb_over_value = 0.;
 }
 
 __global__ void set_clock_kernel(DeviceState* ds, double time, double dt, int ncycle) {

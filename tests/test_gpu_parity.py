@@ -274,6 +274,28 @@ def test_step_parity_variants(oracle_lib, case, over):
         assert np.all(ru < 1e-12), (n, ru)
 
 
+def test_divb_stop():
+    """PrintGlobalMaxDivB's stop (b_flux_ct/b_flux_ct.cpp:620-643): corner divB
+    stays at roundoff under flux-CT, so the default threshold (1e-3) never
+    trips; a threshold below roundoff trips after the first step, and the
+    stop switched off never does."""
+    import kharma_amd as kh
+    cfg, P0, _ = _case("ot2d")
+    for over, trips in ((dict(), False), (dict(kill_on_divb_over=1e-300), True),
+                        (dict(kill_on_large_divb=False, kill_on_divb_over=1e-300), False)):
+        d = kh.Driver(dict(cfg, **over))
+        d.init_prims(_cuda(P0))
+        d.step(3)
+        print(f"[divb stop] {over}: divB {d.scalars()['divb']:.3e}", flush=True)
+        if trips:
+            with pytest.raises(kh.KharmaError, match="DivB exceeds maximum.*after cycle 1"):
+                d.check()
+            d.init_prims(_cuda(P0))   # a fresh start clears the latch
+            d.check()
+        else:
+            d.check()
+
+
 @pytest.mark.parametrize("case,nsteps", [("ot2d", 20), ("fast3d", 6)])
 def test_step_parity_fp64_multistep(oracle_lib, case, nsteps):
     """Many steps: the GPU (which contracts to FMA) may drift from the oracle only
