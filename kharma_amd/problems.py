@@ -76,15 +76,23 @@ def _base_cfg(**kw):
 
 
 # ------------------------------------------------------------- Orszag-Tang
-def orszag_tang_config(n=256, steps=100, **kw):
-    """2-D relativistic Orszag-Tang vortex, Minkowski, periodic [-pi, pi]^2, gamma = 5/3.
+def orszag_tang_config(n=256, steps=100, init="literal", tscale=0.05, phase=math.pi, **kw):
+    """2-D relativistic Orszag-Tang vortex, Minkowski, periodic [-pi, pi]^2, gamma = 5/3;
+    WENO5 + LLF, flux-CT, RK2, CFL 0.9, onedw.
 
-    Follows the literal workload statement (unscaled u, zero phase) rather than
-    prob/orszag_tang.hpp:35-57; WENO5 + LLF, flux-CT, RK2, CFL 0.9, onedw.
+    init "literal": the literal workload statement (unscaled u, zero phase, B
+    given at cell centres).  init "reference": KHARMA's own generator,
+    InitializeOrszagTang (prob/orszag_tang.hpp:25-62: u scaled by tscale^2,
+    velocities of amplitude tscale with the phase) with B the averaged curl
+    (b_flux_ct_functions.hpp:152-166) of the vector potential
+    seed_a<orszag_tang_a> (prob/seed_B.hpp:133-138, A0 = tscale) at the zone
+    corners (prob/seed_B.cpp:239-241,258-370).
     """
+    if init not in ("literal", "reference"):
+        raise ValueError(f"unknown Orszag-Tang init {init!r}")
     return _base_cfg(name="orszag_tang", nx=(n, n, 1),
                      bounds=((-math.pi, math.pi), (-math.pi, math.pi), (-0.01, 0.01)),
-                     gamma=5.0 / 3.0, steps=steps, **kw)
+                     gamma=5.0 / 3.0, steps=steps, ot_init=init, ot_tscale=tscale, ot_phase=phase, **kw)
 
 
 def orszag_tang_prims(cfg):
@@ -95,11 +103,27 @@ def orszag_tang_prims(cfg):
     gam = cfg["gamma"]
     sl = interior(cfg)
     P[(RHO,) + sl] = 25.0 / 9.0
-    P[(UU,) + sl] = 5.0 / (3.0 * (gam - 1.0))
-    P[(U1,) + sl] = -0.05 * np.sin(Y)
-    P[(U2,) + sl] = 0.05 * np.sin(X)
-    P[(B1,) + sl] = -0.05 * np.sin(Y)
-    P[(B2,) + sl] = 0.05 * np.sin(2.0 * X)
+    if cfg.get("ot_init", "literal") == "literal":
+        P[(UU,) + sl] = 5.0 / (3.0 * (gam - 1.0))
+        P[(U1,) + sl] = -0.05 * np.sin(Y)
+        P[(U2,) + sl] = 0.05 * np.sin(X)
+        P[(B1,) + sl] = -0.05 * np.sin(Y)
+        P[(B2,) + sl] = 0.05 * np.sin(2.0 * X)
+        return periodic_fill(P, cfg)
+    ts, ph = cfg["ot_tscale"], cfg["ot_phase"]
+    P[(UU,) + sl] = 5.0 / (3.0 * (gam - 1.0)) * ts * ts
+    P[(U1,) + sl] = -np.sin(Y + ph) * ts
+    P[(U2,) + sl] = np.sin(X + ph) * ts
+    # A3 at the corners (x_c, y_c) of the interior cells, then the averaged curl
+    (x0, x1), (y0, y1) = cfg["bounds"][0], cfg["bounds"][1]
+    nx, ny = cfg["nx"][0], cfg["nx"][1]
+    dx, dy = (x1 - x0) / nx, (y1 - y0) / ny
+    xc = x0 + dx * np.arange(nx + 1)
+    yc = y0 + dy * np.arange(ny + 1)
+    XC, YC = np.meshgrid(xc, yc, indexing="xy")
+    A = ts * (-0.5 * np.cos(2.0 * XC + ph) + np.cos(YC + ph))
+    P[(B1,) + sl] = ((A[1:, :-1] + A[1:, 1:]) / 2 - (A[:-1, :-1] + A[:-1, 1:]) / 2)[None] / dy
+    P[(B2,) + sl] = -((A[:-1, 1:] + A[1:, 1:]) / 2 - (A[:-1, :-1] + A[1:, :-1]) / 2)[None] / dx
     return periodic_fill(P, cfg)
 
 

@@ -239,6 +239,7 @@ VARIANTS = [
     ("ot2d", dict(recon="ppm")),               # reconstruct<ppm>, reconstruction.hpp:351-394
     ("fast3d", dict(recon="linear_mc")),       # reconstruct<linear_mc>, reconstruction.hpp:107-123
     ("fast3d", dict(riemann="hlle", integrator="vl2")),
+    ("ot2d", dict(ot_init="reference")),       # InitializeOrszagTang + seed_a<orszag_tang_a>
 ]
 
 

@@ -198,6 +198,31 @@ def test_divb_roundoff_orszag_tang(oracle_lib):
     assert b.max_divb() < 1e-12      # reference torus_sanity bound is 1e-10
 
 
+def test_orszag_tang_reference_generator(oracle_lib):
+    """KHARMA's own Orszag-Tang set-up (prob/orszag_tang.hpp:25-62, B the
+    averaged curl of seed_a<orszag_tang_a>): A3 = f(x) + g(y) is separable, so
+    the averaged curl is exact differences of f and g across the zone,
+    B1 = -ts sin(y + phase) sin(dy/2)/(dy/2), B2 = -ts sin(2x + phase) sin(dx)/dx,
+    and the corner divB vanishes to roundoff through steps."""
+    cfg = pr.orszag_tang_config(n=48, init="reference")
+    P = pr.orszag_tang_prims(cfg)
+    ts, ph, gam = cfg["ot_tscale"], cfg["ot_phase"], cfg["gamma"]
+    x, y, _ = pr.cell_centers(cfg)
+    X, Y = np.meshgrid(x, y, indexing="xy")
+    dx = dy = 2 * math.pi / 48
+    Pi = interior_of(P, cfg)[:, 0]
+    assert np.allclose(Pi[UU], 5.0 / (3.0 * (gam - 1.0)) * ts * ts, rtol=1e-15, atol=0)
+    assert np.allclose(Pi[U1], -ts * np.sin(Y + ph), rtol=0, atol=1e-16)
+    assert np.allclose(Pi[U2], ts * np.sin(X + ph), rtol=0, atol=1e-16)
+    assert np.abs(Pi[B1] - (-ts * np.sin(Y + ph) * np.sin(dy / 2) / (dy / 2))).max() < 1e-15
+    assert np.abs(Pi[B2] - (-ts * np.sin(2 * X + ph) * np.sin(dx) / dx)).max() < 1e-15
+    b = _block(oracle_lib, cfg)
+    b.init_state(P)
+    assert b.max_divb() < 1e-15
+    b.step(10)
+    assert b.max_divb() < 1e-13
+
+
 def test_dt_ramp(oracle_lib):
     # First step takes dt_start = dt_min = 1e-5; dt then at most doubles per step
     # (grmhd.cpp:88-95,335-339; kharma_driver.cpp:442-448).
