@@ -131,21 +131,26 @@ adj_flux_x1_fused_kernel(BlockDims b, StepConsts c, GeomDev gd, const double* __
     }
 }
 
-// The same for X2 / X3: tiles of 8 transverse columns (outer, i) x 32 slots
-// along DIR; the slots of a column are the cells [s-5, e+5] along DIR, and a
-// tile outputs its 26 slots 3 .. 28 (cells [s-3, e+3]).
+// The same for X2 / X3: tiles of TX transverse columns (outer, i) x TY slots
+// along DIR.  Tile by covers the cells s - 6 + by (TY - 6) + [0, TY) and
+// outputs its slots 3 .. TY - 4; the outputs of a column are the cells
+// [s-3, e+3], its valid slots [s-5, e+5].  Flat space: 6 x 41 (35 outputs a
+// tile: two tiles for 64 cells, four for 128, 246 threads); curved space,
+// whose face adjoint needs more registers: 8 x 32.
+template <int GEO> constexpr int adj_xt_tx() { return GEO ? 8 : 6; }
+template <int GEO> constexpr int adj_xt_ty() { return GEO ? 32 : 41; }
 template <int DIR, int GEO>
 __global__ void __launch_bounds__(256)
 adj_flux_xt_fused_kernel(BlockDims b, StepConsts c, GeomDev gd, const double* __restrict__ P,
                          const double* __restrict__ lF, double* __restrict__ lamP, int o0, int no) {
-    constexpr int TX = 8, TY = 32;
+    constexpr int TX = adj_xt_tx<GEO>(), TY = adj_xt_ty<GEO>();
     __shared__ double sro[NV][TY][TX];
     __shared__ double G[5][TY][TX];
     const int ni = b.nx1 + 2;
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int col = blockIdx.x * TX + tx;
     const int s0 = (DIR == 2) ? b.js : b.ks, e0 = (DIR == 2) ? b.je : b.ke;
-    const int n = s0 - 5 + (int)blockIdx.y * (TY - 6) + ty - 3;
+    const int n = s0 - 6 + (int)blockIdx.y * (TY - 6) + ty;
     const bool valid = col < ni * no && n >= s0 - 5 && n <= e0 + 5;
     const long st = (DIR == 2) ? b.s2 : b.s3;
     int jf = 0, if_ = 0;
@@ -1063,12 +1068,13 @@ hipError_t launch_adj_flux_t(const BlockDims& b, const StepConsts& c, const Geom
             const int no = (dir == 2) ? nk : nj;
             const int o0 = (dir == 2) ? klo : jlo;
             const int ncol = (b.nx1 + 2) * no;
-            const int nslots = ((dir == 2) ? b.nx2 : b.nx3) + 10;
-            const dim3 grid((unsigned)((ncol + 7) / 8), (unsigned)((nslots + 25) / 26));
+            constexpr int TX = adj_xt_tx<GEO>(), TY = adj_xt_ty<GEO>();
+            const int nout = ((dir == 2) ? b.nx2 : b.nx3) + 6;
+            const dim3 grid((unsigned)((ncol + TX - 1) / TX), (unsigned)((nout + TY - 7) / (TY - 6)));
             if (dir == 2)
-                hipLaunchKernelGGL((adj_flux_xt_fused_kernel<2, GEO>), grid, dim3(8, 32), 0, s, b, c, gd, P, lFd, lamP, o0, no);
+                hipLaunchKernelGGL((adj_flux_xt_fused_kernel<2, GEO>), grid, dim3(TX, TY), 0, s, b, c, gd, P, lFd, lamP, o0, no);
             else
-                hipLaunchKernelGGL((adj_flux_xt_fused_kernel<3, GEO>), grid, dim3(8, 32), 0, s, b, c, gd, P, lFd, lamP, o0, no);
+                hipLaunchKernelGGL((adj_flux_xt_fused_kernel<3, GEO>), grid, dim3(TX, TY), 0, s, b, c, gd, P, lFd, lamP, o0, no);
         }
     }
     return hipGetLastError();
