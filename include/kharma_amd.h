@@ -318,8 +318,9 @@ int kh_adjoint_set_stream(kh_adjoint* a, void* hip_stream);
 int kh_adjoint_step(kh_adjoint* a, const double* P, const double* P_next, double dt, double* lam_P);
 /* kh_adjoint_step from the record of the step from P to P_next
  * (kh_driver_set_stage_record, a driver with the same parameters) instead of
- * recomputing its stages.  Same coverage and errors as kh_adjoint_step.
- * Synchronises. */
+ * recomputing its stages.  P and P_next are read as the driver holds its
+ * states (kh_driver_get_state): ghost zones filled, periodic images included.
+ * Same coverage and errors as kh_adjoint_step.  Synchronises. */
 int kh_adjoint_step_recorded(kh_adjoint* a, const double* P, const void* rec, const double* P_next, double dt,
                              double* lam_P);
 /* Deferred checks (on != 0): the following adjoint steps neither synchronise nor

@@ -344,12 +344,9 @@ int kh_adjoint_step_recorded(kh_adjoint* a, const double* P, const void* rec, co
     auto at_i = [&](size_t off) { return reinterpret_cast<int*>(const_cast<char*>(r + off)); };
     if (const int st0 = begin_step(a)) return st0;
     KA_HIP(hipEventRecord(a->ev[0], s));
-    // P_0 with its periodic images, the stages' failed-inversion counts (the
-    // fix-up transposes' gates), and the count of zones the adjoint does not
-    // cover, by the rule of the recompute
-    KA_HIP(hipMemcpyAsync(d->P0, P, sizeof(double) * NV * bd.nc, hipMemcpyDeviceToDevice, s));
-    if (bd.periodic[0] || (bd.ndim > 1 && bd.periodic[1]) || (bd.ndim > 2 && bd.periodic[2]))
-        KA_HIP(launch_sync(bd, d->P0, NV, s));
+    // the stages' failed-inversion counts (the fix-up transposes' gates) and
+    // the count of zones the adjoint does not cover, by the rule of the
+    // recompute; P is read as the driver holds its state, ghosts filled
     KA_HIP(hipMemcpyAsync(reinterpret_cast<char*>(d->ds) + offsetof(DeviceState, fail_count), r + L.fails,
                           2 * sizeof(int), hipMemcpyDeviceToDevice, s));
     for (int q = 0; q < 2; ++q) {
@@ -379,10 +376,10 @@ int kh_adjoint_step_recorded(kh_adjoint* a, const double* P, const void* rec, co
     if ((st = stage_transpose(a, 2, P1, P_next, lam_P, nullptr, a->lU1, a->lU0, false, false, dt, a->lPin, false,
                               a->fixup)) != KH_OK)
         return st;
-    if ((st = stage_transpose(a, 1, d->P0, P1, a->lPin, a->lU1, a->lU0, a->lU0, true, true, dt, lam_P, true,
+    if ((st = stage_transpose(a, 1, P, P1, a->lPin, a->lU1, a->lU0, a->lU0, true, true, dt, lam_P, true,
                               a->fixup)) != KH_OK)
         return st;
-    KA_HIP(launch_adj_ptou(bd, c, b->geo, d->P0, a->lU0, lam_P, s));
+    KA_HIP(launch_adj_ptou(bd, c, b->geo, P, a->lU0, lam_P, s));
     return finish_step(a);
 }
 
