@@ -37,7 +37,39 @@
 
 namespace {
 
+#ifdef KH_ORACLE_TANGENT
+// Forward-mode dual number (value, tangent): the tangent-linear build of this
+// file (liboracle_tangent.so) carries one directional derivative through the
+// step with every branch taken on the values.
+struct Dual {
+    double v = 0., d = 0.;
+    constexpr Dual() = default;
+    constexpr Dual(double x) : v(x), d(0.) {}
+    constexpr Dual(double x, double dx) : v(x), d(dx) {}
+    explicit operator double() const { return v; }
+    explicit operator int() const { return (int)v; }
+    Dual& operator+=(const Dual& o) { v += o.v; d += o.d; return *this; }
+    Dual& operator-=(const Dual& o) { v -= o.v; d -= o.d; return *this; }
+    Dual& operator*=(const Dual& o) { d = d * o.v + v * o.d; v *= o.v; return *this; }
+    Dual& operator/=(const Dual& o) { const double q = v / o.v; d = (d - q * o.d) / o.v; v = q; return *this; }
+};
+inline Dual operator+(Dual a, const Dual& b) { return a += b; }
+inline Dual operator-(Dual a, const Dual& b) { return a -= b; }
+inline Dual operator*(Dual a, const Dual& b) { return a *= b; }
+inline Dual operator/(Dual a, const Dual& b) { return a /= b; }
+inline Dual operator-(const Dual& a) { return Dual(-a.v, -a.d); }
+inline bool operator<(const Dual& a, const Dual& b) { return a.v < b.v; }
+inline bool operator>(const Dual& a, const Dual& b) { return a.v > b.v; }
+inline bool operator<=(const Dual& a, const Dual& b) { return a.v <= b.v; }
+inline bool operator>=(const Dual& a, const Dual& b) { return a.v >= b.v; }
+inline bool operator==(const Dual& a, const Dual& b) { return a.v == b.v; }
+inline bool operator!=(const Dual& a, const Dual& b) { return a.v != b.v; }
+using Real = Dual;
+inline double val(const Dual& x) { return x.v; }
+#else
 using Real = double;
+#endif
+inline double val(double x) { return x; }
 constexpr int NV = 8;   // rho, u, u1, u2, u3, B1, B2, B3
 enum { RHO = 0, UU = 1, U1 = 2, U2 = 3, U3 = 4, B1 = 5, B2 = 6, B3 = 7 };
 // Loci, decs.hpp:103-104 (outer/inner_half are not used on this path)
@@ -61,6 +93,39 @@ inline Real dot(const Real a[4], const Real b[4]) { return a[0]*b[0] + a[1]*b[1]
 struct FourVectors { Real ucon[4], ucov[4], bcon[4], bcov[4]; };
 
 }  // namespace
+
+#ifdef KH_ORACLE_TANGENT
+// Elementary functions of the dual number, found through the std:: calls of the
+// restatement.
+namespace std {
+inline Dual sqrt(const Dual& a) { const double r = std::sqrt(a.v); return Dual(r, a.d / (2. * r)); }
+inline Dual exp(const Dual& a) { const double e = std::exp(a.v); return Dual(e, e * a.d); }
+inline Dual log(const Dual& a) { return Dual(std::log(a.v), a.d / a.v); }
+inline Dual sin(const Dual& a) { return Dual(std::sin(a.v), std::cos(a.v) * a.d); }
+inline Dual cos(const Dual& a) { return Dual(std::cos(a.v), -std::sin(a.v) * a.d); }
+inline Dual fabs(const Dual& a) { return a.v < 0. ? Dual(-a.v, -a.d) : a; }
+inline Dual abs(const Dual& a) { return fabs(a); }
+inline Dual pow(const Dual& a, double p) {
+    const double r = std::pow(a.v, p);
+    return Dual(r, a.d == 0. ? 0. : p * std::pow(a.v, p - 1.) * a.d);
+}
+inline Dual pow(const Dual& a, const Dual& p) {
+    const double r = std::pow(a.v, p.v);
+    double d = 0.;
+    if (a.d != 0.) d += p.v * std::pow(a.v, p.v - 1.) * a.d;
+    if (p.d != 0.) d += r * std::log(a.v) * p.d;
+    return Dual(r, d);
+}
+inline Dual pow(double a, const Dual& p) { return pow(Dual(a), p); }
+inline const Dual& max(const Dual& a, const Dual& b) { return (a < b) ? b : a; }
+inline const Dual& min(const Dual& a, const Dual& b) { return (b < a) ? b : a; }
+inline Dual max(double a, const Dual& b) { return (Dual(a) < b) ? b : Dual(a); }
+inline Dual max(const Dual& a, double b) { return (a < Dual(b)) ? Dual(b) : a; }
+inline Dual min(double a, const Dual& b) { return (b < Dual(a)) ? b : Dual(a); }
+inline Dual min(const Dual& a, double b) { return (Dual(b) < a) ? Dual(b) : a; }
+inline bool isnan(const Dual& a) { return std::isnan(a.v); }
+}  // namespace std
+#endif
 
 extern "C" {
 // Parameters of one block / one run.  Mirrors the .par options the path reads.
@@ -871,7 +936,7 @@ int determine_floors(const Block& b, const Real* P, int j, int i, Real& rhoflr_m
     const Real u = P[UU];
     int fflag = 0;
     if (!f.temp_adjust_u) {
-        const double rhoflr_temp = std::max(u, uflr_max) / f.u_over_rho_max;
+        const Real rhoflr_temp = std::max(u, uflr_max) / f.u_over_rho_max;
         fflag |= (rhoflr_temp > rho) * F_TEMP;
         rhoflr_max = std::max(std::max(rhoflr_geom, rhoflr_b), rhoflr_temp);
     } else {
@@ -892,7 +957,7 @@ int determine_floors(const Block& b, const Real* P, int j, int i, Real& rhoflr_m
 int apply_floors_drift(const Block& b, Real* P, int j, int i, Real rhoflr_max, Real uflr_max) {
     const Real gam = b.p.gam;
     const Real lapse2 = 1. / (-b.gcon_(CENTER, j, i, 0, 0));
-    double beta[4] = {0};
+    Real beta[4] = {0., 0., 0., 0.};
     beta[1] = lapse2 * b.gcon_(CENTER, j, i, 0, 1);
     beta[2] = lapse2 * b.gcon_(CENTER, j, i, 0, 2);
     beta[3] = lapse2 * b.gcon_(CENTER, j, i, 0, 3);
@@ -912,7 +977,7 @@ int apply_floors_drift(const Block& b, Real* P, int j, int i, Real rhoflr_max, R
     Qcov[1] = w_old * D.ucon[0] * D.ucov[1];
     Qcov[2] = w_old * D.ucon[0] * D.ucov[2];
     Qcov[3] = w_old * D.ucon[0] * D.ucov[3];
-    double QdotB = dot(Bcon, Qcov);
+    Real QdotB = dot(Bcon, Qcov);
     Real vpar = QdotB / (B_mag * w_old * D.ucon[0]*D.ucon[0]);
     Real ucon_dr[4] = {0};
     ucon_dr[0] = 1. / std::sqrt(1. / (D.ucon[0]*D.ucon[0]) + vpar*vpar);
@@ -1030,7 +1095,7 @@ void fix_utop(Block& b, std::vector<Real>& U, std::vector<Real>& P) {
     for (int i = pr.is; i <= pr.ie; ++i) {
         const size_t c = b.id(k, j, i);
         if (!(b.pflag[c] > 0)) continue;
-        double wsum = 0., sum[5] = {0};
+        Real wsum = 0., sum[5] = {0., 0., 0., 0., 0.};
         if (b.p.fix_average) {
             for (int n = -1; n <= 1; n++) for (int m = -1; m <= 1; m++) for (int l = -1; l <= 1; l++) {
                 int ii = i + l, jj = j + m, kk = k + n;
@@ -1092,13 +1157,13 @@ double estimate_timestep(const Block& b) {
     for (int j = b.js; j <= b.je; ++j)
     for (int i = b.is; i <= b.ie; ++i) {
         const size_t c = b.id(k, j, i);
-        double ndt_zone = 1.0 / (1 / (dx1 / std::max(b.cmax[c], b.cmin[c])) +
-                                 1 / (dx2 / std::max(b.cmax[b.nc + c], b.cmin[b.nc + c])) +
-                                 1 / (dx3 / std::max(b.cmax[2 * b.nc + c], b.cmin[2 * b.nc + c])));
+        double ndt_zone = val(1.0 / (1 / (dx1 / std::max(b.cmax[c], b.cmin[c])) +
+                                     1 / (dx2 / std::max(b.cmax[b.nc + c], b.cmin[b.nc + c])) +
+                                     1 / (dx3 / std::max(b.cmax[2 * b.nc + c], b.cmin[2 * b.nc + c]))));
         if (!std::isnan(ndt_zone) && ndt_zone < min_ndt) min_ndt = ndt_zone;
     }
     const double dt_max = b.p.max_dt_increase * b.dt_last;
-    return clip(min_ndt * b.p.cfl, b.p.dt_min, dt_max);
+    return val(clip(min_ndt * b.p.cfl, b.p.dt_min, dt_max));
 }
 
 // B_FluxCT::MaxDivB with corner_div, b_flux_ct.cpp:566-607, b_flux_ct_functions.hpp:52-75
@@ -1118,16 +1183,16 @@ double max_divb(const Block& b, const std::vector<Real>& U) {
     for (int i = isb; i <= ieb; ++i) {
         auto id = [&](int kk, int jj, int ii) { return b.id(kk, jj, ii); };
         const double norm = do3 ? 0.25 : 0.5;
-        double term1 = B1v[id(k, j, i)] - B1v[id(k, j, i-1)] + B1v[id(k, j-1, i)] - B1v[id(k, j-1, i-1)];
-        double term2 = B2v[id(k, j, i)] - B2v[id(k, j-1, i)] + B2v[id(k, j, i-1)] - B2v[id(k, j-1, i-1)];
-        double term3 = 0.;
+        Real term1 = B1v[id(k, j, i)] - B1v[id(k, j, i-1)] + B1v[id(k, j-1, i)] - B1v[id(k, j-1, i-1)];
+        Real term2 = B2v[id(k, j, i)] - B2v[id(k, j-1, i)] + B2v[id(k, j, i-1)] - B2v[id(k, j-1, i-1)];
+        Real term3 = 0.;
         if (do3) {
             term1 += B1v[id(k-1, j, i)] + B1v[id(k-1, j-1, i)] - B1v[id(k-1, j, i-1)] - B1v[id(k-1, j-1, i-1)];
             term2 += B2v[id(k-1, j, i)] + B2v[id(k-1, j, i-1)] - B2v[id(k-1, j-1, i)] - B2v[id(k-1, j-1, i-1)];
             term3 = B3v[id(k, j, i)] + B3v[id(k, j-1, i)] + B3v[id(k, j, i-1)] + B3v[id(k, j-1, i-1)]
                   - B3v[id(k-1, j, i)] - B3v[id(k-1, j-1, i)] - B3v[id(k-1, j, i-1)] - B3v[id(k-1, j-1, i-1)];
         }
-        double d = std::fabs(norm*term1/b.dx[0] + norm*term2/b.dx[1] + norm*term3/b.dx[2]);
+        double d = val(std::fabs(norm*term1/b.dx[0] + norm*term2/b.dx[1] + norm*term3/b.dx[2]));
         if (d > mx) mx = d;
     }
     return mx;
@@ -1386,6 +1451,16 @@ double step(Block& b) {
 }  // namespace
 
 // ================================================================== C ABI
+namespace {
+// Values between the ABI's double arrays and the block's Real arrays.
+void load_values(std::vector<Real>& dst, const double* src, size_t n) {
+    for (size_t q = 0; q < n; ++q) dst[q] = Real(src[q]);
+}
+void store_values(const std::vector<Real>& src, double* dst, size_t n) {
+    for (size_t q = 0; q < n; ++q) dst[q] = val(src[q]);
+}
+}  // namespace
+
 extern "C" {
 
 void* or_create(const or_params* p) {
@@ -1427,7 +1502,7 @@ size_t or_ncells(void* h) { return static_cast<Block*>(h)->nc; }
 // dt is dt_start for the first step (grmhd.cpp:283-289).
 void or_init_state(void* h, const double* P) {
     Block& b = *static_cast<Block*>(h);
-    std::memcpy(b.P0.data(), P, sizeof(double) * NV * b.nc);
+    load_values(b.P0, P, (size_t)NV * b.nc);
     // KHARMA::PostInitialize (prob/post_initialize.cpp:55-193): P->U over the
     // entire block, then the boundary sync with physical boundaries.  The input
     // may hold the interior only; its periodic ghosts (all 8 primitives, B
@@ -1444,14 +1519,14 @@ void or_init_state(void* h, const double* P) {
 // Overwrite base-container U and P directly (no P->U), for task-level tests.
 void or_set_state(void* h, const double* P, const double* U) {
     Block& b = *static_cast<Block*>(h);
-    std::memcpy(b.P0.data(), P, sizeof(double) * NV * b.nc);
-    std::memcpy(b.U0.data(), U, sizeof(double) * NV * b.nc);
+    load_values(b.P0, P, (size_t)NV * b.nc);
+    load_values(b.U0, U, (size_t)NV * b.nc);
 }
 
 void or_get_state(void* h, double* P, double* U) {
     Block& b = *static_cast<Block*>(h);
-    if (P) std::memcpy(P, b.P0.data(), sizeof(double) * NV * b.nc);
-    if (U) std::memcpy(U, b.U0.data(), sizeof(double) * NV * b.nc);
+    if (P) store_values(b.P0, P, (size_t)NV * b.nc);
+    if (U) store_values(b.U0, U, (size_t)NV * b.nc);
 }
 
 // Geometry cache: which = 0 gcov [NLOC][gn2][gn1][4][4], 1 gcon, 2 gdet [NLOC][gn2][gn1],
@@ -1460,7 +1535,7 @@ long or_get_geometry(void* h, int which, double* out) {
     Block& b = *static_cast<Block*>(h);
     const std::vector<Real>* v = which == 0 ? &b.gcov : which == 1 ? &b.gcon : which == 2 ? &b.gdet :
                                  which == 3 ? &b.gdet_conn : &b.rcen;
-    if (out) std::memcpy(out, v->data(), sizeof(double) * v->size());
+    if (out) store_values(*v, out, v->size());
     return (long)v->size();
 }
 // Replace the geometry cache (same layouts as or_get_geometry), e.g. to run the
@@ -1469,7 +1544,7 @@ void or_set_geometry(void* h, int which, const double* in) {
     Block& b = *static_cast<Block*>(h);
     std::vector<Real>* v = which == 0 ? &b.gcov : which == 1 ? &b.gcon : which == 2 ? &b.gdet :
                            which == 3 ? &b.gdet_conn : &b.rcen;
-    std::memcpy(v->data(), in, sizeof(double) * v->size());
+    load_values(*v, in, v->size());
 }
 // Physical boundaries on the base state (KBoundaries::ApplyBoundary for each face).
 // The physical boundary fill of the block's driver type (the synced-primitive
@@ -1515,7 +1590,7 @@ void or_get_array(void* h, int which, double* out) {
     case 6: src = &b.emf; break; case 7: src = &b.Pl; break; case 8: src = &b.Pr; break;
     default: return;
     }
-    std::memcpy(out, src->data(), sizeof(double) * src->size());
+    store_values(*src, out, src->size());
 }
 void or_set_array(void* h, int which, const double* in) {
     Block& b = *static_cast<Block*>(h);
@@ -1525,14 +1600,38 @@ void or_set_array(void* h, int which, const double* in) {
     case 3: dst = &b.dUdt; break; case 4: dst = &b.pflag; break; case 5: dst = &b.fflag; break;
     default: return;
     }
-    std::memcpy(dst->data(), in, sizeof(double) * dst->size());
+    load_values(*dst, in, dst->size());
 }
 
 // Scalar kernels exposed for closed-form tests of the oracle itself.
-void or_weno5(double x1, double x2, double x3, double x4, double x5, double* l, double* r) { weno5(x1, x2, x3, x4, x5, *l, *r); }
-void or_reconstruct(int recon, double x1, double x2, double x3, double x4, double x5, double* l, double* r) {
-    reconstruct(recon, x1, x2, x3, x4, x5, *l, *r);
+void or_weno5(double x1, double x2, double x3, double x4, double x5, double* l, double* r) {
+    Real lr, rr;
+    weno5(x1, x2, x3, x4, x5, lr, rr);
+    *l = val(lr); *r = val(rr);
 }
+void or_reconstruct(int recon, double x1, double x2, double x3, double x4, double x5, double* l, double* r) {
+    Real lr, rr;
+    reconstruct(recon, x1, x2, x3, x4, x5, lr, rr);
+    *l = val(lr); *r = val(rr);
+}
+#ifdef KH_ORACLE_TANGENT
+// Tangent-linear build: seed the directions of the base state P, U (entire
+// block, [8][nc]); read the tangents of the base state after steps.
+void or_set_tangent(void* h, const double* dP, const double* dU) {
+    Block& b = *static_cast<Block*>(h);
+    for (size_t q = 0; q < (size_t)NV * b.nc; ++q) {
+        b.P0[q].d = dP ? dP[q] : 0.;
+        b.U0[q].d = dU ? dU[q] : 0.;
+    }
+}
+void or_get_tangent(void* h, double* dP, double* dU) {
+    Block& b = *static_cast<Block*>(h);
+    for (size_t q = 0; q < (size_t)NV * b.nc; ++q) {
+        if (dP) dP[q] = b.P0[q].d;
+        if (dU) dU[q] = b.U0[q].d;
+    }
+}
+#endif
 int or_num_threads() {
 #ifdef _OPENMP
     return omp_get_max_threads();

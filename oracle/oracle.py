@@ -14,6 +14,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 # Same source compiled with FMA contraction: measures a workload's roundoff sensitivity.
 LIB_FMA_PATH = os.path.join(HERE, "liboracle_fma.so")
+# The same source on forward-mode dual numbers: exact directional derivatives (tangent-linear).
+LIB_TANGENT_PATH = os.path.join(HERE, "liboracle_tangent.so")
 
 NV = 8
 
@@ -55,14 +57,14 @@ _libs = {}
 def build(force=False):
     """Compile liboracle.so (and liboracle_fma.so) from oracle/kharma_oracle.cpp with the committed Makefile."""
     src = os.path.getmtime(os.path.join(HERE, "kharma_oracle.cpp"))
-    stale = any(not os.path.exists(p) or os.path.getmtime(p) < src for p in (LIB_PATH, LIB_FMA_PATH))
+    stale = any(not os.path.exists(p) or os.path.getmtime(p) < src for p in (LIB_PATH, LIB_FMA_PATH, LIB_TANGENT_PATH))
     if force or stale:
         subprocess.run(["make", "-s", "-C", HERE] + (["-B"] if force else []), check=True)
 
 
 def lib(variant=""):
     if variant not in _libs:
-        path = LIB_FMA_PATH if variant == "fma" else LIB_PATH
+        path = {"fma": LIB_FMA_PATH, "tangent": LIB_TANGENT_PATH}.get(variant, LIB_PATH)
         if not os.path.exists(path):
             build()
         L = ctypes.CDLL(path)
@@ -100,6 +102,9 @@ def lib(variant=""):
         L.or_apply_bcs.argtypes = [vp]
         L.or_set_geometry.argtypes = [vp, i, dp]
         L.or_fix_flux.argtypes = [vp]
+        if variant == "tangent":
+            L.or_set_tangent.argtypes = [vp, dp, dp]
+            L.or_get_tangent.argtypes = [vp, dp, dp]
         _libs[variant] = L
     return _libs[variant]
 
@@ -202,6 +207,19 @@ class OracleBlock:
         P = np.ascontiguousarray(P, dtype=np.float64)
         assert P.shape == (NV,) + self.shape
         self.L.or_init_state(self.h, _ptr(P))
+
+    def set_tangent(self, dP, dU):
+        """Tangent-linear variant: seed the directions of the base state P, U."""
+        dP = np.ascontiguousarray(dP, dtype=np.float64)
+        dU = np.ascontiguousarray(dU, dtype=np.float64)
+        self.L.or_set_tangent(self.h, _ptr(dP), _ptr(dU))
+
+    def tangent(self):
+        """Tangent-linear variant: the tangents (dP, dU) of the base state."""
+        dP = np.empty((NV,) + self.shape)
+        dU = np.empty((NV,) + self.shape)
+        self.L.or_get_tangent(self.h, _ptr(dP), _ptr(dU))
+        return dP, dU
 
     def set_state(self, P, U):
         P = np.ascontiguousarray(P, dtype=np.float64)

@@ -422,6 +422,38 @@ def test_recorded_steps_match_recompute(smooth_state):
         assert rel < 1e-9, (n, rel)
 
 
+def test_step_against_tangent_linear_oracle(smooth_state, oracle_lib):
+    """<lam, dP_2/dP_0 v> from the FP64 tangent-linear oracle (on the library's
+    geometry cache) against <adjoint(lam), v> from the GPU sweep on the torus:
+    2 steps of the smooth state (no zone floored, recovery to 1e-14), timesteps
+    fixed, directions smooth in every variable and scaled per variable; the
+    ghosts follow the interior through the boundary fill in both."""
+    from tests.helpers import install_library_geometry
+    cfg, evo, Ps = smooth_state
+    nsteps = 2
+    dts = evo.run(Ps, nsteps)[1]
+    _, _, ck, _ = evo.run(Ps, nsteps, dts, checkpoints=True)
+    rng = np.random.default_rng(29)
+    lam = rng.normal(size=tuple(Ps.shape))
+    g = _np(evo.backward(_cuda(lam), ck, dts))
+    P0 = _np(evo._embed(Ps))
+    tb = oracle_lib.OracleBlock(cfg, variant="tangent")
+    install_library_geometry(tb, cfg)
+    for q in range(3):
+        v = _direction(cfg, rng, _np(Ps))
+        tb.init_state(P0)
+        tb.set_tangent(_np(evo._embed(_cuda(v))), np.zeros_like(P0))
+        tb.ptou()
+        tb.apply_bcs()
+        for n in range(nsteps):
+            tb.dt = dts[n]
+            tb.step(1)
+        tl = float((lam * tb.tangent()[0][evo.inner]).sum())
+        ad = float((g * v).sum())
+        print(f"[gr tangent-linear] dir {q}: TL {tl:.15e} adjoint {ad:.15e} rel {_rel(tl, ad):.2e}", flush=True)
+        assert _rel(tl, ad) < 1e-8
+
+
 def test_mass_flux_objective_two_level_checkpoints(smooth_state):
     """J = time-integrated mass flux through the face nearest r = 2.5 over 6 steps,
     gradient with checkpoints every 3 steps (segments re-run) and every step."""
