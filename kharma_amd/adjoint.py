@@ -95,22 +95,24 @@ class Evolution:
             ckpts["records"] = {}
         used = []
         time = 0.0
-        for n in range(nsteps):
-            if dts is not None:
-                self.drv.set_clock(time, float(dts[n]), n)
-            if checkpoints and n % every == 0:
-                ckpts[n] = self.drv.state()[0]
+        try:
+            for n in range(nsteps):
+                if dts is not None:
+                    self.drv.set_clock(time, float(dts[n]), n)
+                if checkpoints and n % every == 0:
+                    ckpts[n] = self.drv.state()[0]
+                if record:
+                    ckpts["records"][n] = self.drv.new_stage_record()
+                    self.drv.set_stage_record(ckpts["records"][n])
+                self.drv.step(1)
+                if checkpoints:
+                    self._count(flagged, floors_ok, fixup_ok)
+                dt = dts[n] if dts is not None else self.drv.dt_history(n + 1)[n]
+                used.append(float(dt))
+                time += float(dt)
+        finally:
             if record:
-                ckpts["records"][n] = self.drv.new_stage_record()
-                self.drv.set_stage_record(ckpts["records"][n])
-            self.drv.step(1)
-            if checkpoints:
-                self._count(flagged, floors_ok, fixup_ok)
-            dt = dts[n] if dts is not None else self.drv.dt_history(n + 1)[n]
-            used.append(float(dt))
-            time += float(dt)
-        if record:
-            self.drv.set_stage_record()
+                self.drv.set_stage_record()   # the driver must not keep a pointer to a freed record
         P, _ = self.drv.state()
         J = self.drv.objective() if self.objective else 0.0
         if checkpoints:
@@ -128,15 +130,17 @@ class Evolution:
         self.drv.init_prims(ckpts[a])
         states, recs = [ckpts[a]], []
         time = float(sum(dts[:a]))
-        for n in range(a, b):
-            self.drv.set_clock(time, float(dts[n]), n)
-            recs.append(self.drv.new_stage_record())
-            self.drv.set_stage_record(recs[-1])
-            self.drv.step(1)
-            time += float(dts[n])
-            if n < b - 1:
-                states.append(self.drv.state()[0])
-        self.drv.set_stage_record()
+        try:
+            for n in range(a, b):
+                self.drv.set_clock(time, float(dts[n]), n)
+                recs.append(self.drv.new_stage_record())
+                self.drv.set_stage_record(recs[-1])
+                self.drv.step(1)
+                time += float(dts[n])
+                if n < b - 1:
+                    states.append(self.drv.state()[0])
+        finally:
+            self.drv.set_stage_record()
         states.append(ckpts[b])
         return states, recs
 
