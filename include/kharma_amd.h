@@ -64,6 +64,12 @@ extern "C" {
 #define KH_DRIVER_KHARMA 0
 #define KH_DRIVER_SIMPLE 1
 
+/* inverter/type (inverter/inverter.cpp:53-70): Inverter::u_to_p<onedw>
+ * (inverter/onedw.hpp:89-233) or u_to_p<kastaun> (inverter/kastaun.hpp:83-400),
+ * the reference's default, whose own defaults are err_tol 1e-12, iter_max 25. */
+#define KH_INVERTER_ONEDW 0
+#define KH_INVERTER_KASTAUN 1
+
 /* Run parameters of one meshblock: the .par options this path reads
  * (GRMHD/*, flux/*, floors/*, inverter/*, parthenon/time/*, parthenon/mesh/*). */
 typedef struct kh_params {
@@ -100,6 +106,7 @@ typedef struct kh_params {
     int driver_type;                /* KH_DRIVER_* (driver/type) */
     int kill_on_large_divb;         /* b_field/kill_on_large_divb (the reference's default: on) */
     double kill_on_divb_over;       /* b_field/kill_on_divb_over (default 1e-3) */
+    int inverter;                   /* KH_INVERTER_* (inverter/type) */
 } kh_params;
 
 typedef struct kh_block kh_block;

@@ -27,6 +27,7 @@ COORDS = {"cartesian_minkowski": 0, "ks": 1}
 TRANSFORM = {"null": 0, "mks": 1, "fmks": 2}
 BC = {"periodic": 0, "outflow": 1, "reflecting": 2}
 DRIVER = {"kharma": 0, "simple": 1}
+INVERTER = {"onedw": 0, "kastaun": 1}
 
 
 class KhParams(ctypes.Structure):
@@ -58,6 +59,7 @@ class KhParams(ctypes.Structure):
         ("zero_polar_flux", ctypes.c_int), ("fix_polar_flux", ctypes.c_int), ("fix_corner_inner", ctypes.c_int),
         ("driver_type", ctypes.c_int),
         ("kill_on_large_divb", ctypes.c_int), ("kill_on_divb_over", ctypes.c_double),
+        ("inverter", ctypes.c_int),
     ]
 
 
@@ -189,6 +191,7 @@ def make_params(cfg):
     p.driver_type = DRIVER[cfg.get("driver", "kharma")]
     p.kill_on_large_divb = int(cfg.get("kill_on_large_divb", True))
     p.kill_on_divb_over = cfg.get("kill_on_divb_over", 1e-3)
+    p.inverter = INVERTER[cfg.get("inverter", "onedw")]
     p.gam = cfg["gamma"]
     p.cfl = cfg.get("cfl", 0.9)
     p.dt_min = cfg.get("dt_min", 1e-5)

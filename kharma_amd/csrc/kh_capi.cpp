@@ -75,6 +75,8 @@ int kh::check_params(const kh_params* p) {
     if (p->integrator != 0 && p->integrator != 1) return fail(KH_ERR_INVALID, "unknown integrator");
     if (p->driver_type != KH_DRIVER_KHARMA && p->driver_type != KH_DRIVER_SIMPLE)
         return fail(KH_ERR_INVALID, "unknown driver type");
+    if (p->inverter != KH_INVERTER_ONEDW && p->inverter != KH_INVERTER_KASTAUN)
+        return fail(KH_ERR_INVALID, "unknown inverter type");
     if (p->kill_on_large_divb && !(p->kill_on_divb_over > 0.))
         return fail(KH_ERR_INVALID, "kill_on_divb_over must be positive when kill_on_large_divb is set");
     if (p->floor_frame < 0 || p->floor_frame > 2) return fail(KH_ERR_INVALID, "unknown floor frame");
@@ -135,6 +137,7 @@ void fill_consts(const kh_params* p, StepConsts& c) {
     c.fix_atmo = p->fix_atmo;
     c.use_hlle = p->use_hlle;
     c.kill_divb = p->kill_on_large_divb;
+    c.inverter = p->inverter;
     c.divb_over = p->kill_on_divb_over;
     FloorParams& f = c.floors;
     f.rho_min_geom = p->rho_min_geom; f.u_min_geom = p->u_min_geom;

@@ -602,6 +602,139 @@ __device__ __forceinline__ void apply_floors_drift(const G& g, T P[NV], double g
     P[U3] = ucon[3] + (beta[3] * ucon[0]);
 }
 
+// Inverter::u_to_p<kastaun>, inverter/kastaun.hpp:83-400 (Kastaun, Kalinani &
+// Ciolfi 2021): the conserved state is moved to the normal observer's frame,
+// the auxiliary function mu sqrt(1 + rbar^2(mu)) - 1 (their eq. 49) is
+// bracketed on [0, 1] and the master function f(mu) = mu - muhat(mu) (eq. 44)
+// solved on [0, root], both by false position with the Illinois halving.  The
+// inverter's own floors (density and specific-energy floors, the velocity
+// ceiling from gamma_max, the specific-energy ceiling) act inside the master
+// function and are reported as the INVERTER_* floor bits; a master solve that
+// does not converge in max_iterations returns S_MAX_ITER.  Always FP64, with
+// IEEE quotients.
+constexpr int F_INVERTER_RHO = 16384, F_INVERTER_U = 32768, F_INVERTER_GAMMA = 65536, F_INVERTER_U_MAX = 131072;
+struct KastaunMaster {
+    double D, q, bsq, bsq_rpsq, rsq, rbsq, v0sq, gam, rho_floor, e_floor, e_max;
+    __device__ __forceinline__ double x_mu(double mu) const { return 1.0 / (1.0 + mu * bsq); }
+    __device__ __forceinline__ double rbarsq_mu(double mu, double x) const { return x * (x * rsq + mu * (1.0 + x) * rbsq); }
+    __device__ __forceinline__ double qbar_mu(double mu, double x) const {
+        const double mux = mu * x;
+        return q - 0.5 * (bsq + mux * mux * bsq_rpsq);
+    }
+    __device__ __forceinline__ double vhatsq_mu(double mu, double rbarsq, int& fl) const {
+        const double v = mu * mu * rbarsq;
+        if (v > v0sq) { fl |= F_INVERTER_GAMMA; return v0sq; }
+        return v;
+    }
+    __device__ __forceinline__ double rhohat_mu(double iW, int& fl) const {
+        const double r = D * iW;
+        if (r <= rho_floor) { fl |= F_INVERTER_RHO; return rho_floor; }
+        return r;
+    }
+    __device__ __forceinline__ double ehat_mu(double mu, double qbar, double rbarsq, double vhatsq, double W,
+                                              int& fl) const {
+        const double e = W * (qbar - mu * rbarsq) + vhatsq * W * W / (1.0 + W);
+        if (e <= e_floor) { fl |= F_INVERTER_U; return e_floor; }
+        if (e > e_max) { fl |= F_INVERTER_U_MAX; return e_max; }
+        return e;
+    }
+    __device__ __forceinline__ double aux(double mu) const {
+        const double x = 1.0 / (1.0 + mu * bsq);
+        const double rbarsq = x * (rsq * x + mu * (1.0 + x) * rbsq);
+        return mu * sqrt(1.0 + rbarsq) - 1.0;
+    }
+    __device__ __forceinline__ double master(double mu) const {
+        int fl = 0;
+        const double x = x_mu(mu);
+        const double rbarsq = rbarsq_mu(mu, x);
+        const double qbar = qbar_mu(mu, x);
+        const double vhatsq = vhatsq_mu(mu, rbarsq, fl);
+        const double iW = sqrt(1.0 - vhatsq);
+        const double W = 1.0 / iW;
+        const double rhohat = rhohat_mu(iW, fl);
+        const double ehat = ehat_mu(mu, qbar, rbarsq, vhatsq, W, fl);
+        const double Phat = ehat * rhohat * (gam - 1.0);
+        const double hhat = rhohat * (1.0 + ehat) + Phat;
+        const double ahat = Phat / (hhat - Phat);
+        const double nua = (1.0 + ahat) * (1.0 + ehat) * iW;
+        const double nub = (1.0 + ahat) * (1.0 + qbar - mu * rbarsq);
+        const double nuhat = nua > nub ? nua : nub;
+        return mu - 1.0 / (nuhat + mu * rbarsq);
+    }
+};
+// False position with the Illinois halving on [zm, zp] (the reference's loop,
+// kastaun.hpp:298-324 and 338-364); returns the last iterate, iter the count.
+template <class F>
+__device__ __forceinline__ double illinois(F&& f, double zm, double zp, int max_iterations, double tol, int& iter) {
+    double fm = f(zm), fp = f(zp);
+    int iterations = max_iterations;
+    if (fabs(zm - zp) < tol || (fabs(fm) + fabs(fp)) < 2.0 * tol) iterations = -1;
+    double z = 0.5 * (zm + zp);
+    for (iter = 0; iter < iterations; ++iter) {
+        z = (zm * fp - zp * fm) / (fp - fm);
+        const double fz = f(z);
+        if (fabs(zm - zp) < tol || fabs(fz) < tol) break;
+        if (fz * fp < 0.0) { zm = zp; fm = fp; zp = z; fp = fz; }
+        else { fm = 0.5 * fm; zp = z; fp = fz; }
+    }
+    if (iterations < 0) iter = 0;
+    return z;
+}
+template <class G>
+__device__ __forceinline__ int u_to_p_kastaun(const G& g, const double U[NV], double P[NV], double gam,
+                                              const FloorParams& fp, int max_iterations, double tol) {
+    double alpha, a_over_g;
+    if constexpr (G::flat) { alpha = 1.; a_over_g = 1.; }
+    else { alpha = 1. / sqrt(-g.gcon(0, 0)); a_over_g = alpha / g.gdet(); }
+    const double D = U[RHO] * a_over_g;
+    const double D_fl = D > fp.rho_min_const ? D : fp.rho_min_const;
+    const double Qcov[4] = {(U[UU] - U[RHO]) * a_over_g, U[U1] * a_over_g, U[U2] * a_over_g, U[U3] * a_over_g};
+    const double ncov[4] = {-alpha, 0., 0., 0.};
+    double ncon[4];
+    g.raise(ncov, ncon);
+    const double q = (-dot4(Qcov, ncon) - D) / D;
+    const double rcov[3] = {U[U1] / U[RHO], U[U2] / U[RHO], U[U3] / U[RHO]};
+    double rcon[3];
+    const double g00 = g.gcon(0, 0);
+    for (int a = 0; a < 3; ++a) {
+        rcon[a] = 0.;
+        for (int b = 0; b < 3; ++b) rcon[a] += (g.gcon(a + 1, b + 1) - g.gcon(0, a + 1) * g.gcon(0, b + 1) / g00) * rcov[b];
+    }
+    const double rsq = rcon[0] * rcov[0] + rcon[1] * rcov[1] + rcon[2] * rcov[2];
+    const double sD = 1.0 / sqrt(D_fl);
+    double bu[3], bdotr = 0.;
+    for (int a = 0; a < 3; ++a) {
+        bu[a] = (U[B1 + a] * a_over_g) * sD;
+        bdotr += bu[a] * rcov[a];
+    }
+    double bsq = 0.;
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) bsq += g.gcov(a + 1, b + 1) * bu[a] * bu[b];
+    bsq = bsq > 0. ? bsq : 0.;
+    const double rbsq = bdotr * bdotr;
+    KastaunMaster m;
+    m.D = D; m.q = q; m.bsq = bsq; m.bsq_rpsq = bsq * rsq - rbsq; m.rsq = rsq; m.rbsq = rbsq;
+    const double vmax = 1.0 - 1.0 / (fp.gamma_max * fp.gamma_max);
+    m.v0sq = rsq / (1.0 + rsq) < vmax ? rsq / (1.0 + rsq) : vmax;
+    m.gam = gam; m.rho_floor = fp.rho_min_const; m.e_floor = fp.u_min_const / D_fl; m.e_max = fp.u_over_rho_max;
+    int iter;
+    const double zb = illinois([&](double mu) { return m.aux(mu); }, 0., 1., max_iterations, tol, iter);
+    const double mu = illinois([&](double z) { return m.master(z); }, 0., zb, max_iterations, tol, iter);
+    if (iter == max_iterations) return S_MAX_ITER;
+    int fl = 0;
+    const double x = m.x_mu(mu);
+    const double rbarsq = m.rbarsq_mu(mu, x);
+    const double vsq = m.vhatsq_mu(mu, rbarsq, fl);
+    const double iW = sqrt(1.0 - vsq);
+    const double W = 1.0 / iW;
+    const double rho = m.rhohat_mu(iW, fl);
+    const double qbar = m.qbar_mu(mu, x);
+    P[RHO] = rho;
+    P[UU] = m.ehat_mu(mu, qbar, rbarsq, vsq, W, fl) * rho;
+    for (int a = 0; a < 3; ++a) P[U1 + a] = W * mu * x * (rcon[a] + mu * bdotr * bu[a]);
+    return fl;
+}
+
 // Normal-observer-frame floors, floors_functions.hpp:260-289 (U is modified in registers)
 template <class G>
 __device__ __forceinline__ int apply_floors_normal(const G& g, double P[NV], double U[NV], double gam,

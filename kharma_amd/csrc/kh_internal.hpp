@@ -57,6 +57,7 @@ struct StepConsts {
     FloorParams floors;
     int kill_divb;                      // b_field/kill_on_large_divb
     double divb_over;                   // b_field/kill_on_divb_over
+    int inverter;                       // inverter/type: 0 onedw, 1 kastaun
 };
 
 enum Precision { PREC_FP64 = 0, PREC_SPLIT = 1 };

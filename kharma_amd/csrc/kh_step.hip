@@ -371,9 +371,16 @@ __device__ __forceinline__ void zone_divergence_ct(const BlockDims& b, const R* 
     }
 }
 
+// The U->P solver of inverter/type (inverter.cpp:53-70): INV 0 onedw, 1 Kastaun.
+template <int INV, class G>
+__device__ __forceinline__ int invert_zone(const G& g, const double U[NV], double P[NV], const StepConsts& cst) {
+    if constexpr (INV == 1) return u_to_p_kastaun(g, U, P, cst.gam, cst.floors, cst.iter_max, cst.err_tol);
+    else return u_to_p_onedw(g, U, P, cst.gam, cst.iter_max, cst.err_tol);
+}
+
 // ------------------------------------------------------- fused stage update
 // Stage 1 (gam0 = 0 and U_sub == U_full) reads the state once: 0*U + 1*U == U.
-template <typename R, bool DO_DT, bool CT, int GEO>
+template <typename R, bool DO_DT, bool CT, int GEO, int INV = 0>
 __global__ void __launch_bounds__(BS, UpdateOcc<GEO>::min_blocks)
 update_kernel(BlockDims b, StepConsts cst, GeomDev gd, int stage, const R* __restrict__ F, const R* __restrict__ ctop,
               const double* __restrict__ Usub, const double* Ufull, const double* __restrict__ Psub,
@@ -416,7 +423,7 @@ update_kernel(BlockDims b, StepConsts cst, GeomDev gd, int stage, const R* __res
         double rf, uf;
         geom_floors<GEO>(cst, gd, j, i, rf, uf);
         with_center_geom<GEO>(gd, j, i, [&](const auto& g) {
-            pf = u_to_p_onedw(g, U, P, gam, cst.iter_max, cst.err_tol);
+            pf = invert_zone<INV>(g, U, P, cst);
             const int ff0 = (pf / F_GEOM_RHO) * F_GEOM_RHO;
             pf = pf % F_GEOM_RHO;
             const double gd_ = g.gdet();
@@ -535,7 +542,7 @@ combine_kernel(BlockDims b, StepConsts cst, GeomDev gd, int stage, const R* __re
 
 // U->P with the stage's starting primitives as guess (kharma_driver.cpp:379-384),
 // B from U, floors in frame FRAME, P->U; U is read from and written back to Uout.
-template <int FRAME>
+template <int FRAME, int INV = 0>
 __global__ void __launch_bounds__(BS, KH_REC_OCC)
 recover_kernel(BlockDims b, StepConsts cst, GeomDev gd, int stage, const double* __restrict__ Psub, double* Uout,
                double* __restrict__ Pout, float* __restrict__ Pout32, int* __restrict__ pflag,
@@ -556,7 +563,7 @@ recover_kernel(BlockDims b, StepConsts cst, GeomDev gd, int stage, const double*
     geom_floors<1>(cst, gd, j, i, rf, uf);
     {
         CachedGeom<double> g(gd.g[GLOC_CENTER], gd.np, (long)j * gd.gn1 + i);
-        pf = u_to_p_onedw(g, U, P, gam, cst.iter_max, cst.err_tol);
+        pf = invert_zone<INV>(g, U, P, cst);
         const int ff0 = (pf / F_GEOM_RHO) * F_GEOM_RHO;
         pf = pf % F_GEOM_RHO;
         const double gd_ = g.gdet();
@@ -945,7 +952,7 @@ __device__ __forceinline__ void zone_ijk(const BlockDims& b, long c, int& k, int
 }
 
 // Inverter::BlockUtoP over the physical range + B_FluxCT::BlockUtoP over the entire block
-template <int GEO>
+template <int GEO, int INV = 0>
 __global__ void __launch_bounds__(BS) utop_kernel(BlockDims b, StepConsts cst, GeomDev gd, const double* __restrict__ U,
                                                   double* __restrict__ P, int* __restrict__ pflag, int* __restrict__ fflag) {
     const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -956,7 +963,7 @@ __global__ void __launch_bounds__(BS) utop_kernel(BlockDims b, StepConsts cst, G
         if (in_physical(b, k, j, i)) {
             double Uz[NV], Pz[NV];
             for (int v = 0; v < NV; ++v) { Uz[v] = U[v * b.nc + c]; Pz[v] = P[v * b.nc + c]; }
-            const int r = u_to_p_onedw(g, Uz, Pz, cst.gam, cst.iter_max, cst.err_tol);
+            const int r = invert_zone<INV>(g, Uz, Pz, cst);
             for (int v = 0; v < 5; ++v) P[v * b.nc + c] = Pz[v];
             pflag[c] = r % F_GEOM_RHO;
             fflag[c] = (r / F_GEOM_RHO) * F_GEOM_RHO;
@@ -1171,11 +1178,17 @@ static void launch_update_geo(const BlockDims& b, const StepConsts& c, const Geo
 #undef KH_CMB
     }
     if (!(parts & 2)) return;
-#define KH_REC(FR) \
-    hipLaunchKernelGGL((recover_kernel<FR>), g, dim3(BS), 0, s, b, c, gd, stage, Psub, Uout, Pout, Pout32, pflag, fflag, fail_list, ds)
-    if (c.floors.frame == 0) KH_REC(0);
-    else if (c.floors.frame == 1) KH_REC(1);
-    else KH_REC(2);
+#define KH_REC(FR, INV) \
+    hipLaunchKernelGGL((recover_kernel<FR, INV>), g, dim3(BS), 0, s, b, c, gd, stage, Psub, Uout, Pout, Pout32, pflag, fflag, fail_list, ds)
+    if (c.inverter == 1) {
+        if (c.floors.frame == 0) KH_REC(0, 1);
+        else if (c.floors.frame == 1) KH_REC(1, 1);
+        else KH_REC(2, 1);
+    } else {
+        if (c.floors.frame == 0) KH_REC(0, 0);
+        else if (c.floors.frame == 1) KH_REC(1, 0);
+        else KH_REC(2, 0);
+    }
 #undef KH_REC
 }
 template <typename R, int GEO>
@@ -1185,11 +1198,16 @@ static void launch_update_t(const BlockDims& b, const StepConsts& c, const GeomD
                             int* fail_list, DeviceState* ds, double* dt_part, hipStream_t s) {
     const dim3 g = grid_for(ninterior_host(b));
     const bool ct = b.ndim > 1;
-#define KH_UPD(DT, CT)                                                                                         \
-    hipLaunchKernelGGL((update_kernel<R, DT, CT, GEO>), g, dim3(BS), 0, s, b, c, gd, stage, (const R*)F, (const R*)ctop, \
+#define KH_UPD(DT, CT, INV)                                                                                         \
+    hipLaunchKernelGGL((update_kernel<R, DT, CT, GEO, INV>), g, dim3(BS), 0, s, b, c, gd, stage, (const R*)F, (const R*)ctop, \
                        Usub, Ufull, Psub, Uout, Pout, Pout32, pflag, fflag, fail_list, ds, dt_part)
-    if (do_dt) { if (ct) KH_UPD(true, true); else KH_UPD(true, false); }
-    else { if (ct) KH_UPD(false, true); else KH_UPD(false, false); }
+    if (c.inverter == 1) {
+        if (do_dt) { if (ct) KH_UPD(true, true, 1); else KH_UPD(true, false, 1); }
+        else { if (ct) KH_UPD(false, true, 1); else KH_UPD(false, false, 1); }
+    } else {
+        if (do_dt) { if (ct) KH_UPD(true, true, 0); else KH_UPD(true, false, 0); }
+        else { if (ct) KH_UPD(false, true, 0); else KH_UPD(false, false, 0); }
+    }
 #undef KH_UPD
 }
 hipError_t launch_update(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int prec, int stage, bool do_dt,
@@ -1285,8 +1303,13 @@ hipError_t launch_weighted_sum(const BlockDims& b, const double* Usub, const dou
 }
 hipError_t launch_utop(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const double* U, double* P,
                        int* pflag, int* fflag, hipStream_t s) {
-    if (b.geo) hipLaunchKernelGGL(utop_kernel<1>, grid_for(b.nc), dim3(BS), 0, s, b, c, gd, U, P, pflag, fflag);
-    else hipLaunchKernelGGL(utop_kernel<0>, grid_for(b.nc), dim3(BS), 0, s, b, c, gd, U, P, pflag, fflag);
+    if (c.inverter == 1) {
+        if (b.geo) hipLaunchKernelGGL((utop_kernel<1, 1>), grid_for(b.nc), dim3(BS), 0, s, b, c, gd, U, P, pflag, fflag);
+        else hipLaunchKernelGGL((utop_kernel<0, 1>), grid_for(b.nc), dim3(BS), 0, s, b, c, gd, U, P, pflag, fflag);
+    } else {
+        if (b.geo) hipLaunchKernelGGL(utop_kernel<1>, grid_for(b.nc), dim3(BS), 0, s, b, c, gd, U, P, pflag, fflag);
+        else hipLaunchKernelGGL(utop_kernel<0>, grid_for(b.nc), dim3(BS), 0, s, b, c, gd, U, P, pflag, fflag);
+    }
     return hipGetLastError();
 }
 hipError_t launch_floors(const BlockDims& b, const StepConsts& c, const GeomDev& gd, double* U, double* P, int* pflag,

@@ -66,9 +66,11 @@ def interior(cfg):
 
 
 def _base_cfg(**kw):
+    # inverter/iter_max and err_tol default by solver (inverter.cpp:67-70)
+    kastaun = kw.get("inverter", "onedw") == "kastaun"
     cfg = dict(ng=4, cfl=0.9, dt_min=1e-5, max_dt_increase=2.0, tlim=1e30,
                recon="weno5", riemann="llf", floors=True, floor_frame="drift",
-               iter_max=8, err_tol=1e-8, fix_average=True, fix_atmo=True,
+               iter_max=25 if kastaun else 8, err_tol=1e-12 if kastaun else 1e-8, fix_average=True, fix_atmo=True,
                integrator="rk2", coords="cartesian_minkowski")
     cfg.update(kw)
     cfg.setdefault("dt_start", cfg["dt_min"])

@@ -79,6 +79,7 @@ enum { FACE1 = 0, FACE2 = 1, FACE3 = 2, CENTER = 3, CORNER = 4, NLOC = 5 };
 constexpr int F_GEOM_RHO = 32, F_GEOM_U = 64, F_B_RHO = 128, F_B_U = 256, F_TEMP = 512,
               F_GAMMA = 1024, F_KTOT = 2048, F_GEOM_RHO_FLUX = 4096, F_GEOM_U_FLUX = 8192;
 constexpr int FFLAG_MINIMUM = F_GEOM_RHO;
+constexpr int F_INVERTER_RHO = 16384, F_INVERTER_U = 32768, F_INVERTER_GAMMA = 65536, F_INVERTER_U_MAX = 131072;
 // Inverter::Status, inverter/invert_template.hpp:53
 enum { S_SUCCESS = 0, S_NEG_INPUT, S_MAX_ITER, S_BAD_UT, S_BAD_GAMMA, S_NEG_RHO, S_NEG_U, S_NEG_RHOU };
 
@@ -155,6 +156,7 @@ struct or_params {
     int zero_polar_flux, fix_polar_flux, fix_corner_inner;
     int periodic1, periodic2, periodic3;
     int driver;                        // driver/type: 0 kharma, 1 simple
+    int inverter;                      // inverter/type: 0 onedw, 1 kastaun
 };
 }
 
@@ -890,6 +892,126 @@ Range3 physical_range(const Block& b) {
     return r;
 }
 
+// Inverter::u_to_p<kastaun>, inverter/kastaun.hpp:83-400 (Kastaun, Kalinani &
+// Ciolfi 2021).  The residual of the master function (their eq. 44) with the
+// inverter's floors (KastaunResidual, kastaun.hpp:83-206), the bracketing
+// auxiliary function (eq. 49), and the two false-position solves with the
+// Illinois halving (kastaun.hpp:290-364); returns the INVERTER_* floor bits, or
+// S_MAX_ITER when the master solve does not converge.
+struct KastaunResidual {
+    Real D, q, bsq, bsq_rpsq, rsq, rbsq, v0sq, gam, rho_floor, e_floor, e_max;
+    bool rho_fl = false, e_fl = false, e_mx = false, g_mx = false;
+    Real x_mu(Real mu) const { return 1.0 / (1.0 + mu * bsq); }
+    Real rbarsq_mu(Real mu, Real x) const { return x * (x * rsq + mu * (1.0 + x) * rbsq); }
+    Real qbar_mu(Real mu, Real x) const { const Real mux = mu * x; return q - 0.5 * (bsq + mux * mux * bsq_rpsq); }
+    Real vhatsq_mu(Real mu, Real rbarsq) {
+        const Real v = mu * mu * rbarsq;
+        g_mx = v > v0sq;
+        return g_mx ? v0sq : v;
+    }
+    Real rhohat_mu(Real iW) {
+        const Real r = D * iW;
+        rho_fl = r <= rho_floor;
+        return rho_fl ? rho_floor : r;
+    }
+    Real ehat_mu(Real mu, Real qbar, Real rbarsq, Real vhatsq, Real W) {
+        const Real e = W * (qbar - mu * rbarsq) + vhatsq * W * W / (1.0 + W);
+        e_fl = false; e_mx = false;
+        if (e <= e_floor) { e_fl = true; return e_floor; }
+        if (e > e_max) { e_mx = true; return e_max; }
+        return e;
+    }
+    Real master(Real mu) {
+        const Real x = x_mu(mu);
+        const Real rbarsq = rbarsq_mu(mu, x);
+        const Real qbar = qbar_mu(mu, x);
+        const Real vhatsq = vhatsq_mu(mu, rbarsq);
+        const Real iW = std::sqrt(1.0 - vhatsq);
+        const Real W = 1.0 / iW;
+        const Real rhohat = rhohat_mu(iW);
+        const Real ehat = ehat_mu(mu, qbar, rbarsq, vhatsq, W);
+        const Real Phat = ehat * rhohat * (gam - 1.0);
+        const Real hhat = rhohat * (1.0 + ehat) + Phat;
+        const Real ahat = Phat / (hhat - Phat);
+        const Real nua = (1.0 + ahat) * (1.0 + ehat) * iW;
+        const Real nub = (1.0 + ahat) * (1.0 + qbar - mu * rbarsq);
+        const Real nuhat = std::max(nua, nub);
+        return mu - 1.0 / (nuhat + mu * rbarsq);
+    }
+    Real aux(Real mu) const {
+        const Real x = 1.0 / (1.0 + mu * bsq);
+        const Real rbarsq = x * (rsq * x + mu * (1.0 + x) * rbsq);
+        return mu * std::sqrt(1.0 + rbarsq) - 1.0;
+    }
+};
+template <class F>
+Real false_position_illinois(F&& f, Real zm, Real zp, int max_iterations, Real tol, int& iter) {
+    Real fm = f(zm), fp = f(zp);
+    int iterations = max_iterations;
+    if (std::fabs(zm - zp) < tol || (std::fabs(fm) + std::fabs(fp)) < 2.0 * tol) iterations = -1;
+    Real z = 0.5 * (zm + zp);
+    for (iter = 0; iter < iterations; ++iter) {
+        z = (zm * fp - zp * fm) / (fp - fm);
+        const Real fz = f(z);
+        if (std::fabs(zm - zp) < tol || std::fabs(fz) < tol) break;
+        if (fz * fp < 0.0) { zm = zp; fm = fp; zp = z; fp = fz; }
+        else { fm = 0.5 * fm; zp = z; fp = fz; }
+    }
+    if (iterations < 0) iter = 0;
+    return z;
+}
+int u_to_p_kastaun(const Block& b, const Real* U, Real* P, int j, int i, int loc, int max_iterations, Real tol) {
+    const auto& f = b.p;
+    const Real alpha = 1. / std::sqrt(-b.gcon_(loc, j, i, 0, 0));
+    const Real a_over_g = alpha / b.gdet_(loc, j, i);
+    const Real D = U[RHO] * a_over_g;
+    const Real D_fl = std::max(D, Real(f.rho_min_const));
+    const Real Qcov[4] = {(U[UU] - U[RHO]) * a_over_g, U[U1] * a_over_g, U[U2] * a_over_g, U[U3] * a_over_g};
+    const Real ncov[4] = {-alpha, 0., 0., 0.};
+    Real ncon[4];
+    b.raise(ncov, ncon, j, i, loc);
+    const Real q = (-dot(Qcov, ncon) - D) / D;
+    const Real rcov[3] = {U[U1] / U[RHO], U[U2] / U[RHO], U[U3] / U[RHO]};
+    Real rcon[3];
+    // spatial inverse metric gamma^ij = g^ij - g^0i g^0j / g^00 (kastaun.hpp:241-257)
+    const Real g00 = b.gcon_(loc, j, i, 0, 0);
+    for (int a = 0; a < 3; ++a) {
+        rcon[a] = 0.;
+        for (int c = 0; c < 3; ++c)
+            rcon[a] += (b.gcon_(loc, j, i, a + 1, c + 1) - b.gcon_(loc, j, i, 0, a + 1) * b.gcon_(loc, j, i, 0, c + 1) / g00) * rcov[c];
+    }
+    const Real rsq = rcon[0] * rcov[0] + rcon[1] * rcov[1] + rcon[2] * rcov[2];
+    const Real sD = 1.0 / std::sqrt(D_fl);
+    Real bu[3], bdotr = 0.;
+    for (int a = 0; a < 3; ++a) { bu[a] = (U[B1 + a] * a_over_g) * sD; bdotr += bu[a] * rcov[a]; }
+    Real bsq = 0.;
+    for (int a = 0; a < 3; ++a) for (int c = 0; c < 3; ++c) bsq += b.gcov_(loc, j, i, a + 1, c + 1) * bu[a] * bu[c];
+    bsq = std::max(Real(0.), bsq);
+    KastaunResidual res;
+    res.D = D; res.q = q; res.bsq = bsq; res.rsq = rsq; res.rbsq = bdotr * bdotr; res.bsq_rpsq = bsq * rsq - res.rbsq;
+    res.v0sq = std::min(rsq / (1.0 + rsq), Real(1.0 - 1.0 / SQR(f.gamma_max)));
+    res.gam = f.gam; res.rho_floor = f.rho_min_const; res.e_floor = f.u_min_const / D_fl; res.e_max = f.u_over_rho_max;
+    int iter;
+    const Real zb = false_position_illinois([&](Real mu) { return res.aux(mu); }, Real(0.), Real(1.), max_iterations, tol, iter);
+    const Real mu = false_position_illinois([&](Real z) { return res.master(z); }, Real(0.), zb, max_iterations, tol, iter);
+    if (iter == max_iterations) return S_MAX_ITER;
+    const Real x = res.x_mu(mu);
+    const Real rbarsq = res.rbarsq_mu(mu, x);
+    const Real vsq = res.vhatsq_mu(mu, rbarsq);
+    const Real iW = std::sqrt(1.0 - vsq);
+    const Real W = 1. / iW;
+    P[RHO] = res.rhohat_mu(iW);
+    const Real qbar = res.qbar_mu(mu, x);
+    P[UU] = res.ehat_mu(mu, qbar, rbarsq, vsq, W) * P[RHO];
+    for (int a = 0; a < 3; ++a) P[U1 + a] = W * mu * x * (rcon[a] + mu * bdotr * bu[a]);
+    int fl = 0;
+    if (res.rho_fl) fl |= F_INVERTER_RHO;
+    if (res.e_fl) fl |= F_INVERTER_U;
+    if (res.g_mx) fl |= F_INVERTER_GAMMA;
+    if (res.e_mx) fl |= F_INVERTER_U_MAX;
+    return fl;
+}
+
 // Packages::MeshUtoP -> Inverter::BlockUtoP (inverter.cpp:131-200, over the
 // physical range whatever the domain) then B_FluxCT::BlockUtoP
 // (b_flux_ct.cpp:195-217, over the domain: entire, or the interior when
@@ -903,7 +1025,8 @@ void utop(Block& b, std::vector<Real>& U, std::vector<Real>& P, bool b_interior 
         const size_t c = b.id(k, j, i);
         Real Uc[NV], Pc[NV];
         load_prims(b, U, c, Uc); load_prims(b, P, c, Pc);
-        int pflagl = u_to_p_onedw(b, Uc, Pc, j, i, CENTER, b.p.iter_max, b.p.err_tol);
+        int pflagl = (b.p.inverter == 1) ? u_to_p_kastaun(b, Uc, Pc, j, i, CENTER, b.p.iter_max, b.p.err_tol)
+                                         : u_to_p_onedw(b, Uc, Pc, j, i, CENTER, b.p.iter_max, b.p.err_tol);
         for (int v = 0; v < 5; ++v) P[v * b.nc + c] = Pc[v];
         b.pflag[c] = pflagl % FFLAG_MINIMUM;
         b.fflag[c] = (pflagl / FFLAG_MINIMUM) * FFLAG_MINIMUM;

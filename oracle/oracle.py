@@ -47,7 +47,7 @@ class OrParams(ctypes.Structure):
         ("check_inflow_inner_x1", ctypes.c_int), ("check_inflow_outer_x1", ctypes.c_int),
         ("zero_polar_flux", ctypes.c_int), ("fix_polar_flux", ctypes.c_int), ("fix_corner_inner", ctypes.c_int),
         ("periodic1", ctypes.c_int), ("periodic2", ctypes.c_int), ("periodic3", ctypes.c_int),
-        ("driver", ctypes.c_int),
+        ("driver", ctypes.c_int), ("inverter", ctypes.c_int),
     ]
 
 
@@ -177,6 +177,7 @@ def make_params(cfg):
     p.fix_polar_flux = int(cfg.get("fix_polar_flux", False))
     p.fix_corner_inner = int(cfg.get("fix_corner_inner", False))
     p.driver = DRIVER[cfg.get("driver", "kharma")]
+    p.inverter = {"onedw": 0, "kastaun": 1}[cfg.get("inverter", "onedw")]
     return p
 
 

@@ -49,6 +49,7 @@ def _create(cfg, **over):
     (dict(coords=1, transform=7), 1),
     (dict(nx1=7), 1),                # periodic axis shorter than twice the ghost depth
     (dict(driver_type=5), 1),        # unknown driver/type
+    (dict(inverter=2), 1),           # unknown inverter/type
     (dict(kill_on_large_divb=1, kill_on_divb_over=0.0), 1),   # divB stop without a threshold
 ])
 def test_invalid_params_rejected_on_host(L, over, code):

@@ -148,6 +148,8 @@ def test_adjoint_rejects_unsupported():
         kh.AdjointStep(dict(cfg, recon="ppm"))
     with pytest.raises(kh.KharmaError):
         kh.AdjointStep(dict(cfg, riemann="hlle"))
+    with pytest.raises(kh.KharmaError, match="onedw"):
+        kh.AdjointStep(dict(cfg, inverter="kastaun"))
     tcfg = pr.torus_config(nx1=32, nx2=16, nx3=1)
     with pytest.raises(kh.KharmaError):
         kh.AdjointStep(dict(tcfg, floor_frame="normal"))

@@ -240,6 +240,8 @@ VARIANTS = [
     ("fast3d", dict(recon="linear_mc")),       # reconstruct<linear_mc>, reconstruction.hpp:107-123
     ("fast3d", dict(riemann="hlle", integrator="vl2")),
     ("ot2d", dict(ot_init="reference")),       # InitializeOrszagTang + seed_a<orszag_tang_a>
+    ("ot2d", dict(inverter="kastaun", err_tol=1e-12, iter_max=25)),     # inverter/kastaun.hpp:83-400
+    ("fast3d", dict(inverter="kastaun", err_tol=1e-12, iter_max=25)),
 ]
 
 

@@ -45,6 +45,8 @@ CASES = {
     # floors/frame (floors.cpp:68): fluid-frame and normal-observer-frame injection
     "torus2d_fluid": lambda: dict(pr.torus_config(64, 48, 1), floor_frame="fluid"),
     "torus2d_normal": lambda: dict(pr.torus_config(64, 48, 1), floor_frame="normal"),
+    # inverter/type = kastaun (inverter/kastaun.hpp:83-400), the reference's default, with its defaults
+    "torus2d_kastaun": lambda: dict(pr.torus_config(64, 48, 1), inverter="kastaun", err_tol=1e-12, iter_max=25),
 }
 
 
@@ -134,7 +136,8 @@ def test_torus_flux_fixes_ct_divergence_source(oracle_lib, case):
     assert np.all(rel_l1(a, b)[:5] < 1e-12), rel_l1(a, b)
 
 
-@pytest.mark.parametrize("case", ["torus2d", "torus3d", "torus2d_simple", "torus2d_fluid", "torus2d_normal"])
+@pytest.mark.parametrize("case", ["torus2d", "torus3d", "torus2d_simple", "torus2d_fluid", "torus2d_normal",
+                                  "torus2d_kastaun"])
 def test_torus_utop_floors_ptou_bounds(oracle_lib, case):
     import kharma_amd as kh
     cfg, _, P, U, _ = _evolved(oracle_lib, case, 5)
@@ -158,7 +161,17 @@ def test_torus_utop_floors_ptou_bounds(oracle_lib, case):
     blk.apply_floors(Ud, Pd, pflag, fflag)
     Po, Uo = ob.state()
     assert np.all(rel_l1(interior_of(_np(Pd), cfg), interior_of(Po, cfg)) < 1e-12)
-    assert np.array_equal(interior_of(_np(fflag)[None], cfg), interior_of(ob.array(5)[None], cfg))
+    fg = interior_of(_np(fflag)[None], cfg).astype(np.int64)
+    fo = interior_of(ob.array(5)[None], cfg).astype(np.int64)
+    diff = fg ^ fo
+    print(f"[torus floors] {case}: fflag bits differing {np.unique(diff)} in {(diff != 0).sum()} zones", flush=True)
+    if cfg.get("inverter") == "kastaun":
+        # Kastaun's velocity ceiling (INVERTER_GAMMA, 65536) compares mu^2 rbar^2 with
+        # its analytic bound rsq / (1 + rsq), which it equals at the solution in cold,
+        # unmagnetised zones: the bit there is decided by rounding, P is not
+        assert np.all((diff & ~65536) == 0) and (diff != 0).mean() < 0.01
+    else:
+        assert np.array_equal(fg, fo)
     print(f"[torus ptou + bounds] {case}", flush=True)
     ob.ptou()
     blk.ptou(Pd, Ud)
@@ -179,7 +192,8 @@ def test_torus_utop_floors_ptou_bounds(oracle_lib, case):
         assert np.array_equal(Ug[B1:], Uo[B1:]) and np.array_equal(Pg[B1:], Po[B1:])
 
 
-@pytest.mark.parametrize("case", ["torus2d", "torus3d", "torus2d_simple", "torus2d_fluid", "torus2d_normal"])
+@pytest.mark.parametrize("case", ["torus2d", "torus3d", "torus2d_simple", "torus2d_fluid", "torus2d_normal",
+                                  "torus2d_kastaun"])
 def test_torus_step_parity_fp64_per_step(oracle_lib, case):
     import kharma_amd as kh
     cfg = CASES[case]()

@@ -131,6 +131,69 @@ def test_onedw_round_trip(oracle_lib):
     assert np.array_equal(Pr[5:], U[5:])  # gdet = 1
 
 
+@pytest.mark.parametrize("metric", ["minkowski", "ks"])
+def test_kastaun_round_trip(oracle_lib, metric):
+    """Inverter::u_to_p<kastaun> (inverter/kastaun.hpp:83-400): u_to_p(p_to_u(P)) = P
+    to the solver's tolerance (1e-12 on mu) from no guess at all, in Minkowski and on
+    the Kerr-Schild metric of the torus block (the normal-frame transform and the
+    spatial inverse metric g^ij - g^0i g^0j / g^00 enter there), magnetised states."""
+    if metric == "minkowski":
+        cfg = pr.mhdmode_config(n=8, ndim=3, inverter="kastaun")
+    else:
+        cfg = dict(pr.torus_config(16, 12, 1), inverter="kastaun", err_tol=1e-12, iter_max=25)
+    rng = np.random.default_rng(9)
+    if metric == "ks":
+        # the torus's own state: random native-coordinate values are unphysical at
+        # large radius in log-r coordinates, where the inverter's ceilings would act
+        P = pr.torus_prims(cfg)
+    else:
+        P = _random_prims(cfg, rng)
+    b = _block(oracle_lib, cfg)
+    b.set_state(P, np.zeros_like(P))
+    b.ptou()
+    _, U = b.state()
+    b.set_state(np.zeros_like(P), U)
+    b.utop()
+    Pr, _ = b.state()
+    sl = (slice(None),) + pr.interior(cfg)
+    assert np.all(b.array(4)[sl[1:]] == 0) and np.all(b.array(5)[sl[1:]] == 0)
+    # rho and u relative, velocities absolute: in the torus's magnetised, nearly
+    # cold atmosphere P -> U -> P itself loses ~1e-9 of u to roundoff (onedw from
+    # the exact answer loses the same), whatever the solver
+    rel = np.abs(Pr[:2][sl] - P[:2][sl]) / np.abs(P[:2][sl])
+    vel = np.abs(Pr[U1:U3 + 1][sl] - P[U1:U3 + 1][sl])
+    assert rel.max() < (1e-12 if metric == "minkowski" else 1e-8), rel.max()
+    assert vel.max() < (1e-12 if metric == "minkowski" else 1e-9), vel.max()
+    # the same solution as onedw converged to 1e-14 from the exact guess
+    o = _block(oracle_lib, dict(cfg, inverter="onedw", err_tol=1e-14, iter_max=30))
+    o.set_state(P, U)
+    o.utop()
+    Po, _ = o.state()
+    assert np.abs(Pr[:2][sl] - Po[:2][sl]).max() <= 1e-12 * np.abs(Po[:2][sl]).max()
+    assert np.abs(Pr[U1:U3 + 1][sl] - Po[U1:U3 + 1][sl]).max() < 1e-11
+
+
+def test_kastaun_inverter_density_floor(oracle_lib):
+    """The density floor inside Kastaun's master function (kastaun.hpp:126-136):
+    a zone whose D / W falls below rho_min_const is recovered at the floor and
+    flagged INVERTER_RHO (16384) in fflag, with pflag 0."""
+    cfg = pr.mhdmode_config(n=8, ndim=3, inverter="kastaun", floor_values=dict(rho_min_const=1e-3))
+    rng = np.random.default_rng(4)
+    P = _random_prims(cfg, rng)
+    sl = pr.interior(cfg)
+    z = tuple(s_.start for s_ in sl)
+    P[(RHO,) + z] = 1e-5
+    b = _block(oracle_lib, cfg)
+    b.set_state(P, np.zeros_like(P))
+    b.ptou()
+    _, U = b.state()
+    b.set_state(np.zeros_like(P), U)
+    b.utop()
+    Pr, _ = b.state()
+    assert b.array(4)[z] == 0 and int(b.array(5)[z]) & 16384
+    assert Pr[(RHO,) + z] == 1e-3
+
+
 def test_onedw_failure_codes(oracle_lib):
     cfg = pr.mhdmode_config(n=8, ndim=3)
     P = pr.mhdmode_prims(cfg)
