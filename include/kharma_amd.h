@@ -107,6 +107,12 @@ typedef struct kh_params {
     int kill_on_large_divb;         /* b_field/kill_on_large_divb (the reference's default: on) */
     double kill_on_divb_over;       /* b_field/kill_on_divb_over (default 1e-3) */
     int inverter;                   /* KH_INVERTER_* (inverter/type) */
+    /* First-order flux correction (fofc/on, polar_cells, eh_buffer; flux/flux.cpp:175-215)
+     * with the floors' own prescription, LLF speeds from both states and no source term
+     * in the trial update.  FP64 form and the default driver only. */
+    int fofc;
+    int fofc_polar_cells;
+    double fofc_eh_buffer;
 } kh_params;
 
 typedef struct kh_block kh_block;

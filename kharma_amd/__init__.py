@@ -60,6 +60,7 @@ class KhParams(ctypes.Structure):
         ("driver_type", ctypes.c_int),
         ("kill_on_large_divb", ctypes.c_int), ("kill_on_divb_over", ctypes.c_double),
         ("inverter", ctypes.c_int),
+        ("fofc", ctypes.c_int), ("fofc_polar_cells", ctypes.c_int), ("fofc_eh_buffer", ctypes.c_double),
     ]
 
 
@@ -192,6 +193,9 @@ def make_params(cfg):
     p.kill_on_large_divb = int(cfg.get("kill_on_large_divb", True))
     p.kill_on_divb_over = cfg.get("kill_on_divb_over", 1e-3)
     p.inverter = INVERTER[cfg.get("inverter", "onedw")]
+    p.fofc = int(cfg.get("fofc", False))
+    p.fofc_polar_cells = cfg.get("fofc_polar_cells", 0)
+    p.fofc_eh_buffer = cfg.get("fofc_eh_buffer", 0.1)
     p.gam = cfg["gamma"]
     p.cfl = cfg.get("cfl", 0.9)
     p.dt_min = cfg.get("dt_min", 1e-5)

@@ -77,6 +77,11 @@ int kh::check_params(const kh_params* p) {
         return fail(KH_ERR_INVALID, "unknown driver type");
     if (p->inverter != KH_INVERTER_ONEDW && p->inverter != KH_INVERTER_KASTAUN)
         return fail(KH_ERR_INVALID, "unknown inverter type");
+    if (p->fofc) {
+        if (p->precision != KH_PREC_FP64) return fail(KH_ERR_UNSUPPORTED, "fofc runs in the FP64 form only");
+        if (p->driver_type != KH_DRIVER_KHARMA) return fail(KH_ERR_UNSUPPORTED, "fofc is a step of the default (kharma) driver");
+        if (p->fofc_polar_cells < 0 || !(p->fofc_eh_buffer >= 0.)) return fail(KH_ERR_INVALID, "fofc polar_cells and eh_buffer must be >= 0");
+    }
     if (p->kill_on_large_divb && !(p->kill_on_divb_over > 0.))
         return fail(KH_ERR_INVALID, "kill_on_divb_over must be positive when kill_on_large_divb is set");
     if (p->floor_frame < 0 || p->floor_frame > 2) return fail(KH_ERR_INVALID, "unknown floor frame");
@@ -376,6 +381,33 @@ int kh_driver_create(const kh_params* p, kh_driver** out) {
     if ((e = hipMalloc(&d->fail_list, sizeof(int) * nc)) != hipSuccess) return cleanup(e, "hipMalloc fail_list");
     if ((e = hipMalloc(&d->ds, sizeof(DeviceState))) != hipSuccess) return cleanup(e, "hipMalloc state");
     if ((e = hipMalloc(&d->dt_hist, sizeof(double) * d->hist_cap)) != hipSuccess) return cleanup(e, "hipMalloc dt_hist");
+    if (blk->p.fofc) {
+        // MarkFOFC's zones marked whatever the trial state (flux/fofc.cpp:61-107):
+        // cell centres within eh_buffer of the horizon, and the first polar_cells
+        // rows at physical X2 faces, in spherical coordinates.
+        const BlockDims& bd = blk->dims;
+        const long np = (long)bd.n1 * bd.n2;
+        std::vector<int> fixed(np, 0);
+        const kh_params& pp = blk->p;
+        if (pp.coords == KH_COORDS_KS) {
+            const double r_eh = 1. + std::sqrt(1. - pp.a * pp.a);
+            for (int j = 0; j < bd.n2; ++j)
+                for (int i = 0; i < bd.n1; ++i) {
+                    double X[4];
+                    locus_coords(pp, bd, GLOC_CENTER, j, i, X);
+                    bool m = embed_r(pp, X) < r_eh + pp.fofc_eh_buffer;
+                    if (bd.ndim > 1 && !bd.periodic[1] && pp.fofc_polar_cells > 0 &&
+                        ((j >= bd.js && j < bd.js + pp.fofc_polar_cells) || (j <= bd.je && j > bd.je - pp.fofc_polar_cells)))
+                        m = true;
+                    fixed[(size_t)j * bd.n1 + i] = m;
+                }
+        }
+        if ((e = hipMalloc(&d->fofc_fixed, sizeof(int) * np)) != hipSuccess) return cleanup(e, "hipMalloc fofc_fixed");
+        if ((e = hipMemcpy(d->fofc_fixed, fixed.data(), sizeof(int) * np, hipMemcpyHostToDevice)) != hipSuccess)
+            return cleanup(e, "hipMemcpy fofc_fixed");
+        if ((e = hipMalloc(&d->fofc_mark, sizeof(int) * nc)) != hipSuccess) return cleanup(e, "hipMalloc fofc_mark");
+        if ((e = hipMemset(d->fofc_mark, 0, sizeof(int) * nc)) != hipSuccess) return cleanup(e, "hipMemset");
+    }
     // Zero everything once at creation (outside any captured work): the signal
     // speeds of an inactive direction must read 0, as the reference's zero-initialised
     // Flux.cmax/cmin do for X3 in 2D.
@@ -400,7 +432,7 @@ void kh_driver_destroy(kh_driver* d) {
     for (hipEvent_t e : d->ev_pool) (void)hipEventDestroy(e);
     for (void* ptr : {(void*)d->U0, (void*)d->P0, (void*)d->U1, (void*)d->P1, (void*)d->P0f, (void*)d->P1f, d->F, d->ctop,
                       (void*)d->pflag, (void*)d->fflag, (void*)d->fail_list, (void*)d->ds, (void*)d->dt_hist,
-                      (void*)d->dt_part, (void*)d->divb_part, (void*)d->obj})
+                      (void*)d->dt_part, (void*)d->divb_part, (void*)d->obj, (void*)d->fofc_fixed, (void*)d->fofc_mark})
         if (ptr) (void)hipFree(ptr);
     kh_block_destroy(d->blk);
     delete d;
@@ -489,6 +521,12 @@ int kh::driver_stage(kh_driver* d, int stage) {
     for (int dir = 1; dir <= bd.ndim; ++dir)
         KH_TIMED(dir - 1, launch_flux_stage(bd, c, b->geo, dir, b->p.recon, prec, speeds, Psub, Psubf,
                                             F + fe * (dir - 1) * NV * bd.nc, ct + fe * (dir - 1) * bd.nc, s));
+    if (b->p.fofc) {
+        // KHARMADriver::AddFOFC (kharma_step.cpp:168-173): on the raw fluxes, before the flux fixes
+        KH_TIMED(3, launch_fofc_mark(bd, c, b->geo, stage, (const double*)d->F, Usub, d->U0, Psub, d->fofc_fixed,
+                                     d->ds, d->fofc_mark, s));
+        KH_TIMED(3, launch_fofc_flux(bd, c, b->geo, speeds, Psub, d->fofc_mark, (double*)d->F, (double*)d->ctop, s));
+    }
     if (physical) KH_TIMED(3, launch_fix_flux(bd, prec, d->F, s));
     if (d->obj_dir)
         KH_HIP(launch_face_flux_objective(bd, prec, d->F, d->obj_dir, d->obj_var, d->obj_face,

@@ -257,4 +257,81 @@ hipError_t launch_flux_stage(const BlockDims& b, const StepConsts& c, const Geom
     return launch_prec<double, double>(b, c, gd, dir, recon, speeds, P, (double*)F, (double*)ctop, nullptr, s);
 }
 
+// ---------------------------------------------- first-order flux correction
+// Flux::FOFC (flux/fofc.cpp:113-226) with its defaults (LLF, speeds from both
+// states, cell-centred B): each face of the consumed ranges next to a marked
+// zone gets the LLF flux of the two zones' own primitives, and its speeds.
+// `mark` holds the interior marks; a ghost zone reads the mark of its periodic
+// image or, across a physical face, of the nearest interior zone -- what the
+// outflow and reflecting fills of the synced flag give for the one ghost layer
+// a consumed face touches.
+__device__ __forceinline__ int fofc_source(int v, int s, int e, int n, int periodic) {
+    if (v >= s && v <= e) return v;
+    if (periodic) return v < s ? v + n : v - n;
+    return v < s ? s : e;
+}
+template <int DIR, int SPEEDS, int GEO>
+__global__ void __launch_bounds__(256)
+fofc_flux_kernel(BlockDims b, StepConsts c, GeomDev gd, const double* __restrict__ P, const int* __restrict__ mark,
+                 double* __restrict__ F, double* __restrict__ ctop) {
+    const int m1 = (DIR == 1) ? b.nx1 + 1 : b.nx1 + 2;
+    const int m2 = (b.ndim < 2) ? 1 : (DIR == 2) ? b.nx2 + 1 : b.nx2 + 2;
+    const int m3 = (b.ndim < 3) ? 1 : (DIR == 3) ? b.nx3 + 1 : b.nx3 + 2;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)m1 * m2 * m3) return;
+    const int i = ((DIR == 1) ? b.is : b.is - 1) + (int)(t % m1);
+    const long r = t / m1;
+    const int j = ((b.ndim < 2 || DIR == 2) ? b.js : b.js - 1) + (int)(r % m2);
+    const int k = ((b.ndim < 3 || DIR == 3) ? b.ks : b.ks - 1) + (int)(r / m2);
+    auto marked = [&](int kk, int jj, int ii) {
+        ii = fofc_source(ii, b.is, b.ie, b.nx1, b.periodic[0]);
+        if (b.ndim > 1) jj = fofc_source(jj, b.js, b.je, b.nx2, b.periodic[1]);
+        if (b.ndim > 2) kk = fofc_source(kk, b.ks, b.ke, b.nx3, b.periodic[2]);
+        return mark[(long)kk * b.s3 + (long)jj * b.s2 + ii] != 0;
+    };
+    if (!marked(k, j, i) && !marked(DIR == 3 ? k - 1 : k, DIR == 2 ? j - 1 : j, DIR == 1 ? i - 1 : i)) return;
+    const long ci = (long)k * b.s3 + (long)j * b.s2 + i;
+    const long st = (DIR == 1) ? 1 : (DIR == 2) ? b.s2 : b.s3;
+    double Pl[NV], Pr[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        Pl[v] = P[v * b.nc + ci - st];
+        Pr[v] = P[v * b.nc + ci];
+    }
+    face_flux<DIR, SPEEDS, double, GEO>(c, b, gd, j, i, Pl, Pr, ci, F, ctop, nullptr);
+}
+
+template <int DIR, int SPEEDS>
+static hipError_t launch_fofc_dir(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const double* P,
+                                  const int* mark, double* F, double* ctop, hipStream_t s) {
+    const long m1 = (DIR == 1) ? b.nx1 + 1 : b.nx1 + 2;
+    const long m2 = (b.ndim < 2) ? 1 : (DIR == 2) ? b.nx2 + 1 : b.nx2 + 2;
+    const long m3 = (b.ndim < 3) ? 1 : (DIR == 3) ? b.nx3 + 1 : b.nx3 + 2;
+    const dim3 grid((unsigned)((m1 * m2 * m3 + 255) / 256));
+    if (b.geo) fofc_flux_kernel<DIR, SPEEDS, 1><<<grid, 256, 0, s>>>(b, c, gd, P, mark, F, ctop);
+    else fofc_flux_kernel<DIR, SPEEDS, 0><<<grid, 256, 0, s>>>(b, c, gd, P, mark, F, ctop);
+    return hipGetLastError();
+}
+
+hipError_t launch_fofc_flux(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int speeds, const double* P,
+                            const int* mark, double* F, double* ctop, hipStream_t s) {
+    StepConsts cl = c;   // first-order states: no reconstruction floors; LLF whatever flux/type says
+    cl.recon_floors = 0;
+    cl.use_hlle = 0;
+    for (int dir = 1; dir <= b.ndim; ++dir) {
+        double* Fd = F + (long)(dir - 1) * NV * b.nc;
+        double* cd = ctop + (long)(dir - 1) * b.nc;
+        hipError_t e;
+        const bool sp = speeds == SPEEDS_CTOP;
+        if (dir == 1) e = sp ? launch_fofc_dir<1, SPEEDS_CTOP>(b, cl, gd, P, mark, Fd, cd, s)
+                             : launch_fofc_dir<1, SPEEDS_NONE>(b, cl, gd, P, mark, Fd, cd, s);
+        else if (dir == 2) e = sp ? launch_fofc_dir<2, SPEEDS_CTOP>(b, cl, gd, P, mark, Fd, cd, s)
+                                  : launch_fofc_dir<2, SPEEDS_NONE>(b, cl, gd, P, mark, Fd, cd, s);
+        else e = sp ? launch_fofc_dir<3, SPEEDS_CTOP>(b, cl, gd, P, mark, Fd, cd, s)
+                    : launch_fofc_dir<3, SPEEDS_NONE>(b, cl, gd, P, mark, Fd, cd, s);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
 }  // namespace kh

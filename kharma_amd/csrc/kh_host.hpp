@@ -33,6 +33,8 @@ struct kh_driver {
     double* obj = nullptr;   // device accumulator
     // step record (kh_driver_set_stage_record): caller's device buffer, or null
     char* rec = nullptr;
+    // first-order flux correction (kh_params.fofc): fixed marks [n2][n1], marks [nc]
+    int *fofc_fixed = nullptr, *fofc_mark = nullptr;
     // kernel timing (kh_driver_set_kernel_timing)
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;

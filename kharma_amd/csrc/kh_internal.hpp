@@ -122,6 +122,13 @@ hipError_t launch_geo_source(const BlockDims& b, const StepConsts& c, const Geom
 // Fused stage update (flux-CT EMFs, divergence, RK update, U->P, floors, P->U,
 // ghost images).  ctop: [3][nc] max signal speeds, read when do_dt (the last
 // stage), which then writes one CFL minimum per block into dt_part.
+// First-order flux correction (kh_step.hip / kh_flux.hip).  mark[nc]: interior
+// marks; fixed[np] (or null): zones marked whatever the trial state.
+hipError_t launch_fofc_mark(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int stage, const double* F,
+                            const double* Usub, const double* Ufull, const double* Psub, const int* fixed,
+                            const DeviceState* ds, int* mark, hipStream_t s);
+hipError_t launch_fofc_flux(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int speeds, const double* P,
+                            const int* mark, double* F, double* ctop, hipStream_t s);
 hipError_t launch_update(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int prec, int stage, bool do_dt,
                          const void* F, const void* ctop, const double* Usub, const double* Ufull, const double* Psub,
                          double* Uout, double* Pout, float* Pout32, int* pflag, int* fflag,

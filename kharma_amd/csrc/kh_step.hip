@@ -1159,6 +1159,70 @@ hipError_t launch_flux_divergence(const BlockDims& b, int prec, const void* F, d
     else hipLaunchKernelGGL(divergence_kernel<double>, grid_for(n), dim3(BS), 0, s, b, (const double*)F, dUdt);
     return hipGetLastError();
 }
+// ---------------------------------------------- first-order flux correction
+// Flux::MarkFOFC (flux/fofc.cpp:45-111) on the trial state of
+// KHARMADriver::AddFOFC (driver/kharma_driver.cpp:292-337): U_trial = gam0 U_sub
+// + gam1 U_full + beta dt (-div F) from the stage's raw fluxes (no source term),
+// its U->P from the stage's primitives with B = U.B / sqrt(-g), and
+// DetermineGRMHDFloors.  A zone is marked when the inverter's floor bits or the
+// floors' flags are set, or when `fixed` marks it (within the horizon buffer,
+// polar rows).  Interior zones only.
+template <int GEO, int INV>
+__global__ void __launch_bounds__(BS)
+fofc_mark_kernel(BlockDims b, StepConsts cst, GeomDev gd, int stage, const double* __restrict__ F,
+                 const double* __restrict__ Usub, const double* Ufull, const double* __restrict__ Psub,
+                 const int* __restrict__ fixed, const DeviceState* __restrict__ ds, int* __restrict__ mark) {
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (unsigned)ninterior(b)) return;
+    int k, j, i;
+    interior_index(b, t, k, j, i);
+    const long c = (long)k * b.s3 + (long)j * b.s2 + i;
+    double dudt[NV];
+    zone_divergence<double>(b, F, nullptr, c, false, dudt);
+    const double g0 = cst.gam0[stage - 1], g1 = cst.gam1[stage - 1];
+    const double beta_dt = cst.beta[stage - 1] * ds->dt;
+    const bool same = (g0 == 0.) && (Usub == Ufull);
+    double U[NV], P[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        const double uf = Ufull[v * b.nc + c];
+        const double u = same ? uf : g0 * Usub[v * b.nc + c] + g1 * uf;
+        U[v] = 1.0 * u + beta_dt * dudt[v];
+    }
+#pragma unroll
+    for (int v = 0; v < 5; ++v) P[v] = Psub[v * b.nc + c];
+    double rf, uf;
+    geom_floors<GEO>(cst, gd, j, i, rf, uf);
+    int ff = 0;
+    with_center_geom<GEO>(gd, j, i, [&](const auto& g) {
+        const int pf = invert_zone<INV>(g, U, P, cst);
+        ff = (pf / F_GEOM_RHO) * F_GEOM_RHO;
+        const double igd = rcp(g.gdet());
+        P[B1] = U[B1] * igd; P[B2] = U[B2] * igd; P[B3] = U[B3] * igd;
+        if (cst.floors.on) {
+            FourVec<double> D;
+            calc_4vecs<double>(g, P, D);
+            double rfm, ufm;
+            ff |= determine_floors(g, P, dot4(D.bcon, D.bcov), cst.gam, cst.floors, rf, uf, rfm, ufm);
+        }
+    });
+    mark[c] = (ff != 0 || (fixed && fixed[(long)j * gd.gn1 + i])) ? 1 : 0;
+}
+
+hipError_t launch_fofc_mark(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int stage, const double* F,
+                            const double* Usub, const double* Ufull, const double* Psub, const int* fixed,
+                            const DeviceState* ds, int* mark, hipStream_t s) {
+    const dim3 grid = grid_for(ninterior_host(b));
+    if (b.geo) {
+        if (c.inverter == 1) fofc_mark_kernel<1, 1><<<grid, BS, 0, s>>>(b, c, gd, stage, F, Usub, Ufull, Psub, fixed, ds, mark);
+        else fofc_mark_kernel<1, 0><<<grid, BS, 0, s>>>(b, c, gd, stage, F, Usub, Ufull, Psub, fixed, ds, mark);
+    } else {
+        if (c.inverter == 1) fofc_mark_kernel<0, 1><<<grid, BS, 0, s>>>(b, c, gd, stage, F, Usub, Ufull, Psub, fixed, ds, mark);
+        else fofc_mark_kernel<0, 0><<<grid, BS, 0, s>>>(b, c, gd, stage, F, Usub, Ufull, Psub, fixed, ds, mark);
+    }
+    return hipGetLastError();
+}
+
 template <typename R>
 static void launch_update_geo(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int stage, bool do_dt,
                               const void* F, const void* ctop, const double* Usub, const double* Ufull,

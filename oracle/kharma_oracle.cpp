@@ -157,6 +157,8 @@ struct or_params {
     int periodic1, periodic2, periodic3;
     int driver;                        // driver/type: 0 kharma, 1 simple
     int inverter;                      // inverter/type: 0 onedw, 1 kastaun
+    int fofc, fofc_polar_cells;        // fofc/on, fofc/polar_cells
+    double fofc_eh_buffer;             // fofc/eh_buffer
 };
 }
 
@@ -1487,6 +1489,97 @@ void integrator_weights(int integrator, int stage, Real& gam0, Real& gam1, Real&
     }
 }
 
+// First-order flux correction (KHARMADriver::AddFOFC, driver/kharma_driver.cpp:292-337;
+// Flux::MarkFOFC and Flux::FOFC, flux/fofc.cpp:45-226) with its defaults (the
+// floors' own prescription, LLF speeds from both states, no source term in the
+// trial).  A trial update of the stage from the raw fluxes (FluxDivergence and
+// AddStateUpdate into the guess, B included), its U->P with the stage's
+// primitives as guess and DetermineGRMHDFloors; zones whose floor or inverter
+// flags are set, zones within eh_buffer of the horizon and the first
+// polar_cells rows at physical X2 faces are marked; the marks are synced (ghost
+// zones take those of their periodic images or, at physical faces, of the
+// nearest interior zone, as the outflow/reflecting fill of a scalar gives); every
+// face next to a marked zone gets the LLF flux of the two zones' own primitives.
+void fofc(Block& b, int stage, const std::vector<Real>& Usub, const std::vector<Real>& Psub) {
+    Real gam0, gam1, beta;
+    integrator_weights(b.p.integrator, stage, gam0, gam1, beta);
+    flux_divergence(b);
+    std::vector<int> mark(b.nc, 0);
+    const Real r_eh = 1. + std::sqrt(1. - b.p.a * b.p.a);
+#pragma omp parallel for collapse(2)
+    for (int k = b.ks; k <= b.ke; ++k)
+    for (int j = b.js; j <= b.je; ++j)
+    for (int i = b.is; i <= b.ie; ++i) {
+        const size_t c = b.id(k, j, i);
+        Real Ug[NV], Pg[NV];
+        for (int v = 0; v < NV; ++v) {
+            const Real u = gam0 * Usub[v * b.nc + c] + gam1 * b.U0[v * b.nc + c];
+            Ug[v] = 1.0 * u + (beta * b.dt) * b.dUdt[v * b.nc + c];
+            Pg[v] = Psub[v * b.nc + c];
+        }
+        const int ret = (b.p.inverter == 1) ? u_to_p_kastaun(b, Ug, Pg, j, i, CENTER, b.p.iter_max, b.p.err_tol)
+                                            : u_to_p_onedw(b, Ug, Pg, j, i, CENTER, b.p.iter_max, b.p.err_tol);
+        for (int v = 0; v < 3; ++v) Pg[B1 + v] = Ug[B1 + v] / b.gdet_(CENTER, j, i);
+        int ff = (ret / FFLAG_MINIMUM) * FFLAG_MINIMUM;
+        if (b.p.floors_on) {
+            Real rf, uf;
+            ff |= determine_floors(b, Pg, j, i, rf, uf);
+        }
+        bool m = ff != 0;
+        if (b.p.spherical && b.rcen[(size_t)j * b.gn1 + i] < r_eh + b.p.fofc_eh_buffer) m = true;
+        if (b.p.spherical && !b.p.periodic2 && b.p.fofc_polar_cells > 0 &&
+            (j < b.js + b.p.fofc_polar_cells || j > b.je - b.p.fofc_polar_cells)) m = true;
+        mark[c] = m;
+    }
+    auto src = [&](int v, int s0, int e0, int n, bool periodic) {
+        if (v >= s0 && v <= e0) return v;
+        if (periodic) { int r = (v - s0) % n; if (r < 0) r += n; return s0 + r; }
+        return v < s0 ? s0 : e0;
+    };
+    auto marked = [&](int k, int j, int i) {
+        const int ii = src(i, b.is, b.ie, b.p.nx1, b.p.periodic1);
+        const int jj = b.ndim > 1 ? src(j, b.js, b.je, b.p.nx2, b.p.periodic2) : j;
+        const int kk = b.ndim > 2 ? src(k, b.ks, b.ke, b.p.nx3, b.p.periodic3) : k;
+        return mark[b.id(kk, jj, ii)] != 0;
+    };
+    for (int dir = 1; dir <= b.ndim; ++dir) {
+        const int loc = dir - 1;
+        // the faces the step consumes (X1: i in [is, ie+1], transverse cells +-1)
+        int is = b.is, ie = b.ie, js = b.js, je = b.je, ks = b.ks, ke = b.ke;
+        if (dir == 1) ie += 1; else { is -= 1; ie += 1; }
+        if (b.ndim > 1) { if (dir == 2) je += 1; else { js -= 1; je += 1; } }
+        if (b.ndim > 2) { if (dir == 3) ke += 1; else { ks -= 1; ke += 1; } }
+        const long st = (dir == 1) ? 1 : (dir == 2) ? (long)b.n1 : (long)b.n1 * b.n2;
+#pragma omp parallel for collapse(2)
+        for (int k = ks; k <= ke; ++k)
+        for (int j = js; j <= je; ++j)
+        for (int i = is; i <= ie; ++i) {
+            const int km = dir == 3 ? k - 1 : k, jm = dir == 2 ? j - 1 : j, im = dir == 1 ? i - 1 : i;
+            if (!marked(k, j, i) && !marked(km, jm, im)) continue;
+            const size_t c = b.id(k, j, i);
+            Real Pl[NV], Pr[NV], Ul[NV], Ur[NV], Fl[NV], Fr[NV];
+            for (int v = 0; v < NV; ++v) { Pl[v] = Psub[v * b.nc + c - st]; Pr[v] = Psub[v * b.nc + c]; }
+            FourVectors D;
+            calc_4vecs(b, Pl, j, i, loc, D);
+            prim_to_flux(b, Pl, D, j, i, 0, Ul, loc);
+            prim_to_flux(b, Pl, D, j, i, dir, Fl, loc);
+            Real cmaxL, cminL;
+            vchar(b, Pl, D, j, i, loc, dir, cmaxL, cminL);
+            calc_4vecs(b, Pr, j, i, loc, D);
+            prim_to_flux(b, Pr, D, j, i, 0, Ur, loc);
+            prim_to_flux(b, Pr, D, j, i, dir, Fr, loc);
+            Real cmaxR, cminR;
+            vchar(b, Pr, D, j, i, loc, dir, cmaxR, cminR);
+            const Real cmx = std::max(std::max(Real(0.), cmaxL), cmaxR);
+            const Real cmn = -std::min(std::min(Real(0.), cminL), cminR);
+            b.cmax[(dir - 1) * b.nc + c] = cmx;
+            b.cmin[(dir - 1) * b.nc + c] = cmn;
+            for (int v = 0; v < NV; ++v)
+                b.F[((size_t)(dir - 1) * NV + v) * b.nc + c] = llf(Fl[v], Fr[v], cmx, cmn, Ul[v], Ur[v]);
+        }
+    }
+}
+
 // One stage of KHARMADriver::MakeDefaultTaskCollection, driver/kharma_step.cpp:77-319.
 // Stage containers follow parthenon's MultiStageDriver naming: stage 1 is
 // base -> "1", stage 2 is "1" -> base.
@@ -1497,6 +1590,8 @@ void run_stage(Block& b, int stage) {
     std::vector<Real>& Pout = (stage == 1) ? b.P1c : b.P0;
     // Flux region (kharma_step.cpp:141-210)
     for (int dir = 1; dir <= 3; ++dir) get_flux(b, Psub, dir);
+    // AddFOFC on the raw fluxes (kharma_step.cpp:166-172), before the flux fixes
+    if (b.p.fofc) fofc(b, stage, Usub, Psub);
     // Packages::FixFlux: B_FluxCT (ZeroBoundaryFlux, FluxCT), then KBoundaries
     zero_polar_b_flux(b);
     flux_ct(b);

@@ -48,6 +48,7 @@ class OrParams(ctypes.Structure):
         ("zero_polar_flux", ctypes.c_int), ("fix_polar_flux", ctypes.c_int), ("fix_corner_inner", ctypes.c_int),
         ("periodic1", ctypes.c_int), ("periodic2", ctypes.c_int), ("periodic3", ctypes.c_int),
         ("driver", ctypes.c_int), ("inverter", ctypes.c_int),
+        ("fofc", ctypes.c_int), ("fofc_polar_cells", ctypes.c_int), ("fofc_eh_buffer", ctypes.c_double),
     ]
 
 
@@ -178,6 +179,9 @@ def make_params(cfg):
     p.fix_corner_inner = int(cfg.get("fix_corner_inner", False))
     p.driver = DRIVER[cfg.get("driver", "kharma")]
     p.inverter = {"onedw": 0, "kastaun": 1}[cfg.get("inverter", "onedw")]
+    p.fofc = int(cfg.get("fofc", False))
+    p.fofc_polar_cells = cfg.get("fofc_polar_cells", 0)
+    p.fofc_eh_buffer = cfg.get("fofc_eh_buffer", 0.1)
     return p
 
 

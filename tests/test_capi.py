@@ -51,6 +51,9 @@ def _create(cfg, **over):
     (dict(driver_type=5), 1),        # unknown driver/type
     (dict(inverter=2), 1),           # unknown inverter/type
     (dict(kill_on_large_divb=1, kill_on_divb_over=0.0), 1),   # divB stop without a threshold
+    (dict(fofc=1, driver_type=1), 3),                          # FOFC belongs to the default driver
+    (dict(fofc=1, driver_type=0, precision=1), 3),             # FOFC in the FP64 form only
+    (dict(fofc=1, driver_type=0, fofc_polar_cells=-1), 1),
 ])
 def test_invalid_params_rejected_on_host(L, over, code):
     st, msg = _create(pr.mhdmode_config(n=16, ndim=3), **over)

@@ -47,6 +47,10 @@ CASES = {
     "torus2d_normal": lambda: dict(pr.torus_config(64, 48, 1), floor_frame="normal"),
     # inverter/type = kastaun (inverter/kastaun.hpp:83-400), the reference's default, with its defaults
     "torus2d_kastaun": lambda: dict(pr.torus_config(64, 48, 1), inverter="kastaun", err_tol=1e-12, iter_max=25),
+    # fofc/on (flux/fofc.cpp): marks from the trial state's floors, the horizon buffer and two polar rows
+    "torus2d_fofc": lambda: dict(pr.torus_config(64, 48, 1), fofc=True, fofc_polar_cells=2),
+    # every zone marked (a horizon buffer past the outer radius): the donor-cell scheme
+    "torus2d_fofc_all": lambda: dict(pr.torus_config(64, 48, 1), fofc=True, fofc_eh_buffer=1e6),
 }
 
 
@@ -193,7 +197,7 @@ def test_torus_utop_floors_ptou_bounds(oracle_lib, case):
 
 
 @pytest.mark.parametrize("case", ["torus2d", "torus3d", "torus2d_simple", "torus2d_fluid", "torus2d_normal",
-                                  "torus2d_kastaun"])
+                                  "torus2d_kastaun", "torus2d_fofc", "torus2d_fofc_all"])
 def test_torus_step_parity_fp64_per_step(oracle_lib, case):
     import kharma_amd as kh
     cfg = CASES[case]()
@@ -331,3 +335,22 @@ def test_torus_full_size_one_step(oracle_lib):
     # field of EMF differences: the field error is measured against |B| as a whole
     rb = np.abs(Ps[5:] - Pd[5:]).sum(axis=(1, 2, 3)) / np.abs(Pd[5:]).sum()
     assert np.all(rb < 1e-6), rb
+
+
+def test_fofc_every_zone_marked_is_the_donor_cell_scheme_on_gpu():
+    """With every zone marked the corrected step is the donor-cell LLF step without
+    reconstruction floors (the oracle shows this bit for bit); on the GPU the two
+    runs take different kernels to the same face arithmetic."""
+    import kharma_amd as kh
+    base = pr.torus_config(64, 48, 1)
+    P0 = pr.torus_prims(base)
+    out = []
+    for kw in (dict(fofc=True, fofc_eh_buffer=1e6), dict(recon="donor_cell_c", recon_floors=False, riemann="llf")):
+        print(f"[fofc all-marked] {kw}", flush=True)
+        drv = kh.Driver(dict(base, **kw))
+        drv.init_prims(_cuda(P0))
+        drv.step(3)
+        out.append([_np(x) for x in drv.state()])
+    for a, d in zip(*out):
+        r = rel_l1(interior_of(a, base), interior_of(d, base))
+        assert np.all(r < 1e-14), r

@@ -498,3 +498,65 @@ def test_tangent_linear_matches_central_differences(oracle_lib):
     errs = [rel(central(3, h)[0], dP) for h in (1e-4, 1e-5, 1e-6)]
     assert errs[1] < 0.05 * errs[0] and errs[2] < 0.1 * errs[1], errs
     assert errs[2] < 1e-6, errs
+
+
+def test_fofc_every_zone_marked_is_the_donor_cell_scheme(oracle_lib):
+    """First-order flux correction (flux/fofc.cpp:113-226) with every zone marked
+    (an eh_buffer beyond the outer radius) replaces every consumed face flux by the
+    LLF flux of the two zones' own primitives: the step is then the donor-cell
+    scheme with LLF and no reconstruction floors, bit for bit, whatever the
+    reconstruction the fluxes were first computed with."""
+    cfg = pr.torus_config(48, 32, 1)
+    P0 = pr.torus_prims(cfg)
+    out = {}
+    for name, kw in (("fofc", dict(fofc=True, fofc_eh_buffer=1e6)),
+                     ("donor", dict(recon="donor_cell_c", recon_floors=False, riemann="llf"))):
+        b = _block(oracle_lib, dict(cfg, **kw))
+        b.init_state(P0)
+        b.step(3)
+        out[name] = b.state()
+    for a, d in zip(out["fofc"], out["donor"]):
+        assert np.array_equal(a, d)
+
+
+def test_fofc_without_marks_changes_nothing(oracle_lib):
+    """A smooth periodic flow far above the floors marks no zone, so the
+    correction leaves the step bit for bit as it is."""
+    cfg = pr.mhdmode_config(n=16, mode="fast", ndim=3, amp=1e-2)
+    P0 = pr.mhdmode_prims(cfg)
+    out = []
+    for on in (False, True):
+        b = _block(oracle_lib, dict(cfg, fofc=on))
+        b.init_state(P0)
+        b.step(4)
+        out.append(b.state())
+    for a, c in zip(*out):
+        assert np.array_equal(a, c)
+
+
+def test_fofc_marks_act_locally(oracle_lib):
+    """Marks are OR-ed per zone, so adding the two polar rows at each pole
+    (fofc/polar_cells) changes the step only near the poles, and widening the
+    horizon buffer (fofc/eh_buffer 0.1 -> 0.6) only near the horizon: one step
+    moves a difference at most the flux stencil (three zones for WENO5, one more
+    for the corner EMFs, one for the fix-up's neighbours) from a marked face.
+    Floors are off for the one step, so that the atmosphere's floor hits (which
+    mark nearly every zone outside the torus) leave these marks visible."""
+    cfg = dict(pr.torus_config(48, 32, 1), floors=False)
+    P0 = pr.torus_prims(cfg)
+    ng = cfg["ng"]
+
+    def run(**kw):
+        b = _block(oracle_lib, dict(cfg, fofc=True, **kw))
+        b.init_state(P0)
+        b.step(1)
+        return b.state()[0][:, 0, ng:-ng, ng:-ng]                     # (v, j, i)
+
+    base = run(fofc_eh_buffer=0.1)
+    j, _ = np.nonzero(np.abs(run(fofc_eh_buffer=0.1, fofc_polar_cells=2) - base).max(axis=0))
+    assert j.size > 0 and np.all((j < 2 + 5) | (j >= 32 - 2 - 5))
+    _, i = np.nonzero(np.abs(run(fofc_eh_buffer=0.6) - base).max(axis=0))
+    x1min, x1max = cfg["bounds"][0]
+    r = np.exp(x1min + (i + 0.5) * (x1max - x1min) / 48)
+    r_eh = 1 + math.sqrt(1 - cfg["a"] ** 2)
+    assert i.size > 0 and np.all(r < np.exp(np.log(r_eh + 0.6) + 5 * (x1max - x1min) / 48))
