@@ -384,12 +384,13 @@ int kh_driver_create(const kh_params* p, kh_driver** out) {
     if (blk->p.fofc) {
         // MarkFOFC's zones marked whatever the trial state (flux/fofc.cpp:61-107):
         // cell centres within eh_buffer of the horizon, and the first polar_cells
-        // rows at physical X2 faces, in spherical coordinates.
-        const BlockDims& bd = blk->dims;
-        const long np = (long)bd.n1 * bd.n2;
-        std::vector<int> fixed(np, 0);
+        // rows at physical X2 faces, in spherical coordinates.  Laid out as the
+        // geometry tables, [gn2][gn1] with gn1 = n1 + 1; absent in Cartesian coordinates.
         const kh_params& pp = blk->p;
         if (pp.coords == KH_COORDS_KS) {
+            const BlockDims& bd = blk->dims;
+            const long np = blk->geo.np, gn1 = blk->geo.gn1;
+            std::vector<int> fixed(np, 0);
             const double r_eh = 1. + std::sqrt(1. - pp.a * pp.a);
             for (int j = 0; j < bd.n2; ++j)
                 for (int i = 0; i < bd.n1; ++i) {
@@ -399,12 +400,12 @@ int kh_driver_create(const kh_params* p, kh_driver** out) {
                     if (bd.ndim > 1 && !bd.periodic[1] && pp.fofc_polar_cells > 0 &&
                         ((j >= bd.js && j < bd.js + pp.fofc_polar_cells) || (j <= bd.je && j > bd.je - pp.fofc_polar_cells)))
                         m = true;
-                    fixed[(size_t)j * bd.n1 + i] = m;
+                    fixed[(size_t)j * gn1 + i] = m;
                 }
+            if ((e = hipMalloc(&d->fofc_fixed, sizeof(int) * np)) != hipSuccess) return cleanup(e, "hipMalloc fofc_fixed");
+            if ((e = hipMemcpy(d->fofc_fixed, fixed.data(), sizeof(int) * np, hipMemcpyHostToDevice)) != hipSuccess)
+                return cleanup(e, "hipMemcpy fofc_fixed");
         }
-        if ((e = hipMalloc(&d->fofc_fixed, sizeof(int) * np)) != hipSuccess) return cleanup(e, "hipMalloc fofc_fixed");
-        if ((e = hipMemcpy(d->fofc_fixed, fixed.data(), sizeof(int) * np, hipMemcpyHostToDevice)) != hipSuccess)
-            return cleanup(e, "hipMemcpy fofc_fixed");
         if ((e = hipMalloc(&d->fofc_mark, sizeof(int) * nc)) != hipSuccess) return cleanup(e, "hipMalloc fofc_mark");
         if ((e = hipMemset(d->fofc_mark, 0, sizeof(int) * nc)) != hipSuccess) return cleanup(e, "hipMemset");
     }

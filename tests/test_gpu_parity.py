@@ -242,6 +242,9 @@ VARIANTS = [
     ("ot2d", dict(ot_init="reference")),       # InitializeOrszagTang + seed_a<orszag_tang_a>
     ("ot2d", dict(inverter="kastaun", err_tol=1e-12, iter_max=25)),     # inverter/kastaun.hpp:83-400
     ("fast3d", dict(inverter="kastaun", err_tol=1e-12, iter_max=25)),
+    # fofc/on (flux/fofc.cpp) with a density floor inside the flow's range [0.95, 1.05]:
+    # the trial state's floor hits mark about half of the zones
+    ("fast3d", dict(driver="kharma", fofc=True, floor_values=dict(rho_min_const=1.0, u_min_const=1e-10))),
 ]
 
 
