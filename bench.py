@@ -163,6 +163,16 @@ def main():
         return
     r64 = _bench_form(dict(cfg, precision="fp64"), P0, a.steps, a.warmup)
     r32 = _bench_form(dict(cfg, precision="split"), P0, a.steps, a.warmup)
+    # cost of the first-order flux correction (fofc/on with its defaults) on the same
+    # workload, FP64 form, default driver only
+    fofc = None
+    if cfg.get("driver", "kharma") == "kharma":
+        nf, wf = min(a.steps, 50), min(a.warmup, 10)
+        rf = _bench_form(dict(cfg, precision="fp64", fofc=True), P0, nf, wf)
+        rb = _bench_form(dict(cfg, precision="fp64"), P0, nf, wf)
+        fofc = {"options": "fofc/on, eh_buffer 0.1, polar_cells 0", "steps": nf, "warmup": wf,
+                "ms_per_step": rf["ms"] / nf, "plain_ms_per_step": rb["ms"] / nf,
+                "over_plain": rf["ms"] / rb["ms"]}
     adj = None if a.no_adjoint else _bench_adjoint()
     adj_gr = None if a.no_adjoint else _bench_adjoint_torus()
     zc64 = nz * a.steps / (r64["ms"] * 1e-3)
@@ -230,6 +240,7 @@ def main():
         "split_rel_l1_vs_fp64": {"after_steps": a.steps + a.warmup, "max": float(rel.max()),
                                  "per_var": [float(x) for x in rel],
                                  "one_step_max": float(rel1.max()), "one_step_per_var": [float(x) for x in rel1]},
+        "fofc_fp64": fofc,
         "adjoint_over_forward": adj["adjoint_over_forward"] if adj else None,
         "adjoint": adj,
         "adjoint_torus": adj_gr,
