@@ -116,6 +116,8 @@ def main():
                     help="run only the differentiation-form measurements (Alfven 64^3 and 2-D torus) and print "
                          "their results (profiling runs)")
     ap.add_argument("--forms", default="fp64,split", help="comma list of fp64,split (profiling runs use one)")
+    ap.add_argument("--fofc", action="store_true",
+                    help="profiling mode (--forms fp64): run the step with the first-order flux correction on")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="after the runs, copy a buffer of this many MiB on the device: a known-size "
                          "reference for scaling HBM byte counters (FETCH_SIZE / WRITE_SIZE)")
@@ -155,8 +157,8 @@ def main():
     if forms != ["fp64", "split"]:
         # profiling mode: run the requested forms only, print their timings, no JSON contract line
         for f in forms:
-            r = _bench_form(dict(cfg, precision=f), P0, a.steps, a.warmup)
-            print(json.dumps({"form": f, "ms_per_step": r["ms"] / a.steps,
+            r = _bench_form(dict(cfg, precision=f, fofc=a.fofc), P0, a.steps, a.warmup)
+            print(json.dumps({"form": f, "fofc": a.fofc, "ms_per_step": r["ms"] / a.steps,
                               "zone_cycles_per_s": nz * a.steps / (r["ms"] * 1e-3)}))
         if a.copy_mb:
             _calibration_copy(a.copy_mb)
