@@ -685,6 +685,12 @@ void flux_divergence(Block& b) {
     }
 }
 
+// A deliberate mistake planted in the restatement by a test (or_plant), to show
+// that the test's pin fails on it.  0: none.  1: the geometric source contracts
+// T^mu_nu with Gamma^lam_{nu mu} (upper and first lower index exchanged) instead
+// of Gamma^nu_{lam mu}.
+static int g_plant = 0;
+
 // Flux::AddGeoSource, flux/flux.cpp:395-444.  Skipped in Cartesian Minkowski (:407).
 void add_geo_source(Block& b, const std::vector<Real>& P) {
     if (b.cart_minkowski) return;
@@ -702,7 +708,8 @@ void add_geo_source(Block& b, const std::vector<Real>& P) {
             calc_tensor_mhd(Pc[RHO], Pc[UU], (gam - 1) * Pc[UU], D, mu, Tmu);
             for (int nu = 0; nu < 4; ++nu)
                 for (int lam = 0; lam < 4; ++lam)
-                    new_du[lam] += Tmu[nu] * b.gdet_conn_(j, i, nu, lam, mu);
+                    new_du[lam] += Tmu[nu] * (g_plant == 1 ? b.gdet_conn_(j, i, lam, nu, mu)
+                                                           : b.gdet_conn_(j, i, nu, lam, mu));
         }
         b.dUdt[(size_t)UU * b.nc + c] += new_du[0];
         for (int v = 0; v < 3; ++v) b.dUdt[(size_t)(U1 + v) * b.nc + c] += new_du[1 + v];
@@ -1850,6 +1857,32 @@ void or_get_tangent(void* h, double* dP, double* dU) {
     }
 }
 #endif
+// WeightedSumData pair of AddStateUpdate (state_update's arithmetic) on
+// caller arrays [8][nc], with the block's dUdt; Uout may alias Usub or Ufull.
+void or_weighted_sum(void* h, const double* Usub, const double* Ufull, double gam0, double gam1, double beta_dt,
+                     double* Uout) {
+    Block& b = *static_cast<Block*>(h);
+    const size_t n = (size_t)NV * b.nc;
+    std::vector<Real> s(n), f(n), o(n);
+    load_values(s, Usub, n);
+    load_values(f, Ufull, n);
+    for (size_t c = 0; c < n; ++c) o[c] = gam0 * s[c] + gam1 * f[c];
+    for (size_t c = 0; c < n; ++c) o[c] = 1.0 * o[c] + beta_dt * b.dUdt[c];
+    store_values(o, Uout, n);
+}
+// The periodic exchange of one block with itself on nvar caller arrays [nvar][nc].
+void or_sync_vars(void* h, double* A, int nvar) {
+    Block& b = *static_cast<Block*>(h);
+    const size_t n = (size_t)nvar * b.nc;
+    std::vector<Real> a(n);
+    load_values(a, A, n);
+    for (int v = 0; v < nvar; ++v) sync_periodic_var(b, a, v);
+    store_values(a, A, n);
+}
+
+// Plant a known mistake in the restatement (see g_plant); 0 removes it.
+void or_plant(int code) { g_plant = code; }
+
 int or_num_threads() {
 #ifdef _OPENMP
     return omp_get_max_threads();

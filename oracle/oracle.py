@@ -103,6 +103,9 @@ def lib(variant=""):
         L.or_apply_bcs.argtypes = [vp]
         L.or_set_geometry.argtypes = [vp, i, dp]
         L.or_fix_flux.argtypes = [vp]
+        L.or_plant.argtypes = [i]
+        L.or_weighted_sum.argtypes = [vp, dp, dp, d, d, d, dp]
+        L.or_sync_vars.argtypes = [vp, dp, i]
         if variant == "tangent":
             L.or_set_tangent.argtypes = [vp, dp, dp]
             L.or_get_tangent.argtypes = [vp, dp, dp]
@@ -294,6 +297,20 @@ class OracleBlock:
     def max_divb(self):
         return self.L.or_max_divb(self.h)
 
+    def weighted_sum(self, Usub, Ufull, gam0, gam1, beta_dt):
+        """gam0 Usub + gam1 Ufull + beta_dt dUdt (the block's dUdt) -> new array."""
+        a = np.ascontiguousarray(Usub, dtype=np.float64)
+        b = np.ascontiguousarray(Ufull, dtype=np.float64)
+        out = np.empty_like(a)
+        self.L.or_weighted_sum(self.h, _ptr(a), _ptr(b), gam0, gam1, beta_dt, _ptr(out))
+        return out
+
+    def sync_vars(self, A):
+        """The periodic exchange of the block with itself on A [nvar, n3, n2, n1] -> new array."""
+        out = np.ascontiguousarray(A, dtype=np.float64).copy()
+        self.L.or_sync_vars(self.h, _ptr(out), out.shape[0])
+        return out
+
     def apply_bcs(self):
         self.L.or_apply_bcs(self.h)
 
@@ -331,3 +348,19 @@ def reconstruct(name, x):
 
 def num_threads():
     return lib().or_num_threads()
+
+
+class planted:
+    """Context manager: plant the numbered known mistake in the restatement
+    (or_plant), so a test can show that its pin fails on it."""
+
+    def __init__(self, code, variant=""):
+        self.code, self.variant = code, variant
+
+    def __enter__(self):
+        lib(self.variant).or_plant(self.code)
+        return self
+
+    def __exit__(self, *exc):
+        lib(self.variant).or_plant(0)
+        return False
