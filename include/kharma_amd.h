@@ -154,13 +154,17 @@ int kh_add_geo_source(kh_block* b, const double* P, double* dUdt);
 
 /* Update::WeightedSumData pair of KHARMADriver::AddStateUpdate
  * (driver/kharma_driver.cpp:339-387), entire block:
- * Uout = gam0*Usub + gam1*Ufull; Uout = Uout + beta_dt*dUdt. */
+ * Uout = gam0*Usub + gam1*Ufull; Uout = Uout + beta_dt*dUdt.
+ * Uout may be Usub or Ufull (the reference updates in place, :366-370). */
 int kh_weighted_sum(kh_block* b, const double* Usub, const double* Ufull, const double* dUdt,
                     double gam0, double gam1, double beta_dt, double* Uout);
 
-/* Inverter::BlockUtoP with the onedw solver (inverter/inverter.cpp:131-200,
- * inverter/onedw.hpp:89-233) then B_FluxCT::BlockUtoP (b_flux_ct.cpp:195-217),
- * entire block.  P holds the guess on entry. */
+/* Inverter::BlockUtoP (inverter/inverter.cpp:131-200) with the solver of
+ * kh_params.inverter (inverter/onedw.hpp:89-233 or inverter/kastaun.hpp:83-400)
+ * over the physical range (interior, plus the ghosts of periodic dimensions),
+ * which is what BlockUtoP sweeps whatever domain it is given
+ * (inverter.cpp:162), then B_FluxCT::BlockUtoP (b_flux_ct.cpp:195-217) over
+ * the entire block.  P holds the guess on entry. */
 int kh_utop(kh_block* b, const double* U, double* P, int* pflag, int* fflag);
 
 /* Floors::ApplyGRMHDFloors (floors/floors.cpp:233-250, floors_impl.hpp:45-128), entire block. */
@@ -196,6 +200,71 @@ int kh_estimate_timestep(kh_block* b, const void* cmax, const void* cmin, double
 
 /* B_FluxCT::MaxDivB (b_flux_ct.cpp:566-607).  Synchronises the stream. */
 int kh_max_divb(kh_block* b, const double* U, double* out);
+
+/* ---- Pack view: the variables where KHARMA keeps them ---------------------
+ * KHARMA holds each variable group as its own Parthenon field: prims.rho,
+ * prims.u, prims.uvec (3 components), prims.B (3) and the cons.* equivalents
+ * (grmhd/grmhd.cpp:170-185, b_flux_ct/b_flux_ct.cpp:116-118), each with its
+ * own flux arrays, and reaches them through the VarMap indices of a pack
+ * (types.hpp:126-218, grmhd/pack.hpp:51-89).  A kh_vars names the 8 variables
+ * of one container by base pointer, in the order rho, u, u1, u2, u3, B1, B2,
+ * B3; each points at a [k][j][i] array over the entire block.  Nothing
+ * relates the 8 pointers to each other.  The *_v entry points below read and
+ * write the fields in place; the contiguous entry points above are the case
+ * var[n] = base + n * nc and run the same kernels.  Flux containers are one
+ * kh_vars per direction (F[dir-1]), float in the split form. */
+typedef struct kh_vars { void* var[8]; } kh_vars;
+
+/* A pack's VarMap (types.hpp:126-218): the pack index of rho, u, the first
+ * velocity and the first field component (U2 = U1 + 1, ..., B3 = B1 + 2). */
+typedef struct kh_varmap { int RHO, UU, U1, B1; } kh_varmap;
+
+/* kh_vars of a pack given as its per-variable base pointers (pack_var[v] is the
+ * device array of pack variable v, v < nvar) and its VarMap; KH_ERR_INVALID
+ * when an index is absent (-1) or out of range.  Host only. */
+int kh_vars_from_pack(void* const* pack_var, int nvar, const kh_varmap* m, kh_vars* out);
+
+int kh_get_flux_v(kh_block* b, int dir, const kh_vars* P, const kh_vars* F_dir, void* cmax_dir, void* cmin_dir);
+int kh_flux_ct_v(kh_block* b, const kh_vars F[3], void* emf);
+int kh_flux_divergence_v(kh_block* b, const kh_vars F[3], const kh_vars* dUdt);
+int kh_add_geo_source_v(kh_block* b, const kh_vars* P, const kh_vars* dUdt);
+/* Uout may be Usub or Ufull (AddStateUpdate updates in place, kharma_driver.cpp:366-370). */
+int kh_weighted_sum_v(kh_block* b, const kh_vars* Usub, const kh_vars* Ufull, const kh_vars* dUdt, double gam0,
+                      double gam1, double beta_dt, const kh_vars* Uout);
+int kh_utop_v(kh_block* b, const kh_vars* U, const kh_vars* P, int* pflag, int* fflag);
+int kh_apply_floors_v(kh_block* b, const kh_vars* U, const kh_vars* P, int* pflag, int* fflag);
+int kh_fix_utop_v(kh_block* b, const kh_vars* U, const kh_vars* P, const int* pflag);
+int kh_ptou_v(kh_block* b, const kh_vars* P, const kh_vars* U);
+int kh_apply_bounds_v(kh_block* b, const kh_vars* U, const kh_vars* P);
+/* the first nvar (1..8) variables of A */
+int kh_sync_bounds_v(kh_block* b, const kh_vars* A, int nvar);
+int kh_max_divb_v(kh_block* b, const kh_vars* U, double* out);
+
+/* pflag / fflag between this library's int32 arrays and the reference's
+ * Real-valued fields (inverter.cpp:104-108, floors.cpp:111-116), n entries;
+ * Real -> int truncates as the reference's static_cast<int> reads do. */
+int kh_flags_to_real(kh_block* b, const int* flag, double* real_flag, long n);
+int kh_flags_from_real(kh_block* b, const double* real_flag, int* flag, long n);
+
+/* ---- First-order flux correction at the task level ------------------------
+ * The tasks of KHARMADriver::AddFOFC (driver/kharma_driver.cpp:292-337) after
+ * the trial update (kh_flux_divergence_v of the raw fluxes, kh_weighted_sum_v
+ * into the guess, kh_utop_v of the guess from the stage's primitives).  Need
+ * kh_params.fofc; FP64 form. */
+/* Floors::DetermineGRMHDFloors (floors/floors.cpp:197-231), entire block:
+ * fflag |= the floor bits of P's zones, P unchanged.  No-op without floors. */
+int kh_determine_floors_v(kh_block* b, const kh_vars* P, int* fflag);
+/* Flux::MarkFOFC (flux/fofc.cpp:45-111), entire block: fofcflag = 1 where
+ * fflag != 0, within fofc_eh_buffer of the horizon, or in the first
+ * fofc_polar_cells rows at physical X2 faces (spherical coordinates); else 0. */
+int kh_mark_fofc(kh_block* b, const int* fflag, int* fofcflag);
+/* Flux::FOFC (flux/fofc.cpp:113-226) with fofc/use_glf off and cell-centred B:
+ * every face the step consumes (as kh_get_flux) next to a marked zone gets the
+ * LLF flux of the two zones' own primitives P, and those states' signal speeds
+ * in cmax/cmin ([3][nc], cmin stored positive).  A ghost zone's mark is read
+ * from its periodic image or, across a physical face, from the nearest interior
+ * zone (the synced fofcflag of the reference). */
+int kh_fofc_v(kh_block* b, const kh_vars* P, const int* fofcflag, const kh_vars F[3], double* cmax, double* cmin);
 
 /* ---- Problem set-up and geometry (host only; no device work) -------------- */
 

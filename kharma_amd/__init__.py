@@ -72,6 +72,16 @@ class KhTorusParams(ctypes.Structure):
     ]
 
 
+class KhVars(ctypes.Structure):
+    """kh_vars: per-variable base pointers (rho, u, u1, u2, u3, B1, B2, B3)."""
+    _fields_ = [("var", ctypes.c_void_p * 8)]
+
+
+class KhVarMap(ctypes.Structure):
+    """kh_varmap: pack indices of rho, u, the first velocity and the first field component."""
+    _fields_ = [("RHO", ctypes.c_int), ("UU", ctypes.c_int), ("U1", ctypes.c_int), ("B1", ctypes.c_int)]
+
+
 class KharmaError(RuntimeError):
     pass
 
@@ -96,6 +106,7 @@ def lib():
     L = ctypes.CDLL(LIB_PATH)
     vp, i, d, l = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_long
     pvp = ctypes.POINTER(ctypes.c_void_p)
+    pv = ctypes.POINTER(KhVars)
     sig = {
         "kh_last_error": (ctypes.c_char_p, []),
         "kh_version": (i, []),
@@ -155,6 +166,24 @@ def lib():
         "kh_driver_set_stage_record": (i, [vp, vp]),
         "kh_driver_stage_record_bytes": (i, [vp, ctypes.POINTER(ctypes.c_size_t)]),
         "kh_driver_check": (i, [vp]),
+        "kh_vars_from_pack": (i, [ctypes.POINTER(ctypes.c_void_p), i, ctypes.POINTER(KhVarMap), ctypes.POINTER(KhVars)]),
+        "kh_get_flux_v": (i, [vp, i, pv, pv, vp, vp]),
+        "kh_flux_ct_v": (i, [vp, pv, vp]),
+        "kh_flux_divergence_v": (i, [vp, pv, pv]),
+        "kh_add_geo_source_v": (i, [vp, pv, pv]),
+        "kh_weighted_sum_v": (i, [vp, pv, pv, pv, d, d, d, pv]),
+        "kh_utop_v": (i, [vp, pv, pv, vp, vp]),
+        "kh_apply_floors_v": (i, [vp, pv, pv, vp, vp]),
+        "kh_fix_utop_v": (i, [vp, pv, pv, vp]),
+        "kh_ptou_v": (i, [vp, pv, pv]),
+        "kh_apply_bounds_v": (i, [vp, pv, pv]),
+        "kh_sync_bounds_v": (i, [vp, pv, i]),
+        "kh_max_divb_v": (i, [vp, pv, ctypes.POINTER(d)]),
+        "kh_flags_to_real": (i, [vp, vp, vp, l]),
+        "kh_flags_from_real": (i, [vp, vp, vp, l]),
+        "kh_determine_floors_v": (i, [vp, pv, vp]),
+        "kh_mark_fofc": (i, [vp, vp, vp]),
+        "kh_fofc_v": (i, [vp, pv, vp, pv, vp, vp]),
         "kh_driver_objective": (i, [vp, ctypes.POINTER(d), i]),
     }
     for name, (res, args) in sig.items():
@@ -220,8 +249,11 @@ def make_params(cfg):
     p.gamma_max = fl.get("gamma_max", 50.0)
     p.use_r_char = int(fl.get("use_r_char", False))
     p.temp_adjust_u = int(fl.get("temp_adjust_u", False))
-    p.iter_max = cfg.get("iter_max", 8)
-    p.err_tol = cfg.get("err_tol", 1e-8)
+    # inverter/iter_max and err_tol default per solver (inverter/inverter.cpp:65-70):
+    # 25 and 1e-12 for kastaun, 8 and 1e-8 for onedw.
+    kastaun = p.inverter == INVERTER["kastaun"]
+    p.iter_max = cfg.get("iter_max", 25 if kastaun else 8)
+    p.err_tol = cfg.get("err_tol", 1e-12 if kastaun else 1e-8)
     p.fix_average = int(cfg.get("fix_average", True))
     p.fix_atmo = int(cfg.get("fix_atmo", True))
     p.integrator = {"rk2": 0, "vl2": 1}[cfg.get("integrator", "rk2")]
@@ -283,6 +315,27 @@ def _ptr(t):
 def _stream():
     torch = _torch()
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def vars_of(arrays):
+    """kh_vars from 8 device tensors, one per variable (rho, u, u1, u2, u3, B1, B2, B3),
+    each [n3, n2, n1] and contiguous; the tensors may live anywhere."""
+    arrays = list(arrays)
+    if len(arrays) != NV:
+        raise KharmaError(f"a pack view needs {NV} variables, got {len(arrays)}")
+    v = KhVars()
+    for n, a in enumerate(arrays):
+        if not a.is_contiguous():
+            raise KharmaError("pack variables must be contiguous [n3, n2, n1] arrays")
+        v.var[n] = a.data_ptr()
+    return v
+
+
+def _vars3(F):
+    arr = (KhVars * 3)()
+    for d, Fd in enumerate(F):
+        arr[d] = vars_of(Fd)
+    return arr
 
 
 class Block:
@@ -376,6 +429,96 @@ class Block:
         out = ctypes.c_double()
         _check(lib().kh_max_divb(self.h, _ptr(U), ctypes.byref(out)))
         return out.value
+
+    # ---- pack view: each argument is 8 per-variable tensors (fluxes: 3 x 8) ----
+    def get_flux_v(self, d, P, F_d, cmax_d, cmin_d):
+        self._bind()
+        _check(lib().kh_get_flux_v(self.h, d, ctypes.byref(vars_of(P)), ctypes.byref(vars_of(F_d)),
+                                   _ptr(cmax_d), _ptr(cmin_d)))
+
+    def flux_ct_v(self, F, emf):
+        self._bind()
+        _check(lib().kh_flux_ct_v(self.h, _vars3(F), _ptr(emf)))
+
+    def flux_divergence_v(self, F, dUdt):
+        self._bind()
+        _check(lib().kh_flux_divergence_v(self.h, _vars3(F), ctypes.byref(vars_of(dUdt))))
+
+    def add_geo_source_v(self, P, dUdt):
+        self._bind()
+        _check(lib().kh_add_geo_source_v(self.h, ctypes.byref(vars_of(P)), ctypes.byref(vars_of(dUdt))))
+
+    def weighted_sum_v(self, Usub, Ufull, dUdt, gam0, gam1, beta_dt, Uout):
+        self._bind()
+        _check(lib().kh_weighted_sum_v(self.h, ctypes.byref(vars_of(Usub)), ctypes.byref(vars_of(Ufull)),
+                                       ctypes.byref(vars_of(dUdt)), gam0, gam1, beta_dt, ctypes.byref(vars_of(Uout))))
+
+    def utop_v(self, U, P, pflag, fflag):
+        self._bind()
+        _check(lib().kh_utop_v(self.h, ctypes.byref(vars_of(U)), ctypes.byref(vars_of(P)), _ptr(pflag), _ptr(fflag)))
+
+    def apply_floors_v(self, U, P, pflag, fflag):
+        self._bind()
+        _check(lib().kh_apply_floors_v(self.h, ctypes.byref(vars_of(U)), ctypes.byref(vars_of(P)), _ptr(pflag),
+                                       _ptr(fflag)))
+
+    def fix_utop_v(self, U, P, pflag):
+        self._bind()
+        _check(lib().kh_fix_utop_v(self.h, ctypes.byref(vars_of(U)), ctypes.byref(vars_of(P)), _ptr(pflag)))
+
+    def ptou_v(self, P, U):
+        self._bind()
+        _check(lib().kh_ptou_v(self.h, ctypes.byref(vars_of(P)), ctypes.byref(vars_of(U))))
+
+    def apply_bounds_v(self, U, P):
+        self._bind()
+        _check(lib().kh_apply_bounds_v(self.h, ctypes.byref(vars_of(U)), ctypes.byref(vars_of(P))))
+
+    def sync_bounds_v(self, A):
+        """A: 1..8 per-variable tensors."""
+        self._bind()
+        A = list(A)
+        v = KhVars()
+        for n, a in enumerate(A):
+            v.var[n] = a.data_ptr()
+        _check(lib().kh_sync_bounds_v(self.h, ctypes.byref(v), len(A)))
+
+    def max_divb_v(self, U):
+        self._bind()
+        out = ctypes.c_double()
+        _check(lib().kh_max_divb_v(self.h, ctypes.byref(vars_of(U)), ctypes.byref(out)))
+        return out.value
+
+    def flags_to_real(self, flag, real_flag):
+        self._bind()
+        _check(lib().kh_flags_to_real(self.h, _ptr(flag), _ptr(real_flag), flag.numel()))
+
+    def flags_from_real(self, real_flag, flag):
+        self._bind()
+        _check(lib().kh_flags_from_real(self.h, _ptr(real_flag), _ptr(flag), flag.numel()))
+
+    # ---- first-order flux correction at the task level ----
+    def determine_floors_v(self, P, fflag):
+        self._bind()
+        _check(lib().kh_determine_floors_v(self.h, ctypes.byref(vars_of(P)), _ptr(fflag)))
+
+    def mark_fofc(self, fflag, fofcflag):
+        self._bind()
+        _check(lib().kh_mark_fofc(self.h, _ptr(fflag), _ptr(fofcflag)))
+
+    def fofc_v(self, P, fofcflag, F, cmax, cmin):
+        self._bind()
+        _check(lib().kh_fofc_v(self.h, ctypes.byref(vars_of(P)), _ptr(fofcflag), _vars3(F), _ptr(cmax), _ptr(cmin)))
+
+
+def vars_from_pack(pack, varmap):
+    """kh_vars_from_pack: `pack` is a list of per-variable device tensors (a pack's
+    variables in pack order), `varmap` a dict with RHO, UU, U1, B1 pack indices."""
+    arr = (ctypes.c_void_p * len(pack))(*[t.data_ptr() for t in pack])
+    m = KhVarMap(varmap["RHO"], varmap["UU"], varmap["U1"], varmap["B1"])
+    out = KhVars()
+    _check(lib().kh_vars_from_pack(arr, len(pack), ctypes.byref(m), ctypes.byref(out)))
+    return out
 
 
 class Driver:

@@ -200,6 +200,66 @@ hipError_t upload_geometry(kh_block* b) {
     return hipSuccess;
 }
 
+// MarkFOFC's zones marked whatever the trial state (flux/fofc.cpp:61-107):
+// cell centres within eh_buffer of the horizon, and the first polar_cells rows
+// at physical X2 faces, in spherical coordinates.  Laid out as the geometry
+// tables, [gn2][gn1] with gn1 = n1 + 1; absent in Cartesian coordinates.
+hipError_t upload_fofc_fixed(kh_block* blk) {
+    const kh_params& pp = blk->p;
+    if (!pp.fofc || pp.coords != KH_COORDS_KS) return hipSuccess;
+    const BlockDims& bd = blk->dims;
+    const long np = blk->geo.np, gn1 = blk->geo.gn1;
+    std::vector<int> fixed(np, 0);
+    const double r_eh = 1. + std::sqrt(1. - pp.a * pp.a);
+    for (int j = 0; j < bd.n2; ++j)
+        for (int i = 0; i < bd.n1; ++i) {
+            double X[4];
+            locus_coords(pp, bd, GLOC_CENTER, j, i, X);
+            bool m = embed_r(pp, X) < r_eh + pp.fofc_eh_buffer;
+            if (bd.ndim > 1 && !bd.periodic[1] && pp.fofc_polar_cells > 0 &&
+                ((j >= bd.js && j < bd.js + pp.fofc_polar_cells) || (j <= bd.je && j > bd.je - pp.fofc_polar_cells)))
+                m = true;
+            fixed[(size_t)j * gn1 + i] = m;
+        }
+    hipError_t e = hipMalloc(&blk->fofc_fixed, sizeof(int) * np);
+    if (e != hipSuccess) return e;
+    return hipMemcpy(blk->fofc_fixed, fixed.data(), sizeof(int) * np, hipMemcpyHostToDevice);
+}
+
+// kh_vars -> device pointer tables; every one of the 8 pointers must be set.
+bool vars_ok(const kh_vars* v) {
+    if (!v) return false;
+    for (int n = 0; n < NV; ++n)
+        if (!v->var[n]) return false;
+    return true;
+}
+bool flux_vars_ok(const kh_block* b, const kh_vars* F) {
+    if (!F) return false;
+    for (int d = 0; d < b->dims.ndim; ++d)
+        if (!vars_ok(&F[d])) return false;
+    return true;
+}
+template <typename T> Vars<T> vars_in(const kh_vars* v) {
+    Vars<T> r;
+    for (int n = 0; n < NV; ++n) r.v[n] = (T*)v->var[n];
+    return r;
+}
+// Directions beyond the block's dimensionality are never read or written.
+FluxVars<void> flux_vars_in(const kh_block* b, const kh_vars* F) {
+    FluxVars<void> r;
+    for (int d = 0; d < 3; ++d)
+        for (int n = 0; n < NV; ++n) r.d[d].v[n] = (d < b->dims.ndim) ? F[d].var[n] : nullptr;
+    return r;
+}
+kh_vars contiguous_vars(const void* base, long nc, size_t elem) {
+    kh_vars r;
+    for (int n = 0; n < NV; ++n) r.var[n] = base ? (void*)((const char*)base + (size_t)n * nc * elem) : nullptr;
+    return r;
+}
+void contiguous_fluxes(const void* base, long nc, size_t elem, kh_vars F[3]) {
+    for (int d = 0; d < 3; ++d) F[d] = contiguous_vars((const char*)base + (size_t)d * NV * nc * elem, nc, elem);
+}
+
 }  // namespace
 
 extern "C" {
@@ -220,6 +280,7 @@ int kh_block_create(const kh_params* p, kh_block** out) {
     hipError_t e = hipMalloc(&b->scratch, sizeof(DeviceState));
     if (e == hipSuccess) e = hipMalloc(&b->partials, sizeof(double) * b->npartials);
     if (e == hipSuccess && b->dims.geo) e = upload_geometry(b);
+    if (e == hipSuccess) e = upload_fofc_fixed(b);
     if (e != hipSuccess) {
         kh_block_destroy(b);
         return fail(KH_ERR_HIP, std::string("device allocation / copy: ") + hipGetErrorString(e));
@@ -233,6 +294,7 @@ void kh_block_destroy(kh_block* b) {
     if (b->scratch) (void)hipFree(b->scratch);
     if (b->partials) (void)hipFree(b->partials);
     for (void* q : b->geo_bufs) (void)hipFree(q);
+    if (b->fofc_fixed) (void)hipFree(b->fofc_fixed);
     delete b;
 }
 
@@ -251,67 +313,162 @@ int kh_block_shape(const kh_block* b, int* n1, int* n2, int* n3, long* nc) {
     return KH_OK;
 }
 
-int kh_get_flux(kh_block* b, int dir, const double* P, void* F_dir, void* cmax_dir, void* cmin_dir) {
-    if (!b || !P || !F_dir || !cmax_dir || !cmin_dir) return fail(KH_ERR_INVALID, "null argument");
+int kh_vars_from_pack(void* const* pack_var, int nvar, const kh_varmap* m, kh_vars* out) {
+    if (!pack_var || !m || !out) return fail(KH_ERR_INVALID, "null argument");
+    const int first[NV] = {m->RHO, m->UU, m->U1, m->U1, m->U1, m->B1, m->B1, m->B1};
+    const int off[NV] = {0, 0, 0, 1, 2, 0, 1, 2};
+    for (int n = 0; n < NV; ++n) {
+        if (first[n] < 0) return fail(KH_ERR_INVALID, "VarMap index absent (-1)");
+        const int v = first[n] + off[n];
+        if (v >= nvar) return fail(KH_ERR_INVALID, "VarMap index beyond the pack");
+        if (!pack_var[v]) return fail(KH_ERR_INVALID, "null pack variable");
+        out->var[n] = pack_var[v];
+    }
+    return KH_OK;
+}
+
+int kh_get_flux_v(kh_block* b, int dir, const kh_vars* P, const kh_vars* F_dir, void* cmax_dir, void* cmin_dir) {
+    if (!b || !vars_ok(P) || !vars_ok(F_dir) || !cmax_dir || !cmin_dir) return fail(KH_ERR_INVALID, "null argument");
     if (dir < 1 || dir > 3) return fail(KH_ERR_INVALID, "dir must be 1, 2 or 3");
-    KH_HIP(launch_flux(b->dims, b->consts, b->geo, dir, b->p.recon, b->p.precision, P, F_dir, cmax_dir, cmin_dir, b->stream));
+    KH_HIP(launch_flux(b->dims, b->consts, b->geo, dir, b->p.recon, b->p.precision, vars_in<const double>(P),
+                       vars_in<void>(F_dir), cmax_dir, cmin_dir, b->stream));
+    return KH_OK;
+}
+
+int kh_get_flux(kh_block* b, int dir, const double* P, void* F_dir, void* cmax_dir, void* cmin_dir) {
+    if (!b || !P || !F_dir) return fail(KH_ERR_INVALID, "null argument");
+    const kh_vars Pv = contiguous_vars(P, b->dims.nc, sizeof(double));
+    const kh_vars Fv = contiguous_vars(F_dir, b->dims.nc, flux_elem(b));
+    return kh_get_flux_v(b, dir, &Pv, &Fv, cmax_dir, cmin_dir);
+}
+
+int kh_flux_ct_v(kh_block* b, const kh_vars F[3], void* emf) {
+    if (!b || !flux_vars_ok(b, F) || !emf) return fail(KH_ERR_INVALID, "null argument");
+    const FluxVars<void> Fv = flux_vars_in(b, F);
+    KH_HIP(launch_fix_flux(b->dims, b->p.precision, Fv, b->stream));
+    KH_HIP(launch_emf(b->dims, b->p.precision, Fv, emf, b->stream));
+    KH_HIP(launch_flux_ct_rewrite(b->dims, b->p.precision, Fv, emf, b->stream));
     return KH_OK;
 }
 
 int kh_flux_ct(kh_block* b, void* F, void* emf) {
-    if (!b || !F || !emf) return fail(KH_ERR_INVALID, "null argument");
-    KH_HIP(launch_fix_flux(b->dims, b->p.precision, F, b->stream));
-    KH_HIP(launch_emf(b->dims, b->p.precision, F, emf, b->stream));
-    KH_HIP(launch_flux_ct_rewrite(b->dims, b->p.precision, F, emf, b->stream));
+    if (!b || !F) return fail(KH_ERR_INVALID, "null argument");
+    kh_vars Fv[3];
+    contiguous_fluxes(F, b->dims.nc, flux_elem(b), Fv);
+    return kh_flux_ct_v(b, Fv, emf);
+}
+
+int kh_flux_divergence_v(kh_block* b, const kh_vars F[3], const kh_vars* dUdt) {
+    if (!b || !flux_vars_ok(b, F) || !vars_ok(dUdt)) return fail(KH_ERR_INVALID, "null argument");
+    KH_HIP(launch_flux_divergence(b->dims, b->p.precision, flux_vars_in(b, F), vars_in<double>(dUdt), b->stream));
     return KH_OK;
 }
 
 int kh_flux_divergence(kh_block* b, const void* F, double* dUdt) {
     if (!b || !F || !dUdt) return fail(KH_ERR_INVALID, "null argument");
-    KH_HIP(launch_flux_divergence(b->dims, b->p.precision, F, dUdt, b->stream));
+    kh_vars Fv[3];
+    contiguous_fluxes(F, b->dims.nc, flux_elem(b), Fv);
+    const kh_vars Dv = contiguous_vars(dUdt, b->dims.nc, sizeof(double));
+    return kh_flux_divergence_v(b, Fv, &Dv);
+}
+
+int kh_add_geo_source_v(kh_block* b, const kh_vars* P, const kh_vars* dUdt) {
+    if (!b || !vars_ok(P) || !vars_ok(dUdt)) return fail(KH_ERR_INVALID, "null argument");
+    KH_HIP(launch_geo_source(b->dims, b->consts, b->geo, vars_in<const double>(P), vars_in<double>(dUdt), b->stream));
     return KH_OK;
 }
 
 int kh_add_geo_source(kh_block* b, const double* P, double* dUdt) {
     if (!b || !P || !dUdt) return fail(KH_ERR_INVALID, "null argument");
-    KH_HIP(launch_geo_source(b->dims, b->consts, b->geo, P, dUdt, b->stream));
+    const kh_vars Pv = contiguous_vars(P, b->dims.nc, sizeof(double)), Dv = contiguous_vars(dUdt, b->dims.nc, sizeof(double));
+    return kh_add_geo_source_v(b, &Pv, &Dv);
+}
+
+int kh_weighted_sum_v(kh_block* b, const kh_vars* Usub, const kh_vars* Ufull, const kh_vars* dUdt, double gam0,
+                      double gam1, double beta_dt, const kh_vars* Uout) {
+    if (!b || !vars_ok(Usub) || !vars_ok(Ufull) || !vars_ok(dUdt) || !vars_ok(Uout))
+        return fail(KH_ERR_INVALID, "null argument");
+    KH_HIP(launch_weighted_sum(b->dims, vars_in<const double>(Usub), vars_in<const double>(Ufull),
+                               vars_in<const double>(dUdt), gam0, gam1, beta_dt, vars_in<double>(Uout), b->stream));
     return KH_OK;
 }
 
 int kh_weighted_sum(kh_block* b, const double* Usub, const double* Ufull, const double* dUdt,
                     double gam0, double gam1, double beta_dt, double* Uout) {
     if (!b || !Usub || !Ufull || !dUdt || !Uout) return fail(KH_ERR_INVALID, "null argument");
-    KH_HIP(launch_weighted_sum(b->dims, Usub, Ufull, dUdt, gam0, gam1, beta_dt, Uout, b->stream));
+    const long nc = b->dims.nc;
+    const kh_vars a = contiguous_vars(Usub, nc, 8), f = contiguous_vars(Ufull, nc, 8), d = contiguous_vars(dUdt, nc, 8),
+                  o = contiguous_vars(Uout, nc, 8);
+    return kh_weighted_sum_v(b, &a, &f, &d, gam0, gam1, beta_dt, &o);
+}
+
+int kh_utop_v(kh_block* b, const kh_vars* U, const kh_vars* P, int* pflag, int* fflag) {
+    if (!b || !vars_ok(U) || !vars_ok(P) || !pflag || !fflag) return fail(KH_ERR_INVALID, "null argument");
+    KH_HIP(launch_utop(b->dims, b->consts, b->geo, vars_in<const double>(U), vars_in<double>(P), pflag, fflag, b->stream));
     return KH_OK;
 }
 
 int kh_utop(kh_block* b, const double* U, double* P, int* pflag, int* fflag) {
-    if (!b || !U || !P || !pflag || !fflag) return fail(KH_ERR_INVALID, "null argument");
-    KH_HIP(launch_utop(b->dims, b->consts, b->geo, U, P, pflag, fflag, b->stream));
+    if (!b || !U || !P) return fail(KH_ERR_INVALID, "null argument");
+    const kh_vars Uv = contiguous_vars(U, b->dims.nc, 8), Pv = contiguous_vars(P, b->dims.nc, 8);
+    return kh_utop_v(b, &Uv, &Pv, pflag, fflag);
+}
+
+int kh_apply_floors_v(kh_block* b, const kh_vars* U, const kh_vars* P, int* pflag, int* fflag) {
+    if (!b || !vars_ok(U) || !vars_ok(P) || !pflag || !fflag) return fail(KH_ERR_INVALID, "null argument");
+    KH_HIP(launch_floors(b->dims, b->consts, b->geo, vars_in<double>(U), vars_in<double>(P), pflag, fflag, b->stream));
     return KH_OK;
 }
 
 int kh_apply_floors(kh_block* b, double* U, double* P, int* pflag, int* fflag) {
-    if (!b || !U || !P || !pflag || !fflag) return fail(KH_ERR_INVALID, "null argument");
-    KH_HIP(launch_floors(b->dims, b->consts, b->geo, U, P, pflag, fflag, b->stream));
+    if (!b || !U || !P) return fail(KH_ERR_INVALID, "null argument");
+    const kh_vars Uv = contiguous_vars(U, b->dims.nc, 8), Pv = contiguous_vars(P, b->dims.nc, 8);
+    return kh_apply_floors_v(b, &Uv, &Pv, pflag, fflag);
+}
+
+int kh_fix_utop_v(kh_block* b, const kh_vars* U, const kh_vars* P, const int* pflag) {
+    if (!b || !vars_ok(U) || !vars_ok(P) || !pflag) return fail(KH_ERR_INVALID, "null argument");
+    KH_HIP(launch_fixup_entire(b->dims, b->consts, b->geo, vars_in<double>(U), vars_in<double>(P), pflag, b->stream));
     return KH_OK;
 }
 
 int kh_fix_utop(kh_block* b, double* U, double* P, const int* pflag) {
-    if (!b || !U || !P || !pflag) return fail(KH_ERR_INVALID, "null argument");
-    KH_HIP(launch_fixup_entire(b->dims, b->consts, b->geo, U, P, pflag, b->stream));
+    if (!b || !U || !P) return fail(KH_ERR_INVALID, "null argument");
+    const kh_vars Uv = contiguous_vars(U, b->dims.nc, 8), Pv = contiguous_vars(P, b->dims.nc, 8);
+    return kh_fix_utop_v(b, &Uv, &Pv, pflag);
+}
+
+int kh_ptou_v(kh_block* b, const kh_vars* P, const kh_vars* U) {
+    if (!b || !vars_ok(P) || !vars_ok(U)) return fail(KH_ERR_INVALID, "null argument");
+    KH_HIP(launch_ptou(b->dims, b->consts, b->geo, vars_in<const double>(P), vars_in<double>(U), b->stream));
     return KH_OK;
 }
 
 int kh_ptou(kh_block* b, const double* P, double* U) {
     if (!b || !P || !U) return fail(KH_ERR_INVALID, "null argument");
-    KH_HIP(launch_ptou(b->dims, b->consts, b->geo, P, U, b->stream));
+    const kh_vars Pv = contiguous_vars(P, b->dims.nc, 8), Uv = contiguous_vars(U, b->dims.nc, 8);
+    return kh_ptou_v(b, &Pv, &Uv);
+}
+
+int kh_apply_bounds_v(kh_block* b, const kh_vars* U, const kh_vars* P) {
+    if (!b || !vars_ok(U) || !vars_ok(P)) return fail(KH_ERR_INVALID, "null argument");
+    KH_HIP(launch_bounds(b->dims, b->consts, b->geo, vars_in<double>(U), vars_in<double>(P), nullptr, b->stream));
     return KH_OK;
 }
 
 int kh_apply_bounds(kh_block* b, double* U, double* P) {
     if (!b || !U || !P) return fail(KH_ERR_INVALID, "null argument");
-    KH_HIP(launch_bounds(b->dims, b->consts, b->geo, U, P, nullptr, b->stream));
+    const kh_vars Uv = contiguous_vars(U, b->dims.nc, 8), Pv = contiguous_vars(P, b->dims.nc, 8);
+    return kh_apply_bounds_v(b, &Uv, &Pv);
+}
+
+int kh_sync_bounds_v(kh_block* b, const kh_vars* A, int nvar) {
+    if (!b || !A || nvar < 1 || nvar > NV) return fail(KH_ERR_INVALID, "bad argument");
+    for (int n = 0; n < nvar; ++n)
+        if (!A->var[n]) return fail(KH_ERR_INVALID, "null variable");
+    kh_vars a = *A;
+    for (int n = nvar; n < NV; ++n) a.var[n] = nullptr;
+    KH_HIP(launch_sync(b->dims, vars_in<double>(&a), nvar, b->stream));
     return KH_OK;
 }
 
@@ -336,13 +493,53 @@ int kh_estimate_timestep(kh_block* b, const void* cmax, const void* cmin, double
     return KH_OK;
 }
 
-int kh_max_divb(kh_block* b, const double* U, double* out) {
-    if (!b || !U || !out) return fail(KH_ERR_INVALID, "null argument");
+int kh_max_divb_v(kh_block* b, const kh_vars* U, double* out) {
+    if (!b || !vars_ok(U) || !out) return fail(KH_ERR_INVALID, "null argument");
     if (b->dims.ndim < 2) { *out = 0.; return KH_OK; }
-    KH_HIP(launch_max_divb(b->dims, U, b->partials, b->stream));
+    KH_HIP(launch_max_divb(b->dims, vars_in<const double>(U), b->partials, b->stream));
     KH_HIP(launch_reduce(b->partials, (int)divb_blocks(b->dims), 1, &b->scratch->last_divb, b->stream));
     KH_HIP(hipMemcpyAsync(out, &b->scratch->last_divb, sizeof(double), hipMemcpyDeviceToHost, b->stream));
     KH_HIP(hipStreamSynchronize(b->stream));
+    return KH_OK;
+}
+
+int kh_max_divb(kh_block* b, const double* U, double* out) {
+    if (!b || !U) return fail(KH_ERR_INVALID, "null argument");
+    const kh_vars Uv = contiguous_vars(U, b->dims.nc, 8);
+    return kh_max_divb_v(b, &Uv, out);
+}
+
+int kh_flags_to_real(kh_block* b, const int* flag, double* real_flag, long n) {
+    if (!b || !flag || !real_flag || n < 0) return fail(KH_ERR_INVALID, "bad argument");
+    if (n) KH_HIP(launch_flags_to_real(flag, real_flag, n, b->stream));
+    return KH_OK;
+}
+
+int kh_flags_from_real(kh_block* b, const double* real_flag, int* flag, long n) {
+    if (!b || !flag || !real_flag || n < 0) return fail(KH_ERR_INVALID, "bad argument");
+    if (n) KH_HIP(launch_flags_from_real(real_flag, flag, n, b->stream));
+    return KH_OK;
+}
+
+int kh_determine_floors_v(kh_block* b, const kh_vars* P, int* fflag) {
+    if (!b || !vars_ok(P) || !fflag) return fail(KH_ERR_INVALID, "null argument");
+    KH_HIP(launch_determine_floors(b->dims, b->consts, b->geo, vars_in<const double>(P), fflag, b->stream));
+    return KH_OK;
+}
+
+int kh_mark_fofc(kh_block* b, const int* fflag, int* fofcflag) {
+    if (!b || !fflag || !fofcflag) return fail(KH_ERR_INVALID, "null argument");
+    if (!b->p.fofc) return fail(KH_ERR_INVALID, "the block was created without fofc");
+    KH_HIP(launch_mark_fofc(b->dims, b->geo, fflag, b->fofc_fixed, fofcflag, b->stream));
+    return KH_OK;
+}
+
+int kh_fofc_v(kh_block* b, const kh_vars* P, const int* fofcflag, const kh_vars F[3], double* cmax, double* cmin) {
+    if (!b || !vars_ok(P) || !fofcflag || !flux_vars_ok(b, F) || !cmax || !cmin)
+        return fail(KH_ERR_INVALID, "null argument");
+    if (!b->p.fofc) return fail(KH_ERR_INVALID, "the block was created without fofc");
+    KH_HIP(launch_fofc_flux(b->dims, b->consts, b->geo, SPEEDS_BOTH, vars_in<const double>(P), fofcflag,
+                            flux_vars_cast<double>(flux_vars_in(b, F)), cmax, cmin, b->stream));
     return KH_OK;
 }
 
@@ -382,30 +579,6 @@ int kh_driver_create(const kh_params* p, kh_driver** out) {
     if ((e = hipMalloc(&d->ds, sizeof(DeviceState))) != hipSuccess) return cleanup(e, "hipMalloc state");
     if ((e = hipMalloc(&d->dt_hist, sizeof(double) * d->hist_cap)) != hipSuccess) return cleanup(e, "hipMalloc dt_hist");
     if (blk->p.fofc) {
-        // MarkFOFC's zones marked whatever the trial state (flux/fofc.cpp:61-107):
-        // cell centres within eh_buffer of the horizon, and the first polar_cells
-        // rows at physical X2 faces, in spherical coordinates.  Laid out as the
-        // geometry tables, [gn2][gn1] with gn1 = n1 + 1; absent in Cartesian coordinates.
-        const kh_params& pp = blk->p;
-        if (pp.coords == KH_COORDS_KS) {
-            const BlockDims& bd = blk->dims;
-            const long np = blk->geo.np, gn1 = blk->geo.gn1;
-            std::vector<int> fixed(np, 0);
-            const double r_eh = 1. + std::sqrt(1. - pp.a * pp.a);
-            for (int j = 0; j < bd.n2; ++j)
-                for (int i = 0; i < bd.n1; ++i) {
-                    double X[4];
-                    locus_coords(pp, bd, GLOC_CENTER, j, i, X);
-                    bool m = embed_r(pp, X) < r_eh + pp.fofc_eh_buffer;
-                    if (bd.ndim > 1 && !bd.periodic[1] && pp.fofc_polar_cells > 0 &&
-                        ((j >= bd.js && j < bd.js + pp.fofc_polar_cells) || (j <= bd.je && j > bd.je - pp.fofc_polar_cells)))
-                        m = true;
-                    fixed[(size_t)j * gn1 + i] = m;
-                }
-            if ((e = hipMalloc(&d->fofc_fixed, sizeof(int) * np)) != hipSuccess) return cleanup(e, "hipMalloc fofc_fixed");
-            if ((e = hipMemcpy(d->fofc_fixed, fixed.data(), sizeof(int) * np, hipMemcpyHostToDevice)) != hipSuccess)
-                return cleanup(e, "hipMemcpy fofc_fixed");
-        }
         if ((e = hipMalloc(&d->fofc_mark, sizeof(int) * nc)) != hipSuccess) return cleanup(e, "hipMalloc fofc_mark");
         if ((e = hipMemset(d->fofc_mark, 0, sizeof(int) * nc)) != hipSuccess) return cleanup(e, "hipMemset");
     }
@@ -433,7 +606,7 @@ void kh_driver_destroy(kh_driver* d) {
     for (hipEvent_t e : d->ev_pool) (void)hipEventDestroy(e);
     for (void* ptr : {(void*)d->U0, (void*)d->P0, (void*)d->U1, (void*)d->P1, (void*)d->P0f, (void*)d->P1f, d->F, d->ctop,
                       (void*)d->pflag, (void*)d->fflag, (void*)d->fail_list, (void*)d->ds, (void*)d->dt_hist,
-                      (void*)d->dt_part, (void*)d->divb_part, (void*)d->obj, (void*)d->fofc_fixed, (void*)d->fofc_mark})
+                      (void*)d->dt_part, (void*)d->divb_part, (void*)d->obj, (void*)d->fofc_mark})
         if (ptr) (void)hipFree(ptr);
     kh_block_destroy(d->blk);
     delete d;
@@ -524,9 +697,10 @@ int kh::driver_stage(kh_driver* d, int stage) {
                                             F + fe * (dir - 1) * NV * bd.nc, ct + fe * (dir - 1) * bd.nc, s));
     if (b->p.fofc) {
         // KHARMADriver::AddFOFC (kharma_step.cpp:168-173): on the raw fluxes, before the flux fixes
-        KH_TIMED(3, launch_fofc_mark(bd, c, b->geo, stage, (const double*)d->F, Usub, d->U0, Psub, d->fofc_fixed,
+        KH_TIMED(3, launch_fofc_mark(bd, c, b->geo, stage, (const double*)d->F, Usub, d->U0, Psub, b->fofc_fixed,
                                      d->ds, d->fofc_mark, s));
-        KH_TIMED(3, launch_fofc_flux(bd, c, b->geo, speeds, Psub, d->fofc_mark, (double*)d->F, (double*)d->ctop, s));
+        KH_TIMED(3, launch_fofc_flux(bd, c, b->geo, speeds, vars_of(Psub, bd.nc), d->fofc_mark,
+                                     flux_vars_of((double*)d->F, bd.nc), (double*)d->ctop, nullptr, s));
     }
     if (physical) KH_TIMED(3, launch_fix_flux(bd, prec, d->F, s));
     if (d->obj_dir)

@@ -40,7 +40,7 @@ __device__ __forceinline__ void cell_states(R x1, R x2, R x3, R x4, R x5, R& lou
 template <int DIR, int SPEEDS, typename R, class G>
 __device__ __forceinline__ void face_flux_g(const StepConsts& c, const BlockDims& b, const G& g, R rflr, R uflr,
                                             R Pl[NV], R Pr[NV], long ci,
-                                            R* __restrict__ F, R* __restrict__ cmax_out, R* __restrict__ cmin_out) {
+                                            const Vars<R>& F, R* __restrict__ cmax_out, R* __restrict__ cmin_out) {
     const R gam = R(c.gam);
     // get_flux.hpp:170-182: geometric floors of the face's cell centre on both states
     if (c.recon_floors) {
@@ -74,12 +74,12 @@ __device__ __forceinline__ void face_flux_g(const StepConsts& c, const BlockDims
         const R inv = rcp(cmx + cmn);
 #pragma unroll
         for (int v = 0; v < NV; ++v)
-            F[v * b.nc + ci] = (cmx * Fl[v] + cmn * Fr[v] - cmx * cmn * (Ur[v] - Ul[v])) * inv;
+            F[v][ci] = (cmx * Fl[v] + cmn * Fr[v] - cmx * cmn * (Ur[v] - Ul[v])) * inv;
     } else {
         const R ctop = smax(cmx, cmn);
 #pragma unroll
         for (int v = 0; v < NV; ++v)
-            F[v * b.nc + ci] = R(0.5) * (Fl[v] + Fr[v] - ctop * (Ur[v] - Ul[v]));
+            F[v][ci] = R(0.5) * (Fl[v] + Fr[v] - ctop * (Ur[v] - Ul[v]));
     }
 }
 
@@ -96,7 +96,7 @@ template <> __device__ __forceinline__ float geo_floor<float>(const GeomDev& gd,
 template <int DIR, int SPEEDS, typename R, int GEO>
 __device__ __forceinline__ void face_flux(const StepConsts& c, const BlockDims& b, const GeomDev& gd, int j, int i,
                                           R Pl[NV], R Pr[NV], long ci,
-                                          R* __restrict__ F, R* __restrict__ cmax_out, R* __restrict__ cmin_out) {
+                                          const Vars<R>& F, R* __restrict__ cmax_out, R* __restrict__ cmin_out) {
     if constexpr (GEO) {
         const long q = (long)j * gd.gn1 + i;
         const int loc = (DIR == 1) ? GLOC_FACE1 : (DIR == 2) ? GLOC_FACE2 : GLOC_CENTER;
@@ -113,7 +113,7 @@ __device__ __forceinline__ void face_flux(const StepConsts& c, const BlockDims& 
 // [255 b, 255 b + 256) and emits the faces of all but its first cell.
 template <int RECON, int SPEEDS, typename R, typename PT, int GEO>
 __global__ void __launch_bounds__(256)
-flux_x1_kernel(BlockDims b, StepConsts c, GeomDev gd, const PT* __restrict__ P, R* __restrict__ F,
+flux_x1_kernel(BlockDims b, StepConsts c, GeomDev gd, Vars<const PT> P, Vars<R> F,
                R* __restrict__ cmax_out, R* __restrict__ cmin_out, int j0, int nj, int k0, int nk) {
     __shared__ R sr[NV][256];
     const int L = b.nx1 + 2;
@@ -131,7 +131,7 @@ flux_x1_kernel(BlockDims b, StepConsts c, GeomDev gd, const PT* __restrict__ P, 
         ci = (long)(k0 + kk) * b.s3 + (long)(j0 + jj) * b.s2 + (b.is - 1 + ii);
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-            const PT* q = P + v * b.nc + ci;
+            const PT* __restrict__ q = P[v] + ci;
             R ro;
             cell_states<RECON, R>(R(q[-2]), R(q[-1]), R(q[0]), R(q[1]), R(q[2]), lo[v], ro);
             sr[v][tid] = ro;
@@ -150,7 +150,7 @@ flux_x1_kernel(BlockDims b, StepConsts c, GeomDev gd, const PT* __restrict__ P, 
 // [s0, s1]; tile y covers cells s0-1+7y .. s0+6+7y.
 template <int DIR, int RECON, int SPEEDS, typename R, typename PT, int GEO>
 __global__ void __launch_bounds__(512)
-flux_xt_kernel(BlockDims b, StepConsts c, GeomDev gd, const PT* __restrict__ P, R* __restrict__ F,
+flux_xt_kernel(BlockDims b, StepConsts c, GeomDev gd, Vars<const PT> P, Vars<R> F,
                R* __restrict__ cmax_out, R* __restrict__ cmin_out, int o0, int no, int s0, int s1) {
     __shared__ R sr[NV][8][64];
     const int ni = b.nx1 + 2;
@@ -171,7 +171,7 @@ flux_xt_kernel(BlockDims b, StepConsts c, GeomDev gd, const PT* __restrict__ P, 
         ci = (long)k * b.s3 + (long)j * b.s2 + i;
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-            const PT* q = P + v * b.nc + ci;
+            const PT* __restrict__ q = P[v] + ci;
             R ro;
             cell_states<RECON, R>(R(q[-2 * st]), R(q[-st]), R(q[0]), R(q[st]), R(q[2 * st]), lo[v], ro);
             sr[v][ty][tx] = ro;
@@ -186,7 +186,7 @@ flux_xt_kernel(BlockDims b, StepConsts c, GeomDev gd, const PT* __restrict__ P, 
 }
 
 template <int RECON, int SPEEDS, typename R, typename PT, int GEO>
-static hipError_t launch_recon(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int dir, const PT* P, R* F,
+static hipError_t launch_recon(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int dir, const Vars<const PT>& P, const Vars<R>& F,
                                R* cmax, R* cmin, hipStream_t s) {
     // Face ranges consumed by flux-CT and the divergence: the face-normal index
     // covers [start, end+1]; transverse indices extend one zone each side.
@@ -214,7 +214,7 @@ static hipError_t launch_recon(const BlockDims& b, const StepConsts& c, const Ge
 }
 
 template <int SPEEDS, typename R, typename PT, int GEO>
-static hipError_t launch_speeds(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int dir, int recon, const PT* P, R* F,
+static hipError_t launch_speeds(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int dir, int recon, const Vars<const PT>& P, const Vars<R>& F,
                                 R* cmax, R* cmin, hipStream_t s) {
     switch (recon) {
     case RECON_DONOR: return launch_recon<RECON_DONOR, SPEEDS, R, PT, GEO>(b, c, gd, dir, P, F, cmax, cmin, s);
@@ -226,7 +226,7 @@ static hipError_t launch_speeds(const BlockDims& b, const StepConsts& c, const G
 
 template <typename R, typename PT, int GEO>
 static hipError_t launch_geo(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int dir, int recon, int speeds,
-                             const PT* P, R* F, R* cmax, R* cmin, hipStream_t s) {
+                             const Vars<const PT>& P, const Vars<R>& F, R* cmax, R* cmin, hipStream_t s) {
     if (speeds == SPEEDS_CTOP) return launch_speeds<SPEEDS_CTOP, R, PT, GEO>(b, c, gd, dir, recon, P, F, cmax, cmin, s);
     if (speeds == SPEEDS_NONE) return launch_speeds<SPEEDS_NONE, R, PT, GEO>(b, c, gd, dir, recon, P, F, cmax, cmin, s);
     return launch_speeds<SPEEDS_BOTH, R, PT, GEO>(b, c, gd, dir, recon, P, F, cmax, cmin, s);
@@ -234,18 +234,20 @@ static hipError_t launch_geo(const BlockDims& b, const StepConsts& c, const Geom
 
 template <typename R, typename PT>
 static hipError_t launch_prec(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int dir, int recon, int speeds,
-                              const PT* P, R* F, R* cmax, R* cmin, hipStream_t s) {
+                              const Vars<const PT>& P, const Vars<R>& F, R* cmax, R* cmin, hipStream_t s) {
     if (b.geo) return launch_geo<R, PT, 1>(b, c, gd, dir, recon, speeds, P, F, cmax, cmin, s);
     return launch_geo<R, PT, 0>(b, c, gd, dir, recon, speeds, P, F, cmax, cmin, s);
 }
 
 hipError_t launch_flux(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int dir, int recon, int prec,
-                       const double* P, void* F, void* cmax, void* cmin, hipStream_t s) {
+                       const Vars<const double>& P, const Vars<void>& F, void* cmax, void* cmin, hipStream_t s) {
     if (dir == 2 && b.ndim < 2) return hipSuccess;
     if (dir == 3 && b.ndim < 3) return hipSuccess;
     if (prec == PREC_SPLIT)
-        return launch_prec<float, double>(b, c, gd, dir, recon, SPEEDS_BOTH, P, (float*)F, (float*)cmax, (float*)cmin, s);
-    return launch_prec<double, double>(b, c, gd, dir, recon, SPEEDS_BOTH, P, (double*)F, (double*)cmax, (double*)cmin, s);
+        return launch_prec<float, double>(b, c, gd, dir, recon, SPEEDS_BOTH, P, vars_cast<float>(F), (float*)cmax,
+                                          (float*)cmin, s);
+    return launch_prec<double, double>(b, c, gd, dir, recon, SPEEDS_BOTH, P, vars_cast<double>(F), (double*)cmax,
+                                       (double*)cmin, s);
 }
 
 hipError_t launch_flux_stage(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int dir, int recon, int prec,
@@ -253,8 +255,10 @@ hipError_t launch_flux_stage(const BlockDims& b, const StepConsts& c, const Geom
     if (dir == 2 && b.ndim < 2) return hipSuccess;
     if (dir == 3 && b.ndim < 3) return hipSuccess;
     if (prec == PREC_SPLIT)
-        return launch_prec<float, float>(b, c, gd, dir, recon, speeds, P32, (float*)F, (float*)ctop, nullptr, s);
-    return launch_prec<double, double>(b, c, gd, dir, recon, speeds, P, (double*)F, (double*)ctop, nullptr, s);
+        return launch_prec<float, float>(b, c, gd, dir, recon, speeds, vars_of(P32, b.nc), vars_of((float*)F, b.nc),
+                                         (float*)ctop, nullptr, s);
+    return launch_prec<double, double>(b, c, gd, dir, recon, speeds, vars_of(P, b.nc), vars_of((double*)F, b.nc),
+                                       (double*)ctop, nullptr, s);
 }
 
 // ---------------------------------------------- first-order flux correction
@@ -272,8 +276,8 @@ __device__ __forceinline__ int fofc_source(int v, int s, int e, int n, int perio
 }
 template <int DIR, int SPEEDS, int GEO>
 __global__ void __launch_bounds__(256)
-fofc_flux_kernel(BlockDims b, StepConsts c, GeomDev gd, const double* __restrict__ P, const int* __restrict__ mark,
-                 double* __restrict__ F, double* __restrict__ ctop) {
+fofc_flux_kernel(BlockDims b, StepConsts c, GeomDev gd, Vars<const double> P, const int* __restrict__ mark,
+                 Vars<double> F, double* __restrict__ cmax_out, double* __restrict__ cmin_out) {
     const int m1 = (DIR == 1) ? b.nx1 + 1 : b.nx1 + 2;
     const int m2 = (b.ndim < 2) ? 1 : (DIR == 2) ? b.nx2 + 1 : b.nx2 + 2;
     const int m3 = (b.ndim < 3) ? 1 : (DIR == 3) ? b.nx3 + 1 : b.nx3 + 2;
@@ -295,40 +299,46 @@ fofc_flux_kernel(BlockDims b, StepConsts c, GeomDev gd, const double* __restrict
     double Pl[NV], Pr[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
-        Pl[v] = P[v * b.nc + ci - st];
-        Pr[v] = P[v * b.nc + ci];
+        Pl[v] = P[v][ci - st];
+        Pr[v] = P[v][ci];
     }
-    face_flux<DIR, SPEEDS, double, GEO>(c, b, gd, j, i, Pl, Pr, ci, F, ctop, nullptr);
+    face_flux<DIR, SPEEDS, double, GEO>(c, b, gd, j, i, Pl, Pr, ci, F, cmax_out, cmin_out);
 }
 
 template <int DIR, int SPEEDS>
-static hipError_t launch_fofc_dir(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const double* P,
-                                  const int* mark, double* F, double* ctop, hipStream_t s) {
+static hipError_t launch_fofc_dir(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const Vars<const double>& P,
+                                  const int* mark, const Vars<double>& F, double* cmax, double* cmin, hipStream_t s) {
     const long m1 = (DIR == 1) ? b.nx1 + 1 : b.nx1 + 2;
     const long m2 = (b.ndim < 2) ? 1 : (DIR == 2) ? b.nx2 + 1 : b.nx2 + 2;
     const long m3 = (b.ndim < 3) ? 1 : (DIR == 3) ? b.nx3 + 1 : b.nx3 + 2;
     const dim3 grid((unsigned)((m1 * m2 * m3 + 255) / 256));
-    if (b.geo) fofc_flux_kernel<DIR, SPEEDS, 1><<<grid, 256, 0, s>>>(b, c, gd, P, mark, F, ctop);
-    else fofc_flux_kernel<DIR, SPEEDS, 0><<<grid, 256, 0, s>>>(b, c, gd, P, mark, F, ctop);
+    if (b.geo) fofc_flux_kernel<DIR, SPEEDS, 1><<<grid, 256, 0, s>>>(b, c, gd, P, mark, F, cmax, cmin);
+    else fofc_flux_kernel<DIR, SPEEDS, 0><<<grid, 256, 0, s>>>(b, c, gd, P, mark, F, cmax, cmin);
     return hipGetLastError();
 }
 
-hipError_t launch_fofc_flux(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int speeds, const double* P,
-                            const int* mark, double* F, double* ctop, hipStream_t s) {
+template <int SPEEDS>
+static hipError_t launch_fofc_speeds(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int dir,
+                                     const Vars<const double>& P, const int* mark, const Vars<double>& F, double* cmax,
+                                     double* cmin, hipStream_t s) {
+    if (dir == 1) return launch_fofc_dir<1, SPEEDS>(b, c, gd, P, mark, F, cmax, cmin, s);
+    if (dir == 2) return launch_fofc_dir<2, SPEEDS>(b, c, gd, P, mark, F, cmax, cmin, s);
+    return launch_fofc_dir<3, SPEEDS>(b, c, gd, P, mark, F, cmax, cmin, s);
+}
+
+hipError_t launch_fofc_flux(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int speeds,
+                            const Vars<const double>& P, const int* mark, const FluxVars<double>& F, double* cmax,
+                            double* cmin, hipStream_t s) {
     StepConsts cl = c;   // first-order states: no reconstruction floors; LLF whatever flux/type says
     cl.recon_floors = 0;
     cl.use_hlle = 0;
     for (int dir = 1; dir <= b.ndim; ++dir) {
-        double* Fd = F + (long)(dir - 1) * NV * b.nc;
-        double* cd = ctop + (long)(dir - 1) * b.nc;
+        double* cx = cmax ? cmax + (long)(dir - 1) * b.nc : nullptr;
+        double* cn = cmin ? cmin + (long)(dir - 1) * b.nc : nullptr;
         hipError_t e;
-        const bool sp = speeds == SPEEDS_CTOP;
-        if (dir == 1) e = sp ? launch_fofc_dir<1, SPEEDS_CTOP>(b, cl, gd, P, mark, Fd, cd, s)
-                             : launch_fofc_dir<1, SPEEDS_NONE>(b, cl, gd, P, mark, Fd, cd, s);
-        else if (dir == 2) e = sp ? launch_fofc_dir<2, SPEEDS_CTOP>(b, cl, gd, P, mark, Fd, cd, s)
-                                  : launch_fofc_dir<2, SPEEDS_NONE>(b, cl, gd, P, mark, Fd, cd, s);
-        else e = sp ? launch_fofc_dir<3, SPEEDS_CTOP>(b, cl, gd, P, mark, Fd, cd, s)
-                    : launch_fofc_dir<3, SPEEDS_NONE>(b, cl, gd, P, mark, Fd, cd, s);
+        if (speeds == SPEEDS_CTOP) e = launch_fofc_speeds<SPEEDS_CTOP>(b, cl, gd, dir, P, mark, F[dir - 1], cx, cn, s);
+        else if (speeds == SPEEDS_BOTH) e = launch_fofc_speeds<SPEEDS_BOTH>(b, cl, gd, dir, P, mark, F[dir - 1], cx, cn, s);
+        else e = launch_fofc_speeds<SPEEDS_NONE>(b, cl, gd, dir, P, mark, F[dir - 1], cx, cn, s);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;

@@ -15,6 +15,9 @@ struct kh_block {
     kh::DeviceState* scratch = nullptr;  // reductions of the task-level entry points
     double* partials = nullptr;      // one value per work-group of a reduction
     long npartials = 0;
+    // First-order flux correction (kh_params.fofc): zones MarkFOFC marks whatever
+    // the trial state, [gn2][gn1] like the geometry tables (spherical coordinates only)
+    int* fofc_fixed = nullptr;
 };
 
 struct kh_driver {
@@ -33,8 +36,8 @@ struct kh_driver {
     double* obj = nullptr;   // device accumulator
     // step record (kh_driver_set_stage_record): caller's device buffer, or null
     char* rec = nullptr;
-    // first-order flux correction (kh_params.fofc): fixed marks [n2][n1], marks [nc]
-    int *fofc_fixed = nullptr, *fofc_mark = nullptr;
+    // first-order flux correction (kh_params.fofc): marks [nc]
+    int* fofc_mark = nullptr;
     // kernel timing (kh_driver_set_kernel_timing)
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;

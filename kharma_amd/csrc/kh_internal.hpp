@@ -62,6 +62,47 @@ struct StepConsts {
 
 enum Precision { PREC_FP64 = 0, PREC_SPLIT = 1 };
 
+// Per-variable base pointers of one 8-variable cell container, in the order
+// rho, u, u1, u2, u3, B1, B2, B3; each points at a [k][j][i] array over the
+// entire block.  KHARMA allocates its variable groups as separate fields, so
+// the task kernels address every variable through its own pointer; a
+// contiguous [8][nc] array is the case v[n] = base + n nc.
+template <typename T> struct Vars {
+    T* v[NV];
+    __host__ __device__ __forceinline__ T* operator[](int n) const { return v[n]; }
+};
+template <typename T> __host__ __device__ inline Vars<T> vars_of(T* base, long nc) {
+    Vars<T> r;
+    for (int n = 0; n < NV; ++n) r.v[n] = base ? base + (long)n * nc : nullptr;
+    return r;
+}
+template <typename T, typename S> inline Vars<T> vars_cast(const Vars<S>& s) {
+    Vars<T> r;
+    for (int n = 0; n < NV; ++n) r.v[n] = (T*)s.v[n];
+    return r;
+}
+// The flux containers of the three directions (index dir - 1).
+template <typename T> struct FluxVars {
+    Vars<T> d[3];
+    __host__ __device__ __forceinline__ const Vars<T>& operator[](int n) const { return d[n]; }
+};
+template <typename T> __host__ __device__ inline FluxVars<T> flux_vars_of(T* base, long nc) {
+    FluxVars<T> r;
+    for (int n = 0; n < 3; ++n) r.d[n] = vars_of(base ? base + (long)n * NV * nc : nullptr, nc);
+    return r;
+}
+template <typename T, typename S> inline FluxVars<T> flux_vars_cast(const FluxVars<S>& s) {
+    FluxVars<T> r;
+    for (int n = 0; n < 3; ++n) r.d[n] = vars_cast<T>(s.d[n]);
+    return r;
+}
+// The flux containers of a contiguous [3][8][nc] array whose element type is
+// the precision's (float for PREC_SPLIT, double otherwise).
+inline FluxVars<void> flux_vars_prec(void* F, int prec, long nc) {
+    if (prec == PREC_SPLIT) return flux_vars_cast<void>(flux_vars_of((float*)F, nc));
+    return flux_vars_cast<void>(flux_vars_of((double*)F, nc));
+}
+
 // ---- geometry cache (kh_geom.cpp) ----
 // Loci of the cache; the X3 face of an axisymmetric metric equals the centre.
 enum { GLOC_CENTER = 0, GLOC_FACE1 = 1, GLOC_FACE2 = 2, GLOC_CORNER = 3, GLOC_N = 4 };
@@ -107,18 +148,19 @@ enum SpeedsOut { SPEEDS_NONE = 0, SPEEDS_BOTH = 1, SPEEDS_CTOP = 2 };
 // inactive dimensions), cyclically for X2/X3.  F/cmax/cmin point at this
 // direction's arrays ([8][nc] and [nc]); element type float when prec == PREC_SPLIT.
 hipError_t launch_flux(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int dir, int recon, int prec,
-                       const double* P, void* F, void* cmax, void* cmin, hipStream_t s);
+                       const Vars<const double>& P, const Vars<void>& F, void* cmax, void* cmin, hipStream_t s);
 // Stage-driver variant: speeds = SPEEDS_NONE or SPEEDS_CTOP (ctop = max(cmax, cmin)
 // into ctop[dir-1]); the split form reads the FP32 shadow P32 of the primitives.
 hipError_t launch_flux_stage(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int dir, int recon, int prec,
                              int speeds, const double* P, const float* P32, void* F, void* ctop, hipStream_t s);
 
 // ---- launchers (kh_step.hip) ----
-hipError_t launch_emf(const BlockDims& b, int prec, const void* F, void* emf, hipStream_t s);
-hipError_t launch_flux_ct_rewrite(const BlockDims& b, int prec, void* F, const void* emf, hipStream_t s);
-hipError_t launch_flux_divergence(const BlockDims& b, int prec, const void* F, double* dUdt, hipStream_t s);
-hipError_t launch_geo_source(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const double* P, double* dUdt,
-                             hipStream_t s);
+hipError_t launch_emf(const BlockDims& b, int prec, const FluxVars<void>& F, void* emf, hipStream_t s);
+hipError_t launch_flux_ct_rewrite(const BlockDims& b, int prec, const FluxVars<void>& F, const void* emf, hipStream_t s);
+hipError_t launch_flux_divergence(const BlockDims& b, int prec, const FluxVars<void>& F, const Vars<double>& dUdt,
+                                  hipStream_t s);
+hipError_t launch_geo_source(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const Vars<const double>& P,
+                             const Vars<double>& dUdt, hipStream_t s);
 // Fused stage update (flux-CT EMFs, divergence, RK update, U->P, floors, P->U,
 // ghost images).  ctop: [3][nc] max signal speeds, read when do_dt (the last
 // stage), which then writes one CFL minimum per block into dt_part.
@@ -127,8 +169,19 @@ hipError_t launch_geo_source(const BlockDims& b, const StepConsts& c, const Geom
 hipError_t launch_fofc_mark(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int stage, const double* F,
                             const double* Usub, const double* Ufull, const double* Psub, const int* fixed,
                             const DeviceState* ds, int* mark, hipStream_t s);
-hipError_t launch_fofc_flux(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int speeds, const double* P,
-                            const int* mark, double* F, double* ctop, hipStream_t s);
+// speeds: SPEEDS_NONE, SPEEDS_CTOP (max(cmax, cmin) into cmax[dir-1]) or SPEEDS_BOTH
+// (cmax/cmin [3][nc]).
+hipError_t launch_fofc_flux(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int speeds,
+                            const Vars<const double>& P, const int* mark, const FluxVars<double>& F, double* cmax,
+                            double* cmin, hipStream_t s);
+// Floors::DetermineGRMHDFloors over the entire block (fflag |= floor bits).
+hipError_t launch_determine_floors(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const Vars<const double>& P,
+                                   int* fflag, hipStream_t s);
+// Flux::MarkFOFC over the entire block: mark = fflag != 0 or fixed[j][i] (fixed may be null).
+hipError_t launch_mark_fofc(const BlockDims& b, const GeomDev& gd, const int* fflag, const int* fixed, int* mark,
+                            hipStream_t s);
+hipError_t launch_flags_to_real(const int* f, double* r, long n, hipStream_t s);
+hipError_t launch_flags_from_real(const double* r, int* f, long n, hipStream_t s);
 hipError_t launch_update(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int prec, int stage, bool do_dt,
                          const void* F, const void* ctop, const double* Usub, const double* Ufull, const double* Psub,
                          double* Uout, double* Pout, float* Pout32, int* pflag, int* fflag,
@@ -157,29 +210,40 @@ hipError_t launch_fixup(const BlockDims& b, const StepConsts& c, const GeomDev& 
 // B_FluxCT::ZeroBoundaryFlux on polar faces (the update kernel's flux-CT then
 // sees mirrored B2 fluxes), the no-inflow mass-flux clamp on X1 faces and the
 // zeroed polar fluxes of KBoundaries::FixFlux.
+hipError_t launch_fix_flux(const BlockDims& b, int prec, const FluxVars<void>& F, hipStream_t s);
 hipError_t launch_fix_flux(const BlockDims& b, int prec, void* F, hipStream_t s);
 // Physical boundary ghost zones of U, P (and the FP32 shadow if non-null) from
 // the final interior state: KBoundaries::ApplyBoundary for each x1/x2 face.
+hipError_t launch_bounds(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const Vars<double>& U,
+                         const Vars<double>& P, float* P32, hipStream_t s);
 hipError_t launch_bounds(const BlockDims& b, const StepConsts& c, const GeomDev& gd, double* U, double* P, float* P32,
                          hipStream_t s);
 hipError_t launch_finalize(const StepConsts& c, DeviceState* ds, const double* dt_part, int n_dt,
                            const double* divb_part, int n_divb, double* dt_hist, int hist_cap, hipStream_t s);
+hipError_t launch_max_divb(const BlockDims& b, const Vars<const double>& U, double* part, hipStream_t s);
 hipError_t launch_max_divb(const BlockDims& b, const double* U, double* part, hipStream_t s);
 hipError_t launch_reduce(const double* part, int n, int is_max, double* out, hipStream_t s);
 hipError_t launch_convert_f32(const double* A, float* B, long n, hipStream_t s);
 hipError_t launch_reset_state(DeviceState* ds, double dt, hipStream_t s);
 hipError_t launch_set_clock(DeviceState* ds, double time, double dt, int ncycle, hipStream_t s);
 // Task-level kernels over the entire block (the reference's IndexDomain::entire).
-hipError_t launch_weighted_sum(const BlockDims& b, const double* Usub, const double* Ufull, const double* dUdt,
-                               double gam0, double gam1, double beta_dt, double* Uout, hipStream_t s);
-hipError_t launch_utop(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const double* U, double* P,
-                       int* pflag, int* fflag, hipStream_t s);
-hipError_t launch_floors(const BlockDims& b, const StepConsts& c, const GeomDev& gd, double* U, double* P, int* pflag,
-                         int* fflag, hipStream_t s);
-hipError_t launch_fixup_entire(const BlockDims& b, const StepConsts& c, const GeomDev& gd, double* U, double* P,
-                               const int* pflag, hipStream_t s);
+// Uout may alias Usub or Ufull (an in-place update).
+hipError_t launch_weighted_sum(const BlockDims& b, const Vars<const double>& Usub, const Vars<const double>& Ufull,
+                               const Vars<const double>& dUdt, double gam0, double gam1, double beta_dt,
+                               const Vars<double>& Uout, hipStream_t s);
+hipError_t launch_utop(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const Vars<const double>& U,
+                       const Vars<double>& P, int* pflag, int* fflag, hipStream_t s);
+hipError_t launch_floors(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const Vars<double>& U,
+                         const Vars<double>& P, int* pflag, int* fflag, hipStream_t s);
+hipError_t launch_fixup_entire(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const Vars<double>& U,
+                               const Vars<double>& P, const int* pflag, hipStream_t s);
+hipError_t launch_ptou(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const Vars<const double>& P,
+                       const Vars<double>& U, hipStream_t s);
 hipError_t launch_ptou(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const double* P, double* U,
                        hipStream_t s);
+// Periodic ghost fill: nvar <= 8 variables through their pointers, or any
+// number of variables of a contiguous [nvar][nc] array.
+hipError_t launch_sync(const BlockDims& b, const Vars<double>& A, int nvar, hipStream_t s);
 hipError_t launch_sync(const BlockDims& b, double* A, int nvar, hipStream_t s);
 hipError_t launch_sync_int(const BlockDims& b, int* A, hipStream_t s);
 

@@ -152,54 +152,54 @@ __device__ __forceinline__ void add_geo_source(const G& g, const GeomDev& gd, lo
 // Corner EMFs, b_flux_ct.cpp:331-342: emf_d at corners i in [is, ie+1],
 // j in [js, je+1], k in [ks, ke(+1 in 3D)].  F = [3][8][nc], emf = [3][nc].
 template <typename R>
-__global__ void __launch_bounds__(BS) emf_kernel(BlockDims b, const R* __restrict__ F, R* __restrict__ emf) {
+__global__ void __launch_bounds__(BS) emf_kernel(BlockDims b, FluxVars<const R> F, R* __restrict__ emf) {
     const int ni = b.nx1 + 1, nj = b.nx2 + 1, nk = (b.ndim > 2) ? b.nx3 + 1 : 1;
     const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (unsigned)(ni * nj * nk)) return;
     const int kk = t / (ni * nj), rem = t - kk * (ni * nj), jj = rem / ni;
     const int i = b.is + rem - jj * ni, j = b.js + jj, k = b.ks + kk;
     const long c = (long)k * b.s3 + (long)j * b.s2 + i;
-    const R* F1 = F;
-    const R* F2 = F + 8 * b.nc;
-    const R* F3 = F + 16 * b.nc;
+    const Vars<const R>& F1 = F[0];
+    const Vars<const R>& F2 = F[1];
+    const Vars<const R>& F3 = F[2];
     if (b.ndim > 2) {
-        emf[c] = R(0.25) * (F2[B3 * b.nc + c] + F2[B3 * b.nc + c - b.s3] - F3[B2 * b.nc + c] - F3[B2 * b.nc + c - b.s2]);
-        emf[b.nc + c] = R(0.25) * (F3[B1 * b.nc + c] + F3[B1 * b.nc + c - 1] - F1[B3 * b.nc + c] - F1[B3 * b.nc + c - b.s3]);
+        emf[c] = R(0.25) * (F2[B3][c] + F2[B3][c - b.s3] - F3[B2][c] - F3[B2][c - b.s2]);
+        emf[b.nc + c] = R(0.25) * (F3[B1][c] + F3[B1][c - 1] - F1[B3][c] - F1[B3][c - b.s3]);
     }
-    emf[2 * b.nc + c] = R(0.25) * (F1[B2 * b.nc + c] + F1[B2 * b.nc + c - b.s2] - F2[B1 * b.nc + c] - F2[B1 * b.nc + c - 1]);
+    emf[2 * b.nc + c] = R(0.25) * (F1[B2][c] + F1[B2][c - b.s2] - F2[B1][c] - F2[B1][c - 1]);
 }
 
 // Rewrite the B fluxes from the EMFs (b_flux_ct.cpp:347-369), for the task-level API.
 template <typename R>
-__global__ void __launch_bounds__(BS) ct_rewrite_kernel(BlockDims b, R* __restrict__ F, const R* __restrict__ emf) {
+__global__ void __launch_bounds__(BS) ct_rewrite_kernel(BlockDims b, FluxVars<R> F, const R* __restrict__ emf) {
     const int ni = b.nx1 + 1, nj = b.nx2 + 1, nk = (b.ndim > 2) ? b.nx3 + 1 : 1;
     const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (unsigned)(ni * nj * nk)) return;
     const int kk = t / (ni * nj), rem = t - kk * (ni * nj), jj = rem / ni;
     const int i = b.is + rem - jj * ni, j = b.js + jj, k = b.ks + kk;
     const long c = (long)k * b.s3 + (long)j * b.s2 + i;
-    R* F1 = F;
-    R* F2 = F + 8 * b.nc;
-    R* F3 = F + 16 * b.nc;
+    const Vars<R>& F1 = F[0];
+    const Vars<R>& F2 = F[1];
+    const Vars<R>& F3 = F[2];
     const R* e1 = emf;
     const R* e2 = emf + b.nc;
     const R* e3 = emf + 2 * b.nc;
     const bool d3 = b.ndim > 2;
     const bool kin = k <= b.ke, jin = j <= b.je, iin = i <= b.ie;
     if (kin && jin) {  // flux_ct_1: i in [is, ie+1]
-        F1[B1 * b.nc + c] = R(0);
-        F1[B2 * b.nc + c] = R(0.5) * (e3[c] + e3[c + b.s2]);
-        if (d3) F1[B3 * b.nc + c] = R(-0.5) * (e2[c] + e2[c + b.s3]);
+        F1[B1][c] = R(0);
+        F1[B2][c] = R(0.5) * (e3[c] + e3[c + b.s2]);
+        if (d3) F1[B3][c] = R(-0.5) * (e2[c] + e2[c + b.s3]);
     }
     if (kin && iin) {  // flux_ct_2: j in [js, je+1]
-        F2[B1 * b.nc + c] = R(-0.5) * (e3[c] + e3[c + 1]);
-        F2[B2 * b.nc + c] = R(0);
-        if (d3) F2[B3 * b.nc + c] = R(0.5) * (e1[c] + e1[c + b.s3]);
+        F2[B1][c] = R(-0.5) * (e3[c] + e3[c + 1]);
+        F2[B2][c] = R(0);
+        if (d3) F2[B3][c] = R(0.5) * (e1[c] + e1[c + b.s3]);
     }
     if (d3 && jin && iin) {  // flux_ct_3: k in [ks, ke+1]
-        F3[B1 * b.nc + c] = R(0.5) * (e2[c] + e2[c + 1]);
-        F3[B2 * b.nc + c] = R(-0.5) * (e1[c] + e1[c + b.s2]);
-        F3[B3 * b.nc + c] = R(0);
+        F3[B1][c] = R(0.5) * (e2[c] + e2[c + 1]);
+        F3[B2][c] = R(-0.5) * (e1[c] + e1[c + b.s2]);
+        F3[B3][c] = R(0);
     }
 }
 
@@ -207,23 +207,23 @@ __global__ void __launch_bounds__(BS) ct_rewrite_kernel(BlockDims b, R* __restri
 // -(sum_d A_d (F_d(hi) - F_d(lo))) / V), with the flux-CT B fluxes
 // evaluated from the EMFs when `ct` is set.
 template <typename R>
-__device__ __forceinline__ void zone_divergence(const BlockDims& b, const R* __restrict__ F, const R* __restrict__ emf,
+__device__ __forceinline__ void zone_divergence(const BlockDims& b, const FluxVars<const R>& F, const R* __restrict__ emf,
                                                 long c, bool ct, double dudt[NV]) {
     const double A1 = b.dx2 * b.dx3, A2 = b.dx1 * b.dx3, A3 = b.dx1 * b.dx2, V = b.dx1 * b.dx2 * b.dx3;
     const double iV = 1. / V;   // one division per zone instead of eight
-    const R* F1 = F;
-    const R* F2 = F + 8 * b.nc;
-    const R* F3 = F + 16 * b.nc;
+    const Vars<const R>& F1 = F[0];
+    const Vars<const R>& F2 = F[1];
+    const Vars<const R>& F3 = F[2];
     const long s2 = b.s2, s3 = b.s3;
     const int nd = b.ndim;
     double f1lo[NV], f1hi[NV], f2lo[NV], f2hi[NV], f3lo[NV], f3hi[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
-        f1lo[v] = (double)F1[v * b.nc + c];
-        f1hi[v] = (double)F1[v * b.nc + c + 1];
+        f1lo[v] = (double)F1[v][c];
+        f1hi[v] = (double)F1[v][c + 1];
         f2lo[v] = f2hi[v] = f3lo[v] = f3hi[v] = 0.;
-        if (nd >= 2 && !(ct && v >= B1)) { f2lo[v] = (double)F2[v * b.nc + c]; f2hi[v] = (double)F2[v * b.nc + c + s2]; }
-        if (nd == 3 && !(ct && v >= B1)) { f3lo[v] = (double)F3[v * b.nc + c]; f3hi[v] = (double)F3[v * b.nc + c + s3]; }
+        if (nd >= 2 && !(ct && v >= B1)) { f2lo[v] = (double)F2[v][c]; f2hi[v] = (double)F2[v][c + s2]; }
+        if (nd == 3 && !(ct && v >= B1)) { f3lo[v] = (double)F3[v][c]; f3hi[v] = (double)F3[v][c + s3]; }
     }
     if (ct) {
         const R* e1 = emf;
@@ -247,8 +247,8 @@ __device__ __forceinline__ void zone_divergence(const BlockDims& b, const R* __r
             f3lo[B3] = 0.; f3hi[B3] = 0.;
         } else {
             // 2D: B3 keeps its Riemann fluxes
-            f2lo[B3] = (double)F2[B3 * b.nc + c];
-            f2hi[B3] = (double)F2[B3 * b.nc + c + s2];
+            f2lo[B3] = (double)F2[B3][c];
+            f2hi[B3] = (double)F2[B3][c + s2];
         }
     }
 #pragma unroll
@@ -397,7 +397,7 @@ update_kernel(BlockDims b, StepConsts cst, GeomDev gd, int stage, const R* __res
         const double gam = cst.gam;
         double dudt[NV];
         if (CT) zone_divergence_ct<R>(b, F, c, dudt);
-        else zone_divergence<R>(b, F, nullptr, c, false, dudt);
+        else zone_divergence<R>(b, flux_vars_of(F, b.nc), nullptr, c, false, dudt);
         if constexpr (GEO) {
             double Ps[NV];
 #pragma unroll
@@ -516,7 +516,7 @@ combine_kernel(BlockDims b, StepConsts cst, GeomDev gd, int stage, const R* __re
         }
         double dudt[NV];
         if constexpr (ND > 1) zone_divergence_ct<R, ND>(b, F, c, dudt);
-        else zone_divergence<R>(b, F, nullptr, c, false, dudt);
+        else zone_divergence<R>(b, flux_vars_of(F, b.nc), nullptr, c, false, dudt);
         {
             double Ps[NV];
 #pragma unroll
@@ -705,7 +705,7 @@ __global__ void set_clock_kernel(DeviceState* ds, double time, double dt, int nc
 // B_FluxCT::MaxDivB with corner_div (b_flux_ct.cpp:566-607, b_flux_ct_functions.hpp:52-75)
 // Corners i in [is, ie+1] for periodic X1; ValidDivBX1 (b_flux_ct.cpp:554-564)
 // skips the corners on physical X1 faces: i in [is+1, ie].
-__global__ void __launch_bounds__(BS) max_divb_kernel(BlockDims b, const double* __restrict__ U, double* __restrict__ part) {
+__global__ void __launch_bounds__(BS) max_divb_kernel(BlockDims b, Vars<const double> U, double* __restrict__ part) {
     const int i0 = b.periodic[0] ? b.is : b.is + 1;
     const int ni = b.periodic[0] ? b.nx1 + 1 : b.nx1 - 1, nj = b.nx2 + 1, nk = (b.ndim > 2) ? b.nx3 + 1 : 1;
     const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -714,9 +714,9 @@ __global__ void __launch_bounds__(BS) max_divb_kernel(BlockDims b, const double*
         const int kk = t / (ni * nj), rem = t - kk * (ni * nj), jj = rem / ni;
         const int i = i0 + rem - jj * ni, j = b.js + jj, k = b.ks + kk;
         const long c = (long)k * b.s3 + (long)j * b.s2 + i;
-        const double* Bx = U + B1 * b.nc;
-        const double* By = U + B2 * b.nc;
-        const double* Bz = U + B3 * b.nc;
+        const double* __restrict__ Bx = U[B1];
+        const double* __restrict__ By = U[B2];
+        const double* __restrict__ Bz = U[B3];
         const long s2 = b.s2, s3 = b.s3;
         const bool do3 = b.ndim > 2;
         const double norm = do3 ? 0.25 : 0.5;
@@ -740,14 +740,13 @@ __global__ void __launch_bounds__(BS) max_divb_kernel(BlockDims b, const double*
 // b_flux_ct.cpp:372-420; KBoundaries::FixFlux, boundaries.cpp:611-690).  One
 // thread per face of the fixed rows; the two kinds of rows are disjoint.
 template <typename R>
-__global__ void __launch_bounds__(BS) fix_flux_kernel(BlockDims b, R* __restrict__ F) {
+__global__ void __launch_bounds__(BS) fix_flux_kernel(BlockDims b, FluxVars<R> F) {
     const int nd = b.ndim;
     const int ni = b.nx1 + 2, nk = (nd > 2) ? b.nx3 + 2 : 1, nj = (nd > 1) ? b.nx2 + 2 : 1;
     const int i0 = b.is - 1, k0 = (nd > 2) ? b.ks - 1 : b.ks, j0 = (nd > 1) ? b.js - 1 : b.js;
-    const long nc = b.nc;
-    R* F1 = F;
-    R* F2 = F + 8 * nc;
-    R* F3 = F + 16 * nc;
+    const Vars<R>& F1 = F[0];
+    const Vars<R>& F2 = F[1];
+    const Vars<R>& F3 = F[2];
     const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
     const unsigned npolar = (nd > 1 && !b.periodic[1]) ? 2u * ni * nk : 0u;
     if (t < npolar) {
@@ -759,13 +758,13 @@ __global__ void __launch_bounds__(BS) fix_flux_kernel(BlockDims b, R* __restrict
         const long cg = side == 0 ? c - b.s2 : c;        // the face row that is mirrored into
         const long cs = side == 0 ? c : c - b.s2;        // and its source
         if (b.fix_polar_flux) {
-            F2[B1 * nc + c] = R(0);
-            F2[B3 * nc + c] = R(0);
-            F1[B2 * nc + cg] = -F1[B2 * nc + cs];
-            if (nd > 2) F3[B2 * nc + cg] = -F3[B2 * nc + cs];
+            F2[B1][c] = R(0);
+            F2[B3][c] = R(0);
+            F1[B2][cg] = -F1[B2][cs];
+            if (nd > 2) F3[B2][cg] = -F3[B2][cs];
         }
         if (b.zero_polar_flux)
-            for (int v = 0; v < NV; ++v) F2[v * nc + c] = R(0);
+            for (int v = 0; v < NV; ++v) F2[v][c] = R(0);
         return;
     }
     const unsigned t2 = t - npolar;
@@ -776,8 +775,8 @@ __global__ void __launch_bounds__(BS) fix_flux_kernel(BlockDims b, R* __restrict
         if (!b.check_inflow[side]) return;
         const int i = side == 0 ? b.is : b.ie + 1;
         const long c = (long)k * b.s3 + (long)j * b.s2 + i;
-        const R f = F1[RHO * nc + c];
-        F1[RHO * nc + c] = side == 0 ? smin(f, R(0)) : smax(f, R(0));
+        const R f = F1[RHO][c];
+        F1[RHO][c] = side == 0 ? smin(f, R(0)) : smax(f, R(0));
     }
 }
 
@@ -795,28 +794,28 @@ __global__ void __launch_bounds__(BS) fix_flux_kernel(BlockDims b, R* __restrict
 // variables from the primitives, so every filled zone -- a corner's
 // intermediate zone included -- holds U.B = (U.B_src / sqrt(-g)) sqrt(-g).
 template <int GEO>
-__device__ __forceinline__ void bc_fill_x1(const BlockDims& b, const GeomDev& gd, const double* __restrict__ Usrc,
-                                           const double* __restrict__ Psrc, int k, int j, int i, double P[NV], double Bu[3]) {
+__device__ __forceinline__ void bc_fill_x1(const BlockDims& b, const GeomDev& gd, const Vars<double>& Usrc,
+                                           const Vars<double>& Psrc, int k, int j, int i, double P[NV], double Bu[3]) {
     const bool inner = i < b.is;
     const int type = b.bc[inner ? 0 : 1];
     const int si = (type == KH_BC_OUTFLOW) ? (inner ? b.is : b.ie) : (inner ? 2 * b.is - 1 - i : 2 * b.ie + 1 - i);
     const long sc = (long)k * b.s3 + (long)j * b.s2 + si;
-    for (int v = 0; v < 5; ++v) P[v] = Psrc[v * b.nc + sc];
-    for (int v = 0; v < 3; ++v) Bu[v] = Usrc[(B1 + v) * b.nc + sc];
+    for (int v = 0; v < 5; ++v) P[v] = Psrc[v][sc];
+    for (int v = 0; v < 3; ++v) Bu[v] = Usrc[B1 + v][sc];
     if (type == KH_BC_REFLECT) { P[U1] = -P[U1]; Bu[0] = -Bu[0]; }
     if (b.check_inflow[inner ? 0 : 1]) {
         with_center_geom<GEO>(gd, j, i, [&](const auto& g) { check_inflow(g, P, inner); });
     }
 }
 template <int GEO>
-__device__ __forceinline__ void bc_fill_x2(const BlockDims& b, const double* __restrict__ Usrc, const double* __restrict__ Psrc,
+__device__ __forceinline__ void bc_fill_x2(const BlockDims& b, const Vars<double>& Usrc, const Vars<double>& Psrc,
                                            int k, int j, int i, double P[NV], double Bu[3]) {
     const bool inner = j < b.js;
     const int type = b.bc[inner ? 2 : 3];
     const int sj = (type == KH_BC_OUTFLOW) ? (inner ? b.js : b.je) : (inner ? 2 * b.js - 1 - j : 2 * b.je + 1 - j);
     const long sc = (long)k * b.s3 + (long)sj * b.s2 + i;
-    for (int v = 0; v < 5; ++v) P[v] = Psrc[v * b.nc + sc];
-    for (int v = 0; v < 3; ++v) Bu[v] = Usrc[(B1 + v) * b.nc + sc];
+    for (int v = 0; v < 5; ++v) P[v] = Psrc[v][sc];
+    for (int v = 0; v < 3; ++v) Bu[v] = Usrc[B1 + v][sc];
     if (type == KH_BC_REFLECT) { P[U2] = -P[U2]; Bu[1] = -Bu[1]; }
 }
 // The zones the physical boundaries fill, enumerated without the interior:
@@ -831,8 +830,8 @@ __host__ __device__ inline long bounds_slab2(const BlockDims& b) {
     return (long)b.n3 * 2 * b.ng * (b.periodic[0] ? b.n1 : b.nx1);
 }
 template <int GEO>
-__global__ void __launch_bounds__(BS) bounds_kernel(BlockDims b, StepConsts cst, GeomDev gd, double* __restrict__ U,
-                                                    double* __restrict__ P, float* __restrict__ P32) {
+__global__ void __launch_bounds__(BS) bounds_kernel(BlockDims b, StepConsts cst, GeomDev gd, Vars<double> U,
+                                                    Vars<double> P, float* __restrict__ P32) {
     long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long n1s = bounds_slab1(b);
     int k, j, i;
@@ -900,15 +899,15 @@ __global__ void __launch_bounds__(BS) bounds_kernel(BlockDims b, StepConsts cst,
         Pz[B1] = Bu[0] / gdet; Pz[B2] = Bu[1] / gdet; Pz[B3] = Bu[2] / gdet;
         p_to_u(g, Pz, cst.gam, Uz);
     });
-    for (int v = 0; v < 5; ++v) U[v * b.nc + t] = Uz[v];
-    for (int v = 0; v < 3; ++v) U[(B1 + v) * b.nc + t] = b.prim_fill ? Uz[B1 + v] : Bu[v];
-    for (int v = 0; v < NV; ++v) P[v * b.nc + t] = Pz[v];
+    for (int v = 0; v < 5; ++v) U[v][t] = Uz[v];
+    for (int v = 0; v < 3; ++v) U[B1 + v][t] = b.prim_fill ? Uz[B1 + v] : Bu[v];
+    for (int v = 0; v < NV; ++v) P[v][t] = Pz[v];
     if (P32) for (int v = 0; v < NV; ++v) P32[v * b.nc + t] = (float)Pz[v];
 }
 
 // ----------------------------------------------------- task-level kernels
 template <typename R>
-__global__ void __launch_bounds__(BS) divergence_kernel(BlockDims b, const R* __restrict__ F, double* __restrict__ dUdt) {
+__global__ void __launch_bounds__(BS) divergence_kernel(BlockDims b, FluxVars<const R> F, Vars<double> dUdt) {
     const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (unsigned)ninterior(b)) return;
     int k, j, i;
@@ -916,33 +915,36 @@ __global__ void __launch_bounds__(BS) divergence_kernel(BlockDims b, const R* __
     const long c = (long)k * b.s3 + (long)j * b.s2 + i;
     double d[NV];
     zone_divergence<R>(b, F, nullptr, c, false, d);
-    for (int v = 0; v < NV; ++v) dUdt[v * b.nc + c] = d[v];
+    for (int v = 0; v < NV; ++v) dUdt[v][c] = d[v];
 }
 
 // Flux::AddGeoSource (flux/flux.cpp:395-444) over the interior, from the primitives P
-__global__ void __launch_bounds__(BS) geo_source_kernel(BlockDims b, StepConsts cst, GeomDev gd, const double* __restrict__ P,
-                                                        double* __restrict__ dUdt) {
+__global__ void __launch_bounds__(BS) geo_source_kernel(BlockDims b, StepConsts cst, GeomDev gd, Vars<const double> P,
+                                                        Vars<double> dUdt) {
     const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (unsigned)ninterior(b)) return;
     int k, j, i;
     interior_index(b, t, k, j, i);
     const long c = (long)k * b.s3 + (long)j * b.s2 + i;
     double Pz[NV], d[NV];
-    for (int v = 0; v < NV; ++v) { Pz[v] = P[v * b.nc + c]; d[v] = dUdt[v * b.nc + c]; }
+    for (int v = 0; v < NV; ++v) { Pz[v] = P[v][c]; d[v] = dUdt[v][c]; }
     const long q = (long)j * gd.gn1 + i;
     CachedGeom<double> g(gd.g[GLOC_CENTER], gd.np, q);
     add_geo_source(g, gd, q, Pz, cst.gam, d);
-    for (int v = UU; v <= U3; ++v) dUdt[v * b.nc + c] = d[v];
+    for (int v = UU; v <= U3; ++v) dUdt[v][c] = d[v];
 }
 
-// parthenon::Update::WeightedSumData pair used by AddStateUpdate, entire block
-__global__ void __launch_bounds__(BS) weighted_sum_kernel(long n, const double* __restrict__ Usub, const double* __restrict__ Ufull,
-                                                          const double* __restrict__ dUdt, double g0, double g1, double bdt,
-                                                          double* __restrict__ Uout) {
+// parthenon::Update::WeightedSumData pair used by AddStateUpdate, entire block;
+// variable blockIdx.y.  Uout may be Usub or Ufull: AddStateUpdate updates in
+// place (kharma_driver.cpp:366-370), and each element is read before it is written.
+__global__ void __launch_bounds__(BS) weighted_sum_kernel(long n, Vars<const double> Usub, Vars<const double> Ufull,
+                                                          Vars<const double> dUdt, double g0, double g1, double bdt,
+                                                          Vars<double> Uout) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int v = blockIdx.y;
     if (t >= n) return;
-    const double u = g0 * Usub[t] + g1 * Ufull[t];
-    Uout[t] = 1.0 * u + bdt * dUdt[t];
+    const double u = g0 * Usub[v][t] + g1 * Ufull[v][t];
+    Uout[v][t] = 1.0 * u + bdt * dUdt[v][t];
 }
 
 __device__ __forceinline__ void zone_ijk(const BlockDims& b, long c, int& k, int& j, int& i) {
@@ -953,8 +955,8 @@ __device__ __forceinline__ void zone_ijk(const BlockDims& b, long c, int& k, int
 
 // Inverter::BlockUtoP over the physical range + B_FluxCT::BlockUtoP over the entire block
 template <int GEO, int INV = 0>
-__global__ void __launch_bounds__(BS) utop_kernel(BlockDims b, StepConsts cst, GeomDev gd, const double* __restrict__ U,
-                                                  double* __restrict__ P, int* __restrict__ pflag, int* __restrict__ fflag) {
+__global__ void __launch_bounds__(BS) utop_kernel(BlockDims b, StepConsts cst, GeomDev gd, Vars<const double> U,
+                                                  Vars<double> P, int* __restrict__ pflag, int* __restrict__ fflag) {
     const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= b.nc) return;
     int k, j, i;
@@ -962,28 +964,28 @@ __global__ void __launch_bounds__(BS) utop_kernel(BlockDims b, StepConsts cst, G
     with_center_geom<GEO>(gd, j, i, [&](const auto& g) {
         if (in_physical(b, k, j, i)) {
             double Uz[NV], Pz[NV];
-            for (int v = 0; v < NV; ++v) { Uz[v] = U[v * b.nc + c]; Pz[v] = P[v * b.nc + c]; }
+            for (int v = 0; v < NV; ++v) { Uz[v] = U[v][c]; Pz[v] = P[v][c]; }
             const int r = invert_zone<INV>(g, Uz, Pz, cst);
-            for (int v = 0; v < 5; ++v) P[v * b.nc + c] = Pz[v];
+            for (int v = 0; v < 5; ++v) P[v][c] = Pz[v];
             pflag[c] = r % F_GEOM_RHO;
             fflag[c] = (r / F_GEOM_RHO) * F_GEOM_RHO;
         }
         const double gdet = g.gdet();
-        for (int v = 0; v < 3; ++v) P[(B1 + v) * b.nc + c] = U[(B1 + v) * b.nc + c] / gdet;
+        for (int v = 0; v < 3; ++v) P[B1 + v][c] = U[B1 + v][c] / gdet;
     });
 }
 
 // Floors::ApplyGRMHDFloors over the entire block: floors, ceilings and the
 // 5 hydrodynamic conserved variables of modified zones (p_to_u_mhd).
 template <int GEO>
-__global__ void __launch_bounds__(BS) floors_kernel(BlockDims b, StepConsts cst, GeomDev gd, double* __restrict__ U,
-                                                    double* __restrict__ P, int* __restrict__ pflag, int* __restrict__ fflag) {
+__global__ void __launch_bounds__(BS) floors_kernel(BlockDims b, StepConsts cst, GeomDev gd, Vars<double> U,
+                                                    Vars<double> P, int* __restrict__ pflag, int* __restrict__ fflag) {
     const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= b.nc) return;
     int k, j, i;
     zone_ijk(b, c, k, j, i);
     double Uz[NV], Pz[NV];
-    for (int v = 0; v < NV; ++v) { Uz[v] = U[v * b.nc + c]; Pz[v] = P[v * b.nc + c]; }
+    for (int v = 0; v < NV; ++v) { Uz[v] = U[v][c]; Pz[v] = P[v][c]; }
     double rf, uf;
     geom_floors<GEO>(cst, gd, j, i, rf, uf);
     with_center_geom<GEO>(gd, j, i, [&](const auto& g) {
@@ -996,14 +998,14 @@ __global__ void __launch_bounds__(BS) floors_kernel(BlockDims b, StepConsts cst,
         if (ff) {
             double Un[NV];
             p_to_u_D(g, Pz, D, cst.gam, Un);
-            for (int v = 0; v < 5; ++v) { P[v * b.nc + c] = Pz[v]; U[v * b.nc + c] = Un[v]; }
+            for (int v = 0; v < 5; ++v) { P[v][c] = Pz[v]; U[v][c] = Un[v]; }
         }
     });
 }
 
 // Inverter::FixUtoP over the physical range; two passes so averages only read
 // non-failed zones.
-__global__ void __launch_bounds__(BS) fixup_avg_kernel(BlockDims b, StepConsts cst, double* __restrict__ P,
+__global__ void __launch_bounds__(BS) fixup_avg_kernel(BlockDims b, StepConsts cst, Vars<double> P,
                                                        const int* __restrict__ pflag) {
     const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= b.nc || !(pflag[c] > 0)) return;
@@ -1019,50 +1021,50 @@ __global__ void __launch_bounds__(BS) fixup_avg_kernel(BlockDims b, StepConsts c
             if (!(pflag[cc] > 0)) {
                 const double w = 1. / (abs(dl) + abs(dm) + abs(dn) + 1);
                 wsum += w;
-                for (int p = 0; p < 5; ++p) sum[p] += w * P[p * b.nc + cc];
+                for (int p = 0; p < 5; ++p) sum[p] += w * P[p][cc];
             }
         }
     }
-    for (int p = 0; p < 5; ++p) P[p * b.nc + c] = (wsum < 1.e-10) ? 0. : sum[p] / wsum;
+    for (int p = 0; p < 5; ++p) P[p][c] = (wsum < 1.e-10) ? 0. : sum[p] / wsum;
 }
 template <int GEO>
-__global__ void __launch_bounds__(BS) fixup_floor_kernel(BlockDims b, StepConsts cst, GeomDev gd, double* __restrict__ U,
-                                                         double* __restrict__ P, const int* __restrict__ pflag) {
+__global__ void __launch_bounds__(BS) fixup_floor_kernel(BlockDims b, StepConsts cst, GeomDev gd, Vars<double> U,
+                                                         Vars<double> P, const int* __restrict__ pflag) {
     const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= b.nc || !(pflag[c] > 0)) return;
     int k, j, i;
     zone_ijk(b, c, k, j, i);
     if (!in_physical(b, k, j, i)) return;
     double Pz[NV], Uz[NV];
-    for (int v = 0; v < NV; ++v) Pz[v] = P[v * b.nc + c];
+    for (int v = 0; v < NV; ++v) Pz[v] = P[v][c];
     double rf, uf;
     geom_floors<GEO>(cst, gd, j, i, rf, uf);
     apply_geo_floors<double>(Pz, rf, uf);
     with_center_geom<GEO>(gd, j, i, [&](const auto& g) { p_to_u(g, Pz, cst.gam, Uz); });
-    for (int v = 0; v < 5; ++v) { P[v * b.nc + c] = Pz[v]; U[v * b.nc + c] = Uz[v]; }
+    for (int v = 0; v < 5; ++v) { P[v][c] = Pz[v]; U[v][c] = Uz[v]; }
 }
 
 // Flux::MeshPtoU over the entire block
 template <int GEO>
-__global__ void __launch_bounds__(BS) ptou_kernel(BlockDims b, StepConsts cst, GeomDev gd, const double* __restrict__ P,
-                                                  double* __restrict__ U) {
+__global__ void __launch_bounds__(BS) ptou_kernel(BlockDims b, StepConsts cst, GeomDev gd, Vars<const double> P,
+                                                  Vars<double> U) {
     const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= b.nc) return;
     int k, j, i;
     zone_ijk(b, c, k, j, i);
     double Pz[NV], Uz[NV];
-    for (int v = 0; v < NV; ++v) Pz[v] = P[v * b.nc + c];
+    for (int v = 0; v < NV; ++v) Pz[v] = P[v][c];
     with_center_geom<GEO>(gd, j, i, [&](const auto& g) { p_to_u(g, Pz, cst.gam, Uz); });
-    for (int v = 0; v < NV; ++v) U[v * b.nc + c] = Uz[v];
+    for (int v = 0; v < NV; ++v) U[v][c] = Uz[v];
 }
 
-// Periodic ghost fill of nvar variables: every ghost zone copies its interior image.
+// Periodic ghost fill of up to 8 variables (variable blockIdx.y): every ghost
+// zone copies its interior image.
 template <typename T>
-__global__ void __launch_bounds__(BS) sync_kernel(BlockDims b, T* __restrict__ A, int nvar) {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= b.nc * nvar) return;
-    const int v = (int)(t / b.nc);
-    const long c = t - (long)v * b.nc;
+__global__ void __launch_bounds__(BS) sync_kernel(BlockDims b, Vars<T> A) {
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int v = blockIdx.y;
+    if (c >= b.nc) return;
     const int k = (int)(c / b.s3), j = (int)((c - (long)k * b.s3) / b.s2), i = (int)(c - (long)k * b.s3 - (long)j * b.s2);
     const bool in_i = i >= b.is && i <= b.ie, in_j = j >= b.js && j <= b.je, in_k = k >= b.ks && k <= b.ke;
     if (in_i && in_j && in_k) return;
@@ -1071,7 +1073,7 @@ __global__ void __launch_bounds__(BS) sync_kernel(BlockDims b, T* __restrict__ A
     const int sj = (b.ndim > 1 && b.periodic[1] && !in_j) ? wrap(j, b.js, b.nx2) : j;
     const int sk = (b.ndim > 2 && b.periodic[2] && !in_k) ? wrap(k, b.ks, b.nx3) : k;
     const long src = (long)sk * b.s3 + (long)sj * b.s2 + si;
-    if (src != c) A[(long)v * b.nc + c] = A[(long)v * b.nc + src];
+    if (src != c) A[v][c] = A[v][src];
 }
 
 template <typename R>
@@ -1124,7 +1126,75 @@ __global__ void __launch_bounds__(BS) face_flux_objective_kernel(BlockDims b, co
     }
 }
 
+// Floors::DetermineGRMHDFloors (floors/floors.cpp:197-231) over the entire
+// block: fflag |= the floor and ceiling bits the zone's primitives would take
+// (determine_floors, floors/floors_functions.hpp), the primitives unchanged.
+template <int GEO>
+__global__ void __launch_bounds__(BS) determine_floors_kernel(BlockDims b, StepConsts cst, GeomDev gd, Vars<const double> P,
+                                                              int* __restrict__ fflag) {
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= b.nc) return;
+    int k, j, i;
+    zone_ijk(b, c, k, j, i);
+    double Pz[NV];
+    for (int v = 0; v < NV; ++v) Pz[v] = P[v][c];
+    double rf, uf;
+    geom_floors<GEO>(cst, gd, j, i, rf, uf);
+    int ff = 0;
+    with_center_geom<GEO>(gd, j, i, [&](const auto& g) {
+        FourVec<double> D;
+        calc_4vecs<double>(g, Pz, D);
+        double rfm, ufm;
+        ff = determine_floors(g, Pz, dot4(D.bcon, D.bcov), cst.gam, cst.floors, rf, uf, rfm, ufm);
+    });
+    fflag[c] |= ff;
+}
+
+// Flux::MarkFOFC (flux/fofc.cpp:45-111) over the entire block: a zone is marked
+// when the floor flag of its trial state is set, or when `fixed` ([n2+1][n1+1],
+// or null) marks it: cell centres within eh_buffer of the horizon and the first
+// polar_cells rows at physical X2 faces.
+__global__ void __launch_bounds__(BS) mark_fofc_kernel(BlockDims b, int gn1, const int* __restrict__ fflag,
+                                                       const int* __restrict__ fixed, int* __restrict__ mark) {
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= b.nc) return;
+    int k, j, i;
+    zone_ijk(b, c, k, j, i);
+    mark[c] = (fflag[c] != 0 || (fixed && fixed[(long)j * gn1 + i])) ? 1 : 0;
+}
+
+// pflag / fflag between the int32 arrays of this library and the reference's
+// Real-valued fields, read back with static_cast<int> as the reference does.
+__global__ void __launch_bounds__(BS) flags_to_real_kernel(long n, const int* __restrict__ f, double* __restrict__ r) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) r[t] = (double)f[t];
+}
+__global__ void __launch_bounds__(BS) flags_from_real_kernel(long n, const double* __restrict__ r, int* __restrict__ f) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) f[t] = (int)r[t];
+}
+
 // ---------------------------------------------------------------- launchers
+hipError_t launch_determine_floors(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const Vars<const double>& P,
+                                   int* fflag, hipStream_t s) {
+    if (!c.floors.on) return hipSuccess;
+    if (b.geo) hipLaunchKernelGGL(determine_floors_kernel<1>, grid_for(b.nc), dim3(BS), 0, s, b, c, gd, P, fflag);
+    else hipLaunchKernelGGL(determine_floors_kernel<0>, grid_for(b.nc), dim3(BS), 0, s, b, c, gd, P, fflag);
+    return hipGetLastError();
+}
+hipError_t launch_mark_fofc(const BlockDims& b, const GeomDev& gd, const int* fflag, const int* fixed, int* mark,
+                            hipStream_t s) {
+    hipLaunchKernelGGL(mark_fofc_kernel, grid_for(b.nc), dim3(BS), 0, s, b, gd.gn1, fflag, fixed, mark);
+    return hipGetLastError();
+}
+hipError_t launch_flags_to_real(const int* f, double* r, long n, hipStream_t s) {
+    hipLaunchKernelGGL(flags_to_real_kernel, grid_for(n), dim3(BS), 0, s, n, f, r);
+    return hipGetLastError();
+}
+hipError_t launch_flags_from_real(const double* r, int* f, long n, hipStream_t s) {
+    hipLaunchKernelGGL(flags_from_real_kernel, grid_for(n), dim3(BS), 0, s, n, r, f);
+    return hipGetLastError();
+}
 hipError_t launch_face_flux_objective(const BlockDims& b, int prec, const void* F, int dir, int var, int face, double w,
                                       const DeviceState* ds, double* acc, hipStream_t s) {
     if (prec == PREC_SPLIT)
@@ -1133,30 +1203,37 @@ hipError_t launch_face_flux_objective(const BlockDims& b, int prec, const void* 
         hipLaunchKernelGGL(face_flux_objective_kernel<double>, dim3(1), dim3(BS), 0, s, b, (const double*)F, dir, var, face, w, ds, acc);
     return hipGetLastError();
 }
-hipError_t launch_emf(const BlockDims& b, int prec, const void* F, void* emf, hipStream_t s) {
+hipError_t launch_emf(const BlockDims& b, int prec, const FluxVars<void>& F, void* emf, hipStream_t s) {
     if (b.ndim < 2) return hipSuccess;
     const long n = (long)(b.nx1 + 1) * (b.nx2 + 1) * ((b.ndim > 2) ? b.nx3 + 1 : 1);
-    if (prec == PREC_SPLIT) hipLaunchKernelGGL(emf_kernel<float>, grid_for(n), dim3(BS), 0, s, b, (const float*)F, (float*)emf);
-    else hipLaunchKernelGGL(emf_kernel<double>, grid_for(n), dim3(BS), 0, s, b, (const double*)F, (double*)emf);
+    if (prec == PREC_SPLIT)
+        hipLaunchKernelGGL(emf_kernel<float>, grid_for(n), dim3(BS), 0, s, b, flux_vars_cast<const float>(F), (float*)emf);
+    else
+        hipLaunchKernelGGL(emf_kernel<double>, grid_for(n), dim3(BS), 0, s, b, flux_vars_cast<const double>(F), (double*)emf);
     return hipGetLastError();
 }
-hipError_t launch_flux_ct_rewrite(const BlockDims& b, int prec, void* F, const void* emf, hipStream_t s) {
+hipError_t launch_flux_ct_rewrite(const BlockDims& b, int prec, const FluxVars<void>& F, const void* emf, hipStream_t s) {
     if (b.ndim < 2) return hipSuccess;
     const long n = (long)(b.nx1 + 1) * (b.nx2 + 1) * ((b.ndim > 2) ? b.nx3 + 1 : 1);
-    if (prec == PREC_SPLIT) hipLaunchKernelGGL(ct_rewrite_kernel<float>, grid_for(n), dim3(BS), 0, s, b, (float*)F, (const float*)emf);
-    else hipLaunchKernelGGL(ct_rewrite_kernel<double>, grid_for(n), dim3(BS), 0, s, b, (double*)F, (const double*)emf);
+    if (prec == PREC_SPLIT)
+        hipLaunchKernelGGL(ct_rewrite_kernel<float>, grid_for(n), dim3(BS), 0, s, b, flux_vars_cast<float>(F), (const float*)emf);
+    else
+        hipLaunchKernelGGL(ct_rewrite_kernel<double>, grid_for(n), dim3(BS), 0, s, b, flux_vars_cast<double>(F), (const double*)emf);
     return hipGetLastError();
 }
-hipError_t launch_geo_source(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const double* P, double* dUdt,
-                             hipStream_t s) {
+hipError_t launch_geo_source(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const Vars<const double>& P,
+                             const Vars<double>& dUdt, hipStream_t s) {
     if (!b.geo) return hipSuccess;   // Cartesian Minkowski: no source (flux.cpp:407)
     hipLaunchKernelGGL(geo_source_kernel, grid_for(ninterior_host(b)), dim3(BS), 0, s, b, c, gd, P, dUdt);
     return hipGetLastError();
 }
-hipError_t launch_flux_divergence(const BlockDims& b, int prec, const void* F, double* dUdt, hipStream_t s) {
+hipError_t launch_flux_divergence(const BlockDims& b, int prec, const FluxVars<void>& F, const Vars<double>& dUdt,
+                                  hipStream_t s) {
     const long n = (long)b.nx1 * (b.ndim > 1 ? b.nx2 : 1) * (b.ndim > 2 ? b.nx3 : 1);
-    if (prec == PREC_SPLIT) hipLaunchKernelGGL(divergence_kernel<float>, grid_for(n), dim3(BS), 0, s, b, (const float*)F, dUdt);
-    else hipLaunchKernelGGL(divergence_kernel<double>, grid_for(n), dim3(BS), 0, s, b, (const double*)F, dUdt);
+    if (prec == PREC_SPLIT)
+        hipLaunchKernelGGL(divergence_kernel<float>, grid_for(n), dim3(BS), 0, s, b, flux_vars_cast<const float>(F), dUdt);
+    else
+        hipLaunchKernelGGL(divergence_kernel<double>, grid_for(n), dim3(BS), 0, s, b, flux_vars_cast<const double>(F), dUdt);
     return hipGetLastError();
 }
 // ---------------------------------------------- first-order flux correction
@@ -1178,7 +1255,7 @@ fofc_mark_kernel(BlockDims b, StepConsts cst, GeomDev gd, int stage, const doubl
     interior_index(b, t, k, j, i);
     const long c = (long)k * b.s3 + (long)j * b.s2 + i;
     double dudt[NV];
-    zone_divergence<double>(b, F, nullptr, c, false, dudt);
+    zone_divergence<double>(b, flux_vars_of(F, b.nc), nullptr, c, false, dudt);
     const double g0 = cst.gam0[stage - 1], g1 = cst.gam1[stage - 1];
     const double beta_dt = cst.beta[stage - 1] * ds->dt;
     const bool same = (g0 == 0.) && (Usub == Ufull);
@@ -1320,32 +1397,42 @@ hipError_t launch_fixup(const BlockDims& b, const StepConsts& c, const GeomDev& 
     else hipLaunchKernelGGL(fixup_list_kernel<0>, dim3(64), dim3(BS), 0, s, b, c, gd, stage, U, P, P32, pflag, fail_list, ds);
     return hipGetLastError();
 }
-hipError_t launch_fix_flux(const BlockDims& b, int prec, void* F, hipStream_t s) {
+hipError_t launch_fix_flux(const BlockDims& b, int prec, const FluxVars<void>& F, hipStream_t s) {
     const long npolar = (b.ndim > 1 && !b.periodic[1]) ? 2L * (b.nx1 + 2) * ((b.ndim > 2) ? b.nx3 + 2 : 1) : 0;
     const long nx1f = !b.periodic[0] ? 2L * ((b.ndim > 1) ? b.nx2 + 2 : 1) * ((b.ndim > 2) ? b.nx3 + 2 : 1) : 0;
     const long n = npolar + nx1f;
     if (n == 0) return hipSuccess;
-    if (prec == PREC_SPLIT) hipLaunchKernelGGL(fix_flux_kernel<float>, grid_for(n), dim3(BS), 0, s, b, (float*)F);
-    else hipLaunchKernelGGL(fix_flux_kernel<double>, grid_for(n), dim3(BS), 0, s, b, (double*)F);
+    if (prec == PREC_SPLIT) hipLaunchKernelGGL(fix_flux_kernel<float>, grid_for(n), dim3(BS), 0, s, b, flux_vars_cast<float>(F));
+    else hipLaunchKernelGGL(fix_flux_kernel<double>, grid_for(n), dim3(BS), 0, s, b, flux_vars_cast<double>(F));
     return hipGetLastError();
 }
-hipError_t launch_bounds(const BlockDims& b, const StepConsts& c, const GeomDev& gd, double* U, double* P, float* P32,
-                         hipStream_t s) {
+hipError_t launch_fix_flux(const BlockDims& b, int prec, void* F, hipStream_t s) {
+    return launch_fix_flux(b, prec, flux_vars_prec(F, prec, b.nc), s);
+}
+hipError_t launch_bounds(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const Vars<double>& U,
+                         const Vars<double>& P, float* P32, hipStream_t s) {
     if (b.periodic[0] && (b.ndim < 2 || b.periodic[1])) return hipSuccess;
     const long n = bounds_slab1(b) + bounds_slab2(b);
     if (b.geo) hipLaunchKernelGGL(bounds_kernel<1>, grid_for(n), dim3(BS), 0, s, b, c, gd, U, P, P32);
     else hipLaunchKernelGGL(bounds_kernel<0>, grid_for(n), dim3(BS), 0, s, b, c, gd, U, P, P32);
     return hipGetLastError();
 }
+hipError_t launch_bounds(const BlockDims& b, const StepConsts& c, const GeomDev& gd, double* U, double* P, float* P32,
+                         hipStream_t s) {
+    return launch_bounds(b, c, gd, vars_of(U, b.nc), vars_of(P, b.nc), P32, s);
+}
 hipError_t launch_finalize(const StepConsts& c, DeviceState* ds, const double* dt_part, int n_dt,
                            const double* divb_part, int n_divb, double* dt_hist, int hist_cap, hipStream_t s) {
     hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(1024), 0, s, c, ds, dt_part, n_dt, divb_part, n_divb, dt_hist, hist_cap);
     return hipGetLastError();
 }
-hipError_t launch_max_divb(const BlockDims& b, const double* U, double* part, hipStream_t s) {
+hipError_t launch_max_divb(const BlockDims& b, const Vars<const double>& U, double* part, hipStream_t s) {
     if (b.ndim < 2) return hipSuccess;
     hipLaunchKernelGGL(max_divb_kernel, dim3((unsigned)divb_blocks(b)), dim3(BS), 0, s, b, U, part);
     return hipGetLastError();
+}
+hipError_t launch_max_divb(const BlockDims& b, const double* U, double* part, hipStream_t s) {
+    return launch_max_divb(b, vars_of(U, b.nc), part, s);
 }
 hipError_t launch_reduce(const double* part, int n, int is_max, double* out, hipStream_t s) {
     hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(BS), 0, s, part, n, is_max, out);
@@ -1359,14 +1446,15 @@ hipError_t launch_set_clock(DeviceState* ds, double time, double dt, int ncycle,
     hipLaunchKernelGGL(set_clock_kernel, dim3(1), dim3(64), 0, s, ds, time, dt, ncycle);
     return hipGetLastError();
 }
-hipError_t launch_weighted_sum(const BlockDims& b, const double* Usub, const double* Ufull, const double* dUdt,
-                               double gam0, double gam1, double beta_dt, double* Uout, hipStream_t s) {
-    const long n = (long)NV * b.nc;
-    hipLaunchKernelGGL(weighted_sum_kernel, grid_for(n), dim3(BS), 0, s, n, Usub, Ufull, dUdt, gam0, gam1, beta_dt, Uout);
+hipError_t launch_weighted_sum(const BlockDims& b, const Vars<const double>& Usub, const Vars<const double>& Ufull,
+                               const Vars<const double>& dUdt, double gam0, double gam1, double beta_dt,
+                               const Vars<double>& Uout, hipStream_t s) {
+    const dim3 grid((unsigned)((b.nc + BS - 1) / BS), NV);
+    hipLaunchKernelGGL(weighted_sum_kernel, grid, dim3(BS), 0, s, b.nc, Usub, Ufull, dUdt, gam0, gam1, beta_dt, Uout);
     return hipGetLastError();
 }
-hipError_t launch_utop(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const double* U, double* P,
-                       int* pflag, int* fflag, hipStream_t s) {
+hipError_t launch_utop(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const Vars<const double>& U,
+                       const Vars<double>& P, int* pflag, int* fflag, hipStream_t s) {
     if (c.inverter == 1) {
         if (b.geo) hipLaunchKernelGGL((utop_kernel<1, 1>), grid_for(b.nc), dim3(BS), 0, s, b, c, gd, U, P, pflag, fflag);
         else hipLaunchKernelGGL((utop_kernel<0, 1>), grid_for(b.nc), dim3(BS), 0, s, b, c, gd, U, P, pflag, fflag);
@@ -1376,33 +1464,46 @@ hipError_t launch_utop(const BlockDims& b, const StepConsts& c, const GeomDev& g
     }
     return hipGetLastError();
 }
-hipError_t launch_floors(const BlockDims& b, const StepConsts& c, const GeomDev& gd, double* U, double* P, int* pflag,
-                         int* fflag, hipStream_t s) {
+hipError_t launch_floors(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const Vars<double>& U,
+                         const Vars<double>& P, int* pflag, int* fflag, hipStream_t s) {
     if (!c.floors.on) return hipSuccess;
     if (b.geo) hipLaunchKernelGGL(floors_kernel<1>, grid_for(b.nc), dim3(BS), 0, s, b, c, gd, U, P, pflag, fflag);
     else hipLaunchKernelGGL(floors_kernel<0>, grid_for(b.nc), dim3(BS), 0, s, b, c, gd, U, P, pflag, fflag);
     return hipGetLastError();
 }
-hipError_t launch_fixup_entire(const BlockDims& b, const StepConsts& c, const GeomDev& gd, double* U, double* P,
-                               const int* pflag, hipStream_t s) {
+hipError_t launch_fixup_entire(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const Vars<double>& U,
+                               const Vars<double>& P, const int* pflag, hipStream_t s) {
     if (!c.fix_average && !c.fix_atmo) return hipSuccess;
     hipLaunchKernelGGL(fixup_avg_kernel, grid_for(b.nc), dim3(BS), 0, s, b, c, P, pflag);
     if (b.geo) hipLaunchKernelGGL(fixup_floor_kernel<1>, grid_for(b.nc), dim3(BS), 0, s, b, c, gd, U, P, pflag);
     else hipLaunchKernelGGL(fixup_floor_kernel<0>, grid_for(b.nc), dim3(BS), 0, s, b, c, gd, U, P, pflag);
     return hipGetLastError();
 }
-hipError_t launch_ptou(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const double* P, double* U,
-                       hipStream_t s) {
+hipError_t launch_ptou(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const Vars<const double>& P,
+                       const Vars<double>& U, hipStream_t s) {
     if (b.geo) hipLaunchKernelGGL(ptou_kernel<1>, grid_for(b.nc), dim3(BS), 0, s, b, c, gd, P, U);
     else hipLaunchKernelGGL(ptou_kernel<0>, grid_for(b.nc), dim3(BS), 0, s, b, c, gd, P, U);
     return hipGetLastError();
 }
-hipError_t launch_sync(const BlockDims& b, double* A, int nvar, hipStream_t s) {
-    hipLaunchKernelGGL(sync_kernel<double>, grid_for(b.nc * nvar), dim3(BS), 0, s, b, A, nvar);
+hipError_t launch_ptou(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const double* P, double* U,
+                       hipStream_t s) {
+    return launch_ptou(b, c, gd, vars_of(P, b.nc), vars_of(U, b.nc), s);
+}
+hipError_t launch_sync(const BlockDims& b, const Vars<double>& A, int nvar, hipStream_t s) {
+    if (nvar < 1 || nvar > NV) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)((b.nc + BS - 1) / BS), (unsigned)nvar);
+    hipLaunchKernelGGL(sync_kernel<double>, grid, dim3(BS), 0, s, b, A);
     return hipGetLastError();
 }
+hipError_t launch_sync(const BlockDims& b, double* A, int nvar, hipStream_t s) {
+    for (int v0 = 0; v0 < nvar; v0 += NV) {
+        const hipError_t e = launch_sync(b, vars_of(A + (long)v0 * b.nc, b.nc), nvar - v0 < NV ? nvar - v0 : NV, s);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
 hipError_t launch_sync_int(const BlockDims& b, int* A, hipStream_t s) {
-    hipLaunchKernelGGL(sync_kernel<int>, grid_for(b.nc), dim3(BS), 0, s, b, A, 1);
+    hipLaunchKernelGGL(sync_kernel<int>, grid_for(b.nc), dim3(BS), 0, s, b, vars_of(A, b.nc));
     return hipGetLastError();
 }
 hipError_t launch_estimate_dt(const BlockDims& b, const StepConsts& c, int prec, const void* cmax, const void* cmin,

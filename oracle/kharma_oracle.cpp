@@ -184,6 +184,9 @@ struct Block {
     std::vector<Real> emf;   // [3][...]
     std::vector<Real> dUdt;  // [NV][...]
     std::vector<Real> pflag, fflag, rho_floor, u_floor;
+    // First-order flux correction, last call of fofc(): interior marks, the trial
+    // primitives and each trial state's relative distance to its nearest floor threshold
+    std::vector<Real> fofc_mark, fofc_trial, fofc_margin;
     double time = 0., dt = 0., dt_last = 0.;
     long ncycle = 0;
     bool in_loop = false;
@@ -1507,11 +1510,35 @@ void integrator_weights(int integrator, int stage, Real& gam0, Real& gam1, Real&
 // zones take those of their periodic images or, at physical faces, of the
 // nearest interior zone, as the outflow/reflecting fill of a scalar gives); every
 // face next to a marked zone gets the LLF flux of the two zones' own primitives.
+// Relative distance of the primitives P from the nearest threshold that
+// determine_floors compares them with (test instrumentation: a zone within
+// rounding of a threshold takes its floor flag from the last bits of P).
+Real floor_margin(const Block& b, const Real* P, int j, int i) {
+    const auto& f = b.p;
+    Real rg, ug;
+    geom_floors(b, j, i, rg, ug);
+    FourVectors D;
+    calc_4vecs(b, P, j, i, CENTER, D);
+    const Real bsq = dot(D.bcon, D.bcov);
+    auto rel = [](Real x, Real y) {
+        const Real s = std::max(std::fabs(val(x)), std::fabs(val(y)));
+        return s > 0 ? Real(std::fabs(val(x) - val(y)) / s) : Real(1.);
+    };
+    Real m = std::min(rel(rg, P[RHO]), rel(ug, P[UU]));
+    m = std::min(m, std::min(rel(bsq / f.bsq_over_rho_max, P[RHO]), rel(bsq / f.bsq_over_u_max, P[UU])));
+    m = std::min(m, rel(std::max(P[UU], std::max(ug, bsq / f.bsq_over_u_max)) / f.u_over_rho_max, P[RHO]));
+    m = std::min(m, rel(lorentz_calc(b, P, j, i, CENTER), f.gamma_max));
+    return m;
+}
+
 void fofc(Block& b, int stage, const std::vector<Real>& Usub, const std::vector<Real>& Psub) {
     Real gam0, gam1, beta;
     integrator_weights(b.p.integrator, stage, gam0, gam1, beta);
     flux_divergence(b);
     std::vector<int> mark(b.nc, 0);
+    b.fofc_mark.assign(b.nc, 0.);
+    b.fofc_trial.assign((size_t)NV * b.nc, 0.);
+    b.fofc_margin.assign(b.nc, 1.);
     const Real r_eh = 1. + std::sqrt(1. - b.p.a * b.p.a);
 #pragma omp parallel for collapse(2)
     for (int k = b.ks; k <= b.ke; ++k)
@@ -1537,6 +1564,9 @@ void fofc(Block& b, int stage, const std::vector<Real>& Usub, const std::vector<
         if (b.p.spherical && !b.p.periodic2 && b.p.fofc_polar_cells > 0 &&
             (j < b.js + b.p.fofc_polar_cells || j > b.je - b.p.fofc_polar_cells)) m = true;
         mark[c] = m;
+        b.fofc_mark[c] = m;
+        for (int v = 0; v < NV; ++v) b.fofc_trial[v * b.nc + c] = Pg[v];
+        if (b.p.floors_on) b.fofc_margin[c] = floor_margin(b, Pg, j, i);
     }
     auto src = [&](int v, int s0, int e0, int n, bool periodic) {
         if (v >= s0 && v <= e0) return v;
@@ -1795,6 +1825,9 @@ int or_step(void* h, int nsteps, double* dts) {
 // Task-level entry points on the base container.
 void or_get_flux(void* h, int dir) { Block& b = *static_cast<Block*>(h); get_flux(b, b.P0, dir); }
 void or_flux_ct(void* h) { flux_ct(*static_cast<Block*>(h)); }
+// The first-order flux correction of stage 1 from the block's state (U_sub = U_full = U0, P_sub = P0)
+// on the fluxes of the preceding or_get_flux calls, which it rewrites with cmax/cmin.
+void or_fofc_stage1(void* h) { Block& b = *static_cast<Block*>(h); fofc(b, 1, b.U0, b.P0); }
 void or_flux_divergence(void* h) { flux_divergence(*static_cast<Block*>(h)); }
 void or_add_geo_source(void* h) { Block& b = *static_cast<Block*>(h); add_geo_source(b, b.P0); }
 void or_utop(void* h) { Block& b = *static_cast<Block*>(h); utop(b, b.U0, b.P0); }
@@ -1805,7 +1838,8 @@ double or_estimate_timestep(void* h) { return estimate_timestep(*static_cast<Blo
 double or_max_divb(void* h) { Block& b = *static_cast<Block*>(h); return max_divb(b, b.U0); }
 
 // Copy out internal arrays: which = 0 F (3*NV*nc), 1 cmax (3*nc), 2 cmin (3*nc),
-// 3 dUdt (NV*nc), 4 pflag (nc), 5 fflag (nc), 6 emf (3*nc), 7 Pl, 8 Pr (NV*nc)
+// 3 dUdt (NV*nc), 4 pflag (nc), 5 fflag (nc), 6 emf (3*nc), 7 Pl, 8 Pr (NV*nc),
+// after fofc(): 9 marks (nc), 10 trial primitives (NV*nc), 11 floor margins (nc)
 void or_get_array(void* h, int which, double* out) {
     Block& b = *static_cast<Block*>(h);
     const std::vector<Real>* src = nullptr;
@@ -1813,6 +1847,7 @@ void or_get_array(void* h, int which, double* out) {
     case 0: src = &b.F; break; case 1: src = &b.cmax; break; case 2: src = &b.cmin; break;
     case 3: src = &b.dUdt; break; case 4: src = &b.pflag; break; case 5: src = &b.fflag; break;
     case 6: src = &b.emf; break; case 7: src = &b.Pl; break; case 8: src = &b.Pr; break;
+    case 9: src = &b.fofc_mark; break; case 10: src = &b.fofc_trial; break; case 11: src = &b.fofc_margin; break;
     default: return;
     }
     store_values(*src, out, src->size());

@@ -87,7 +87,7 @@ def lib(variant=""):
         L.or_step.argtypes = [vp, i, dp]
         L.or_get_flux.argtypes = [vp, i]
         for name in ("or_flux_ct", "or_flux_divergence", "or_utop", "or_apply_floors", "or_fix_utop", "or_ptou",
-                     "or_add_geo_source"):
+                     "or_add_geo_source", "or_fofc_stage1"):
             getattr(L, name).argtypes = [vp]
         L.or_estimate_timestep.restype = d
         L.or_estimate_timestep.argtypes = [vp]
@@ -258,7 +258,8 @@ class OracleBlock:
         return self.L.or_get_time(self.h)
 
     def array(self, which):
-        sizes = {0: (3, NV), 1: (3,), 2: (3,), 3: (NV,), 4: (), 5: (), 6: (3,), 7: (NV,), 8: (NV,)}
+        sizes = {0: (3, NV), 1: (3,), 2: (3,), 3: (NV,), 4: (), 5: (), 6: (3,), 7: (NV,), 8: (NV,),
+                 9: (), 10: (NV,), 11: ()}
         out = np.empty(sizes[which] + self.shape)
         self.L.or_get_array(self.h, which, _ptr(out))
         return out
@@ -272,6 +273,11 @@ class OracleBlock:
 
     def flux_ct(self):
         self.L.or_flux_ct(self.h)
+
+    def fofc_stage1(self):
+        """First-order flux correction of stage 1 on the current fluxes; marks,
+        trial primitives and floor margins then read as array(9), (10), (11)."""
+        self.L.or_fofc_stage1(self.h)
 
     def flux_divergence(self):
         self.L.or_flux_divergence(self.h)
