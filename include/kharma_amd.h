@@ -201,6 +201,17 @@ int kh_estimate_timestep(kh_block* b, const void* cmax, const void* cmin, double
 /* B_FluxCT::MaxDivB (b_flux_ct.cpp:566-607).  Synchronises the stream. */
 int kh_max_divb(kh_block* b, const double* U, double* out);
 
+/* kh_estimate_timestep for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  The same estimate and clip, with the
+ * per-work-group partials in a static device array of 65536 doubles (512 KiB
+ * per GPU, held until the process ends) instead of the block's buffer: the
+ * block's interior may hold at most 65536 * 256 zones (KH_ERR_INVALID beyond).
+ * The clipped timestep is written to the device double *ndt_device; the call
+ * neither allocates nor synchronises.  Calls that overlap on one GPU share the
+ * array and race. */
+int kh_estimate_timestep_noalloc(kh_block* b, const void* cmax, const void* cmin, double dt_last, double* ndt_device);
+
 /* ---- Pack view: the variables where KHARMA keeps them ---------------------
  * KHARMA holds each variable group as its own Parthenon field: prims.rho,
  * prims.u, prims.uvec (3 components), prims.B (3) and the cons.* equivalents

@@ -128,6 +128,7 @@ def lib():
         "kh_torus_prims": (i, [ctypes.POINTER(KhParams), ctypes.POINTER(KhTorusParams), vp]),
         "kh_geometry_host": (i, [ctypes.POINTER(KhParams), i, i, vp]),
         "kh_estimate_timestep": (i, [vp, vp, vp, d, ctypes.POINTER(d)]),
+        "kh_estimate_timestep_noalloc": (i, [vp, vp, vp, d, vp]),
         "kh_max_divb": (i, [vp, vp, ctypes.POINTER(d)]),
         "kh_driver_create": (i, [ctypes.POINTER(KhParams), pvp]),
         "kh_driver_destroy": (None, [vp]),
@@ -423,6 +424,12 @@ class Block:
         out = ctypes.c_double()
         _check(lib().kh_estimate_timestep(self.h, _ptr(cmax), _ptr(cmin), dt_last, ctypes.byref(out)))
         return out.value
+
+    def estimate_timestep_noalloc(self, cmax, cmin, dt_last, out):
+        """kh_estimate_timestep_noalloc: the clipped timestep into the device tensor `out`
+        (one float64), asynchronously, with no allocation."""
+        self._bind()
+        _check(lib().kh_estimate_timestep_noalloc(self.h, _ptr(cmax), _ptr(cmin), dt_last, _ptr(out)))
 
     def max_divb(self, U):
         self._bind()

@@ -493,6 +493,15 @@ int kh_estimate_timestep(kh_block* b, const void* cmax, const void* cmin, double
     return KH_OK;
 }
 
+int kh_estimate_timestep_noalloc(kh_block* b, const void* cmax, const void* cmin, double dt_last, double* ndt) {
+    if (!b || !cmax || !cmin || !ndt) return fail(KH_ERR_INVALID, "null argument");
+    if (update_blocks(b->dims) > KH_NOALLOC_PARTIALS)
+        return fail(KH_ERR_INVALID, "block interior exceeds the static scratch of kh_estimate_timestep_noalloc (" +
+                                        std::to_string(KH_NOALLOC_PARTIALS * 256) + " zones)");
+    KH_HIP(launch_estimate_dt_noalloc(b->dims, b->consts, b->p.precision, cmax, cmin, dt_last, ndt, b->stream));
+    return KH_OK;
+}
+
 int kh_max_divb_v(kh_block* b, const kh_vars* U, double* out) {
     if (!b || !vars_ok(U) || !out) return fail(KH_ERR_INVALID, "null argument");
     if (b->dims.ndim < 2) { *out = 0.; return KH_OK; }

@@ -305,5 +305,14 @@ hipError_t launch_adj_floors(const BlockDims& b, const StepConsts& c, const Geom
 hipError_t launch_adj_count_flags(const BlockDims& b, const int* pflag, const int* fflag, int* err, hipStream_t s);
 hipError_t launch_estimate_dt(const BlockDims& b, const StepConsts& c, int prec, const void* cmax, const void* cmin,
                               double* part, hipStream_t s);
+// kh_estimate_timestep_noalloc: the estimate with its partials in a static device
+// array of KH_NOALLOC_PARTIALS doubles (one per work-group of BS interior
+// zones) and the clipped timestep written to the device double ndt; no
+// allocation, no synchronisation.  hipErrorInvalidValue when the block needs
+// more partials than the array holds.
+constexpr long KH_NOALLOC_PARTIALS = 1L << 16;
+constexpr int KH_MAX_DEVICES = 64;
+hipError_t launch_estimate_dt_noalloc(const BlockDims& b, const StepConsts& c, int prec, const void* cmax,
+                                      const void* cmin, double dt_last, double* ndt, hipStream_t s);
 
 }  // namespace kh

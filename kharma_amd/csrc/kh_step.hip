@@ -11,6 +11,8 @@
 //   + (last stage) the CFL estimate (GRMHD::EstimateTimestep, grmhd.cpp:313-328)
 // and writes U, P and the flags once, also into the periodic ghost images of
 // the zone (the boundary exchange), so no separate sync pass reads the state.
+#include <mutex>
+
 #include "kh_internal.hpp"
 
 #include <cfloat>
@@ -1513,6 +1515,64 @@ hipError_t launch_estimate_dt(const BlockDims& b, const StepConsts& c, int prec,
     else hipLaunchKernelGGL(estimate_dt_kernel<double>, g, dim3(BS), 0, s, b, (const double*)cmax, (const double*)cmin, part);
     return hipGetLastError();
 }
+// ---------------------------------------------- allocation-free timestep estimate
+// Scratch of kh_estimate_timestep_noalloc: one CFL minimum per work-group of
+// estimate_dt_kernel.  Each GPU holds its own copy for the life of the process;
+// calls on one GPU share it, so calls that overlap on one GPU race.  Every
+// partial a call reads is written by the same call, so nothing needs clearing.
+static __device__ __align__(256) double scratch[KH_NOALLOC_PARTIALS];
+
+// The reduced minimum clipped as kh_estimate_timestep does on the host:
+// n = m cfl, then min(max(dt_min, n), max_dt_increase dt_last).
+__global__ void __launch_bounds__(BS) reduce_dt_clip_kernel(const double* __restrict__ part, int n, double cfl,
+                                                            double dt_min, double hi, double* __restrict__ out) {
+    double v = DBL_MAX;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) v = fmin(v, part[q]);
+    v = block_min(v);
+    if (threadIdx.x == 0) {
+        double w = v * cfl;
+        w = (w < dt_min) ? dt_min : w;
+        w = (hi < w) ? hi : w;
+        *out = w;
+    }
+}
+
+namespace {
+std::mutex g_scratch_mu;
+double* g_scratch_dev[KH_MAX_DEVICES] = {};
+}  // namespace
+
+// Device address of `scratch` on the current device, resolved once per device.
+hipError_t noalloc_scratch(double** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= KH_MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_dev[dev] = (double*)p;
+    }
+    *out = g_scratch_dev[dev];
+    return hipSuccess;
+}
+
+hipError_t launch_estimate_dt_noalloc(const BlockDims& b, const StepConsts& c, int prec, const void* cmax,
+                                      const void* cmin, double dt_last, double* ndt, hipStream_t s) {
+    double* part = nullptr;
+    hipError_t e = noalloc_scratch(&part);
+    if (e != hipSuccess) return e;
+    const long n = update_blocks(b);
+    if (n > KH_NOALLOC_PARTIALS) return hipErrorInvalidValue;
+    e = launch_estimate_dt(b, c, prec, cmax, cmin, part, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(reduce_dt_clip_kernel, dim3(1), dim3(BS), 0, s, part, (int)n, c.cfl, c.dt_min,
+                       c.max_dt_increase * dt_last, ndt);
+    return hipGetLastError();
+}
+
 hipError_t launch_convert_f32(const double* A, float* B, long n, hipStream_t s) {
     hipLaunchKernelGGL(convert_f32_kernel, grid_for(n), dim3(BS), 0, s, A, B, n);
     return hipGetLastError();
