@@ -85,12 +85,24 @@ def _kernel_io_bytes(kind, prec_bytes):
     return None
 
 
-def _cpu_baseline(cfg, P0, what, budget_s=12.0, max_steps=8):
+def _cpu_model():
+    """The host CPU's model name (from /proc/cpuinfo), or None."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def _cpu_baseline(cfg, P0, what, budget_s=12.0, max_steps=8, warm=1):
     from oracle import oracle
     oracle.build()
     ob = oracle.OracleBlock(cfg)
     ob.init_state(P0)
-    ob.step(1)  # warm-up (first touch, dt ramp start)
+    ob.step(warm)  # warm-up (first touch, dt ramp start)
     nz = cfg["nx"][0] * cfg["nx"][1] * cfg["nx"][2]
     t0 = time.perf_counter()
     n = 0
@@ -99,8 +111,44 @@ def _cpu_baseline(cfg, P0, what, budget_s=12.0, max_steps=8):
         n += 1
     dt = time.perf_counter() - t0
     return dict(value=nz * n / dt, unit="zone-cycles/s", cores=oracle.num_threads(), kind="port",
-                sample=f"{n} RK2 steps of the same {what} after 1 warm-up step, "
+                cpu_model=_cpu_model(),
+                sample=f"{n} RK2 steps of the same {what} after {warm} warm-up step(s), "
                        f"CPU restatement of KHARMA's algorithm (C++ FP64, OpenMP over k,j)")
+
+
+def _other_configs(with_cpu):
+    """The CPU-runnable configurations beside the headline: C1 (2-D Orszag-Tang
+    256^2, FP64, 100 steps) and C2 at 64^3 (3-D fast mode, FP64 and split),
+    zone-cycles/s on the GPU (HIP events around the steps, after 2 warm-up
+    steps) and of the CPU restatement on a bounded sample (2 warm-up steps)."""
+    import torch
+    import kharma_amd as kh
+    from kharma_amd import problems as pr
+    out = {}
+    runs = [("c1_orszag_tang_256", pr.orszag_tang_config(n=256), pr.orszag_tang_prims, 100, ("fp64",)),
+            ("c2_fast_mode_64", pr.mhdmode_config(n=64, ndim=3, mode="fast", one_period=False), pr.mhdmode_prims, 50,
+             ("fp64", "split"))]
+    for name, cfg, gen, steps, forms in runs:
+        P0 = gen(cfg)
+        nz = cfg["nx"][0] * cfg["nx"][1] * cfg["nx"][2]
+        rec = {"zones": nz, "steps": steps}
+        for f in forms:
+            d = kh.Driver(dict(cfg, precision=f))
+            d.init_prims(torch.from_numpy(P0).cuda())
+            d.step(2)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            e0.record()
+            d.step(steps)
+            e1.record()
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1)
+            rec[f"gpu_{f}"] = {"zone_cycles_per_s": nz * steps / (ms * 1e-3), "ms_per_step": ms / steps}
+        if with_cpu:
+            rec["cpu"] = _cpu_baseline(cfg, P0, name.replace("_", " "), budget_s=6.0, max_steps=steps, warm=2)
+            rec["gpu_fp64_over_cpu"] = rec["gpu_fp64"]["zone_cycles_per_s"] / rec["cpu"]["value"]
+        out[name] = rec
+    return out
 
 
 def main():
@@ -259,6 +307,7 @@ def main():
         out["cpu_baseline"] = _cpu_baseline(cfg, P0, what)
     else:
         out["cpu_baseline"] = None
+    out["other_configs"] = _other_configs(not a.no_cpu)
     print(json.dumps(out))
     if a.copy_mb:
         _calibration_copy(a.copy_mb)
@@ -508,37 +557,14 @@ def _torus_objective_smooth_check(cfg, face, u0, seed, nsteps=6, every=3, ndirs=
     recovery converged to 1e-14; nsteps steps with checkpoints every `every`
     (segment re-runs); smooth directions in all variables, each scaled by the
     variable's local magnitude."""
-    import math
     import numpy as np
     import torch
     from kharma_amd import problems as pr
-    from kharma_amd.adjoint import Evolution, evolve_objective
+    from kharma_amd.adjoint import Evolution, evolve_objective, smooth_torus_state
 
     cs = dict(cfg, err_tol=1e-14, iter_max=30)
-    inner = (slice(None),) + pr.interior(cs)
-    x, y, _ = pr.cell_centers(cs)
-    X2, X1 = np.meshgrid(y, x, indexing="ij")
-    L1 = cs["bounds"][0][1] - cs["bounds"][0][0]
     rng = np.random.default_rng(seed + 1)
-
-    def modes():
-        v = np.zeros((1,) + X1.shape)
-        for _m in range(3):
-            k1, k2 = rng.integers(1, 4, size=2)
-            v[0] += rng.normal() * np.cos(2 * math.pi * k1 * X1 / L1 + rng.uniform(0, 2 * math.pi)) * \
-                np.cos(math.pi * k2 * X2 + rng.uniform(0, 2 * math.pi))
-        return v
-    P = pr.torus_prims(cs)[inner].copy()
-    bmax = np.abs(P[pr.B1:]).max()
-    P[pr.RHO] *= 1 + 1e-3 * modes()[0]
-    P[pr.UU] *= 1 + 1e-3 * modes()[0]
-    for q in (pr.U1, pr.U2, pr.U3):
-        P[q] += 1e-3 * modes()[0]
-    for q in (pr.B1, pr.B2, pr.B3):
-        P[q] += 1e-6 * bmax * modes()[0]
-    atm = P[pr.RHO] < 1e-2
-    P[pr.RHO][atm] *= 1.5
-    P[pr.UU][atm] *= 1.5
+    P, direction = smooth_torus_state(cs, rng)
     P0 = torch.as_tensor(P, device="cuda")
     evo = Evolution(cs, objective=dict(dir=1, var=pr.RHO, face=face, weight=1.0))
     dts = evo.run(P0, nsteps)[1]
@@ -546,14 +572,9 @@ def _torus_objective_smooth_check(cfg, face, u0, seed, nsteps=6, every=3, ndirs=
     _, J = evolve_objective(Pg, evo, nsteps, dts, every=every)
     J.backward()
     g = Pg.grad.cpu().numpy()
-    scale = np.empty_like(P)
-    scale[pr.RHO] = np.abs(P[pr.RHO])
-    scale[pr.UU] = np.abs(P[pr.UU])
-    scale[pr.U1:pr.B1] = np.abs(P[pr.U1:pr.B1]) + 0.1
-    scale[pr.B1:] = np.abs(P[pr.B1:]) + 1e-6 * bmax
     fd, ad = [], []
     for _ in range(ndirs):
-        v = np.concatenate([scale[q:q + 1] * modes() for q in range(pr.NV)])
+        v = direction()
         vt = torch.as_tensor(v, device="cuda")
         fd.append((evo.run(P0 + h * vt, nsteps, dts)[3] - evo.run(P0 - h * vt, nsteps, dts)[3]) / (2 * h))
         ad.append(float((g * v).sum()))

@@ -259,3 +259,47 @@ def generic_state(cfg, rng, amp=1e-3, base=None):
         scale = max(abs(P[q]).max(), 0.1)
         P[q] = P[q] + amp * scale * np.cos(2 * math.pi * (k[0] * X / L[0] + k[1] * Y / L[1] + k[2] * Z / L[2]) + ph)
     return P
+
+
+def smooth_torus_state(cfg, rng):
+    """A torus state (interior, numpy) on which a multi-step objective is
+    differentiable in every direction, for gradient checks: the torus with its
+    atmosphere at 1.5 times its floors (no zone floored for a few steps), small
+    smooth modes in every variable (the bare atmosphere is uniform in theta,
+    where WENO5's smoothness indicators vanish and finite differences measure
+    the weights of the perturbation).  Returns (P, direction), where
+    direction() draws a random smooth direction in all variables, each scaled by
+    the variable's local magnitude.  Run it with the recovery converged tightly
+    (err_tol 1e-14) so the inversion's own tolerance does not limit the check."""
+    x, y, _ = pr.cell_centers(cfg)
+    X2, X1 = np.meshgrid(y, x, indexing="ij")
+    L1 = cfg["bounds"][0][1] - cfg["bounds"][0][0]
+    inner = (slice(None),) + pr.interior(cfg)
+
+    def modes():
+        v = np.zeros((1,) + X1.shape)
+        for _m in range(3):
+            k1, k2 = rng.integers(1, 4, size=2)
+            v[0] += rng.normal() * np.cos(2 * math.pi * k1 * X1 / L1 + rng.uniform(0, 2 * math.pi)) * \
+                np.cos(math.pi * k2 * X2 + rng.uniform(0, 2 * math.pi))
+        return v
+    P = pr.torus_prims(cfg)[inner].copy()
+    bmax = np.abs(P[pr.B1:]).max()
+    P[pr.RHO] *= 1 + 1e-3 * modes()[0]
+    P[pr.UU] *= 1 + 1e-3 * modes()[0]
+    for q in (pr.U1, pr.U2, pr.U3):
+        P[q] += 1e-3 * modes()[0]
+    for q in (pr.B1, pr.B2, pr.B3):
+        P[q] += 1e-6 * bmax * modes()[0]
+    atm = P[pr.RHO] < 1e-2
+    P[pr.RHO][atm] *= 1.5
+    P[pr.UU][atm] *= 1.5
+    scale = np.empty_like(P)
+    scale[pr.RHO] = np.abs(P[pr.RHO])
+    scale[pr.UU] = np.abs(P[pr.UU])
+    scale[pr.U1:pr.B1] = np.abs(P[pr.U1:pr.B1]) + 0.1
+    scale[pr.B1:] = np.abs(P[pr.B1:]) + 1e-6 * bmax
+
+    def direction():
+        return np.concatenate([scale[q:q + 1] * modes() for q in range(pr.NV)])
+    return P, direction

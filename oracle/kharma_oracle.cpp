@@ -691,7 +691,8 @@ void flux_divergence(Block& b) {
 // A deliberate mistake planted in the restatement by a test (or_plant), to show
 // that the test's pin fails on it.  0: none.  1: the geometric source contracts
 // T^mu_nu with Gamma^lam_{nu mu} (upper and first lower index exchanged) instead
-// of Gamma^nu_{lam mu}.
+// of Gamma^nu_{lam mu}.  2: the drift-frame floors solve for the parallel
+// velocity with the pre-floor enthalpy.
 static int g_plant = 0;
 
 // Flux::AddGeoSource, flux/flux.cpp:395-444.  Skipped in Cartesian Minkowski (:407).
@@ -1121,7 +1122,8 @@ int apply_floors_drift(const Block& b, Real* P, int j, int i, Real rhoflr_max, R
     P[UU] = std::max(uu, uflr_max);
     const Real pg_new = (gam - 1.) * P[UU];
     const Real w_new = P[RHO] + P[UU] + pg_new;
-    const Real x = (2. * QdotB) / (B_mag * w_new * ucon_dr[0]);
+    // plant 2: the pre-floor enthalpy in the parallel-momentum solve
+    const Real x = (2. * QdotB) / (B_mag * (g_plant == 2 ? w_old : w_new) * ucon_dr[0]);
     vpar = x / (1 + std::sqrt(1 + x*x)) * (1. / ucon_dr[0]);
     D.ucon[0] = 1. / std::sqrt(1/(ucon_dr[0]*ucon_dr[0]) - vpar*vpar);
     for (int mu = 0; mu < 4; ++mu) D.ucon[mu] = ucon_dr[mu] * (D.ucon[0] / ucon_dr[0]) + (vpar * Bcon[mu] * D.ucon[0] / B_mag);

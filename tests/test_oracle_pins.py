@@ -20,8 +20,11 @@ here in numpy from the oracle's own (separately pinned) metric:
 - KBoundaries::FixFlux (boundaries.cpp:656-689): the X1 mass flux through the
   inner face is clamped to <= 0, through the outer face to >= 0, and polar
   fluxes are zeroed, with every other flux unchanged.
+- the drift-frame floors (floors/floors_functions.hpp:186-257) change the
+  coordinate velocity only along B and conserve the momentum along B.
 The first two are also run with a transposed contraction planted in the oracle
-(oracle.planted) to show that they fail on it.
+(oracle.planted) to show that they fail on it, and the last with the pre-floor
+enthalpy planted in its parallel-momentum solve.
 """
 import math
 
@@ -387,3 +390,60 @@ def test_x1_mass_flux_clamp_and_polar_zeroing():
     inner[:, je + 1, :] = False
     for v in range(5):
         assert np.array_equal(Ff[1, v][inner], F[1, v][inner])
+
+
+# ------------------------------------------------------ drift-frame floors
+def test_drift_floor_moves_velocity_along_field_and_keeps_parallel_momentum():
+    """Drift-frame injection (floors/floors_functions.hpp:186-257, Ressler et al.
+    2017 App. B) on a Kerr-Schild block: the floored zone's rho and u come up to
+    the floors, its coordinate velocity V^i = u^i / u^0 changes only along the
+    field (the drift velocity, V minus its part along B^i, is kept), and the
+    momentum along the field, B^mu Q_mu with Q_mu = w u^0 u_mu + p delta^0_mu,
+    is conserved with the new enthalpy.  Those two invariants and the floors fix
+    the new state; a slip in the algebra (a sign, a missing u^0 factor, the old
+    enthalpy) breaks one of them."""
+    oracle = _oracle()
+    cfg = dict(pr.torus_config(16, 16, 1), floor_frame="drift")
+    x1 = math.log(8.0)
+    cfg["bounds"] = ((x1, x1 + 0.5), (0.25, 0.75), cfg["bounds"][2])
+    cfg["fix_corner_inner"] = False
+    ob = oracle.OracleBlock(cfg)
+    g, gi, gd = centre_metric(ob)
+    rng = np.random.default_rng(11)
+    shape = ob.shape
+    P = random_state(shape, g, rng)
+    P[RHO] *= 1e-9                      # every zone far below the geometric density floor
+    P[UU] *= 1e-11
+    ob.set_state(P, conserved(P, g, gi, gd, cfg["gamma"]))
+    ob.apply_floors()
+    Pn, _ = ob.state()
+    ng = cfg["ng"]
+    sl = (slice(None), 0, slice(ng, -ng), slice(ng, -ng))
+    P0, P1 = P[sl], Pn[sl]
+    gs, gis = g[ng:-ng, ng:-ng], gi[ng:-ng, ng:-ng]
+    assert np.all(P1[RHO] > P0[RHO])
+    assert np.array_equal(P1[B1:], P0[B1:])
+    u0, c0, _, _ = fourvecs(P0, gs, gis)
+    u1, c1, _, _ = fourvecs(P1, gs, gis)
+    V0 = u0[..., 1:] / u0[..., :1]
+    V1 = u1[..., 1:] / u1[..., :1]
+    B = np.moveaxis(P0[B1:], 0, -1)
+    dV = V1 - V0
+    c = np.einsum("...i,...i->...", dV, B) / np.einsum("...i,...i->...", B, B)
+    perp = dV - c[..., None] * B
+    assert np.abs(perp).max() <= 1e-12 * np.abs(V0).max(), np.abs(perp).max()
+    assert np.abs(dV).max() > 1e-6            # the floors did move the velocity
+
+    def qdotb(Pz, ucon, ucov):
+        pg = (cfg["gamma"] - 1.0) * Pz[UU]
+        w = Pz[RHO] + Pz[UU] + pg
+        return w * ucon[..., 0] * np.einsum("...i,...i->...", ucov[..., 1:], np.moveaxis(Pz[B1:], 0, -1))
+    q0, q1 = qdotb(P0, u0, c0), qdotb(P1, u1, c1)
+    assert np.all(np.abs(q1 - q0) <= 1e-11 * np.abs(q0) + 1e-300), np.abs(q1 - q0).max()
+    # the pre-floor enthalpy planted in the parallel solve breaks the momentum pin
+    with oracle.planted(2):
+        ob.set_state(P, conserved(P, g, gi, gd, cfg["gamma"]))
+        ob.apply_floors()
+        Pb = ob.state()[0][sl]
+    ub, cb, _, _ = fourvecs(Pb, gs, gis)
+    assert np.abs(qdotb(Pb, ub, cb) - q0).max() > 1e-3 * np.abs(q0).max()
