@@ -239,9 +239,12 @@ def test_torus_step_parity_fp64_per_step(oracle_lib, case):
         # the oracle's own FMA-contracted build differs from it by that much after
         # one step.  The GPU (which contracts) may differ by no more than 10x that;
         # density, energy and field stay within the 1e-12 bar.
-        for A, Ao, Af in ((Pg, Po, Pf), (Ug, Uo, Uf)):
+        for name, A, Ao, Af in (("P", Pg, Po, Pf), ("U", Ug, Uo, Uf)):
             sens = rel_l1(interior_of(Af, cfg), interior_of(Ao, cfg))
             r = rel_l1(interior_of(A, cfg), interior_of(Ao, cfg))
+            print(f"[torus step] {case} from step {n}: {name} rel L1 GPU vs oracle "
+                  f"{np.array2string(r, precision=2)}; oracle FMA vs no-FMA {np.array2string(sens, precision=2)}",
+                  flush=True)
             assert np.all(r < np.maximum(1e-12, 10 * sens)), (n, r, sens)
             assert np.all(r[[0, 1, 5, 6, 7]] < 1e-12), (n, r)
             # ghost zones (physical boundaries and periodic images) too
