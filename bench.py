@@ -36,6 +36,24 @@ METRIC = "zone-cycles/s on 1 MI355X: FP32-split vs FP64 plain; adjoint/forward t
 HBM_PEAK_GBS = 8000.0
 
 
+def _bench_mesh(cfg, P0, nb, steps, warmup):
+    """zone-cycles timing of the same workload split into nb blocks (kh_mesh_*):
+    HIP events on the stream around `steps` steps after `warmup`."""
+    import torch
+    import kharma_amd as kh
+    m = kh.Mesh(cfg, nb)
+    m.init_prims(torch.from_numpy(P0).cuda())
+    m.step(warmup)
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    m.step(steps)
+    stop.record()
+    torch.cuda.synchronize()
+    m.check()
+    return start.elapsed_time(stop)
+
+
 def _bench_form(cfg, P0, steps, warmup):
     import torch
     import kharma_amd as kh
@@ -223,6 +241,15 @@ def main():
         fofc = {"options": "fofc/on, eh_buffer 0.1, polar_cells 0", "steps": nf, "warmup": wf,
                 "ms_per_step": rf["ms"] / nf, "plain_ms_per_step": rb["ms"] / nf,
                 "over_plain": rf["ms"] / rb["ms"]}
+    # the benchmark's own block layout: 192x128x128 as 16 blocks of 96x64x32
+    layout = None
+    if a.workload == "torus":
+        nb = (2, 2, 4)
+        ms = {f: _bench_mesh(dict(cfg, precision=f), P0, nb, a.steps, a.warmup) for f in ("fp64", "split")}
+        layout = {"blocks": "2x2x4 blocks of 96x64x32 (ghost zones per block 4, each block's own)",
+                  "fp64": {"zone_cycles_per_s": nz * a.steps / (ms["fp64"] * 1e-3), "ms_per_step": ms["fp64"] / a.steps},
+                  "split": {"zone_cycles_per_s": nz * a.steps / (ms["split"] * 1e-3),
+                            "ms_per_step": ms["split"] / a.steps}}
     adj = None if a.no_adjoint else _bench_adjoint()
     adj_gr = None if a.no_adjoint else _bench_adjoint_torus()
     zc64 = nz * a.steps / (r64["ms"] * 1e-3)
@@ -284,13 +311,15 @@ def main():
         "vs_baseline": None,
         "dtype": "f32/f64",
         "data": "synthetic",
-        "config": {"workload": desc, "zones": nz, "ghosts": cfg["ng"], "forms": ["fp64", "split"]},
+        "config": {"workload": desc, "zones": nz, "ghosts": cfg["ng"], "forms": ["fp64", "split"],
+                   "layout": "one meshblock (value); the 16-block layout is reported in layout_16_blocks"},
         "fp64": {"value": zc64, "ms_per_step": r64["ms"] / a.steps, "dtype": "f64"},
         "split_over_fp64": zc32 / zc64,
         "split_rel_l1_vs_fp64": {"after_steps": a.steps + a.warmup, "max": float(rel.max()),
                                  "per_var": [float(x) for x in rel],
                                  "one_step_max": float(rel1.max()), "one_step_per_var": [float(x) for x in rel1]},
         "fofc_fp64": fofc,
+        "layout_16_blocks": layout,
         "adjoint_over_forward": adj["adjoint_over_forward"] if adj else None,
         "adjoint": adj,
         "adjoint_torus": adj_gr,

@@ -326,6 +326,17 @@ int kh_driver_step(kh_driver* d, int nsteps);
 int kh_driver_set_clock(kh_driver* d, double time, double dt, int ncycle);
 /* Copy the base-container state out (device pointers; either may be NULL). */
 int kh_driver_get_state(kh_driver* d, double* P, double* U);
+/* Restart dump and reload: the fields KHARMA marks Metadata::Restart -- the
+ * primitive and conserved variables (driver/kharma_driver.cpp:114-118,
+ * grmhd/grmhd.cpp:155-156) -- over the entire block, and the clock (time, dt,
+ * ncycle; KHARMA::ResetGlobals, kharma.cpp:107-119).  Parthenon writes these
+ * to HDF5 (.rhdf); this is a flat binary file: a header (magic "KHAMDRST",
+ * version 1, entire extents, ncycle, time, dt, the run's kh_params) followed
+ * by P and U as [8][n3][n2][n1] doubles.  Loading checks the mesh, geometry and
+ * gamma against the driver's, restores U and P exactly (not U from P) and the
+ * clock, so the run continues bit for bit.  Both synchronise. */
+int kh_driver_save_restart(kh_driver* d, const char* path);
+int kh_driver_load_restart(kh_driver* d, const char* path);
 /* Inversion / floor flags of the last stage, int32 [nc] device arrays (either may be NULL):
  * the reference's pflag / fflag fields (inverter/inverter.cpp:104-108, floors/floors.cpp:111-116). */
 int kh_driver_get_flags(kh_driver* d, int* pflag, int* fflag);
@@ -374,6 +385,51 @@ int kh_driver_objective(kh_driver* d, double* value, int reset);
 /* Sum of launch durations (ms) and launch counts per kind since the last call;
  * synchronises and resets the accumulators. */
 int kh_driver_kernel_times(kh_driver* d, double* total_ms, int* launches, int nkinds);
+
+/* ---- Several meshblocks on one GPU ---------------------------------------
+ * A uniform mesh of nb1 x nb2 x nb3 equal meshblocks (parthenon/mesh/nx* split
+ * into parthenon/meshblock/nx*), advanced by MakeDefaultTaskCollection
+ * (driver/kharma_step.cpp:77-319) with the ghost zones of shared faces kept
+ * current as the boundary exchange does (KHARMADriver::AddBoundarySync,
+ * kharma_driver.cpp:138-183) and the timestep and divB reduced over all blocks
+ * (kharma_driver.cpp:439-460, b_flux_ct.cpp:566-643).  p describes the whole
+ * mesh (nx1..3 the mesh's zones, the mesh's bounds and boundaries); each block
+ * holds nx/nb zones per axis, at least 2 ng along a split axis.  Per face a
+ * block has either the mesh's physical boundary or its neighbour's images, the
+ * neighbour of the last block of a periodic axis being the first.  The blocks'
+ * states sit in one allocation per container, [block][8][nc], block index b1
+ * fastest; each block's update stores the zones it shares into its
+ * neighbours' ghosts, so no exchange pass runs.  The mesh state reads and
+ * writes as one block of the whole mesh ([8][N3+2ng][N2+2ng][N1+2ng]).  The
+ * same arithmetic per zone as the one-block driver: a mesh of several blocks
+ * gives the one-block result bit for bit.  The first-order flux correction
+ * runs on one-block meshes (kh_driver_*) only. */
+typedef struct kh_mesh kh_mesh;
+int kh_mesh_create(const kh_params* p, int nb1, int nb2, int nb3, kh_mesh** out);
+void kh_mesh_destroy(kh_mesh* m);
+int kh_mesh_set_stream(kh_mesh* m, void* hip_stream);
+/* block count and one block's entire sizes and zone count */
+int kh_mesh_shape(const kh_mesh* m, int* nblocks, int* n1, int* n2, int* n3, long* nc);
+/* Initial primitives of the whole mesh (device, mesh layout; the interior is
+ * read, with the ghosts of physical faces refilled): P->U, boundaries, dt_start. */
+int kh_mesh_init_prims(kh_mesh* m, const double* P);
+int kh_mesh_step(kh_mesh* m, int nsteps);
+int kh_mesh_set_clock(kh_mesh* m, double time, double dt, int ncycle);
+/* The base-container state in the mesh layout (device; either may be NULL):
+ * interior zones from their blocks, periodic ghosts as images, the ghosts of
+ * physical faces from the edge blocks. */
+int kh_mesh_get_state(kh_mesh* m, double* P, double* U);
+int kh_mesh_get_scalars(kh_mesh* m, double* time, double* dt, double* last_divb, int* ncycle, int* total_fails);
+int kh_mesh_get_dt_history(kh_mesh* m, double* out, int n);
+int kh_mesh_check(kh_mesh* m);
+/* Restart dump and reload of the mesh in the mesh layout: the same file as a
+ * one-block run of the mesh (kh_driver_save_restart), readable by either. */
+int kh_mesh_save_restart(kh_mesh* m, const char* path);
+int kh_mesh_load_restart(kh_mesh* m, const char* path);
+/* Block b's base-container arrays ([8][nc] each) and its context, for the task
+ * entry points on one block of the mesh. */
+int kh_mesh_block_arrays(kh_mesh* m, int b, double** U, double** P);
+int kh_mesh_block(kh_mesh* m, int b, kh_block** out);
 
 /* ------------------------------------------------------------------------
  * Differentiation form: the reverse-mode adjoint of one driver step.  No

@@ -167,6 +167,25 @@ def lib():
         "kh_driver_set_stage_record": (i, [vp, vp]),
         "kh_driver_stage_record_bytes": (i, [vp, ctypes.POINTER(ctypes.c_size_t)]),
         "kh_driver_check": (i, [vp]),
+        "kh_driver_save_restart": (i, [vp, ctypes.c_char_p]),
+        "kh_driver_load_restart": (i, [vp, ctypes.c_char_p]),
+        "kh_mesh_save_restart": (i, [vp, ctypes.c_char_p]),
+        "kh_mesh_load_restart": (i, [vp, ctypes.c_char_p]),
+        "kh_mesh_create": (i, [ctypes.POINTER(KhParams), i, i, i, pvp]),
+        "kh_mesh_destroy": (None, [vp]),
+        "kh_mesh_set_stream": (i, [vp, vp]),
+        "kh_mesh_shape": (i, [vp, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(i),
+                              ctypes.POINTER(l)]),
+        "kh_mesh_init_prims": (i, [vp, vp]),
+        "kh_mesh_step": (i, [vp, i]),
+        "kh_mesh_set_clock": (i, [vp, d, d, i]),
+        "kh_mesh_get_state": (i, [vp, vp, vp]),
+        "kh_mesh_get_scalars": (i, [vp, ctypes.POINTER(d), ctypes.POINTER(d), ctypes.POINTER(d),
+                                    ctypes.POINTER(i), ctypes.POINTER(i)]),
+        "kh_mesh_get_dt_history": (i, [vp, ctypes.POINTER(d), i]),
+        "kh_mesh_check": (i, [vp]),
+        "kh_mesh_block_arrays": (i, [vp, i, pvp, pvp]),
+        "kh_mesh_block": (i, [vp, i, pvp]),
         "kh_vars_from_pack": (i, [ctypes.POINTER(ctypes.c_void_p), i, ctypes.POINTER(KhVarMap), ctypes.POINTER(KhVars)]),
         "kh_get_flux_v": (i, [vp, i, pv, pv, vp, vp]),
         "kh_flux_ct_v": (i, [vp, pv, vp]),
@@ -577,6 +596,16 @@ class Driver:
         _check(lib().kh_driver_get_state(self.h, _ptr(P), _ptr(U)))
         return P, U
 
+    def save_restart(self, path):
+        """kh_driver_save_restart: the state (P, U) and clock to a restart file."""
+        self._bind()
+        _check(lib().kh_driver_save_restart(self.h, os.fsencode(path)))
+
+    def load_restart(self, path):
+        """kh_driver_load_restart: continue from a restart file of the same mesh."""
+        self._bind()
+        _check(lib().kh_driver_load_restart(self.h, os.fsencode(path)))
+
     def flags(self):
         """(pflag, fflag) of the last stage as int32 device tensors [n3, n2, n1]."""
         torch = _torch()
@@ -641,6 +670,109 @@ class Driver:
         self._bind()
         _check(lib().kh_driver_get_dt_history(self.h, arr, n))
         return list(arr)
+
+
+class Mesh:
+    """A uniform mesh of nb1 x nb2 x nb3 meshblocks on the GPU (kh_mesh_*): the
+    default stage driver over all blocks.  cfg describes the whole mesh; states
+    read and write in the layout of one block of the whole mesh."""
+
+    def __init__(self, cfg, nblocks):
+        self.cfg = cfg
+        self._p = make_params(cfg)
+        nb = tuple(nblocks) + (1,) * (3 - len(nblocks))
+        h = ctypes.c_void_p()
+        _check(lib().kh_mesh_create(ctypes.byref(self._p), nb[0], nb[1], nb[2], ctypes.byref(h)))
+        self.h = h
+        self.nblocks = nb
+        nbk, n1, n2, n3, nc = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_long()
+        _check(lib().kh_mesh_shape(self.h, ctypes.byref(nbk), ctypes.byref(n1), ctypes.byref(n2), ctypes.byref(n3),
+                                   ctypes.byref(nc)))
+        self.block_shape = (n3.value, n2.value, n1.value)
+        ndim, _ = problems.block_shape(cfg)
+        ng = self._p.ng
+        self.shape = (self._p.nx3 + 2 * ng if ndim > 2 else 1, self._p.nx2 + 2 * ng if ndim > 1 else 1,
+                      self._p.nx1 + 2 * ng)
+
+    def __del__(self):
+        try:
+            lib().kh_mesh_destroy(self.h)
+        except Exception:
+            pass
+
+    def _bind(self):
+        _check(lib().kh_mesh_set_stream(self.h, _stream()))
+
+    def init_prims(self, P):
+        torch = _torch()
+        P = torch.as_tensor(P, dtype=torch.float64).to("cuda").contiguous()
+        assert tuple(P.shape) == (NV,) + self.shape, (P.shape, self.shape)
+        self._bind()
+        _check(lib().kh_mesh_init_prims(self.h, _ptr(P)))
+        self._P_init = P
+
+    def step(self, n=1):
+        self._bind()
+        _check(lib().kh_mesh_step(self.h, n))
+
+    def set_clock(self, time, dt, ncycle=0):
+        self._bind()
+        _check(lib().kh_mesh_set_clock(self.h, time, dt, ncycle))
+
+    def state(self):
+        torch = _torch()
+        P = torch.empty((NV,) + self.shape, dtype=torch.float64, device="cuda")
+        U = torch.empty_like(P)
+        self._bind()
+        _check(lib().kh_mesh_get_state(self.h, _ptr(P), _ptr(U)))
+        return P, U
+
+    def scalars(self):
+        t, dt, divb = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        nc, nf = ctypes.c_int(), ctypes.c_int()
+        self._bind()
+        _check(lib().kh_mesh_get_scalars(self.h, ctypes.byref(t), ctypes.byref(dt), ctypes.byref(divb),
+                                         ctypes.byref(nc), ctypes.byref(nf)))
+        return dict(time=t.value, dt=dt.value, divb=divb.value, ncycle=nc.value, fails=nf.value)
+
+    def dt_history(self, n):
+        arr = (ctypes.c_double * n)()
+        self._bind()
+        _check(lib().kh_mesh_get_dt_history(self.h, arr, n))
+        return list(arr)
+
+    def check(self):
+        self._bind()
+        _check(lib().kh_mesh_check(self.h))
+
+    def save_restart(self, path):
+        self._bind()
+        _check(lib().kh_mesh_save_restart(self.h, os.fsencode(path)))
+
+    def load_restart(self, path):
+        self._bind()
+        _check(lib().kh_mesh_load_restart(self.h, os.fsencode(path)))
+
+
+RESTART_MAGIC = b"KHAMDRST"
+
+
+def read_restart(path):
+    """A restart file (kh_driver_save_restart / kh_mesh_save_restart) on the host:
+    dict(time, dt, ncycle, params (KhParams), P, U (float64 [8, n3, n2, n1]))."""
+    import numpy as np
+    raw = open(path, "rb").read()
+    if raw[:8] != RESTART_MAGIC:
+        raise KharmaError(f"{path}: not a restart file of this library")
+    version, hbytes = np.frombuffer(raw, dtype=np.uint32, count=2, offset=8)
+    n1, n2, n3, ncycle = np.frombuffer(raw, dtype=np.int32, count=4, offset=16)
+    time, dt = np.frombuffer(raw, dtype=np.float64, count=2, offset=32)
+    params = KhParams.from_buffer_copy(raw[48:48 + ctypes.sizeof(KhParams)])
+    cells = 8 * int(n1) * int(n2) * int(n3)
+    data = np.frombuffer(raw, dtype=np.float64, count=2 * cells, offset=int(hbytes))
+    shape = (8, int(n3), int(n2), int(n1))
+    return dict(time=float(time), dt=float(dt), ncycle=int(ncycle), version=int(version), params=params,
+                P=data[:cells].reshape(shape).copy(), U=data[cells:].reshape(shape).copy())
 
 
 class AdjointStep:

@@ -119,6 +119,11 @@ void kh::make_dims(const kh_params* p, BlockDims& b) {
     b.fix_polar_flux = !b.periodic[1] && p->fix_polar_flux;
     b.fix_corner_inner = !b.periodic[0] && !b.periodic[1] && p->fix_corner_inner;
     b.prim_fill = p->driver_type == KH_DRIVER_SIMPLE;
+    for (int d = 0; d < 3; ++d) {
+        b.img[d][0] = b.img[d][1] = b.periodic[d];
+        b.nbd[d][0] = b.nbd[d][1] = 0;
+        b.goff[d] = 0;
+    }
 }
 
 namespace {
@@ -288,6 +293,67 @@ int kh_block_create(const kh_params* p, kh_block** out) {
     *out = b;
     return KH_OK;
 }
+
+}  // extern "C"
+
+// One block of a uniform mesh of nb[0] x nb[1] x nb[2] blocks (block index
+// bidx): the block's own extents, the mesh's zone widths and coordinates
+// (offset by its place in the mesh, so its geometry is the one-block mesh's at
+// the same zones), and per face either the mesh's physical boundary or images
+// of the neighbouring block (periodic axes wrap around the mesh).
+int kh::block_create_in_mesh(const kh_params* mesh, const int nb[3], const int bidx[3], kh_block** out) {
+    if (!out) return fail(KH_ERR_INVALID, "null output pointer");
+    *out = nullptr;
+    kh_params pb = *mesh;
+    pb.nx1 = mesh->nx1 / nb[0];
+    pb.nx2 = mesh->nx2 / nb[1];
+    pb.nx3 = mesh->nx3 / nb[2];
+    int st = validate(&pb);
+    if (st != KH_OK) return st;
+    kh_block* b = new kh_block();
+    b->p = pb;
+    BlockDims& bd = b->dims;
+    fill_dims(&pb, bd);
+    bd.dx1 = (mesh->x1max - mesh->x1min) / mesh->nx1;
+    bd.dx2 = (mesh->x2max - mesh->x2min) / mesh->nx2;
+    bd.dx3 = (mesh->x3max - mesh->x3min) / mesh->nx3;
+    const int stride[3] = {1, nb[0], nb[0] * nb[1]};
+    const int nx[3] = {bd.nx1, bd.nx2, bd.nx3};
+    const int per[3] = {mesh->periodic1, bd.ndim > 1 ? mesh->periodic2 : 1, 1};
+    for (int d = 0; d < 3; ++d) {
+        bd.goff[d] = (d < bd.ndim) ? bidx[d] * nx[d] : 0;
+        if (d >= bd.ndim || nb[d] == 1) continue;   // as a one-block mesh along this axis
+        const bool first = bidx[d] == 0, last = bidx[d] == nb[d] - 1;
+        bd.img[d][0] = !first || per[d];
+        bd.nbd[d][0] = first ? (nb[d] - 1) * stride[d] : -stride[d];
+        bd.img[d][1] = !last || per[d];
+        bd.nbd[d][1] = last ? -(nb[d] - 1) * stride[d] : stride[d];
+        bd.periodic[d] = bd.img[d][0] && bd.img[d][1];
+    }
+    bd.bc[0] = bd.img[0][0] ? 0 : mesh->bc_inner_x1;
+    bd.bc[1] = bd.img[0][1] ? 0 : mesh->bc_outer_x1;
+    bd.bc[2] = bd.img[1][0] ? 0 : mesh->bc_inner_x2;
+    bd.bc[3] = bd.img[1][1] ? 0 : mesh->bc_outer_x2;
+    bd.check_inflow[0] = !bd.img[0][0] && mesh->check_inflow_inner_x1;
+    bd.check_inflow[1] = !bd.img[0][1] && mesh->check_inflow_outer_x1;
+    const bool polar = bd.ndim > 1 && (!bd.img[1][0] || !bd.img[1][1]);
+    bd.zero_polar_flux = polar && mesh->zero_polar_flux;
+    bd.fix_polar_flux = polar && mesh->fix_polar_flux;
+    bd.fix_corner_inner = !bd.img[0][0] && polar && mesh->fix_corner_inner;
+    fill_consts(&pb, b->consts);
+    b->npartials = std::max(update_blocks(bd), divb_blocks(bd)) + 1;
+    hipError_t e = hipMalloc(&b->scratch, sizeof(DeviceState));
+    if (e == hipSuccess) e = hipMalloc(&b->partials, sizeof(double) * b->npartials);
+    if (e == hipSuccess && bd.geo) e = upload_geometry(b);
+    if (e != hipSuccess) {
+        kh_block_destroy(b);
+        return fail(KH_ERR_HIP, std::string("device allocation / copy: ") + hipGetErrorString(e));
+    }
+    *out = b;
+    return KH_OK;
+}
+
+extern "C" {
 
 void kh_block_destroy(kh_block* b) {
     if (!b) return;
@@ -552,6 +618,69 @@ int kh_fofc_v(kh_block* b, const kh_vars* P, const int* fofcflag, const kh_vars 
     return KH_OK;
 }
 
+}  // extern "C"
+
+// ----------------------------------------------------------------- restart
+// The reference's restart dump holds the fields marked Metadata::Restart --
+// the conserved and primitive variables (driver/kharma_driver.cpp:114-118,
+// grmhd/grmhd.cpp:155-156) -- and Parthenon's SimTime.  Parthenon writes them to
+// HDF5 (.rhdf); this library writes the same content to a flat binary file.
+int kh::write_restart(const char* path, const kh_params& p, const int n[3], double time, double dt, int ncycle,
+                      const double* P, const double* U, hipStream_t s) {
+    if (!path) return fail(KH_ERR_INVALID, "null path");
+    const size_t cells = (size_t)NV * n[0] * n[1] * n[2];
+    std::vector<double> host(2 * cells);
+    KH_HIP(hipMemcpyAsync(host.data(), P, sizeof(double) * cells, hipMemcpyDeviceToHost, s));
+    KH_HIP(hipMemcpyAsync(host.data() + cells, U, sizeof(double) * cells, hipMemcpyDeviceToHost, s));
+    KH_HIP(hipStreamSynchronize(s));
+    RestartHeader hd{};
+    std::memcpy(hd.magic, "KHAMDRST", 8);
+    hd.version = 1;
+    hd.header_bytes = sizeof(RestartHeader);
+    for (int d = 0; d < 3; ++d) hd.n[d] = n[d];
+    hd.ncycle = ncycle;
+    hd.time = time;
+    hd.dt = dt;
+    hd.params = p;
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return fail(KH_ERR_INVALID, std::string("cannot open ") + path + " for writing");
+    const bool ok = std::fwrite(&hd, sizeof(hd), 1, f) == 1 && std::fwrite(host.data(), sizeof(double), host.size(), f) == host.size();
+    if (std::fclose(f) != 0 || !ok) return fail(KH_ERR_INVALID, std::string("short write to ") + path);
+    return KH_OK;
+}
+
+int kh::read_restart(const char* path, const kh_params& p, const int n[3], double* time, double* dt, int* ncycle,
+                     double* P, double* U, hipStream_t s) {
+    if (!path) return fail(KH_ERR_INVALID, "null path");
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return fail(KH_ERR_INVALID, std::string("cannot open ") + path);
+    RestartHeader hd{};
+    const bool hok = std::fread(&hd, sizeof(hd), 1, f) == 1;
+    auto bad = [&](const std::string& why) { std::fclose(f); return fail(KH_ERR_INVALID, std::string(path) + ": " + why); };
+    if (!hok || std::memcmp(hd.magic, "KHAMDRST", 8) != 0 || hd.version != 1 || hd.header_bytes != sizeof(RestartHeader))
+        return bad("not a restart file of this library");
+    const kh_params& q = hd.params;
+    if (hd.n[0] != n[0] || hd.n[1] != n[1] || hd.n[2] != n[2] || q.nx1 != p.nx1 || q.nx2 != p.nx2 || q.nx3 != p.nx3 ||
+        q.ng != p.ng)
+        return bad("the mesh's zones differ from the run's");
+    if (q.coords != p.coords || q.transform != p.transform || q.a != p.a || q.gam != p.gam || q.x1min != p.x1min ||
+        q.x1max != p.x1max || q.x2min != p.x2min || q.x2max != p.x2max || q.x3min != p.x3min || q.x3max != p.x3max)
+        return bad("the geometry or gamma differ from the run's");
+    const size_t cells = (size_t)NV * n[0] * n[1] * n[2];
+    std::vector<double> host(2 * cells);
+    if (std::fread(host.data(), sizeof(double), host.size(), f) != host.size()) return bad("truncated");
+    std::fclose(f);
+    KH_HIP(hipMemcpyAsync(P, host.data(), sizeof(double) * cells, hipMemcpyHostToDevice, s));
+    KH_HIP(hipMemcpyAsync(U, host.data() + cells, sizeof(double) * cells, hipMemcpyHostToDevice, s));
+    KH_HIP(hipStreamSynchronize(s));
+    *time = hd.time;
+    *dt = hd.dt;
+    *ncycle = hd.ncycle;
+    return KH_OK;
+}
+
+extern "C" {
+
 // ------------------------------------------------------------------ driver
 int kh_driver_create(const kh_params* p, kh_driver** out) {
     if (!out) return fail(KH_ERR_INVALID, "null output pointer");
@@ -692,7 +821,7 @@ int kh::driver_stage(kh_driver* d, int stage) {
     hipStream_t s = b->stream;
     char* F = (char*)d->F;
     char* ct = (char*)d->ctop;
-    const bool physical = !bd.periodic[0] || !bd.periodic[1];
+    const bool physical = has_physical_face(bd);
     const double* Psub = (stage == 1) ? d->P0 : d->P1;
     const float* Psubf = (stage == 1) ? d->P0f : d->P1f;
     const double* Usub = (stage == 1) ? d->U0 : d->U1;
@@ -771,6 +900,36 @@ int kh_driver_get_state(kh_driver* d, double* P, double* U) {
     const size_t st_bytes = sizeof(double) * NV * b->dims.nc;
     if (P) KH_HIP(hipMemcpyAsync(P, d->P0, st_bytes, hipMemcpyDeviceToDevice, b->stream));
     if (U) KH_HIP(hipMemcpyAsync(U, d->U0, st_bytes, hipMemcpyDeviceToDevice, b->stream));
+    return KH_OK;
+}
+
+int kh_driver_save_restart(kh_driver* d, const char* path) {
+    if (!d) return fail(KH_ERR_INVALID, "null driver");
+    kh_block* b = d->blk;
+    DeviceState h;
+    KH_HIP(hipMemcpyAsync(&h, d->ds, sizeof(h), hipMemcpyDeviceToHost, b->stream));
+    KH_HIP(hipStreamSynchronize(b->stream));
+    const int n[3] = {b->dims.n1, b->dims.n2, b->dims.n3};
+    return write_restart(path, b->p, n, h.time, h.dt, h.ncycle, d->P0, d->U0, b->stream);
+}
+
+int kh_driver_load_restart(kh_driver* d, const char* path) {
+    if (!d) return fail(KH_ERR_INVALID, "null driver");
+    kh_block* b = d->blk;
+    const int n[3] = {b->dims.n1, b->dims.n2, b->dims.n3};
+    double time = 0., dt = 0.;
+    int ncycle = 0;
+    const int st = read_restart(path, b->p, n, &time, &dt, &ncycle, d->P0, d->U0, b->stream);
+    if (st != KH_OK) return st;
+    const size_t st_bytes = sizeof(double) * NV * b->dims.nc;
+    KH_HIP(hipMemcpyAsync(d->P1, d->P0, st_bytes, hipMemcpyDeviceToDevice, b->stream));
+    KH_HIP(hipMemcpyAsync(d->U1, d->U0, st_bytes, hipMemcpyDeviceToDevice, b->stream));
+    if (d->P0f) {
+        KH_HIP(launch_convert_f32(d->P0, d->P0f, NV * b->dims.nc, b->stream));
+        KH_HIP(launch_convert_f32(d->P0, d->P1f, NV * b->dims.nc, b->stream));
+    }
+    KH_HIP(launch_reset_state(d->ds, dt, b->stream));
+    KH_HIP(launch_set_clock(d->ds, time, dt, ncycle, b->stream));
     return KH_OK;
 }
 

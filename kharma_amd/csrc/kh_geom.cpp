@@ -175,8 +175,8 @@ void locus_coords(const kh_params& p, const BlockDims& b, int loc, int j, int i,
     const bool f1 = (loc == GLOC_FACE1 || loc == GLOC_CORNER);
     const bool f2 = (loc == GLOC_FACE2 || loc == GLOC_CORNER);
     X[0] = 0.;
-    X[1] = p.x1min + ((i - b.is) + (f1 ? 0. : 0.5)) * b.dx1;
-    X[2] = (b.ndim > 1) ? p.x2min + ((j - b.js) + (f2 ? 0. : 0.5)) * b.dx2 : 0.5 * (p.x2min + p.x2max);
+    X[1] = p.x1min + ((i - b.is + b.goff[0]) + (f1 ? 0. : 0.5)) * b.dx1;
+    X[2] = (b.ndim > 1) ? p.x2min + ((j - b.js + b.goff[1]) + (f2 ? 0. : 0.5)) * b.dx2 : 0.5 * (p.x2min + p.x2max);
     X[3] = p.x3min + 0.5 * b.dx3;
 }
 

@@ -1,6 +1,7 @@
 // kh_host.hpp -- host-side objects shared by the C ABI translation units:
 // the block context and the stage driver (include/kharma_amd.h).
 #pragma once
+#include <cstdint>
 #include <vector>
 
 #include "kh_internal.hpp"
@@ -58,6 +59,25 @@ struct kh_driver {
 };
 
 namespace kh {
+// One block of a uniform mesh of nb[0] x nb[1] x nb[2] blocks (kh_capi.cpp).
+int block_create_in_mesh(const kh_params* mesh, const int nb[3], const int bidx[3], kh_block** out);
+
+// Restart file (kh_capi.cpp): a fixed header, then P and U as [8][n3][n2][n1]
+// doubles over the entire extent of a one-block mesh.
+struct RestartHeader {
+    char magic[8];          // "KHAMDRST"
+    uint32_t version;       // 1
+    uint32_t header_bytes;  // sizeof(RestartHeader)
+    int32_t n[3];           // entire extents n1, n2, n3 of the arrays
+    int32_t ncycle;
+    double time, dt;
+    kh_params params;       // the run's parameters (the reader checks the mesh against them)
+};
+// Write / read the clock and the state; the device arrays hold 8 n1 n2 n3 doubles.
+int write_restart(const char* path, const kh_params& p, const int n[3], double time, double dt, int ncycle,
+                  const double* P, const double* U, hipStream_t s);
+int read_restart(const char* path, const kh_params& p, const int n[3], double* time, double* dt, int* ncycle,
+                 double* P, double* U, hipStream_t s);
 // Bytes per element of the face-flux arrays (FP32 in the split form).
 inline size_t flux_elem(const kh_block* b) { return b->p.precision == KH_PREC_SPLIT ? sizeof(float) : sizeof(double); }
 // One RK stage (1 or 2) of the driver's step on its current buffers.

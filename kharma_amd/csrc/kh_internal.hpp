@@ -30,7 +30,23 @@ struct BlockDims {
     // Physical boundary fill of driver "simple" (primitives are the synced
     // fields): each filled zone's U.B is recomputed from its P.B = U.B / sqrt(-g).
     int prim_fill;
+    // Faces of the block (axis d, side 0 low / 1 high).  img: the face's ghosts
+    // are images of interior zones -- of this block (a periodic axis of a
+    // one-block mesh) or of the neighbouring block of a mesh of blocks -- rather
+    // than a physical boundary.  nbd: index offset of the neighbouring block
+    // across the face in the mesh's block order (0 for the block itself).  A
+    // block's arrays sit at a fixed stride per block, so an image zone lies at
+    // the same offset as in a one-block mesh plus nbd times that stride.
+    int img[3][2];
+    int nbd[3][2];
+    // Interior offset of this block in the mesh (zones per axis); coordinates are
+    // x = xmin(mesh) + (i - is + goff + 1/2) dx with the mesh's dx.
+    int goff[3];
 };
+// The block has a physical face (outflow / reflecting ghosts to fill).
+__host__ __device__ inline bool has_physical_face(const BlockDims& b) {
+    return !b.img[0][0] || !b.img[0][1] || (b.ndim > 1 && (!b.img[1][0] || !b.img[1][1]));
+}
 
 // Scalars that change every step live on the device so a run of steps needs no
 // host round trip (dt is produced and consumed on the GPU).
@@ -218,8 +234,11 @@ hipError_t launch_bounds(const BlockDims& b, const StepConsts& c, const GeomDev&
                          const Vars<double>& P, float* P32, hipStream_t s);
 hipError_t launch_bounds(const BlockDims& b, const StepConsts& c, const GeomDev& gd, double* U, double* P, float* P32,
                          hipStream_t s);
+// End-of-step reductions and clock; ds: nds block states (a mesh), the first
+// carrying the clock and totals.
 hipError_t launch_finalize(const StepConsts& c, DeviceState* ds, const double* dt_part, int n_dt,
-                           const double* divb_part, int n_divb, double* dt_hist, int hist_cap, hipStream_t s);
+                           const double* divb_part, int n_divb, double* dt_hist, int hist_cap, hipStream_t s,
+                           int nds = 1);
 hipError_t launch_max_divb(const BlockDims& b, const Vars<const double>& U, double* part, hipStream_t s);
 hipError_t launch_max_divb(const BlockDims& b, const double* U, double* part, hipStream_t s);
 hipError_t launch_reduce(const double* part, int n, int is_max, double* out, hipStream_t s);
@@ -246,6 +265,22 @@ hipError_t launch_ptou(const BlockDims& b, const StepConsts& c, const GeomDev& g
 hipError_t launch_sync(const BlockDims& b, const Vars<double>& A, int nvar, hipStream_t s);
 hipError_t launch_sync(const BlockDims& b, double* A, int nvar, hipStream_t s);
 hipError_t launch_sync_int(const BlockDims& b, int* A, hipStream_t s);
+
+// ---- mesh of several blocks (kh_mesh.hip) ----
+// Extents of a uniform mesh of nb[0] x nb[1] x nb[2] equal blocks: N interior
+// zones per axis (nx = N / nb per block), Ne = N + 2 ng entire (1 for inactive
+// axes); n, nc: one block's entire extent and zone count; ncm: the mesh's.
+struct MeshDims {
+    int N[3], Ne[3], nb[3], nx[3], n[3];
+    int ng, ndim, nblk;
+    int periodic[3];
+    long nc, ncm;
+};
+// Mesh layout [8][Ne3][Ne2][Ne1] -> every block's [8][nc] (entire extents,
+// periodic ghosts from their images), and back (interior from the owning
+// block, physical ghosts from the edge block's own ghosts).
+hipError_t launch_mesh_scatter(const MeshDims& m, const double* Pm, double* Pb, hipStream_t s);
+hipError_t launch_mesh_gather(const MeshDims& m, const double* Pb, double* Pm, hipStream_t s);
 
 // ---- adjoint (kh_adjoint.hip) ----
 hipError_t launch_adj_fill(double* a, long n, double v, hipStream_t s);

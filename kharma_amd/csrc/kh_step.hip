@@ -36,38 +36,46 @@ __global__ void __launch_bounds__(256) convert_f32_kernel(const double* __restri
 }
 
 // ------------------------------------------------------------ ghost images
-// Periodic image of interior zone (k,j,i) inside the ghost region, as element
-// offsets per axis (0 = none).  With nx >= 2 ng on every periodic axis a zone
-// has at most one image offset per axis, so up to 7 image copies (corners).
+// Images of interior zone (k,j,i) in ghost regions: per axis, the zone's
+// offset in the image block's layout (0 = no image) and the index offset of
+// that block (0: this block, a periodic axis of a one-block mesh; otherwise
+// the neighbour across the face the zone lies within ng of).  With nx >= 2 ng
+// on every image axis a zone has at most one image per axis, so up to 7 image
+// copies (edges and corners land in the diagonal neighbour, whose block offset
+// is the sum of the axes' offsets).
 struct ImageSet {
     long oi, oj, ok;
+    int bi, bj, bk;
 };
 __device__ __forceinline__ void image_set(const BlockDims& b, int k, int j, int i, ImageSet& s) {
     s.oi = s.oj = s.ok = 0;
-    if (b.periodic[0]) {
-        if (i - b.is < b.ng) s.oi = b.nx1;
-        else if (b.ie - i < b.ng) s.oi = -b.nx1;
+    s.bi = s.bj = s.bk = 0;
+    if (b.img[0][0] && i - b.is < b.ng) { s.oi = b.nx1; s.bi = b.nbd[0][0]; }
+    else if (b.img[0][1] && b.ie - i < b.ng) { s.oi = -b.nx1; s.bi = b.nbd[0][1]; }
+    if (b.ndim > 1) {
+        if (b.img[1][0] && j - b.js < b.ng) { s.oj = (long)b.nx2 * b.s2; s.bj = b.nbd[1][0]; }
+        else if (b.img[1][1] && b.je - j < b.ng) { s.oj = -(long)b.nx2 * b.s2; s.bj = b.nbd[1][1]; }
     }
-    if (b.ndim > 1 && b.periodic[1]) {
-        if (j - b.js < b.ng) s.oj = (long)b.nx2 * b.s2;
-        else if (b.je - j < b.ng) s.oj = -(long)b.nx2 * b.s2;
-    }
-    if (b.ndim > 2 && b.periodic[2]) {
-        if (k - b.ks < b.ng) s.ok = (long)b.nx3 * b.s3;
-        else if (b.ke - k < b.ng) s.ok = -(long)b.nx3 * b.s3;
+    if (b.ndim > 2) {
+        if (b.img[2][0] && k - b.ks < b.ng) { s.ok = (long)b.nx3 * b.s3; s.bk = b.nbd[2][0]; }
+        else if (b.img[2][1] && b.ke - k < b.ng) { s.ok = -(long)b.nx3 * b.s3; s.bk = b.nbd[2][1]; }
     }
 }
+// base[c] = val and the same at every image; bstride: elements between the
+// same array of consecutive blocks (NV nc for a state, nc for a flag array).
 template <typename T>
-__device__ __forceinline__ void store_with_images(const BlockDims&, const ImageSet& s, T* base, long c, T val) {
+__device__ __forceinline__ void store_with_images(const BlockDims&, const ImageSet& s, T* base, long c, T val,
+                                                  long bstride) {
     base[c] = val;
     if ((s.oi | s.oj | s.ok) == 0) return;
-    if (s.oi) base[c + s.oi] = val;
-    if (s.oj) base[c + s.oj] = val;
-    if (s.ok) base[c + s.ok] = val;
-    if (s.oi && s.oj) base[c + s.oi + s.oj] = val;
-    if (s.oi && s.ok) base[c + s.oi + s.ok] = val;
-    if (s.oj && s.ok) base[c + s.oj + s.ok] = val;
-    if (s.oi && s.oj && s.ok) base[c + s.oi + s.oj + s.ok] = val;
+    const long di = s.oi + s.bi * bstride, dj = s.oj + s.bj * bstride, dk = s.ok + s.bk * bstride;
+    if (s.oi) base[c + di] = val;
+    if (s.oj) base[c + dj] = val;
+    if (s.ok) base[c + dk] = val;
+    if (s.oi && s.oj) base[c + di + dj] = val;
+    if (s.oi && s.ok) base[c + di + dk] = val;
+    if (s.oj && s.ok) base[c + dj + dk] = val;
+    if (s.oi && s.oj && s.ok) base[c + di + dj + dk] = val;
 }
 
 __device__ __forceinline__ void interior_index(const BlockDims& b, unsigned t, int& k, int& j, int& i) {
@@ -106,13 +114,14 @@ __device__ __forceinline__ void geom_floors(const StepConsts& c, const GeomDev& 
         uf = c.floors.u_min_const;
     }
 }
-// KDomain::GetPhysicalRange (domain.hpp:161-187): interior along physical-boundary
-// dimensions, the entire block along periodic ones.
+// KDomain::GetPhysicalRange (domain.hpp:161-187): the entire block, less the
+// ghosts across physical-boundary faces (the ghosts of periodic faces and of
+// faces shared with another block are in range).
 __device__ __forceinline__ bool in_physical(const BlockDims& b, int k, int j, int i) {
     if (i < 0 || i >= b.n1 || j < 0 || j >= b.n2 || k < 0 || k >= b.n3) return false;
-    if (!b.periodic[0] && (i < b.is || i > b.ie)) return false;
-    if (b.ndim > 1 && !b.periodic[1] && (j < b.js || j > b.je)) return false;
-    if (b.ndim > 2 && !b.periodic[2] && (k < b.ks || k > b.ke)) return false;
+    if ((i < b.is && !b.img[0][0]) || (i > b.ie && !b.img[0][1])) return false;
+    if (b.ndim > 1 && ((j < b.js && !b.img[1][0]) || (j > b.je && !b.img[1][1]))) return false;
+    if (b.ndim > 2 && ((k < b.ks && !b.img[2][0]) || (k > b.ke && !b.img[2][1]))) return false;
     return true;
 }
 
@@ -441,11 +450,11 @@ update_kernel(BlockDims b, StepConsts cst, GeomDev gd, int stage, const R* __res
         image_set(b, k, j, i, s);
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-            store_with_images(b, s, Uout + v * b.nc, c, U[v]);
-            store_with_images(b, s, Pout + v * b.nc, c, P[v]);
-            if (Pout32) store_with_images(b, s, Pout32 + v * b.nc, c, (float)P[v]);
+            store_with_images(b, s, Uout + v * b.nc, c, U[v], NV * b.nc);
+            store_with_images(b, s, Pout + v * b.nc, c, P[v], NV * b.nc);
+            if (Pout32) store_with_images(b, s, Pout32 + v * b.nc, c, (float)P[v], NV * b.nc);
         }
-        store_with_images(b, s, pflag, c, pf);
+        store_with_images(b, s, pflag, c, pf, b.nc);
         fflag[c] = ff;
         if (pf > 0) {
             const int slot = atomicAdd(&ds->fail_count[stage - 1], 1);
@@ -581,11 +590,11 @@ recover_kernel(BlockDims b, StepConsts cst, GeomDev gd, int stage, const double*
     image_set(b, k, j, i, s);
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
-        store_with_images(b, s, Uout + v * b.nc, c, U[v]);
-        store_with_images(b, s, Pout + v * b.nc, c, P[v]);
-        if (Pout32) store_with_images(b, s, Pout32 + v * b.nc, c, (float)P[v]);
+        store_with_images(b, s, Uout + v * b.nc, c, U[v], NV * b.nc);
+        store_with_images(b, s, Pout + v * b.nc, c, P[v], NV * b.nc);
+        if (Pout32) store_with_images(b, s, Pout32 + v * b.nc, c, (float)P[v], NV * b.nc);
     }
-    store_with_images(b, s, pflag, c, pf);
+    store_with_images(b, s, pflag, c, pf, b.nc);
     fflag[c] = ff;
     if (pf > 0) {
         const int slot = atomicAdd(&ds->fail_count[stage - 1], 1);
@@ -630,18 +639,21 @@ fixup_list_kernel(BlockDims b, StepConsts cst, GeomDev gd, int stage, double* __
         ImageSet s;
         image_set(b, k, j, i, s);
         for (int v = 0; v < 5; ++v) {
-            store_with_images(b, s, P + v * b.nc, c, Pz[v]);
-            if (P32) store_with_images(b, s, P32 + v * b.nc, c, (float)Pz[v]);
+            store_with_images(b, s, P + v * b.nc, c, Pz[v], NV * b.nc);
+            if (P32) store_with_images(b, s, P32 + v * b.nc, c, (float)Pz[v], NV * b.nc);
         }
-        for (int v = 0; v < NV; ++v) store_with_images(b, s, U + v * b.nc, c, Uz[v]);
+        for (int v = 0; v < NV; ++v) store_with_images(b, s, U + v * b.nc, c, Uz[v], NV * b.nc);
     }
 }
 
 // End-of-step scalar work on the device: the CFL minimum and divB maximum of
 // the per-block partials, then SetGlobalTimeStep (kharma_driver.cpp:439-460)
 // with GRMHD::EstimateTimestep's clip (grmhd.cpp:335-339), time advance, resets.
+// ds holds nds states, one per block of a mesh: the first carries the mesh's
+// clock and totals, which are then copied to the others; every block's
+// failed-inversion counts are added up and cleared.
 __global__ void __launch_bounds__(1024)
-finalize_kernel(StepConsts cst, DeviceState* ds, const double* __restrict__ dt_part, int n_dt,
+finalize_kernel(StepConsts cst, DeviceState* ds, int nds, const double* __restrict__ dt_part, int n_dt,
                 const double* __restrict__ divb_part, int n_divb, double* dt_hist, int hist_cap) {
     double mn0 = DBL_MAX, mn1 = DBL_MAX, mx0 = 0., mx1 = 0.;
     const int st = blockDim.x;
@@ -672,14 +684,22 @@ finalize_kernel(StepConsts cst, DeviceState* ds, const double* __restrict__ dt_p
     ds->dt = nd;
     ds->last_ndt = min_ndt;
     ds->ncycle += 1;
-    ds->total_fails += ds->fail_count[0] + ds->fail_count[1];
+    for (int q = 0; q < nds; ++q) {
+        ds->total_fails += ds[q].fail_count[0] + ds[q].fail_count[1];
+        ds[q].fail_count[0] = 0;
+        ds[q].fail_count[1] = 0;
+    }
     ds->last_divb = mx;
     if (cst.kill_divb && mx > cst.divb_over && ds->divb_over_cycle == 0) {
         ds->divb_over_cycle = ds->ncycle;
         ds->divb_over_value = mx;
     }
-    ds->fail_count[0] = 0;
-    ds->fail_count[1] = 0;
+    for (int q = 1; q < nds; ++q) {
+        ds[q].dt = ds->dt; ds[q].dt_last = ds->dt_last; ds[q].time = ds->time;
+        ds[q].last_divb = ds->last_divb; ds[q].last_ndt = ds->last_ndt;
+        ds[q].ncycle = ds->ncycle; ds[q].total_fails = ds->total_fails;
+        ds[q].divb_over_cycle = ds->divb_over_cycle; ds[q].divb_over_value = ds->divb_over_value;
+    }
 }
 
 // One-block reduction of per-block partials into *out (min or max).
@@ -705,11 +725,11 @@ __global__ void set_clock_kernel(DeviceState* ds, double time, double dt, int nc
 }
 
 // B_FluxCT::MaxDivB with corner_div (b_flux_ct.cpp:566-607, b_flux_ct_functions.hpp:52-75)
-// Corners i in [is, ie+1] for periodic X1; ValidDivBX1 (b_flux_ct.cpp:554-564)
-// skips the corners on physical X1 faces: i in [is+1, ie].
+// Corners i in [is, ie+1]; ValidDivBX1 (b_flux_ct.cpp:554-564) skips the
+// corners on a physical X1 face (i = is inner, i = ie + 1 outer).
 __global__ void __launch_bounds__(BS) max_divb_kernel(BlockDims b, Vars<const double> U, double* __restrict__ part) {
-    const int i0 = b.periodic[0] ? b.is : b.is + 1;
-    const int ni = b.periodic[0] ? b.nx1 + 1 : b.nx1 - 1, nj = b.nx2 + 1, nk = (b.ndim > 2) ? b.nx3 + 1 : 1;
+    const int i0 = b.img[0][0] ? b.is : b.is + 1;
+    const int ni = (b.img[0][1] ? b.ie + 1 : b.ie) - i0 + 1, nj = b.nx2 + 1, nk = (b.ndim > 2) ? b.nx3 + 1 : 1;
     const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
     double d = 0.;
     if (t < (unsigned)(ni * nj * nk)) {
@@ -750,10 +770,12 @@ __global__ void __launch_bounds__(BS) fix_flux_kernel(BlockDims b, FluxVars<R> F
     const Vars<R>& F2 = F[1];
     const Vars<R>& F3 = F[2];
     const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
-    const unsigned npolar = (nd > 1 && !b.periodic[1]) ? 2u * ni * nk : 0u;
+    const bool pol[2] = {nd > 1 && !b.img[1][0], nd > 1 && !b.img[1][1]};
+    const unsigned npolar = (pol[0] || pol[1]) ? 2u * ni * nk : 0u;
     if (t < npolar) {
         // polar faces j = js (side 0) and j = je + 1 (side 1), i in [is-1, ie+1], k in [ks-1, ke+1]
         const int side = t / (ni * nk), rem = t - side * (ni * nk);
+        if (!pol[side]) return;
         const int k = k0 + rem / ni, i = i0 + rem % ni;
         const int j = side == 0 ? b.js : b.je + 1;
         const long c = (long)k * b.s3 + (long)j * b.s2 + i;
@@ -770,7 +792,7 @@ __global__ void __launch_bounds__(BS) fix_flux_kernel(BlockDims b, FluxVars<R> F
         return;
     }
     const unsigned t2 = t - npolar;
-    if (!b.periodic[0] && t2 < 2u * nj * nk) {
+    if ((!b.img[0][0] || !b.img[0][1]) && t2 < 2u * nj * nk) {
         // X1 faces i = is (side 0) and i = ie + 1 (side 1), j in [js-1, je+1], k in [ks-1, ke+1]
         const int side = t2 / (nj * nk), rem = t2 - side * (nj * nk);
         const int k = k0 + rem / nj, j = j0 + rem % nj;
@@ -821,15 +843,20 @@ __device__ __forceinline__ void bc_fill_x2(const BlockDims& b, const Vars<double
     if (type == KH_BC_REFLECT) { P[U2] = -P[U2]; Bu[1] = -Bu[1]; }
 }
 // The zones the physical boundaries fill, enumerated without the interior:
-// slab 1 is every zone of the 2 ng X1 ghost columns (all j, k) when X1 is
-// physical; slab 2 is the 2 ng X2 ghost rows over the X1 interior (or over all
-// i when X1 is periodic) when X2 is physical.
+// slab 1 is every zone of the ng X1 ghost columns (all j, k) of each physical
+// X1 face; slab 2 is the ng X2 ghost rows of each physical X2 face over the X1
+// interior, widened to the ghost columns of X1 faces that are not physical.
+__host__ __device__ inline int nphys(const BlockDims& b, int d) {
+    return (d < b.ndim) ? (!b.img[d][0]) + (!b.img[d][1]) : 0;
+}
 __host__ __device__ inline long bounds_slab1(const BlockDims& b) {
-    return b.periodic[0] ? 0 : (long)b.n3 * b.n2 * 2 * b.ng;
+    return (long)b.n3 * b.n2 * b.ng * nphys(b, 0);
+}
+__host__ __device__ inline int bounds_slab2_width(const BlockDims& b) {
+    return (b.img[0][1] ? b.n1 - 1 : b.ie) - (b.img[0][0] ? 0 : b.is) + 1;
 }
 __host__ __device__ inline long bounds_slab2(const BlockDims& b) {
-    if (b.ndim < 2 || b.periodic[1]) return 0;
-    return (long)b.n3 * 2 * b.ng * (b.periodic[0] ? b.n1 : b.nx1);
+    return (long)b.n3 * b.ng * nphys(b, 1) * bounds_slab2_width(b);
 }
 template <int GEO>
 __global__ void __launch_bounds__(BS) bounds_kernel(BlockDims b, StepConsts cst, GeomDev gd, Vars<double> U,
@@ -838,25 +865,27 @@ __global__ void __launch_bounds__(BS) bounds_kernel(BlockDims b, StepConsts cst,
     const long n1s = bounds_slab1(b);
     int k, j, i;
     if (r < n1s) {
-        const int g = (int)(r % (2 * b.ng));
-        r /= 2 * b.ng;
+        const int m = b.ng * nphys(b, 0);
+        const int g = (int)(r % m);
+        r /= m;
         j = (int)(r % b.n2);
         k = (int)(r / b.n2);
-        i = (g < b.ng) ? g : b.ie + 1 + (g - b.ng);
+        i = (!b.img[0][0] && g < b.ng) ? g : b.ie + 1 + (g - (b.img[0][0] ? 0 : b.ng));
     } else {
         r -= n1s;
         if (r >= bounds_slab2(b)) return;
-        const int w = b.periodic[0] ? b.n1 : b.nx1;
+        const int w = bounds_slab2_width(b);
         const int ii = (int)(r % w);
         r /= w;
-        const int g = (int)(r % (2 * b.ng));
-        k = (int)(r / (2 * b.ng));
-        j = (g < b.ng) ? g : b.je + 1 + (g - b.ng);
-        i = (b.periodic[0] ? 0 : b.is) + ii;
+        const int m = b.ng * nphys(b, 1);
+        const int g = (int)(r % m);
+        k = (int)(r / m);
+        j = (!b.img[1][0] && g < b.ng) ? g : b.je + 1 + (g - (b.img[1][0] ? 0 : b.ng));
+        i = (b.img[0][0] ? 0 : b.is) + ii;
     }
     const long t = (long)k * b.s3 + (long)j * b.s2 + i;
-    const bool x1g = !b.periodic[0] && (i < b.is || i > b.ie);
-    const bool x2g = b.ndim > 1 && !b.periodic[1] && (j < b.js || j > b.je);
+    const bool x1g = (i < b.is && !b.img[0][0]) || (i > b.ie && !b.img[0][1]);
+    const bool x2g = b.ndim > 1 && ((j < b.js && !b.img[1][0]) || (j > b.je && !b.img[1][1]));
     double Pz[NV], Bu[3];
     // U.B that a filled zone (jj, ii) ends with, under b.prim_fill
     auto settle = [&](double B[3], int jj, int ii) {
@@ -1400,8 +1429,8 @@ hipError_t launch_fixup(const BlockDims& b, const StepConsts& c, const GeomDev& 
     return hipGetLastError();
 }
 hipError_t launch_fix_flux(const BlockDims& b, int prec, const FluxVars<void>& F, hipStream_t s) {
-    const long npolar = (b.ndim > 1 && !b.periodic[1]) ? 2L * (b.nx1 + 2) * ((b.ndim > 2) ? b.nx3 + 2 : 1) : 0;
-    const long nx1f = !b.periodic[0] ? 2L * ((b.ndim > 1) ? b.nx2 + 2 : 1) * ((b.ndim > 2) ? b.nx3 + 2 : 1) : 0;
+    const long npolar = nphys(b, 1) ? 2L * (b.nx1 + 2) * ((b.ndim > 2) ? b.nx3 + 2 : 1) : 0;
+    const long nx1f = nphys(b, 0) ? 2L * ((b.ndim > 1) ? b.nx2 + 2 : 1) * ((b.ndim > 2) ? b.nx3 + 2 : 1) : 0;
     const long n = npolar + nx1f;
     if (n == 0) return hipSuccess;
     if (prec == PREC_SPLIT) hipLaunchKernelGGL(fix_flux_kernel<float>, grid_for(n), dim3(BS), 0, s, b, flux_vars_cast<float>(F));
@@ -1413,7 +1442,7 @@ hipError_t launch_fix_flux(const BlockDims& b, int prec, void* F, hipStream_t s)
 }
 hipError_t launch_bounds(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const Vars<double>& U,
                          const Vars<double>& P, float* P32, hipStream_t s) {
-    if (b.periodic[0] && (b.ndim < 2 || b.periodic[1])) return hipSuccess;
+    if (!has_physical_face(b)) return hipSuccess;
     const long n = bounds_slab1(b) + bounds_slab2(b);
     if (b.geo) hipLaunchKernelGGL(bounds_kernel<1>, grid_for(n), dim3(BS), 0, s, b, c, gd, U, P, P32);
     else hipLaunchKernelGGL(bounds_kernel<0>, grid_for(n), dim3(BS), 0, s, b, c, gd, U, P, P32);
@@ -1424,8 +1453,9 @@ hipError_t launch_bounds(const BlockDims& b, const StepConsts& c, const GeomDev&
     return launch_bounds(b, c, gd, vars_of(U, b.nc), vars_of(P, b.nc), P32, s);
 }
 hipError_t launch_finalize(const StepConsts& c, DeviceState* ds, const double* dt_part, int n_dt,
-                           const double* divb_part, int n_divb, double* dt_hist, int hist_cap, hipStream_t s) {
-    hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(1024), 0, s, c, ds, dt_part, n_dt, divb_part, n_divb, dt_hist, hist_cap);
+                           const double* divb_part, int n_divb, double* dt_hist, int hist_cap, hipStream_t s, int nds) {
+    hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(1024), 0, s, c, ds, nds, dt_part, n_dt, divb_part, n_divb, dt_hist,
+                       hist_cap);
     return hipGetLastError();
 }
 hipError_t launch_max_divb(const BlockDims& b, const Vars<const double>& U, double* part, hipStream_t s) {
