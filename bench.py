@@ -103,6 +103,28 @@ def _kernel_io_bytes(kind, prec_bytes):
     return None
 
 
+def _valu_fractions(tag):
+    """Compute-side roofline of the split form's kernels from the committed SQ
+    counter summary (profiles/valu_<workload>.json): VALU instructions issued as
+    a fraction of the chip's issue capacity (one wave64 instruction per SIMD per
+    4 cycles, which for FP64 FMA is the 78.6 TFLOP/s vector peak) and the share
+    of the resident waves' cycles spent waiting on memory.  The update pair is
+    weighted by the kernels' times."""
+    f = os.path.join(ROOT, "profiles", f"valu_{tag}.json")
+    if not os.path.exists(f):
+        return None
+    k = json.load(open(f))["kernels"]["split"]
+    out = {"peak": "78.6 TFLOP/s FP64 vector (VALU issue capacity)", "source": os.path.relpath(f, ROOT)}
+    for ro in ("combine_s2", "recover_s2", "flux_x1_s2", "flux_x2_s2", "flux_x3_s2"):
+        if ro in k:
+            out[ro] = {"valu_issue_frac": k[ro]["valu_issue_frac"], "wait_frac": k[ro]["wait_any_per_wave_cycle"]}
+    if "combine_s2" in k and "recover_s2" in k:
+        a, b = k["combine_s2"], k["recover_s2"]
+        ta, tb = a["avg_us_under_counters"], b["avg_us_under_counters"]
+        out["update_s2_valu_issue_frac"] = (a["valu_issue_frac"] * ta + b["valu_issue_frac"] * tb) / (ta + tb)
+    return out
+
+
 def _cpu_model():
     """The host CPU's model name (from /proc/cpuinfo), or None."""
     try:
@@ -291,7 +313,8 @@ def main():
             "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic,
             "kernel": dom, "avg_launch_ms": avg_ms, "algorithmic_bytes_per_launch": algo,
             "algorithmic_bytes_per_zone": per_zone, "zones_per_launch": nz,
-            "kernel_io_bytes_per_launch": io * nz if io else None}
+            "kernel_io_bytes_per_launch": io * nz if io else None,
+            "valu": _valu_fractions(tag)}
     step_bytes = STEP_STATE_BYTES * nz
     kernel_ms = {k: {"total_ms": v[0], "launches": v[1], "avg_ms": (v[0] / v[1]) if v[1] else None}
                  for k, v in r32["kernels"].items()}

@@ -340,7 +340,6 @@ int kh_driver_load_restart(kh_driver* d, const char* path);
 /* Inversion / floor flags of the last stage, int32 [nc] device arrays (either may be NULL):
  * the reference's pflag / fflag fields (inverter/inverter.cpp:104-108, floors/floors.cpp:111-116). */
 int kh_driver_get_flags(kh_driver* d, int* pflag, int* fflag);
-/* Synchronises; any output may be NULL. */
 /* PrintGlobalMaxDivB's stop (b_flux_ct/b_flux_ct.cpp:620-643): with
  * kill_on_large_divb, each step's maximum corner divB is compared on the device
  * with kill_on_divb_over and the first step above it is latched, so steps stay
@@ -348,6 +347,7 @@ int kh_driver_get_flags(kh_driver* d, int* pflag, int* fflag);
  * cycle and value in kh_last_error) once a step has exceeded it, KH_OK
  * otherwise.  kh_driver_init_prims clears the latch. */
 int kh_driver_check(kh_driver* d);
+/* Synchronises; any output may be NULL. */
 int kh_driver_get_scalars(kh_driver* d, double* time, double* dt, double* last_divb, int* ncycle, int* total_fails);
 /* dt taken by each of the first n steps since init (host array). Synchronises. */
 int kh_driver_get_dt_history(kh_driver* d, double* out, int n);

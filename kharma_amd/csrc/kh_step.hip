@@ -504,6 +504,48 @@ __device__ __forceinline__ void interior_index_jki(const BlockDims& b, unsigned 
     j = b.js + (int)jj;
 }
 
+// In 3-D the recover kernel runs on tiles of 64 x1 cells by 4 x3 rows at one
+// x2 row, one x3 row per wavefront: the four wavefronts of a work-group share
+// the geometry of their 64 (j, i) columns (the 22 metric components and the 2
+// geometric floors), which the work-group stages once in LDS as [component][64]
+// instead of each zone loading its own.  Tiles are numbered i fastest, then k,
+// then j, and renumbered per XCD as above.  (The same staging in the combine
+// kernel, with the connection, measured slower: its loads of fluxes and states
+// dominate, and the barrier keeps its two waves per SIMD from overlapping them.)
+constexpr int TILE_I = 64, TILE_K = BS / TILE_I;
+constexpr int STG_RECOVER = GEO_NCOMP + 2;
+__host__ __device__ __forceinline__ int tiles_i(const BlockDims& b) { return (b.nx1 + TILE_I - 1) / TILE_I; }
+__host__ __device__ __forceinline__ int tiles_k(const BlockDims& b) { return (b.nx3 + TILE_K - 1) / TILE_K; }
+static inline long update_tiles_host(const BlockDims& b) { return (long)tiles_i(b) * tiles_k(b) * b.nx2; }
+// Zone (k, j, i) of this thread; i0 is the tile's first x1 cell.
+__device__ __forceinline__ bool tile_zone(const BlockDims& b, int& k, int& j, int& i, int& i0) {
+    const unsigned lb = xcd_logical_block(blockIdx.x, gridDim.x);
+    const unsigned nti = (unsigned)tiles_i(b), ntk = (unsigned)tiles_k(b);
+    const unsigned ti = lb % nti, rest = lb / nti;
+    const unsigned tk = rest % ntk, jj = rest / ntk;
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / TILE_I));
+    i0 = b.is + (int)ti * TILE_I;
+    i = i0 + (int)(threadIdx.x % TILE_I);
+    k = b.ks + (int)tk * TILE_K + w;
+    j = b.js + (int)jj;
+    return i <= b.ie && k <= b.ke;
+}
+// Stage the tile's metric and floors (x1 clamped to the table for a ragged
+// last tile) into sg; every thread of the work-group calls it.
+__device__ __forceinline__ void stage_recover_geometry(const GeomDev& gd, int j, int i0, double* sg) {
+    const int lane = (int)(threadIdx.x % TILE_I);
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / TILE_I));
+    const long q = (long)j * gd.gn1 + min(i0 + lane, gd.gn1 - 1);
+#pragma unroll
+    for (int c0 = 0; c0 < STG_RECOVER; c0 += TILE_K) {
+        const int comp = c0 + w;
+        if (comp < STG_RECOVER)
+            sg[comp * TILE_I + lane] = comp < GEO_NCOMP ? gd.g[GLOC_CENTER][(long)comp * gd.np + q]
+                                                        : gd.flr[(long)(comp - GEO_NCOMP) * gd.np + q];
+    }
+    __syncthreads();
+}
+
 // ND: dimensionality (flux-CT for 2 and 3); SAME: stage with gam0 = 0 and
 // U_sub == U_full, which reads the state once (0*U + 1*U == U).
 template <typename R, bool DO_DT, int ND, bool SAME>
@@ -553,15 +595,23 @@ combine_kernel(BlockDims b, StepConsts cst, GeomDev gd, int stage, const R* __re
 
 // U->P with the stage's starting primitives as guess (kharma_driver.cpp:379-384),
 // B from U, floors in frame FRAME, P->U; U is read from and written back to Uout.
-template <int FRAME, int INV = 0>
+template <int FRAME, int INV, bool STG>
 __global__ void __launch_bounds__(BS, KH_REC_OCC)
 recover_kernel(BlockDims b, StepConsts cst, GeomDev gd, int stage, const double* __restrict__ Psub, double* Uout,
                double* __restrict__ Pout, float* __restrict__ Pout32, int* __restrict__ pflag,
                int* __restrict__ fflag, int* __restrict__ fail_list, DeviceState* __restrict__ ds) {
-    const unsigned t = xcd_logical_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (t >= (unsigned)ninterior(b)) return;
+    __shared__ double sg[STG ? STG_RECOVER * TILE_I : 1];
     int k, j, i;
-    interior_index_jki(b, t, k, j, i);
+    if constexpr (STG) {
+        int i0;
+        const bool active = tile_zone(b, k, j, i, i0);
+        stage_recover_geometry(gd, j, i0, sg);
+        if (!active) return;
+    } else {
+        const unsigned t = xcd_logical_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+        if (t >= (unsigned)ninterior(b)) return;
+        interior_index_jki(b, t, k, j, i);
+    }
     const long c = (long)k * b.s3 + (long)j * b.s2 + i;
     const double gam = cst.gam;
     double U[NV], P[NV];
@@ -571,9 +621,15 @@ recover_kernel(BlockDims b, StepConsts cst, GeomDev gd, int stage, const double*
     for (int v = 0; v < 5; ++v) P[v] = Psub[v * b.nc + c];
     int pf = 0, ff = 0;
     double rf, uf;
-    geom_floors<1>(cst, gd, j, i, rf, uf);
+    const int lane = (int)(threadIdx.x % TILE_I);
+    if constexpr (STG) {
+        rf = sg[GEO_NCOMP * TILE_I + lane];
+        uf = sg[(GEO_NCOMP + 1) * TILE_I + lane];
+    } else {
+        geom_floors<1>(cst, gd, j, i, rf, uf);
+    }
     {
-        CachedGeom<double> g(gd.g[GLOC_CENTER], gd.np, (long)j * gd.gn1 + i);
+        CachedGeom<double> g(STG ? sg : gd.g[GLOC_CENTER], STG ? TILE_I : gd.np, STG ? lane : (long)j * gd.gn1 + i);
         pf = invert_zone<INV>(g, U, P, cst);
         const int ff0 = (pf / F_GEOM_RHO) * F_GEOM_RHO;
         pf = pf % F_GEOM_RHO;
@@ -1337,6 +1393,8 @@ static void launch_update_geo(const BlockDims& b, const StepConsts& c, const Geo
                               const double* Psub, double* Uout, double* Pout, float* Pout32, int* pflag, int* fflag,
                               int* fail_list, DeviceState* ds, double* dt_part, hipStream_t s, int parts = 3) {
     const dim3 g = grid_for(ninterior_host(b));
+    const bool rstg = b.ndim == 3;   // recover on LDS-staged tiles
+    const dim3 gr = rstg ? dim3((unsigned)update_tiles_host(b)) : g;
     const bool same = (c.gam0[stage - 1] == 0.) && (Usub == Ufull);
     if (parts & 1) {
 #define KH_CMB(DT, ND, SAME)                                                                                          \
@@ -1350,8 +1408,13 @@ static void launch_update_geo(const BlockDims& b, const StepConsts& c, const Geo
 #undef KH_CMB
     }
     if (!(parts & 2)) return;
-#define KH_REC(FR, INV) \
-    hipLaunchKernelGGL((recover_kernel<FR, INV>), g, dim3(BS), 0, s, b, c, gd, stage, Psub, Uout, Pout, Pout32, pflag, fflag, fail_list, ds)
+#define KH_REC(FR, INV)                                                                                             \
+    do {                                                                                                            \
+        if (rstg) hipLaunchKernelGGL((recover_kernel<FR, INV, true>), gr, dim3(BS), 0, s, b, c, gd, stage, Psub,        \
+                                     Uout, Pout, Pout32, pflag, fflag, fail_list, ds);                                  \
+        else hipLaunchKernelGGL((recover_kernel<FR, INV, false>), gr, dim3(BS), 0, s, b, c, gd, stage, Psub, Uout, Pout, \
+                                Pout32, pflag, fflag, fail_list, ds);                                                   \
+    } while (0)
     if (c.inverter == 1) {
         if (c.floors.frame == 0) KH_REC(0, 1);
         else if (c.floors.frame == 1) KH_REC(1, 1);
