@@ -69,6 +69,14 @@ extern "C" {
  * the reference's default, whose own defaults are err_tol 1e-12, iter_max 25. */
 #define KH_INVERTER_ONEDW 0
 #define KH_INVERTER_KASTAUN 1
+#define KH_BSOLVER_FLUX_CT 0
+#define KH_BSOLVER_FACE_CT 1
+/* b_field/ct_scheme (b_ct.cpp:66-68): bs99 (flux average), gs05_0, gs05_c
+ * (upwinded cell-centre EMF correction; sg07 is the same scheme) */
+#define KH_CT_BS99 0
+#define KH_CT_GS05_0 1
+#define KH_CT_GS05_C 2
+#define KH_CT_SG07 3
 
 /* Run parameters of one meshblock: the .par options this path reads
  * (GRMHD/*, flux/*, floors/*, inverter/*, parthenon/time/*, parthenon/mesh/*). */
@@ -113,6 +121,11 @@ typedef struct kh_params {
     int fofc;
     int fofc_polar_cells;
     double fofc_eh_buffer;
+    /* b_field/solver: KH_BSOLVER_FLUX_CT (B_FluxCT, cell-centred B) or
+     * KH_BSOLVER_FACE_CT (B_CT, face-centred B, b_ct/b_ct.cpp), and
+     * b_field/ct_scheme (KH_CT_*) for the latter. */
+    int b_solver;
+    int ct_scheme;
 } kh_params;
 
 typedef struct kh_block kh_block;
@@ -201,6 +214,7 @@ int kh_estimate_timestep(kh_block* b, const void* cmax, const void* cmin, double
 /* B_FluxCT::MaxDivB (b_flux_ct.cpp:566-607).  Synchronises the stream. */
 int kh_max_divb(kh_block* b, const double* U, double* out);
 
+
 /* kh_estimate_timestep for callers that cannot allocate device memory (code
  * under HIP graph capture, serving loops, harnesses that forbid hipMalloc).  No
  * counterpart in the reference.  The same estimate and clip, with the
@@ -277,6 +291,37 @@ int kh_mark_fofc(kh_block* b, const int* fflag, int* fofcflag);
  * zone (the synced fofcflag of the reference). */
 int kh_fofc_v(kh_block* b, const kh_vars* P, const int* fofcflag, const kh_vars F[3], double* cmax, double* cmin);
 
+/* ---- Face-centred constrained transport (the B_CT package, b_ct/b_ct.cpp) ----
+ * With kh_params.b_solver = KH_BSOLVER_FACE_CT the magnetic field is
+ * cons.fB = sqrt(-g) B^d on cell faces: face and edge arrays are
+ * [3][n3+1][n2+1][n1+1] doubles (components F1/F2/F3 or E1/E2/E3; element
+ * (k, j, i) is the lower face / edge of cell (k, j, i)), nf = (n1+1)(n2+1)(n3+1)
+ * per component.  Built for periodic Cartesian Minkowski blocks in 2-D and 3-D
+ * (the reference's boundary EMF treatments, b_ct_boundaries.cpp, rest on
+ * Parthenon's face-field fills), the default driver, both precisions. */
+int kh_block_face_count(const kh_block* b, long* nf);
+/* B_CT::CalculateEMF (b_ct.cpp:263-416): edge EMFs emf[3][nf] over i in [is, ie+1],
+ * j in [js, je+1], k in [ks, ke+1] from the B fluxes F[3] (flux element type by
+ * precision) and, for gs05_0 / gs05_c / sg07, the cell-centre EMF of P. */
+int kh_ct_calculate_emf_v(kh_block* b, const kh_vars* P, const kh_vars F[3], double* emf);
+int kh_ct_calculate_emf(kh_block* b, const double* P, const void* F, double* emf);
+/* B_CT::AddSource (b_ct.cpp:418-470): dfB[3][nf] = dB_Uf/dt on the interior faces
+ * (F_d's own index to e+1; F3 one layer in 2-D) from the edge circulation. */
+int kh_ct_add_source(kh_block* b, const double* emf, double* dfB);
+/* KHARMADriver::WeightedSumDataFace (driver/kharma_driver.hpp:167-190): z = w1 x + w2 y
+ * over [3][nf]; z may alias x or y. */
+int kh_ct_weighted_sum_face(kh_block* b, const double* x, const double* y, double w1, double w2, double* z);
+/* The periodic face-field exchange of the block with itself (the fB part of the
+ * boundary sync, kharma_step.cpp:129-138): ghost faces from their images. */
+int kh_ct_sync_faces(kh_block* b, double* fB);
+/* B_CT::BlockUtoP (b_ct.cpp:173-205) over the entire block: P.B from the face
+ * averages, U.B = sqrt(-g) P.B (ghost faces must be current). */
+int kh_ct_utop_v(kh_block* b, const double* fB, const kh_vars* U, const kh_vars* P);
+int kh_ct_utop(kh_block* b, const double* fB, double* U, double* P);
+/* B_CT::MaxDivB (b_ct.cpp:472-497): max |face divergence| over the interior.
+ * Synchronises the stream. */
+int kh_ct_max_divb(kh_block* b, const double* fB, double* out);
+
 /* ---- Problem set-up and geometry (host only; no device work) -------------- */
 
 /* Fishbone-Moncrief torus with a SANE seed field (torus/rin, rmax, kappa;
@@ -320,6 +365,14 @@ int kh_driver_set_stream(kh_driver* d, void* hip_stream);
 /* Initial primitives P[8][nc] (device pointer): sync, P->U over the entire
  * block, dt = dt_start (KHARMA::PostInitialize, prob/post_initialize.cpp:55-193). */
 int kh_driver_init_prims(kh_driver* d, const double* P);
+/* The face_ct driver's initial state: primitives P[8][nc] (B ignored) and face
+ * fields fB[3][nf] as seeded (prob/seed_B.cpp:137-191, 336-353); face sync,
+ * B_CT::BlockUtoP, then what kh_driver_init_prims does.  kh_driver_get_faces
+ * copies the base container's face fields out (ghost faces synced).  The step
+ * runs B_CT::CalculateEMF and AddSource in place of the flux-CT rewrite
+ * (kharma_step.cpp:175-202) and the per-step divB is B_CT::MaxDivB. */
+int kh_driver_init_faces(kh_driver* d, const double* P, const double* fB);
+int kh_driver_get_faces(kh_driver* d, double* fB);
 int kh_driver_step(kh_driver* d, int nsteps);
 /* Set the clock as a restart does (Parthenon SimTime time / dt / ncycle restored,
  * KHARMA::ResetGlobals, kharma.cpp:107-119): the next step takes dt. */

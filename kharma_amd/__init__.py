@@ -28,6 +28,9 @@ TRANSFORM = {"null": 0, "mks": 1, "fmks": 2}
 BC = {"periodic": 0, "outflow": 1, "reflecting": 2}
 DRIVER = {"kharma": 0, "simple": 1}
 INVERTER = {"onedw": 0, "kastaun": 1}
+# b_field/solver and b_field/ct_scheme (b_ct.cpp:66-68)
+BSOLVER = {"flux_ct": 0, "face_ct": 1}
+CT_SCHEME = {"bs99": 0, "gs05_0": 1, "gs05_c": 2, "sg07": 3}
 
 
 class KhParams(ctypes.Structure):
@@ -61,6 +64,7 @@ class KhParams(ctypes.Structure):
         ("kill_on_large_divb", ctypes.c_int), ("kill_on_divb_over", ctypes.c_double),
         ("inverter", ctypes.c_int),
         ("fofc", ctypes.c_int), ("fofc_polar_cells", ctypes.c_int), ("fofc_eh_buffer", ctypes.c_double),
+        ("b_solver", ctypes.c_int), ("ct_scheme", ctypes.c_int),
     ]
 
 
@@ -130,6 +134,17 @@ def lib():
         "kh_estimate_timestep": (i, [vp, vp, vp, d, ctypes.POINTER(d)]),
         "kh_estimate_timestep_noalloc": (i, [vp, vp, vp, d, vp]),
         "kh_max_divb": (i, [vp, vp, ctypes.POINTER(d)]),
+        "kh_block_face_count": (i, [vp, ctypes.POINTER(l)]),
+        "kh_ct_calculate_emf": (i, [vp, vp, vp, vp]),
+        "kh_ct_calculate_emf_v": (i, [vp, pv, pv, vp]),
+        "kh_ct_add_source": (i, [vp, vp, vp]),
+        "kh_ct_weighted_sum_face": (i, [vp, vp, vp, d, d, vp]),
+        "kh_ct_sync_faces": (i, [vp, vp]),
+        "kh_ct_utop": (i, [vp, vp, vp, vp]),
+        "kh_ct_utop_v": (i, [vp, vp, pv, pv]),
+        "kh_ct_max_divb": (i, [vp, vp, ctypes.POINTER(d)]),
+        "kh_driver_init_faces": (i, [vp, vp, vp]),
+        "kh_driver_get_faces": (i, [vp, vp]),
         "kh_driver_create": (i, [ctypes.POINTER(KhParams), pvp]),
         "kh_driver_destroy": (None, [vp]),
         "kh_driver_set_stream": (i, [vp, vp]),
@@ -245,6 +260,8 @@ def make_params(cfg):
     p.fofc = int(cfg.get("fofc", False))
     p.fofc_polar_cells = cfg.get("fofc_polar_cells", 0)
     p.fofc_eh_buffer = cfg.get("fofc_eh_buffer", 0.1)
+    p.b_solver = BSOLVER[cfg.get("b_solver", "flux_ct")]
+    p.ct_scheme = CT_SCHEME[cfg.get("ct_scheme", "gs05_c")]
     p.gam = cfg["gamma"]
     p.cfl = cfg.get("cfl", 0.9)
     p.dt_min = cfg.get("dt_min", 1e-5)
@@ -456,6 +473,48 @@ class Block:
         _check(lib().kh_max_divb(self.h, _ptr(U), ctypes.byref(out)))
         return out.value
 
+    # ---- face-centred CT (B_CT): face / edge tensors [3, n3+1, n2+1, n1+1] ----
+    @property
+    def face_shape(self):
+        n3, n2, n1 = self.shape
+        return (3, n3 + 1, n2 + 1, n1 + 1)
+
+    def ct_calculate_emf(self, P, F, emf):
+        self._bind()
+        _check(lib().kh_ct_calculate_emf(self.h, _ptr(P), _ptr(F), _ptr(emf)))
+
+    def ct_calculate_emf_v(self, P, F, emf):
+        self._bind()
+        Pv, Fv = vars_of(P), _vars3(F)
+        _check(lib().kh_ct_calculate_emf_v(self.h, ctypes.byref(Pv), Fv, _ptr(emf)))
+
+    def ct_add_source(self, emf, dfB):
+        self._bind()
+        _check(lib().kh_ct_add_source(self.h, _ptr(emf), _ptr(dfB)))
+
+    def ct_weighted_sum_face(self, x, y, w1, w2, z):
+        self._bind()
+        _check(lib().kh_ct_weighted_sum_face(self.h, _ptr(x), _ptr(y), w1, w2, _ptr(z)))
+
+    def ct_sync_faces(self, fB):
+        self._bind()
+        _check(lib().kh_ct_sync_faces(self.h, _ptr(fB)))
+
+    def ct_utop(self, fB, U, P):
+        self._bind()
+        _check(lib().kh_ct_utop(self.h, _ptr(fB), _ptr(U), _ptr(P)))
+
+    def ct_utop_v(self, fB, U, P):
+        self._bind()
+        Uv, Pv = vars_of(U), vars_of(P)
+        _check(lib().kh_ct_utop_v(self.h, _ptr(fB), ctypes.byref(Uv), ctypes.byref(Pv)))
+
+    def ct_max_divb(self, fB):
+        self._bind()
+        out = ctypes.c_double()
+        _check(lib().kh_ct_max_divb(self.h, _ptr(fB), ctypes.byref(out)))
+        return out.value
+
     # ---- pack view: each argument is 8 per-variable tensors (fluxes: 3 x 8) ----
     def get_flux_v(self, d, P, F_d, cmax_d, cmin_d):
         self._bind()
@@ -579,6 +638,27 @@ class Driver:
         self._bind()
         _check(lib().kh_driver_init_prims(self.h, _ptr(P)))
         self._P_init = P  # keep alive until the copy is done
+
+    def init_faces(self, P, fB):
+        """kh_driver_init_faces: primitives P (B ignored) and face fields fB
+        [3, n3+1, n2+1, n1+1] of a face_ct run."""
+        torch = _torch()
+        P = torch.as_tensor(P, dtype=torch.float64).to("cuda").contiguous()
+        fB = torch.as_tensor(fB, dtype=torch.float64).to("cuda").contiguous()
+        n3, n2, n1 = self.shape
+        assert tuple(P.shape) == (NV,) + self.shape and tuple(fB.shape) == (3, n3 + 1, n2 + 1, n1 + 1)
+        self._bind()
+        _check(lib().kh_driver_init_faces(self.h, _ptr(P), _ptr(fB)))
+        self._P_init, self._fB_init = P, fB
+
+    def faces(self):
+        """The base container's face fields [3, n3+1, n2+1, n1+1] (face_ct runs)."""
+        torch = _torch()
+        n3, n2, n1 = self.shape
+        fB = torch.empty((3, n3 + 1, n2 + 1, n1 + 1), dtype=torch.float64, device="cuda")
+        self._bind()
+        _check(lib().kh_driver_get_faces(self.h, _ptr(fB)))
+        return fB
 
     def step(self, n=1):
         self._bind()
@@ -759,7 +839,8 @@ RESTART_MAGIC = b"KHAMDRST"
 
 def read_restart(path):
     """A restart file (kh_driver_save_restart / kh_mesh_save_restart) on the host:
-    dict(time, dt, ncycle, params (KhParams), P, U (float64 [8, n3, n2, n1]))."""
+    dict(time, dt, ncycle, params (KhParams), P, U (float64 [8, n3, n2, n1]), and
+    fB (float64 [3, n3+1, n2+1, n1+1]) for a face_ct run)."""
     import numpy as np
     raw = open(path, "rb").read()
     if raw[:8] != RESTART_MAGIC:
@@ -769,10 +850,15 @@ def read_restart(path):
     time, dt = np.frombuffer(raw, dtype=np.float64, count=2, offset=32)
     params = KhParams.from_buffer_copy(raw[48:48 + ctypes.sizeof(KhParams)])
     cells = 8 * int(n1) * int(n2) * int(n3)
-    data = np.frombuffer(raw, dtype=np.float64, count=2 * cells, offset=int(hbytes))
+    fshape = (3, int(n3) + 1, int(n2) + 1, int(n1) + 1)
+    faces = int(np.prod(fshape)) if params.b_solver == BSOLVER["face_ct"] else 0
+    data = np.frombuffer(raw, dtype=np.float64, count=2 * cells + faces, offset=int(hbytes))
     shape = (8, int(n3), int(n2), int(n1))
-    return dict(time=float(time), dt=float(dt), ncycle=int(ncycle), version=int(version), params=params,
-                P=data[:cells].reshape(shape).copy(), U=data[cells:].reshape(shape).copy())
+    out = dict(time=float(time), dt=float(dt), ncycle=int(ncycle), version=int(version), params=params,
+               P=data[:cells].reshape(shape).copy(), U=data[cells:2 * cells].reshape(shape).copy())
+    if faces:
+        out["fB"] = data[2 * cells:].reshape(fshape).copy()
+    return out
 
 
 class AdjointStep:

@@ -16,7 +16,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libkharma_amd.so")
 INCLUDE = os.path.abspath(os.path.join(HERE, "..", "include"))
 
-SOURCES = ["kh_flux.hip", "kh_step.hip", "kh_adjoint.hip", "kh_mesh.hip", "kh_capi.cpp", "kh_adjoint_capi.cpp",
+SOURCES = ["kh_flux.hip", "kh_step.hip", "kh_adjoint.hip", "kh_mesh.hip", "kh_ct.hip", "kh_capi.cpp", "kh_adjoint_capi.cpp",
            "kh_mesh_capi.cpp", "kh_geom.cpp", "kh_problem.cpp"]
 HEADERS = ["kh_device.hpp", "kh_adjoint.hpp", "kh_internal.hpp", "kh_host.hpp"]
 

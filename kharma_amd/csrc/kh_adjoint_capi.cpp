@@ -164,6 +164,7 @@ int kh_adjoint_create(const kh_params* p, kh_adjoint** out) {
     if (p->recon != KH_RECON_WENO5) return api_fail(KH_ERR_UNSUPPORTED, "the adjoint supports WENO5 reconstruction");
     if (p->use_hlle) return api_fail(KH_ERR_UNSUPPORTED, "the adjoint supports the LLF flux");
     if (p->fofc) return api_fail(KH_ERR_UNSUPPORTED, "the adjoint does not differentiate the first-order flux correction");
+    if (p->b_solver == KH_BSOLVER_FACE_CT) return api_fail(KH_ERR_UNSUPPORTED, "the adjoint differentiates flux-CT only");
     if (p->inverter != KH_INVERTER_ONEDW)
         return api_fail(KH_ERR_UNSUPPORTED, "the adjoint supports the onedw inverter (Kastaun's inverter floors "
                                             "act inside the recovery, outside the implicit-function transpose)");

@@ -82,6 +82,19 @@ int kh::check_params(const kh_params* p) {
         if (p->driver_type != KH_DRIVER_KHARMA) return fail(KH_ERR_UNSUPPORTED, "fofc is a step of the default (kharma) driver");
         if (p->fofc_polar_cells < 0 || !(p->fofc_eh_buffer >= 0.)) return fail(KH_ERR_INVALID, "fofc polar_cells and eh_buffer must be >= 0");
     }
+    if (p->b_solver != KH_BSOLVER_FLUX_CT && p->b_solver != KH_BSOLVER_FACE_CT)
+        return fail(KH_ERR_INVALID, "unknown b_field solver");
+    if (p->b_solver == KH_BSOLVER_FACE_CT) {
+        if (p->ct_scheme < KH_CT_BS99 || p->ct_scheme > KH_CT_SG07) return fail(KH_ERR_INVALID, "unknown ct_scheme");
+        // The EMF treatments B_CT applies at physical boundaries (b_ct_boundaries.cpp)
+        // rest on Parthenon's face-field boundary fills, which are not part of this path.
+        if (p->coords != KH_COORDS_CARTESIAN_MINKOWSKI || !p->periodic1 || (p->nx2 > 1 && !p->periodic2))
+            return fail(KH_ERR_UNSUPPORTED, "face_ct is built for periodic Cartesian Minkowski blocks");
+        if (p->nx2 == 1) return fail(KH_ERR_UNSUPPORTED, "face_ct needs a 2-D or 3-D block");
+        if (p->driver_type != KH_DRIVER_KHARMA)
+            return fail(KH_ERR_UNSUPPORTED, "face_ct runs with the default (kharma) driver (kharma_driver.hpp:94)");
+        if (p->fofc) return fail(KH_ERR_UNSUPPORTED, "fofc with face_ct is not built");
+    }
     if (p->kill_on_large_divb && !(p->kill_on_divb_over > 0.))
         return fail(KH_ERR_INVALID, "kill_on_divb_over must be positive when kill_on_large_divb is set");
     if (p->floor_frame < 0 || p->floor_frame > 2) return fail(KH_ERR_INVALID, "unknown floor frame");
@@ -124,6 +137,11 @@ void kh::make_dims(const kh_params* p, BlockDims& b) {
         b.nbd[d][0] = b.nbd[d][1] = 0;
         b.goff[d] = 0;
     }
+    b.face_ct = p->b_solver == KH_BSOLVER_FACE_CT;
+    b.ct_scheme = p->ct_scheme;
+    b.fs2 = b.n1 + 1;
+    b.fs3 = (long)(b.n1 + 1) * (b.n2 + 1);
+    b.nf = b.fs3 * (b.n3 + 1);
 }
 
 namespace {
@@ -286,6 +304,7 @@ int kh_block_create(const kh_params* p, kh_block** out) {
     if (e == hipSuccess) e = hipMalloc(&b->partials, sizeof(double) * b->npartials);
     if (e == hipSuccess && b->dims.geo) e = upload_geometry(b);
     if (e == hipSuccess) e = upload_fofc_fixed(b);
+    if (e == hipSuccess && b->dims.face_ct) e = hipMalloc(&b->ct_cemf, sizeof(double) * 3 * b->dims.nc);
     if (e != hipSuccess) {
         kh_block_destroy(b);
         return fail(KH_ERR_HIP, std::string("device allocation / copy: ") + hipGetErrorString(e));
@@ -361,6 +380,7 @@ void kh_block_destroy(kh_block* b) {
     if (b->partials) (void)hipFree(b->partials);
     for (void* q : b->geo_bufs) (void)hipFree(q);
     if (b->fofc_fixed) (void)hipFree(b->fofc_fixed);
+    if (b->ct_cemf) (void)hipFree(b->ct_cemf);
     delete b;
 }
 
@@ -584,6 +604,89 @@ int kh_max_divb(kh_block* b, const double* U, double* out) {
     return kh_max_divb_v(b, &Uv, out);
 }
 
+// ---------------------------------------------------------------- B_CT tasks
+namespace {
+int need_face_ct(const kh_block* b) {
+    if (!b) return fail(KH_ERR_INVALID, "null block");
+    if (!b->dims.face_ct) return fail(KH_ERR_INVALID, "the block's b_solver is not face_ct");
+    return KH_OK;
+}
+}  // namespace
+
+int kh_block_face_count(const kh_block* b, long* nf) {
+    if (!b || !nf) return fail(KH_ERR_INVALID, "null argument");
+    *nf = b->dims.nf;
+    return KH_OK;
+}
+
+int kh_ct_calculate_emf_v(kh_block* b, const kh_vars* P, const kh_vars F[3], double* emf) {
+    int st = need_face_ct(b);
+    if (st != KH_OK) return st;
+    if (!vars_ok(P) || !flux_vars_ok(b, F) || !emf) return fail(KH_ERR_INVALID, "null argument");
+    if (b->dims.ct_scheme != KH_CT_BS99)
+        KH_HIP(launch_ct_cemf(b->dims, vars_in<const double>(P), b->ct_cemf, b->stream));
+    KH_HIP(launch_ct_emf(b->dims, b->p.precision, flux_vars_cast<const void>(flux_vars_in(b, F)), b->ct_cemf, emf,
+                         b->stream));
+    return KH_OK;
+}
+
+int kh_ct_calculate_emf(kh_block* b, const double* P, const void* F, double* emf) {
+    if (!b || !P || !F) return fail(KH_ERR_INVALID, "null argument");
+    const kh_vars Pv = contiguous_vars(P, b->dims.nc, 8);
+    kh_vars Fv[3];
+    contiguous_fluxes(F, b->dims.nc, flux_elem(b), Fv);
+    return kh_ct_calculate_emf_v(b, &Pv, Fv, emf);
+}
+
+int kh_ct_add_source(kh_block* b, const double* emf, double* dfB) {
+    int st = need_face_ct(b);
+    if (st != KH_OK) return st;
+    if (!emf || !dfB) return fail(KH_ERR_INVALID, "null argument");
+    KH_HIP(launch_ct_face_update(b->dims, emf, nullptr, nullptr, 0., 0., false, nullptr, 0., dfB, b->stream));
+    return KH_OK;
+}
+
+int kh_ct_weighted_sum_face(kh_block* b, const double* x, const double* y, double w1, double w2, double* z) {
+    int st = need_face_ct(b);
+    if (st != KH_OK) return st;
+    if (!x || !y || !z) return fail(KH_ERR_INVALID, "null argument");
+    KH_HIP(launch_ct_weighted_sum(b->dims, x, y, w1, w2, z, b->stream));
+    return KH_OK;
+}
+
+int kh_ct_sync_faces(kh_block* b, double* fB) {
+    int st = need_face_ct(b);
+    if (st != KH_OK) return st;
+    if (!fB) return fail(KH_ERR_INVALID, "null argument");
+    KH_HIP(launch_ct_sync(b->dims, fB, b->stream));
+    return KH_OK;
+}
+
+int kh_ct_utop_v(kh_block* b, const double* fB, const kh_vars* U, const kh_vars* P) {
+    int st = need_face_ct(b);
+    if (st != KH_OK) return st;
+    if (!fB || !vars_ok(U) || !vars_ok(P)) return fail(KH_ERR_INVALID, "null argument");
+    KH_HIP(launch_ct_utop(b->dims, fB, vars_in<double>(U), vars_in<double>(P), b->stream));
+    return KH_OK;
+}
+
+int kh_ct_utop(kh_block* b, const double* fB, double* U, double* P) {
+    if (!b || !U || !P) return fail(KH_ERR_INVALID, "null argument");
+    const kh_vars Uv = contiguous_vars(U, b->dims.nc, 8), Pv = contiguous_vars(P, b->dims.nc, 8);
+    return kh_ct_utop_v(b, fB, &Uv, &Pv);
+}
+
+int kh_ct_max_divb(kh_block* b, const double* fB, double* out) {
+    int st = need_face_ct(b);
+    if (st != KH_OK) return st;
+    if (!fB || !out) return fail(KH_ERR_INVALID, "null argument");
+    KH_HIP(launch_ct_max_divb(b->dims, fB, b->partials, b->stream));
+    KH_HIP(launch_reduce(b->partials, (int)divb_blocks(b->dims), 1, &b->scratch->last_divb, b->stream));
+    KH_HIP(hipMemcpyAsync(out, &b->scratch->last_divb, sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    KH_HIP(hipStreamSynchronize(b->stream));
+    return KH_OK;
+}
+
 int kh_flags_to_real(kh_block* b, const int* flag, double* real_flag, long n) {
     if (!b || !flag || !real_flag || n < 0) return fail(KH_ERR_INVALID, "bad argument");
     if (n) KH_HIP(launch_flags_to_real(flag, real_flag, n, b->stream));
@@ -626,12 +729,14 @@ int kh_fofc_v(kh_block* b, const kh_vars* P, const int* fofcflag, const kh_vars 
 // grmhd/grmhd.cpp:155-156) -- and Parthenon's SimTime.  Parthenon writes them to
 // HDF5 (.rhdf); this library writes the same content to a flat binary file.
 int kh::write_restart(const char* path, const kh_params& p, const int n[3], double time, double dt, int ncycle,
-                      const double* P, const double* U, hipStream_t s) {
+                      const double* P, const double* U, hipStream_t s, const double* fB) {
     if (!path) return fail(KH_ERR_INVALID, "null path");
     const size_t cells = (size_t)NV * n[0] * n[1] * n[2];
-    std::vector<double> host(2 * cells);
+    const size_t faces = fB ? (size_t)3 * (n[0] + 1) * (n[1] + 1) * (n[2] + 1) : 0;
+    std::vector<double> host(2 * cells + faces);
     KH_HIP(hipMemcpyAsync(host.data(), P, sizeof(double) * cells, hipMemcpyDeviceToHost, s));
     KH_HIP(hipMemcpyAsync(host.data() + cells, U, sizeof(double) * cells, hipMemcpyDeviceToHost, s));
+    if (fB) KH_HIP(hipMemcpyAsync(host.data() + 2 * cells, fB, sizeof(double) * faces, hipMemcpyDeviceToHost, s));
     KH_HIP(hipStreamSynchronize(s));
     RestartHeader hd{};
     std::memcpy(hd.magic, "KHAMDRST", 8);
@@ -650,7 +755,7 @@ int kh::write_restart(const char* path, const kh_params& p, const int n[3], doub
 }
 
 int kh::read_restart(const char* path, const kh_params& p, const int n[3], double* time, double* dt, int* ncycle,
-                     double* P, double* U, hipStream_t s) {
+                     double* P, double* U, hipStream_t s, double* fB) {
     if (!path) return fail(KH_ERR_INVALID, "null path");
     FILE* f = std::fopen(path, "rb");
     if (!f) return fail(KH_ERR_INVALID, std::string("cannot open ") + path);
@@ -666,12 +771,16 @@ int kh::read_restart(const char* path, const kh_params& p, const int n[3], doubl
     if (q.coords != p.coords || q.transform != p.transform || q.a != p.a || q.gam != p.gam || q.x1min != p.x1min ||
         q.x1max != p.x1max || q.x2min != p.x2min || q.x2max != p.x2max || q.x3min != p.x3min || q.x3max != p.x3max)
         return bad("the geometry or gamma differ from the run's");
+    if (q.b_solver != p.b_solver || (fB == nullptr) != (q.b_solver != KH_BSOLVER_FACE_CT))
+        return bad("the B field solver (flux_ct / face_ct) differs from the run's");
     const size_t cells = (size_t)NV * n[0] * n[1] * n[2];
-    std::vector<double> host(2 * cells);
+    const size_t faces = fB ? (size_t)3 * (n[0] + 1) * (n[1] + 1) * (n[2] + 1) : 0;
+    std::vector<double> host(2 * cells + faces);
     if (std::fread(host.data(), sizeof(double), host.size(), f) != host.size()) return bad("truncated");
     std::fclose(f);
     KH_HIP(hipMemcpyAsync(P, host.data(), sizeof(double) * cells, hipMemcpyHostToDevice, s));
     KH_HIP(hipMemcpyAsync(U, host.data() + cells, sizeof(double) * cells, hipMemcpyHostToDevice, s));
+    if (fB) KH_HIP(hipMemcpyAsync(fB, host.data() + 2 * cells, sizeof(double) * faces, hipMemcpyHostToDevice, s));
     KH_HIP(hipStreamSynchronize(s));
     *time = hd.time;
     *dt = hd.dt;
@@ -720,6 +829,14 @@ int kh_driver_create(const kh_params* p, kh_driver** out) {
         if ((e = hipMalloc(&d->fofc_mark, sizeof(int) * nc)) != hipSuccess) return cleanup(e, "hipMalloc fofc_mark");
         if ((e = hipMemset(d->fofc_mark, 0, sizeof(int) * nc)) != hipSuccess) return cleanup(e, "hipMemset");
     }
+    if (blk->dims.face_ct) {
+        const size_t fb = sizeof(double) * 3 * blk->dims.nf;
+        for (double** q : {&d->fB0, &d->fB1, &d->emf}) {
+            if ((e = hipMalloc(q, fb)) != hipSuccess) return cleanup(e, "hipMalloc face fields");
+            if ((e = hipMemset(*q, 0, fb)) != hipSuccess) return cleanup(e, "hipMemset");
+        }
+        if ((e = hipMemset(blk->ct_cemf, 0, sizeof(double) * 3 * nc)) != hipSuccess) return cleanup(e, "hipMemset");
+    }
     // Zero everything once at creation (outside any captured work): the signal
     // speeds of an inactive direction must read 0, as the reference's zero-initialised
     // Flux.cmax/cmin do for X3 in 2D.
@@ -744,7 +861,8 @@ void kh_driver_destroy(kh_driver* d) {
     for (hipEvent_t e : d->ev_pool) (void)hipEventDestroy(e);
     for (void* ptr : {(void*)d->U0, (void*)d->P0, (void*)d->U1, (void*)d->P1, (void*)d->P0f, (void*)d->P1f, d->F, d->ctop,
                       (void*)d->pflag, (void*)d->fflag, (void*)d->fail_list, (void*)d->ds, (void*)d->dt_hist,
-                      (void*)d->dt_part, (void*)d->divb_part, (void*)d->obj, (void*)d->fofc_mark})
+                      (void*)d->dt_part, (void*)d->divb_part, (void*)d->obj, (void*)d->fofc_mark,
+                      (void*)d->fB0, (void*)d->fB1, (void*)d->emf})
         if (ptr) (void)hipFree(ptr);
     kh_block_destroy(d->blk);
     delete d;
@@ -770,6 +888,7 @@ int kh_driver_arrays(kh_driver* d, double** U, double** P) {
 
 int kh_driver_init_prims(kh_driver* d, const double* P) {
     if (!d || !P) return fail(KH_ERR_INVALID, "null argument");
+    if (d->blk->dims.face_ct) return fail(KH_ERR_INVALID, "a face_ct driver starts from kh_driver_init_faces");
     kh_block* b = d->blk;
     const size_t st_bytes = sizeof(double) * NV * b->dims.nc;
     KH_HIP(hipMemcpyAsync(d->P0, P, st_bytes, hipMemcpyDeviceToDevice, b->stream));
@@ -784,6 +903,41 @@ int kh_driver_init_prims(kh_driver* d, const double* P) {
         KH_HIP(launch_convert_f32(d->P0, d->P1f, NV * b->dims.nc, b->stream));
     }
     KH_HIP(launch_reset_state(d->ds, b->p.dt_start, b->stream));
+    return KH_OK;
+}
+
+int kh_driver_init_faces(kh_driver* d, const double* P, const double* fB) {
+    if (!d || !P || !fB) return fail(KH_ERR_INVALID, "null argument");
+    kh_block* b = d->blk;
+    const BlockDims& bd = b->dims;
+    if (!bd.face_ct) return fail(KH_ERR_INVALID, "kh_driver_init_faces needs b_solver = KH_BSOLVER_FACE_CT");
+    const size_t st_bytes = sizeof(double) * NV * bd.nc, fb = sizeof(double) * 3 * bd.nf;
+    hipStream_t s = b->stream;
+    KH_HIP(hipMemcpyAsync(d->P0, P, st_bytes, hipMemcpyDeviceToDevice, s));
+    KH_HIP(hipMemcpyAsync(d->fB0, fB, fb, hipMemcpyDeviceToDevice, s));
+    KH_HIP(launch_ct_sync(bd, d->fB0, s));
+    KH_HIP(launch_sync(bd, d->P0, NV, s));
+    // B_CT::BlockUtoP over the entire block (prob/seed_B.cpp:191), then PostInitialize's P->U
+    KH_HIP(launch_ct_utop(bd, d->fB0, vars_of(d->U0, bd.nc), vars_of(d->P0, bd.nc), s));
+    KH_HIP(launch_ptou(bd, b->consts, b->geo, d->P0, d->U0, s));
+    KH_HIP(launch_sync(bd, d->U0, NV, s));
+    KH_HIP(hipMemcpyAsync(d->P1, d->P0, st_bytes, hipMemcpyDeviceToDevice, s));
+    KH_HIP(hipMemcpyAsync(d->U1, d->U0, st_bytes, hipMemcpyDeviceToDevice, s));
+    KH_HIP(hipMemcpyAsync(d->fB1, d->fB0, fb, hipMemcpyDeviceToDevice, s));
+    if (d->P0f) {
+        KH_HIP(launch_convert_f32(d->P0, d->P0f, NV * bd.nc, s));
+        KH_HIP(launch_convert_f32(d->P0, d->P1f, NV * bd.nc, s));
+    }
+    KH_HIP(launch_reset_state(d->ds, b->p.dt_start, s));
+    return KH_OK;
+}
+
+int kh_driver_get_faces(kh_driver* d, double* fB) {
+    if (!d || !fB) return fail(KH_ERR_INVALID, "null argument");
+    const BlockDims& bd = d->blk->dims;
+    if (!bd.face_ct) return fail(KH_ERR_INVALID, "the driver has no face fields (b_solver is flux_ct)");
+    KH_HIP(launch_ct_sync(bd, d->fB0, d->blk->stream));
+    KH_HIP(hipMemcpyAsync(fB, d->fB0, sizeof(double) * 3 * bd.nf, hipMemcpyDeviceToDevice, d->blk->stream));
     return KH_OK;
 }
 
@@ -848,7 +1002,22 @@ int kh::driver_stage(kh_driver* d, int stage) {
     const int q = stage - 1;
     if (d->rec && physical)
         KH_HIP(launch_save_x1_flux(bd, (const double*)d->F, (double*)(d->rec + L.x1f[q]), s));
-    if (d->rec) {
+    if (bd.face_ct) {
+        // B_CT::CalculateEMF (kharma_step.cpp:183-190) on the stage's fluxes and
+        // primitives, B_CT::AddSource with the face half of AddStateUpdate, then
+        // the cell update with the cell B from the new faces
+        double* fBsub = (stage == 1) ? d->fB0 : d->fB1;
+        double* fBout = (stage == 1) ? d->fB1 : d->fB0;
+        const bool same = (c.gam0[stage - 1] == 0.) && (fBsub == d->fB0);
+        if (bd.ct_scheme != KH_CT_BS99) KH_TIMED(3, launch_ct_cemf(bd, vars_of(Psub, bd.nc), b->ct_cemf, s));
+        KH_TIMED(3, launch_ct_emf(bd, prec, flux_vars_cast<const void>(flux_vars_prec(d->F, prec, bd.nc)), b->ct_cemf,
+                                  d->emf, s));
+        KH_TIMED(3, launch_ct_face_update(bd, d->emf, fBsub, d->fB0, c.gam0[stage - 1], c.gam1[stage - 1], same, d->ds,
+                                          c.beta[stage - 1], fBout, s));
+        KH_TIMED(3 + stage, launch_update_faces(bd, c, b->geo, prec, stage, stage == 2, d->F, d->ctop, Usub, d->U0, Psub,
+                                                Uout, Pout, Poutf, d->pflag, d->fflag, d->fail_list, d->ds, d->dt_part,
+                                                fBout, s));
+    } else if (d->rec) {
         KH_TIMED(3 + stage, launch_update_recorded(bd, c, b->geo, stage, stage == 2, d->F, d->ctop, Usub, d->U0, Psub,
                                                    Uout, Pout, d->pflag, d->fflag, d->fail_list, d->ds, d->dt_part,
                                                    (double*)(d->rec + L.Ustar[q]), s));
@@ -887,7 +1056,8 @@ int kh_driver_step(kh_driver* d, int nsteps) {
             const int st = driver_stage(d, stage);
             if (st != KH_OK) return st;
         }
-        KH_TIMED(7, launch_max_divb(bd, d->U0, d->divb_part, s));
+        if (bd.face_ct) KH_TIMED(7, launch_ct_max_divb(bd, d->fB0, d->divb_part, s));
+        else KH_TIMED(7, launch_max_divb(bd, d->U0, d->divb_part, s));
         KH_TIMED(8, launch_finalize(c, d->ds, d->dt_part, n_dt, d->divb_part, bd.ndim > 1 ? n_divb : 0,
                                     d->dt_hist, d->hist_cap, s));
     }
@@ -910,7 +1080,8 @@ int kh_driver_save_restart(kh_driver* d, const char* path) {
     KH_HIP(hipMemcpyAsync(&h, d->ds, sizeof(h), hipMemcpyDeviceToHost, b->stream));
     KH_HIP(hipStreamSynchronize(b->stream));
     const int n[3] = {b->dims.n1, b->dims.n2, b->dims.n3};
-    return write_restart(path, b->p, n, h.time, h.dt, h.ncycle, d->P0, d->U0, b->stream);
+    if (b->dims.face_ct) KH_HIP(launch_ct_sync(b->dims, d->fB0, b->stream));
+    return write_restart(path, b->p, n, h.time, h.dt, h.ncycle, d->P0, d->U0, b->stream, d->fB0);
 }
 
 int kh_driver_load_restart(kh_driver* d, const char* path) {
@@ -919,9 +1090,11 @@ int kh_driver_load_restart(kh_driver* d, const char* path) {
     const int n[3] = {b->dims.n1, b->dims.n2, b->dims.n3};
     double time = 0., dt = 0.;
     int ncycle = 0;
-    const int st = read_restart(path, b->p, n, &time, &dt, &ncycle, d->P0, d->U0, b->stream);
+    const int st = read_restart(path, b->p, n, &time, &dt, &ncycle, d->P0, d->U0, b->stream, d->fB0);
     if (st != KH_OK) return st;
     const size_t st_bytes = sizeof(double) * NV * b->dims.nc;
+    if (d->fB0)
+        KH_HIP(hipMemcpyAsync(d->fB1, d->fB0, sizeof(double) * 3 * b->dims.nf, hipMemcpyDeviceToDevice, b->stream));
     KH_HIP(hipMemcpyAsync(d->P1, d->P0, st_bytes, hipMemcpyDeviceToDevice, b->stream));
     KH_HIP(hipMemcpyAsync(d->U1, d->U0, st_bytes, hipMemcpyDeviceToDevice, b->stream));
     if (d->P0f) {
@@ -984,6 +1157,7 @@ int kh_driver_set_stage_record(kh_driver* d, void* rec) {
     if (!d) return fail(KH_ERR_INVALID, "null driver");
     if (rec && d->blk->p.precision != KH_PREC_FP64)
         return fail(KH_ERR_UNSUPPORTED, "step records are kept by the FP64 driver (the adjoint's form)");
+    if (rec && d->blk->dims.face_ct) return fail(KH_ERR_UNSUPPORTED, "the adjoint does not cover face_ct");
     if (rec && (reinterpret_cast<uintptr_t>(rec) & 255))
         return fail(KH_ERR_INVALID, "a step record must be 256-byte aligned");
     d->rec = static_cast<char*>(rec);

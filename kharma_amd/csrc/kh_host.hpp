@@ -19,6 +19,8 @@ struct kh_block {
     // First-order flux correction (kh_params.fofc): zones MarkFOFC marks whatever
     // the trial state, [gn2][gn1] like the geometry tables (spherical coordinates only)
     int* fofc_fixed = nullptr;
+    // Face-centred CT (b_solver == KH_BSOLVER_FACE_CT): cell-centre EMFs [3][nc]
+    double* ct_cemf = nullptr;
 };
 
 struct kh_driver {
@@ -39,6 +41,9 @@ struct kh_driver {
     char* rec = nullptr;
     // first-order flux correction (kh_params.fofc): marks [nc]
     int* fofc_mark = nullptr;
+    // face-centred CT: face fields of the base and stage-1 containers and the
+    // edge EMFs, [3][nf] each
+    double *fB0 = nullptr, *fB1 = nullptr, *emf = nullptr;
     // kernel timing (kh_driver_set_kernel_timing)
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
@@ -74,10 +79,12 @@ struct RestartHeader {
     kh_params params;       // the run's parameters (the reader checks the mesh against them)
 };
 // Write / read the clock and the state; the device arrays hold 8 n1 n2 n3 doubles.
+// With face-centred B the face fields [3][n3+1][n2+1][n1+1] follow U (fB non-null
+// exactly when the run's b_solver is face_ct).
 int write_restart(const char* path, const kh_params& p, const int n[3], double time, double dt, int ncycle,
-                  const double* P, const double* U, hipStream_t s);
+                  const double* P, const double* U, hipStream_t s, const double* fB = nullptr);
 int read_restart(const char* path, const kh_params& p, const int n[3], double* time, double* dt, int* ncycle,
-                 double* P, double* U, hipStream_t s);
+                 double* P, double* U, hipStream_t s, double* fB = nullptr);
 // Bytes per element of the face-flux arrays (FP32 in the split form).
 inline size_t flux_elem(const kh_block* b) { return b->p.precision == KH_PREC_SPLIT ? sizeof(float) : sizeof(double); }
 // One RK stage (1 or 2) of the driver's step on its current buffers.

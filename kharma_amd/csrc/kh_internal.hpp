@@ -42,6 +42,11 @@ struct BlockDims {
     // Interior offset of this block in the mesh (zones per axis); coordinates are
     // x = xmin(mesh) + (i - is + goff + 1/2) dx with the mesh's dx.
     int goff[3];
+    // Face-centred constrained transport (B_CT): face and edge arrays are
+    // [3][n3+1][n2+1][n1+1], index (k fs3 + j fs2 + i) for the lower face / edge of
+    // cell (k, j, i), component stride nf.
+    int face_ct, ct_scheme;
+    long fs2, fs3, nf;
 };
 // The block has a physical face (outflow / reflecting ghosts to fill).
 __host__ __device__ inline bool has_physical_face(const BlockDims& b) {
@@ -218,6 +223,31 @@ hipError_t launch_update_recorded(const BlockDims& b, const StepConsts& c, const
 // acc += w dt sum_faces F_dir[var] dA (one stage of a face-flux objective).
 hipError_t launch_face_flux_objective(const BlockDims& b, int prec, const void* F, int dir, int var, int face, double w,
                                       const DeviceState* ds, double* acc, hipStream_t s);
+// ---- face-centred constrained transport (kh_ct.hip) ----
+// Cell-centre EMFs [3][nc] over the entire block from the primitives (Minkowski).
+hipError_t launch_ct_cemf(const BlockDims& b, const Vars<const double>& P, double* cemf, hipStream_t s);
+// Edge EMFs [3][nf] from the face fluxes (element type by precision) and cemf
+// (unused by bs99), scheme b.ct_scheme.
+hipError_t launch_ct_emf(const BlockDims& b, int prec, const FluxVars<const void>& F, const double* cemf, double* emf,
+                         hipStream_t s);
+// Faces from the edge circulation.  ds == nullptr: fBout = dB_Uf/dt on the
+// interior face ranges; otherwise fBout = 1 (g0 fBsub + g1 fBfull) + beta dt dB/dt
+// with dt from ds (same: the stage reads fBfull alone).
+hipError_t launch_ct_face_update(const BlockDims& b, const double* emf, const double* fBsub, const double* fBfull,
+                                 double g0, double g1, bool same, const DeviceState* ds, double beta, double* fBout,
+                                 hipStream_t s);
+hipError_t launch_ct_weighted_sum(const BlockDims& b, const double* x, const double* y, double w1, double w2, double* z,
+                                  hipStream_t s);
+hipError_t launch_ct_sync(const BlockDims& b, double* fB, hipStream_t s);
+hipError_t launch_ct_utop(const BlockDims& b, const double* fB, const Vars<double>& U, const Vars<double>& P,
+                          hipStream_t s);
+// divb_blocks(b) partial maxima of the face divergence of the interior cells.
+hipError_t launch_ct_max_divb(const BlockDims& b, const double* fB, double* part, hipStream_t s);
+// The flat stage update with the cell B taken from the stage's new faces fB.
+hipError_t launch_update_faces(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int prec, int stage,
+                               bool do_dt, const void* F, const void* ctop, const double* Usub, const double* Ufull,
+                               const double* Psub, double* Uout, double* Pout, float* Pout32, int* pflag, int* fflag,
+                               int* fail_list, DeviceState* ds, double* dt_part, const double* fB, hipStream_t s);
 long update_blocks(const BlockDims& b);
 long divb_blocks(const BlockDims& b);
 hipError_t launch_fixup(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int stage, double* U, double* P,

@@ -118,6 +118,8 @@ int kh_mesh_create(const kh_params* p, int nb1, int nb2, int nb3, kh_mesh** out)
     }
     if (p->fofc && nb1 * nb2 * nb3 > 1)
         return api_fail(KH_ERR_UNSUPPORTED, "fofc runs on one-block meshes only");
+    if (p->b_solver == KH_BSOLVER_FACE_CT)
+        return api_fail(KH_ERR_UNSUPPORTED, "face_ct runs on one-block meshes (kh_driver_*) only");
     kh_mesh* m = new kh_mesh();
     m->p = *p;
     MeshDims& md = m->md;

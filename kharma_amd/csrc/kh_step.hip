@@ -219,7 +219,7 @@ __global__ void __launch_bounds__(BS) ct_rewrite_kernel(BlockDims b, FluxVars<R>
 // evaluated from the EMFs when `ct` is set.
 template <typename R>
 __device__ __forceinline__ void zone_divergence(const BlockDims& b, const FluxVars<const R>& F, const R* __restrict__ emf,
-                                                long c, bool ct, double dudt[NV]) {
+                                                long c, bool ct, double dudt[NV], bool skip_b = false) {
     const double A1 = b.dx2 * b.dx3, A2 = b.dx1 * b.dx3, A3 = b.dx1 * b.dx2, V = b.dx1 * b.dx2 * b.dx3;
     const double iV = 1. / V;   // one division per zone instead of eight
     const Vars<const R>& F1 = F[0];
@@ -230,9 +230,10 @@ __device__ __forceinline__ void zone_divergence(const BlockDims& b, const FluxVa
     double f1lo[NV], f1hi[NV], f2lo[NV], f2hi[NV], f3lo[NV], f3hi[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
+        f1lo[v] = f1hi[v] = f2lo[v] = f2hi[v] = f3lo[v] = f3hi[v] = 0.;
+        if (skip_b && v >= B1) continue;   // face-centred B: the cell B comes from the faces
         f1lo[v] = (double)F1[v][c];
         f1hi[v] = (double)F1[v][c + 1];
-        f2lo[v] = f2hi[v] = f3lo[v] = f3hi[v] = 0.;
         if (nd >= 2 && !(ct && v >= B1)) { f2lo[v] = (double)F2[v][c]; f2hi[v] = (double)F2[v][c + s2]; }
         if (nd == 3 && !(ct && v >= B1)) { f3lo[v] = (double)F3[v][c]; f3hi[v] = (double)F3[v][c + s3]; }
     }
@@ -391,13 +392,16 @@ __device__ __forceinline__ int invert_zone(const G& g, const double U[NV], doubl
 
 // ------------------------------------------------------- fused stage update
 // Stage 1 (gam0 = 0 and U_sub == U_full) reads the state once: 0*U + 1*U == U.
-template <typename R, bool DO_DT, bool CT, int GEO, int INV = 0>
+// BM: 0 no B transport (1-D), 1 flux-CT EMFs from the face fluxes, 2 face-centred
+// B (B_CT): the cell B is the average of the stage's new faces fB (B_CT::BlockUtoP,
+// b_ct.cpp:173-205, which the reference runs before the inversion).
+template <typename R, bool DO_DT, int BM, int GEO, int INV = 0>
 __global__ void __launch_bounds__(BS, UpdateOcc<GEO>::min_blocks)
 update_kernel(BlockDims b, StepConsts cst, GeomDev gd, int stage, const R* __restrict__ F, const R* __restrict__ ctop,
               const double* __restrict__ Usub, const double* Ufull, const double* __restrict__ Psub,
               double* Uout, double* __restrict__ Pout, float* __restrict__ Pout32,
               int* __restrict__ pflag, int* __restrict__ fflag, int* __restrict__ fail_list,
-              DeviceState* __restrict__ ds, double* __restrict__ dt_part) {
+              DeviceState* __restrict__ ds, double* __restrict__ dt_part, const double* __restrict__ fB) {
     const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = t < (unsigned)ninterior(b);
     double ndt = DBL_MAX;
@@ -407,8 +411,8 @@ update_kernel(BlockDims b, StepConsts cst, GeomDev gd, int stage, const R* __res
         const long c = (long)k * b.s3 + (long)j * b.s2 + i;
         const double gam = cst.gam;
         double dudt[NV];
-        if (CT) zone_divergence_ct<R>(b, F, c, dudt);
-        else zone_divergence<R>(b, flux_vars_of(F, b.nc), nullptr, c, false, dudt);
+        if (BM == 1) zone_divergence_ct<R>(b, F, c, dudt);
+        else zone_divergence<R>(b, flux_vars_of(F, b.nc), nullptr, c, false, dudt, BM == 2);
         if constexpr (GEO) {
             double Ps[NV];
 #pragma unroll
@@ -426,6 +430,12 @@ update_kernel(BlockDims b, StepConsts cst, GeomDev gd, int stage, const R* __res
             const double uf = Ufull[v * b.nc + c];
             const double u = same ? uf : g0 * Usub[v * b.nc + c] + g1 * uf;
             U[v] = 1.0 * u + beta_dt * dudt[v];
+        }
+        if constexpr (BM == 2) {
+            const long f = (long)k * b.fs3 + (long)j * b.fs2 + i;
+            U[B1] = (fB[f] + fB[f + 1]) / 2;
+            U[B2] = (fB[b.nf + f] + fB[b.nf + f + b.fs2]) / 2;
+            U[B3] = (b.ndim > 2) ? (fB[2 * b.nf + f] + fB[2 * b.nf + f + b.fs3]) / 2 : fB[2 * b.nf + f];
         }
         // U->P guess: the stage's starting primitives (kharma_driver.cpp:379-384)
 #pragma unroll
@@ -1430,20 +1440,34 @@ template <typename R, int GEO>
 static void launch_update_t(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int stage, bool do_dt,
                             const void* F, const void* ctop, const double* Usub, const double* Ufull, const double* Psub,
                             double* Uout, double* Pout, float* Pout32, int* pflag, int* fflag,
-                            int* fail_list, DeviceState* ds, double* dt_part, hipStream_t s) {
+                            int* fail_list, DeviceState* ds, double* dt_part, hipStream_t s, const double* fB = nullptr) {
     const dim3 g = grid_for(ninterior_host(b));
-    const bool ct = b.ndim > 1;
-#define KH_UPD(DT, CT, INV)                                                                                         \
-    hipLaunchKernelGGL((update_kernel<R, DT, CT, GEO, INV>), g, dim3(BS), 0, s, b, c, gd, stage, (const R*)F, (const R*)ctop, \
-                       Usub, Ufull, Psub, Uout, Pout, Pout32, pflag, fflag, fail_list, ds, dt_part)
+    const int bm = fB ? 2 : (b.ndim > 1 ? 1 : 0);
+#define KH_UPD(DT, BM, INV)                                                                                         \
+    hipLaunchKernelGGL((update_kernel<R, DT, BM, GEO, INV>), g, dim3(BS), 0, s, b, c, gd, stage, (const R*)F, (const R*)ctop, \
+                       Usub, Ufull, Psub, Uout, Pout, Pout32, pflag, fflag, fail_list, ds, dt_part, fB)
+#define KH_UPD_BM(DT, INV) \
+    { if (bm == 2) KH_UPD(DT, 2, INV); else if (bm == 1) KH_UPD(DT, 1, INV); else KH_UPD(DT, 0, INV); }
     if (c.inverter == 1) {
-        if (do_dt) { if (ct) KH_UPD(true, true, 1); else KH_UPD(true, false, 1); }
-        else { if (ct) KH_UPD(false, true, 1); else KH_UPD(false, false, 1); }
+        if (do_dt) KH_UPD_BM(true, 1) else KH_UPD_BM(false, 1)
     } else {
-        if (do_dt) { if (ct) KH_UPD(true, true, 0); else KH_UPD(true, false, 0); }
-        else { if (ct) KH_UPD(false, true, 0); else KH_UPD(false, false, 0); }
+        if (do_dt) KH_UPD_BM(true, 0) else KH_UPD_BM(false, 0)
     }
+#undef KH_UPD_BM
 #undef KH_UPD
+}
+hipError_t launch_update_faces(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int prec, int stage,
+                               bool do_dt, const void* F, const void* ctop, const double* Usub, const double* Ufull,
+                               const double* Psub, double* Uout, double* Pout, float* Pout32, int* pflag, int* fflag,
+                               int* fail_list, DeviceState* ds, double* dt_part, const double* fB, hipStream_t s) {
+    if (b.geo || !fB) return hipErrorInvalidValue;
+    if (prec == PREC_SPLIT)
+        launch_update_t<float, 0>(b, c, gd, stage, do_dt, F, ctop, Usub, Ufull, Psub, Uout, Pout, Pout32, pflag, fflag,
+                                  fail_list, ds, dt_part, s, fB);
+    else
+        launch_update_t<double, 0>(b, c, gd, stage, do_dt, F, ctop, Usub, Ufull, Psub, Uout, Pout, Pout32, pflag, fflag,
+                                   fail_list, ds, dt_part, s, fB);
+    return hipGetLastError();
 }
 hipError_t launch_update(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int prec, int stage, bool do_dt,
                          const void* F, const void* ctop, const double* Usub, const double* Ufull, const double* Psub,

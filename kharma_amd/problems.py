@@ -129,6 +129,101 @@ def orszag_tang_prims(cfg):
     return periodic_fill(P, cfg)
 
 
+# ------------------------------------------------------ face-centred B (B_CT)
+def face_shape(cfg):
+    """Shape of a face-field array of the block: [3][n3+1][n2+1][n1+1] (F1, F2, F3)."""
+    _, (n3, n2, n1) = block_shape(cfg)
+    return (3, n3 + 1, n2 + 1, n1 + 1)
+
+
+def face_positions(cfg, f):
+    """Coordinates (x, y, z) of face component f (0: F1 ... 2: F3) over the extended
+    index space [n3+1][n2+1][n1+1]: the face's own axis at the cell's lower face,
+    the others at cell centres."""
+    ng = cfg.get("ng", 4)
+    ndim, (n3, n2, n1) = block_shape(cfg)
+    out = []
+    for d, n in enumerate((n1, n2, n3)):
+        lo, hi = cfg["bounds"][d]
+        dx = (hi - lo) / cfg["nx"][d]
+        g = ng if d < ndim else 0
+        idx = np.arange(n + 1) - g
+        out.append(lo + (idx + (0.0 if d == f else 0.5)) * dx)
+    Z, Y, X = np.meshgrid(out[2], out[1], out[0], indexing="ij")
+    return X, Y, Z
+
+
+def orszag_tang_faces(cfg):
+    """Face fields of the Orszag-Tang vortex (sqrt(-g) = 1).  init "reference":
+    the edge curl (b_ct_functions.hpp:76-88) of A3 = seed_a<orszag_tang_a>
+    (prob/seed_B.hpp:133-138) at the zone corners (prob/seed_B.cpp:250-353);
+    init "literal": the literal field at each face centre
+    (seed_b<orszag_tang>, prob/seed_B.hpp:182-187, with amplitude 0.05 and no phase)."""
+    fB = np.zeros(face_shape(cfg))
+    ng = cfg.get("ng", 4)
+    if cfg.get("ot_init", "literal") == "literal":
+        X, Y, _ = face_positions(cfg, 0)
+        fB[0] = -0.05 * np.sin(Y)
+        X, Y, _ = face_positions(cfg, 1)
+        fB[1] = 0.05 * np.sin(2.0 * X)
+        return _trim_faces(fB, cfg)
+    ts, ph = cfg["ot_tscale"], cfg["ot_phase"]
+    (x0, x1), (y0, y1) = cfg["bounds"][0], cfg["bounds"][1]
+    dx, dy = (x1 - x0) / cfg["nx"][0], (y1 - y0) / cfg["nx"][1]
+    _, (n3, n2, n1) = block_shape(cfg)
+    xc = x0 + dx * (np.arange(n1 + 1) - ng)
+    yc = y0 + dy * (np.arange(n2 + 1) - ng)
+    XC, YC = np.meshgrid(xc, yc, indexing="xy")          # corners [j][i]
+    A = ts * (-0.5 * np.cos(2.0 * XC + ph) + np.cos(YC + ph))
+    fB[0, :, :-1, :] = ((A[1:, :] - A[:-1, :]) / dy)[None]
+    fB[1, :, :, :-1] = (-(A[:, 1:] - A[:, :-1]) / dx)[None]
+    return _trim_faces(fB, cfg)
+
+
+def mhdmode_faces(cfg, phase=0.0):
+    """Face fields of the linear mode: seed_b<wave> (prob/seed_B.hpp:161-169) at
+    each face's centre (prob/seed_B.cpp:137-191)."""
+    ev = MODES_DIR3[cfg["mode"]]
+    amp = cfg["amp"]
+    fB = np.zeros(face_shape(cfg))
+    k1 = k2 = 2.0 * math.pi
+    for f, (bg, key) in enumerate(((1.0, "dB1"), (0.0, "dB2"), (0.0, "dB3"))):
+        X, Y, _ = face_positions(cfg, f)
+        fB[f] = bg + ev.get(key, 0.) * amp * np.cos(k1 * X + k2 * Y + phase)
+    return _trim_faces(fB, cfg)
+
+
+def _trim_faces(fB, cfg):
+    """Zero the slots of the extended layout that hold no face (an axis's extra
+    index across a face, and along an inactive axis)."""
+    ndim, _ = block_shape(cfg)
+    for f in range(3):
+        for d in range(3):
+            ax = 3 - d       # array axis of spatial direction d
+            if d != f or d >= ndim:
+                idx = [slice(None)] * 4
+                idx[0] = f
+                idx[ax] = -1
+                fB[tuple(idx)] = 0.0
+    return fB
+
+
+def face_divergence(fB, cfg):
+    """Face divergence (b_ct_functions.hpp:47-56) of every interior cell."""
+    ndim, _ = block_shape(cfg)
+    sl = interior(cfg)
+    dx = [(cfg["bounds"][d][1] - cfg["bounds"][d][0]) / cfg["nx"][d] for d in range(3)]
+    k, j, i = sl
+    def sh(s, d):
+        return slice(s.start + d, s.stop + d)
+    div = (fB[0][k, j, sh(i, 1)] - fB[0][k, j, i]) / dx[0]
+    if ndim > 1:
+        div = div + (fB[1][k, sh(j, 1), i] - fB[1][k, j, i]) / dx[1]
+    if ndim > 2:
+        div = div + (fB[2][sh(k, 1), j, i] - fB[2][k, j, i]) / dx[2]
+    return div
+
+
 # ------------------------------------------------------------- MHD modes
 # Eigenvectors and frequencies for the (1,1,0) wave, prob/mhdmodes.hpp:130-191
 # (dir = 3), valid for gamma = 4/3 and background rho = u = 1, B = (1, 0, 0).

@@ -159,6 +159,8 @@ struct or_params {
     int inverter;                      // inverter/type: 0 onedw, 1 kastaun
     int fofc, fofc_polar_cells;        // fofc/on, fofc/polar_cells
     double fofc_eh_buffer;             // fofc/eh_buffer
+    int face_ct;                       // b_field/solver: 0 flux_ct, 1 face_ct (the B_CT package)
+    int ct_scheme;                     // b_field/ct_scheme: 0 bs99, 1 gs05_0, 2 gs05_c, 3 sg07
 };
 }
 
@@ -187,11 +189,18 @@ struct Block {
     // First-order flux correction, last call of fofc(): interior marks, the trial
     // primitives and each trial state's relative distance to its nearest floor threshold
     std::vector<Real> fofc_mark, fofc_trial, fofc_margin;
+    // B_CT: face fields cons.fB of the base and stage-1 containers, edge EMFs
+    // B_CT.emf, their rate dB_Uf/dt and the cell-centre EMFs B_CT.cemf.  Face and
+    // edge arrays are [3][n3+1][n2+1][n1+1] (component F1/F2/F3 or E1/E2/E3).
+    int fn1 = 0, fn2 = 0, fn3 = 0;
+    size_t nf = 0;
+    std::vector<Real> fB0, fB1c, femf, dfB, cemf;
     double time = 0., dt = 0., dt_last = 0.;
     long ncycle = 0;
     bool in_loop = false;
 
     size_t id(int k, int j, int i) const { return ((size_t)k * n2 + j) * n1 + i; }
+    size_t fid(int k, int j, int i) const { return ((size_t)k * fn2 + j) * fn1 + i; }
     size_t gidx(int loc, int j, int i) const { return ((size_t)loc * gn2 + j) * gn1 + i; }
     Real gcov_(int loc, int j, int i, int mu, int nu) const { return gcov[gidx(loc, j, i) * 16 + mu * 4 + nu]; }
     Real gcon_(int loc, int j, int i, int mu, int nu) const { return gcon[gidx(loc, j, i) * 16 + mu * 4 + nu]; }
@@ -664,6 +673,246 @@ void flux_ct(Block& b) {
     }
 }
 
+// A deliberate mistake planted in the restatement by a test (or_plant), to show
+// that the test's pin fails on it.  0: none.  1: the geometric source contracts
+// T^mu_nu with Gamma^lam_{nu mu} (upper and first lower index exchanged) instead
+// of Gamma^nu_{lam mu}.  2: the drift-frame floors solve for the parallel
+// velocity with the pre-floor enthalpy.  3: B_CT's AddSource takes the F1 face's
+// upper E3 edge one cell along x1 instead of along x2.  4: B_CT's gs05_c
+// upwinding takes the downwind side.
+static int g_plant = 0;
+
+// ============================================================== B_CT
+// Face-centred constrained transport (b_ct/b_ct.cpp).  Parthenon's
+// UniformCartesian element sizes (absent from the tree): an edge along x_d has
+// length dx_d, a face normal to x_d area dx_e dx_f, whatever its position.
+inline Real& fface(Block& b, std::vector<Real>& A, int el, int k, int j, int i) {
+    return A[(size_t)el * b.nf + b.fid(k, j, i)];
+}
+
+// B_CT::CalculateEMF, b_ct.cpp:263-416: the BS99 average of the B fluxes onto
+// each edge (the whole of bs99), then the GS05 corrections from the cell-centre
+// EMF -(v x B) sqrt(-g): gs05_0 subtracts its 4-cell average from twice the
+// average; gs05_c / sg07 add the upwinded differences, the upwind side chosen
+// by the sign of the mass flux through the neighbouring faces.  Edges over
+// GetRange(interior, 0, 1): i in [is, ie+1], j in [js, je+1], k in [ks, ke+1]
+// (collapsed in inactive directions).
+void ct_calculate_emf(Block& b, const std::vector<Real>& P) {
+    const int nd = b.ndim;
+    auto F = [&](int dir, int v, int k, int j, int i) -> Real {
+        return b.F[((size_t)(dir-1) * NV + v) * b.nc + b.id(k, j, i)];
+    };
+    auto E = [&](int e, int k, int j, int i) -> Real& { return fface(b, b.femf, e, k, j, i); };
+    auto C = [&](int v, int k, int j, int i) -> Real { return b.cemf[(size_t)v * b.nc + b.id(k, j, i)]; };
+    const int ile = b.ie + 1, jle = (nd > 1) ? b.je + 1 : b.je, kle = (nd > 2) ? b.ke + 1 : b.ke;
+#pragma omp parallel for collapse(2)
+    for (int k = b.ks; k <= kle; ++k)
+    for (int j = b.js; j <= jle; ++j)
+    for (int i = b.is; i <= ile; ++i) {
+        if (nd > 2) {
+            E(0, k, j, i) = 0.25 * (-F(2, B3, k - 1, j, i) - F(2, B3, k, j, i)
+                                    + F(3, B2, k, j - 1, i) + F(3, B2, k, j, i));
+            E(1, k, j, i) = 0.25 * (-F(3, B1, k, j, i - 1) - F(3, B1, k, j, i)
+                                    + F(1, B3, k - 1, j, i) + F(1, B3, k, j, i));
+            E(2, k, j, i) = 0.25 * (-F(1, B2, k, j - 1, i) - F(1, B2, k, j, i)
+                                    + F(2, B1, k, j, i - 1) + F(2, B1, k, j, i));
+        } else if (nd > 1) {
+            E(0, k, j, i) = -F(2, B3, k, j, i);
+            E(1, k, j, i) = F(1, B3, k, j, i);
+            E(2, k, j, i) = 0.25 * (-F(1, B2, k, j - 1, i) - F(1, B2, k, j, i)
+                                    + F(2, B1, k, j, i - 1) + F(2, B1, k, j, i));
+        } else {
+            E(0, k, j, i) = 0;
+            E(1, k, j, i) = F(1, B3, k, j, i);
+            E(2, k, j, i) = -F(1, B2, k, j, i);
+        }
+    }
+    if (nd < 2 || b.p.ct_scheme == 0) return;
+    // cell-centre EMF over the entire block (b_ct.cpp:318-335)
+#pragma omp parallel for collapse(2)
+    for (int k = 0; k < b.n3; ++k)
+    for (int j = 0; j < b.n2; ++j)
+    for (int i = 0; i < b.n1; ++i) {
+        const size_t c = b.id(k, j, i);
+        Real Pc[NV]; load_prims(b, P, c, Pc);
+        FourVectors D;
+        calc_4vecs(b, Pc, j, i, CENTER, D);
+        const Real gdet = b.gdet_(CENTER, j, i);
+        b.cemf[0 * b.nc + c] = (D.bcon[2] * D.ucon[3] - D.bcon[3] * D.ucon[2]) * gdet;
+        b.cemf[1 * b.nc + c] = (D.bcon[3] * D.ucon[1] - D.bcon[1] * D.ucon[3]) * gdet;
+        b.cemf[2 * b.nc + c] = (D.bcon[1] * D.ucon[2] - D.bcon[2] * D.ucon[1]) * gdet;
+    }
+    if (b.p.ct_scheme == 1) {
+        // gs05_0, b_ct.cpp:337-356.  The E1 average repeats (k, j-1) and omits
+        // (k-1, j), as the reference writes it.
+#pragma omp parallel for collapse(2)
+        for (int k = b.ks; k <= kle; ++k)
+        for (int j = b.js; j <= jle; ++j)
+        for (int i = b.is; i <= ile; ++i) {
+            if (nd > 2) {
+                E(0, k, j, i) = 2 * E(0, k, j, i)
+                    - 0.25 * (C(0, k, j, i) + C(0, k, j - 1, i) + C(0, k, j - 1, i) + C(0, k - 1, j - 1, i));
+                E(1, k, j, i) = 2 * E(1, k, j, i)
+                    - 0.25 * (C(1, k, j, i) + C(1, k, j, i - 1) + C(1, k - 1, j, i) + C(1, k - 1, j, i - 1));
+            }
+            E(2, k, j, i) = 2 * E(2, k, j, i)
+                - 0.25 * (C(2, k, j, i) + C(2, k, j, i - 1) + C(2, k, j - 1, i) + C(2, k, j - 1, i - 1));
+        }
+        return;
+    }
+    // gs05_c / sg07, b_ct.cpp:357-412
+    auto up = [&](Real mass_flux) { return g_plant == 4 ? (mass_flux < 0.0) : (mass_flux >= 0.0); };
+#pragma omp parallel for collapse(2)
+    for (int k = b.ks; k <= kle; ++k)
+    for (int j = b.js; j <= jle; ++j)
+    for (int i = b.is; i <= ile; ++i) {
+        if (nd > 2) {
+            const Real e1_l3 = up(F(2, RHO, k - 1, j, i)) ? F(3, B2, k, j - 1, i) - C(0, k - 1, j - 1, i)
+                                                               : F(3, B2, k, j, i) - C(0, k - 1, j, i);
+            const Real e1_r3 = up(F(2, RHO, k, j, i)) ? F(3, B2, k, j - 1, i) - C(0, k, j - 1, i)
+                                                           : F(3, B2, k, j, i) - C(0, k, j, i);
+            const Real e1_l2 = up(F(3, RHO, k, j - 1, i)) ? -F(2, B3, k - 1, j, i) - C(0, k - 1, j - 1, i)
+                                                               : -F(2, B3, k, j, i) - C(0, k, j - 1, i);
+            const Real e1_r2 = up(F(3, RHO, k, j, i)) ? -F(2, B3, k - 1, j, i) - C(0, k - 1, j, i)
+                                                           : -F(2, B3, k, j, i) - C(0, k, j, i);
+            E(0, k, j, i) += 0.25 * (e1_l3 + e1_r3 + e1_l2 + e1_r2);
+            const Real e2_l3 = up(F(1, RHO, k - 1, j, i)) ? -F(3, B1, k, j, i - 1) - C(1, k - 1, j, i - 1)
+                                                               : -F(3, B1, k, j, i) - C(1, k - 1, j, i);
+            const Real e2_r3 = up(F(1, RHO, k, j, i)) ? -F(3, B1, k, j, i - 1) - C(1, k, j, i - 1)
+                                                           : -F(3, B1, k, j, i) - C(1, k, j, i);
+            const Real e2_l1 = up(F(3, RHO, k, j, i - 1)) ? F(1, B3, k - 1, j, i) - C(1, k - 1, j, i - 1)
+                                                               : F(1, B3, k, j, i) - C(1, k, j, i - 1);
+            const Real e2_r1 = up(F(3, RHO, k, j, i)) ? F(1, B3, k - 1, j, i) - C(1, k - 1, j, i)
+                                                           : F(1, B3, k, j, i) - C(1, k, j, i);
+            E(1, k, j, i) += 0.25 * (e2_l3 + e2_r3 + e2_l1 + e2_r1);
+        }
+        const Real e3_l2 = up(F(1, RHO, k, j - 1, i)) ? F(2, B1, k, j, i - 1) - C(2, k, j - 1, i - 1)
+                                                           : F(2, B1, k, j, i) - C(2, k, j - 1, i);
+        const Real e3_r2 = up(F(1, RHO, k, j, i)) ? F(2, B1, k, j, i - 1) - C(2, k, j, i - 1)
+                                                       : F(2, B1, k, j, i) - C(2, k, j, i);
+        const Real e3_l1 = up(F(2, RHO, k, j, i - 1)) ? -F(1, B2, k, j - 1, i) - C(2, k, j - 1, i - 1)
+                                                           : -F(1, B2, k, j, i) - C(2, k, j, i - 1);
+        const Real e3_r1 = up(F(2, RHO, k, j, i)) ? -F(1, B2, k, j - 1, i) - C(2, k, j - 1, i)
+                                                       : -F(1, B2, k, j, i) - C(2, k, j, i);
+        E(2, k, j, i) += 0.25 * (e3_l2 + e3_r2 + e3_l1 + e3_r1);
+    }
+}
+
+// B_CT::AddSource, b_ct.cpp:418-470: dB_Uf/dt on each face from the circulation
+// of the edge EMFs, per unit face area.  Faces over GetRange(interior, F_d): the
+// face direction's index runs to e+1 (in 2-D F3 has its one layer k = ks).
+void ct_add_source(Block& b) {
+    const int nd = b.ndim;
+    const Real dx1 = b.dx[0], dx2 = b.dx[1], dx3 = b.dx[2];
+    auto E = [&](int e, int k, int j, int i) -> Real { return fface(b, b.femf, e, k, j, i); };
+    auto D = [&](int f, int k, int j, int i) -> Real& { return fface(b, b.dfB, f, k, j, i); };
+    const int jle = (nd > 1) ? b.je + 1 : b.je, kle = (nd > 2) ? b.ke + 1 : b.ke;
+#pragma omp parallel for collapse(2)
+    for (int k = b.ks; k <= b.ke; ++k)
+    for (int j = b.js; j <= b.je; ++j)
+    for (int i = b.is; i <= b.ie + 1; ++i) {
+        D(0, k, j, i) = (-dx3 * (g_plant == 3 ? E(2, k, j, i + 1) : E(2, k, j + 1, i)) + dx3 * E(2, k, j, i));
+        if (nd > 2) D(0, k, j, i) += (dx2 * E(1, k + 1, j, i) - dx2 * E(1, k, j, i));
+        D(0, k, j, i) /= (dx2 * dx3);
+    }
+#pragma omp parallel for collapse(2)
+    for (int k = b.ks; k <= b.ke; ++k)
+    for (int j = b.js; j <= jle; ++j)
+    for (int i = b.is; i <= b.ie; ++i) {
+        D(1, k, j, i) = (dx3 * E(2, k, j, i + 1) - dx3 * E(2, k, j, i));
+        if (nd > 2) D(1, k, j, i) += (-dx1 * E(0, k + 1, j, i) + dx1 * E(0, k, j, i));
+        D(1, k, j, i) /= (dx1 * dx3);
+    }
+#pragma omp parallel for collapse(2)
+    for (int k = b.ks; k <= kle; ++k)
+    for (int j = b.js; j <= b.je; ++j)
+    for (int i = b.is; i <= b.ie; ++i) {
+        D(2, k, j, i) = (-dx2 * E(1, k, j, i + 1) + dx2 * E(1, k, j, i)
+                         + dx1 * E(0, k, j + 1, i) - dx1 * E(0, k, j, i)) / (dx1 * dx2);
+    }
+}
+
+// KHARMADriver::WeightedSumDataFace, driver/kharma_driver.hpp:167-190: z = w1 x + w2 y
+// over the whole face arrays.
+void ct_weighted_sum_face(Block& b, const std::vector<Real>& x, const std::vector<Real>& y, Real w1, Real w2,
+                          std::vector<Real>& z) {
+    const size_t n = (size_t)3 * b.nf;
+#pragma omp parallel for
+    for (size_t q = 0; q < n; ++q) z[q] = w1 * x[q] + w2 * y[q];
+}
+
+// Periodic exchange of the face fields of one block with itself (Parthenon's
+// face-field boundary communication, not in the reference tree): along a face's
+// own direction its faces s..e+1 are the block's and the others take the value
+// N faces in; across, its cells wrap as a cell field's do.
+void ct_sync_faces(Block& b, std::vector<Real>& A) {
+    const int N[3] = {b.p.nx1, b.p.nx2, b.p.nx3};
+    const int per[3] = {b.p.periodic1, b.ndim > 1 && b.p.periodic2, b.ndim > 2 && b.p.periodic3};
+    const int lo[3] = {b.is, b.js, b.ks}, hi[3] = {b.ie, b.je, b.ke}, n[3] = {b.n1, b.n2, b.n3};
+    for (int f = 0; f < 3; ++f) {
+        // face index bounds per axis: along f one more face than cells
+        int top[3];
+        for (int d = 0; d < 3; ++d) top[d] = (d == f && d < b.ndim) ? n[d] : n[d] - 1;
+        for (int d = 0; d < 3; ++d) {
+            if (!per[d]) continue;
+            const int ehi = (d == f) ? hi[d] + 1 : hi[d];
+#pragma omp parallel for collapse(2)
+            for (int k = 0; k <= top[2]; ++k)
+            for (int j = 0; j <= top[1]; ++j)
+            for (int i = 0; i <= top[0]; ++i) {
+                int x[3] = {i, j, k};
+                if (x[d] >= lo[d] && x[d] <= ehi) continue;
+                int y[3] = {i, j, k};
+                y[d] = (x[d] < lo[d]) ? x[d] + N[d] : x[d] - N[d];
+                fface(b, A, f, k, j, i) = fface(b, A, f, y[2], y[1], y[0]);
+            }
+        }
+    }
+}
+
+// B_CT::BlockUtoP, b_ct.cpp:173-205, over the entire block: cell-centred
+// primitive B from the faces (each divided by its sqrt(-g)), conserved B.
+void ct_block_utop(Block& b, std::vector<Real>& fB, std::vector<Real>& U, std::vector<Real>& P) {
+    const int nd = b.ndim;
+#pragma omp parallel for collapse(2)
+    for (int k = 0; k < b.n3; ++k)
+    for (int j = 0; j < b.n2; ++j)
+    for (int i = 0; i < b.n1; ++i) {
+        const size_t c = b.id(k, j, i);
+        const Real p1 = (fface(b, fB, 0, k, j, i) / b.gdet_(FACE1, j, i)
+                         + fface(b, fB, 0, k, j, i + 1) / b.gdet_(FACE1, j, i + 1)) / 2;
+        const Real p2 = (nd > 1) ? (fface(b, fB, 1, k, j, i) / b.gdet_(FACE2, j, i)
+                                    + fface(b, fB, 1, k, j + 1, i) / b.gdet_(FACE2, j + 1, i)) / 2
+                                 : fface(b, fB, 1, k, j, i) / b.gdet_(FACE2, j, i);
+        const Real p3 = (nd > 2) ? (fface(b, fB, 2, k, j, i) / b.gdet_(FACE3, j, i)
+                                    + fface(b, fB, 2, k + 1, j, i) / b.gdet_(FACE3, j, i)) / 2
+                                 : fface(b, fB, 2, k, j, i) / b.gdet_(FACE3, j, i);
+        P[(size_t)B1 * b.nc + c] = p1;
+        P[(size_t)B2 * b.nc + c] = p2;
+        P[(size_t)B3 * b.nc + c] = p3;
+        for (int v = 0; v < 3; ++v)
+            U[(size_t)(B1 + v) * b.nc + c] = P[(size_t)(B1 + v) * b.nc + c] * b.gdet_(CENTER, j, i);
+    }
+}
+
+// B_CT::MaxDivB, b_ct.cpp:472-497, with face_div (b_ct_functions.hpp:47-56) on
+// the interior cells.
+double ct_max_divb(const Block& b, const std::vector<Real>& fB) {
+    const int nd = b.ndim;
+    Block& bb = const_cast<Block&>(b);
+    std::vector<Real>& A = const_cast<std::vector<Real>&>(fB);
+    double m = 0.;
+    for (int k = b.ks; k <= b.ke; ++k)
+    for (int j = b.js; j <= b.je; ++j)
+    for (int i = b.is; i <= b.ie; ++i) {
+        Real du = (fface(bb, A, 0, k, j, i + 1) - fface(bb, A, 0, k, j, i)) / b.dx[0];
+        if (nd > 1) du += (fface(bb, A, 1, k, j + 1, i) - fface(bb, A, 1, k, j, i)) / b.dx[1];
+        if (nd > 2) du += (fface(bb, A, 2, k + 1, j, i) - fface(bb, A, 2, k, j, i)) / b.dx[2];
+        m = std::max(m, std::fabs(val(du)));
+    }
+    return m;
+}
+
 // KHARMADriver::FluxDivergence, driver/kharma_driver.hpp:208-231 ->
 // parthenon::Update::FluxDivHelper (not in the reference tree; restated from
 // UniformCartesian face areas / cell volume -- summation order parity unpinned).
@@ -688,12 +937,6 @@ void flux_divergence(Block& b) {
     }
 }
 
-// A deliberate mistake planted in the restatement by a test (or_plant), to show
-// that the test's pin fails on it.  0: none.  1: the geometric source contracts
-// T^mu_nu with Gamma^lam_{nu mu} (upper and first lower index exchanged) instead
-// of Gamma^nu_{lam mu}.  2: the drift-frame floors solve for the parallel
-// velocity with the pre-floor enthalpy.
-static int g_plant = 0;
 
 // Flux::AddGeoSource, flux/flux.cpp:395-444.  Skipped in Cartesian Minkowski (:407).
 void add_geo_source(Block& b, const std::vector<Real>& P) {
@@ -1044,6 +1287,10 @@ void utop(Block& b, std::vector<Real>& U, std::vector<Real>& P, bool b_interior 
         b.pflag[c] = pflagl % FFLAG_MINIMUM;
         b.fflag[c] = (pflagl / FFLAG_MINIMUM) * FFLAG_MINIMUM;
     }
+    // B_FluxCT::BlockUtoP (b_flux_ct.cpp:172-217); with B_CT the primitive B came
+    // from the faces before the inversion (Packages::BlockUtoP runs B_CT first,
+    // kharma_package.cpp:58-90)
+    if (b.p.face_ct) return;
     const Range3 br = b_interior ? Range3{b.is, b.ie, b.js, b.je, b.ks, b.ke}
                                  : Range3{0, b.n1 - 1, 0, b.n2 - 1, 0, b.n3 - 1};
 #pragma omp parallel for collapse(2)
@@ -1631,19 +1878,40 @@ void run_stage(Block& b, int stage) {
     for (int dir = 1; dir <= 3; ++dir) get_flux(b, Psub, dir);
     // AddFOFC on the raw fluxes (kharma_step.cpp:166-172), before the flux fixes
     if (b.p.fofc) fofc(b, stage, Usub, Psub);
-    // Packages::FixFlux: B_FluxCT (ZeroBoundaryFlux, FluxCT), then KBoundaries
-    zero_polar_b_flux(b);
-    flux_ct(b);
-    kboundaries_fix_flux(b);
-    flux_divergence(b);
-    add_geo_source(b, Psub);
     Real gam0, gam1, beta;
     integrator_weights(b.p.integrator, stage, gam0, gam1, beta);
+    std::vector<Real>& fBsub = (stage == 1) ? b.fB0 : b.fB1c;
+    std::vector<Real>& fBout = (stage == 1) ? b.fB1c : b.fB0;
+    if (b.p.face_ct) {
+        // Packages::FixFlux (KBoundaries only), then B_CT::CalculateEMF
+        // (kharma_step.cpp:175-190); FluxDivergence does not update the derived
+        // cell-centred B, B_CT::AddSource sets dB_Uf/dt (kharma_step.cpp:196-202)
+        kboundaries_fix_flux(b);
+        ct_calculate_emf(b, Psub);
+        flux_divergence(b);
+        add_geo_source(b, Psub);
+        ct_add_source(b);
+    } else {
+        // Packages::FixFlux: B_FluxCT (ZeroBoundaryFlux, FluxCT), then KBoundaries
+        zero_polar_b_flux(b);
+        flux_ct(b);
+        kboundaries_fix_flux(b);
+        flux_divergence(b);
+        add_geo_source(b, Psub);
+    }
     state_update(b, Usub, b.U0, Psub, Uout, Pout, gam0, gam1, beta * b.dt);
+    if (b.p.face_ct) {
+        // AddStateUpdate's face half (kharma_driver.cpp:353-373)
+        ct_weighted_sum_face(b, fBsub, b.fB0, gam0, gam1, fBout);
+        ct_weighted_sum_face(b, fBout, b.dfB, 1.0, beta * b.dt, fBout);
+    }
     // boundary exchange, which applies the physical boundaries too (kharma_driver.cpp:148-150)
     sync_bounds(b, Uout, Pout);
+    if (b.p.face_ct) ct_sync_faces(b, fBout);
     apply_physical_bcs(b, Uout, Pout);
-    // Fix region (kharma_step.cpp:216-288)
+    // Fix region (kharma_step.cpp:216-288); Packages::MeshUtoP runs B_CT's
+    // BlockUtoP over the entire block before the inversion
+    if (b.p.face_ct) ct_block_utop(b, fBout, Uout, Pout);
     utop(b, Uout, Pout);
     apply_floors(b, Uout, Pout);
     fix_utop(b, Uout, Pout);
@@ -1746,6 +2014,13 @@ void* or_create(const or_params* p) {
     b->emf.assign(3 * b->nc, 0.); b->dUdt.assign(nvc, 0.);
     b->pflag.assign(b->nc, 0.); b->fflag.assign(b->nc, 0.);
     b->rho_floor.assign(b->nc, 0.); b->u_floor.assign(b->nc, 0.);
+    b->fn1 = b->n1 + 1; b->fn2 = b->n2 + 1; b->fn3 = b->n3 + 1;
+    b->nf = (size_t)b->fn1 * b->fn2 * b->fn3;
+    if (p->face_ct) {
+        b->fB0.assign(3 * b->nf, 0.); b->fB1c.assign(3 * b->nf, 0.);
+        b->femf.assign(3 * b->nf, 0.); b->dfB.assign(3 * b->nf, 0.);
+        b->cemf.assign(3 * b->nc, 0.);
+    }
     return b;
 }
 
@@ -1771,6 +2046,42 @@ void or_init_state(void* h, const double* P) {
     b.time = 0.; b.ncycle = 0; b.in_loop = false;
     b.dt = b.p.dt_start;
     b.dt_last = b.dt;
+}
+
+// Initial state with face-centred B (B_CT): the primitives P (B ignored) and the
+// face fields fB [3][n3+1][n2+1][n1+1] as seeded (prob/seed_B.cpp:137-191,
+// 336-353): face sync, B_CT::BlockUtoP over the entire block, then what
+// or_init_state does.
+void or_init_state_faces(void* h, const double* P, const double* fB) {
+    Block& b = *static_cast<Block*>(h);
+    load_values(b.P0, P, (size_t)NV * b.nc);
+    load_values(b.fB0, fB, 3 * b.nf);
+    ct_sync_faces(b, b.fB0);
+    for (int v = 0; v < NV; ++v) sync_periodic_var(b, b.P0, v);
+    ct_block_utop(b, b.fB0, b.U0, b.P0);
+    ptou(b, b.P0, b.U0);
+    sync_bounds(b, b.U0, b.P0);
+    apply_physical_bcs(b, b.U0, b.P0);
+    b.time = 0.; b.ncycle = 0; b.in_loop = false;
+    b.dt = b.p.dt_start;
+    b.dt_last = b.dt;
+}
+size_t or_nfaces(void* h) { return static_cast<Block*>(h)->nf; }
+// B_CT task entry points on the block's arrays: CalculateEMF from the stored
+// fluxes and P0; AddSource into dfB; BlockUtoP of fB0 into U0, P0; the face sync
+// of fB0; MaxDivB of fB0; WeightedSumDataFace on caller arrays [3][nf].
+void or_ct_calculate_emf(void* h) { Block& b = *static_cast<Block*>(h); ct_calculate_emf(b, b.P0); }
+void or_ct_add_source(void* h) { ct_add_source(*static_cast<Block*>(h)); }
+void or_ct_block_utop(void* h) { Block& b = *static_cast<Block*>(h); ct_block_utop(b, b.fB0, b.U0, b.P0); }
+void or_ct_sync_faces(void* h) { Block& b = *static_cast<Block*>(h); ct_sync_faces(b, b.fB0); }
+double or_ct_max_divb(void* h) { Block& b = *static_cast<Block*>(h); return ct_max_divb(b, b.fB0); }
+void or_ct_weighted_sum_face(void* h, const double* x, const double* y, double w1, double w2, double* z) {
+    Block& b = *static_cast<Block*>(h);
+    const size_t n = 3 * b.nf;
+    std::vector<Real> X(n), Y(n), Z(n);
+    load_values(X, x, n); load_values(Y, y, n);
+    ct_weighted_sum_face(b, X, Y, w1, w2, Z);
+    store_values(Z, z, n);
 }
 
 // Overwrite base-container U and P directly (no P->U), for task-level tests.
@@ -1841,7 +2152,8 @@ double or_max_divb(void* h) { Block& b = *static_cast<Block*>(h); return max_div
 
 // Copy out internal arrays: which = 0 F (3*NV*nc), 1 cmax (3*nc), 2 cmin (3*nc),
 // 3 dUdt (NV*nc), 4 pflag (nc), 5 fflag (nc), 6 emf (3*nc), 7 Pl, 8 Pr (NV*nc),
-// after fofc(): 9 marks (nc), 10 trial primitives (NV*nc), 11 floor margins (nc)
+// after fofc(): 9 marks (nc), 10 trial primitives (NV*nc), 11 floor margins (nc);
+// B_CT: 12 face fields fB0, 13 edge EMFs, 14 dB_Uf/dt (3*nf each), 15 cell-centre EMFs (3*nc)
 void or_get_array(void* h, int which, double* out) {
     Block& b = *static_cast<Block*>(h);
     const std::vector<Real>* src = nullptr;
@@ -1850,6 +2162,8 @@ void or_get_array(void* h, int which, double* out) {
     case 3: src = &b.dUdt; break; case 4: src = &b.pflag; break; case 5: src = &b.fflag; break;
     case 6: src = &b.emf; break; case 7: src = &b.Pl; break; case 8: src = &b.Pr; break;
     case 9: src = &b.fofc_mark; break; case 10: src = &b.fofc_trial; break; case 11: src = &b.fofc_margin; break;
+    case 12: src = &b.fB0; break; case 13: src = &b.femf; break; case 14: src = &b.dfB; break;
+    case 15: src = &b.cemf; break;
     default: return;
     }
     store_values(*src, out, src->size());
@@ -1860,6 +2174,7 @@ void or_set_array(void* h, int which, const double* in) {
     switch (which) {
     case 0: dst = &b.F; break; case 1: dst = &b.cmax; break; case 2: dst = &b.cmin; break;
     case 3: dst = &b.dUdt; break; case 4: dst = &b.pflag; break; case 5: dst = &b.fflag; break;
+    case 12: dst = &b.fB0; break; case 13: dst = &b.femf; break; case 14: dst = &b.dfB; break;
     default: return;
     }
     load_values(*dst, in, dst->size());

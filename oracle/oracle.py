@@ -49,6 +49,7 @@ class OrParams(ctypes.Structure):
         ("periodic1", ctypes.c_int), ("periodic2", ctypes.c_int), ("periodic3", ctypes.c_int),
         ("driver", ctypes.c_int), ("inverter", ctypes.c_int),
         ("fofc", ctypes.c_int), ("fofc_polar_cells", ctypes.c_int), ("fofc_eh_buffer", ctypes.c_double),
+        ("face_ct", ctypes.c_int), ("ct_scheme", ctypes.c_int),
     ]
 
 
@@ -106,6 +107,14 @@ def lib(variant=""):
         L.or_plant.argtypes = [i]
         L.or_weighted_sum.argtypes = [vp, dp, dp, d, d, d, dp]
         L.or_sync_vars.argtypes = [vp, dp, i]
+        L.or_init_state_faces.argtypes = [vp, dp, dp]
+        L.or_nfaces.restype = ctypes.c_size_t
+        L.or_nfaces.argtypes = [vp]
+        for name in ("or_ct_calculate_emf", "or_ct_add_source", "or_ct_block_utop", "or_ct_sync_faces"):
+            getattr(L, name).argtypes = [vp]
+        L.or_ct_max_divb.restype = d
+        L.or_ct_max_divb.argtypes = [vp]
+        L.or_ct_weighted_sum_face.argtypes = [vp, dp, dp, d, d, dp]
         if variant == "tangent":
             L.or_set_tangent.argtypes = [vp, dp, dp]
             L.or_get_tangent.argtypes = [vp, dp, dp]
@@ -122,6 +131,7 @@ FRAME = {"fluid": 0, "normal": 1, "drift": 2}
 TRANSFORM = {"null": 0, "mks": 1, "fmks": 2}
 BC = {"periodic": 0, "outflow": 1, "reflecting": 2}
 DRIVER = {"kharma": 0, "simple": 1}
+CT_SCHEME = {"bs99": 0, "gs05_0": 1, "gs05_c": 2, "sg07": 3}
 
 
 def make_params(cfg):
@@ -185,6 +195,8 @@ def make_params(cfg):
     p.fofc = int(cfg.get("fofc", False))
     p.fofc_polar_cells = cfg.get("fofc_polar_cells", 0)
     p.fofc_eh_buffer = cfg.get("fofc_eh_buffer", 0.1)
+    p.face_ct = int(cfg.get("b_solver", "flux_ct") == "face_ct")
+    p.ct_scheme = CT_SCHEME[cfg.get("ct_scheme", "gs05_c")]
     return p
 
 
@@ -204,6 +216,8 @@ class OracleBlock:
         n3 = nx3 + 2 * ng if self.ndim > 2 else 1
         self.shape = (n3, n2, n1)
         self.nc = n1 * n2 * n3
+        # face / edge arrays of B_CT: [3][n3+1][n2+1][n1+1]
+        self.fshape = (3, n3 + 1, n2 + 1, n1 + 1)
 
     def __del__(self):
         try:
@@ -258,8 +272,12 @@ class OracleBlock:
         return self.L.or_get_time(self.h)
 
     def array(self, which):
+        if which in (12, 13, 14):
+            out = np.empty(self.fshape)
+            self.L.or_get_array(self.h, which, _ptr(out))
+            return out
         sizes = {0: (3, NV), 1: (3,), 2: (3,), 3: (NV,), 4: (), 5: (), 6: (3,), 7: (NV,), 8: (NV,),
-                 9: (), 10: (NV,), 11: ()}
+                 9: (), 10: (NV,), 11: (), 15: (3,)}
         out = np.empty(sizes[which] + self.shape)
         self.L.or_get_array(self.h, which, _ptr(out))
         return out
@@ -310,6 +328,39 @@ class OracleBlock:
         out = np.empty_like(a)
         self.L.or_weighted_sum(self.h, _ptr(a), _ptr(b), gam0, gam1, beta_dt, _ptr(out))
         return out
+
+    # ---- B_CT (face-centred constrained transport)
+    def init_state_faces(self, P, fB):
+        """Initial state with face fields fB [3][n3+1][n2+1][n1+1] (the B of P is ignored)."""
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        fB = np.ascontiguousarray(fB, dtype=np.float64)
+        assert P.shape == (NV,) + self.shape and fB.shape == self.fshape
+        self.L.or_init_state_faces(self.h, _ptr(P), _ptr(fB))
+
+    def faces(self):
+        return self.array(12)
+
+    def ct_calculate_emf(self):
+        self.L.or_ct_calculate_emf(self.h)
+
+    def ct_add_source(self):
+        self.L.or_ct_add_source(self.h)
+
+    def ct_block_utop(self):
+        self.L.or_ct_block_utop(self.h)
+
+    def ct_sync_faces(self):
+        self.L.or_ct_sync_faces(self.h)
+
+    def ct_max_divb(self):
+        return self.L.or_ct_max_divb(self.h)
+
+    def ct_weighted_sum_face(self, x, y, w1, w2):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        z = np.empty_like(x)
+        self.L.or_ct_weighted_sum_face(self.h, _ptr(x), _ptr(y), w1, w2, _ptr(z))
+        return z
 
     def sync_vars(self, A):
         """The periodic exchange of the block with itself on A [nvar, n3, n2, n1] -> new array."""
