@@ -137,11 +137,14 @@ def _cpu_model():
     return None
 
 
-def _cpu_baseline(cfg, P0, what, budget_s=12.0, max_steps=8, warm=1):
+def _cpu_baseline(cfg, P0, what, budget_s=12.0, max_steps=8, warm=1, fB=None):
     from oracle import oracle
     oracle.build()
     ob = oracle.OracleBlock(cfg)
-    ob.init_state(P0)
+    if fB is not None:
+        ob.init_state_faces(P0, fB)
+    else:
+        ob.init_state(P0)
     ob.step(warm)  # warm-up (first touch, dt ramp start)
     nz = cfg["nx"][0] * cfg["nx"][1] * cfg["nx"][2]
     t0 = time.perf_counter()
@@ -158,23 +161,31 @@ def _cpu_baseline(cfg, P0, what, budget_s=12.0, max_steps=8, warm=1):
 
 def _other_configs(with_cpu):
     """The CPU-runnable configurations beside the headline: C1 (2-D Orszag-Tang
-    256^2, FP64, 100 steps) and C2 at 64^3 (3-D fast mode, FP64 and split),
+    256^2, FP64, 100 steps), C2 at 64^3 (3-D fast mode, FP64 and split) and the
+    Orszag-Tang vortex with face-centred CT (gs05_c, HLLE, 256^2, both forms),
     zone-cycles/s on the GPU (HIP events around the steps, after 2 warm-up
     steps) and of the CPU restatement on a bounded sample (2 warm-up steps)."""
     import torch
     import kharma_amd as kh
     from kharma_amd import problems as pr
     out = {}
+    ot_ct = dict(pr.orszag_tang_config(n=256, init="reference", riemann="hlle"), b_solver="face_ct", ct_scheme="gs05_c")
     runs = [("c1_orszag_tang_256", pr.orszag_tang_config(n=256), pr.orszag_tang_prims, 100, ("fp64",)),
             ("c2_fast_mode_64", pr.mhdmode_config(n=64, ndim=3, mode="fast", one_period=False), pr.mhdmode_prims, 50,
-             ("fp64", "split"))]
+             ("fp64", "split")),
+            # the reference's face-CT Orszag-Tang setting (pars/tests/orszag_tang.par:37-43) at 256^2
+            ("orszag_tang_256_face_ct_gs05_c", ot_ct, pr.orszag_tang_prims, 100, ("fp64", "split"))]
     for name, cfg, gen, steps, forms in runs:
         P0 = gen(cfg)
+        fB0 = pr.orszag_tang_faces(cfg) if cfg.get("b_solver") == "face_ct" else None
         nz = cfg["nx"][0] * cfg["nx"][1] * cfg["nx"][2]
         rec = {"zones": nz, "steps": steps}
         for f in forms:
             d = kh.Driver(dict(cfg, precision=f))
-            d.init_prims(torch.from_numpy(P0).cuda())
+            if fB0 is not None:
+                d.init_faces(torch.from_numpy(P0).cuda(), torch.from_numpy(fB0).cuda())
+            else:
+                d.init_prims(torch.from_numpy(P0).cuda())
             d.step(2)
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             torch.cuda.synchronize()
@@ -185,7 +196,7 @@ def _other_configs(with_cpu):
             ms = e0.elapsed_time(e1)
             rec[f"gpu_{f}"] = {"zone_cycles_per_s": nz * steps / (ms * 1e-3), "ms_per_step": ms / steps}
         if with_cpu:
-            rec["cpu"] = _cpu_baseline(cfg, P0, name.replace("_", " "), budget_s=6.0, max_steps=steps, warm=2)
+            rec["cpu"] = _cpu_baseline(cfg, P0, name.replace("_", " "), budget_s=6.0, max_steps=steps, warm=2, fB=fB0)
             rec["gpu_fp64_over_cpu"] = rec["gpu_fp64"]["zone_cycles_per_s"] / rec["cpu"]["value"]
         out[name] = rec
     return out
