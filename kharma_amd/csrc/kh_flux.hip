@@ -147,15 +147,18 @@ flux_x1_kernel(BlockDims b, StepConsts c, GeomDev gd, Vars<const PT> P, Vars<R> 
 
 // X2 / X3 faces.  Transverse columns: flattened (outer, i) with i in [is-1, ie+1]
 // fastest; outer = k (X2) or j (X3) from o0.  Along the normal: faces s in
-// [s0, s1]; tile y covers cells s0-1+(TY-1)y .. s0-2+TY+(TY-1)y with TY = 8
-// cells along the normal (a 6-deep FP64 tile, which lets two 384-lane blocks
-// share a CU at the FP64 variant's ~140 VGPRs, measured 20 % slower).
-template <typename R> struct XtTile { static constexpr int TY = 8; };
+// [s0, s1]; tile y covers cells s0-1+(TY-1)y .. s0-2+TY+(TY-1)y.  TY matches the
+// block to the waves a CU holds at the variant's registers: the curved-space
+// FP64 variant (~142 VGPRs, 3 waves per SIMD) runs 64 x 12 blocks, one per CU at
+// 12 waves (5 % faster than 64 x 8, whose one block per CU holds 8); the others
+// (FP32 ~92, flat FP64 ~104 VGPRs) run 64 x 8, two blocks per CU (64 x 10 and
+// 64 x 16 FP32 blocks measured 24 % and 2 % slower).
+template <typename R, int GEO> struct XtTile { static constexpr int TY = (sizeof(R) == 8 && GEO) ? 12 : 8; };
 template <int DIR, int RECON, int SPEEDS, typename R, typename PT, int GEO>
-__global__ void __launch_bounds__(64 * XtTile<R>::TY)
+__global__ void __launch_bounds__((64 * XtTile<R, GEO>::TY))
 flux_xt_kernel(BlockDims b, StepConsts c, GeomDev gd, Vars<const PT> P, Vars<R> F,
                R* __restrict__ cmax_out, R* __restrict__ cmin_out, int o0, int no, int s0, int s1) {
-    constexpr int TY = XtTile<R>::TY;
+    constexpr int TY = XtTile<R, GEO>::TY;
     __shared__ R sr[NV][TY][64];
     const int ni = b.nx1 + 2;
     const int tx = threadIdx.x, ty = threadIdx.y;
@@ -205,14 +208,14 @@ static hipError_t launch_recon(const BlockDims& b, const StepConsts& c, const Ge
         const int no = khi - klo + 1;
         const int ncol = (b.nx1 + 2) * no;
         const int nfaces = b.nx2 + 1;
-        constexpr int TY = XtTile<R>::TY;
+        constexpr int TY = XtTile<R, GEO>::TY;
         const dim3 grid((unsigned)((ncol + 63) / 64), (unsigned)((nfaces + TY - 2) / (TY - 1)));
         hipLaunchKernelGGL((flux_xt_kernel<2, RECON, SPEEDS, R, PT, GEO>), grid, dim3(64, TY), 0, s, b, c, gd, P, F, cmax, cmin, klo, no, b.js, b.je + 1);
     } else {
         const int no = jhi - jlo + 1;
         const int ncol = (b.nx1 + 2) * no;
         const int nfaces = b.nx3 + 1;
-        constexpr int TY = XtTile<R>::TY;
+        constexpr int TY = XtTile<R, GEO>::TY;
         const dim3 grid((unsigned)((ncol + 63) / 64), (unsigned)((nfaces + TY - 2) / (TY - 1)));
         hipLaunchKernelGGL((flux_xt_kernel<3, RECON, SPEEDS, R, PT, GEO>), grid, dim3(64, TY), 0, s, b, c, gd, P, F, cmax, cmin, jlo, no, b.ks, b.ke + 1);
     }
