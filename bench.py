@@ -36,12 +36,14 @@ METRIC = "zone-cycles/s on 1 MI355X: FP32-split vs FP64 plain; adjoint/forward t
 HBM_PEAK_GBS = 8000.0
 
 
-def _bench_mesh(cfg, P0, nb, steps, warmup):
+def _bench_mesh(cfg, P0, nb, steps, warmup, graph=False):
     """zone-cycles timing of the same workload split into nb blocks (kh_mesh_*):
-    HIP events on the stream around `steps` steps after `warmup`."""
+    HIP events on the stream around `steps` steps after `warmup`; graph replays
+    each step as a captured HIP graph (kh_mesh_set_graph)."""
     import torch
     import kharma_amd as kh
     m = kh.Mesh(cfg, nb)
+    m.set_graph(graph)
     m.init_prims(torch.from_numpy(P0).cuda())
     m.step(warmup)
     torch.cuda.synchronize()
@@ -278,11 +280,12 @@ def main():
     layout = None
     if a.workload == "torus":
         nb = (2, 2, 4)
-        ms = {f: _bench_mesh(dict(cfg, precision=f), P0, nb, a.steps, a.warmup) for f in ("fp64", "split")}
-        layout = {"blocks": "2x2x4 blocks of 96x64x32 (ghost zones per block 4, each block's own)",
-                  "fp64": {"zone_cycles_per_s": nz * a.steps / (ms["fp64"] * 1e-3), "ms_per_step": ms["fp64"] / a.steps},
-                  "split": {"zone_cycles_per_s": nz * a.steps / (ms["split"] * 1e-3),
-                            "ms_per_step": ms["split"] / a.steps}}
+        layout = {"blocks": "2x2x4 blocks of 96x64x32 (ghost zones per block 4, each block's own)"}
+        for f in ("fp64", "split"):
+            for graph in (False, True):
+                ms = _bench_mesh(dict(cfg, precision=f), P0, nb, a.steps, a.warmup, graph)
+                layout[f + ("_graph" if graph else "")] = {"zone_cycles_per_s": nz * a.steps / (ms * 1e-3),
+                                                           "ms_per_step": ms / a.steps}
     adj = None if a.no_adjoint else _bench_adjoint()
     adj_gr = None if a.no_adjoint else _bench_adjoint_torus()
     zc64 = nz * a.steps / (r64["ms"] * 1e-3)

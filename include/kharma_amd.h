@@ -467,6 +467,14 @@ int kh_mesh_shape(const kh_mesh* m, int* nblocks, int* n1, int* n2, int* n3, lon
  * read, with the ghosts of physical faces refilled): P->U, boundaries, dt_start. */
 int kh_mesh_init_prims(kh_mesh* m, const double* P);
 int kh_mesh_step(kh_mesh* m, int nsteps);
+/* Replay the mesh's step as a HIP graph (no counterpart in the reference, whose
+ * task lists are rebuilt each cycle): with enable != 0 the next kh_mesh_step
+ * captures one step (its kernel launches over all blocks) on a private stream
+ * and every step after that launches the captured graph on the mesh's stream,
+ * which removes the per-launch host cost of a mesh of small blocks.  The graph
+ * holds the mesh's own buffers, so it stays valid for the mesh's life; enable
+ * = 0 drops it.  Results are those of the eager step, bit for bit. */
+int kh_mesh_set_graph(kh_mesh* m, int enable);
 int kh_mesh_set_clock(kh_mesh* m, double time, double dt, int ncycle);
 /* The base-container state in the mesh layout (device; either may be NULL):
  * interior zones from their blocks, periodic ghosts as images, the ghosts of

@@ -193,6 +193,7 @@ def lib():
                               ctypes.POINTER(l)]),
         "kh_mesh_init_prims": (i, [vp, vp]),
         "kh_mesh_step": (i, [vp, i]),
+        "kh_mesh_set_graph": (i, [vp, i]),
         "kh_mesh_set_clock": (i, [vp, d, d, i]),
         "kh_mesh_get_state": (i, [vp, vp, vp]),
         "kh_mesh_get_scalars": (i, [vp, ctypes.POINTER(d), ctypes.POINTER(d), ctypes.POINTER(d),
@@ -794,6 +795,10 @@ class Mesh:
     def step(self, n=1):
         self._bind()
         _check(lib().kh_mesh_step(self.h, n))
+
+    def set_graph(self, enable=True):
+        """kh_mesh_set_graph: replay each step as a captured HIP graph."""
+        _check(lib().kh_mesh_set_graph(self.h, int(bool(enable))))
 
     def set_clock(self, time, dt, ncycle=0):
         self._bind()

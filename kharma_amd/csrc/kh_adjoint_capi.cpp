@@ -86,11 +86,8 @@ int recompute_stage(kh_adjoint* a, int stage) {
         KA_HIP(launch_save_x1_flux(bd, F, a->x1f[stage - 1], s));
     }
     if (a->geo) {
-        KA_HIP(launch_update_parts(bd, c, b->geo, stage, 1, F, Usub, d->U0, Psub, Uout, Pout, pflag, fflag,
-                                   d->fail_list, d->ds, s));
-        KA_HIP(hipMemcpyAsync(a->Ustar[stage - 1], Uout, sizeof(double) * NV * bd.nc, hipMemcpyDeviceToDevice, s));
-        KA_HIP(launch_update_parts(bd, c, b->geo, stage, 2, F, Usub, d->U0, Psub, Uout, Pout, pflag, fflag,
-                                   d->fail_list, d->ds, s));
+        KA_HIP(launch_update_parts(bd, c, b->geo, stage, 3, F, Usub, d->U0, Psub, Uout, Pout, pflag, fflag,
+                                   d->fail_list, d->ds, s, a->Ustar[stage - 1]));
     } else {
         KA_HIP(launch_update(bd, c, b->geo, b->p.precision, stage, false, F, ct, Usub, d->U0, Psub, Uout, Pout,
                              nullptr, pflag, fflag, d->fail_list, d->ds, d->dt_part, s));

@@ -208,14 +208,17 @@ hipError_t launch_update(const BlockDims& b, const StepConsts& c, const GeomDev&
                          double* Uout, double* Pout, float* Pout32, int* pflag, int* fflag,
                          int* fail_list, DeviceState* ds, double* dt_part, hipStream_t s);
 // The two halves of launch_update on the curved-space FP64 path: parts bit 1
-// runs the combine (U* of the stage into Uout), bit 2 the recovery (U* -> P,
-// floors, P -> U).  The adjoint's recompute reads U* in between.
+// runs the combine (U* = g0 U_sub + g1 U_full + beta dt (div F + S) into Uout),
+// bit 2 the recovery (U* -> P, floors, P -> U).  With Ustar non-null the
+// recovery also stores the interior zones' U* there as it reads them (the
+// adjoint's recompute uses it).
 hipError_t launch_update_parts(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int stage, int parts,
                                const double* F, const double* Usub, const double* Ufull, const double* Psub, double* Uout,
-                               double* Pout, int* pflag, int* fflag, int* fail_list, DeviceState* ds, hipStream_t s);
-// launch_update on the FP64 path with the stage's U* also copied to Ustar
-// (curved space: the combine, the copy, the recovery; flat space has no
-// separate U* and runs the fused update).
+                               double* Pout, int* pflag, int* fflag, int* fail_list, DeviceState* ds, hipStream_t s,
+                               double* Ustar = nullptr);
+// launch_update on the FP64 path with the stage's U* also stored to Ustar
+// (curved space: the recovery stores the interior zones' U* as it reads them;
+// flat space has no separate U* and runs the fused update).
 hipError_t launch_update_recorded(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int stage, bool do_dt,
                                   const void* F, const void* ctop, const double* Usub, const double* Ufull,
                                   const double* Psub, double* Uout, double* Pout, int* pflag, int* fflag,

@@ -38,6 +38,12 @@ struct kh_mesh {
     double *dt_part = nullptr, *divb_part = nullptr, *dt_hist = nullptr;
     long n_dt = 0, n_divb = 0;              // partials per block
     int hist_cap = 1 << 16;
+    // kh_mesh_set_graph: one step captured once (on a private stream) as a HIP
+    // graph and replayed on the mesh's stream
+    bool use_graph = false;
+    hipStream_t cap = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
 };
 
 namespace {
@@ -48,12 +54,11 @@ namespace {
         if (e_ != hipSuccess) return api_fail(KH_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-int mesh_stage(kh_mesh* m, int stage) {
+int mesh_stage(kh_mesh* m, int stage, hipStream_t s) {
     const MeshDims& md = m->md;
     const long nc = md.nc;
     const int prec = m->p.precision;
     const size_t fe = (prec == KH_PREC_SPLIT) ? sizeof(float) : sizeof(double);
-    hipStream_t s = m->stream;
     double* Psub = (stage == 1) ? m->P0 : m->P1;
     float* Psubf = (stage == 1) ? m->P0f : m->P1f;
     double* Usub = (stage == 1) ? m->U0 : m->U1;
@@ -95,6 +100,31 @@ int mesh_stage(kh_mesh* m, int stage) {
             KM_HIP(launch_bounds(k->dims, k->consts, k->geo, Uout + st * b, Pout + st * b, f32(Poutf, b), s));
     }
     return KH_OK;
+}
+
+// One full step on stream s: both stages over all blocks, then divB and the
+// finalize over the partials of all blocks.
+int mesh_step_body(kh_mesh* m, hipStream_t s) {
+    const MeshDims& md = m->md;
+    const long st = NV * md.nc;
+    const StepConsts& c = m->blk[0]->consts;
+    for (int stage = 1; stage <= 2; ++stage) {
+        const int r = mesh_stage(m, stage, s);
+        if (r != KH_OK) return r;
+    }
+    if (md.ndim > 1)
+        for (int b = 0; b < md.nblk; ++b)
+            KM_HIP(launch_max_divb(m->blk[b]->dims, m->U0 + st * b, m->divb_part + m->n_divb * b, s));
+    KM_HIP(launch_finalize(c, m->ds, m->dt_part, (int)(m->n_dt * md.nblk), m->divb_part,
+                           md.ndim > 1 ? (int)(m->n_divb * md.nblk) : 0, m->dt_hist, m->hist_cap, s, md.nblk));
+    return KH_OK;
+}
+
+void drop_graph(kh_mesh* m) {
+    if (m->gexec) (void)hipGraphExecDestroy(m->gexec);
+    if (m->graph) (void)hipGraphDestroy(m->graph);
+    m->gexec = nullptr;
+    m->graph = nullptr;
 }
 
 }  // namespace
@@ -197,6 +227,8 @@ void kh_mesh_destroy(kh_mesh* m) {
                     (void*)m->dt_part, (void*)m->divb_part, (void*)m->dt_hist})
         if (q) (void)hipFree(q);
     for (kh_block* b : m->blk) kh_block_destroy(b);
+    drop_graph(m);
+    if (m->cap) (void)hipStreamDestroy(m->cap);
     delete m;
 }
 
@@ -244,21 +276,34 @@ int kh_mesh_init_prims(kh_mesh* m, const double* P) {
 
 int kh_mesh_step(kh_mesh* m, int nsteps) {
     if (!m) return api_fail(KH_ERR_INVALID, "null mesh");
-    const MeshDims& md = m->md;
-    const long st = NV * md.nc;
-    hipStream_t s = m->stream;
-    const StepConsts& c = m->blk[0]->consts;
-    for (int n = 0; n < nsteps; ++n) {
-        for (int stage = 1; stage <= 2; ++stage) {
-            const int r = mesh_stage(m, stage);
+    if (!m->use_graph) {
+        for (int n = 0; n < nsteps; ++n) {
+            const int r = mesh_step_body(m, m->stream);
             if (r != KH_OK) return r;
         }
-        if (md.ndim > 1)
-            for (int b = 0; b < md.nblk; ++b)
-                KM_HIP(launch_max_divb(m->blk[b]->dims, m->U0 + st * b, m->divb_part + m->n_divb * b, s));
-        KM_HIP(launch_finalize(c, m->ds, m->dt_part, (int)(m->n_dt * md.nblk), m->divb_part,
-                               md.ndim > 1 ? (int)(m->n_divb * md.nblk) : 0, m->dt_hist, m->hist_cap, s, md.nblk));
+        return KH_OK;
     }
+    if (!m->gexec) {
+        if (!m->cap) KM_HIP(hipStreamCreate(&m->cap));
+        KM_HIP(hipStreamBeginCapture(m->cap, hipStreamCaptureModeThreadLocal));
+        const int r = mesh_step_body(m, m->cap);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(m->cap, &g);
+        if (r != KH_OK || e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return r != KH_OK ? r : api_fail(KH_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+        }
+        m->graph = g;
+        KM_HIP(hipGraphInstantiate(&m->gexec, g, nullptr, nullptr, 0));
+    }
+    for (int n = 0; n < nsteps; ++n) KM_HIP(hipGraphLaunch(m->gexec, m->stream));
+    return KH_OK;
+}
+
+int kh_mesh_set_graph(kh_mesh* m, int enable) {
+    if (!m) return api_fail(KH_ERR_INVALID, "null mesh");
+    m->use_graph = enable != 0;
+    if (!m->use_graph) drop_graph(m);
     return KH_OK;
 }
 

@@ -130,9 +130,10 @@ __device__ __forceinline__ bool in_physical(const BlockDims& b, int k, int j, in
 // sqrt(-g) Gamma^a_{b c} at index 16 a + 4 b + c.  All 64 terms are summed,
 // in the order mu, nu, lam, with no test for zero components: a branch per
 // component would make every load wait for the one before it.
-template <class G>
-__device__ __forceinline__ void add_geo_source(const G& g, const GeomDev& gd, long q, const double Psub[NV], double gam,
-                                               double dudt[NV]) {
+// conn(n) returns component n = 16 a + 4 b + c of this zone's table.
+template <class G, class C>
+__device__ __forceinline__ void add_geo_source_c(const G& g, const C& conn, const double Psub[NV], double gam,
+                                                 double dudt[NV]) {
     FourVec<double> D;
     calc_4vecs<double>(g, Psub, D);
     const double bsq = dot4(D.bcon, D.bcov);
@@ -147,7 +148,7 @@ __device__ __forceinline__ void add_geo_source(const G& g, const GeomDev& gd, lo
         for (int nu = 0; nu < 4; ++nu) T[nu] = eta * D.ucon[mu] * D.ucov[nu] + (mu == nu ? ptot : 0.) - D.bcon[mu] * D.bcov[nu];
         double cm[16];   // this mu's 16 components, loaded together
 #pragma unroll
-        for (int nl = 0; nl < 16; ++nl) cm[nl] = gd.conn[(long)(nl * 4 + mu) * gd.np + q];
+        for (int nl = 0; nl < 16; ++nl) cm[nl] = conn(nl * 4 + mu);
 #pragma unroll
         for (int nu = 0; nu < 4; ++nu)
 #pragma unroll
@@ -157,6 +158,11 @@ __device__ __forceinline__ void add_geo_source(const G& g, const GeomDev& gd, lo
     dudt[U1] += nd[1];
     dudt[U2] += nd[2];
     dudt[U3] += nd[3];
+}
+template <class G>
+__device__ __forceinline__ void add_geo_source(const G& g, const GeomDev& gd, long q, const double Psub[NV], double gam,
+                                               double dudt[NV]) {
+    add_geo_source_c(g, [&](int n) { return gd.conn[(long)n * gd.np + q]; }, Psub, gam, dudt);
 }
 
 // ----------------------------------------------------------------- flux-CT
@@ -519,9 +525,12 @@ __device__ __forceinline__ void interior_index_jki(const BlockDims& b, unsigned 
 // the geometry of their 64 (j, i) columns (the 22 metric components and the 2
 // geometric floors), which the work-group stages once in LDS as [component][64]
 // instead of each zone loading its own.  Tiles are numbered i fastest, then k,
-// then j, and renumbered per XCD as above.  (The same staging in the combine
-// kernel, with the connection, measured slower: its loads of fluxes and states
-// dominate, and the barrier keeps its two waves per SIMD from overlapping them.)
+// then j, and renumbered per XCD as above.  Each zone's U and P loads are
+// issued before the staging so that they overlap it (272 -> 262 us on the
+// 192x128x128 torus).  (Staging the connection as well and adding the geometric
+// source here instead of in the combine shortens the combine -- 75-102 VGPRs
+// instead of 190-254 -- but slows the recovery by more: measured 223 + 469
+// against 371 + 262 us for the split stage-2 update.)
 constexpr int TILE_I = 64, TILE_K = BS / TILE_I;
 constexpr int STG_RECOVER = GEO_NCOMP + 2;
 __host__ __device__ __forceinline__ int tiles_i(const BlockDims& b) { return (b.nx1 + TILE_I - 1) / TILE_I; }
@@ -605,33 +614,23 @@ combine_kernel(BlockDims b, StepConsts cst, GeomDev gd, int stage, const R* __re
 
 // U->P with the stage's starting primitives as guess (kharma_driver.cpp:379-384),
 // B from U, floors in frame FRAME, P->U; U is read from and written back to Uout.
+// One zone (k, j, i) at c, whose U and guess P the caller has loaded.
 template <int FRAME, int INV, bool STG>
-__global__ void __launch_bounds__(BS, KH_REC_OCC)
-recover_kernel(BlockDims b, StepConsts cst, GeomDev gd, int stage, const double* __restrict__ Psub, double* Uout,
-               double* __restrict__ Pout, float* __restrict__ Pout32, int* __restrict__ pflag,
-               int* __restrict__ fflag, int* __restrict__ fail_list, DeviceState* __restrict__ ds) {
-    __shared__ double sg[STG ? STG_RECOVER * TILE_I : 1];
-    int k, j, i;
-    if constexpr (STG) {
-        int i0;
-        const bool active = tile_zone(b, k, j, i, i0);
-        stage_recover_geometry(gd, j, i0, sg);
-        if (!active) return;
-    } else {
-        const unsigned t = xcd_logical_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-        if (t >= (unsigned)ninterior(b)) return;
-        interior_index_jki(b, t, k, j, i);
-    }
-    const long c = (long)k * b.s3 + (long)j * b.s2 + i;
+__device__ __forceinline__ void recover_zone(const BlockDims& b, const StepConsts& cst, const GeomDev& gd, int stage,
+                                             double* Uout, double* __restrict__ Pout,
+                                             float* __restrict__ Pout32, int* __restrict__ pflag,
+                                             int* __restrict__ fflag, int* __restrict__ fail_list,
+                                             DeviceState* __restrict__ ds, double* __restrict__ Ustar,
+                                             const double* sg, int k, int j, int i, long c, double (&U)[NV],
+                                             double (&P)[NV]) {
     const double gam = cst.gam;
-    double U[NV], P[NV];
-#pragma unroll
-    for (int v = 0; v < NV; ++v) U[v] = Uout[v * b.nc + c];
-#pragma unroll
-    for (int v = 0; v < 5; ++v) P[v] = Psub[v * b.nc + c];
     int pf = 0, ff = 0;
     double rf, uf;
     const int lane = (int)(threadIdx.x % TILE_I);
+    if (Ustar) {   // the stage's U* for the differentiation form's record
+#pragma unroll
+        for (int v = 0; v < NV; ++v) Ustar[v * b.nc + c] = U[v];
+    }
     if constexpr (STG) {
         rf = sg[GEO_NCOMP * TILE_I + lane];
         uf = sg[(GEO_NCOMP + 1) * TILE_I + lane];
@@ -645,7 +644,7 @@ recover_kernel(BlockDims b, StepConsts cst, GeomDev gd, int stage, const double*
         pf = pf % F_GEOM_RHO;
         const double gd_ = g.gdet();
         const double igd = rcp(gd_);
-            P[B1] = U[B1] * igd; P[B2] = U[B2] * igd; P[B3] = U[B3] * igd;
+        P[B1] = U[B1] * igd; P[B2] = U[B2] * igd; P[B3] = U[B3] * igd;
         FourVec<double> D;
         calc_4vecs<double>(g, P, D);
         ff = ff0;
@@ -666,6 +665,45 @@ recover_kernel(BlockDims b, StepConsts cst, GeomDev gd, int stage, const double*
         const int slot = atomicAdd(&ds->fail_count[stage - 1], 1);
         fail_list[slot] = (int)c;
     }
+}
+
+template <int FRAME, int INV, bool STG>
+__global__ void __launch_bounds__(BS, KH_REC_OCC)
+recover_kernel(BlockDims b, StepConsts cst, GeomDev gd, int stage, const double* __restrict__ Psub, double* Uout,
+               double* __restrict__ Pout, float* __restrict__ Pout32, int* __restrict__ pflag,
+               int* __restrict__ fflag, int* __restrict__ fail_list, DeviceState* __restrict__ ds,
+               double* __restrict__ Ustar) {
+    __shared__ double sg[STG ? STG_RECOVER * TILE_I : 1];
+    constexpr int NP = 5;   // the guess: the fluid primitives (B follows from U)
+    int k, j, i;
+    long c;
+    double U[NV], P[NV];
+    if constexpr (STG) {
+        // the zone's U and P are requested before the tile's geometry is staged,
+        // so that their latency overlaps the staging and its barrier
+        int i0;
+        const bool active = tile_zone(b, k, j, i, i0);
+        c = (long)k * b.s3 + (long)j * b.s2 + i;
+        if (active) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) U[v] = Uout[v * b.nc + c];
+#pragma unroll
+            for (int v = 0; v < NP; ++v) P[v] = Psub[v * b.nc + c];
+        }
+        stage_recover_geometry(gd, j, i0, sg);
+        if (!active) return;
+    } else {
+        const unsigned t = xcd_logical_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+        if (t >= (unsigned)ninterior(b)) return;
+        interior_index_jki(b, t, k, j, i);
+        c = (long)k * b.s3 + (long)j * b.s2 + i;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) U[v] = Uout[v * b.nc + c];
+#pragma unroll
+        for (int v = 0; v < NP; ++v) P[v] = Psub[v * b.nc + c];
+    }
+    recover_zone<FRAME, INV, STG>(b, cst, gd, stage, Uout, Pout, Pout32, pflag, fflag, fail_list, ds, Ustar, sg, k, j,
+                                  i, c, U, P);
 }
 
 // Inverter::FixUtoP (inverter/fixup.cpp:47-156) on the zones listed by the
@@ -1401,7 +1439,8 @@ template <typename R>
 static void launch_update_geo(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int stage, bool do_dt,
                               const void* F, const void* ctop, const double* Usub, const double* Ufull,
                               const double* Psub, double* Uout, double* Pout, float* Pout32, int* pflag, int* fflag,
-                              int* fail_list, DeviceState* ds, double* dt_part, hipStream_t s, int parts = 3) {
+                              int* fail_list, DeviceState* ds, double* dt_part, hipStream_t s, int parts = 3,
+                              double* Ustar = nullptr) {
     const dim3 g = grid_for(ninterior_host(b));
     const bool rstg = b.ndim == 3;   // recover on LDS-staged tiles
     const dim3 gr = rstg ? dim3((unsigned)update_tiles_host(b)) : g;
@@ -1421,9 +1460,9 @@ static void launch_update_geo(const BlockDims& b, const StepConsts& c, const Geo
 #define KH_REC(FR, INV)                                                                                             \
     do {                                                                                                            \
         if (rstg) hipLaunchKernelGGL((recover_kernel<FR, INV, true>), gr, dim3(BS), 0, s, b, c, gd, stage, Psub,        \
-                                     Uout, Pout, Pout32, pflag, fflag, fail_list, ds);                                  \
+                                     Uout, Pout, Pout32, pflag, fflag, fail_list, ds, Ustar);                           \
         else hipLaunchKernelGGL((recover_kernel<FR, INV, false>), gr, dim3(BS), 0, s, b, c, gd, stage, Psub, Uout, Pout, \
-                                Pout32, pflag, fflag, fail_list, ds);                                                   \
+                                Pout32, pflag, fflag, fail_list, ds, Ustar);                                            \
     } while (0)
     if (c.inverter == 1) {
         if (c.floors.frame == 0) KH_REC(0, 1);
@@ -1484,10 +1523,11 @@ hipError_t launch_update(const BlockDims& b, const StepConsts& c, const GeomDev&
 }
 hipError_t launch_update_parts(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int stage, int parts,
                                const double* F, const double* Usub, const double* Ufull, const double* Psub, double* Uout,
-                               double* Pout, int* pflag, int* fflag, int* fail_list, DeviceState* ds, hipStream_t s) {
+                               double* Pout, int* pflag, int* fflag, int* fail_list, DeviceState* ds, hipStream_t s,
+                               double* Ustar) {
     if (!b.geo) return hipErrorInvalidValue;
     launch_update_geo<double>(b, c, gd, stage, false, F, nullptr, Usub, Ufull, Psub, Uout, Pout, nullptr, pflag, fflag,
-                              fail_list, ds, nullptr, s, parts);
+                              fail_list, ds, nullptr, s, parts, Ustar);
     return hipGetLastError();
 }
 hipError_t launch_update_recorded(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int stage, bool do_dt,
@@ -1498,11 +1538,7 @@ hipError_t launch_update_recorded(const BlockDims& b, const StepConsts& c, const
         return launch_update(b, c, gd, PREC_FP64, stage, do_dt, F, ctop, Usub, Ufull, Psub, Uout, Pout, nullptr, pflag,
                              fflag, fail_list, ds, dt_part, s);
     launch_update_geo<double>(b, c, gd, stage, do_dt, F, ctop, Usub, Ufull, Psub, Uout, Pout, nullptr, pflag, fflag,
-                              fail_list, ds, dt_part, s, 1);
-    hipError_t e = hipMemcpyAsync(Ustar, Uout, sizeof(double) * NV * b.nc, hipMemcpyDeviceToDevice, s);
-    if (e != hipSuccess) return e;
-    launch_update_geo<double>(b, c, gd, stage, do_dt, F, ctop, Usub, Ufull, Psub, Uout, Pout, nullptr, pflag, fflag,
-                              fail_list, ds, dt_part, s, 2);
+                              fail_list, ds, dt_part, s, 3, Ustar);
     return hipGetLastError();
 }
 long update_blocks(const BlockDims& b) { return (ninterior_host(b) + BS - 1) / BS; }

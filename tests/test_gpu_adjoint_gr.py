@@ -362,8 +362,11 @@ def test_floored_step_velocity_and_field_directions():
     evaluation noise that grows as 1/h (the floored rows sit on ties: uniform in
     theta, on the reconstruction floor's kink, at zero WENO5 smoothness
     indicators; the drift frame is ill-conditioned where B crosses zero), so the
-    check uses h = 1e-5, where that noise and the truncation error are both near
-    1e-6; smaller steps are printed for the record."""
+    check uses h = 1e-5; smaller steps are printed for the record.  The noise
+    measured at h = 1e-6 and 1e-7 (up to 6e-5 and 2e-4 in a field direction)
+    extrapolates as 1/h to about 6e-6 at h = 1e-5, and two builds whose forward
+    steps differ only in rounding gave 2.6e-6 and 5.2e-6 there for the same
+    direction, so the bound is 1.5e-5 (a wrong transpose term shows as O(1))."""
     from kharma_amd.adjoint import Evolution
     cfg = dict(CFG(), err_tol=1e-14, iter_max=30)
     rng = np.random.default_rng(9)
@@ -397,7 +400,7 @@ def test_floored_step_velocity_and_field_directions():
                 errs[h] = _rel(fd, ad)
             print(f"[gr floored step] {name} dir {q}: adjoint {ad:.12e} rel err " +
                   " ".join(f"h {h:.0e}: {e:.2e}" for h, e in errs.items()), flush=True)
-            assert errs[1e-5] < 5e-6
+            assert errs[1e-5] < 1.5e-5
 
 
 def test_recorded_steps_match_recompute(smooth_state):

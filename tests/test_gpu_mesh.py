@@ -127,6 +127,33 @@ def test_mesh_torus_step_against_oracle(oracle_lib):
     assert np.all(r <= np.maximum(1e-12, 10 * sens)), (r, sens)
 
 
+@pytest.mark.parametrize("case", ["torus3d_2x2x2", "fast3d_split_2x1x2"])
+def test_mesh_graph_replay_equals_eager(case):
+    """kh_mesh_set_graph: every replay of the captured step equals the eager
+    step bit for bit, with the program's own eager work (state read-outs,
+    synchronisation) between the replays."""
+    import kharma_amd as kh
+    mk, gen, nb, _ = CASES[case]
+    cfg = mk()
+    P0 = torch.from_numpy(gen(cfg)).cuda()
+    a = kh.Mesh(cfg, nb)
+    a.init_prims(P0)
+    g = kh.Mesh(cfg, nb)
+    g.init_prims(P0)
+    g.set_graph(True)
+    for n in range(4):
+        a.step(1)
+        g.step(1)
+        Pa, Ua = a.state()
+        Pg, Ug = g.state()
+        torch.cuda.synchronize()
+        assert torch.equal(Pa, Pg) and torch.equal(Ua, Ug), (case, n)
+    g.step(3)
+    a.step(3)
+    assert torch.equal(a.state()[0], g.state()[0])
+    assert a.dt_history(7) == g.dt_history(7) and a.scalars() == g.scalars()
+
+
 def test_mesh_rejects_bad_layouts():
     import kharma_amd as kh
     cfg = pr.mhdmode_config(n=32, ndim=3, mode="fast")
