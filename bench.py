@@ -36,14 +36,14 @@ METRIC = "zone-cycles/s on 1 MI355X: FP32-split vs FP64 plain; adjoint/forward t
 HBM_PEAK_GBS = 8000.0
 
 
-def _bench_mesh(cfg, P0, nb, steps, warmup, graph=False):
+def _bench_mesh(cfg, P0, nb, steps, warmup, packed=True):
     """zone-cycles timing of the same workload split into nb blocks (kh_mesh_*):
-    HIP events on the stream around `steps` steps after `warmup`; graph replays
-    each step as a captured HIP graph (kh_mesh_set_graph)."""
+    HIP events on the stream around `steps` steps after `warmup`; packed: each
+    kernel launched once over all blocks (kh_mesh_set_packed), else once per block."""
     import torch
     import kharma_amd as kh
     m = kh.Mesh(cfg, nb)
-    m.set_graph(graph)
+    m.set_packed(packed)
     m.init_prims(torch.from_numpy(P0).cuda())
     m.step(warmup)
     torch.cuda.synchronize()
@@ -280,12 +280,13 @@ def main():
     layout = None
     if a.workload == "torus":
         nb = (2, 2, 4)
-        layout = {"blocks": "2x2x4 blocks of 96x64x32 (ghost zones per block 4, each block's own)"}
+        layout = {"blocks": "2x2x4 blocks of 96x64x32 (ghost zones per block 4, each block's own)",
+                  "launches": "fp64/split: each kernel once over all 16 blocks; *_per_block: once per block"}
         for f in ("fp64", "split"):
-            for graph in (False, True):
-                ms = _bench_mesh(dict(cfg, precision=f), P0, nb, a.steps, a.warmup, graph)
-                layout[f + ("_graph" if graph else "")] = {"zone_cycles_per_s": nz * a.steps / (ms * 1e-3),
-                                                           "ms_per_step": ms / a.steps}
+            for packed in (True, False):
+                ms = _bench_mesh(dict(cfg, precision=f), P0, nb, a.steps, a.warmup, packed)
+                layout[f + ("" if packed else "_per_block")] = {"zone_cycles_per_s": nz * a.steps / (ms * 1e-3),
+                                                                "ms_per_step": ms / a.steps}
     adj = None if a.no_adjoint else _bench_adjoint()
     adj_gr = None if a.no_adjoint else _bench_adjoint_torus()
     zc64 = nz * a.steps / (r64["ms"] * 1e-3)

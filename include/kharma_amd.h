@@ -475,6 +475,13 @@ int kh_mesh_step(kh_mesh* m, int nsteps);
  * holds the mesh's own buffers, so it stays valid for the mesh's life; enable
  * = 0 drops it.  Results are those of the eager step, bit for bit. */
 int kh_mesh_set_graph(kh_mesh* m, int enable);
+/* How the mesh step launches its kernels.  enable != 0 (the default): each
+ * kernel once over all blocks, as KHARMA launches over a MeshBlockPack
+ * (driver/kharma_step.cpp:77-319 with a pack of every block; the pack loops of
+ * kharma_driver.cpp:339-387 and flux/get_flux.hpp:60-373 run one kernel over
+ * (b, k, j, i)); enable = 0: once per block.  Results are the same bit for bit;
+ * a change drops a captured graph. */
+int kh_mesh_set_packed(kh_mesh* m, int enable);
 int kh_mesh_set_clock(kh_mesh* m, double time, double dt, int ncycle);
 /* The base-container state in the mesh layout (device; either may be NULL):
  * interior zones from their blocks, periodic ghosts as images, the ghosts of

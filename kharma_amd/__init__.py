@@ -194,6 +194,7 @@ def lib():
         "kh_mesh_init_prims": (i, [vp, vp]),
         "kh_mesh_step": (i, [vp, i]),
         "kh_mesh_set_graph": (i, [vp, i]),
+        "kh_mesh_set_packed": (i, [vp, i]),
         "kh_mesh_set_clock": (i, [vp, d, d, i]),
         "kh_mesh_get_state": (i, [vp, vp, vp]),
         "kh_mesh_get_scalars": (i, [vp, ctypes.POINTER(d), ctypes.POINTER(d), ctypes.POINTER(d),
@@ -799,6 +800,10 @@ class Mesh:
     def set_graph(self, enable=True):
         """kh_mesh_set_graph: replay each step as a captured HIP graph."""
         _check(lib().kh_mesh_set_graph(self.h, int(bool(enable))))
+
+    def set_packed(self, enable=True):
+        """kh_mesh_set_packed: each kernel once over all blocks (default) or once per block."""
+        _check(lib().kh_mesh_set_packed(self.h, int(bool(enable))))
 
     def set_clock(self, time, dt, ncycle=0):
         self._bind()

@@ -111,11 +111,30 @@ __device__ __forceinline__ void face_flux(const StepConsts& c, const BlockDims& 
 // X1 faces i in [is, ie+1] of rows j in [j0, j0+nj), k in [k0, k0+nk).
 // Cells i in [is-1, ie+1] (L per row); block b handles flattened cells
 // [255 b, 255 b + 256) and emits the faces of all but its first cell.
-template <int RECON, int SPEEDS, typename R, typename PT, int GEO>
-__global__ void __launch_bounds__(256)
+// Waves per SIMD the FP32 flux kernels are built for (the second launch bound):
+// 6 for the X2/X3 kernels (68 VGPRs instead of 91: 11-14 % faster on the
+// 192x128x128 torus) and for the X1 kernel that also writes the signal speeds
+// (68 instead of 115: 4 % faster); the one-block X1 kernel without speeds
+// allocates 72 by itself, and a bound of 6 on it measured 3 % slower (its pack
+// form would take 115 unbounded).  The FP64 kernels keep their 140-142 VGPRs:
+// a 4-wave bound spills 28-36 bytes per lane and is slower.
+template <typename R, int SPEEDS, bool PK> struct X1Occ {
+    static constexpr int w = (sizeof(R) == 4 && (SPEEDS != SPEEDS_NONE || PK)) ? 6 : 1;
+};
+template <typename R> struct XtOcc { static constexpr int w = sizeof(R) == 4 ? 6 : 1; };
+template <int RECON, int SPEEDS, typename R, typename PT, int GEO, bool PK>
+__global__ void __launch_bounds__(256, (X1Occ<R, SPEEDS, PK>::w))
 flux_x1_kernel(BlockDims b, StepConsts c, GeomDev gd, Vars<const PT> P, Vars<R> F,
-               R* __restrict__ cmax_out, R* __restrict__ cmin_out, int j0, int nj, int k0, int nk) {
+               R* __restrict__ cmax_out, R* __restrict__ cmin_out, int j0, int nj, int k0, int nk,
+               const BlockEntry* __restrict__ pk) {
     __shared__ R sr[NV][256];
+    if constexpr (PK) {
+        const long blk = pack_select(pk, b, gd);
+        P = pack_at(P, blk * NV * b.nc);
+        F = pack_at(F, blk * 3 * NV * b.nc);
+        cmax_out = pack_at(cmax_out, blk * 3 * b.nc);
+        cmin_out = pack_at(cmin_out, blk * 3 * b.nc);
+    }
     const int L = b.nx1 + 2;
     const int tid = threadIdx.x;
     const long g = (long)blockIdx.x * 255 + tid;
@@ -154,12 +173,20 @@ flux_x1_kernel(BlockDims b, StepConsts c, GeomDev gd, Vars<const PT> P, Vars<R> 
 // (FP32 ~92, flat FP64 ~104 VGPRs) run 64 x 8, two blocks per CU (64 x 10 and
 // 64 x 16 FP32 blocks measured 24 % and 2 % slower).
 template <typename R, int GEO> struct XtTile { static constexpr int TY = (sizeof(R) == 8 && GEO) ? 12 : 8; };
-template <int DIR, int RECON, int SPEEDS, typename R, typename PT, int GEO>
-__global__ void __launch_bounds__((64 * XtTile<R, GEO>::TY))
+template <int DIR, int RECON, int SPEEDS, typename R, typename PT, int GEO, bool PK>
+__global__ void __launch_bounds__((64 * XtTile<R, GEO>::TY), XtOcc<R>::w)
 flux_xt_kernel(BlockDims b, StepConsts c, GeomDev gd, Vars<const PT> P, Vars<R> F,
-               R* __restrict__ cmax_out, R* __restrict__ cmin_out, int o0, int no, int s0, int s1) {
+               R* __restrict__ cmax_out, R* __restrict__ cmin_out, int o0, int no, int s0, int s1,
+               const BlockEntry* __restrict__ pk) {
     constexpr int TY = XtTile<R, GEO>::TY;
     __shared__ R sr[NV][TY][64];
+    if constexpr (PK) {
+        const long blk = pack_select(pk, b, gd);
+        P = pack_at(P, blk * NV * b.nc);
+        F = pack_at(F, blk * 3 * NV * b.nc);
+        cmax_out = pack_at(cmax_out, blk * 3 * b.nc);
+        cmin_out = pack_at(cmin_out, blk * 3 * b.nc);
+    }
     const int ni = b.nx1 + 2;
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int col = blockIdx.x * 64 + tx;
@@ -192,9 +219,10 @@ flux_xt_kernel(BlockDims b, StepConsts c, GeomDev gd, Vars<const PT> P, Vars<R> 
     face_flux<DIR, SPEEDS, R, GEO>(c, b, gd, jf, if_, Pl, lo, ci, F, cmax_out, cmin_out);
 }
 
-template <int RECON, int SPEEDS, typename R, typename PT, int GEO>
+template <int RECON, int SPEEDS, typename R, typename PT, int GEO, bool PK>
 static hipError_t launch_recon(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int dir, const Vars<const PT>& P, const Vars<R>& F,
-                               R* cmax, R* cmin, hipStream_t s) {
+                               R* cmax, R* cmin, hipStream_t s, const Pack* pk) {
+    const BlockEntry* pd = pack_dev(pk);
     // Face ranges consumed by flux-CT and the divergence: the face-normal index
     // covers [start, end+1]; transverse indices extend one zone each side.
     const int jlo = (b.ndim > 1) ? b.js - 1 : b.js, jhi = (b.ndim > 1) ? b.je + 1 : b.je;
@@ -202,50 +230,52 @@ static hipError_t launch_recon(const BlockDims& b, const StepConsts& c, const Ge
     if (dir == 1) {
         const int nj = jhi - jlo + 1, nk = khi - klo + 1;
         const long cells = (long)(b.nx1 + 2) * nj * nk;
-        const dim3 grid((unsigned)((cells - 1 + 254) / 255));
-        hipLaunchKernelGGL((flux_x1_kernel<RECON, SPEEDS, R, PT, GEO>), grid, dim3(256), 0, s, b, c, gd, P, F, cmax, cmin, jlo, nj, klo, nk);
+        const dim3 grid = pack_grid(dim3((unsigned)((cells - 1 + 254) / 255)), pk);
+        hipLaunchKernelGGL((flux_x1_kernel<RECON, SPEEDS, R, PT, GEO, PK>), grid, dim3(256), 0, s, b, c, gd, P, F, cmax, cmin, jlo, nj, klo, nk, pd);
     } else if (dir == 2) {
         const int no = khi - klo + 1;
         const int ncol = (b.nx1 + 2) * no;
         const int nfaces = b.nx2 + 1;
         constexpr int TY = XtTile<R, GEO>::TY;
-        const dim3 grid((unsigned)((ncol + 63) / 64), (unsigned)((nfaces + TY - 2) / (TY - 1)));
-        hipLaunchKernelGGL((flux_xt_kernel<2, RECON, SPEEDS, R, PT, GEO>), grid, dim3(64, TY), 0, s, b, c, gd, P, F, cmax, cmin, klo, no, b.js, b.je + 1);
+        const dim3 grid = pack_grid(dim3((unsigned)((ncol + 63) / 64), (unsigned)((nfaces + TY - 2) / (TY - 1))), pk);
+        hipLaunchKernelGGL((flux_xt_kernel<2, RECON, SPEEDS, R, PT, GEO, PK>), grid, dim3(64, TY), 0, s, b, c, gd, P, F, cmax, cmin, klo, no, b.js, b.je + 1, pd);
     } else {
         const int no = jhi - jlo + 1;
         const int ncol = (b.nx1 + 2) * no;
         const int nfaces = b.nx3 + 1;
         constexpr int TY = XtTile<R, GEO>::TY;
-        const dim3 grid((unsigned)((ncol + 63) / 64), (unsigned)((nfaces + TY - 2) / (TY - 1)));
-        hipLaunchKernelGGL((flux_xt_kernel<3, RECON, SPEEDS, R, PT, GEO>), grid, dim3(64, TY), 0, s, b, c, gd, P, F, cmax, cmin, jlo, no, b.ks, b.ke + 1);
+        const dim3 grid = pack_grid(dim3((unsigned)((ncol + 63) / 64), (unsigned)((nfaces + TY - 2) / (TY - 1))), pk);
+        hipLaunchKernelGGL((flux_xt_kernel<3, RECON, SPEEDS, R, PT, GEO, PK>), grid, dim3(64, TY), 0, s, b, c, gd, P, F, cmax, cmin, jlo, no, b.ks, b.ke + 1, pd);
     }
     return hipGetLastError();
 }
 
-template <int SPEEDS, typename R, typename PT, int GEO>
+template <int SPEEDS, typename R, typename PT, int GEO, bool PK>
 static hipError_t launch_speeds(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int dir, int recon, const Vars<const PT>& P, const Vars<R>& F,
-                                R* cmax, R* cmin, hipStream_t s) {
+                                R* cmax, R* cmin, hipStream_t s, const Pack* pk) {
     switch (recon) {
-    case RECON_DONOR: return launch_recon<RECON_DONOR, SPEEDS, R, PT, GEO>(b, c, gd, dir, P, F, cmax, cmin, s);
-    case RECON_LINEAR_MC: return launch_recon<RECON_LINEAR_MC, SPEEDS, R, PT, GEO>(b, c, gd, dir, P, F, cmax, cmin, s);
-    case RECON_PPM: return launch_recon<RECON_PPM, SPEEDS, R, PT, GEO>(b, c, gd, dir, P, F, cmax, cmin, s);
-    default: return launch_recon<RECON_WENO5, SPEEDS, R, PT, GEO>(b, c, gd, dir, P, F, cmax, cmin, s);
+    case RECON_DONOR: return launch_recon<RECON_DONOR, SPEEDS, R, PT, GEO, PK>(b, c, gd, dir, P, F, cmax, cmin, s, pk);
+    case RECON_LINEAR_MC: return launch_recon<RECON_LINEAR_MC, SPEEDS, R, PT, GEO, PK>(b, c, gd, dir, P, F, cmax, cmin, s, pk);
+    case RECON_PPM: return launch_recon<RECON_PPM, SPEEDS, R, PT, GEO, PK>(b, c, gd, dir, P, F, cmax, cmin, s, pk);
+    default: return launch_recon<RECON_WENO5, SPEEDS, R, PT, GEO, PK>(b, c, gd, dir, P, F, cmax, cmin, s, pk);
     }
 }
 
-template <typename R, typename PT, int GEO>
+template <typename R, typename PT, int GEO, bool PK>
 static hipError_t launch_geo(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int dir, int recon, int speeds,
-                             const Vars<const PT>& P, const Vars<R>& F, R* cmax, R* cmin, hipStream_t s) {
-    if (speeds == SPEEDS_CTOP) return launch_speeds<SPEEDS_CTOP, R, PT, GEO>(b, c, gd, dir, recon, P, F, cmax, cmin, s);
-    if (speeds == SPEEDS_NONE) return launch_speeds<SPEEDS_NONE, R, PT, GEO>(b, c, gd, dir, recon, P, F, cmax, cmin, s);
-    return launch_speeds<SPEEDS_BOTH, R, PT, GEO>(b, c, gd, dir, recon, P, F, cmax, cmin, s);
+                             const Vars<const PT>& P, const Vars<R>& F, R* cmax, R* cmin, hipStream_t s, const Pack* pk) {
+    if (speeds == SPEEDS_CTOP) return launch_speeds<SPEEDS_CTOP, R, PT, GEO, PK>(b, c, gd, dir, recon, P, F, cmax, cmin, s, pk);
+    if (speeds == SPEEDS_NONE) return launch_speeds<SPEEDS_NONE, R, PT, GEO, PK>(b, c, gd, dir, recon, P, F, cmax, cmin, s, pk);
+    if constexpr (!PK) return launch_speeds<SPEEDS_BOTH, R, PT, GEO, PK>(b, c, gd, dir, recon, P, F, cmax, cmin, s, pk);
+    return hipErrorInvalidValue;   // a pack launch is a stage's: no cmin array
 }
 
-template <typename R, typename PT>
+template <typename R, typename PT, bool PK = false>
 static hipError_t launch_prec(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int dir, int recon, int speeds,
-                              const Vars<const PT>& P, const Vars<R>& F, R* cmax, R* cmin, hipStream_t s) {
-    if (b.geo) return launch_geo<R, PT, 1>(b, c, gd, dir, recon, speeds, P, F, cmax, cmin, s);
-    return launch_geo<R, PT, 0>(b, c, gd, dir, recon, speeds, P, F, cmax, cmin, s);
+                              const Vars<const PT>& P, const Vars<R>& F, R* cmax, R* cmin, hipStream_t s,
+                              const Pack* pk = nullptr) {
+    if (b.geo) return launch_geo<R, PT, 1, PK>(b, c, gd, dir, recon, speeds, P, F, cmax, cmin, s, pk);
+    return launch_geo<R, PT, 0, PK>(b, c, gd, dir, recon, speeds, P, F, cmax, cmin, s, pk);
 }
 
 hipError_t launch_flux(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int dir, int recon, int prec,
@@ -260,9 +290,17 @@ hipError_t launch_flux(const BlockDims& b, const StepConsts& c, const GeomDev& g
 }
 
 hipError_t launch_flux_stage(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int dir, int recon, int prec,
-                             int speeds, const double* P, const float* P32, void* F, void* ctop, hipStream_t s) {
+                             int speeds, const double* P, const float* P32, void* F, void* ctop, hipStream_t s,
+                             const Pack* pk) {
     if (dir == 2 && b.ndim < 2) return hipSuccess;
     if (dir == 3 && b.ndim < 3) return hipSuccess;
+    if (pk) {
+        if (prec == PREC_SPLIT)
+            return launch_prec<float, float, true>(b, c, gd, dir, recon, speeds, vars_of(P32, b.nc),
+                                                   vars_of((float*)F, b.nc), (float*)ctop, nullptr, s, pk);
+        return launch_prec<double, double, true>(b, c, gd, dir, recon, speeds, vars_of(P, b.nc),
+                                                 vars_of((double*)F, b.nc), (double*)ctop, nullptr, s, pk);
+    }
     if (prec == PREC_SPLIT)
         return launch_prec<float, float>(b, c, gd, dir, recon, speeds, vars_of(P32, b.nc), vars_of((float*)F, b.nc),
                                          (float*)ctop, nullptr, s);

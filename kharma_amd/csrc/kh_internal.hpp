@@ -163,6 +163,54 @@ struct GeomDev {
 };
 enum SpeedsOut { SPEEDS_NONE = 0, SPEEDS_BOTH = 1, SPEEDS_CTOP = 2 };
 
+// ---- launches over all blocks of a mesh ----
+// KHARMA launches its kernels over a MeshBlockPack, every block of the mesh in
+// one launch.  Here a launch given a Pack runs gridDim.z = nblk copies of the
+// one-block grid: copy z reads block z's dims and geometry from the device
+// table and addresses block z's part of every container, which the mesh keeps
+// at a fixed stride per block ([block][...]: NV nc for a state, 3 NV nc for the
+// fluxes, 3 nc for the signal speeds, nc for a flag, one DeviceState, one
+// partial per work-group).  The launch arguments are block 0's.
+struct BlockEntry {
+    BlockDims dims;
+    GeomDev geo;
+};
+struct Pack {
+    const BlockEntry* dev = nullptr;   // nblk entries in device memory
+    int nblk = 1;
+    long fix_flux_n = 0;               // threads of the largest block's physical-face flux fix
+    long bounds_n = 0;                 // and of its physical boundary fill
+};
+// This work-group's block of a pack launch (pk null: the one-block launch).
+__device__ __forceinline__ long pack_select(const BlockEntry* pk, BlockDims& b, GeomDev& gd) {
+    if (!pk) return 0;
+    const int blk = blockIdx.z;
+    b = pk[blk].dims;
+    gd = pk[blk].geo;
+    return blk;
+}
+__device__ __forceinline__ long pack_select(const BlockEntry* pk, BlockDims& b) {
+    if (!pk) return 0;
+    const int blk = blockIdx.z;
+    b = pk[blk].dims;
+    return blk;
+}
+template <typename T> __host__ __device__ __forceinline__ T* pack_at(T* p, long off) { return p ? p + off : p; }
+template <typename T> __host__ __device__ __forceinline__ Vars<T> pack_at(const Vars<T>& V, long off) {
+    Vars<T> r;
+    for (int n = 0; n < NV; ++n) r.v[n] = pack_at(V.v[n], off);
+    return r;
+}
+template <typename T> __host__ __device__ __forceinline__ FluxVars<T> pack_at(const FluxVars<T>& F, long off) {
+    FluxVars<T> r;
+    for (int n = 0; n < 3; ++n) r.d[n] = pack_at(F.d[n], off);
+    return r;
+}
+inline dim3 pack_grid(dim3 g, const Pack* pk) { return dim3(g.x, g.y, pk ? (unsigned)pk->nblk : 1u); }
+// The thread counts of the physical-face kernels: the largest over the blocks.
+void pack_set_sizes(Pack& pk, const BlockDims* dims, int nblk);
+inline const BlockEntry* pack_dev(const Pack* pk) { return pk ? pk->dev : nullptr; }
+
 // ---- launchers (kh_flux.hip) ----
 // Face fluxes through the faces normal to `dir` that the step consumes:
 // X1 faces i in [is, ie+1], j in [js-1, je+1], k in [ks-1, ke+1] (collapsed in
@@ -173,7 +221,8 @@ hipError_t launch_flux(const BlockDims& b, const StepConsts& c, const GeomDev& g
 // Stage-driver variant: speeds = SPEEDS_NONE or SPEEDS_CTOP (ctop = max(cmax, cmin)
 // into ctop[dir-1]); the split form reads the FP32 shadow P32 of the primitives.
 hipError_t launch_flux_stage(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int dir, int recon, int prec,
-                             int speeds, const double* P, const float* P32, void* F, void* ctop, hipStream_t s);
+                             int speeds, const double* P, const float* P32, void* F, void* ctop, hipStream_t s,
+                             const Pack* pk = nullptr);
 
 // ---- launchers (kh_step.hip) ----
 hipError_t launch_emf(const BlockDims& b, int prec, const FluxVars<void>& F, void* emf, hipStream_t s);
@@ -206,7 +255,7 @@ hipError_t launch_flags_from_real(const double* r, int* f, long n, hipStream_t s
 hipError_t launch_update(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int prec, int stage, bool do_dt,
                          const void* F, const void* ctop, const double* Usub, const double* Ufull, const double* Psub,
                          double* Uout, double* Pout, float* Pout32, int* pflag, int* fflag,
-                         int* fail_list, DeviceState* ds, double* dt_part, hipStream_t s);
+                         int* fail_list, DeviceState* ds, double* dt_part, hipStream_t s, const Pack* pk = nullptr);
 // The two halves of launch_update on the curved-space FP64 path: parts bit 1
 // runs the combine (U* = g0 U_sub + g1 U_full + beta dt (div F + S) into Uout),
 // bit 2 the recovery (U* -> P, floors, P -> U).  With Ustar non-null the
@@ -254,26 +303,27 @@ hipError_t launch_update_faces(const BlockDims& b, const StepConsts& c, const Ge
 long update_blocks(const BlockDims& b);
 long divb_blocks(const BlockDims& b);
 hipError_t launch_fixup(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int stage, double* U, double* P,
-                        float* P32, const int* pflag, const int* fail_list, DeviceState* ds, hipStream_t s);
+                        float* P32, const int* pflag, const int* fail_list, DeviceState* ds, hipStream_t s,
+                        const Pack* pk = nullptr);
 // Flux fixes at physical boundaries, on the Riemann fluxes before the update:
 // B_FluxCT::ZeroBoundaryFlux on polar faces (the update kernel's flux-CT then
 // sees mirrored B2 fluxes), the no-inflow mass-flux clamp on X1 faces and the
 // zeroed polar fluxes of KBoundaries::FixFlux.
 hipError_t launch_fix_flux(const BlockDims& b, int prec, const FluxVars<void>& F, hipStream_t s);
-hipError_t launch_fix_flux(const BlockDims& b, int prec, void* F, hipStream_t s);
+hipError_t launch_fix_flux(const BlockDims& b, int prec, void* F, hipStream_t s, const Pack* pk = nullptr);
 // Physical boundary ghost zones of U, P (and the FP32 shadow if non-null) from
 // the final interior state: KBoundaries::ApplyBoundary for each x1/x2 face.
 hipError_t launch_bounds(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const Vars<double>& U,
                          const Vars<double>& P, float* P32, hipStream_t s);
 hipError_t launch_bounds(const BlockDims& b, const StepConsts& c, const GeomDev& gd, double* U, double* P, float* P32,
-                         hipStream_t s);
+                         hipStream_t s, const Pack* pk = nullptr);
 // End-of-step reductions and clock; ds: nds block states (a mesh), the first
 // carrying the clock and totals.
 hipError_t launch_finalize(const StepConsts& c, DeviceState* ds, const double* dt_part, int n_dt,
                            const double* divb_part, int n_divb, double* dt_hist, int hist_cap, hipStream_t s,
                            int nds = 1);
 hipError_t launch_max_divb(const BlockDims& b, const Vars<const double>& U, double* part, hipStream_t s);
-hipError_t launch_max_divb(const BlockDims& b, const double* U, double* part, hipStream_t s);
+hipError_t launch_max_divb(const BlockDims& b, const double* U, double* part, hipStream_t s, const Pack* pk = nullptr);
 hipError_t launch_reduce(const double* part, int n, int is_max, double* out, hipStream_t s);
 hipError_t launch_convert_f32(const double* A, float* B, long n, hipStream_t s);
 hipError_t launch_reset_state(DeviceState* ds, double dt, hipStream_t s);

@@ -13,7 +13,10 @@
 // physical-face flux fixes, the updates, the fix-ups and the physical boundary
 // fills, each over every block, so that no block reads a ghost zone before the
 // block owning it has written it; per step one divB and one finalize over the
-// partials of all blocks.
+// partials of all blocks.  By default every one of these kernels is one launch
+// over all blocks (the blocks' dims and geometry in a device table, kh::Pack),
+// as the reference launches over a MeshBlockPack; kh_mesh_set_packed(m, 0)
+// launches once per block instead, with the same results bit for bit.
 #include "kharma_amd.h"
 
 #include <algorithm>
@@ -44,6 +47,10 @@ struct kh_mesh {
     hipStream_t cap = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
+    // one launch per kernel over all blocks (kh_mesh_set_packed)
+    bool packed = true;
+    Pack pack;
+    BlockEntry* pack_dev = nullptr;
 };
 
 namespace {
@@ -70,6 +77,23 @@ int mesh_stage(kh_mesh* m, int stage, hipStream_t s) {
     auto F = [&](int b) { return (char*)m->F + fe * 3 * st * b; };
     auto ct = [&](int b) { return (char*)m->ctop + fe * 3 * nc * b; };
     auto f32 = [&](float* a, int b) { return a ? a + st * b : nullptr; };
+    const StepConsts& c = m->blk[0]->consts;
+    if (m->packed) {
+        // every kernel once over all blocks, with block 0's arguments
+        const kh_block* k = m->blk[0];
+        const Pack* pk = &m->pack;
+        for (int dir = 1; dir <= k->dims.ndim; ++dir)
+            KM_HIP(launch_flux_stage(k->dims, k->consts, k->geo, dir, k->p.recon, prec, speeds, Psub, Psubf,
+                                     F(0) + fe * (dir - 1) * st, ct(0) + fe * (dir - 1) * nc, s, pk));
+        if (pk->fix_flux_n) KM_HIP(launch_fix_flux(k->dims, prec, F(0), s, pk));
+        KM_HIP(launch_update(k->dims, k->consts, k->geo, prec, stage, stage == 2, F(0), ct(0), Usub, m->U0, Psub, Uout,
+                             Pout, Poutf, m->pflag, m->fflag, m->fail_list, m->ds, m->dt_part, s, pk));
+        if (c.fix_average || c.fix_atmo)
+            KM_HIP(launch_fixup(k->dims, k->consts, k->geo, stage, Uout, Pout, Poutf, m->pflag, m->fail_list, m->ds, s,
+                                pk));
+        if (pk->bounds_n) KM_HIP(launch_bounds(k->dims, k->consts, k->geo, Uout, Pout, Poutf, s, pk));
+        return KH_OK;
+    }
     for (int b = 0; b < md.nblk; ++b) {
         const kh_block* k = m->blk[b];
         for (int dir = 1; dir <= k->dims.ndim; ++dir)
@@ -87,7 +111,6 @@ int mesh_stage(kh_mesh* m, int stage, hipStream_t s) {
                              m->pflag + nc * b, m->fflag + nc * b, m->fail_list + nc * b, m->ds + b,
                              m->dt_part + m->n_dt * b, s));
     }
-    const StepConsts& c = m->blk[0]->consts;
     if (c.fix_average || c.fix_atmo)
         for (int b = 0; b < md.nblk; ++b) {
             const kh_block* k = m->blk[b];
@@ -112,7 +135,9 @@ int mesh_step_body(kh_mesh* m, hipStream_t s) {
         const int r = mesh_stage(m, stage, s);
         if (r != KH_OK) return r;
     }
-    if (md.ndim > 1)
+    if (md.ndim > 1 && m->packed)
+        KM_HIP(launch_max_divb(m->blk[0]->dims, m->U0, m->divb_part, s, &m->pack));
+    else if (md.ndim > 1)
         for (int b = 0; b < md.nblk; ++b)
             KM_HIP(launch_max_divb(m->blk[b]->dims, m->U0 + st * b, m->divb_part + m->n_divb * b, s));
     KM_HIP(launch_finalize(c, m->ds, m->dt_part, (int)(m->n_dt * md.nblk), m->divb_part,
@@ -213,6 +238,19 @@ int kh_mesh_create(const kh_params* p, int nb1, int nb2, int nb3, kh_mesh** out)
     alloc((void**)&m->dt_part, sizeof(double) * (m->n_dt * nblk + 1));
     alloc((void**)&m->divb_part, sizeof(double) * (m->n_divb * nblk + 1));
     alloc((void**)&m->dt_hist, sizeof(double) * m->hist_cap);
+    alloc((void**)&m->pack_dev, sizeof(BlockEntry) * nblk);
+    if (e == hipSuccess) {
+        std::vector<BlockEntry> tab(nblk);
+        std::vector<BlockDims> dims(nblk);
+        for (long b = 0; b < nblk; ++b) {
+            tab[b].dims = dims[b] = m->blk[b]->dims;
+            tab[b].geo = m->blk[b]->geo;
+        }
+        e = hipMemcpy(m->pack_dev, tab.data(), sizeof(BlockEntry) * nblk, hipMemcpyHostToDevice);
+        m->pack.dev = m->pack_dev;
+        m->pack.nblk = (int)nblk;
+        pack_set_sizes(m->pack, dims.data(), (int)nblk);
+    }
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) return cleanup(KH_ERR_HIP, std::string("device allocation: ") + hipGetErrorString(e));
     *out = m;
@@ -224,7 +262,7 @@ void kh_mesh_destroy(kh_mesh* m) {
     (void)hipDeviceSynchronize();
     for (void* q : {(void*)m->U0, (void*)m->P0, (void*)m->U1, (void*)m->P1, (void*)m->P0f, (void*)m->P1f, m->F,
                     m->ctop, (void*)m->pflag, (void*)m->fflag, (void*)m->fail_list, (void*)m->ds,
-                    (void*)m->dt_part, (void*)m->divb_part, (void*)m->dt_hist})
+                    (void*)m->dt_part, (void*)m->divb_part, (void*)m->dt_hist, (void*)m->pack_dev})
         if (q) (void)hipFree(q);
     for (kh_block* b : m->blk) kh_block_destroy(b);
     drop_graph(m);
@@ -297,6 +335,13 @@ int kh_mesh_step(kh_mesh* m, int nsteps) {
         KM_HIP(hipGraphInstantiate(&m->gexec, g, nullptr, nullptr, 0));
     }
     for (int n = 0; n < nsteps; ++n) KM_HIP(hipGraphLaunch(m->gexec, m->stream));
+    return KH_OK;
+}
+
+int kh_mesh_set_packed(kh_mesh* m, int enable) {
+    if (!m) return api_fail(KH_ERR_INVALID, "null mesh");
+    if (m->packed != (enable != 0)) drop_graph(m);
+    m->packed = enable != 0;
     return KH_OK;
 }
 

@@ -11,6 +11,7 @@
 //   + (last stage) the CFL estimate (GRMHD::EstimateTimestep, grmhd.cpp:313-328)
 // and writes U, P and the flags once, also into the periodic ghost images of
 // the zone (the boundary exchange), so no separate sync pass reads the state.
+#include <algorithm>
 #include <mutex>
 
 #include "kh_internal.hpp"
@@ -401,13 +402,25 @@ __device__ __forceinline__ int invert_zone(const G& g, const double U[NV], doubl
 // BM: 0 no B transport (1-D), 1 flux-CT EMFs from the face fluxes, 2 face-centred
 // B (B_CT): the cell B is the average of the stage's new faces fB (B_CT::BlockUtoP,
 // b_ct.cpp:173-205, which the reference runs before the inversion).
-template <typename R, bool DO_DT, int BM, int GEO, int INV = 0>
+template <typename R, bool DO_DT, int BM, int GEO, int INV = 0, bool PK = false>
 __global__ void __launch_bounds__(BS, UpdateOcc<GEO>::min_blocks)
 update_kernel(BlockDims b, StepConsts cst, GeomDev gd, int stage, const R* __restrict__ F, const R* __restrict__ ctop,
               const double* __restrict__ Usub, const double* Ufull, const double* __restrict__ Psub,
               double* Uout, double* __restrict__ Pout, float* __restrict__ Pout32,
               int* __restrict__ pflag, int* __restrict__ fflag, int* __restrict__ fail_list,
-              DeviceState* __restrict__ ds, double* __restrict__ dt_part, const double* __restrict__ fB) {
+              DeviceState* __restrict__ ds, double* __restrict__ dt_part, const double* __restrict__ fB,
+              const BlockEntry* __restrict__ pk) {
+    if constexpr (PK) {
+        const long blk = pack_select(pk, b, gd);
+        const long sv = blk * NV * b.nc;
+        F = pack_at(F, 3 * sv);
+        ctop = pack_at(ctop, blk * 3 * b.nc);
+        Usub = pack_at(Usub, sv); Ufull = pack_at(Ufull, sv); Psub = pack_at(Psub, sv);
+        Uout = pack_at(Uout, sv); Pout = pack_at(Pout, sv); Pout32 = pack_at(Pout32, sv);
+        pflag = pack_at(pflag, blk * b.nc); fflag = pack_at(fflag, blk * b.nc); fail_list = pack_at(fail_list, blk * b.nc);
+        ds += blk;
+        dt_part = pack_at(dt_part, blk * gridDim.x);
+    }
     const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = t < (unsigned)ninterior(b);
     double ndt = DBL_MAX;
@@ -567,11 +580,20 @@ __device__ __forceinline__ void stage_recover_geometry(const GeomDev& gd, int j,
 
 // ND: dimensionality (flux-CT for 2 and 3); SAME: stage with gam0 = 0 and
 // U_sub == U_full, which reads the state once (0*U + 1*U == U).
-template <typename R, bool DO_DT, int ND, bool SAME>
+template <typename R, bool DO_DT, int ND, bool SAME, bool PK>
 __global__ void __launch_bounds__(BS, KH_CMB_OCC)
 combine_kernel(BlockDims b, StepConsts cst, GeomDev gd, int stage, const R* __restrict__ F, const R* __restrict__ ctop,
                const double* __restrict__ Usub, const double* Ufull, const double* __restrict__ Psub, double* Uout,
-               const DeviceState* __restrict__ ds, double* __restrict__ dt_part) {
+               const DeviceState* __restrict__ ds, double* __restrict__ dt_part, const BlockEntry* __restrict__ pk) {
+    if constexpr (PK) {
+        const long blk = pack_select(pk, b, gd);
+        const long sv = blk * NV * b.nc;
+        F = pack_at(F, 3 * sv);
+        ctop = pack_at(ctop, blk * 3 * b.nc);
+        Usub = pack_at(Usub, sv); Ufull = pack_at(Ufull, sv); Psub = pack_at(Psub, sv); Uout = pack_at(Uout, sv);
+        ds += blk;
+        dt_part = pack_at(dt_part, blk * gridDim.x);
+    }
     const unsigned t = xcd_logical_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const bool active = t < (unsigned)ninterior(b);
     double ndt = DBL_MAX;
@@ -667,13 +689,21 @@ __device__ __forceinline__ void recover_zone(const BlockDims& b, const StepConst
     }
 }
 
-template <int FRAME, int INV, bool STG>
+template <int FRAME, int INV, bool STG, bool PK>
 __global__ void __launch_bounds__(BS, KH_REC_OCC)
 recover_kernel(BlockDims b, StepConsts cst, GeomDev gd, int stage, const double* __restrict__ Psub, double* Uout,
                double* __restrict__ Pout, float* __restrict__ Pout32, int* __restrict__ pflag,
                int* __restrict__ fflag, int* __restrict__ fail_list, DeviceState* __restrict__ ds,
-               double* __restrict__ Ustar) {
+               double* __restrict__ Ustar, const BlockEntry* __restrict__ pk) {
     __shared__ double sg[STG ? STG_RECOVER * TILE_I : 1];
+    if constexpr (PK) {
+        const long blk = pack_select(pk, b, gd);
+        const long sv = blk * NV * b.nc;
+        Psub = pack_at(Psub, sv); Uout = pack_at(Uout, sv); Pout = pack_at(Pout, sv); Pout32 = pack_at(Pout32, sv);
+        Ustar = pack_at(Ustar, sv);
+        pflag = pack_at(pflag, blk * b.nc); fflag = pack_at(fflag, blk * b.nc); fail_list = pack_at(fail_list, blk * b.nc);
+        ds += blk;
+    }
     constexpr int NP = 5;   // the guess: the fluid primitives (B follows from U)
     int k, j, i;
     long c;
@@ -709,11 +739,18 @@ recover_kernel(BlockDims b, StepConsts cst, GeomDev gd, int stage, const double*
 // Inverter::FixUtoP (inverter/fixup.cpp:47-156) on the zones listed by the
 // update kernel: distance-weighted average of non-failed neighbours, atmosphere
 // fallback, geometric floors, then the conserved state of the final P->U.
-template <int GEO>
+template <int GEO, bool PK>
 __global__ void __launch_bounds__(BS)
 fixup_list_kernel(BlockDims b, StepConsts cst, GeomDev gd, int stage, double* __restrict__ U, double* __restrict__ P,
                   float* __restrict__ P32, const int* __restrict__ pflag, const int* __restrict__ fail_list,
-                  const DeviceState* __restrict__ ds) {
+                  const DeviceState* __restrict__ ds, const BlockEntry* __restrict__ pk) {
+    if constexpr (PK) {
+        const long blk = pack_select(pk, b, gd);
+        const long sv = blk * NV * b.nc;
+        U = pack_at(U, sv); P = pack_at(P, sv); P32 = pack_at(P32, sv);
+        pflag = pack_at(pflag, blk * b.nc); fail_list = pack_at(fail_list, blk * b.nc);
+        ds += blk;
+    }
     const int n = ds->fail_count[stage - 1];
     for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
         const long c = fail_list[q];
@@ -831,7 +868,14 @@ __global__ void set_clock_kernel(DeviceState* ds, double time, double dt, int nc
 // B_FluxCT::MaxDivB with corner_div (b_flux_ct.cpp:566-607, b_flux_ct_functions.hpp:52-75)
 // Corners i in [is, ie+1]; ValidDivBX1 (b_flux_ct.cpp:554-564) skips the
 // corners on a physical X1 face (i = is inner, i = ie + 1 outer).
-__global__ void __launch_bounds__(BS) max_divb_kernel(BlockDims b, Vars<const double> U, double* __restrict__ part) {
+template <bool PK>
+__global__ void __launch_bounds__(BS) max_divb_kernel(BlockDims b, Vars<const double> U, double* __restrict__ part,
+                                                      const BlockEntry* __restrict__ pk) {
+    if constexpr (PK) {
+        const long blk = pack_select(pk, b);
+        U = pack_at(U, blk * NV * b.nc);
+        part = pack_at(part, blk * gridDim.x);
+    }
     const int i0 = b.img[0][0] ? b.is : b.is + 1;
     const int ni = (b.img[0][1] ? b.ie + 1 : b.ie) - i0 + 1, nj = b.nx2 + 1, nk = (b.ndim > 2) ? b.nx3 + 1 : 1;
     const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -866,7 +910,7 @@ __global__ void __launch_bounds__(BS) max_divb_kernel(BlockDims b, Vars<const do
 // b_flux_ct.cpp:372-420; KBoundaries::FixFlux, boundaries.cpp:611-690).  One
 // thread per face of the fixed rows; the two kinds of rows are disjoint.
 template <typename R>
-__global__ void __launch_bounds__(BS) fix_flux_kernel(BlockDims b, FluxVars<R> F) {
+__device__ __forceinline__ void fix_flux_zone(const BlockDims& b, const FluxVars<R>& F, long off) {
     const int nd = b.ndim;
     const int ni = b.nx1 + 2, nk = (nd > 2) ? b.nx3 + 2 : 1, nj = (nd > 1) ? b.nx2 + 2 : 1;
     const int i0 = b.is - 1, k0 = (nd > 2) ? b.ks - 1 : b.ks, j0 = (nd > 1) ? b.js - 1 : b.js;
@@ -882,7 +926,7 @@ __global__ void __launch_bounds__(BS) fix_flux_kernel(BlockDims b, FluxVars<R> F
         if (!pol[side]) return;
         const int k = k0 + rem / ni, i = i0 + rem % ni;
         const int j = side == 0 ? b.js : b.je + 1;
-        const long c = (long)k * b.s3 + (long)j * b.s2 + i;
+        const long c = (long)k * b.s3 + (long)j * b.s2 + i + off;
         const long cg = side == 0 ? c - b.s2 : c;        // the face row that is mirrored into
         const long cs = side == 0 ? c : c - b.s2;        // and its source
         if (b.fix_polar_flux) {
@@ -902,9 +946,20 @@ __global__ void __launch_bounds__(BS) fix_flux_kernel(BlockDims b, FluxVars<R> F
         const int k = k0 + rem / nj, j = j0 + rem % nj;
         if (!b.check_inflow[side]) return;
         const int i = side == 0 ? b.is : b.ie + 1;
-        const long c = (long)k * b.s3 + (long)j * b.s2 + i;
+        const long c = (long)k * b.s3 + (long)j * b.s2 + i + off;
         const R f = F1[RHO][c];
         F1[RHO][c] = side == 0 ? smin(f, R(0)) : smax(f, R(0));
+    }
+}
+// off: this block's part of the pack's fluxes, as an index offset; the block's
+// dims are read in place from the pack's table.
+template <typename R, bool PK>
+__global__ void __launch_bounds__(BS) fix_flux_kernel(BlockDims b, FluxVars<R> F, const BlockEntry* __restrict__ pk) {
+    if constexpr (PK) {
+        const BlockDims& bb = pk[blockIdx.z].dims;
+        fix_flux_zone<R>(bb, F, (long)blockIdx.z * 3 * NV * bb.nc);
+    } else {
+        fix_flux_zone<R>(b, F, 0);
     }
 }
 
@@ -962,9 +1017,16 @@ __host__ __device__ inline int bounds_slab2_width(const BlockDims& b) {
 __host__ __device__ inline long bounds_slab2(const BlockDims& b) {
     return (long)b.n3 * b.ng * nphys(b, 1) * bounds_slab2_width(b);
 }
-template <int GEO>
+template <int GEO, bool PK>
 __global__ void __launch_bounds__(BS) bounds_kernel(BlockDims b, StepConsts cst, GeomDev gd, Vars<double> U,
-                                                    Vars<double> P, float* __restrict__ P32) {
+                                                    Vars<double> P, float* __restrict__ P32,
+                                                    const BlockEntry* __restrict__ pk) {
+    if constexpr (PK) {
+        const long blk = pack_select(pk, b, gd);
+        if (!has_physical_face(b)) return;
+        const long sv = blk * NV * b.nc;
+        U = pack_at(U, sv); P = pack_at(P, sv); P32 = pack_at(P32, sv);
+    }
     long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long n1s = bounds_slab1(b);
     int k, j, i;
@@ -1435,20 +1497,21 @@ hipError_t launch_fofc_mark(const BlockDims& b, const StepConsts& c, const GeomD
     return hipGetLastError();
 }
 
-template <typename R>
-static void launch_update_geo(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int stage, bool do_dt,
+template <typename R, bool PK>
+static void launch_update_geo_k(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int stage, bool do_dt,
                               const void* F, const void* ctop, const double* Usub, const double* Ufull,
                               const double* Psub, double* Uout, double* Pout, float* Pout32, int* pflag, int* fflag,
-                              int* fail_list, DeviceState* ds, double* dt_part, hipStream_t s, int parts = 3,
-                              double* Ustar = nullptr) {
-    const dim3 g = grid_for(ninterior_host(b));
+                              int* fail_list, DeviceState* ds, double* dt_part, hipStream_t s, int parts,
+                              double* Ustar, const Pack* pk) {
+    const dim3 g = pack_grid(grid_for(ninterior_host(b)), pk);
     const bool rstg = b.ndim == 3;   // recover on LDS-staged tiles
-    const dim3 gr = rstg ? dim3((unsigned)update_tiles_host(b)) : g;
+    const dim3 gr = rstg ? pack_grid(dim3((unsigned)update_tiles_host(b)), pk) : g;
+    const BlockEntry* pd = pack_dev(pk);
     const bool same = (c.gam0[stage - 1] == 0.) && (Usub == Ufull);
     if (parts & 1) {
 #define KH_CMB(DT, ND, SAME)                                                                                          \
-    hipLaunchKernelGGL((combine_kernel<R, DT, ND, SAME>), g, dim3(BS), 0, s, b, c, gd, stage, (const R*)F, (const R*)ctop, \
-                       Usub, Ufull, Psub, Uout, ds, dt_part)
+    hipLaunchKernelGGL((combine_kernel<R, DT, ND, SAME, PK>), g, dim3(BS), 0, s, b, c, gd, stage, (const R*)F, (const R*)ctop, \
+                       Usub, Ufull, Psub, Uout, ds, dt_part, pd)
 #define KH_CMB_ND(DT, SAME) \
     { if (b.ndim == 3) KH_CMB(DT, 3, SAME); else if (b.ndim == 2) KH_CMB(DT, 2, SAME); else KH_CMB(DT, 1, SAME); }
     if (do_dt) { if (same) KH_CMB_ND(true, true) else KH_CMB_ND(true, false) }
@@ -1459,10 +1522,10 @@ static void launch_update_geo(const BlockDims& b, const StepConsts& c, const Geo
     if (!(parts & 2)) return;
 #define KH_REC(FR, INV)                                                                                             \
     do {                                                                                                            \
-        if (rstg) hipLaunchKernelGGL((recover_kernel<FR, INV, true>), gr, dim3(BS), 0, s, b, c, gd, stage, Psub,        \
-                                     Uout, Pout, Pout32, pflag, fflag, fail_list, ds, Ustar);                           \
-        else hipLaunchKernelGGL((recover_kernel<FR, INV, false>), gr, dim3(BS), 0, s, b, c, gd, stage, Psub, Uout, Pout, \
-                                Pout32, pflag, fflag, fail_list, ds, Ustar);                                            \
+        if (rstg) hipLaunchKernelGGL((recover_kernel<FR, INV, true, PK>), gr, dim3(BS), 0, s, b, c, gd, stage, Psub,        \
+                                     Uout, Pout, Pout32, pflag, fflag, fail_list, ds, Ustar, pd);                       \
+        else hipLaunchKernelGGL((recover_kernel<FR, INV, false, PK>), gr, dim3(BS), 0, s, b, c, gd, stage, Psub, Uout, Pout, \
+                                Pout32, pflag, fflag, fail_list, ds, Ustar, pd);                                        \
     } while (0)
     if (c.inverter == 1) {
         if (c.floors.frame == 0) KH_REC(0, 1);
@@ -1475,16 +1538,29 @@ static void launch_update_geo(const BlockDims& b, const StepConsts& c, const Geo
     }
 #undef KH_REC
 }
-template <typename R, int GEO>
-static void launch_update_t(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int stage, bool do_dt,
+template <typename R>
+static void launch_update_geo(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int stage, bool do_dt,
+                              const void* F, const void* ctop, const double* Usub, const double* Ufull,
+                              const double* Psub, double* Uout, double* Pout, float* Pout32, int* pflag, int* fflag,
+                              int* fail_list, DeviceState* ds, double* dt_part, hipStream_t s, int parts = 3,
+                              double* Ustar = nullptr, const Pack* pk = nullptr) {
+    if (pk) launch_update_geo_k<R, true>(b, c, gd, stage, do_dt, F, ctop, Usub, Ufull, Psub, Uout, Pout, Pout32, pflag,
+                                         fflag, fail_list, ds, dt_part, s, parts, Ustar, pk);
+    else launch_update_geo_k<R, false>(b, c, gd, stage, do_dt, F, ctop, Usub, Ufull, Psub, Uout, Pout, Pout32, pflag,
+                                       fflag, fail_list, ds, dt_part, s, parts, Ustar, pk);
+}
+template <typename R, int GEO, bool PK>
+static void launch_update_t_k(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int stage, bool do_dt,
                             const void* F, const void* ctop, const double* Usub, const double* Ufull, const double* Psub,
                             double* Uout, double* Pout, float* Pout32, int* pflag, int* fflag,
-                            int* fail_list, DeviceState* ds, double* dt_part, hipStream_t s, const double* fB = nullptr) {
-    const dim3 g = grid_for(ninterior_host(b));
+                            int* fail_list, DeviceState* ds, double* dt_part, hipStream_t s, const double* fB,
+                            const Pack* pk) {
+    const dim3 g = pack_grid(grid_for(ninterior_host(b)), pk);
     const int bm = fB ? 2 : (b.ndim > 1 ? 1 : 0);
+    const BlockEntry* pd = pack_dev(pk);
 #define KH_UPD(DT, BM, INV)                                                                                         \
-    hipLaunchKernelGGL((update_kernel<R, DT, BM, GEO, INV>), g, dim3(BS), 0, s, b, c, gd, stage, (const R*)F, (const R*)ctop, \
-                       Usub, Ufull, Psub, Uout, Pout, Pout32, pflag, fflag, fail_list, ds, dt_part, fB)
+    hipLaunchKernelGGL((update_kernel<R, DT, BM, GEO, INV, PK>), g, dim3(BS), 0, s, b, c, gd, stage, (const R*)F, (const R*)ctop, \
+                       Usub, Ufull, Psub, Uout, Pout, Pout32, pflag, fflag, fail_list, ds, dt_part, fB, pd)
 #define KH_UPD_BM(DT, INV) \
     { if (bm == 2) KH_UPD(DT, 2, INV); else if (bm == 1) KH_UPD(DT, 1, INV); else KH_UPD(DT, 0, INV); }
     if (c.inverter == 1) {
@@ -1494,6 +1570,17 @@ static void launch_update_t(const BlockDims& b, const StepConsts& c, const GeomD
     }
 #undef KH_UPD_BM
 #undef KH_UPD
+}
+template <typename R, int GEO>
+static void launch_update_t(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int stage, bool do_dt,
+                            const void* F, const void* ctop, const double* Usub, const double* Ufull, const double* Psub,
+                            double* Uout, double* Pout, float* Pout32, int* pflag, int* fflag,
+                            int* fail_list, DeviceState* ds, double* dt_part, hipStream_t s, const double* fB = nullptr,
+                            const Pack* pk = nullptr) {
+    if (pk) launch_update_t_k<R, GEO, true>(b, c, gd, stage, do_dt, F, ctop, Usub, Ufull, Psub, Uout, Pout, Pout32,
+                                            pflag, fflag, fail_list, ds, dt_part, s, fB, pk);
+    else launch_update_t_k<R, GEO, false>(b, c, gd, stage, do_dt, F, ctop, Usub, Ufull, Psub, Uout, Pout, Pout32,
+                                          pflag, fflag, fail_list, ds, dt_part, s, fB, pk);
 }
 hipError_t launch_update_faces(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int prec, int stage,
                                bool do_dt, const void* F, const void* ctop, const double* Usub, const double* Ufull,
@@ -1511,13 +1598,13 @@ hipError_t launch_update_faces(const BlockDims& b, const StepConsts& c, const Ge
 hipError_t launch_update(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int prec, int stage, bool do_dt,
                          const void* F, const void* ctop, const double* Usub, const double* Ufull, const double* Psub,
                          double* Uout, double* Pout, float* Pout32, int* pflag, int* fflag,
-                         int* fail_list, DeviceState* ds, double* dt_part, hipStream_t s) {
+                         int* fail_list, DeviceState* ds, double* dt_part, hipStream_t s, const Pack* pk) {
     if (prec == PREC_SPLIT) {
-        if (b.geo) launch_update_geo<float>(b, c, gd, stage, do_dt, F, ctop, Usub, Ufull, Psub, Uout, Pout, Pout32, pflag, fflag, fail_list, ds, dt_part, s);
-        else launch_update_t<float, 0>(b, c, gd, stage, do_dt, F, ctop, Usub, Ufull, Psub, Uout, Pout, Pout32, pflag, fflag, fail_list, ds, dt_part, s);
+        if (b.geo) launch_update_geo<float>(b, c, gd, stage, do_dt, F, ctop, Usub, Ufull, Psub, Uout, Pout, Pout32, pflag, fflag, fail_list, ds, dt_part, s, 3, nullptr, pk);
+        else launch_update_t<float, 0>(b, c, gd, stage, do_dt, F, ctop, Usub, Ufull, Psub, Uout, Pout, Pout32, pflag, fflag, fail_list, ds, dt_part, s, nullptr, pk);
     } else {
-        if (b.geo) launch_update_geo<double>(b, c, gd, stage, do_dt, F, ctop, Usub, Ufull, Psub, Uout, Pout, Pout32, pflag, fflag, fail_list, ds, dt_part, s);
-        else launch_update_t<double, 0>(b, c, gd, stage, do_dt, F, ctop, Usub, Ufull, Psub, Uout, Pout, Pout32, pflag, fflag, fail_list, ds, dt_part, s);
+        if (b.geo) launch_update_geo<double>(b, c, gd, stage, do_dt, F, ctop, Usub, Ufull, Psub, Uout, Pout, Pout32, pflag, fflag, fail_list, ds, dt_part, s, 3, nullptr, pk);
+        else launch_update_t<double, 0>(b, c, gd, stage, do_dt, F, ctop, Usub, Ufull, Psub, Uout, Pout, Pout32, pflag, fflag, fail_list, ds, dt_part, s, nullptr, pk);
     }
     return hipGetLastError();
 }
@@ -1546,34 +1633,67 @@ long divb_blocks(const BlockDims& b) {
     return ((long)(b.nx1 + 1) * (b.nx2 + 1) * ((b.ndim > 2) ? b.nx3 + 1 : 1) + BS - 1) / BS;
 }
 hipError_t launch_fixup(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int stage, double* U, double* P,
-                        float* P32, const int* pflag, const int* fail_list, DeviceState* ds, hipStream_t s) {
-    if (b.geo) hipLaunchKernelGGL(fixup_list_kernel<1>, dim3(64), dim3(BS), 0, s, b, c, gd, stage, U, P, P32, pflag, fail_list, ds);
-    else hipLaunchKernelGGL(fixup_list_kernel<0>, dim3(64), dim3(BS), 0, s, b, c, gd, stage, U, P, P32, pflag, fail_list, ds);
+                        float* P32, const int* pflag, const int* fail_list, DeviceState* ds, hipStream_t s,
+                        const Pack* pk) {
+    const dim3 g = pack_grid(dim3(64), pk);
+    const BlockEntry* pd = pack_dev(pk);
+#define KH_FIX(GEO, PK) \
+    hipLaunchKernelGGL((fixup_list_kernel<GEO, PK>), g, dim3(BS), 0, s, b, c, gd, stage, U, P, P32, pflag, fail_list, ds, pd)
+    if (pk) { if (b.geo) KH_FIX(1, true); else KH_FIX(0, true); }
+    else { if (b.geo) KH_FIX(1, false); else KH_FIX(0, false); }
+#undef KH_FIX
+    return hipGetLastError();
+}
+static long fix_flux_threads(const BlockDims& b) {
+    const long npolar = nphys(b, 1) ? 2L * (b.nx1 + 2) * ((b.ndim > 2) ? b.nx3 + 2 : 1) : 0;
+    const long nx1f = nphys(b, 0) ? 2L * ((b.ndim > 1) ? b.nx2 + 2 : 1) * ((b.ndim > 2) ? b.nx3 + 2 : 1) : 0;
+    return npolar + nx1f;
+}
+static hipError_t launch_fix_flux_pk(const BlockDims& b, int prec, const FluxVars<void>& F, hipStream_t s,
+                                     const Pack* pk) {
+    const long n = pk ? pk->fix_flux_n : fix_flux_threads(b);
+    if (n == 0) return hipSuccess;
+    const dim3 g = pack_grid(grid_for(n), pk);
+    const BlockEntry* pd = pack_dev(pk);
+#define KH_FF(R, PK) hipLaunchKernelGGL((fix_flux_kernel<R, PK>), g, dim3(BS), 0, s, b, flux_vars_cast<R>(F), pd)
+    if (pk) { if (prec == PREC_SPLIT) KH_FF(float, true); else KH_FF(double, true); }
+    else { if (prec == PREC_SPLIT) KH_FF(float, false); else KH_FF(double, false); }
+#undef KH_FF
     return hipGetLastError();
 }
 hipError_t launch_fix_flux(const BlockDims& b, int prec, const FluxVars<void>& F, hipStream_t s) {
-    const long npolar = nphys(b, 1) ? 2L * (b.nx1 + 2) * ((b.ndim > 2) ? b.nx3 + 2 : 1) : 0;
-    const long nx1f = nphys(b, 0) ? 2L * ((b.ndim > 1) ? b.nx2 + 2 : 1) * ((b.ndim > 2) ? b.nx3 + 2 : 1) : 0;
-    const long n = npolar + nx1f;
-    if (n == 0) return hipSuccess;
-    if (prec == PREC_SPLIT) hipLaunchKernelGGL(fix_flux_kernel<float>, grid_for(n), dim3(BS), 0, s, b, flux_vars_cast<float>(F));
-    else hipLaunchKernelGGL(fix_flux_kernel<double>, grid_for(n), dim3(BS), 0, s, b, flux_vars_cast<double>(F));
-    return hipGetLastError();
+    return launch_fix_flux_pk(b, prec, F, s, nullptr);
 }
-hipError_t launch_fix_flux(const BlockDims& b, int prec, void* F, hipStream_t s) {
-    return launch_fix_flux(b, prec, flux_vars_prec(F, prec, b.nc), s);
+hipError_t launch_fix_flux(const BlockDims& b, int prec, void* F, hipStream_t s, const Pack* pk) {
+    return launch_fix_flux_pk(b, prec, flux_vars_prec(F, prec, b.nc), s, pk);
+}
+static hipError_t launch_bounds_pk(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const Vars<double>& U,
+                                   const Vars<double>& P, float* P32, hipStream_t s, const Pack* pk) {
+    if (!pk && !has_physical_face(b)) return hipSuccess;
+    const long n = pk ? pk->bounds_n : bounds_slab1(b) + bounds_slab2(b);
+    if (n == 0) return hipSuccess;
+    const dim3 g = pack_grid(grid_for(n), pk);
+    const BlockEntry* pd = pack_dev(pk);
+#define KH_BD(GEO, PK) hipLaunchKernelGGL((bounds_kernel<GEO, PK>), g, dim3(BS), 0, s, b, c, gd, U, P, P32, pd)
+    if (pk) { if (b.geo) KH_BD(1, true); else KH_BD(0, true); }
+    else { if (b.geo) KH_BD(1, false); else KH_BD(0, false); }
+#undef KH_BD
+    return hipGetLastError();
 }
 hipError_t launch_bounds(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const Vars<double>& U,
                          const Vars<double>& P, float* P32, hipStream_t s) {
-    if (!has_physical_face(b)) return hipSuccess;
-    const long n = bounds_slab1(b) + bounds_slab2(b);
-    if (b.geo) hipLaunchKernelGGL(bounds_kernel<1>, grid_for(n), dim3(BS), 0, s, b, c, gd, U, P, P32);
-    else hipLaunchKernelGGL(bounds_kernel<0>, grid_for(n), dim3(BS), 0, s, b, c, gd, U, P, P32);
-    return hipGetLastError();
+    return launch_bounds_pk(b, c, gd, U, P, P32, s, nullptr);
 }
 hipError_t launch_bounds(const BlockDims& b, const StepConsts& c, const GeomDev& gd, double* U, double* P, float* P32,
-                         hipStream_t s) {
-    return launch_bounds(b, c, gd, vars_of(U, b.nc), vars_of(P, b.nc), P32, s);
+                         hipStream_t s, const Pack* pk) {
+    return launch_bounds_pk(b, c, gd, vars_of(U, b.nc), vars_of(P, b.nc), P32, s, pk);
+}
+void pack_set_sizes(Pack& pk, const BlockDims* dims, int nblk) {
+    pk.fix_flux_n = pk.bounds_n = 0;
+    for (int n = 0; n < nblk; ++n) {
+        pk.fix_flux_n = std::max(pk.fix_flux_n, fix_flux_threads(dims[n]));
+        if (has_physical_face(dims[n])) pk.bounds_n = std::max(pk.bounds_n, bounds_slab1(dims[n]) + bounds_slab2(dims[n]));
+    }
 }
 hipError_t launch_finalize(const StepConsts& c, DeviceState* ds, const double* dt_part, int n_dt,
                            const double* divb_part, int n_divb, double* dt_hist, int hist_cap, hipStream_t s, int nds) {
@@ -1581,13 +1701,19 @@ hipError_t launch_finalize(const StepConsts& c, DeviceState* ds, const double* d
                        hist_cap);
     return hipGetLastError();
 }
-hipError_t launch_max_divb(const BlockDims& b, const Vars<const double>& U, double* part, hipStream_t s) {
+static hipError_t launch_max_divb_pk(const BlockDims& b, const Vars<const double>& U, double* part, hipStream_t s,
+                                     const Pack* pk) {
     if (b.ndim < 2) return hipSuccess;
-    hipLaunchKernelGGL(max_divb_kernel, dim3((unsigned)divb_blocks(b)), dim3(BS), 0, s, b, U, part);
+    const dim3 g = pack_grid(dim3((unsigned)divb_blocks(b)), pk);
+    if (pk) hipLaunchKernelGGL(max_divb_kernel<true>, g, dim3(BS), 0, s, b, U, part, pack_dev(pk));
+    else hipLaunchKernelGGL(max_divb_kernel<false>, g, dim3(BS), 0, s, b, U, part, nullptr);
     return hipGetLastError();
 }
-hipError_t launch_max_divb(const BlockDims& b, const double* U, double* part, hipStream_t s) {
-    return launch_max_divb(b, vars_of(U, b.nc), part, s);
+hipError_t launch_max_divb(const BlockDims& b, const Vars<const double>& U, double* part, hipStream_t s) {
+    return launch_max_divb_pk(b, U, part, s, nullptr);
+}
+hipError_t launch_max_divb(const BlockDims& b, const double* U, double* part, hipStream_t s, const Pack* pk) {
+    return launch_max_divb_pk(b, vars_of(U, b.nc), part, s, pk);
 }
 hipError_t launch_reduce(const double* part, int n, int is_max, double* out, hipStream_t s) {
     hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(BS), 0, s, part, n, is_max, out);

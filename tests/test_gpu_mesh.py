@@ -154,6 +154,30 @@ def test_mesh_graph_replay_equals_eager(case):
     assert a.dt_history(7) == g.dt_history(7) and a.scalars() == g.scalars()
 
 
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_mesh_packed_equals_per_block(case):
+    """kh_mesh_set_packed: the step launched once over all blocks (the default)
+    equals the step launched once per block, bit for bit, state, flags' effects,
+    timestep history and scalars."""
+    import kharma_amd as kh
+    mk, gen, nb, _ = CASES[case]
+    cfg = mk()
+    P0 = torch.from_numpy(gen(cfg)).cuda()
+    a = kh.Mesh(cfg, nb)
+    a.set_packed(False)
+    a.init_prims(P0)
+    p = kh.Mesh(cfg, nb)
+    p.init_prims(P0)
+    for n in range(3):
+        a.step(2)
+        p.step(2)
+        Pa, Ua = a.state()
+        Pp, Up = p.state()
+        torch.cuda.synchronize()
+        assert torch.equal(Pa, Pp) and torch.equal(Ua, Up), (case, n)
+    assert a.dt_history(6) == p.dt_history(6) and a.scalars() == p.scalars()
+
+
 def test_mesh_rejects_bad_layouts():
     import kharma_amd as kh
     cfg = pr.mhdmode_config(n=32, ndim=3, mode="fast")
