@@ -219,6 +219,9 @@ def main():
     ap.add_argument("--forms", default="fp64,split", help="comma list of fp64,split (profiling runs use one)")
     ap.add_argument("--fofc", action="store_true",
                     help="profiling mode (--forms fp64): run the step with the first-order flux correction on")
+    ap.add_argument("--mesh", default=None,
+                    help="profiling mode: run the torus as this block layout (e.g. 2x2x4) with the mesh "
+                         "driver (each kernel once over all blocks) and print its timings")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="after the runs, copy a buffer of this many MiB on the device: a known-size "
                          "reference for scaling HBM byte counters (FETCH_SIZE / WRITE_SIZE)")
@@ -255,6 +258,13 @@ def main():
     nz = cfg["nx"][0] * cfg["nx"][1] * cfg["nx"][2]
 
     forms = a.forms.split(",")
+    if a.mesh:
+        nb = tuple(int(x) for x in a.mesh.split("x"))
+        for f in forms:
+            ms = _bench_mesh(dict(cfg, precision=f), P0, nb, a.steps, a.warmup)
+            print(json.dumps({"form": f, "blocks": a.mesh, "ms_per_step": ms / a.steps,
+                              "zone_cycles_per_s": nz * a.steps / (ms * 1e-3)}))
+        return
     if forms != ["fp64", "split"]:
         # profiling mode: run the requested forms only, print their timings, no JSON contract line
         for f in forms:

@@ -210,6 +210,16 @@ int kh_mesh_create(const kh_params* p, int nb1, int nb2, int nb3, kh_mesh** out)
                 }
                 m->blk.push_back(k);
             }
+    // The curved-space tables depend on (x1, x2) only, so every column of blocks
+    // along x3 reads the tables of its first block (one copy per column in the
+    // caches instead of one per block).
+    if (m->blk[0]->dims.geo)
+        for (int b = nb1 * nb2; b < md.nblk; ++b) {
+            kh_block* k = m->blk[b];
+            for (void* q : k->geo_bufs) (void)hipFree(q);
+            k->geo_bufs.clear();
+            k->geo = m->blk[b % (nb1 * nb2)]->geo;
+        }
     const BlockDims& bd = m->blk[0]->dims;
     m->n_dt = update_blocks(bd);
     m->n_divb = divb_blocks(bd);

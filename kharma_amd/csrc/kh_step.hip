@@ -533,47 +533,60 @@ __device__ __forceinline__ void interior_index_jki(const BlockDims& b, unsigned 
     j = b.js + (int)jj;
 }
 
-// In 3-D the recover kernel runs on tiles of 64 x1 cells by 4 x3 rows at one
-// x2 row, one x3 row per wavefront: the four wavefronts of a work-group share
-// the geometry of their 64 (j, i) columns (the 22 metric components and the 2
-// geometric floors), which the work-group stages once in LDS as [component][64]
-// instead of each zone loading its own.  Tiles are numbered i fastest, then k,
-// then j, and renumbered per XCD as above.  Each zone's U and P loads are
-// issued before the staging so that they overlap it (272 -> 262 us on the
-// 192x128x128 torus).  (Staging the connection as well and adding the geometric
-// source here instead of in the combine shortens the combine -- 75-102 VGPRs
-// instead of 190-254 -- but slows the recovery by more: measured 223 + 469
-// against 371 + 262 us for the split stage-2 update.)
-constexpr int TILE_I = 64, TILE_K = BS / TILE_I;
+// In 3-D the recover kernel runs on tiles of TI = 64 x1 cells by 4 x3 rows at
+// one x2 row, one x3 row per wavefront: the four wavefronts of a work-group
+// share the geometry of their 64 (j, i) columns (the 22 metric components and
+// the 2 geometric floors), which the work-group stages once in LDS as
+// [component][64] instead of each zone loading its own.  A block whose x1 width
+// is an odd multiple of 32 (the 96-wide blocks of a 2x2x4 mesh of the
+// 192x128x128 torus) runs TI = 32 by 8 rows instead, two rows per wavefront, so
+// that no tile is half empty.  Tiles are numbered i fastest, then k, then j, and
+// renumbered per XCD as above.  Each zone's U and P loads are issued before the
+// staging so that they overlap it (272 -> 262 us on the 192x128x128 torus).
+// (Staging the connection as well and adding the geometric source here instead
+// of in the combine shortens the combine -- 75-102 VGPRs instead of 190-254 --
+// but slows the recovery by more: measured 223 + 469 against 371 + 262 us for
+// the split stage-2 update.)
 constexpr int STG_RECOVER = GEO_NCOMP + 2;
-__host__ __device__ __forceinline__ int tiles_i(const BlockDims& b) { return (b.nx1 + TILE_I - 1) / TILE_I; }
-__host__ __device__ __forceinline__ int tiles_k(const BlockDims& b) { return (b.nx3 + TILE_K - 1) / TILE_K; }
-static inline long update_tiles_host(const BlockDims& b) { return (long)tiles_i(b) * tiles_k(b) * b.nx2; }
+template <int TI> __host__ __device__ __forceinline__ int tiles_i(const BlockDims& b) { return (b.nx1 + TI - 1) / TI; }
+template <int TI> __host__ __device__ __forceinline__ int tiles_k(const BlockDims& b) {
+    return (b.nx3 + BS / TI - 1) / (BS / TI);
+}
+// The recovery's tile width for block b: 32 where it leaves fewer idle lanes.
+static inline int recover_tile_i(const BlockDims& b) {
+    return (tiles_i<32>(b) * 32 - b.nx1 < tiles_i<64>(b) * 64 - b.nx1) ? 32 : 64;
+}
+static inline long update_tiles_host(const BlockDims& b) {
+    return recover_tile_i(b) == 32 ? (long)tiles_i<32>(b) * tiles_k<32>(b) * b.nx2
+                                   : (long)tiles_i<64>(b) * tiles_k<64>(b) * b.nx2;
+}
 // Zone (k, j, i) of this thread; i0 is the tile's first x1 cell.
+template <int TI>
 __device__ __forceinline__ bool tile_zone(const BlockDims& b, int& k, int& j, int& i, int& i0) {
     const unsigned lb = xcd_logical_block(blockIdx.x, gridDim.x);
-    const unsigned nti = (unsigned)tiles_i(b), ntk = (unsigned)tiles_k(b);
+    const unsigned nti = (unsigned)tiles_i<TI>(b), ntk = (unsigned)tiles_k<TI>(b);
     const unsigned ti = lb % nti, rest = lb / nti;
     const unsigned tk = rest % ntk, jj = rest / ntk;
-    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / TILE_I));
-    i0 = b.is + (int)ti * TILE_I;
-    i = i0 + (int)(threadIdx.x % TILE_I);
-    k = b.ks + (int)tk * TILE_K + w;
+    const int row = (TI == 64) ? __builtin_amdgcn_readfirstlane((int)(threadIdx.x / TI)) : (int)(threadIdx.x / TI);
+    i0 = b.is + (int)ti * TI;
+    i = i0 + (int)(threadIdx.x % TI);
+    k = b.ks + (int)tk * (BS / TI) + row;
     j = b.js + (int)jj;
     return i <= b.ie && k <= b.ke;
 }
 // Stage the tile's metric and floors (x1 clamped to the table for a ragged
 // last tile) into sg; every thread of the work-group calls it.
+template <int TI>
 __device__ __forceinline__ void stage_recover_geometry(const GeomDev& gd, int j, int i0, double* sg) {
-    const int lane = (int)(threadIdx.x % TILE_I);
-    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / TILE_I));
+    const int lane = (int)(threadIdx.x % TI);
+    const int w = (TI == 64) ? __builtin_amdgcn_readfirstlane((int)(threadIdx.x / TI)) : (int)(threadIdx.x / TI);
     const long q = (long)j * gd.gn1 + min(i0 + lane, gd.gn1 - 1);
 #pragma unroll
-    for (int c0 = 0; c0 < STG_RECOVER; c0 += TILE_K) {
+    for (int c0 = 0; c0 < STG_RECOVER; c0 += BS / TI) {
         const int comp = c0 + w;
         if (comp < STG_RECOVER)
-            sg[comp * TILE_I + lane] = comp < GEO_NCOMP ? gd.g[GLOC_CENTER][(long)comp * gd.np + q]
-                                                        : gd.flr[(long)(comp - GEO_NCOMP) * gd.np + q];
+            sg[comp * TI + lane] = comp < GEO_NCOMP ? gd.g[GLOC_CENTER][(long)comp * gd.np + q]
+                                                    : gd.flr[(long)(comp - GEO_NCOMP) * gd.np + q];
     }
     __syncthreads();
 }
@@ -637,7 +650,7 @@ combine_kernel(BlockDims b, StepConsts cst, GeomDev gd, int stage, const R* __re
 // U->P with the stage's starting primitives as guess (kharma_driver.cpp:379-384),
 // B from U, floors in frame FRAME, P->U; U is read from and written back to Uout.
 // One zone (k, j, i) at c, whose U and guess P the caller has loaded.
-template <int FRAME, int INV, bool STG>
+template <int FRAME, int INV, bool STG, int TI>
 __device__ __forceinline__ void recover_zone(const BlockDims& b, const StepConsts& cst, const GeomDev& gd, int stage,
                                              double* Uout, double* __restrict__ Pout,
                                              float* __restrict__ Pout32, int* __restrict__ pflag,
@@ -648,19 +661,19 @@ __device__ __forceinline__ void recover_zone(const BlockDims& b, const StepConst
     const double gam = cst.gam;
     int pf = 0, ff = 0;
     double rf, uf;
-    const int lane = (int)(threadIdx.x % TILE_I);
+    const int lane = (int)(threadIdx.x % TI);
     if (Ustar) {   // the stage's U* for the differentiation form's record
 #pragma unroll
         for (int v = 0; v < NV; ++v) Ustar[v * b.nc + c] = U[v];
     }
     if constexpr (STG) {
-        rf = sg[GEO_NCOMP * TILE_I + lane];
-        uf = sg[(GEO_NCOMP + 1) * TILE_I + lane];
+        rf = sg[GEO_NCOMP * TI + lane];
+        uf = sg[(GEO_NCOMP + 1) * TI + lane];
     } else {
         geom_floors<1>(cst, gd, j, i, rf, uf);
     }
     {
-        CachedGeom<double> g(STG ? sg : gd.g[GLOC_CENTER], STG ? TILE_I : gd.np, STG ? lane : (long)j * gd.gn1 + i);
+        CachedGeom<double> g(STG ? sg : gd.g[GLOC_CENTER], STG ? TI : gd.np, STG ? lane : (long)j * gd.gn1 + i);
         pf = invert_zone<INV>(g, U, P, cst);
         const int ff0 = (pf / F_GEOM_RHO) * F_GEOM_RHO;
         pf = pf % F_GEOM_RHO;
@@ -689,13 +702,13 @@ __device__ __forceinline__ void recover_zone(const BlockDims& b, const StepConst
     }
 }
 
-template <int FRAME, int INV, bool STG, bool PK>
+template <int FRAME, int INV, bool STG, bool PK, int TI = 64>
 __global__ void __launch_bounds__(BS, KH_REC_OCC)
 recover_kernel(BlockDims b, StepConsts cst, GeomDev gd, int stage, const double* __restrict__ Psub, double* Uout,
                double* __restrict__ Pout, float* __restrict__ Pout32, int* __restrict__ pflag,
                int* __restrict__ fflag, int* __restrict__ fail_list, DeviceState* __restrict__ ds,
                double* __restrict__ Ustar, const BlockEntry* __restrict__ pk) {
-    __shared__ double sg[STG ? STG_RECOVER * TILE_I : 1];
+    __shared__ double sg[STG ? STG_RECOVER * TI : 1];
     if constexpr (PK) {
         const long blk = pack_select(pk, b, gd);
         const long sv = blk * NV * b.nc;
@@ -712,7 +725,7 @@ recover_kernel(BlockDims b, StepConsts cst, GeomDev gd, int stage, const double*
         // the zone's U and P are requested before the tile's geometry is staged,
         // so that their latency overlaps the staging and its barrier
         int i0;
-        const bool active = tile_zone(b, k, j, i, i0);
+        const bool active = tile_zone<TI>(b, k, j, i, i0);
         c = (long)k * b.s3 + (long)j * b.s2 + i;
         if (active) {
 #pragma unroll
@@ -720,7 +733,7 @@ recover_kernel(BlockDims b, StepConsts cst, GeomDev gd, int stage, const double*
 #pragma unroll
             for (int v = 0; v < NP; ++v) P[v] = Psub[v * b.nc + c];
         }
-        stage_recover_geometry(gd, j, i0, sg);
+        stage_recover_geometry<TI>(gd, j, i0, sg);
         if (!active) return;
     } else {
         const unsigned t = xcd_logical_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
@@ -732,7 +745,7 @@ recover_kernel(BlockDims b, StepConsts cst, GeomDev gd, int stage, const double*
 #pragma unroll
         for (int v = 0; v < NP; ++v) P[v] = Psub[v * b.nc + c];
     }
-    recover_zone<FRAME, INV, STG>(b, cst, gd, stage, Uout, Pout, Pout32, pflag, fflag, fail_list, ds, Ustar, sg, k, j,
+    recover_zone<FRAME, INV, STG, TI>(b, cst, gd, stage, Uout, Pout, Pout32, pflag, fflag, fail_list, ds, Ustar, sg, k, j,
                                   i, c, U, P);
 }
 
@@ -1505,6 +1518,7 @@ static void launch_update_geo_k(const BlockDims& b, const StepConsts& c, const G
                               double* Ustar, const Pack* pk) {
     const dim3 g = pack_grid(grid_for(ninterior_host(b)), pk);
     const bool rstg = b.ndim == 3;   // recover on LDS-staged tiles
+    const bool ti32 = recover_tile_i(b) == 32;
     const dim3 gr = rstg ? pack_grid(dim3((unsigned)update_tiles_host(b)), pk) : g;
     const BlockEntry* pd = pack_dev(pk);
     const bool same = (c.gam0[stage - 1] == 0.) && (Usub == Ufull);
@@ -1522,8 +1536,10 @@ static void launch_update_geo_k(const BlockDims& b, const StepConsts& c, const G
     if (!(parts & 2)) return;
 #define KH_REC(FR, INV)                                                                                             \
     do {                                                                                                            \
-        if (rstg) hipLaunchKernelGGL((recover_kernel<FR, INV, true, PK>), gr, dim3(BS), 0, s, b, c, gd, stage, Psub,        \
-                                     Uout, Pout, Pout32, pflag, fflag, fail_list, ds, Ustar, pd);                       \
+        if (rstg && ti32) hipLaunchKernelGGL((recover_kernel<FR, INV, true, PK, 32>), gr, dim3(BS), 0, s, b, c, gd,     \
+                                             stage, Psub, Uout, Pout, Pout32, pflag, fflag, fail_list, ds, Ustar, pd);  \
+        else if (rstg) hipLaunchKernelGGL((recover_kernel<FR, INV, true, PK>), gr, dim3(BS), 0, s, b, c, gd, stage, Psub,   \
+                                          Uout, Pout, Pout32, pflag, fflag, fail_list, ds, Ustar, pd);                  \
         else hipLaunchKernelGGL((recover_kernel<FR, INV, false, PK>), gr, dim3(BS), 0, s, b, c, gd, stage, Psub, Uout, Pout, \
                                 Pout32, pflag, fflag, fail_list, ds, Ustar, pd);                                        \
     } while (0)
