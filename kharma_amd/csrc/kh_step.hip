@@ -513,12 +513,16 @@ update_kernel(BlockDims b, StepConsts cst, GeomDev gd, int stage, const R* __res
 // renumbered so each XCD (workgroup w runs on XCD w % 8) owns one contiguous
 // range of j: the axisymmetric geometry tables ([comp][j][i]) a workgroup
 // reads then stay within one eighth of the table per XCD L2.
-#ifndef KH_CMB_OCC
-#define KH_CMB_OCC 1
-#endif
-#ifndef KH_REC_OCC
-#define KH_REC_OCC 1
-#endif
+// Waves per SIMD the two halves are built for (the second launch bound).  The
+// staged 3-D recovery on 64-wide tiles runs 4 (128 VGPRs with a 12-byte spill
+// instead of 140: 262 -> 234 us on the 192x128x128 torus); on 32-wide tiles
+// (28-byte spill) 4 waves measured 1 % slower, and the recovery in the
+// normal-observer frame (182-241 VGPRs) and the unstaged ones would spill
+// hundreds of bytes, so those keep their natural allocation.  The combine keeps
+// its 2 waves: a 3-wave bound (168 VGPRs, 20-90 byte spill) measured 1-7 % slower.
+template <int FRAME, bool STG, int TI> struct RecOcc {
+    static constexpr int w = (STG && FRAME != 1 && TI == 64) ? 4 : 1;
+};
 constexpr unsigned N_XCD = 8;
 __device__ __forceinline__ unsigned xcd_logical_block(unsigned b, unsigned nb) {
     const unsigned x = b % N_XCD, r = b / N_XCD, q = nb / N_XCD, m = nb % N_XCD;
@@ -594,7 +598,7 @@ __device__ __forceinline__ void stage_recover_geometry(const GeomDev& gd, int j,
 // ND: dimensionality (flux-CT for 2 and 3); SAME: stage with gam0 = 0 and
 // U_sub == U_full, which reads the state once (0*U + 1*U == U).
 template <typename R, bool DO_DT, int ND, bool SAME, bool PK>
-__global__ void __launch_bounds__(BS, KH_CMB_OCC)
+__global__ void __launch_bounds__(BS)
 combine_kernel(BlockDims b, StepConsts cst, GeomDev gd, int stage, const R* __restrict__ F, const R* __restrict__ ctop,
                const double* __restrict__ Usub, const double* Ufull, const double* __restrict__ Psub, double* Uout,
                const DeviceState* __restrict__ ds, double* __restrict__ dt_part, const BlockEntry* __restrict__ pk) {
@@ -703,7 +707,7 @@ __device__ __forceinline__ void recover_zone(const BlockDims& b, const StepConst
 }
 
 template <int FRAME, int INV, bool STG, bool PK, int TI = 64>
-__global__ void __launch_bounds__(BS, KH_REC_OCC)
+__global__ void __launch_bounds__(BS, (RecOcc<FRAME, STG, TI>::w))
 recover_kernel(BlockDims b, StepConsts cst, GeomDev gd, int stage, const double* __restrict__ Psub, double* Uout,
                double* __restrict__ Pout, float* __restrict__ Pout32, int* __restrict__ pflag,
                int* __restrict__ fflag, int* __restrict__ fail_list, DeviceState* __restrict__ ds,
