@@ -233,10 +233,15 @@ void build_geometry(const kh_params& p, const BlockDims& b, GeomHost& G) {
                     if (E.spherical && j < b.n2 && i < b.n1) {
                         double cn[4][4][4];
                         conn_native(E, X, cn);
+                        // Gamma^mu_{nu lam} is symmetric in (nu, lam); the centred
+                        // differences leave the two orders apart by rounding (up to
+                        // ~2e-11 relative), and the table holds their mean, exactly
+                        // symmetric, so the source kernel reads each pair once.
                         for (int mu = 0; mu < 4; ++mu)
                             for (int nu = 0; nu < 4; ++nu)
                                 for (int lam = 0; lam < 4; ++lam)
-                                    G.conn[(size_t)(mu * 16 + nu * 4 + lam) * np + q] = gdet * cn[mu][nu][lam];
+                                    G.conn[(size_t)(mu * 16 + nu * 4 + lam) * np + q] =
+                                        gdet * (0.5 * (cn[mu][nu][lam] + cn[mu][lam][nu]));
                     }
                     // Geometric floors, floors_functions.hpp:97-108 / 312-323 (radius of the cell centre)
                     const double r = embed_r(p, X);

@@ -127,10 +127,14 @@ __device__ __forceinline__ bool in_physical(const BlockDims& b, int k, int j, in
 }
 
 // Flux::AddGeoSource (flux/flux.cpp:395-444): the T^mu_nu Gamma contraction of
-// zone (j, i) added to the energy and momentum rates.  conn holds
-// sqrt(-g) Gamma^a_{b c} at index 16 a + 4 b + c.  All 64 terms are summed,
-// in the order mu, nu, lam, with no test for zero components: a branch per
-// component would make every load wait for the one before it.
+// zone (j, i) added to the energy and momentum rates,
+// S_lam = sum_{mu, nu} T^mu_nu sqrt(-g) Gamma^nu_{lam mu}.  conn holds
+// sqrt(-g) Gamma^a_{b c} at index 16 a + 4 b + c, exactly symmetric in (b, c)
+// (the geometry build stores the mean of the two finite-difference orders), so
+// each of the 40 distinct components is loaded once and serves both (lam, mu)
+// and (mu, lam): 40 loads instead of 64 (the reference sums all 64; the sum
+// here differs from it by rounding only).  No test for zero components: a
+// branch per component would make every load wait for the one before it.
 // conn(n) returns component n = 16 a + 4 b + c of this zone's table.
 template <class G, class C>
 __device__ __forceinline__ void add_geo_source_c(const G& g, const C& conn, const double Psub[NV], double gam,
@@ -142,18 +146,25 @@ __device__ __forceinline__ void add_geo_source_c(const G& g, const C& conn, cons
     const double eta = pgas + Psub[RHO] + Psub[UU] + bsq;
     const double ptot = pgas + 0.5 * bsq;
     double nd[4] = {0., 0., 0., 0.};
+    double T[4][4];   // T^mu_nu
 #pragma unroll
-    for (int mu = 0; mu < 4; ++mu) {
-        double T[4];
+    for (int mu = 0; mu < 4; ++mu)
 #pragma unroll
-        for (int nu = 0; nu < 4; ++nu) T[nu] = eta * D.ucon[mu] * D.ucov[nu] + (mu == nu ? ptot : 0.) - D.bcon[mu] * D.bcov[nu];
-        double cm[16];   // this mu's 16 components, loaded together
+        for (int nu = 0; nu < 4; ++nu) T[mu][nu] = eta * D.ucon[mu] * D.ucov[nu] + (mu == nu ? ptot : 0.) - D.bcon[mu] * D.bcov[nu];
 #pragma unroll
-        for (int nl = 0; nl < 16; ++nl) cm[nl] = conn(nl * 4 + mu);
+    for (int nu = 0; nu < 4; ++nu) {
+        double cs[10];
 #pragma unroll
-        for (int nu = 0; nu < 4; ++nu)
+        for (int lam = 0, n = 0; lam < 4; ++lam)
 #pragma unroll
-            for (int lam = 0; lam < 4; ++lam) nd[lam] += T[nu] * cm[nu * 4 + lam];
+            for (int mu = lam; mu < 4; ++mu, ++n) cs[n] = conn(nu * 16 + lam * 4 + mu);
+#pragma unroll
+        for (int lam = 0, n = 0; lam < 4; ++lam)
+#pragma unroll
+            for (int mu = lam; mu < 4; ++mu, ++n) {
+                nd[lam] += T[mu][nu] * cs[n];
+                if (mu != lam) nd[mu] += T[lam][nu] * cs[n];
+            }
     }
     dudt[UU] += nd[0];
     dudt[U1] += nd[1];

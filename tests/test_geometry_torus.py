@@ -101,6 +101,17 @@ def test_ks_inverse_and_lapse():
     assert np.allclose(alpha, 1 / np.sqrt(1 + 2 * r / rho2), rtol=1e-10)
 
 
+def test_connection_table_symmetric_in_lower_indices():
+    """The cached sqrt(-g) Gamma^a_{b c} is exactly symmetric in (b, c): the build
+    stores the mean of the two finite-difference orders, which differ by rounding
+    only (the source kernel reads each (b, c) pair once)."""
+    cfg = small_cfg(nx=(24, 16, 1))
+    conn = np.stack([kh.geometry_host(cfg, "center", 22 + n) for n in range(64)], axis=-1)
+    conn = conn.reshape(conn.shape[:2] + (4, 4, 4))
+    assert np.array_equal(conn, conn.transpose(0, 1, 2, 4, 3))
+    assert np.abs(conn).max() > 0
+
+
 def test_connection_metric_compatibility():
     # a block 1e-3 wide in x1 so that differences of the cached face metrics
     # give d_1 g to ~1e-9 relative
