@@ -70,6 +70,12 @@ int kh::check_params(const kh_params* p) {
     if (short_axis(p->nx1, p->periodic1) || short_axis(p->nx2, p->periodic2) || short_axis(p->nx3, p->periodic3))
         return fail(KH_ERR_INVALID, "each axis needs nx >= 2 ng zones (periodic) or nx >= ng (physical faces)");
     if (p->nx1 == 1) return fail(KH_ERR_INVALID, "nx1 must exceed 1");
+    {
+        // the kernels address one variable of a block with 32-bit byte offsets
+        auto ext = [&](int nx) { return nx > 1 ? (double)nx + 2. * p->ng + 1. : 2.; };
+        if (ext(p->nx1) * ext(p->nx2) * ext(p->nx3) * 8. >= 4294967296.)
+            return fail(KH_ERR_UNSUPPORTED, "a block's variable must stay below 4 GiB (split the mesh into blocks)");
+    }
     if (p->recon < 0 || p->recon > 3) return fail(KH_ERR_INVALID, "unknown reconstruction");
     if (p->precision != KH_PREC_FP64 && p->precision != KH_PREC_SPLIT) return fail(KH_ERR_INVALID, "unknown precision");
     if (p->integrator != 0 && p->integrator != 1) return fail(KH_ERR_INVALID, "unknown integrator");

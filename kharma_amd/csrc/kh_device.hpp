@@ -65,12 +65,27 @@ template <typename R> struct MinkowskiGeom {
     }
 };
 
+// Element e of the array at p through a 32-bit byte offset: a wave-uniform base
+// plus an unsigned offset is one base-register + offset memory instruction,
+// where a 64-bit index costs 64-bit address arithmetic per access.  The arrays
+// addressed this way (one variable of a block, one geometry table) are kept
+// below 4 GiB by the block-size check of the C ABI.
+template <class T> __device__ __forceinline__ T& at32(T* p, unsigned e) {
+    return *(T*)((char*)p + (size_t)(e * (unsigned)sizeof(T)));
+}
+template <class T> __device__ __forceinline__ const T& at32(const T* p, unsigned e) {
+    return *(const T*)((const char*)p + (size_t)(e * (unsigned)sizeof(T)));
+}
+
 // Metric of one point read from the geometry cache: 10 + 10 symmetric
 // components, sqrt(-g) and the lapse, stored component-major ([22][np]) so a
 // wavefront's loads of one component are contiguous.  Components are read
 // where they are used; the pointer is const and restrict, so the compiler is
 // free to keep or reload them.
-template <typename R> struct CachedGeom {
+// A32: components addressed through 32-bit offsets (at32), which the VALU-bound
+// flux kernels use; the update kernels keep 64-bit indices, with which their
+// register allocation measured better.
+template <typename R, bool A32 = false> struct CachedGeom {
     static constexpr bool flat = false;
     const R* __restrict__ p;
     long np, q;
@@ -78,11 +93,15 @@ template <typename R> struct CachedGeom {
     static __device__ __forceinline__ constexpr int sym(int a, int b) {
         return (a <= b) ? (a == 0 ? b : a == 1 ? 3 + b : a == 2 ? 5 + b : 9) : sym(b, a);
     }
-    __device__ __forceinline__ R gcov(int a, int b) const { return p[(long)sym(a, b) * np + q]; }
-    __device__ __forceinline__ R gcon(int a, int b) const { return p[(long)(10 + sym(a, b)) * np + q]; }
+    __device__ __forceinline__ R comp(int n) const {
+        if constexpr (A32) return at32(p, (unsigned)(n * np + q));
+        else return p[(long)n * np + q];
+    }
+    __device__ __forceinline__ R gcov(int a, int b) const { return comp(sym(a, b)); }
+    __device__ __forceinline__ R gcon(int a, int b) const { return comp(10 + sym(a, b)); }
     __device__ __forceinline__ R gcon0(int v) const { return gcon(0, v); }
-    __device__ __forceinline__ R gdet() const { return p[20 * np + q]; }
-    __device__ __forceinline__ R alpha() const { return p[21 * np + q]; }
+    __device__ __forceinline__ R gdet() const { return comp(20); }
+    __device__ __forceinline__ R alpha() const { return comp(21); }
     __device__ __forceinline__ void lower(const R v[4], R o[4]) const {
 #pragma unroll
         for (int m = 0; m < 4; ++m)

@@ -18,6 +18,15 @@
 
 namespace kh {
 
+// The FP32 kernels, which are VALU-bound, address their arrays through 32-bit
+// offsets (at32: 5-6 % fewer VALU instructions, 6-7 % faster X2/X3); the FP64
+// kernels keep 64-bit indices, which measured faster for them (X1 8 %).
+template <typename R> struct A32Of { static constexpr bool value = sizeof(R) == 4; };
+template <bool A32, class T> __device__ __forceinline__ T& at_idx(T* p, long e) {
+    if constexpr (A32) return at32(p, (unsigned)e);
+    else return p[e];
+}
+
 template <int RECON, typename R>
 __device__ __forceinline__ void cell_states(R x1, R x2, R x3, R x4, R x5, R& lout, R& rout) {
     if constexpr (RECON == RECON_WENO5) {
@@ -41,6 +50,7 @@ template <int DIR, int SPEEDS, typename R, class G>
 __device__ __forceinline__ void face_flux_g(const StepConsts& c, const BlockDims& b, const G& g, R rflr, R uflr,
                                             R Pl[NV], R Pr[NV], long ci,
                                             const Vars<R>& F, R* __restrict__ cmax_out, R* __restrict__ cmin_out) {
+    constexpr bool A32 = A32Of<R>::value;
     const R gam = R(c.gam);
     // get_flux.hpp:170-182: geometric floors of the face's cell centre on both states
     if (c.recon_floors) {
@@ -65,21 +75,21 @@ __device__ __forceinline__ void face_flux_g(const StepConsts& c, const BlockDims
     const R cmx = smax(smax(R(0), cmaxL), cmaxR);
     const R cmn = -smin(smin(R(0), cminL), cminR);
     if constexpr (SPEEDS == SPEEDS_BOTH) {
-        cmax_out[ci] = cmx;
-        cmin_out[ci] = cmn;
+        at_idx<A32>(cmax_out, ci) = cmx;
+        at_idx<A32>(cmin_out, ci) = cmn;
     } else if constexpr (SPEEDS == SPEEDS_CTOP) {
-        cmax_out[ci] = smax(cmx, cmn);
+        at_idx<A32>(cmax_out, ci) = smax(cmx, cmn);
     }
     if (c.use_hlle) {
         const R inv = rcp(cmx + cmn);
 #pragma unroll
         for (int v = 0; v < NV; ++v)
-            F[v][ci] = (cmx * Fl[v] + cmn * Fr[v] - cmx * cmn * (Ur[v] - Ul[v])) * inv;
+            at_idx<A32>(F[v], ci) = (cmx * Fl[v] + cmn * Fr[v] - cmx * cmn * (Ur[v] - Ul[v])) * inv;
     } else {
         const R ctop = smax(cmx, cmn);
 #pragma unroll
         for (int v = 0; v < NV; ++v)
-            F[v][ci] = R(0.5) * (Fl[v] + Fr[v] - ctop * (Ur[v] - Ul[v]));
+            at_idx<A32>(F[v], ci) = R(0.5) * (Fl[v] + Fr[v] - ctop * (Ur[v] - Ul[v]));
     }
 }
 
@@ -100,7 +110,7 @@ __device__ __forceinline__ void face_flux(const StepConsts& c, const BlockDims& 
     if constexpr (GEO) {
         const long q = (long)j * gd.gn1 + i;
         const int loc = (DIR == 1) ? GLOC_FACE1 : (DIR == 2) ? GLOC_FACE2 : GLOC_CENTER;
-        CachedGeom<R> g(geo_table<R>(gd, loc), gd.np, q);
+        CachedGeom<R, A32Of<R>::value> g(geo_table<R>(gd, loc), gd.np, q);
         face_flux_g<DIR, SPEEDS, R>(c, b, g, geo_floor<R>(gd, 0, q), geo_floor<R>(gd, 1, q), Pl, Pr, ci, F, cmax_out, cmin_out);
     } else {
         MinkowskiGeom<R> g;
@@ -113,14 +123,10 @@ __device__ __forceinline__ void face_flux(const StepConsts& c, const BlockDims& 
 // [255 b, 255 b + 256) and emits the faces of all but its first cell.
 // Waves per SIMD the FP32 flux kernels are built for (the second launch bound):
 // 6 for the X2/X3 kernels (68 VGPRs instead of 91: 11-14 % faster on the
-// 192x128x128 torus) and for the X1 kernel that also writes the signal speeds
-// (68 instead of 115: 4 % faster); the one-block X1 kernel without speeds
-// allocates 72 by itself, and a bound of 6 on it measured 3 % slower (its pack
-// form would take 115 unbounded).  The FP64 kernels keep their 140-142 VGPRs:
-// a 4-wave bound spills 28-36 bytes per lane and is slower.
-template <typename R, int SPEEDS, bool PK> struct X1Occ {
-    static constexpr int w = (sizeof(R) == 4 && (SPEEDS != SPEEDS_NONE || PK)) ? 6 : 1;
-};
+// 192x128x128 torus) and for the X1 kernels (67-68 instead of 90-115).  The
+// FP64 kernels keep their 140-142 VGPRs: a 4-wave bound spills 28-36 bytes per
+// lane and is slower.
+template <typename R, int SPEEDS, bool PK> struct X1Occ { static constexpr int w = sizeof(R) == 4 ? 6 : 1; };
 template <typename R> struct XtOcc { static constexpr int w = sizeof(R) == 4 ? 6 : 1; };
 template <int RECON, int SPEEDS, typename R, typename PT, int GEO, bool PK>
 __global__ void __launch_bounds__(256, (X1Occ<R, SPEEDS, PK>::w))
@@ -150,9 +156,16 @@ flux_x1_kernel(BlockDims b, StepConsts c, GeomDev gd, Vars<const PT> P, Vars<R> 
         ci = (long)(k0 + kk) * b.s3 + (long)(j0 + jj) * b.s2 + (b.is - 1 + ii);
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-            const PT* __restrict__ q = P[v] + ci;
             R ro;
-            cell_states<RECON, R>(R(q[-2]), R(q[-1]), R(q[0]), R(q[1]), R(q[2]), lo[v], ro);
+            if constexpr (A32Of<R>::value) {
+                const PT* __restrict__ q = P[v];
+                const unsigned cu = (unsigned)ci;
+                cell_states<RECON, R>(R(at32(q, cu - 2)), R(at32(q, cu - 1)), R(at32(q, cu)), R(at32(q, cu + 1)),
+                                      R(at32(q, cu + 2)), lo[v], ro);
+            } else {
+                const PT* __restrict__ q = P[v] + ci;
+                cell_states<RECON, R>(R(q[-2]), R(q[-1]), R(q[0]), R(q[1]), R(q[2]), lo[v], ro);
+            }
             sr[v][tid] = ro;
         }
     }
@@ -203,11 +216,19 @@ flux_xt_kernel(BlockDims b, StepConsts c, GeomDev gd, Vars<const PT> P, Vars<R> 
         const int k = (DIR == 2) ? outer : s;
         jf = j; if_ = i;
         ci = (long)k * b.s3 + (long)j * b.s2 + i;
+        constexpr bool A = A32Of<R>::value;
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-            const PT* __restrict__ q = P[v] + ci;
             R ro;
-            cell_states<RECON, R>(R(q[-2 * st]), R(q[-st]), R(q[0]), R(q[st]), R(q[2 * st]), lo[v], ro);
+            if constexpr (A) {
+                const PT* __restrict__ q = P[v];
+                cell_states<RECON, R>(R(at32(q, unsigned(ci - 2 * st))), R(at32(q, unsigned(ci - st))),
+                                      R(at32(q, unsigned(ci))), R(at32(q, unsigned(ci + st))),
+                                      R(at32(q, unsigned(ci + 2 * st))), lo[v], ro);
+            } else {
+                const PT* __restrict__ q = P[v] + ci;
+                cell_states<RECON, R>(R(q[-2 * st]), R(q[-st]), R(q[0]), R(q[st]), R(q[2 * st]), lo[v], ro);
+            }
             sr[v][ty][tx] = ro;
         }
     }
