@@ -54,6 +54,7 @@ def _create(cfg, **over):
     (dict(fofc=1, driver_type=1), 3),                          # FOFC belongs to the default driver
     (dict(fofc=1, driver_type=0, precision=1), 3),             # FOFC in the FP64 form only
     (dict(fofc=1, driver_type=0, fofc_polar_cells=-1), 1),
+    (dict(nx1=1024, nx2=1024, nx3=512), 3),                    # a variable of the block would pass 4 GiB
 ])
 def test_invalid_params_rejected_on_host(L, over, code):
     st, msg = _create(pr.mhdmode_config(n=16, ndim=3), **over)
