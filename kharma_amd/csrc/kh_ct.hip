@@ -52,12 +52,12 @@ template <typename R, int SCHEME, int ND>
 __global__ void __launch_bounds__(CBS) ct_emf_kernel(BlockDims b, FluxVars<const R> F, const double* __restrict__ cemf,
                                                       double* __restrict__ emf) {
     const int ni = b.nx1 + 1, nj = b.nx2 + 1;
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long ne = (long)ni * nj * (ND > 2 ? b.nx3 + 1 : 1);
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;   // edge / face counts stay below 2^31
+    const unsigned ne = (unsigned)ni * (unsigned)nj * (unsigned)(ND > 2 ? b.nx3 + 1 : 1);
     if (t >= ne) return;
-    const int i = b.is + (int)(t % ni);
-    const int j = b.js + (int)((t / ni) % nj);
-    const int k = b.ks + (int)(t / ((long)ni * nj));
+    const int i = b.is + (int)(t % (unsigned)ni);
+    const int j = b.js + (int)((t / (unsigned)ni) % (unsigned)nj);
+    const int k = b.ks + (int)(t / ((unsigned)ni * (unsigned)nj));
     const long s2 = b.s2, s3 = b.s3, nc = b.nc;
     const long c = (long)k * s3 + (long)j * s2 + i;
     auto f = [&](int dir, int v, long cc) -> double { return (double)F[dir - 1][v][cc]; };
@@ -115,12 +115,12 @@ __global__ void __launch_bounds__(CBS)
 ct_face_kernel(BlockDims b, const double* __restrict__ emf, const double* fBsub, const double* fBfull, double g0,
                double g1, const DeviceState* __restrict__ ds, double beta, double* fBout) {
     const int ni = b.nx1 + 1, nj = b.nx2 + 1;
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long nface = (long)ni * nj * (ND > 2 ? b.nx3 + 1 : 1);
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;   // edge / face counts stay below 2^31
+    const unsigned nface = (unsigned)ni * (unsigned)nj * (unsigned)(ND > 2 ? b.nx3 + 1 : 1);
     if (t >= nface) return;
-    const int i = b.is + (int)(t % ni);
-    const int j = b.js + (int)((t / ni) % nj);
-    const int k = b.ks + (int)(t / ((long)ni * nj));
+    const int i = b.is + (int)(t % (unsigned)ni);
+    const int j = b.js + (int)((t / (unsigned)ni) % (unsigned)nj);
+    const int k = b.ks + (int)(t / ((unsigned)ni * (unsigned)nj));
     const long f = (long)k * b.fs3 + (long)j * b.fs2 + i;
     const long fs2 = b.fs2, fs3 = b.fs3, nf = b.nf;
     const double dx1 = b.dx1, dx2 = b.dx2, dx3 = b.dx3;
@@ -173,10 +173,11 @@ __device__ __forceinline__ int ct_wrap(int x, int lo, int hi, int N) {
     return (x < lo) ? x + N : (x > hi) ? x - N : x;
 }
 __global__ void __launch_bounds__(CBS) ct_sync_kernel(BlockDims b, double* fB) {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;   // face positions stay below 2^31
     const int f = blockIdx.y;
-    if (t >= b.nf) return;
-    const int i = (int)(t % b.fs2), j = (int)((t / b.fs2) % (b.n2 + 1)), k = (int)(t / b.fs3);
+    if (t >= (unsigned)b.nf) return;
+    const int i = (int)(t % (unsigned)b.fs2), j = (int)((t / (unsigned)b.fs2) % (unsigned)(b.n2 + 1)),
+              k = (int)(t / (unsigned)b.fs3);
     const int x[3] = {i, j, k};
     const int n[3] = {b.n1, b.n2, b.n3};
     const int lo[3] = {b.is, b.js, b.ks}, hi[3] = {b.ie, b.je, b.ke};

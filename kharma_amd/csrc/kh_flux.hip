@@ -143,15 +143,17 @@ flux_x1_kernel(BlockDims b, StepConsts c, GeomDev gd, Vars<const PT> P, Vars<R> 
     }
     const int L = b.nx1 + 2;
     const int tid = threadIdx.x;
-    const long g = (long)blockIdx.x * 255 + tid;
-    const bool valid = g < (long)L * nj * nk;
+    // cell counts stay below 2^31 (the C ABI bounds a block's size), so the
+    // flattened index is divided in 32 bits
+    const unsigned g = blockIdx.x * 255u + (unsigned)tid;
+    const bool valid = g < (unsigned)L * (unsigned)nj * (unsigned)nk;
     int ii = 0, jf = 0;
     long ci = 0;
     R lo[NV];
     if (valid) {
-        const int row = (int)(g / L);
-        ii = (int)(g - (long)row * L);
-        const int jj = row % nj, kk = row / nj;
+        const int row = (int)(g / (unsigned)L);
+        ii = (int)(g - (unsigned)row * (unsigned)L);
+        const int jj = (int)((unsigned)row % (unsigned)nj), kk = (int)((unsigned)row / (unsigned)nj);
         jf = j0 + jj;
         ci = (long)(k0 + kk) * b.s3 + (long)(j0 + jj) * b.s2 + (b.is - 1 + ii);
 #pragma unroll
@@ -349,12 +351,12 @@ fofc_flux_kernel(BlockDims b, StepConsts c, GeomDev gd, Vars<const double> P, co
     const int m1 = (DIR == 1) ? b.nx1 + 1 : b.nx1 + 2;
     const int m2 = (b.ndim < 2) ? 1 : (DIR == 2) ? b.nx2 + 1 : b.nx2 + 2;
     const int m3 = (b.ndim < 3) ? 1 : (DIR == 3) ? b.nx3 + 1 : b.nx3 + 2;
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long)m1 * m2 * m3) return;
-    const int i = ((DIR == 1) ? b.is : b.is - 1) + (int)(t % m1);
-    const long r = t / m1;
-    const int j = ((b.ndim < 2 || DIR == 2) ? b.js : b.js - 1) + (int)(r % m2);
-    const int k = ((b.ndim < 3 || DIR == 3) ? b.ks : b.ks - 1) + (int)(r / m2);
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;   // face counts stay below 2^31
+    if (t >= (unsigned)m1 * (unsigned)m2 * (unsigned)m3) return;
+    const int i = ((DIR == 1) ? b.is : b.is - 1) + (int)(t % (unsigned)m1);
+    const unsigned r = t / (unsigned)m1;
+    const int j = ((b.ndim < 2 || DIR == 2) ? b.js : b.js - 1) + (int)(r % (unsigned)m2);
+    const int k = ((b.ndim < 3 || DIR == 3) ? b.ks : b.ks - 1) + (int)(r / (unsigned)m2);
     auto marked = [&](int kk, int jj, int ii) {
         ii = fofc_source(ii, b.is, b.ie, b.nx1, b.periodic[0]);
         if (b.ndim > 1) jj = fofc_source(jj, b.js, b.je, b.nx2, b.periodic[1]);

@@ -1069,23 +1069,24 @@ __global__ void __launch_bounds__(BS) bounds_kernel(BlockDims b, StepConsts cst,
         const long sv = blk * NV * b.nc;
         U = pack_at(U, sv); P = pack_at(P, sv); P32 = pack_at(P32, sv);
     }
-    long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long n1s = bounds_slab1(b);
+    // slab sizes stay below 2^31 (the C ABI bounds a block's size): 32-bit division
+    unsigned r = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned n1s = (unsigned)bounds_slab1(b);
     int k, j, i;
     if (r < n1s) {
-        const int m = b.ng * nphys(b, 0);
+        const unsigned m = (unsigned)(b.ng * nphys(b, 0));
         const int g = (int)(r % m);
         r /= m;
-        j = (int)(r % b.n2);
-        k = (int)(r / b.n2);
+        j = (int)(r % (unsigned)b.n2);
+        k = (int)(r / (unsigned)b.n2);
         i = (!b.img[0][0] && g < b.ng) ? g : b.ie + 1 + (g - (b.img[0][0] ? 0 : b.ng));
     } else {
         r -= n1s;
-        if (r >= bounds_slab2(b)) return;
-        const int w = bounds_slab2_width(b);
+        if (r >= (unsigned)bounds_slab2(b)) return;
+        const unsigned w = (unsigned)bounds_slab2_width(b);
         const int ii = (int)(r % w);
         r /= w;
-        const int m = b.ng * nphys(b, 1);
+        const unsigned m = (unsigned)(b.ng * nphys(b, 1));
         const int g = (int)(r % m);
         k = (int)(r / m);
         j = (!b.img[1][0] && g < b.ng) ? g : b.je + 1 + (g - (b.img[1][0] ? 0 : b.ng));
