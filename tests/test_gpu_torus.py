@@ -55,10 +55,14 @@ CASES = {
                                                    rho_min_geom=1e-8, u_min_geom=1e-10)),
     # every zone marked (a horizon buffer past the outer radius): the donor-cell scheme
     "torus2d_fofc_all": lambda: dict(pr.torus_config(64, 48, 1), fofc=True, fofc_eh_buffer=1e6),
+    # 3-D: the marks are taken on the recovery's LDS-staged tiles
+    "torus3d_fofc": lambda: dict(pr.torus_config(48, 32, 16), fofc=True, fofc_polar_cells=2,
+                                 floor_values=dict(pr.torus_config(48, 32, 16)["floor_values"],
+                                                   rho_min_geom=1e-8, u_min_geom=1e-10)),
 }
 
 # cases whose initial state comes from another configuration
-PRIMS_FROM = {"torus2d_fofc": lambda: pr.torus_config(64, 48, 1)}
+PRIMS_FROM = {"torus2d_fofc": lambda: pr.torus_config(64, 48, 1), "torus3d_fofc": lambda: pr.torus_config(48, 32, 16)}
 
 
 def _oracle(oracle_lib, cfg, variant=""):
@@ -204,7 +208,7 @@ def test_torus_utop_floors_ptou_bounds(oracle_lib, case):
 
 
 @pytest.mark.parametrize("case", ["torus2d", "torus3d", "torus2d_simple", "torus2d_fluid", "torus2d_normal",
-                                  "torus2d_kastaun", "torus2d_fofc", "torus2d_fofc_all"])
+                                  "torus2d_kastaun", "torus2d_fofc", "torus2d_fofc_all", "torus3d_fofc"])
 def test_torus_step_parity_fp64_per_step(oracle_lib, case):
     import kharma_amd as kh
     cfg = CASES[case]()
