@@ -455,8 +455,10 @@ int kh_driver_kernel_times(kh_driver* d, double* total_ms, int* launches, int nk
  * neighbours' ghosts, so no exchange pass runs.  The mesh state reads and
  * writes as one block of the whole mesh ([8][N3+2ng][N2+2ng][N1+2ng]).  The
  * same arithmetic per zone as the one-block driver: a mesh of several blocks
- * gives the one-block result bit for bit.  The first-order flux correction
- * runs on one-block meshes (kh_driver_*) only. */
+ * gives the one-block result bit for bit.  With fofc on, each block's marks
+ * are stored with their images in its neighbours' ghosts (the flag's boundary
+ * sync, kharma_driver.cpp:328-331) before any block replaces its fluxes, and
+ * the polar rows are marked on the blocks of the physical X2 faces only. */
 typedef struct kh_mesh kh_mesh;
 int kh_mesh_create(const kh_params* p, int nb1, int nb2, int nb3, kh_mesh** out);
 void kh_mesh_destroy(kh_mesh* m);

@@ -245,8 +245,10 @@ hipError_t upload_fofc_fixed(kh_block* blk) {
             double X[4];
             locus_coords(pp, bd, GLOC_CENTER, j, i, X);
             bool m = embed_r(pp, X) < r_eh + pp.fofc_eh_buffer;
-            if (bd.ndim > 1 && !bd.periodic[1] && pp.fofc_polar_cells > 0 &&
-                ((j >= bd.js && j < bd.js + pp.fofc_polar_cells) || (j <= bd.je && j > bd.je - pp.fofc_polar_cells)))
+            // the polar rows of a block's physical X2 faces only (fofc.cpp:82-85)
+            const int pc = pp.fofc_polar_cells;
+            if (bd.ndim > 1 && pc > 0 &&
+                ((!bd.img[1][0] && j >= bd.js && j < bd.js + pc) || (!bd.img[1][1] && j <= bd.je && j > bd.je - pc)))
                 m = true;
             fixed[(size_t)j * gn1 + i] = m;
         }
@@ -370,6 +372,7 @@ int kh::block_create_in_mesh(const kh_params* mesh, const int nb[3], const int b
     hipError_t e = hipMalloc(&b->scratch, sizeof(DeviceState));
     if (e == hipSuccess) e = hipMalloc(&b->partials, sizeof(double) * b->npartials);
     if (e == hipSuccess && bd.geo) e = upload_geometry(b);
+    if (e == hipSuccess) e = upload_fofc_fixed(b);
     if (e != hipSuccess) {
         kh_block_destroy(b);
         return fail(KH_ERR_HIP, std::string("device allocation / copy: ") + hipGetErrorString(e));

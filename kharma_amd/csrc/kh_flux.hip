@@ -347,7 +347,7 @@ __device__ __forceinline__ int fofc_source(int v, int s, int e, int n, int perio
 template <int DIR, int SPEEDS, int GEO>
 __global__ void __launch_bounds__(256)
 fofc_flux_kernel(BlockDims b, StepConsts c, GeomDev gd, Vars<const double> P, const int* __restrict__ mark,
-                 Vars<double> F, double* __restrict__ cmax_out, double* __restrict__ cmin_out) {
+                 Vars<double> F, double* __restrict__ cmax_out, double* __restrict__ cmin_out, bool ghost_marks) {
     const int m1 = (DIR == 1) ? b.nx1 + 1 : b.nx1 + 2;
     const int m2 = (b.ndim < 2) ? 1 : (DIR == 2) ? b.nx2 + 1 : b.nx2 + 2;
     const int m3 = (b.ndim < 3) ? 1 : (DIR == 3) ? b.nx3 + 1 : b.nx3 + 2;
@@ -357,10 +357,15 @@ fofc_flux_kernel(BlockDims b, StepConsts c, GeomDev gd, Vars<const double> P, co
     const unsigned r = t / (unsigned)m1;
     const int j = ((b.ndim < 2 || DIR == 2) ? b.js : b.js - 1) + (int)(r % (unsigned)m2);
     const int k = ((b.ndim < 3 || DIR == 3) ? b.ks : b.ks - 1) + (int)(r / (unsigned)m2);
+    // a ghost of a shared face keeps its own (synced) mark when ghost_marks is set
+    auto src = [&](int v, int s, int e, int n, int d) {
+        if (ghost_marks && ((v < s && b.img[d][0]) || (v > e && b.img[d][1]))) return v;
+        return fofc_source(v, s, e, n, b.periodic[d]);
+    };
     auto marked = [&](int kk, int jj, int ii) {
-        ii = fofc_source(ii, b.is, b.ie, b.nx1, b.periodic[0]);
-        if (b.ndim > 1) jj = fofc_source(jj, b.js, b.je, b.nx2, b.periodic[1]);
-        if (b.ndim > 2) kk = fofc_source(kk, b.ks, b.ke, b.nx3, b.periodic[2]);
+        ii = src(ii, b.is, b.ie, b.nx1, 0);
+        if (b.ndim > 1) jj = src(jj, b.js, b.je, b.nx2, 1);
+        if (b.ndim > 2) kk = src(kk, b.ks, b.ke, b.nx3, 2);
         return mark[(long)kk * b.s3 + (long)jj * b.s2 + ii] != 0;
     };
     if (!marked(k, j, i) && !marked(DIR == 3 ? k - 1 : k, DIR == 2 ? j - 1 : j, DIR == 1 ? i - 1 : i)) return;
@@ -377,28 +382,29 @@ fofc_flux_kernel(BlockDims b, StepConsts c, GeomDev gd, Vars<const double> P, co
 
 template <int DIR, int SPEEDS>
 static hipError_t launch_fofc_dir(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const Vars<const double>& P,
-                                  const int* mark, const Vars<double>& F, double* cmax, double* cmin, hipStream_t s) {
+                                  const int* mark, const Vars<double>& F, double* cmax, double* cmin, hipStream_t s,
+                                  bool gm) {
     const long m1 = (DIR == 1) ? b.nx1 + 1 : b.nx1 + 2;
     const long m2 = (b.ndim < 2) ? 1 : (DIR == 2) ? b.nx2 + 1 : b.nx2 + 2;
     const long m3 = (b.ndim < 3) ? 1 : (DIR == 3) ? b.nx3 + 1 : b.nx3 + 2;
     const dim3 grid((unsigned)((m1 * m2 * m3 + 255) / 256));
-    if (b.geo) fofc_flux_kernel<DIR, SPEEDS, 1><<<grid, 256, 0, s>>>(b, c, gd, P, mark, F, cmax, cmin);
-    else fofc_flux_kernel<DIR, SPEEDS, 0><<<grid, 256, 0, s>>>(b, c, gd, P, mark, F, cmax, cmin);
+    if (b.geo) fofc_flux_kernel<DIR, SPEEDS, 1><<<grid, 256, 0, s>>>(b, c, gd, P, mark, F, cmax, cmin, gm);
+    else fofc_flux_kernel<DIR, SPEEDS, 0><<<grid, 256, 0, s>>>(b, c, gd, P, mark, F, cmax, cmin, gm);
     return hipGetLastError();
 }
 
 template <int SPEEDS>
 static hipError_t launch_fofc_speeds(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int dir,
                                      const Vars<const double>& P, const int* mark, const Vars<double>& F, double* cmax,
-                                     double* cmin, hipStream_t s) {
-    if (dir == 1) return launch_fofc_dir<1, SPEEDS>(b, c, gd, P, mark, F, cmax, cmin, s);
-    if (dir == 2) return launch_fofc_dir<2, SPEEDS>(b, c, gd, P, mark, F, cmax, cmin, s);
-    return launch_fofc_dir<3, SPEEDS>(b, c, gd, P, mark, F, cmax, cmin, s);
+                                     double* cmin, hipStream_t s, bool gm) {
+    if (dir == 1) return launch_fofc_dir<1, SPEEDS>(b, c, gd, P, mark, F, cmax, cmin, s, gm);
+    if (dir == 2) return launch_fofc_dir<2, SPEEDS>(b, c, gd, P, mark, F, cmax, cmin, s, gm);
+    return launch_fofc_dir<3, SPEEDS>(b, c, gd, P, mark, F, cmax, cmin, s, gm);
 }
 
 hipError_t launch_fofc_flux(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int speeds,
                             const Vars<const double>& P, const int* mark, const FluxVars<double>& F, double* cmax,
-                            double* cmin, hipStream_t s) {
+                            double* cmin, hipStream_t s, bool ghost_marks) {
     StepConsts cl = c;   // first-order states: no reconstruction floors; LLF whatever flux/type says
     cl.recon_floors = 0;
     cl.use_hlle = 0;
@@ -406,9 +412,9 @@ hipError_t launch_fofc_flux(const BlockDims& b, const StepConsts& c, const GeomD
         double* cx = cmax ? cmax + (long)(dir - 1) * b.nc : nullptr;
         double* cn = cmin ? cmin + (long)(dir - 1) * b.nc : nullptr;
         hipError_t e;
-        if (speeds == SPEEDS_CTOP) e = launch_fofc_speeds<SPEEDS_CTOP>(b, cl, gd, dir, P, mark, F[dir - 1], cx, cn, s);
-        else if (speeds == SPEEDS_BOTH) e = launch_fofc_speeds<SPEEDS_BOTH>(b, cl, gd, dir, P, mark, F[dir - 1], cx, cn, s);
-        else e = launch_fofc_speeds<SPEEDS_NONE>(b, cl, gd, dir, P, mark, F[dir - 1], cx, cn, s);
+        if (speeds == SPEEDS_CTOP) e = launch_fofc_speeds<SPEEDS_CTOP>(b, cl, gd, dir, P, mark, F[dir - 1], cx, cn, s, ghost_marks);
+        else if (speeds == SPEEDS_BOTH) e = launch_fofc_speeds<SPEEDS_BOTH>(b, cl, gd, dir, P, mark, F[dir - 1], cx, cn, s, ghost_marks);
+        else e = launch_fofc_speeds<SPEEDS_NONE>(b, cl, gd, dir, P, mark, F[dir - 1], cx, cn, s, ghost_marks);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;

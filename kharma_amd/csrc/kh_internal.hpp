@@ -235,15 +235,19 @@ hipError_t launch_geo_source(const BlockDims& b, const StepConsts& c, const Geom
 // ghost images).  ctop: [3][nc] max signal speeds, read when do_dt (the last
 // stage), which then writes one CFL minimum per block into dt_part.
 // First-order flux correction (kh_step.hip / kh_flux.hip).  mark[nc]: interior
-// marks; fixed[np] (or null): zones marked whatever the trial state.
+// marks; fixed[np] (or null): zones marked whatever the trial state.  images:
+// each mark is also stored at its images (the ghosts of shared faces, in the
+// neighbouring blocks' mark arrays at a stride of nc): the synced flag of a mesh.
 hipError_t launch_fofc_mark(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int stage, const double* F,
                             const double* Usub, const double* Ufull, const double* Psub, const int* fixed,
-                            const DeviceState* ds, int* mark, hipStream_t s);
+                            const DeviceState* ds, int* mark, hipStream_t s, bool images = false);
 // speeds: SPEEDS_NONE, SPEEDS_CTOP (max(cmax, cmin) into cmax[dir-1]) or SPEEDS_BOTH
-// (cmax/cmin [3][nc]).
+// (cmax/cmin [3][nc]).  ghost_marks: the ghosts of shared faces hold their
+// marks (launch_fofc_mark with images); otherwise they are read at their
+// periodic image.
 hipError_t launch_fofc_flux(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int speeds,
                             const Vars<const double>& P, const int* mark, const FluxVars<double>& F, double* cmax,
-                            double* cmin, hipStream_t s);
+                            double* cmin, hipStream_t s, bool ghost_marks = false);
 // Floors::DetermineGRMHDFloors over the entire block (fflag |= floor bits).
 hipError_t launch_determine_floors(const BlockDims& b, const StepConsts& c, const GeomDev& gd, const Vars<const double>& P,
                                    int* fflag, hipStream_t s);

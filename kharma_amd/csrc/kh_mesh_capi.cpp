@@ -37,6 +37,7 @@ struct kh_mesh {
     float *P0f = nullptr, *P1f = nullptr;
     void *F = nullptr, *ctop = nullptr;
     int *pflag = nullptr, *fflag = nullptr, *fail_list = nullptr;
+    int* fofc_mark = nullptr;               // fofc/on: marks of all blocks, ghosts of shared faces synced
     DeviceState* ds = nullptr;              // one per block; the first carries the clock
     double *dt_part = nullptr, *divb_part = nullptr, *dt_hist = nullptr;
     long n_dt = 0, n_divb = 0;              // partials per block
@@ -78,6 +79,25 @@ int mesh_stage(kh_mesh* m, int stage, hipStream_t s) {
     auto ct = [&](int b) { return (char*)m->ctop + fe * 3 * nc * b; };
     auto f32 = [&](float* a, int b) { return a ? a + st * b : nullptr; };
     const StepConsts& c = m->blk[0]->consts;
+    // KHARMADriver::AddFOFC (kharma_driver.cpp:291-337) after every block's raw
+    // fluxes: the marks of all blocks, stored with their images in the
+    // neighbours' ghosts (the flag's boundary sync), then the replaced fluxes.
+    auto fofc = [&]() -> int {
+        if (!m->p.fofc) return KH_OK;
+        for (int b = 0; b < md.nblk; ++b) {
+            const kh_block* k = m->blk[b];
+            KM_HIP(launch_fofc_mark(k->dims, k->consts, k->geo, stage, (const double*)F(b), Usub + st * b,
+                                    m->U0 + st * b, Psub + st * b, k->fofc_fixed, m->ds + b, m->fofc_mark + nc * b,
+                                    s, true));
+        }
+        for (int b = 0; b < md.nblk; ++b) {
+            const kh_block* k = m->blk[b];
+            KM_HIP(launch_fofc_flux(k->dims, k->consts, k->geo, speeds, vars_of((const double*)(Psub + st * b), nc),
+                                    m->fofc_mark + nc * b, flux_vars_of((double*)F(b), nc), (double*)ct(b), nullptr,
+                                    s, true));
+        }
+        return KH_OK;
+    };
     if (m->packed) {
         // every kernel once over all blocks, with block 0's arguments
         const kh_block* k = m->blk[0];
@@ -85,6 +105,7 @@ int mesh_stage(kh_mesh* m, int stage, hipStream_t s) {
         for (int dir = 1; dir <= k->dims.ndim; ++dir)
             KM_HIP(launch_flux_stage(k->dims, k->consts, k->geo, dir, k->p.recon, prec, speeds, Psub, Psubf,
                                      F(0) + fe * (dir - 1) * st, ct(0) + fe * (dir - 1) * nc, s, pk));
+        if (const int r = fofc(); r != KH_OK) return r;
         if (pk->fix_flux_n) KM_HIP(launch_fix_flux(k->dims, prec, F(0), s, pk));
         KM_HIP(launch_update(k->dims, k->consts, k->geo, prec, stage, stage == 2, F(0), ct(0), Usub, m->U0, Psub, Uout,
                              Pout, Poutf, m->pflag, m->fflag, m->fail_list, m->ds, m->dt_part, s, pk));
@@ -100,6 +121,7 @@ int mesh_stage(kh_mesh* m, int stage, hipStream_t s) {
             KM_HIP(launch_flux_stage(k->dims, k->consts, k->geo, dir, k->p.recon, prec, speeds, Psub + st * b,
                                      f32(Psubf, b), F(b) + fe * (dir - 1) * st, ct(b) + fe * (dir - 1) * nc, s));
     }
+    if (const int r = fofc(); r != KH_OK) return r;
     for (int b = 0; b < md.nblk; ++b) {
         const kh_block* k = m->blk[b];
         if (has_physical_face(k->dims)) KM_HIP(launch_fix_flux(k->dims, prec, F(b), s));
@@ -171,8 +193,6 @@ int kh_mesh_create(const kh_params* p, int nb1, int nb2, int nb3, kh_mesh** out)
         if (nb[d] > 1 && N[d] / nb[d] < 2 * p->ng)
             return api_fail(KH_ERR_INVALID, "a block shared along an axis needs nx >= 2 ng zones along it");
     }
-    if (p->fofc && nb1 * nb2 * nb3 > 1)
-        return api_fail(KH_ERR_UNSUPPORTED, "fofc runs on one-block meshes only");
     if (p->b_solver == KH_BSOLVER_FACE_CT)
         return api_fail(KH_ERR_UNSUPPORTED, "face_ct runs on one-block meshes (kh_driver_*) only");
     kh_mesh* m = new kh_mesh();
@@ -244,6 +264,7 @@ int kh_mesh_create(const kh_params* p, int nb1, int nb2, int nb3, kh_mesh** out)
     alloc((void**)&m->pflag, sizeof(int) * nc * nblk);
     alloc((void**)&m->fflag, sizeof(int) * nc * nblk);
     alloc((void**)&m->fail_list, sizeof(int) * nc * nblk);
+    if (p->fofc) alloc((void**)&m->fofc_mark, sizeof(int) * nc * nblk);
     alloc((void**)&m->ds, sizeof(DeviceState) * nblk);
     alloc((void**)&m->dt_part, sizeof(double) * (m->n_dt * nblk + 1));
     alloc((void**)&m->divb_part, sizeof(double) * (m->n_divb * nblk + 1));
@@ -271,7 +292,7 @@ void kh_mesh_destroy(kh_mesh* m) {
     if (!m) return;
     (void)hipDeviceSynchronize();
     for (void* q : {(void*)m->U0, (void*)m->P0, (void*)m->U1, (void*)m->P1, (void*)m->P0f, (void*)m->P1f, m->F,
-                    m->ctop, (void*)m->pflag, (void*)m->fflag, (void*)m->fail_list, (void*)m->ds,
+                    m->ctop, (void*)m->pflag, (void*)m->fflag, (void*)m->fail_list, (void*)m->fofc_mark, (void*)m->ds,
                     (void*)m->dt_part, (void*)m->divb_part, (void*)m->dt_hist, (void*)m->pack_dev})
         if (q) (void)hipFree(q);
     for (kh_block* b : m->blk) kh_block_destroy(b);

@@ -1493,7 +1493,8 @@ template <int GEO, int INV, bool STG, int TI = 64>
 __global__ void __launch_bounds__(BS)
 fofc_mark_kernel(BlockDims b, StepConsts cst, GeomDev gd, int stage, const double* __restrict__ F,
                  const double* __restrict__ Usub, const double* Ufull, const double* __restrict__ Psub,
-                 const int* __restrict__ fixed, const DeviceState* __restrict__ ds, int* __restrict__ mark) {
+                 const int* __restrict__ fixed, const DeviceState* __restrict__ ds, int* __restrict__ mark,
+                 bool images) {
     __shared__ double sg[STG ? STG_RECOVER * TI : 1];
     int k, j, i, i0 = 0;
     bool active;
@@ -1553,16 +1554,23 @@ fofc_mark_kernel(BlockDims b, StepConsts cst, GeomDev gd, int stage, const doubl
     } else {
         with_center_geom<GEO>(gd, j, i, trial);
     }
-    mark[c] = (ff != 0 || (fixed && fixed[(long)j * gd.gn1 + i])) ? 1 : 0;
+    const int mv = (ff != 0 || (fixed && fixed[(long)j * gd.gn1 + i])) ? 1 : 0;
+    if (images) {   // the boundary sync of the flag (kharma_driver.cpp:328-331)
+        ImageSet s;
+        image_set(b, k, j, i, s);
+        store_with_images(b, s, mark, c, mv, b.nc);
+    } else {
+        mark[c] = mv;
+    }
 }
 
 hipError_t launch_fofc_mark(const BlockDims& b, const StepConsts& c, const GeomDev& gd, int stage, const double* F,
                             const double* Usub, const double* Ufull, const double* Psub, const int* fixed,
-                            const DeviceState* ds, int* mark, hipStream_t s) {
+                            const DeviceState* ds, int* mark, hipStream_t s, bool images) {
     const dim3 grid = grid_for(ninterior_host(b));
 #define KH_MARK(GEO, INV, STG, TI, G) \
     hipLaunchKernelGGL((fofc_mark_kernel<GEO, INV, STG, TI>), G, dim3(BS), 0, s, b, c, gd, stage, F, Usub, Ufull, Psub, \
-                       fixed, ds, mark)
+                       fixed, ds, mark, images)
     if (b.geo && b.ndim == 3) {
         const dim3 gt((unsigned)update_tiles_host(b));
         const bool ti32 = recover_tile_i(b) == 32;

@@ -62,6 +62,12 @@ CASES = {
     "torus3d_3x1x2": (lambda: pr.torus_config(48, 24, 16), pr.torus_prims, (3, 1, 2), 6),
     "torus3d_split_2x2x1": (lambda: dict(pr.torus_config(64, 48, 16), precision="split"), pr.torus_prims,
                             (2, 2, 1), 6),
+    # fofc/on (kharma_driver.cpp:291-337): marks synced into the neighbours' ghosts,
+    # polar rows on the blocks of the physical X2 faces only
+    "torus2d_fofc_2x2": (lambda: dict(pr.torus_config(64, 48, 1), fofc=True, fofc_polar_cells=2), pr.torus_prims,
+                         (2, 2, 1), 10),
+    "torus3d_fofc_2x2x2": (lambda: dict(pr.torus_config(48, 32, 16), fofc=True, fofc_polar_cells=1),
+                           pr.torus_prims, (2, 2, 2), 6),
 }
 
 
@@ -185,7 +191,5 @@ def test_mesh_rejects_bad_layouts():
     for nb, what in (((3, 1, 1), "divide evenly"), ((8, 1, 1), "nx >= 2 ng")):
         with pytest.raises(kh.KharmaError, match=what):
             kh.Mesh(cfg, nb)
-    with pytest.raises(kh.KharmaError, match="one-block meshes"):
-        kh.Mesh(dict(pr.torus_config(64, 48, 1), fofc=True), (2, 1))
     with pytest.raises(kh.KharmaError, match="inactive axis"):
         kh.Mesh(pr.orszag_tang_config(n=64), (2, 2, 2))
