@@ -133,7 +133,7 @@ def test_mesh_torus_step_against_oracle(oracle_lib):
     assert np.all(r <= np.maximum(1e-12, 10 * sens)), (r, sens)
 
 
-@pytest.mark.parametrize("case", ["torus3d_2x2x2", "fast3d_split_2x1x2"])
+@pytest.mark.parametrize("case", ["torus3d_2x2x2", "fast3d_split_2x1x2", "torus3d_fofc_2x2x2"])
 def test_mesh_graph_replay_equals_eager(case):
     """kh_mesh_set_graph: every replay of the captured step equals the eager
     step bit for bit, with the program's own eager work (state read-outs,
