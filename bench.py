@@ -297,6 +297,11 @@ def main():
                 ms = _bench_mesh(dict(cfg, precision=f), P0, nb, a.steps, a.warmup, packed)
                 layout[f + ("" if packed else "_per_block")] = {"zone_cycles_per_s": nz * a.steps / (ms * 1e-3),
                                                                 "ms_per_step": ms / a.steps}
+        if fofc is not None:   # the same first-order flux correction over the 16 blocks
+            nf, wf = fofc["steps"], fofc["warmup"]
+            ms = _bench_mesh(dict(cfg, precision="fp64", fofc=True), P0, nb, nf, wf)
+            layout["fofc_fp64"] = {"steps": nf, "ms_per_step": ms / nf,
+                                   "over_plain": (ms / nf) / layout["fp64"]["ms_per_step"]}
     adj = None if a.no_adjoint else _bench_adjoint()
     adj_gr = None if a.no_adjoint else _bench_adjoint_torus()
     zc64 = nz * a.steps / (r64["ms"] * 1e-3)
